@@ -1,0 +1,127 @@
+"""Benchmark: particle-updates/s of the plain SPH step on the 1M-particle dam break.
+
+Workload: dam break, fluid column 1.0 m x 2.0 m at dx = sqrt(2) mm (999,698 fluid
+particles) in a 4 m x 3 m tank with 3 layers of fixed wall particles (21,228),
+cubic spline h = 1.3 dx, Tait EOS, artificial viscosity 0.1, dt = 0.25 h / c0.
+A step is one full leapfrog step of every particle on one GPU: cell binning and
+sort, fused neighbour search + pair sums, FP64 integration.  Inputs are resident
+in HBM before the timed region; the timed region is K steps bracketed by stream
+synchronisation, timed with HIP events on the library's stream, as is the fused
+pair kernel inside it (the dominant kernel, used for the roofline).
+
+cpu_baseline: the FP64 CPU oracle (C/OpenMP restatement of the reference's
+path, oracle/uph_oracle.c) on the same workload for a bounded number of steps.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_UPDATE = 80.0     # FP64 x, y, vx, vy, rho read once and written once
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(sc, budget_s):
+    from oracle.oracle import Sim
+    nthreads = min(16, os.cpu_count() or 1)
+    o = Sim(sc, nthreads=nthreads)
+    t0 = time.perf_counter()
+    o.step(1)
+    t1 = time.perf_counter()
+    first = t1 - t0
+    steps = max(1, min(50, int(budget_s / max(first, 1e-3))))
+    t0 = time.perf_counter()
+    o.step(steps)
+    el = time.perf_counter() - t0
+    return dict(value=sc.n_fluid * steps / el, unit="particle-updates/s", cores=nthreads,
+                kind="port",
+                sample=f"{steps} steps of the same 1M dam break after 1 untimed step, "
+                       f"FP64 oracle (C/OpenMP, {nthreads} threads), {el:.1f} s")
+
+
+def traffic_from_profiles():
+    p = os.path.join(ROOT, "profiles", "pair_kernel_traffic.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        return json.load(f).get("hbm_bytes_per_launch")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--cpu-budget", type=float, default=15.0,
+                    help="seconds of CPU-baseline stepping (0 disables)")
+    ap.add_argument("--graph", action="store_true", help="also time hipGraph replay")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(scenes would shard across GPUs with no communication)", file=sys.stderr)
+        sys.exit(2)
+    from uph_amd import scenes
+    from uph_amd.solver import GpuSolver
+    sc = scenes.dam_break(width=1.0, height=2.0, dx=2 ** 0.5 * 1e-3)
+    log(f"workload: {sc.n_fluid} fluid + {sc.n_boundary} wall particles")
+    g = GpuSolver(sc)
+    g.step(args.warmup)
+    g.sync()
+    t0 = time.perf_counter()
+    ms_total, ms_pair = g.time_steps(args.steps)
+    overflow = g.sync()
+    wall = time.perf_counter() - t0
+    ms_step = ms_total / args.steps
+    value = sc.n_fluid * args.steps / (ms_total * 1e-3)
+    achieved = BYTES_PER_UPDATE * sc.n_fluid / (ms_pair * 1e-3) / 1e9
+    log(f"timed {args.steps} steps: {ms_total:.3f} ms (host wall {wall*1e3:.1f} ms), "
+        f"pair kernel {ms_pair*1e3:.1f} us")
+    out = {
+        "metric": "particle-updates/s, 1M-particle dam break (plain path)",
+        "value": value,
+        "unit": "particle-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 pair sums, fp64 integration",
+        "data": "synthetic (generated dam-break lattice)",
+        "config": {"workload": "dam break 1.0x2.0 m column, dx=sqrt(2) mm, 4x3 m tank",
+                   "fluid_particles": sc.n_fluid, "wall_particles": sc.n_boundary,
+                   "h_over_dx": 1.3, "dt": sc.dt, "capped_steps": overflow},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic_from_profiles(),
+                     "kernel": "k_pair<false>", "kernel_ms": ms_pair},
+    }
+    if args.graph:
+        gg = GpuSolver(sc, use_graph=True)
+        gg.step(args.warmup)
+        gg.sync()
+        t0 = time.perf_counter()
+        gg.step(args.steps)
+        gg.sync()
+        out["graph_ms_per_step"] = (time.perf_counter() - t0) * 1e3 / args.steps
+        gg.close()
+    g.close()
+    if args.cpu_budget > 0:
+        log("cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(sc, args.cpu_budget)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,133 @@
+/*
+ * uph.h -- C ABI of libuph_gpu.so, the MI355X (gfx950) implementation of the
+ * per-step SPH hot path of Uph.
+ *
+ * Two kinds of entry points:
+ *
+ *  1. Drop-in replacements for the reference's own solver seam:
+ *       SearchNeighbors   -- same signature and contract as src/lib/Search.cc:5
+ *                            (declared extern at src/lib/derivatives.c:34-42,
+ *                            called from derivatives.c:1076-1086).
+ *
+ *  2. A device-resident stepping API: the leapfrog of src/bin/sph.cc:90-168
+ *     with the rates of derivatives() (src/lib/derivatives.c:3137) computed on
+ *     the GPU, so a host driver runs many steps -- of one scene or of a batch of
+ *     independent scenes -- with no per-step host round trip.  The reference has
+ *     no library step function (its integrator lives in main), so this mirrors
+ *     the reference's call order rather than a reference symbol:
+ *       uph_create   ~ CreateSystem + input   (System.c:38, input.c:22)
+ *       uph_step     ~ N iterations of the loop body sph.cc:90-168
+ *       uph_get_state~ reading sys->Position / Velocity / density
+ *       uph_destroy  ~ DestroySystem          (System.c:347)
+ *
+ * Conventions: every function returns 0 (or a count) on success and a negative
+ * value on failure, never exits the process; uph_last_error() describes the
+ * last failure on a handle (or of uph_create when the handle is NULL).  One
+ * host thread per handle; each handle owns one HIP stream and its device
+ * memory; the caller owns every host buffer.  Particle arrays are in the
+ * reference's order (fluid, solid, fixed boundary, moving boundary) and
+ * positions/velocities are interleaved (x0, y0, x1, y1, ...).
+ */
+#ifndef UPH_H_
+#define UPH_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- drop-in */
+
+/*
+ * Replaces SearchNeighbors (src/lib/Search.cc:5-44).  For every fluid query
+ * s < NumberOfFluidParticles, takes the NumberOfNeighbors nearest particles of
+ * all NumberOfParticles other than s itself (ranked by squared distance, ties
+ * by index) and emits (s, j, d2) for those with j > s, d2 < dist and
+ * j < ParticleBoundary, query-major, each query's pairs in rank order.
+ * Squared distances are FP64 dx*dx + dy*dy with dx = x_s - x_j, uncontracted.
+ * Returns the pair count niac.  Like the reference it trusts the caller's
+ * buffers to hold every pair; uph_search_neighbors_cap adds a capacity check.
+ * Returns -1 on a device error.
+ */
+int SearchNeighbors(const double **Position, int *i_pair, int *j_pair, double *rij2,
+                    const int NumberOfFluidParticles, const int NumberOfParticles,
+                    int ParticleBoundary, const int NumberOfNeighbors, const double dist);
+
+/* SearchNeighbors on a contiguous N x 2 position array with a pair capacity:
+ * returns -2 (writing nothing) when more than `capacity` pairs qualify. */
+int uph_search_neighbors_cap(const double *pos_xy, int *i_pair, int *j_pair, double *rij2,
+                             int n_fluid, int n_particles, int particle_boundary,
+                             int n_neighbors, double dist, long capacity);
+
+/* Message of the last failure of SearchNeighbors / uph_search_neighbors_cap. */
+const char *uph_search_last_error(void);
+
+/* ---------------------------------------------------------- stepping API */
+
+/* Moving-boundary kinematics of itype-3 particles (inputbp.c:194-207). */
+enum {
+  UPH_MOTION_NONE = 0,
+  UPH_MOTION_PISTON = 1,      /* x = S/2 (1 - cos wt), vx = S w/2 sin wt (reference piston) */
+  UPH_MOTION_SINE_OFFSET = 2  /* x = x_deck + A sin wt, vx = A w cos wt */
+};
+
+/* One independent scene.  Arrays hold n = n_fluid + n_fixed + n_moving rows. */
+typedef struct uph_scene {
+  int n_fluid, n_fixed, n_moving;
+  const double *pos;    /* 2n */
+  const double *vel;    /* 2n */
+  const double *rho;    /* n  : density (boundary rows must equal rho0)   */
+  const double *mass;   /* n  */
+  const int *itype;     /* n  : 1 fluid, 2 fixed, 3 moving, 5 spring wall   */
+  const double *lvel;   /* 2n : localVelocity deck columns                  */
+  double dx;            /* lattice spacing; spring range r0 = 3 dx          */
+  double h;             /* smoothing length (all particles)                 */
+  double rho0, gamma, c0, B;  /* Tait: P = B((rho/rho0)^gamma - 1)          */
+  double g;
+  int gravity;          /* 1: a_y -= g on fluid                            */
+  double alpha_av;      /* Monaghan artificial viscosity alpha (beta = 0)   */
+  double k_spring, c_damp;    /* springConstant0, dampingConstant0          */
+  double dt;
+  int K;                /* neighbour cap of the search (40 in the reference) */
+  int motion;           /* UPH_MOTION_*                                     */
+  double m_amp, m_omega;
+  double box[4];        /* xmin, ymin, xmax, ymax covered by the cell grid  */
+} uph_scene;
+
+typedef struct uph_opts {
+  int use_graph;        /* 1: replay each step as one captured hipGraph     */
+  int keep_rates;       /* 1: keep the last step's FP32 rates for uph_get_rates */
+} uph_opts;
+
+typedef struct uph_handle uph_t;
+
+int uph_create(const uph_scene *scenes, int n_scenes, const uph_opts *opts, uph_t **out);
+void uph_destroy(uph_t *h);
+
+/* Advance every scene by n_steps leapfrog steps (asynchronous on the handle's stream). */
+int uph_step(uph_t *h, int n_steps);
+
+/* Wait for the handle's stream; returns the number of steps in which some fluid
+ * particle had more than K neighbours in range (exact capped path taken). */
+int uph_sync(uph_t *h);
+
+/* Current state of one scene in the reference's order (any pointer may be NULL). */
+int uph_get_state(uph_t *h, int scene, double *pos, double *vel, double *rho);
+
+/* Rates of the last step (needs opts.keep_rates): R = ddensitydt, Rt = ddensitydt_approx,
+ * a = dvdt (= dvdt_approx in this scheme), fluid rows only (n_fluid, n_fluid, 2 n_fluid). */
+int uph_get_rates(uph_t *h, int scene, float *R, float *Rt, float *a);
+
+/* Completed steps. */
+int uph_itime(const uph_t *h);
+
+/* Run n_steps with HIP events on the handle's stream: *ms_total spans the
+ * whole loop, *ms_pair is the mean duration of the fused pair kernel. */
+int uph_time_steps(uph_t *h, int n_steps, double *ms_total, double *ms_pair);
+
+const char *uph_last_error(const uph_t *h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* UPH_H_ */

@@ -1,0 +1,617 @@
+/*
+ * uph_oracle.c -- FP64 CPU restatement of the reference's per-step SPH hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker the GPU library is
+ * compared against, and the CPU baseline that bench.py times.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg load it; the product
+ * library (uph_amd/libuph_gpu.so) never links or calls it.
+ *
+ * It restates, in the reference's own pair-list + scatter form and FP64
+ * throughout, the "config-mode" scheme:
+ *   - neighbour contract of SearchNeighbors           Search.cc:5-44
+ *   - W / grad W of the cubic spline                   derivatives.c:44-66
+ *   - pair kernel evaluation                           derivatives.c:1063-1106
+ *   - continuity                                       derivatives.c:1520-1567
+ *   - predicting continuity (order dependent)          derivatives.c:1217-1252
+ *   - Tait equation of state (feeds the momentum eq.)  derivatives.c:1674-1682
+ *   - pressure gradient + predicting variant           derivatives.c:1109-1163, 1333-1377
+ *   - Monaghan (1992) artificial viscosity added to both pressure terms
+ *     (absent from the reference: alpha is parsed at System.c:75 but unused)
+ *   - gravity                                          derivatives.c:1167-1178, 1379-1390
+ *   - spring boundary force + predicting variant       derivatives.c:2396-2449, 1392-1446
+ *   - boundary reset and piston kinematics             inputbp.c:117-150, 194-207
+ *   - leapfrog predictor / corrector                   sph.cc:79-168
+ * with the fluctuation fields and epsilon terms zero (they multiply fields the
+ * scheme does not evolve).  The reference's own build cannot be compiled in
+ * this image (it needs <mkl.h> and libnabo), so nothing here is linked against
+ * it; each function is pinned by the closed forms, brute-force checks and
+ * invariants in tests/test_oracle.py.
+ *
+ * Build: oracle/Makefile (gcc -O2 -fopenmp -ffp-contract=off).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+typedef struct {
+  double dx, h, rho0, gamma, c0, B, g;
+  int gravity;
+  double alpha_av, k_spring, c_damp, dt;
+  int K, motion;
+  double m_amp, m_omega;
+} orc_params;
+
+typedef struct {
+  int nf, nfixed, nmoving, n;
+  orc_params p;
+  double *x, *v, *vmin, *vloc, *rho, *rhomin, *rholoc, *P, *mass, *hsml;
+  int *itype;
+  double *R, *Rt, *a, *at;              /* ddensitydt, ddensitydt_approx, dvdt, dvdt_approx */
+  double *deck_x, *deck_v, *deck_rho, *deck_P, *deck_lvel;
+  int *ip, *jp;
+  double *rij2, *dwdx;
+  long cap;
+  int niac;
+  int itime;
+  int nthreads;
+  double *tmp;                          /* nthreads x 2n scatter buffers */
+} orc_sim;
+
+static int nthreads_of(int want) {
+#ifdef _OPENMP
+  return want > 0 ? want : omp_get_max_threads();
+#else
+  (void)want;
+  return 1;
+#endif
+}
+
+/* ------------------------------------------------------------------ kernel */
+
+/* Cubic spline W and grad W (derivatives.c:44-66): alpha = 10/(7 pi h^2). */
+void orc_cubic(const double *dx, double r, double h, double *w, double *dw) {
+  const double alpha = 10.0 / (7. * M_PI * h * h);
+  const double q = r / h;
+  if (q >= 0.0 && q < 1.0) {
+    double q2 = q * q;
+    double q3 = q * q * q;
+    *w = alpha * (1 - 1.5 * q2 + 0.75 * q3);
+    for (int d = 0; d < 2; d++) dw[d] = alpha * (-3 + 2.25 * q) * dx[d] / (h * h);
+  } else if (q >= 1.0 && q < 2.0) {
+    double qp2 = (2 - q) * (2 - q);
+    double qp3 = qp2 * (2 - q);
+    *w = alpha * 0.25 * qp3;
+    for (int d = 0; d < 2; d++) dw[d] = -alpha * 0.75 * qp2 * dx[d] / (h * r);
+  } else {
+    *w = 0.0;
+    dw[0] = dw[1] = 0.0;
+  }
+}
+
+/* ------------------------------------------------------- neighbour search */
+
+typedef struct { double d2; int j; } cand_t;
+
+static int cand_less(const cand_t *a, const cand_t *b) {
+  return a->d2 < b->d2 || (a->d2 == b->d2 && a->j < b->j);
+}
+
+static int cand_cmp(const void *pa, const void *pb) {
+  const cand_t *a = pa, *b = pb;
+  if (cand_less(a, b)) return -1;
+  if (cand_less(b, a)) return 1;
+  return 0;
+}
+
+static inline double dist2(const double *pos, int s, int j) {
+  const double ex = pos[2 * s] - pos[2 * j];
+  const double ey = pos[2 * s + 1] - pos[2 * j + 1];
+  return ex * ex + ey * ey;
+}
+
+/*
+ * Brute-force statement of the SearchNeighbors contract (Search.cc:5-44):
+ * for each fluid query s the K nearest particles other than s itself, ranked
+ * by (d2, index); keep j if j > s, d2 < dist and j < ParticleBoundary; emit in
+ * rank order.  O(N^2 log N): for small inputs only.
+ */
+int orc_search_brute(const double *pos, int *ip, int *jp, double *rij2, int nf, int n, int pb,
+                     int K, double dist, long cap) {
+  cand_t *c = malloc(sizeof(cand_t) * (size_t)(n > 1 ? n : 1));
+  long niac = 0;
+  for (int s = 0; s < nf; s++) {
+    int m = 0;
+    for (int j = 0; j < n; j++) {
+      if (j == s) continue;
+      c[m].d2 = dist2(pos, s, j);
+      c[m].j = j;
+      m++;
+    }
+    qsort(c, m, sizeof(cand_t), cand_cmp);
+    int kk = m < K ? m : K;
+    for (int k = 0; k < kk; k++) {
+      if (c[k].j > s && c[k].d2 < dist && c[k].j < pb) {
+        if (niac >= cap) { free(c); return -1; }
+        ip[niac] = s; jp[niac] = c[k].j; rij2[niac] = c[k].d2;
+        niac++;
+      }
+    }
+  }
+  free(c);
+  return (int)niac;
+}
+
+typedef struct {
+  int ncx, ncy;
+  double x0, y0, inv;
+  int *start;   /* ncx*ncy + 1 */
+  int *idx;     /* particle ids sorted by cell, ascending id within a cell */
+} grid_t;
+
+static void grid_build(grid_t *g, const double *pos, int n, double cell) {
+  double lx = pos[0], ly = pos[1], hx = pos[0], hy = pos[1];
+  for (int i = 1; i < n; i++) {
+    if (pos[2 * i] < lx) lx = pos[2 * i];
+    if (pos[2 * i] > hx) hx = pos[2 * i];
+    if (pos[2 * i + 1] < ly) ly = pos[2 * i + 1];
+    if (pos[2 * i + 1] > hy) hy = pos[2 * i + 1];
+  }
+  g->x0 = lx; g->y0 = ly; g->inv = 1.0 / cell;
+  g->ncx = (int)floor((hx - lx) * g->inv) + 1;
+  g->ncy = (int)floor((hy - ly) * g->inv) + 1;
+  long nc = (long)g->ncx * g->ncy;
+  g->start = calloc(nc + 1, sizeof(int));
+  g->idx = malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+  int *cell_of = malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+  for (int i = 0; i < n; i++) {
+    int cx = (int)floor((pos[2 * i] - lx) * g->inv);
+    int cy = (int)floor((pos[2 * i + 1] - ly) * g->inv);
+    if (cx >= g->ncx) cx = g->ncx - 1;
+    if (cy >= g->ncy) cy = g->ncy - 1;
+    cell_of[i] = cy * g->ncx + cx;
+    g->start[cell_of[i] + 1]++;
+  }
+  for (long c = 0; c < nc; c++) g->start[c + 1] += g->start[c];
+  int *fill = malloc(sizeof(int) * (size_t)(nc > 0 ? nc : 1));
+  for (long c = 0; c < nc; c++) fill[c] = g->start[c];
+  for (int i = 0; i < n; i++) g->idx[fill[cell_of[i]]++] = i;
+  free(fill);
+  free(cell_of);
+}
+
+static void grid_free(grid_t *g) { free(g->start); free(g->idx); }
+
+/* In-range candidates of query s (d2 < dist, j != s), sorted by (d2, j). */
+static int grid_candidates(const grid_t *g, const double *pos, int s, double dist, cand_t **buf,
+                           int *bufcap) {
+  int cx = (int)floor((pos[2 * s] - g->x0) * g->inv);
+  int cy = (int)floor((pos[2 * s + 1] - g->y0) * g->inv);
+  if (cx >= g->ncx) cx = g->ncx - 1;
+  if (cy >= g->ncy) cy = g->ncy - 1;
+  int m = 0;
+  for (int yy = cy - 1; yy <= cy + 1; yy++) {
+    if (yy < 0 || yy >= g->ncy) continue;
+    for (int xx = cx - 1; xx <= cx + 1; xx++) {
+      if (xx < 0 || xx >= g->ncx) continue;
+      int c = yy * g->ncx + xx;
+      for (int t = g->start[c]; t < g->start[c + 1]; t++) {
+        int j = g->idx[t];
+        if (j == s) continue;
+        double d2 = dist2(pos, s, j);
+        if (!(d2 < dist)) continue;
+        if (m == *bufcap) {
+          *bufcap = *bufcap * 2 + 64;
+          *buf = realloc(*buf, sizeof(cand_t) * (size_t)*bufcap);
+        }
+        (*buf)[m].d2 = d2;
+        (*buf)[m].j = j;
+        m++;
+      }
+    }
+  }
+  /* insertion sort: candidate lists are short */
+  cand_t *c = *buf;
+  for (int a = 1; a < m; a++) {
+    cand_t key = c[a];
+    int b = a - 1;
+    while (b >= 0 && cand_less(&key, &c[b])) { c[b + 1] = c[b]; b--; }
+    c[b + 1] = key;
+  }
+  return m;
+}
+
+/*
+ * Cell-grid statement of the same contract.  When at most K particles lie
+ * inside the search radius they are all among the K nearest, so only in-range
+ * candidates need ranking; otherwise the first K of them by (d2, j) are kept.
+ */
+int orc_search(const double *pos, int *ip, int *jp, double *rij2, int nf, int n, int pb, int K,
+               double dist, long cap, int nthreads) {
+  if (nf <= 0) return 0;
+  grid_t g;
+  grid_build(&g, pos, n, sqrt(dist) * (1.0 + 1e-9));
+  int *cnt = calloc((size_t)nf + 1, sizeof(int));
+  int nt = nthreads_of(nthreads);
+#pragma omp parallel num_threads(nt)
+  {
+    int bc = 128;
+    cand_t *buf = malloc(sizeof(cand_t) * bc);
+#pragma omp for schedule(dynamic, 256)
+    for (int s = 0; s < nf; s++) {
+      int m = grid_candidates(&g, pos, s, dist, &buf, &bc);
+      int kk = m < K ? m : K;
+      int q = 0;
+      for (int k = 0; k < kk; k++)
+        if (buf[k].j > s && buf[k].j < pb) q++;
+      cnt[s + 1] = q;
+    }
+    free(buf);
+  }
+  long *off = malloc(sizeof(long) * ((size_t)nf + 1));
+  off[0] = 0;
+  for (int s = 0; s < nf; s++) off[s + 1] = off[s] + cnt[s + 1];
+  long total = off[nf];
+  if (total > cap) { free(off); free(cnt); grid_free(&g); return -1; }
+#pragma omp parallel num_threads(nt)
+  {
+    int bc = 128;
+    cand_t *buf = malloc(sizeof(cand_t) * bc);
+#pragma omp for schedule(dynamic, 256)
+    for (int s = 0; s < nf; s++) {
+      int m = grid_candidates(&g, pos, s, dist, &buf, &bc);
+      int kk = m < K ? m : K;
+      long o = off[s];
+      for (int k = 0; k < kk; k++)
+        if (buf[k].j > s && buf[k].j < pb) {
+          ip[o] = s; jp[o] = buf[k].j; rij2[o] = buf[k].d2;
+          o++;
+        }
+    }
+    free(buf);
+  }
+  free(off);
+  free(cnt);
+  grid_free(&g);
+  return (int)total;
+}
+
+/* ---------------------------------------------------------------- system */
+
+orc_sim *orc_create(int nf, int nfixed, int nmoving, const orc_params *p, const double *pos,
+                    const double *vel, const double *rho, const double *pres, const double *mass,
+                    const int *itype, const double *lvel, int nthreads) {
+  orc_sim *s = calloc(1, sizeof(orc_sim));
+  int n = nf + nfixed + nmoving;
+  s->nf = nf; s->nfixed = nfixed; s->nmoving = nmoving; s->n = n;
+  s->p = *p;
+  s->nthreads = nthreads_of(nthreads);
+  size_t n1 = (size_t)(n > 0 ? n : 1);
+#define A(f, c) s->f = calloc(n1 * (c), sizeof(double))
+  A(x, 2); A(v, 2); A(vmin, 2); A(vloc, 2); A(rho, 1); A(rhomin, 1); A(rholoc, 1); A(P, 1);
+  A(mass, 1); A(hsml, 1); A(R, 1); A(Rt, 1); A(a, 2); A(at, 2);
+  A(deck_x, 2); A(deck_v, 2); A(deck_rho, 1); A(deck_P, 1); A(deck_lvel, 2);
+#undef A
+  s->itype = calloc(n1, sizeof(int));
+  memcpy(s->x, pos, sizeof(double) * 2 * n);
+  memcpy(s->v, vel, sizeof(double) * 2 * n);
+  memcpy(s->vloc, lvel, sizeof(double) * 2 * n);
+  memcpy(s->rho, rho, sizeof(double) * n);
+  memcpy(s->P, pres, sizeof(double) * n);
+  memcpy(s->mass, mass, sizeof(double) * n);
+  memcpy(s->itype, itype, sizeof(int) * n);
+  memcpy(s->deck_x, pos, sizeof(double) * 2 * n);
+  memcpy(s->deck_v, vel, sizeof(double) * 2 * n);
+  memcpy(s->deck_rho, rho, sizeof(double) * n);
+  memcpy(s->deck_P, pres, sizeof(double) * n);
+  memcpy(s->deck_lvel, lvel, sizeof(double) * 2 * n);
+  for (int i = 0; i < n; i++) {
+    s->hsml[i] = p->h;
+    /* config-mode choice: localDensity starts equal to density (input.c:109 sets rho0) */
+    s->rholoc[i] = i < nf ? rho[i] : p->rho0;
+  }
+  memcpy(s->rhomin, rho, sizeof(double) * n);
+  memcpy(s->vmin, vel, sizeof(double) * 2 * n);
+  s->cap = (long)nf * (p->K > 0 ? p->K : 1) + 16;
+  s->ip = malloc(sizeof(int) * s->cap);
+  s->jp = malloc(sizeof(int) * s->cap);
+  s->rij2 = malloc(sizeof(double) * s->cap);
+  s->dwdx = malloc(sizeof(double) * 2 * s->cap);
+  s->tmp = malloc(sizeof(double) * 2 * n1 * s->nthreads);
+  s->itime = 0;
+  return s;
+}
+
+void orc_destroy(orc_sim *s) {
+  if (!s) return;
+  free(s->x); free(s->v); free(s->vmin); free(s->vloc); free(s->rho); free(s->rhomin);
+  free(s->rholoc); free(s->P); free(s->mass); free(s->hsml); free(s->R); free(s->Rt);
+  free(s->a); free(s->at); free(s->deck_x); free(s->deck_v); free(s->deck_rho); free(s->deck_P);
+  free(s->deck_lvel); free(s->itype); free(s->ip); free(s->jp); free(s->rij2); free(s->dwdx);
+  free(s->tmp); free(s);
+}
+
+/* Boundary reset and wall kinematics (inputbp.c:117-150, 194-207, 285-290). */
+static void boundary_reset(orc_sim *s, double t) {
+  const orc_params *p = &s->p;
+  for (int i = s->nf; i < s->n; i++) {
+    s->x[2 * i] = s->deck_x[2 * i]; s->x[2 * i + 1] = s->deck_x[2 * i + 1];
+    s->v[2 * i] = s->deck_v[2 * i]; s->v[2 * i + 1] = s->deck_v[2 * i + 1];
+    s->vloc[2 * i] = s->deck_lvel[2 * i]; s->vloc[2 * i + 1] = s->deck_lvel[2 * i + 1];
+    s->rho[i] = s->deck_rho[i];
+    s->P[i] = s->deck_P[i];
+    s->hsml[i] = p->h;
+    s->rholoc[i] = p->rho0;
+  }
+  if (p->motion == 1 || p->motion == 2) {
+    double X, V;
+    if (p->motion == 1) {          /* reference piston, absolute position */
+      X = 0.5 * p->m_amp * (1.0 - cos(p->m_omega * t));
+      V = 0.5 * p->m_amp * p->m_omega * sin(p->m_omega * t);
+    } else {                       /* sinusoidal offset of the deck position */
+      X = p->m_amp * sin(p->m_omega * t);
+      V = p->m_amp * p->m_omega * cos(p->m_omega * t);
+    }
+    for (int i = s->nf + s->nfixed; i < s->n; i++) {
+      s->x[2 * i] = (p->motion == 1) ? X : s->deck_x[2 * i] + X;
+      s->v[2 * i] = V;
+      s->vloc[2 * i] = V;
+    }
+  }
+}
+
+/* Monaghan (1992) viscous term Pi_ij, beta = 0, mean sound speed c0. */
+static inline double av_pi(const orc_sim *s, int i, int j, const double *dx, double r2, double h) {
+  const double dvx = s->v[2 * i] - s->v[2 * j];
+  const double dvy = s->v[2 * i + 1] - s->v[2 * j + 1];
+  const double vr = dvx * dx[0] + dvy * dx[1];
+  if (!(vr < 0.0)) return 0.0;
+  const double mu = h * vr / (r2 + 0.01 * h * h);
+  const double rhobar = 0.5 * (s->rho[i] + s->rho[j]);
+  return -s->p.alpha_av * s->p.c0 * mu / rhobar;
+}
+
+/* Per-thread scatter accumulation then thread-ordered reduction, the pattern of
+ * derivatives.c:1114-1157. `width` is 1 (density) or 2 (vector). */
+#define SCATTER_BEGIN(width)                                               \
+  {                                                                        \
+    const int W_ = (width);                                                \
+    _Pragma("omp parallel num_threads(s->nthreads)")                       \
+    {                                                                      \
+      int tid_ = 0;                                                        \
+      SCATTER_TID;                                                         \
+      double *tb = s->tmp + (size_t)tid_ * 2 * s->n;                       \
+      memset(tb, 0, sizeof(double) * W_ * s->n);                           \
+      _Pragma("omp for schedule(static)")                                  \
+      for (int k = 0; k < s->niac; k++) {
+#define SCATTER_END(dst)                                                   \
+      }                                                                    \
+    }                                                                      \
+    for (int th = 0; th < s->nthreads; th++) {                             \
+      const double *tb = s->tmp + (size_t)th * 2 * s->n;                   \
+      for (int q = 0; q < W_ * s->nf; q++) (dst)[q] += tb[q];              \
+    }                                                                      \
+  }
+#ifdef _OPENMP
+#define SCATTER_TID tid_ = omp_get_thread_num()
+#else
+#define SCATTER_TID (void)0
+#endif
+
+/*
+ * One call of derivatives(sys, t) in config mode (derivatives.c:3137-4315 with
+ * the scheme's substitutions): rates R, Rt, a, at are recomputed from scratch.
+ */
+int orc_derivatives(orc_sim *s, double t) {
+  const orc_params *p = &s->p;
+  const int n = s->n;
+  memset(s->R, 0, sizeof(double) * n);
+  memset(s->Rt, 0, sizeof(double) * n);
+  memset(s->a, 0, sizeof(double) * 2 * n);
+  memset(s->at, 0, sizeof(double) * 2 * n);
+  boundary_reset(s, t);
+
+  /* CalculateKernel (derivatives.c:1063-1106) with the cubic spline */
+  const int pb = s->nf + s->nfixed + s->nmoving - 1;
+  int niac = orc_search(s->x, s->ip, s->jp, s->rij2, s->nf, n, pb, p->K,
+                        s->hsml[0] * s->hsml[0] * 4, s->cap, s->nthreads);
+  if (niac < 0) return -1;
+  s->niac = niac;
+#pragma omp parallel for num_threads(s->nthreads) schedule(static)
+  for (int k = 0; k < niac; k++) {
+    const int i = s->ip[k], j = s->jp[k];
+    double dx[2], w;
+    const double h = 0.5 * (s->hsml[i] + s->hsml[j]);
+    for (int d = 0; d < 2; d++) dx[d] = s->x[2 * i + d] - s->x[2 * j + d];
+    const double r = sqrt(s->rij2[k]);
+    orc_cubic(dx, r, h, &w, &s->dwdx[2 * k]);
+  }
+
+  /* EquationOfState (derivatives.c:1674-1682); config mode feeds it to the momentum terms */
+  for (int i = 0; i < n; i++) {
+    const double density = s->rho[i] / p->rho0;
+    s->P[i] = p->B * (pow(density, p->gamma) - 1.0);
+  }
+
+  /* PredictingDensityEquation (derivatives.c:1217-1252) */
+  SCATTER_BEGIN(1)
+    const int i = s->ip[k], j = s->jp[k];
+    const double *dw = &s->dwdx[2 * k];
+    double dv[2];
+    for (int d = 0; d < 2; d++) dv[d] = s->v[2 * i + d] - s->vloc[2 * j + d];
+    for (int d = 0; d < 2; d++) {
+      double vdw = dv[d] * dw[d];
+      tb[i] += s->mass[j] * vdw;
+      tb[j] += s->mass[j] * vdw;
+    }
+  SCATTER_END(s->Rt)
+
+  /* PredictingPressureGradientForce (derivatives.c:1333-1377) + artificial viscosity */
+  SCATTER_BEGIN(2)
+    const int i = s->ip[k], j = s->jp[k];
+    const double *dw = &s->dwdx[2 * k];
+    const double rhoi = 1.0 / (s->rho[i] * s->rho[i]);
+    const double rhoj = 1.0 / (s->rho[j] * s->rho[j]);
+    double pij = s->P[i] * rhoi + s->P[j] * rhoj;
+    double dx[2] = {s->x[2 * i] - s->x[2 * j], s->x[2 * i + 1] - s->x[2 * j + 1]};
+    pij += av_pi(s, i, j, dx, s->rij2[k], 0.5 * (s->hsml[i] + s->hsml[j]));
+    for (int d = 0; d < 2; d++) {
+      tb[2 * i + d] -= s->mass[j] * pij * dw[d];
+      tb[2 * j + d] += s->mass[i] * pij * dw[d];
+    }
+  SCATTER_END(s->at)
+
+  /* DensityEquation (derivatives.c:1520-1567) */
+  SCATTER_BEGIN(1)
+    const int i = s->ip[k], j = s->jp[k];
+    const double *dw = &s->dwdx[2 * k];
+    double dvij[2], dvji[2];
+    for (int d = 0; d < 2; d++) {
+      dvij[d] = s->v[2 * i + d] - s->vloc[2 * j + d];
+      dvji[d] = s->v[2 * j + d] - s->vloc[2 * i + d];
+    }
+    for (int d = 0; d < 2; d++) {
+      double vdwij = dvij[d] * dw[d];
+      double vdwji = dvji[d] * dw[d];
+      double volumei = s->mass[i] / s->rho[i];
+      double volumej = s->mass[j] / s->rho[j];
+      double mj = volumej * s->rholoc[j];
+      double mi = volumei * s->rholoc[i];
+      tb[i] += mj * vdwij;
+      tb[j] -= mi * vdwji;
+    }
+  SCATTER_END(s->R)
+
+  /* PressureGradientForce (derivatives.c:1109-1163) + artificial viscosity */
+  SCATTER_BEGIN(2)
+    const int i = s->ip[k], j = s->jp[k];
+    const double *dw = &s->dwdx[2 * k];
+    const double rhoi = s->rholoc[j] / (s->rho[i] * s->rho[i] * s->rho[j]);
+    const double rhoj = s->rholoc[i] / (s->rho[j] * s->rho[j] * s->rho[i]);
+    double pij = s->P[i] * rhoi + s->P[j] * rhoj;
+    double dx[2] = {s->x[2 * i] - s->x[2 * j], s->x[2 * i + 1] - s->x[2 * j + 1]};
+    pij += av_pi(s, i, j, dx, s->rij2[k], 0.5 * (s->hsml[i] + s->hsml[j]));
+    for (int d = 0; d < 2; d++) {
+      tb[2 * i + d] -= s->mass[j] * pij * dw[d];
+      tb[2 * j + d] += s->mass[i] * pij * dw[d];
+    }
+  SCATTER_END(s->a)
+
+  /* BodyForce / PredictingBodyForce (derivatives.c:1167-1178, 1379-1390) */
+  if (p->gravity == 1)
+    for (int i = 0; i < s->nf; i++) {
+      s->a[2 * i] -= 0.0; s->a[2 * i + 1] -= p->g;
+      s->at[2 * i] -= 0.0; s->at[2 * i + 1] -= p->g;
+    }
+
+  /* SpringBoundaryForces + PredictingSpringBoundaryForces (derivatives.c:2396-2449,
+   * 1392-1446): identical formulas accumulated into a and at. */
+  for (int pass = 0; pass < 2; pass++) {
+    double *dst = pass == 0 ? s->a : s->at;
+    const double r0 = 3.0 * p->dx;
+    SCATTER_BEGIN(2)
+      const int i = s->ip[k], j = s->jp[k];
+      if ((s->itype[i] == 1) && ((s->itype[j] == 2) || (s->itype[j] == 3) || (s->itype[j] == 5))) {
+        double dx[2], dv[2];
+        double rr = 0.0, rvij = 0.0;
+        for (int d = 0; d < 2; d++) {
+          dx[d] = s->x[2 * i + d] - s->x[2 * j + d];
+          dv[d] = s->v[2 * i + d] - s->v[2 * j + d];
+          rr += dx[d] * dx[d];
+          rvij += dx[d] * dv[d];
+        }
+        rr = sqrt(rr);
+        if (rr < r0) {
+          for (int d = 0; d < 2; d++) {
+            tb[2 * i + d] -= p->k_spring * (r0 - rr) * dx[d] / (rr * s->mass[i]);
+            tb[2 * i + d] -= p->c_damp * rvij * dx[d] / (rr * rr * s->mass[i]);
+          }
+        }
+      }
+    SCATTER_END(dst)
+  }
+  return 0;
+}
+
+/* Leapfrog time step (sph.cc:90-168), fluctuation fields zero. */
+int orc_step(orc_sim *s) {
+  const double dt = s->p.dt;
+  const int nf = s->nf, n = s->n;
+  int itime = ++s->itime;
+  double t = itime * dt;
+  if (itime != 1) {
+    memcpy(s->rhomin, s->rho, sizeof(double) * nf);
+    memcpy(s->rholoc, s->rho, sizeof(double) * nf);
+    memcpy(s->vloc, s->v, sizeof(double) * 2 * n);
+    memcpy(s->vmin, s->v, sizeof(double) * 2 * n);
+    for (int i = 0; i < nf; i++) {
+      s->rho[i] += 0.5 * dt * s->Rt[i];
+      s->rholoc[i] = s->rholoc[i] + 0.5 * dt * s->Rt[i];
+    }
+    for (int i = 0; i < nf; i++)
+      for (int d = 0; d < 2; d++) {
+        s->vmin[2 * i + d] = s->v[2 * i + d];
+        s->v[2 * i + d] += 0.5 * dt * s->at[2 * i + d];
+        s->vloc[2 * i + d] = s->v[2 * i + d] + 0.5 * dt * s->at[2 * i + d];
+      }
+  }
+  if (orc_derivatives(s, t) != 0) return -1;
+  if (itime == 1) {
+    for (int i = 0; i < nf; i++) {
+      s->rholoc[i] = s->rho[i] + 0.5 * dt * s->R[i];
+      s->rho[i] += 0.5 * dt * s->R[i];
+    }
+    for (int i = 0; i < nf; i++)
+      for (int d = 0; d < 2; d++) {
+        s->vloc[2 * i + d] = s->v[2 * i + d] + 0.5 * dt * s->a[2 * i + d];
+        s->v[2 * i + d] += 0.5 * dt * s->a[2 * i + d];
+        s->x[2 * i + d] += dt * s->v[2 * i + d];
+      }
+  } else {
+    for (int i = 0; i < nf; i++) {
+      s->rholoc[i] = s->rhomin[i] + dt * s->R[i];
+      s->rho[i] = s->rhomin[i] + dt * s->R[i];
+    }
+    for (int i = 0; i < nf; i++)
+      for (int d = 0; d < 2; d++) {
+        s->vloc[2 * i + d] = s->vmin[2 * i + d] + dt * s->a[2 * i + d];
+        s->v[2 * i + d] = s->vmin[2 * i + d] + dt * s->a[2 * i + d];
+        s->x[2 * i + d] += dt * s->v[2 * i + d];
+      }
+  }
+  return 0;
+}
+
+int orc_run(orc_sim *s, int nsteps) {
+  for (int k = 0; k < nsteps; k++)
+    if (orc_step(s) != 0) return -1;
+  return 0;
+}
+
+/* Accessors (all arrays in reference order, length n or 2n). */
+void orc_get_state(const orc_sim *s, double *x, double *v, double *rho, double *vloc,
+                   double *rholoc) {
+  if (x) memcpy(x, s->x, sizeof(double) * 2 * s->n);
+  if (v) memcpy(v, s->v, sizeof(double) * 2 * s->n);
+  if (rho) memcpy(rho, s->rho, sizeof(double) * s->n);
+  if (vloc) memcpy(vloc, s->vloc, sizeof(double) * 2 * s->n);
+  if (rholoc) memcpy(rholoc, s->rholoc, sizeof(double) * s->n);
+}
+
+void orc_get_rates(const orc_sim *s, double *R, double *Rt, double *a, double *at, double *P) {
+  if (R) memcpy(R, s->R, sizeof(double) * s->n);
+  if (Rt) memcpy(Rt, s->Rt, sizeof(double) * s->n);
+  if (a) memcpy(a, s->a, sizeof(double) * 2 * s->n);
+  if (at) memcpy(at, s->at, sizeof(double) * 2 * s->n);
+  if (P) memcpy(P, s->P, sizeof(double) * s->n);
+}
+
+int orc_niac(const orc_sim *s) { return s->niac; }
+int orc_itime(const orc_sim *s) { return s->itime; }

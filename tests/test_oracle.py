@@ -1,0 +1,379 @@
+"""Pins the FP64 oracle (oracle/uph_oracle.c) without running the reference.
+
+The reference's build needs <mkl.h> and libnabo, absent here, so every oracle
+function is pinned by mathematics and independent brute force instead:
+closed forms of the cubic spline, an independent numpy statement of the
+SearchNeighbors contract, the SPH identities the pair sums must satisfy
+(divergence of a linear velocity field, hydrostatic balance, Galilean
+invariance, momentum conservation), the Tait inverse, and the exact
+constant-acceleration trajectory of the leapfrog.
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy import integrate
+
+from conftest import load_golden_scene
+from uph_amd import scenes
+
+
+def np_search(pos, nf, n, pb, K, dist):
+    """SearchNeighbors contract by full sort: K nearest other than s by (d2, j)."""
+    pos = np.asarray(pos, dtype=np.float64)
+    idx = np.arange(n)
+    out_i, out_j, out_d = [], [], []
+    for s in range(nf):
+        d = pos[:n] - pos[s]
+        d2 = d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]
+        m = idx != s
+        cj, cd = idx[m], d2[m]
+        order = np.lexsort((cj, cd))[:K]
+        for j, dd in zip(cj[order], cd[order]):
+            if j > s and dd < dist and j < pb:
+                out_i.append(s); out_j.append(j); out_d.append(dd)
+    return np.array(out_i, np.int32), np.array(out_j, np.int32), np.array(out_d)
+
+
+# ------------------------------------------------------------------ kernel
+
+H = 0.026
+
+
+def test_cubic_w0(oracle_lib):
+    w, gx, gy = oracle_lib.cubic(0.0, 0.0, 0.0, H)
+    assert w == pytest.approx(10.0 / (7 * math.pi * H * H), rel=1e-15)
+    assert gx == 0.0 and gy == 0.0
+
+
+def test_cubic_normalised(oracle_lib):
+    f = lambda r: 2 * math.pi * r * oracle_lib.cubic(r, 0.0, r, H)[0]
+    val = sum(integrate.quad(f, a, b, epsabs=1e-14, epsrel=1e-13)[0]
+              for a, b in [(0, H), (H, 2 * H)])
+    assert val == pytest.approx(1.0, abs=1e-11)
+
+
+def test_cubic_compact_and_continuous(oracle_lib):
+    for q in (1.0, 2.0):
+        lo = oracle_lib.cubic(q * H * (1 - 1e-12), 0, q * H * (1 - 1e-12), H)
+        hi = oracle_lib.cubic(q * H * (1 + 1e-12), 0, q * H * (1 + 1e-12), H)
+        assert lo[0] == pytest.approx(hi[0], abs=1e-6)
+        assert lo[1] == pytest.approx(hi[1], rel=1e-6, abs=1e-3)
+    assert oracle_lib.cubic(2.0 * H, 0, 2.0 * H, H)[0] == 0.0
+
+
+def test_cubic_gradient_is_derivative_of_w(oracle_lib):
+    rng = np.random.default_rng(1)
+    for _ in range(50):
+        r = rng.uniform(0.05, 1.95) * H
+        th = rng.uniform(0, 2 * math.pi)
+        dx, dy = r * math.cos(th), r * math.sin(th)
+        _, gx, gy = oracle_lib.cubic(dx, dy, r, H)
+        e = 1e-7 * H
+        wp = oracle_lib.cubic(dx + e, dy, math.hypot(dx + e, dy), H)[0]
+        wm = oracle_lib.cubic(dx - e, dy, math.hypot(dx - e, dy), H)[0]
+        assert gx == pytest.approx((wp - wm) / (2 * e), rel=1e-5, abs=1e-3)
+        _, gx2, gy2 = oracle_lib.cubic(-dx, -dy, r, H)
+        assert gx2 == -gx and gy2 == -gy
+
+
+# ------------------------------------------------------------------ search
+
+def _search_cases():
+    sc1 = scenes.dam_break(width=0.4, height=0.4, dx=0.02, tank=(0.8, 0.6))
+    scj = scenes.dam_break(width=0.4, height=0.4, dx=0.02, tank=(0.8, 0.6), jitter=0.05, seed=7)
+    return {"lattice": sc1, "jitter": scj}
+
+
+@pytest.mark.parametrize("case", ["lattice", "jitter"])
+def test_search_matches_numpy_statement(oracle_lib, case):
+    sc = _search_cases()[case]
+    ref = np_search(sc.pos, sc.n_fluid, sc.n, sc.particle_boundary, 40, sc.cutoff2)
+    for brute in (False, True):
+        got = oracle_lib.search(sc.pos, sc.n_fluid, sc.n, sc.particle_boundary, 40, sc.cutoff2,
+                                brute=brute)
+        for a, b in zip(got, ref):
+            assert np.array_equal(a, b)
+
+
+def test_search_lattice_counts(oracle_lib):
+    """h = 1.3 dx: 20 lattice points within 2h (d^2 = 1,2,4,5 dx^2), 10 pairs each."""
+    sc = scenes.dam_break(width=0.6, height=0.6, dx=0.02, tank=(1.2, 1.0))
+    ip, jp, r2 = oracle_lib.search(sc.pos, sc.n_fluid, sc.n, sc.particle_boundary, 40,
+                                   sc.cutoff2)
+    assert np.all(jp > ip)
+    assert np.all(r2 < sc.cutoff2) and np.all(r2 > 0)
+    assert not np.any(jp == sc.n - 1)          # the ParticleBoundary off-by-one
+    x = sc.pos[:sc.n_fluid]
+    deg = np.bincount(ip, minlength=sc.n) + np.bincount(jp, minlength=sc.n)
+    interior = ((x[:, 0] > 0.1) & (x[:, 0] < 0.5) & (x[:, 1] > 0.1) & (x[:, 1] < 0.5))
+    assert np.all(deg[:sc.n_fluid][interior] == 20)
+    d2u = np.unique(np.round(r2 / sc.dx ** 2, 9))
+    assert set(d2u.tolist()) <= {1.0, 2.0, 4.0, 5.0}
+
+
+def test_search_reference_deck_hydrostatic(oracle_lib):
+    """The reference's relaxed hydrostatic deck (FluidParticleInit.dat)."""
+    sc = load_golden_scene("hydrostatic_init.npz")
+    ref = np_search(sc.pos, sc.n_fluid, sc.n, sc.particle_boundary, 40, sc.cutoff2)
+    got = oracle_lib.search(sc.pos, sc.n_fluid, sc.n, sc.particle_boundary, 40, sc.cutoff2)
+    for a, b in zip(got, ref):
+        assert np.array_equal(a, b)
+
+
+def test_search_reference_deck_cap_binds(oracle_lib):
+    """tests/dambreak: nearNeighbor = 91 puts ~88 particles within 2h, so K = 40 truncates."""
+    sc = load_golden_scene("dambreak.npz")
+    sub = 1200   # first fluid queries only (the numpy statement is O(N^2))
+    ref = np_search(sc.pos, sub, sc.n, sc.particle_boundary, 40, sc.cutoff2)
+    ip, jp, r2 = oracle_lib.search(sc.pos, sc.n_fluid, sc.n, sc.particle_boundary, 40,
+                                   sc.cutoff2)
+    m = ip < sub
+    for a, b in zip((ip[m], jp[m], r2[m]), ref):
+        assert np.array_equal(a, b)
+    # the cap really binds: some query has more than 40 candidates in range
+    d = sc.pos[:sc.n] - sc.pos[500]
+    assert np.count_nonzero((d * d).sum(1) < sc.cutoff2) - 1 > 40
+
+
+def test_search_edge_cases(oracle_lib):
+    pos = np.array([[0.0, 0.0], [0.5, 0.0], [0.0, 0.5]])
+    assert len(oracle_lib.search(pos, 0, 3, 2, 40, 1.0)[0]) == 0
+    ip, jp, _ = oracle_lib.search(pos, 3, 3, 2, 40, 1.0)       # pb = 2 drops j = 2
+    assert list(zip(ip, jp)) == [(0, 1)]
+    ip, jp, _ = oracle_lib.search(pos, 3, 3, 3, 1, 1.0)        # K = 1: one nearest each
+    assert list(zip(ip, jp)) == [(0, 1)]                      # tie (0,1)/(0,2) -> lower index
+    ip, jp, _ = oracle_lib.search(pos, 1, 3, 3, 40, 0.25)      # strict d2 < dist
+    assert len(ip) == 0
+    with pytest.raises(RuntimeError):
+        oracle_lib.search(pos, 3, 3, 3, 40, 1.0, cap=1)
+
+
+# --------------------------------------------------------------- pair sums
+
+def _fluid_patch(n=24, dx=0.01, jitter=0.1, seed=3):
+    """Fluid-only lattice patch (no boundary), m = rho0 dx^2, h = 1.3 dx."""
+    rng = np.random.default_rng(seed)
+    i, j = np.meshgrid(np.arange(n), np.arange(n), indexing="xy")
+    pos = np.stack([(i.ravel() + 0.5) * dx, (j.ravel() + 0.5) * dx], 1)
+    pos += rng.uniform(-jitter * dx, jitter * dx, pos.shape)
+    N = len(pos)
+    c0, B = scenes.tait_constants(1000.0, 7.0, 9.81, 1.0)
+    return scenes.Scene(pos=pos, vel=np.zeros((N, 2)), rho=np.full(N, 1000.0),
+                        pressure=np.zeros(N), mass=np.full(N, 1000.0 * dx * dx),
+                        itype=np.ones(N, np.int32), lvel=np.zeros((N, 2)), n_fluid=N,
+                        n_fixed=0, n_moving=0, dx=dx, h=1.3 * dx, c0=c0, B=B, gravity=0,
+                        alpha_av=0.1, dt=0.25 * 1.3 * dx / c0,
+                        box=(-0.05, -0.05, n * dx + 0.05, n * dx + 0.05))
+
+
+def _interior(sc, margin):
+    x = sc.pos[:sc.n_fluid]
+    lo, hi = x.min(0) + margin, x.max(0) - margin
+    return np.all((x > lo) & (x < hi), axis=1)
+
+
+def test_continuity_divergence_of_linear_field(oracle_lib):
+    """v = a (x - xc): SPH continuity gives R = -rho div v = -2 a rho in the interior."""
+    sc = _fluid_patch(jitter=0.0)
+    a = 0.7
+    xc = sc.pos.mean(0)
+    sc.vel = a * (sc.pos - xc)
+    sc.lvel = sc.vel.copy()
+    from oracle.oracle import Sim
+    s = Sim(sc)
+    s.derivatives(sc.dt)
+    r = s.rates()
+    m = _interior(sc, 3 * sc.h)
+    # exact lattice sum: (v_i - v_j).grad_i W = a r W'(r) for every lattice offset in range
+    h, dx = sc.h, sc.dx
+    alpha = 10.0 / (7 * math.pi * h * h)
+    lat = 0.0
+    for di in range(-3, 4):
+        for dj in range(-3, 4):
+            rr = dx * math.hypot(di, dj)
+            if 0 < rr and rr * rr < 4 * h * h:
+                qq = rr / h
+                dwdq = alpha * (-3 * qq + 2.25 * qq * qq) if qq < 1 else -0.75 * alpha * (2 - qq) ** 2
+                lat += sc.mass[0] * a * rr * dwdq / h
+    assert np.allclose(r["R"][m], lat, rtol=1e-12)
+    assert np.allclose(r["Rt"][m], lat, rtol=1e-12)
+    # and the continuum value -rho div v within the lattice's discretisation error
+    assert np.allclose(r["R"][m], -2 * a * 1000.0, rtol=2e-2)
+
+
+def test_hydrostatic_balance_interior(oracle_lib):
+    """Tait-inverse hydrostatic density: pressure gradient cancels gravity (a ~ 0)."""
+    sc = scenes.dam_break(width=0.6, height=0.6, dx=0.01, tank=(1.0, 1.0))
+    from oracle.oracle import Sim
+    s = Sim(sc)
+    s.derivatives(sc.dt)
+    r = s.rates()
+    m = _interior(sc, 4 * sc.h)
+    assert np.abs(r["a"][:sc.n_fluid][m]).max() < 0.02 * sc.g
+    # P(rho) recovers rho0 g (H - y): the Tait inverse used to build the deck
+    H = 60 * 0.01
+    y = sc.pos[:sc.n_fluid, 1]
+    assert np.allclose(r["P"][:sc.n_fluid], 1000.0 * 9.81 * (H - y), rtol=1e-9, atol=1e-7)
+    assert np.all(r["P"][sc.n_fluid:] == 0.0)          # boundary at rho0: P(rho0) = 0
+
+
+def test_galilean_invariance(oracle_lib):
+    sc = _fluid_patch()
+    sc.vel[:] = (0.3, -0.2)
+    sc.lvel = sc.vel.copy()
+    from oracle.oracle import Sim
+    s = Sim(sc)
+    s.derivatives(sc.dt)
+    r = s.rates()
+    scale = 1000.0 * 0.3 / sc.h
+    assert np.abs(r["R"]).max() < 1e-9 * scale
+    assert np.abs(r["Rt"]).max() < 1e-9 * scale
+
+
+def test_momentum_conserved(oracle_lib):
+    """No gravity, no walls: pairwise antisymmetry gives sum m a = 0 (AV included)."""
+    sc = _fluid_patch(jitter=0.2)
+    rng = np.random.default_rng(5)
+    sc.vel = rng.normal(0, 0.5, sc.vel.shape)
+    sc.lvel = sc.vel.copy()
+    sc.rho = 1000.0 * (1 + rng.uniform(0, 0.01, sc.n))
+    from oracle.oracle import Sim
+    s = Sim(sc)
+    s.derivatives(sc.dt)
+    r = s.rates()
+    ma = sc.mass[:, None] * r["a"]
+    assert np.abs(ma.sum(0)).max() < 1e-9 * np.abs(ma).sum()
+
+
+def test_artificial_viscosity_dissipates():
+    """Two approaching particles at rho0 (P = 0): AV decelerates both; receding: no force."""
+    from oracle.oracle import Sim
+    dx = 0.01
+    for sign, expect in ((+1, True), (-1, False)):
+        # a distant third particle: the last index is never a neighbour (Search.cc:33)
+        pos = np.array([[0.0, 0.0], [1.5 * dx, 0.0], [0.5, 0.5]])
+        vel = np.array([[sign * 1.0, 0.0], [-sign * 1.0, 0.0], [0.0, 0.0]])
+        sc = scenes.Scene(pos=pos, vel=vel, rho=np.full(3, 1000.0), pressure=np.zeros(3),
+                          mass=np.full(3, 0.1), itype=np.ones(3, np.int32), lvel=vel.copy(),
+                          n_fluid=3, n_fixed=0, n_moving=0, dx=dx, h=1.3 * dx, c0=40.0,
+                          B=1000 * 1600 / 7, gravity=0, alpha_av=0.1, dt=1e-5,
+                          box=(-1, -1, 1, 1))
+        s = Sim(sc)
+        s.derivatives(1e-5)
+        a = s.rates()["a"]
+        if expect:
+            assert a[0, 0] < 0 < a[1, 0]          # opposes the approach
+            assert a[0, 0] == pytest.approx(-a[1, 0], rel=1e-14)
+        else:
+            assert np.all(a[:2] == 0.0)
+
+
+def test_wall_spring_pushes_out():
+    from oracle.oracle import Sim
+    dx = 0.01
+    pos = np.array([[0.0, 1.0 * dx], [0.0, 0.0]])     # fluid above one wall particle
+    vel = np.array([[0.0, -0.1], [0.0, 0.0]])
+    sc = scenes.Scene(pos=pos, vel=vel, rho=np.full(2, 1000.0), pressure=np.zeros(2),
+                      mass=np.full(2, 0.1), itype=np.array([1, 2], np.int32), lvel=vel.copy(),
+                      n_fluid=1, n_fixed=1, n_moving=0, dx=dx, h=1.3 * dx, c0=40.0,
+                      B=1000 * 1600 / 7, gravity=0, alpha_av=0.0, k_spring=2.0, c_damp=0.01,
+                      dt=1e-5, K=40, box=(-1, -1, 1, 1))
+    # the ParticleBoundary quirk (j < Nf+Nb-1) drops the only wall particle: add a second
+    pos = np.vstack([pos, [[5 * dx, 0.0]]])
+    sc = scenes.Scene(**{**sc.__dict__, "pos": pos, "vel": np.vstack([vel, [[0, 0]]]),
+                         "rho": np.full(3, 1000.0), "pressure": np.zeros(3),
+                         "mass": np.full(3, 0.1), "itype": np.array([1, 2, 2], np.int32),
+                         "lvel": np.vstack([vel, [[0, 0]]]), "n_fixed": 2})
+    s = Sim(sc)
+    s.derivatives(1e-5)
+    a = s.rates()["a"][0]
+    # derivatives.c:2433-2437: a_i -= k0 (r0 - r) dx/(r m_i) + c0 (dx.dv) dx/(r^2 m_i) with
+    # dx = x_i - x_j: the spring term points from the fluid to the wall (as written in the
+    # reference), the damping term opposes the approach.
+    r0 = 3 * dx
+    k_part = 2.0 * (r0 - dx) * dx / (dx * 0.1)
+    c_part = 0.01 * (dx * -0.1) * dx / (dx * dx * 0.1)
+    assert a[1] == pytest.approx(-k_part - c_part, rel=1e-12)
+    assert -c_part > 0                       # damping pushes the approaching particle back
+    assert a[0] == 0.0
+
+
+def test_leapfrog_free_fall_exact():
+    """Isolated particles under gravity: x_n = x0 + v0 t - g t^2 / 2 (sph.cc:90-168)."""
+    from oracle.oracle import Sim
+    N = 3
+    pos = np.array([[0.0, 0.0], [10.0, 0.0], [20.0, 5.0]])
+    vel = np.array([[0.5, 1.0], [0.0, 0.0], [-1.0, 2.0]])
+    sc = scenes.Scene(pos=pos, vel=vel, rho=np.full(N, 1000.0), pressure=np.zeros(N),
+                      mass=np.full(N, 0.1), itype=np.ones(N, np.int32), lvel=vel.copy(),
+                      n_fluid=N, n_fixed=0, n_moving=0, dx=0.01, h=0.013, c0=40.0,
+                      B=1000 * 1600 / 7, gravity=1, alpha_av=0.1, dt=1e-3,
+                      box=(-1, -50, 30, 10))
+    s = Sim(sc)
+    n = 500
+    s.step(n)
+    t = n * sc.dt
+    st = s.state()
+    xe = pos + vel * t + np.array([0.0, -0.5 * 9.81 * t * t])
+    ve = vel + np.array([0.0, -9.81 * (t - 0.5 * sc.dt)])
+    assert np.allclose(st["x"], xe, rtol=0, atol=1e-11)
+    assert np.allclose(st["v"], ve, rtol=0, atol=1e-11)
+    assert np.all(st["rho"] == 1000.0)
+
+
+def test_gather_form_agrees_with_scatter(oracle_lib):
+    """Per-particle gather sums over an independently found pair set equal the oracle."""
+    sc = scenes.dam_break(width=0.3, height=0.3, dx=0.02, tank=(0.6, 0.5), jitter=0.1, seed=2)
+    rng = np.random.default_rng(9)
+    sc.vel[:sc.n_fluid] = rng.normal(0, 0.3, (sc.n_fluid, 2))
+    sc.lvel[:sc.n_fluid] = sc.vel[:sc.n_fluid] + rng.normal(0, 0.05, (sc.n_fluid, 2))
+    from oracle.oracle import Sim
+    s = Sim(sc)
+    s.derivatives(sc.dt)
+    st, r = s.state(), s.rates()
+    x, v, vl, rho, m = st["x"], st["v"], st["vloc"], st["rho"], sc.mass
+    ip, jp, r2 = np_search(x, sc.n_fluid, sc.n, sc.particle_boundary, 40, sc.cutoff2)
+    h = sc.h
+    alpha = 10.0 / (7 * math.pi * h * h)
+    P = sc.B * ((rho / sc.rho0) ** 7 - 1)
+    R = np.zeros(sc.n); Rt = np.zeros(sc.n); A = np.zeros((sc.n, 2))
+    for i, j, d2 in zip(ip, jp, r2):
+        for p, q in ((i, j), (j, i)):
+            dxp = x[p] - x[q]
+            rr = math.sqrt(d2)
+            qq = rr / h
+            dwdq = alpha * (-3 * qq + 2.25 * qq * qq) if qq < 1 else -0.75 * alpha * (2 - qq) ** 2
+            g = dwdq / h * dxp / rr                      # grad_p W_pq from dW/dq
+            R[p] += m[q] * np.dot(v[p] - vl[q], g)       # continuity, gather form
+            Rt[p] += (m[q] * np.dot(v[p] - vl[q], g) if p == i
+                      else m[p] * np.dot(vl[p] - v[q], g))
+            vr = np.dot(v[p] - v[q], dxp)
+            Pi = 0.0
+            if vr < 0:
+                Pi = -sc.alpha_av * sc.c0 * (h * vr / (d2 + 0.01 * h * h)) / (0.5 * (rho[p] + rho[q]))
+            A[p] -= m[q] * (P[p] / rho[p] ** 2 + P[q] / rho[q] ** 2 + Pi) * g
+            if p == i and sc.itype[q] in (2, 3, 5) and rr < 3 * sc.dx:
+                A[p] -= sc.k_spring * (3 * sc.dx - rr) * dxp / (rr * m[p])
+                A[p] -= sc.c_damp * np.dot(dxp, v[p] - v[q]) * dxp / (d2 * m[p])
+    A[:, 1] -= sc.g
+    nf = sc.n_fluid
+    sR = np.abs(R[:nf]).max()
+    assert np.abs(r["R"][:nf] - R[:nf]).max() < 1e-11 * sR
+    assert np.abs(r["Rt"][:nf] - Rt[:nf]).max() < 1e-11 * sR
+    sA = np.abs(A[:nf]).max()
+    assert np.abs(r["a"][:nf] - A[:nf]).max() < 1e-11 * sA
+    assert np.abs(r["at"][:nf] - A[:nf]).max() < 1e-11 * sA
+
+
+def test_multistep_dam_break_stays_physical(oracle_lib):
+    from oracle.oracle import Sim
+    sc = scenes.dam_break(width=0.5, height=1.0, dx=0.02)
+    s = Sim(sc, nthreads=4)
+    s.step(300)
+    st = s.state()
+    nf = sc.n_fluid
+    assert np.all(np.isfinite(st["x"]))
+    assert st["rho"][:nf].min() > 950 and st["rho"][:nf].max() < 1100
+    assert st["x"][:nf, 1].min() > -sc.dx       # no particle through the floor
+    assert np.abs(st["v"][:nf]).max() < 2 * math.sqrt(2 * 9.81 * 1.0)

@@ -1,0 +1,1100 @@
+// uph_gpu.hip -- kernels and C ABI of the MI355X SPH hot path.
+//
+// One step of sph.cc:90-168 (the predictor, derivatives() of
+// derivatives.c:3137 in the scheme's configuration, the corrector) is, for all
+// scenes of a batch at once:
+//   k_bin        cell key of every particle, rank inside its cell (atomic)
+//   k_scan_*     exclusive scan of cell counts -> cell start offsets
+//   k_scatter    provisional sorted slot of every particle
+//   k_reorder    final slot by (cell, original index), gather of the FP64
+//                state, predictor and FP64 Tait pressure -> FP32 neighbour mirror
+//   k_pair       fused neighbour search + W/gradW + continuity (both forms) +
+//                pressure gradient + artificial viscosity + gravity + wall
+//                springs, FP32 compensated sums, FP64 leapfrog corrector
+//   k_kth/k_pair<exact>  only when some fluid particle has more than K
+//                neighbours in range: the capped (K-nearest) neighbour sets
+//                of Search.cc:27-40, then the pair sums again.
+#include "uph_kernels.h"
+#include "../../include/uph.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <string>
+#include <vector>
+#include <mutex>
+#include <algorithm>
+#include <cmath>
+
+#define BLK 256
+#define SCAN_ITEMS 16
+#define SCAN_BLK 256
+#define SCAN_TILE (SCAN_ITEMS * SCAN_BLK)
+
+// ------------------------------------------------------------------ sort
+
+__global__ __launch_bounds__(BLK) void k_bin(int n, const double2* __restrict__ pos, const int* __restrict__ sceneOf,
+                                             const DevScene* __restrict__ sc, const uint32_t* __restrict__ tagA,
+                                             uint32_t* __restrict__ tagB, int* __restrict__ keys,
+                                             int* __restrict__ rank, int* __restrict__ count) {
+  const int i = blockIdx.x * BLK + threadIdx.x;
+  if (i >= n) return;
+  const DevScene& S = sc[sceneOf[i]];
+  const double2 p = pos[i];
+  int cx, cy;
+  cell_of(S, p.x, p.y, cx, cy);
+  const int key = S.coff + cy * S.ncx + cx;
+  keys[i] = key;
+  rank[i] = atomicAdd(&count[key], 1);
+  tagB[i] = tagA[i];
+}
+
+// Block-wide exclusive scan of SCAN_TILE ints; returns the tile total.
+__device__ int block_scan_tile(int* vals /* SCAN_ITEMS per thread, in registers via array */, int* lds) {
+  const int tid = threadIdx.x;
+  int sum = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    const int v = vals[k];
+    vals[k] = sum;
+    sum += v;
+  }
+  // wavefront inclusive scan of thread sums
+  const int lane = tid & 63, wid = tid >> 6;
+  int x = sum;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) lds[wid] = x;
+  __syncthreads();
+  if (tid == 0) {
+    int acc = 0;
+    for (int w = 0; w < SCAN_BLK / 64; ++w) {
+      const int t = lds[w];
+      lds[w] = acc;
+      acc += t;
+    }
+    lds[SCAN_BLK / 64] = acc;
+  }
+  __syncthreads();
+  const int excl = x - sum + lds[wid];
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) vals[k] += excl;
+  const int total = lds[SCAN_BLK / 64];
+  __syncthreads();
+  return total;
+}
+
+__global__ __launch_bounds__(SCAN_BLK) void k_scan_reduce(int m, const int* __restrict__ in, int* __restrict__ partial) {
+  __shared__ int lds[SCAN_BLK / 64 + 1];
+  const long base = (long)blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+  int s = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    const long i = base + k;
+    if (i < m) s += in[i];
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+  if (lane == 0) lds[wid] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int w = 0; w < SCAN_BLK / 64; ++w) t += lds[w];
+    partial[blockIdx.x] = t;
+  }
+}
+
+// Single block: exclusive scan of the per-tile partials.  Also opens a step
+// (itime += 1, overflow flag cleared) when `step_ctl` is given.
+__global__ __launch_bounds__(1024) void k_scan_top(int nb, int* __restrict__ partial, int* __restrict__ step_ctl) {
+  __shared__ int lds[1024 / 64 + 1];
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int b0 = 0; b0 < nb; b0 += 1024) {
+    const int i = b0 + threadIdx.x;
+    const int v = i < nb ? partial[i] : 0;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) lds[wid] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int acc = 0;
+      for (int w = 0; w < 16; ++w) {
+        const int t = lds[w];
+        lds[w] = acc;
+        acc += t;
+      }
+      lds[16] = acc;
+    }
+    __syncthreads();
+    if (i < nb) partial[i] = carry + lds[wid] + x - v;
+    __syncthreads();
+    if (threadIdx.x == 0) carry += lds[16];
+    __syncthreads();
+  }
+  if (step_ctl && threadIdx.x == 0) {
+    step_ctl[0] += 1;   // itime of the step being computed
+    step_ctl[1] = 0;    // overflow flag of this step
+  }
+}
+
+// Exclusive scan of `count` into `start` (m+1 entries), zeroing `count`.
+__global__ __launch_bounds__(SCAN_BLK) void k_scan_down(int m, int* __restrict__ count, const int* __restrict__ partial,
+                                                        int* __restrict__ start) {
+  __shared__ int lds[SCAN_BLK / 64 + 1];
+  const long base = (long)blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+  int v[SCAN_ITEMS];
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    const long i = base + k;
+    v[k] = i < m ? count[i] : 0;
+  }
+  block_scan_tile(v, lds);
+  const int off = partial[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    const long i = base + k;
+    if (i < m) {
+      start[i] = v[k] + off;
+      if (i == m - 1) start[m] = v[k] + off + count[i];
+      count[i] = 0;
+    }
+  }
+}
+
+__global__ __launch_bounds__(BLK) void k_scatter(int n, const int* __restrict__ keys, const int* __restrict__ rank,
+                                                 const int* __restrict__ cellStart, int* __restrict__ tmpIdx) {
+  const int i = blockIdx.x * BLK + threadIdx.x;
+  if (i >= n) return;
+  tmpIdx[cellStart[keys[i]] + rank[i]] = i;
+}
+
+struct StepBufs {
+  // A: sorted input of the pair kernel
+  double2 *posA, *velA;
+  double* rhoA;
+  float4* rtA;
+  uint32_t* tagA;
+  float4 *mA, *mB;
+  // B: pair-kernel output, same slot order as A
+  double2 *posB, *velB;
+  double* rhoB;
+  float4* rtB;
+  uint32_t* tagB;
+  // sort scratch
+  int *keys, *rank, *tmpIdx, *cellStart, *cellCount, *partial;
+  int* sceneOf;
+  // capped search
+  double* kthD2;
+  uint32_t* kthId;
+  // per original index (global id)
+  double2 *deckPos, *deckVel, *deckLvel;
+  // control: [0] itime, [1] overflow flag of the step, [2] overflowed steps
+  int* ctl;
+  float4* dbgRates;
+};
+
+// Final slot = cell start + number of same-cell particles with a smaller
+// original index (deterministic order); gather the state and build the
+// predictor mirror (sph.cc:93-126: rho += dt/2 Rt, v += dt/2 at,
+// vloc = v + dt/2 at; Tait pressure of the predicted density).
+__global__ __launch_bounds__(BLK) void k_reorder(int n, StepBufs b, const DevScene* __restrict__ sc, int search_only) {
+  const int d = blockIdx.x * BLK + threadIdx.x;
+  if (d >= n) return;
+  const int i = b.tmpIdx[d];
+  const int c = b.keys[i];
+  const int c0 = b.cellStart[c], c1 = b.cellStart[c + 1];
+  const uint32_t tg = b.tagB[i];
+  const uint32_t my = tag_id(tg);
+  int r = 0;
+  for (int t = c0; t < c1; ++t) r += (tag_id(b.tagB[b.tmpIdx[t]]) < my) ? 1 : 0;
+  const int dst = c0 + r;
+  const double2 p = b.posB[i];
+  b.posA[dst] = p;
+  b.tagA[dst] = tg;
+  if (search_only) return;
+  const DevScene& S = sc[b.sceneOf[i]];
+  const double2 v = b.velB[i];
+  const double rho = b.rhoB[i];
+  const float4 rt = b.rtB[i];
+  b.velA[dst] = v;
+  b.rhoA[dst] = rho;
+  b.rtA[dst] = rt;
+  const int itime = b.ctl[0];
+  const double dt = S.dt;
+  double vx = v.x, vy = v.y, lx, ly, rp = rho;
+  if (tag_type(tg) == 1) {
+    if (itime != 1) {
+      rp = rho + 0.5 * dt * (double)rt.x;
+      vx = v.x + 0.5 * dt * (double)rt.y;
+      vy = v.y + 0.5 * dt * (double)rt.z;
+      lx = vx + 0.5 * dt * (double)rt.y;
+      ly = vy + 0.5 * dt * (double)rt.z;
+    } else {
+      const double2 l0 = b.deckLvel[my];
+      lx = l0.x;
+      ly = l0.y;
+    }
+  } else {
+    const double2 l0 = b.deckLvel[my];
+    lx = l0.x;
+    ly = l0.y;
+    if (tag_type(tg) == 3 && S.motion != 0) {
+      double xx = p.x, vv = v.x;
+      wall_motion(S, itime * dt, b.deckPos[my].x, xx, vv);
+      lx = vv;
+    }
+  }
+  const double P = tait(S, rp);
+  const double pr = P / (rp * rp);
+  b.mA[dst] = make_float4((float)vx, (float)vy, (float)lx, (float)ly);
+  b.mB[dst] = make_float4((float)rp, (float)pr, rt.w, __uint_as_float(tg));
+}
+
+// ------------------------------------------------------------- pair kernel
+
+// Blocks are spread over the 8 XCDs round-robin by the dispatcher; remap so
+// each XCD works on one contiguous run of slots (shared neighbour rows stay in
+// that XCD's L2).  Grid size is a multiple of 8.
+__device__ __forceinline__ int xcd_block() {
+  const int nb = gridDim.x;
+  const int per = nb >> 3;
+  const int b = blockIdx.x;
+  return (b & 7) * per + (b >> 3);
+}
+
+// Count of in-range candidates (d2 < cutoff, not itself) of slot p and, when
+// it exceeds K, the K-th smallest (d2, id) key: the neighbour set of
+// Search.cc:27-40 is then the in-range candidates with key <= that key.
+__device__ void kth_key(const DevScene& S, int p, const double2* __restrict__ pos, const uint32_t* __restrict__ tag,
+                        const int* __restrict__ cellStart, int& cnt, double& kd2, uint32_t& kid) {
+  const double2 pp = pos[p];
+  const double cut = S.cutoff;
+  int c = 0;
+  for_candidates(S, pp, cellStart, [&](int q) {
+    if (q != p && dist2(pp, pos[q]) < cut) ++c;
+  });
+  cnt = c;
+  kd2 = INFINITY;
+  kid = 0xffffffffu;
+  if (c <= S.K) return;
+  // K-th smallest by repeated minimum above the previous threshold
+  double td = -1.0;
+  uint32_t ti = 0;
+  for (int k = 0; k < S.K; ++k) {
+    double bd = INFINITY;
+    uint32_t bi = 0xffffffffu;
+    for_candidates(S, pp, cellStart, [&](int q) {
+      if (q == p) return;
+      const double d2 = dist2(pp, pos[q]);
+      if (!(d2 < cut)) return;
+      const uint32_t id = tag_id(tag[q]);
+      const bool above = d2 > td || (d2 == td && id > ti);
+      if (above && (d2 < bd || (d2 == bd && id < bi))) {
+        bd = d2;
+        bi = id;
+      }
+    });
+    td = bd;
+    ti = bi;
+  }
+  kd2 = td;
+  kid = ti;
+}
+
+__global__ __launch_bounds__(BLK) void k_kth(int n, StepBufs b, const DevScene* __restrict__ sc) {
+  if (b.ctl[1] == 0) return;
+  const int p = xcd_block() * BLK + threadIdx.x;
+  if (p == 0) atomicAdd(&b.ctl[2], 1);
+  if (p >= n) return;
+  if (tag_type(b.tagA[p]) != 1) return;
+  const DevScene& S = sc[b.sceneOf[p]];
+  int cnt;
+  double kd2;
+  uint32_t kid;
+  kth_key(S, p, b.posA, b.tagA, b.cellStart, cnt, kd2, kid);
+  b.kthD2[p] = kd2;
+  b.kthId[p] = kid;
+}
+
+template <bool EXACT>
+__global__ __launch_bounds__(BLK) void k_pair(int n, StepBufs b, const DevScene* __restrict__ sc) {
+  if (EXACT && b.ctl[1] == 0) return;
+  const int p = xcd_block() * BLK + threadIdx.x;
+  if (p >= n) return;
+  const uint32_t tg = b.tagA[p];
+  const DevScene& S = sc[b.sceneOf[p]];
+  const int itime = b.ctl[0];
+  const double dt = S.dt;
+  const double2 xp = b.posA[p];
+  const double2 vn = b.velA[p];
+  const float4 rt = b.rtA[p];
+  if (tag_type(tg) != 1) {
+    // boundary: kinematics for the next step's time, state otherwise frozen
+    const uint32_t id = tag_id(tg);
+    double2 x = b.deckPos[id], v = b.deckVel[id];
+    if (tag_type(tg) == 3 && S.motion != 0) wall_motion(S, (itime + 1) * dt, x.x, x.x, v.x);
+    b.posB[p] = x;
+    b.velB[p] = v;
+    b.rhoB[p] = b.rhoA[p];
+    b.rtB[p] = rt;
+    return;
+  }
+  const uint32_t idp = tag_id(tg);
+  const float4 ap = b.mA[p];
+  const float4 bp = b.mB[p];
+  const float vpx = ap.x, vpy = ap.y, lpx = ap.z, lpy = ap.w;
+  const float rhop = bp.x, prp = bp.y, mp = bp.z;
+  const double cut = S.cutoff;
+  const float h = S.fh, invh = S.finvh, walpha = S.fwalpha;
+  const float h2eps = 0.01f * h * h;
+  const bool pb_ok = (int)idp < S.pb_gid;
+  double kpd2 = INFINITY;
+  uint32_t kpid = 0xffffffffu;
+  if (EXACT) {
+    kpd2 = b.kthD2[p];
+    kpid = b.kthId[p];
+  }
+  CSum sR, sRt, sAx, sAy;
+  int cnt = 0;
+  for_candidates(S, xp, b.cellStart, [&](int q) {
+    if (q == p) return;
+    const double2 xq = b.posA[q];
+    const double ex = xp.x - xq.x, ey = xp.y - xq.y;
+    const double d2 = __dadd_rn(__dmul_rn(ex, ex), __dmul_rn(ey, ey));
+    if (!(d2 < cut)) return;
+    ++cnt;
+    const float4 bq = b.mB[q];
+    const uint32_t tq = __float_as_uint(bq.w);
+    const uint32_t idq = tag_id(tq);
+    const bool p_is_i = idq > idp;   // pair (i=p, j=q) or (i=q, j=p) of Search.cc:33
+    if (p_is_i) {
+      if ((int)idq >= S.pb_gid) return;
+      if (EXACT && !key_le(d2, idq, kpd2, kpid)) return;
+    } else {
+      if (!pb_ok) return;
+      if (EXACT && !key_le(d2, idp, b.kthD2[q], b.kthId[q])) return;
+    }
+    const float4 aq = b.mA[q];
+    const float dx = (float)ex, dy = (float)ey;
+    const float r2 = (float)d2;
+    const float r = sqrtf(r2);
+    const float qq = r * invh;
+    // cubic spline gradient factor (derivatives.c:44-66): gradW = f * (x_p - x_q)
+    float f;
+    if (qq < 1.0f) {
+      f = walpha * (-3.0f + 2.25f * qq) * invh * invh;
+    } else {
+      const float t = 2.0f - qq;
+      f = -walpha * 0.75f * t * t * invh / r;
+    }
+    const float gx = f * dx, gy = f * dy;
+    const float mq = bq.z;
+    // continuity (derivatives.c:1520-1567), symmetric in the pair order
+    sR.add(mq * ((vpx - aq.z) * gx + (vpy - aq.w) * gy));
+    // predicting continuity (derivatives.c:1217-1252), depends on the pair order
+    if (p_is_i)
+      sRt.add(mq * ((vpx - aq.z) * gx + (vpy - aq.w) * gy));
+    else
+      sRt.add(mp * ((lpx - aq.x) * gx + (lpy - aq.y) * gy));
+    // pressure gradient (derivatives.c:1109-1163) + Monaghan viscosity
+    const float dvx = vpx - aq.x, dvy = vpy - aq.y;
+    const float vr = dvx * dx + dvy * dy;
+    float pij = prp + bq.y;
+    if (vr < 0.0f) {
+      const float mu = h * vr / (r2 + h2eps);
+      pij += -S.fav * S.fc0 * mu / (0.5f * (rhop + bq.x));
+    }
+    float ax = -mq * pij * gx;
+    float ay = -mq * pij * gy;
+    // wall spring (derivatives.c:2396-2449)
+    const uint32_t tyq = tag_type(tq);
+    if (p_is_i && (tyq == 2 || tyq == 3 || tyq == 5) && r < S.fr0) {
+      const float kr = S.fk * (S.fr0 - r) / (r * mp);
+      const float cr = S.fcd * vr / (r2 * mp);
+      ax -= kr * dx + cr * dx;
+      ay -= kr * dy + cr * dy;
+    }
+    sAx.add(ax);
+    sAy.add(ay);
+  });
+  if (!EXACT && cnt > S.K) atomicOr(&b.ctl[1], 1);
+  const float R = sR.val(), Rt = sRt.val();
+  float axf = sAx.val(), ayf = sAy.val();
+  double ax = (double)axf, ay = (double)ayf;
+  if (S.gravity == 1) ay -= S.g;     // BodyForce (derivatives.c:1167-1178)
+  // corrector (sph.cc:137-168), FP64
+  const double rho_n = b.rhoA[p];
+  double rho1, v1x, v1y;
+  if (itime == 1) {
+    rho1 = rho_n + 0.5 * dt * (double)R;
+    v1x = vn.x + 0.5 * dt * ax;
+    v1y = vn.y + 0.5 * dt * ay;
+  } else {
+    rho1 = rho_n + dt * (double)R;
+    v1x = vn.x + dt * ax;
+    v1y = vn.y + dt * ay;
+  }
+  b.posB[p] = make_double2(xp.x + dt * v1x, xp.y + dt * v1y);
+  b.velB[p] = make_double2(v1x, v1y);
+  b.rhoB[p] = rho1;
+  b.rtB[p] = make_float4(Rt, (float)ax, (float)ay, rt.w);
+  if (b.dbgRates) b.dbgRates[p] = make_float4(R, Rt, (float)ax, (float)ay);
+}
+
+// ---------------------------------------------------------------- search
+
+__global__ __launch_bounds__(BLK) void k_search_count(int nf_slots_unused, int n, StepBufs b, const DevScene* __restrict__ sc,
+                                                      int* __restrict__ qcount) {
+  const int p = xcd_block() * BLK + threadIdx.x;
+  if (p >= n) return;
+  const uint32_t tg = b.tagA[p];
+  if (tag_type(tg) != 1) return;
+  const DevScene& S = sc[0];
+  int cnt;
+  double kd2;
+  uint32_t kid;
+  kth_key(S, p, b.posA, b.tagA, b.cellStart, cnt, kd2, kid);
+  b.kthD2[p] = kd2;
+  b.kthId[p] = kid;
+  const uint32_t idp = tag_id(tg);
+  const double2 pp = b.posA[p];
+  int m = 0;
+  for_candidates(S, pp, b.cellStart, [&](int q) {
+    if (q == p) return;
+    const double d2 = dist2(pp, b.posA[q]);
+    if (!(d2 < S.cutoff)) return;
+    const uint32_t idq = tag_id(b.tagA[q]);
+    if (idq > idp && (int)idq < S.pb_gid && key_le(d2, idq, kd2, kid)) ++m;
+  });
+  qcount[idp] = m;
+}
+
+__global__ __launch_bounds__(BLK) void k_search_emit(int n, StepBufs b, const DevScene* __restrict__ sc,
+                                                     const int* __restrict__ qoff, int* __restrict__ ip,
+                                                     int* __restrict__ jp, double* __restrict__ r2) {
+  const int p = xcd_block() * BLK + threadIdx.x;
+  if (p >= n) return;
+  const uint32_t tg = b.tagA[p];
+  if (tag_type(tg) != 1) return;
+  const DevScene& S = sc[0];
+  const uint32_t idp = tag_id(tg);
+  const double kd2 = b.kthD2[p];
+  const uint32_t kid = b.kthId[p];
+  const double2 pp = b.posA[p];
+  const int o = qoff[idp];
+  for_candidates(S, pp, b.cellStart, [&](int q) {
+    if (q == p) return;
+    const double d2 = dist2(pp, b.posA[q]);
+    if (!(d2 < S.cutoff)) return;
+    const uint32_t idq = tag_id(b.tagA[q]);
+    if (!(idq > idp && (int)idq < S.pb_gid && key_le(d2, idq, kd2, kid))) return;
+    // rank among the qualifying candidates by (d2, id)
+    int rnk = 0;
+    for_candidates(S, pp, b.cellStart, [&](int u) {
+      if (u == p) return;
+      const double e2 = dist2(pp, b.posA[u]);
+      if (!(e2 < S.cutoff)) return;
+      const uint32_t idu = tag_id(b.tagA[u]);
+      if (!(idu > idp && (int)idu < S.pb_gid && key_le(e2, idu, kd2, kid))) return;
+      if (e2 < d2 || (e2 == d2 && idu < idq)) ++rnk;
+    });
+    ip[o + rnk] = (int)idp;
+    jp[o + rnk] = (int)idq;
+    r2[o + rnk] = d2;
+  });
+}
+
+// ================================================================== host
+
+namespace {
+
+struct Err {
+  std::string msg;
+};
+
+#define HIPCHK(expr, errobj)                                                              \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess) {                                                               \
+      char b_[512];                                                                       \
+      snprintf(b_, sizeof b_, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
+               __LINE__);                                                                 \
+      (errobj).msg = b_;                                                                  \
+      return -1;                                                                          \
+    }                                                                                     \
+  } while (0)
+
+template <class T>
+int dalloc(T** p, size_t count, Err& err) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  HIPCHK(hipMalloc((void**)p, count * sizeof(T)), err);
+  return 0;
+}
+
+int grid_of(int n) { return (n + BLK - 1) / BLK; }
+int grid8_of(int n) { return ((grid_of(n) + 7) / 8) * 8; }
+
+struct CellScan {
+  int m = 0, nb = 0;
+};
+
+}  // namespace
+
+struct uph_handle {
+  Err err;
+  hipStream_t stream = nullptr;
+  int n = 0, ncells = 0, nscenes = 0;
+  int itime = 0;
+  std::vector<DevScene> hs;
+  std::vector<int> fluid_n, tot_n;
+  DevScene* dsc = nullptr;
+  StepBufs b{};
+  CellScan cs;
+  uph_opts opts{};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+static void fill_scene(DevScene& S, const uph_scene& u) {
+  memset(&S, 0, sizeof S);
+  S.cutoff = u.h * u.h * 4;
+  const double cs = sqrt(S.cutoff) * (1.0 + 1e-9);
+  S.cell_inv = 1.0 / cs;
+  S.xmin = u.box[0];
+  S.ymin = u.box[1];
+  S.ncx = (int)floor((u.box[2] - u.box[0]) * S.cell_inv) + 1;
+  S.ncy = (int)floor((u.box[3] - u.box[1]) * S.cell_inv) + 1;
+  S.nf = u.n_fluid;
+  S.np = u.n_fluid + u.n_fixed + u.n_moving;
+  S.K = u.K;
+  S.gravity = u.gravity;
+  S.motion = u.motion;
+  S.h = u.h;
+  S.dt = u.dt;
+  S.rho0 = u.rho0;
+  S.gamma = u.gamma;
+  S.gamma_int = (u.gamma == floor(u.gamma) && u.gamma >= 1 && u.gamma <= 16) ? (int)u.gamma : 0;
+  S.B = u.B;
+  S.c0 = u.c0;
+  S.g = u.g;
+  S.r0 = 3.0 * u.dx;
+  S.m_amp = u.m_amp;
+  S.m_omega = u.m_omega;
+  S.fh = (float)u.h;
+  S.finvh = (float)(1.0 / u.h);
+  S.fwalpha = (float)(10.0 / (7. * M_PI * u.h * u.h));
+  S.fav = (float)u.alpha_av;
+  S.fc0 = (float)u.c0;
+  S.fk = (float)u.k_spring;
+  S.fcd = (float)u.c_damp;
+  S.fr0 = (float)(3.0 * u.dx);
+}
+
+static int alloc_bufs(StepBufs& b, int n, int ncells, int nblocks_scan, Err& err, bool search_only) {
+  memset(&b, 0, sizeof b);
+  if (dalloc(&b.posA, n, err) || dalloc(&b.posB, n, err) || dalloc(&b.tagA, n, err) ||
+      dalloc(&b.tagB, n, err) || dalloc(&b.keys, n, err) || dalloc(&b.rank, n, err) ||
+      dalloc(&b.tmpIdx, n, err) || dalloc(&b.cellStart, (size_t)ncells + 1, err) ||
+      dalloc(&b.cellCount, ncells, err) || dalloc(&b.partial, nblocks_scan, err) ||
+      dalloc(&b.sceneOf, n, err) || dalloc(&b.kthD2, n, err) || dalloc(&b.kthId, n, err) ||
+      dalloc(&b.ctl, 4, err))
+    return -1;
+  if (search_only) return 0;
+  if (dalloc(&b.velA, n, err) || dalloc(&b.velB, n, err) || dalloc(&b.rhoA, n, err) ||
+      dalloc(&b.rhoB, n, err) || dalloc(&b.rtA, n, err) || dalloc(&b.rtB, n, err) ||
+      dalloc(&b.mA, n, err) || dalloc(&b.mB, n, err) || dalloc(&b.deckPos, n, err) ||
+      dalloc(&b.deckVel, n, err) || dalloc(&b.deckLvel, n, err))
+    return -1;
+  return 0;
+}
+
+static void free_bufs(StepBufs& b) {
+  void* ptrs[] = {b.posA, b.velA, b.rhoA, b.rtA, b.tagA, b.mA, b.mB, b.posB, b.velB, b.rhoB,
+                  b.rtB, b.tagB, b.keys, b.rank, b.tmpIdx, b.cellStart, b.cellCount, b.partial,
+                  b.sceneOf, b.kthD2, b.kthId, b.deckPos, b.deckVel, b.deckLvel, b.ctl,
+                  b.dbgRates};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  memset(&b, 0, sizeof b);
+}
+
+// The sort kernels: bin, scan, scatter, reorder.
+static int launch_sort(hipStream_t st, int n, int ncells, int nb, StepBufs& b, const DevScene* dsc, int search_only,
+                       bool step_ctl, Err& err) {
+  hipLaunchKernelGGL(k_bin, dim3(grid_of(n)), dim3(BLK), 0, st, n, b.posB, b.sceneOf, dsc, b.tagA, b.tagB, b.keys,
+                     b.rank, b.cellCount);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_BLK), 0, st, ncells, b.cellCount, b.partial);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, nb, b.partial, step_ctl ? b.ctl : (int*)nullptr);
+  hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(SCAN_BLK), 0, st, ncells, b.cellCount, b.partial, b.cellStart);
+  hipLaunchKernelGGL(k_scatter, dim3(grid_of(n)), dim3(BLK), 0, st, n, b.keys, b.rank, b.cellStart, b.tmpIdx);
+  hipLaunchKernelGGL(k_reorder, dim3(grid_of(n)), dim3(BLK), 0, st, n, b, dsc, search_only);
+  HIPCHK(hipGetLastError(), err);
+  return 0;
+}
+
+static int launch_step(uph_handle* h, hipEvent_t pa, hipEvent_t pb) {
+  hipStream_t st = h->stream;
+  if (launch_sort(st, h->n, h->ncells, h->cs.nb, h->b, h->dsc, 0, true, h->err)) return -1;
+  if (pa) HIPCHK(hipEventRecord(pa, st), h->err);
+  hipLaunchKernelGGL(k_pair<false>, dim3(grid8_of(h->n)), dim3(BLK), 0, st, h->n, h->b, h->dsc);
+  if (pb) HIPCHK(hipEventRecord(pb, st), h->err);
+  hipLaunchKernelGGL(k_kth, dim3(grid8_of(h->n)), dim3(BLK), 0, st, h->n, h->b, h->dsc);
+  hipLaunchKernelGGL(k_pair<true>, dim3(grid8_of(h->n)), dim3(BLK), 0, st, h->n, h->b, h->dsc);
+  HIPCHK(hipGetLastError(), h->err);
+  return 0;
+}
+
+static std::string g_create_err;
+
+extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts* opts, uph_t** out) {
+  *out = nullptr;
+  if (!scenes || n_scenes <= 0) {
+    g_create_err = "uph_create: no scenes";
+    return -1;
+  }
+  uph_handle* h = new uph_handle();
+  if (opts) h->opts = *opts;
+  h->nscenes = n_scenes;
+  h->hs.resize(n_scenes);
+  long n = 0, nc = 0;
+  for (int s = 0; s < n_scenes; ++s) {
+    const uph_scene& u = scenes[s];
+    const int np = u.n_fluid + u.n_fixed + u.n_moving;
+    if (u.n_fluid < 0 || u.n_fixed < 0 || u.n_moving < 0 || np <= 0 || !(u.h > 0) || !(u.dt > 0) ||
+        u.K <= 0) {
+      g_create_err = "uph_create: scene " + std::to_string(s) + " has invalid sizes or constants";
+      delete h;
+      return -1;
+    }
+    for (int i = u.n_fluid; i < np; ++i)
+      if (u.rho[i] != u.rho0) {
+        g_create_err = "uph_create: boundary density must equal rho0 (localDensity = rho0, inputbp.c:288)";
+        delete h;
+        return -1;
+      }
+    fill_scene(h->hs[s], u);
+    DevScene& S = h->hs[s];
+    S.poff = (int)n;
+    S.coff = (int)nc;
+    S.pb_gid = (int)n + (u.n_fluid + u.n_fixed + u.n_moving - 1);
+    n += np;
+    nc += (long)S.ncx * S.ncy;
+    h->fluid_n.push_back(u.n_fluid);
+    h->tot_n.push_back(np);
+  }
+  if (n >= (long)UPH_ID_MASK || nc >= (1L << 30)) {
+    g_create_err = "uph_create: too many particles or cells";
+    delete h;
+    return -1;
+  }
+  h->n = (int)n;
+  h->ncells = (int)nc;
+  h->cs.m = (int)nc;
+  h->cs.nb = (int)((nc + SCAN_TILE - 1) / SCAN_TILE);
+  Err& err = h->err;
+  auto fail = [&]() {
+    g_create_err = err.msg;
+    free_bufs(h->b);
+    if (h->dsc) (void)hipFree(h->dsc);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return -1;
+  };
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    err.msg = "hipStreamCreate failed";
+    return fail();
+  }
+  if (alloc_bufs(h->b, h->n, h->ncells, h->cs.nb, err, false)) return fail();
+  if (h->opts.keep_rates && dalloc(&h->b.dbgRates, h->n, err)) return fail();
+  if (dalloc(&h->dsc, n_scenes, err)) return fail();
+  // host staging in original order (global index = slot at start)
+  std::vector<double2> pos(n), vel(n), lv(n);
+  std::vector<double> rho(n);
+  std::vector<float4> rt(n);
+  std::vector<uint32_t> tag(n);
+  std::vector<int> sceneOf(n);
+  for (int s = 0; s < n_scenes; ++s) {
+    const uph_scene& u = scenes[s];
+    const DevScene& S = h->hs[s];
+    for (int i = 0; i < S.np; ++i) {
+      const long g = S.poff + i;
+      pos[g] = make_double2(u.pos[2 * i], u.pos[2 * i + 1]);
+      vel[g] = make_double2(u.vel[2 * i], u.vel[2 * i + 1]);
+      lv[g] = make_double2(u.lvel[2 * i], u.lvel[2 * i + 1]);
+      rho[g] = u.rho[i];
+      rt[g] = make_float4(0.f, 0.f, 0.f, (float)u.mass[i]);
+      int ty = u.itype[i];
+      if (i < u.n_fluid) ty = 1;
+      else if (ty == 1 || ty < 0 || ty > 7) ty = 2;
+      tag[g] = ((uint32_t)ty << UPH_ID_BITS) | (uint32_t)g;
+      sceneOf[g] = s;
+    }
+  }
+  if (hipMemcpy(h->b.deckPos, pos.data(), n * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess) {
+    err.msg = "uph_create: upload failed";
+    return fail();
+  }
+  // boundary particles start at their step-1 positions (derivatives is called with t = dt)
+  for (int s = 0; s < n_scenes; ++s) {
+    const DevScene& S = h->hs[s];
+    if (S.motion == 0) continue;
+    for (int i = S.nf; i < S.np; ++i) {
+      const long g = S.poff + i;
+      if ((tag[g] >> UPH_ID_BITS) != 3) continue;
+      const double t = 1 * S.dt;
+      if (S.motion == 1) {
+        pos[g].x = 0.5 * S.m_amp * (1.0 - cos(S.m_omega * t));
+        vel[g].x = 0.5 * S.m_amp * S.m_omega * sin(S.m_omega * t);
+      } else {
+        pos[g].x = pos[g].x + S.m_amp * sin(S.m_omega * t);
+        vel[g].x = S.m_amp * S.m_omega * cos(S.m_omega * t);
+      }
+    }
+  }
+  std::vector<double2> vdeck(n);
+  for (int s = 0; s < n_scenes; ++s) {
+    const uph_scene& u = scenes[s];
+    for (int i = 0; i < h->hs[s].np; ++i) vdeck[h->hs[s].poff + i] = make_double2(u.vel[2 * i], u.vel[2 * i + 1]);
+  }
+  int ctl[4] = {0, 0, 0, 0};
+  if (hipMemcpy(h->b.deckVel, vdeck.data(), n * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(h->b.deckLvel, lv.data(), n * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(h->b.posB, pos.data(), n * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(h->b.velB, vel.data(), n * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(h->b.rhoB, rho.data(), n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(h->b.rtB, rt.data(), n * sizeof(float4), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(h->b.tagA, tag.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(h->b.sceneOf, sceneOf.data(), n * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(h->dsc, h->hs.data(), n_scenes * sizeof(DevScene), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(h->b.ctl, ctl, sizeof ctl, hipMemcpyHostToDevice) != hipSuccess) {
+    err.msg = "uph_create: upload failed";
+    return fail();
+  }
+  // cell counts start at zero; every scan leaves them zero again
+  if (hipMemset(h->b.cellCount, 0, (size_t)h->ncells * sizeof(int)) != hipSuccess ||
+      hipDeviceSynchronize() != hipSuccess) {
+    err.msg = "uph_create: init failed";
+    return fail();
+  }
+  if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) {
+    err.msg = "hipEventCreate failed";
+    return fail();
+  }
+  *out = h;
+  return 0;
+}
+
+extern "C" void uph_destroy(uph_t* h) {
+  if (!h) return;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+  if (h->graph) (void)hipGraphDestroy(h->graph);
+  if (h->ev0) (void)hipEventDestroy(h->ev0);
+  if (h->ev1) (void)hipEventDestroy(h->ev1);
+  free_bufs(h->b);
+  if (h->dsc) (void)hipFree(h->dsc);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+}
+
+extern "C" int uph_step(uph_t* h, int n_steps) {
+  if (!h) return -1;
+  if (n_steps < 0) {
+    h->err.msg = "uph_step: negative step count";
+    return -1;
+  }
+  if (h->opts.use_graph && n_steps > 0) {
+    if (!h->gexec) {
+      HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal), h->err);
+      int rc = launch_step(h, nullptr, nullptr);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(h->stream, &g);
+      if (rc || e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        if (!rc) h->err.msg = std::string("hipStreamEndCapture: ") + hipGetErrorString(e);
+        return -1;
+      }
+      h->graph = g;
+      HIPCHK(hipGraphInstantiate(&h->gexec, h->graph, nullptr, nullptr, 0), h->err);
+    }
+    for (int k = 0; k < n_steps; ++k) HIPCHK(hipGraphLaunch(h->gexec, h->stream), h->err);
+  } else {
+    for (int k = 0; k < n_steps; ++k)
+      if (launch_step(h, nullptr, nullptr)) return -1;
+  }
+  h->itime += n_steps;
+  return 0;
+}
+
+extern "C" int uph_sync(uph_t* h) {
+  if (!h) return -1;
+  HIPCHK(hipStreamSynchronize(h->stream), h->err);
+  int ctl[4];
+  HIPCHK(hipMemcpy(ctl, h->b.ctl, sizeof ctl, hipMemcpyDeviceToHost), h->err);
+  return ctl[2];
+}
+
+extern "C" int uph_itime(const uph_t* h) { return h ? h->itime : -1; }
+
+extern "C" int uph_time_steps(uph_t* h, int n_steps, double* ms_total, double* ms_pair) {
+  if (!h || n_steps <= 0) return -1;
+  std::vector<hipEvent_t> ev(2 * n_steps);
+  for (auto& e : ev) HIPCHK(hipEventCreate(&e), h->err);
+  HIPCHK(hipStreamSynchronize(h->stream), h->err);
+  HIPCHK(hipEventRecord(h->ev0, h->stream), h->err);
+  for (int k = 0; k < n_steps; ++k)
+    if (launch_step(h, ev[2 * k], ev[2 * k + 1])) return -1;
+  HIPCHK(hipEventRecord(h->ev1, h->stream), h->err);
+  HIPCHK(hipEventSynchronize(h->ev1), h->err);
+  float tot = 0.f;
+  HIPCHK(hipEventElapsedTime(&tot, h->ev0, h->ev1), h->err);
+  double pair = 0.0;
+  for (int k = 0; k < n_steps; ++k) {
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, ev[2 * k], ev[2 * k + 1]), h->err);
+    pair += t;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  h->itime += n_steps;
+  if (ms_total) *ms_total = tot;
+  if (ms_pair) *ms_pair = pair / n_steps;
+  return 0;
+}
+
+extern "C" int uph_get_state(uph_t* h, int scene, double* pos, double* vel, double* rho) {
+  if (!h) return -1;
+  if (scene < 0 || scene >= h->nscenes) {
+    h->err.msg = "uph_get_state: scene out of range";
+    return -1;
+  }
+  HIPCHK(hipStreamSynchronize(h->stream), h->err);
+  const DevScene& S = h->hs[scene];
+  std::vector<double2> p(S.np), v(S.np);
+  std::vector<double> r(S.np);
+  std::vector<uint32_t> t(S.np);
+  HIPCHK(hipMemcpy(p.data(), h->b.posB + S.poff, S.np * sizeof(double2), hipMemcpyDeviceToHost), h->err);
+  HIPCHK(hipMemcpy(v.data(), h->b.velB + S.poff, S.np * sizeof(double2), hipMemcpyDeviceToHost), h->err);
+  HIPCHK(hipMemcpy(r.data(), h->b.rhoB + S.poff, S.np * sizeof(double), hipMemcpyDeviceToHost), h->err);
+  HIPCHK(hipMemcpy(t.data(), h->b.tagA + S.poff, S.np * sizeof(uint32_t), hipMemcpyDeviceToHost), h->err);
+  for (int k = 0; k < S.np; ++k) {
+    const int i = (int)((t[k] & UPH_ID_MASK) - (uint32_t)S.poff);
+    if (i < 0 || i >= S.np) {
+      h->err.msg = "uph_get_state: corrupt particle tags";
+      return -1;
+    }
+    if (pos) { pos[2 * i] = p[k].x; pos[2 * i + 1] = p[k].y; }
+    if (vel) { vel[2 * i] = v[k].x; vel[2 * i + 1] = v[k].y; }
+    if (rho) rho[i] = r[k];
+  }
+  return 0;
+}
+
+extern "C" int uph_get_rates(uph_t* h, int scene, float* R, float* Rt, float* a) {
+  if (!h) return -1;
+  if (!h->b.dbgRates) {
+    h->err.msg = "uph_get_rates: handle created without keep_rates";
+    return -1;
+  }
+  if (scene < 0 || scene >= h->nscenes) {
+    h->err.msg = "uph_get_rates: scene out of range";
+    return -1;
+  }
+  HIPCHK(hipStreamSynchronize(h->stream), h->err);
+  const DevScene& S = h->hs[scene];
+  std::vector<float4> d(S.np);
+  std::vector<uint32_t> t(S.np);
+  HIPCHK(hipMemcpy(d.data(), h->b.dbgRates + S.poff, S.np * sizeof(float4), hipMemcpyDeviceToHost), h->err);
+  HIPCHK(hipMemcpy(t.data(), h->b.tagA + S.poff, S.np * sizeof(uint32_t), hipMemcpyDeviceToHost), h->err);
+  for (int k = 0; k < S.np; ++k) {
+    const int i = (int)((t[k] & UPH_ID_MASK) - (uint32_t)S.poff);
+    if (i < 0 || i >= S.np) continue;
+    if (i >= S.nf) continue;
+    if (R) R[i] = d[k].x;
+    if (Rt) Rt[i] = d[k].y;
+    if (a) { a[2 * i] = d[k].z; a[2 * i + 1] = d[k].w; }
+  }
+  return 0;
+}
+
+extern "C" const char* uph_last_error(const uph_t* h) { return h ? h->err.msg.c_str() : g_create_err.c_str(); }
+
+// ------------------------------------------------------ SearchNeighbors
+
+namespace {
+struct SearchCtx {
+  std::mutex mu;
+  Err err;
+  hipStream_t st = nullptr;
+  int cap_n = 0, cap_cells = 0, cap_nb = 0;
+  long cap_pairs = 0;
+  StepBufs b{};
+  DevScene* dsc = nullptr;
+  int *qcount = nullptr, *qoff = nullptr, *qpartial = nullptr;
+  int *ip = nullptr, *jp = nullptr;
+  double* r2 = nullptr;
+  double2* hpos = nullptr;  // pinned staging
+};
+SearchCtx g_search;
+
+int search_reserve(SearchCtx& c, int n, int ncells, int nb) {
+  if (!c.st) HIPCHK(hipStreamCreateWithFlags(&c.st, hipStreamNonBlocking), c.err);
+  if (!c.dsc) HIPCHK(hipMalloc((void**)&c.dsc, sizeof(DevScene)), c.err);
+  if (n > c.cap_n || ncells > c.cap_cells || nb > c.cap_nb) {
+    free_bufs(c.b);
+    if (c.qcount) (void)hipFree(c.qcount);
+    if (c.qoff) (void)hipFree(c.qoff);
+    if (c.qpartial) (void)hipFree(c.qpartial);
+    if (c.hpos) (void)hipHostFree(c.hpos);
+    c.qcount = c.qoff = c.qpartial = nullptr;
+    c.hpos = nullptr;
+    const int nn = std::max(n, c.cap_n), nc = std::max(ncells, c.cap_cells);
+    const int nbb = std::max(std::max(nb, c.cap_nb), (nn + SCAN_TILE - 1) / SCAN_TILE + 1);
+    if (alloc_bufs(c.b, nn, nc, nbb, c.err, true)) return -1;
+    HIPCHK(hipMemset(c.b.cellCount, 0, (size_t)nc * sizeof(int)), c.err);
+    HIPCHK(hipMalloc((void**)&c.qcount, ((size_t)nn + 1) * sizeof(int)), c.err);
+    HIPCHK(hipMalloc((void**)&c.qoff, ((size_t)nn + 1) * sizeof(int)), c.err);
+    HIPCHK(hipMalloc((void**)&c.qpartial, (size_t)nbb * sizeof(int)), c.err);
+    HIPCHK(hipHostMalloc((void**)&c.hpos, (size_t)nn * sizeof(double2), hipHostMallocDefault), c.err);
+    c.cap_n = nn;
+    c.cap_cells = nc;
+    c.cap_nb = nbb;
+  }
+  return 0;
+}
+
+int search_run(const double* const* rows, const double* flat, int* i_pair, int* j_pair, double* rij2, int nf, int n,
+               int pb, int K, double dist, long capacity) {
+  SearchCtx& c = g_search;
+  std::lock_guard<std::mutex> lk(c.mu);
+  c.err.msg.clear();
+  if (nf <= 0 || n <= 0) return 0;
+  if (nf > n || K <= 0 || !(dist > 0)) {
+    c.err.msg = "SearchNeighbors: invalid sizes";
+    return -1;
+  }
+  if (n >= (int)UPH_ID_MASK) {
+    c.err.msg = "SearchNeighbors: too many particles";
+    return -1;
+  }
+  // one grid over the bounding box of all particles
+  double lx = INFINITY, ly = INFINITY, hx = -INFINITY, hy = -INFINITY;
+  std::vector<double2> tmp;
+  for (int i = 0; i < n; ++i) {
+    const double x = rows ? rows[i][0] : flat[2 * i];
+    const double y = rows ? rows[i][1] : flat[2 * i + 1];
+    if (!std::isfinite(x) || !std::isfinite(y)) {
+      c.err.msg = "SearchNeighbors: non-finite position";
+      return -1;
+    }
+    lx = std::min(lx, x); hx = std::max(hx, x);
+    ly = std::min(ly, y); hy = std::max(hy, y);
+  }
+  DevScene S;
+  memset(&S, 0, sizeof S);
+  S.cutoff = dist;
+  const double cs = sqrt(dist) * (1.0 + 1e-9);
+  S.cell_inv = 1.0 / cs;
+  S.xmin = lx;
+  S.ymin = ly;
+  double ncx = floor((hx - lx) * S.cell_inv) + 1, ncy = floor((hy - ly) * S.cell_inv) + 1;
+  // bound the grid: coarser cells keep the result exact, only slower
+  double cell = cs;
+  while (ncx * ncy > 4.0 * n + 1024) {
+    cell *= 2.0;
+    S.cell_inv = 1.0 / cell;
+    ncx = floor((hx - lx) * S.cell_inv) + 1;
+    ncy = floor((hy - ly) * S.cell_inv) + 1;
+  }
+  // a coarser grid must still hold every in-range pair inside a 3x3 stencil: cell >= sqrt(dist)
+  S.ncx = (int)ncx;
+  S.ncy = (int)ncy;
+  S.coff = 0;
+  S.poff = 0;
+  S.np = n;
+  S.nf = nf;
+  S.pb_gid = pb;
+  S.K = K;
+  const int ncells = S.ncx * S.ncy;
+  const int nb = (ncells + SCAN_TILE - 1) / SCAN_TILE;
+  if (search_reserve(c, n, ncells, nb)) return -1;
+  for (int i = 0; i < n; ++i)
+    c.hpos[i] = rows ? make_double2(rows[i][0], rows[i][1]) : make_double2(flat[2 * i], flat[2 * i + 1]);
+  std::vector<uint32_t> tag(n);
+  for (int i = 0; i < n; ++i) tag[i] = ((uint32_t)(i < nf ? 1 : 2) << UPH_ID_BITS) | (uint32_t)i;
+  hipStream_t st = c.st;
+  HIPCHK(hipMemcpyAsync(c.b.posB, c.hpos, n * sizeof(double2), hipMemcpyHostToDevice, st), c.err);
+  HIPCHK(hipMemcpyAsync(c.b.tagA, tag.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, st), c.err);
+  HIPCHK(hipMemsetAsync(c.b.sceneOf, 0, n * sizeof(int), st), c.err);
+  HIPCHK(hipMemcpyAsync(c.dsc, &S, sizeof S, hipMemcpyHostToDevice, st), c.err);
+  if (launch_sort(st, n, ncells, nb, c.b, c.dsc, 1, false, c.err)) return -1;
+  hipLaunchKernelGGL(k_search_count, dim3(grid8_of(n)), dim3(BLK), 0, st, nf, n, c.b, c.dsc, c.qcount);
+  // exclusive scan of per-query counts (nf entries) into qoff
+  const int qnb = (nf + SCAN_TILE - 1) / SCAN_TILE;
+  hipLaunchKernelGGL(k_scan_reduce, dim3(qnb), dim3(SCAN_BLK), 0, st, nf, c.qcount, c.qpartial);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, qnb, c.qpartial, (int*)nullptr);
+  hipLaunchKernelGGL(k_scan_down, dim3(qnb), dim3(SCAN_BLK), 0, st, nf, c.qcount, c.qpartial, c.qoff);
+  HIPCHK(hipGetLastError(), c.err);
+  int niac = 0;
+  HIPCHK(hipMemcpyAsync(&niac, c.qoff + nf, sizeof(int), hipMemcpyDeviceToHost, st), c.err);
+  HIPCHK(hipStreamSynchronize(st), c.err);
+  if (niac < 0) {
+    c.err.msg = "SearchNeighbors: pair count overflow";
+    return -1;
+  }
+  if ((long)niac > capacity) {
+    c.err.msg = "SearchNeighbors: more pairs than the caller's capacity";
+    return -2;
+  }
+  if (niac > c.cap_pairs) {
+    if (c.ip) (void)hipFree(c.ip);
+    if (c.jp) (void)hipFree(c.jp);
+    if (c.r2) (void)hipFree(c.r2);
+    c.ip = c.jp = nullptr;
+    c.r2 = nullptr;
+    c.cap_pairs = 0;
+    HIPCHK(hipMalloc((void**)&c.ip, (size_t)niac * sizeof(int)), c.err);
+    HIPCHK(hipMalloc((void**)&c.jp, (size_t)niac * sizeof(int)), c.err);
+    HIPCHK(hipMalloc((void**)&c.r2, (size_t)niac * sizeof(double)), c.err);
+    c.cap_pairs = niac;
+  }
+  if (niac > 0) {
+    hipLaunchKernelGGL(k_search_emit, dim3(grid8_of(n)), dim3(BLK), 0, st, n, c.b, c.dsc, c.qoff, c.ip, c.jp, c.r2);
+    HIPCHK(hipGetLastError(), c.err);
+    HIPCHK(hipMemcpyAsync(i_pair, c.ip, (size_t)niac * sizeof(int), hipMemcpyDeviceToHost, st), c.err);
+    HIPCHK(hipMemcpyAsync(j_pair, c.jp, (size_t)niac * sizeof(int), hipMemcpyDeviceToHost, st), c.err);
+    HIPCHK(hipMemcpyAsync(rij2, c.r2, (size_t)niac * sizeof(double), hipMemcpyDeviceToHost, st), c.err);
+    HIPCHK(hipStreamSynchronize(st), c.err);
+  }
+  return niac;
+}
+}  // namespace
+
+extern "C" int SearchNeighbors(const double** Position, int* i_pair, int* j_pair, double* rij2,
+                               const int NumberOfFluidParticles, const int NumberOfParticles, int ParticleBoundary,
+                               const int NumberOfNeighbors, const double dist) {
+  return search_run(Position, nullptr, i_pair, j_pair, rij2, NumberOfFluidParticles, NumberOfParticles,
+                    ParticleBoundary, NumberOfNeighbors, dist, 0x7fffffffffffffffL);
+}
+
+extern "C" int uph_search_neighbors_cap(const double* pos_xy, int* i_pair, int* j_pair, double* rij2, int n_fluid,
+                                        int n_particles, int particle_boundary, int n_neighbors, double dist,
+                                        long capacity) {
+  return search_run(nullptr, pos_xy, i_pair, j_pair, rij2, n_fluid, n_particles, particle_boundary, n_neighbors, dist,
+                    capacity);
+}
+
+extern "C" const char* uph_search_last_error(void) { return g_search.err.msg.c_str(); }

@@ -1,0 +1,192 @@
+"""Host-side mirror of the reference's solver seam over libuph_gpu.so (ctypes).
+
+`search_neighbors` is the reference's SearchNeighbors (src/lib/Search.cc:5)
+on numpy arrays; `GpuSolver` runs the leapfrog of src/bin/sph.cc:90-168 with
+derivatives() (src/lib/derivatives.c:3137) on the GPU for one scene or a batch
+of independent scenes.  There is no CPU fallback: if the HIP library is not
+built, loading fails with an error.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Sequence
+
+import numpy as np
+
+from .scenes import Scene
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libuph_gpu.so")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class UphScene(C.Structure):
+    _fields_ = [("n_fluid", C.c_int), ("n_fixed", C.c_int), ("n_moving", C.c_int),
+                ("pos", _dp), ("vel", _dp), ("rho", _dp), ("mass", _dp), ("itype", _ip),
+                ("lvel", _dp), ("dx", C.c_double), ("h", C.c_double), ("rho0", C.c_double),
+                ("gamma", C.c_double), ("c0", C.c_double), ("B", C.c_double), ("g", C.c_double),
+                ("gravity", C.c_int), ("alpha_av", C.c_double), ("k_spring", C.c_double),
+                ("c_damp", C.c_double), ("dt", C.c_double), ("K", C.c_int), ("motion", C.c_int),
+                ("m_amp", C.c_double), ("m_omega", C.c_double), ("box", C.c_double * 4)]
+
+
+class UphOpts(C.Structure):
+    _fields_ = [("use_graph", C.c_int), ("keep_rates", C.c_int)]
+
+
+# every entry point of include/uph.h: name -> (restype, argtypes)
+SIGNATURES = {
+    "SearchNeighbors": (C.c_int, [C.POINTER(_dp), _ip, _ip, _dp, C.c_int, C.c_int, C.c_int,
+                                  C.c_int, C.c_double]),
+    "uph_search_neighbors_cap": (C.c_int, [_dp, _ip, _ip, _dp, C.c_int, C.c_int, C.c_int, C.c_int,
+                                           C.c_double, C.c_long]),
+    "uph_search_last_error": (C.c_char_p, []),
+    "uph_create": (C.c_int, [C.POINTER(UphScene), C.c_int, C.POINTER(UphOpts),
+                             C.POINTER(C.c_void_p)]),
+    "uph_destroy": (None, [C.c_void_p]),
+    "uph_step": (C.c_int, [C.c_void_p, C.c_int]),
+    "uph_sync": (C.c_int, [C.c_void_p]),
+    "uph_get_state": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _dp]),
+    "uph_get_rates": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_float),
+                                C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "uph_itime": (C.c_int, [C.c_void_p]),
+    "uph_time_steps": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp]),
+    "uph_last_error": (C.c_char_p, [C.c_void_p]),
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libuph_gpu.so is not built ({LIB_PATH}); run "
+                               "`python -c 'import __graft_entry__ as g; g.build()'`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _ptr(a: np.ndarray, ct=C.c_double):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def search_neighbors(pos: np.ndarray, n_fluid: int, n_particles: int, particle_boundary: int,
+                     n_neighbors: int, dist: float, capacity: int | None = None):
+    """SearchNeighbors (Search.cc:5-44) on the GPU: (i_pair, j_pair, rij2) arrays."""
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 2)
+    cap = int(capacity) if capacity is not None else max(1, n_fluid * n_neighbors)
+    ip = np.zeros(cap, np.int32)
+    jp = np.zeros(cap, np.int32)
+    r2 = np.zeros(cap)
+    m = lib().uph_search_neighbors_cap(_ptr(pos), _ptr(ip, C.c_int), _ptr(jp, C.c_int), _ptr(r2),
+                                       int(n_fluid), int(n_particles), int(particle_boundary),
+                                       int(n_neighbors), float(dist), cap)
+    if m < 0:
+        raise RuntimeError("SearchNeighbors failed: " + lib().uph_search_last_error().decode())
+    return ip[:m].copy(), jp[:m].copy(), r2[:m].copy()
+
+
+def search_neighbors_rows(pos: np.ndarray, i_pair: np.ndarray, j_pair: np.ndarray,
+                          rij2: np.ndarray, n_fluid: int, n_particles: int,
+                          particle_boundary: int, n_neighbors: int, dist: float) -> int:
+    """The exact reference signature: Position passed as a double** row table."""
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 2)
+    rows = (_dp * len(pos))(*[C.cast(pos[i:].ctypes.data, _dp) for i in range(len(pos))])
+    m = lib().SearchNeighbors(rows, _ptr(i_pair, C.c_int), _ptr(j_pair, C.c_int), _ptr(rij2),
+                              int(n_fluid), int(n_particles), int(particle_boundary),
+                              int(n_neighbors), float(dist))
+    if m < 0:
+        raise RuntimeError("SearchNeighbors failed: " + lib().uph_search_last_error().decode())
+    return m
+
+
+class GpuSolver:
+    """Device-resident leapfrog over a batch of independent scenes."""
+
+    def __init__(self, scenes: Sequence[Scene] | Scene, use_graph: bool = False,
+                 keep_rates: bool = False):
+        if isinstance(scenes, Scene):
+            scenes = [scenes]
+        self.scenes = list(scenes)
+        arr = (UphScene * len(self.scenes))()
+        self._keep = []
+        for k, sc in enumerate(self.scenes):
+            pos = np.ascontiguousarray(sc.pos, dtype=np.float64).reshape(-1)
+            vel = np.ascontiguousarray(sc.vel, dtype=np.float64).reshape(-1)
+            rho = np.ascontiguousarray(sc.rho, dtype=np.float64)
+            mass = np.ascontiguousarray(sc.mass, dtype=np.float64)
+            ity = np.ascontiguousarray(sc.itype, dtype=np.int32)
+            lvel = np.ascontiguousarray(sc.lvel, dtype=np.float64).reshape(-1)
+            self._keep += [pos, vel, rho, mass, ity, lvel]
+            u = arr[k]
+            u.n_fluid, u.n_fixed, u.n_moving = sc.n_fluid, sc.n_fixed, sc.n_moving
+            u.pos, u.vel, u.rho, u.mass = _ptr(pos), _ptr(vel), _ptr(rho), _ptr(mass)
+            u.itype, u.lvel = _ptr(ity, C.c_int), _ptr(lvel)
+            u.dx, u.h, u.rho0, u.gamma, u.c0, u.B, u.g = (sc.dx, sc.h, sc.rho0, sc.gamma, sc.c0,
+                                                          sc.B, sc.g)
+            u.gravity, u.alpha_av, u.k_spring, u.c_damp = (int(sc.gravity), sc.alpha_av,
+                                                           sc.k_spring, sc.c_damp)
+            u.dt, u.K, u.motion, u.m_amp, u.m_omega = (sc.dt, int(sc.K), int(sc.motion),
+                                                       sc.m_amp, sc.m_omega)
+            for q in range(4):
+                u.box[q] = float(sc.box[q])
+        opts = UphOpts(int(bool(use_graph)), int(bool(keep_rates)))
+        h = C.c_void_p()
+        rc = lib().uph_create(arr, len(self.scenes), C.byref(opts), C.byref(h))
+        if rc != 0:
+            raise RuntimeError("uph_create failed: " + lib().uph_last_error(None).decode())
+        self.h = h
+        self._keep = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().uph_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def _check(self, rc):
+        if rc < 0:
+            raise RuntimeError(lib().uph_last_error(self.h).decode())
+        return rc
+
+    def step(self, n: int = 1):
+        self._check(lib().uph_step(self.h, int(n)))
+
+    def sync(self) -> int:
+        """Waits; returns how many steps took the capped (more than K neighbours) path."""
+        return self._check(lib().uph_sync(self.h))
+
+    @property
+    def itime(self) -> int:
+        return lib().uph_itime(self.h)
+
+    def state(self, scene: int = 0):
+        sc = self.scenes[scene]
+        x = np.zeros((sc.n, 2)); v = np.zeros((sc.n, 2)); rho = np.zeros(sc.n)
+        self._check(lib().uph_get_state(self.h, scene, _ptr(x), _ptr(v), _ptr(rho)))
+        return dict(x=x, v=v, rho=rho)
+
+    def rates(self, scene: int = 0):
+        nf = self.scenes[scene].n_fluid
+        R = np.zeros(nf, np.float32); Rt = np.zeros(nf, np.float32)
+        a = np.zeros((nf, 2), np.float32)
+        self._check(lib().uph_get_rates(self.h, scene, _ptr(R, C.c_float), _ptr(Rt, C.c_float),
+                                        _ptr(a, C.c_float)))
+        return dict(R=R, Rt=Rt, a=a)
+
+    def time_steps(self, n: int):
+        """(ms for the whole loop, mean ms of the fused pair kernel), HIP events."""
+        tot = C.c_double(); pair = C.c_double()
+        self._check(lib().uph_time_steps(self.h, int(n), C.byref(tot), C.byref(pair)))
+        return tot.value, pair.value
