@@ -96,6 +96,7 @@ typedef struct uph_scene {
 typedef struct uph_opts {
   int use_graph;        /* 1: replay each step as one captured hipGraph     */
   int keep_rates;       /* 1: keep the last step's FP32 rates for uph_get_rates */
+  int pair_variant;     /* 0: compacted neighbour list in LDS (default), 1: direct loop */
 } uph_opts;
 
 typedef struct uph_handle uph_t;

@@ -17,6 +17,10 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
          "-Wall", "-Wno-unused-result"]
 
 
+DRIVER_SRC = os.path.join(HERE, "host", "uph_run.c")
+DRIVER = os.path.join(HERE, "uph_run")
+
+
 def build(verbose: bool = False) -> str:
     tmp = OUT + ".tmp"
     cmd = [HIPCC, *FLAGS, "-I", os.path.join(HERE, "..", "include"), "-o", tmp, SRC]
@@ -24,6 +28,13 @@ def build(verbose: bool = False) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, OUT)
+    # the C host driver links the C ABI only
+    dcmd = ["gcc", "-std=gnu11", "-O2", "-Wall", "-o", DRIVER + ".tmp", DRIVER_SRC,
+            "-L", HERE, "-luph_gpu", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib", "-lm"]
+    if verbose:
+        print(" ".join(dcmd), flush=True)
+    subprocess.run(dcmd, check=True)
+    os.replace(DRIVER + ".tmp", DRIVER)
     return OUT
 
 

@@ -328,65 +328,42 @@ __global__ __launch_bounds__(BLK) void k_kth(int n, StepBufs b, const DevScene* 
   b.kthId[p] = kid;
 }
 
-template <bool EXACT>
-__global__ __launch_bounds__(BLK) void k_pair(int n, StepBufs b, const DevScene* __restrict__ sc) {
-  if (EXACT && b.ctl[1] == 0) return;
-  const int p = xcd_block() * BLK + threadIdx.x;
-  if (p >= n) return;
-  const uint32_t tg = b.tagA[p];
-  const DevScene& S = sc[b.sceneOf[p]];
-  const int itime = b.ctl[0];
-  const double dt = S.dt;
-  const double2 xp = b.posA[p];
-  const double2 vn = b.velA[p];
-  const float4 rt = b.rtA[p];
-  if (tag_type(tg) != 1) {
-    // boundary: kinematics for the next step's time, state otherwise frozen
-    const uint32_t id = tag_id(tg);
-    double2 x = b.deckPos[id], v = b.deckVel[id];
-    if (tag_type(tg) == 3 && S.motion != 0) wall_motion(S, (itime + 1) * dt, x.x, x.x, v.x);
-    b.posB[p] = x;
-    b.velB[p] = v;
-    b.rhoB[p] = b.rhoA[p];
-    b.rtB[p] = rt;
-    return;
-  }
-  const uint32_t idp = tag_id(tg);
-  const float4 ap = b.mA[p];
-  const float4 bp = b.mB[p];
-  const float vpx = ap.x, vpy = ap.y, lpx = ap.z, lpy = ap.w;
-  const float rhop = bp.x, prp = bp.y, mp = bp.z;
-  const double cut = S.cutoff;
-  const float h = S.fh, invh = S.finvh, walpha = S.fwalpha;
-  const float h2eps = 0.01f * h * h;
-  const bool pb_ok = (int)idp < S.pb_gid;
-  double kpd2 = INFINITY;
-  uint32_t kpid = 0xffffffffu;
-  if (EXACT) {
-    kpd2 = b.kthD2[p];
-    kpid = b.kthId[p];
-  }
+// State of one fluid particle p during its pair sums.
+struct PairCtx {
+  uint32_t idp;
+  bool pb_ok;
+  float vpx, vpy, lpx, lpy, rhop, prp, mp;
+  float h, invh, walpha, h2eps;
   CSum sR, sRt, sAx, sAy;
-  int cnt = 0;
-  for_candidates(S, xp, b.cellStart, [&](int q) {
-    if (q == p) return;
-    const double2 xq = b.posA[q];
-    const double ex = xp.x - xq.x, ey = xp.y - xq.y;
-    const double d2 = __dadd_rn(__dmul_rn(ex, ex), __dmul_rn(ey, ey));
-    if (!(d2 < cut)) return;
-    ++cnt;
-    const float4 bq = b.mB[q];
-    const uint32_t tq = __float_as_uint(bq.w);
-    const uint32_t idq = tag_id(tq);
-    const bool p_is_i = idq > idp;   // pair (i=p, j=q) or (i=q, j=p) of Search.cc:33
-    if (p_is_i) {
-      if ((int)idq >= S.pb_gid) return;
-      if (EXACT && !key_le(d2, idq, kpd2, kpid)) return;
+
+  __device__ __forceinline__ void init(const DevScene& S, uint32_t tg, float4 ap, float4 bp) {
+    idp = tag_id(tg);
+    pb_ok = (int)idp < S.pb_gid;
+    vpx = ap.x; vpy = ap.y; lpx = ap.z; lpy = ap.w;
+    rhop = bp.x; prp = bp.y; mp = bp.z;
+    h = S.fh; invh = S.finvh; walpha = S.fwalpha;
+    h2eps = 0.01f * h * h;
+  }
+
+  // Admission of an in-range pair by the rules of Search.cc:33 (original index
+  // order, ParticleBoundary bound) and, when EXACT, the K-nearest cap.
+  template <bool EXACT>
+  __device__ __forceinline__ bool admit(const DevScene& S, uint32_t idq, double d2, double kpd2, uint32_t kpid,
+                                        const StepBufs& b, int q) const {
+    if (idq > idp) {
+      if ((int)idq >= S.pb_gid) return false;
+      if (EXACT && !key_le(d2, idq, kpd2, kpid)) return false;
     } else {
-      if (!pb_ok) return;
-      if (EXACT && !key_le(d2, idp, b.kthD2[q], b.kthId[q])) return;
+      if (!pb_ok) return false;
+      if (EXACT && !key_le(d2, idp, b.kthD2[q], b.kthId[q])) return false;
     }
-    const float4 aq = b.mA[q];
+    return true;
+  }
+
+  // One admitted pair: x_p - x_q = (ex, ey), |.|^2 = d2 (FP64 geometry, FP32 sums).
+  __device__ __forceinline__ void add(const DevScene& S, double ex, double ey, double d2, uint32_t tq, float4 aq,
+                                      float4 bq) {
+    const bool p_is_i = tag_id(tq) > idp;   // pair (i=p, j=q) or (i=q, j=p)
     const float dx = (float)ex, dy = (float)ey;
     const float r2 = (float)d2;
     const float r = sqrtf(r2);
@@ -428,13 +405,28 @@ __global__ __launch_bounds__(BLK) void k_pair(int n, StepBufs b, const DevScene*
     }
     sAx.add(ax);
     sAy.add(ay);
-  });
-  if (!EXACT && cnt > S.K) atomicOr(&b.ctl[1], 1);
-  const float R = sR.val(), Rt = sRt.val();
-  float axf = sAx.val(), ayf = sAy.val();
-  double ax = (double)axf, ay = (double)ayf;
+  }
+};
+
+// Boundary slot: wall state for the next step (deck + kinematics at t = (itime+1) dt).
+__device__ __forceinline__ void wall_update(const DevScene& S, const StepBufs& b, int p, uint32_t tg, int itime,
+                                            float4 rt) {
+  const uint32_t id = tag_id(tg);
+  double2 x = b.deckPos[id], v = b.deckVel[id];
+  if (tag_type(tg) == 3 && S.motion != 0) wall_motion(S, (itime + 1) * S.dt, x.x, x.x, v.x);
+  b.posB[p] = x;
+  b.velB[p] = v;
+  b.rhoB[p] = b.rhoA[p];
+  b.rtB[p] = rt;
+}
+
+// Gravity and the FP64 leapfrog corrector (sph.cc:137-168) for fluid slot p.
+__device__ __forceinline__ void correct(const DevScene& S, const StepBufs& b, int p, int itime, double2 xp,
+                                        double2 vn, float4 rt, const PairCtx& c) {
+  const double dt = S.dt;
+  const float R = c.sR.val(), Rt = c.sRt.val();
+  double ax = (double)c.sAx.val(), ay = (double)c.sAy.val();
   if (S.gravity == 1) ay -= S.g;     // BodyForce (derivatives.c:1167-1178)
-  // corrector (sph.cc:137-168), FP64
   const double rho_n = b.rhoA[p];
   double rho1, v1x, v1y;
   if (itime == 1) {
@@ -451,6 +443,123 @@ __global__ __launch_bounds__(BLK) void k_pair(int n, StepBufs b, const DevScene*
   b.rhoB[p] = rho1;
   b.rtB[p] = make_float4(Rt, (float)ax, (float)ay, rt.w);
   if (b.dbgRates) b.dbgRates[p] = make_float4(R, Rt, (float)ax, (float)ay);
+}
+
+// Direct form: one pass over the candidates, physics inside the distance test.
+// EXACT applies the K-nearest cap (run only when some query overflowed).
+template <bool EXACT>
+__global__ __launch_bounds__(BLK) void k_pair(int n, StepBufs b, const DevScene* __restrict__ sc) {
+  if (EXACT && b.ctl[1] == 0) return;
+  const int p = xcd_block() * BLK + threadIdx.x;
+  if (p >= n) return;
+  const uint32_t tg = b.tagA[p];
+  const DevScene& S = sc[b.sceneOf[p]];
+  const int itime = b.ctl[0];
+  const double2 xp = b.posA[p];
+  const double2 vn = b.velA[p];
+  const float4 rt = b.rtA[p];
+  if (tag_type(tg) != 1) {
+    wall_update(S, b, p, tg, itime, rt);
+    return;
+  }
+  PairCtx c;
+  c.init(S, tg, b.mA[p], b.mB[p]);
+  const double cut = S.cutoff;
+  double kpd2 = INFINITY;
+  uint32_t kpid = 0xffffffffu;
+  if (EXACT) {
+    kpd2 = b.kthD2[p];
+    kpid = b.kthId[p];
+  }
+  int cnt = 0;
+  for_candidates(S, xp, b.cellStart, [&](int q) {
+    if (q == p) return;
+    const double2 xq = b.posA[q];
+    const double ex = xp.x - xq.x, ey = xp.y - xq.y;
+    const double d2 = __dadd_rn(__dmul_rn(ex, ex), __dmul_rn(ey, ey));
+    if (!(d2 < cut)) return;
+    ++cnt;
+    const float4 bq = b.mB[q];
+    const uint32_t tq = __float_as_uint(bq.w);
+    if (!c.template admit<EXACT>(S, tag_id(tq), d2, kpd2, kpid, b, q)) return;
+    c.add(S, ex, ey, d2, tq, b.mA[q], bq);
+  });
+  if (!EXACT && cnt > S.K) atomicOr(&b.ctl[1], 1);
+  correct(S, b, p, itime, xp, vn, rt, c);
+}
+
+// Compacted form (default): pass 1 runs only the FP64 distance test over the
+// three candidate row ranges and records the in-range slots in LDS (16-bit
+// offsets into the row ranges); pass 2 runs the physics over that list, so the
+// lanes of a wavefront stay busy on in-range pairs.  A query with more than
+// min(K, UPH_NL) candidates in range, or an offset beyond 14 bits, raises the
+// overflow flag: the exact direct kernel then recomputes the step.
+#define UPH_NL 48
+__global__ __launch_bounds__(BLK) void k_pair_compact(int n, StepBufs b, const DevScene* __restrict__ sc) {
+  __shared__ uint16_t nl[UPH_NL][BLK];
+  const int p = xcd_block() * BLK + threadIdx.x;
+  if (p >= n) return;
+  const uint32_t tg = b.tagA[p];
+  const DevScene& S = sc[b.sceneOf[p]];
+  const int itime = b.ctl[0];
+  const double2 xp = b.posA[p];
+  const double2 vn = b.velA[p];
+  const float4 rt = b.rtA[p];
+  if (tag_type(tg) != 1) {
+    wall_update(S, b, p, tg, itime, rt);
+    return;
+  }
+  const double cut = S.cutoff;
+  const int tid = threadIdx.x;
+  int cx, cy;
+  cell_of(S, xp.x, xp.y, cx, cy);
+  const int x0 = cx > 0 ? cx - 1 : 0;
+  const int x1 = cx < S.ncx - 1 ? cx + 1 : S.ncx - 1;
+  int rs[3];
+  int cnt = 0;
+  bool bad = false;
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const int row = cy - 1 + r;
+    rs[r] = 0;
+    if (row < 0 || row >= S.ncy) continue;
+    const int base = S.coff + row * S.ncx;
+    const int qb = b.cellStart[base + x0];
+    const int qe = b.cellStart[base + x1 + 1];
+    rs[r] = qb;
+#pragma unroll 1
+    for (int q = qb; q < qe; ++q) {
+      if (q == p) continue;
+      const double2 xq = b.posA[q];
+      const double ex = xp.x - xq.x, ey = xp.y - xq.y;
+      const double d2 = __dadd_rn(__dmul_rn(ex, ex), __dmul_rn(ey, ey));
+      if (!(d2 < cut)) continue;
+      const int off = q - qb;
+      if (cnt < UPH_NL && off < (1 << 14)) nl[cnt][tid] = (uint16_t)((r << 14) | off);
+      else bad = true;
+      ++cnt;
+    }
+  }
+  if (bad || cnt > S.K) {
+    atomicOr(&b.ctl[1], 1);
+    return;   // the exact kernel recomputes this step
+  }
+  PairCtx c;
+  c.init(S, tg, b.mA[p], b.mB[p]);
+#pragma unroll 1
+  for (int k = 0; k < cnt; ++k) {
+    const uint32_t e = nl[k][tid];
+    const int r = (int)(e >> 14);
+    const int q = (r == 0 ? rs[0] : (r == 1 ? rs[1] : rs[2])) + (int)(e & 0x3fffu);
+    const double2 xq = b.posA[q];
+    const double ex = xp.x - xq.x, ey = xp.y - xq.y;
+    const double d2 = __dadd_rn(__dmul_rn(ex, ex), __dmul_rn(ey, ey));
+    const float4 bq = b.mB[q];
+    const uint32_t tq = __float_as_uint(bq.w);
+    if (!c.template admit<false>(S, tag_id(tq), d2, 0.0, 0u, b, q)) continue;
+    c.add(S, ex, ey, d2, tq, b.mA[q], bq);
+  }
+  correct(S, b, p, itime, xp, vn, rt, c);
 }
 
 // ---------------------------------------------------------------- search
@@ -650,7 +759,10 @@ static int launch_step(uph_handle* h, hipEvent_t pa, hipEvent_t pb) {
   hipStream_t st = h->stream;
   if (launch_sort(st, h->n, h->ncells, h->cs.nb, h->b, h->dsc, 0, true, h->err)) return -1;
   if (pa) HIPCHK(hipEventRecord(pa, st), h->err);
-  hipLaunchKernelGGL(k_pair<false>, dim3(grid8_of(h->n)), dim3(BLK), 0, st, h->n, h->b, h->dsc);
+  if (h->opts.pair_variant == 1)
+    hipLaunchKernelGGL(k_pair<false>, dim3(grid8_of(h->n)), dim3(BLK), 0, st, h->n, h->b, h->dsc);
+  else
+    hipLaunchKernelGGL(k_pair_compact, dim3(grid8_of(h->n)), dim3(BLK), 0, st, h->n, h->b, h->dsc);
   if (pb) HIPCHK(hipEventRecord(pb, st), h->err);
   hipLaunchKernelGGL(k_kth, dim3(grid8_of(h->n)), dim3(BLK), 0, st, h->n, h->b, h->dsc);
   hipLaunchKernelGGL(k_pair<true>, dim3(grid8_of(h->n)), dim3(BLK), 0, st, h->n, h->b, h->dsc);

@@ -1,0 +1,211 @@
+/*
+ * uph_run.c -- C host driver: the reference's `sph <nsteps>` (src/bin/sph.cc)
+ * with the time loop on the GPU through libuph_gpu.so.
+ *
+ * Run it in a directory holding the reference's input decks:
+ *   parameters.dat (34 values, System.c:60-94), FluidParticle.dat,
+ *   FixedBoundaryParticle.dat, MovingBoundaryParticle.dat, SolidParticle.dat
+ * (12-column rows, input.c:57-72 / inputbp.c:88-150).
+ *
+ *   uph_run <nsteps> [--fluid FILE] [--h-factor F] [--alpha A] [--graph] [--out FILE]
+ *
+ * Constants follow System.c:292-313: c = sqrt(g H)/Mach (sqrt(2 g H)/Mach when
+ * movingBoundary == 2), B = rho0 c^2 / gamma, eta = sqrt(Nn / 4 pi), hsml = eta dx
+ * (input.c:107) unless --h-factor sets h = F dx.  The scheme is the library's:
+ * cubic spline, Tait pressure in the momentum equation, artificial viscosity
+ * `alpha` (deck field 15 unless --alpha), walls reset from the decks, the
+ * piston of inputbp.c:194-207 when movingBoundary == 1.  After the run the
+ * fluid state is written in the deck format (default FluidParticleFinal.dat).
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include "../../include/uph.h"
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+typedef struct {
+  int n;
+  double *pos, *vel, *rho, *pres, *mass, *lvel;
+  int *itype;
+} deck_t;
+
+static int read_deck(const char *path, deck_t *d) {
+  memset(d, 0, sizeof *d);
+  FILE *f = fopen(path, "r");
+  if (!f) {
+    fprintf(stderr, "uph_run: cannot open %s\n", path);
+    return -1;
+  }
+  if (fscanf(f, "%d", &d->n) != 1 || d->n < 0) {
+    fprintf(stderr, "uph_run: bad count in %s\n", path);
+    fclose(f);
+    return -1;
+  }
+  size_t n = d->n > 0 ? (size_t)d->n : 1;
+  d->pos = calloc(2 * n, sizeof(double));
+  d->vel = calloc(2 * n, sizeof(double));
+  d->lvel = calloc(2 * n, sizeof(double));
+  d->rho = calloc(n, sizeof(double));
+  d->pres = calloc(n, sizeof(double));
+  d->mass = calloc(n, sizeof(double));
+  d->itype = calloc(n, sizeof(int));
+  for (int i = 0; i < d->n; i++) {
+    double t0, t1;
+    int got = fscanf(f, "%lf %lf %lf %lf %lf %lf %lf %lf %lf %d %lf %lf", &d->pos[2 * i],
+                     &d->pos[2 * i + 1], &d->vel[2 * i], &d->vel[2 * i + 1], &d->rho[i], &d->pres[i],
+                     &d->mass[i], &t0, &t1, &d->itype[i], &d->lvel[2 * i], &d->lvel[2 * i + 1]);
+    if (got != 12) {
+      fprintf(stderr, "uph_run: %s row %d: expected 12 columns\n", path, i + 1);
+      fclose(f);
+      return -1;
+    }
+  }
+  fclose(f);
+  return 0;
+}
+
+static void free_deck(deck_t *d) {
+  free(d->pos); free(d->vel); free(d->lvel); free(d->rho); free(d->pres); free(d->mass);
+  free(d->itype);
+}
+
+static double now_s(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+int main(int argc, char **argv) {
+  if (argc < 2) {
+    fprintf(stderr, "Give a maximum number of time step\n");
+    return 1;
+  }
+  int nsteps = atoi(argv[1]);
+  const char *fluid_file = "FluidParticle.dat", *out_file = "FluidParticleFinal.dat";
+  double h_factor = 0.0, alpha = -1.0;
+  int use_graph = 0;
+  for (int a = 2; a < argc; a++) {
+    if (!strcmp(argv[a], "--fluid") && a + 1 < argc) fluid_file = argv[++a];
+    else if (!strcmp(argv[a], "--h-factor") && a + 1 < argc) h_factor = atof(argv[++a]);
+    else if (!strcmp(argv[a], "--alpha") && a + 1 < argc) alpha = atof(argv[++a]);
+    else if (!strcmp(argv[a], "--out") && a + 1 < argc) out_file = argv[++a];
+    else if (!strcmp(argv[a], "--graph")) use_graph = 1;
+    else {
+      fprintf(stderr, "uph_run: unknown option %s\n", argv[a]);
+      return 1;
+    }
+  }
+  double v[34];
+  FILE *fp = fopen("parameters.dat", "r");
+  if (!fp) {
+    fprintf(stderr, "cannot find parameter file parameters.dat\n");
+    return 1;
+  }
+  for (int k = 0; k < 34; k++)
+    if (fscanf(fp, "%lf", &v[k]) != 1) {
+      fprintf(stderr, "uph_run: parameters.dat has fewer than 34 values\n");
+      fclose(fp);
+      return 1;
+    }
+  fclose(fp);
+  /* field map of System.c:60-94 (0-based) */
+  const double stroke = v[0], period = v[1], depth = v[2], rho0 = v[4], gamma = v[5], g = v[6];
+  const double dx = v[7], mach = v[11], alpha_deck = v[14], dt = v[18], nn = v[20];
+  const double k0 = v[23], c0d = v[24];
+  const int gravity = (int)v[26], moving = (int)v[28];
+
+  deck_t fl, fx, mv, so;
+  if (read_deck(fluid_file, &fl) || read_deck("FixedBoundaryParticle.dat", &fx) ||
+      read_deck("MovingBoundaryParticle.dat", &mv) || read_deck("SolidParticle.dat", &so))
+    return 1;
+  if (so.n != 0) {
+    fprintf(stderr, "uph_run: solid particles are not supported (SolidParticle.dat has %d)\n", so.n);
+    return 1;
+  }
+  const int n = fl.n + fx.n + mv.n;
+  deck_t all;
+  memset(&all, 0, sizeof all);
+  all.pos = malloc(sizeof(double) * 2 * n); all.vel = malloc(sizeof(double) * 2 * n);
+  all.lvel = malloc(sizeof(double) * 2 * n); all.rho = malloc(sizeof(double) * n);
+  all.mass = malloc(sizeof(double) * n); all.itype = malloc(sizeof(int) * n);
+  deck_t *parts[3] = {&fl, &fx, &mv};
+  int o = 0;
+  double lo[2] = {1e300, 1e300}, hi[2] = {-1e300, -1e300};
+  for (int p = 0; p < 3; p++)
+    for (int i = 0; i < parts[p]->n; i++, o++) {
+      for (int d = 0; d < 2; d++) {
+        all.pos[2 * o + d] = parts[p]->pos[2 * i + d];
+        all.vel[2 * o + d] = parts[p]->vel[2 * i + d];
+        all.lvel[2 * o + d] = parts[p]->lvel[2 * i + d];
+        if (all.pos[2 * o + d] < lo[d]) lo[d] = all.pos[2 * o + d];
+        if (all.pos[2 * o + d] > hi[d]) hi[d] = all.pos[2 * o + d];
+      }
+      all.rho[o] = parts[p]->rho[i];
+      all.mass[o] = parts[p]->mass[i];
+      all.itype[o] = parts[p]->itype[i];
+    }
+  const double c = (moving == 2 ? sqrt(2.0 * g * depth) : sqrt(g * depth)) / mach;
+  const double eta = sqrt(nn / (4.0 * M_PI));
+  uph_scene s;
+  memset(&s, 0, sizeof s);
+  s.n_fluid = fl.n; s.n_fixed = fx.n; s.n_moving = mv.n;
+  s.pos = all.pos; s.vel = all.vel; s.rho = all.rho; s.mass = all.mass; s.itype = all.itype;
+  s.lvel = all.lvel;
+  s.dx = dx;
+  s.h = h_factor > 0 ? h_factor * dx : eta * dx;
+  s.rho0 = rho0; s.gamma = gamma; s.c0 = c; s.B = (1. / gamma) * rho0 * c * c; s.g = g;
+  s.gravity = gravity; s.alpha_av = alpha >= 0 ? alpha : alpha_deck;
+  s.k_spring = k0; s.c_damp = c0d; s.dt = dt; s.K = 40;
+  s.motion = moving == 1 ? UPH_MOTION_PISTON : UPH_MOTION_NONE;
+  s.m_amp = stroke; s.m_omega = 2.0 * M_PI / period;
+  const double margin = 10 * s.h + (moving == 1 ? stroke : 0.0);
+  s.box[0] = lo[0] - margin; s.box[1] = lo[1] - margin;
+  s.box[2] = hi[0] + margin; s.box[3] = hi[1] + 2 * (hi[1] - lo[1]);
+  uph_opts opts = {use_graph, 0};
+  uph_t *h = NULL;
+  if (uph_create(&s, 1, &opts, &h) != 0) {
+    fprintf(stderr, "uph_run: %s\n", uph_last_error(NULL));
+    return 1;
+  }
+  double t0 = now_s();
+  if (uph_step(h, nsteps) != 0 || uph_sync(h) < 0) {
+    fprintf(stderr, "uph_run: %s\n", uph_last_error(h));
+    uph_destroy(h);
+    return 1;
+  }
+  double el = now_s() - t0;
+  printf("%d fluid + %d wall particles, %d steps in %.3f s: %.4g particle-updates/s\n", fl.n,
+         fx.n + mv.n, nsteps, el, el > 0 ? (double)fl.n * nsteps / el : 0.0);
+  double *pos = malloc(sizeof(double) * 2 * n), *vel = malloc(sizeof(double) * 2 * n);
+  double *rho = malloc(sizeof(double) * n);
+  if (uph_get_state(h, 0, pos, vel, rho) != 0) {
+    fprintf(stderr, "uph_run: %s\n", uph_last_error(h));
+    uph_destroy(h);
+    return 1;
+  }
+  FILE *fo = fopen(out_file, "w");
+  if (!fo) {
+    fprintf(stderr, "uph_run: cannot write %s\n", out_file);
+    uph_destroy(h);
+    return 1;
+  }
+  fprintf(fo, "%d\n", fl.n);
+  for (int i = 0; i < fl.n; i++) {
+    const double P = s.B * (pow(rho[i] / rho0, gamma) - 1.0);
+    fprintf(fo, "%.10lf %.10lf %.10lf %.10lf %.10lf %.10lf %.10lf %.10lf %.10lf %d %.10lf %.10lf\n",
+            pos[2 * i], pos[2 * i + 1], vel[2 * i], vel[2 * i + 1], rho[i], P, all.mass[i], 0.0, 0.0, 1,
+            vel[2 * i], vel[2 * i + 1]);
+  }
+  fclose(fo);
+  uph_destroy(h);
+  free(pos); free(vel); free(rho);
+  free(all.pos); free(all.vel); free(all.lvel); free(all.rho); free(all.mass); free(all.itype);
+  free_deck(&fl); free_deck(&fx); free_deck(&mv); free_deck(&so);
+  return 0;
+}

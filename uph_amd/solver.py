@@ -34,7 +34,7 @@ class UphScene(C.Structure):
 
 
 class UphOpts(C.Structure):
-    _fields_ = [("use_graph", C.c_int), ("keep_rates", C.c_int)]
+    _fields_ = [("use_graph", C.c_int), ("keep_rates", C.c_int), ("pair_variant", C.c_int)]
 
 
 # every entry point of include/uph.h: name -> (restype, argtypes)
@@ -114,7 +114,7 @@ class GpuSolver:
     """Device-resident leapfrog over a batch of independent scenes."""
 
     def __init__(self, scenes: Sequence[Scene] | Scene, use_graph: bool = False,
-                 keep_rates: bool = False):
+                 keep_rates: bool = False, pair_variant: int = 0):
         if isinstance(scenes, Scene):
             scenes = [scenes]
         self.scenes = list(scenes)
@@ -140,7 +140,7 @@ class GpuSolver:
                                                        sc.m_amp, sc.m_omega)
             for q in range(4):
                 u.box[q] = float(sc.box[q])
-        opts = UphOpts(int(bool(use_graph)), int(bool(keep_rates)))
+        opts = UphOpts(int(bool(use_graph)), int(bool(keep_rates)), int(pair_variant))
         h = C.c_void_p()
         rc = lib().uph_create(arr, len(self.scenes), C.byref(opts), C.byref(h))
         if rc != 0:
