@@ -164,6 +164,8 @@ static void grid_build(grid_t *g, const double *pos, int n, double cell) {
     if (pos[2 * i + 1] < ly) ly = pos[2 * i + 1];
     if (pos[2 * i + 1] > hy) hy = pos[2 * i + 1];
   }
+  /* widely spread inputs: coarser cells keep the 3x3 stencil exact (cell >= radius) */
+  while ((floor((hx - lx) / cell) + 1) * (floor((hy - ly) / cell) + 1) > 4.0 * n + 1024) cell *= 2.0;
   g->x0 = lx; g->y0 = ly; g->inv = 1.0 / cell;
   g->ncx = (int)floor((hx - lx) * g->inv) + 1;
   g->ncy = (int)floor((hy - ly) * g->inv) + 1;

@@ -186,8 +186,15 @@ def test_errors_are_reported():
     with pytest.raises(RuntimeError):
         GpuSolver(sc)
     g = GpuSolver(scenes.dam_break(width=0.2, height=0.2))
-    with pytest.raises(RuntimeError):
+    with pytest.raises(IndexError):
         g.state(5)
+    # the C entry point itself rejects the index
+    import ctypes
+    from uph_amd import solver
+    x = np.zeros(8)
+    assert solver.lib().uph_get_state(g.h, 5, x.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                      None, None) == -1
+    assert b"out of range" in solver.lib().uph_last_error(g.h)
     with pytest.raises(RuntimeError):
         g.rates()                           # created without keep_rates
 

@@ -147,6 +147,13 @@ def test_search_edge_cases(oracle_lib):
     assert len(ip) == 0
     with pytest.raises(RuntimeError):
         oracle_lib.search(pos, 3, 3, 3, 40, 1.0, cap=1)
+    # widely spread particles: the grid is coarsened, the result must not change
+    rng = np.random.default_rng(4)
+    far = np.concatenate([rng.uniform(0, 0.05, (300, 2)), [[1e4, -1e4], [-3e3, 2e3]]])
+    ref = np_search(far, 302, 302, 302, 40, 0.008 ** 2)
+    got = oracle_lib.search(far, 302, 302, 302, 40, 0.008 ** 2)
+    for a, b in zip(got, ref):
+        assert np.array_equal(a, b)
 
 
 # --------------------------------------------------------------- pair sums

@@ -395,9 +395,10 @@ struct PairCtx {
     }
     float ax = -mq * pij * gx;
     float ay = -mq * pij * gy;
-    // wall spring (derivatives.c:2396-2449)
+    // wall spring (derivatives.c:2396-2449).  The damping part does not vanish at
+    // r0, so the range test is the reference's FP64 sqrt(d2) < r0.
     const uint32_t tyq = tag_type(tq);
-    if (p_is_i && (tyq == 2 || tyq == 3 || tyq == 5) && r < S.fr0) {
+    if (p_is_i && (tyq == 2 || tyq == 3 || tyq == 5) && sqrt(d2) < S.r0) {
       const float kr = S.fk * (S.fr0 - r) / (r * mp);
       const float cr = S.fcd * vr / (r2 * mp);
       ax -= kr * dx + cr * dx;

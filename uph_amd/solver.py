@@ -171,14 +171,19 @@ class GpuSolver:
     def itime(self) -> int:
         return lib().uph_itime(self.h)
 
+    def _scene(self, scene: int):
+        if not 0 <= scene < len(self.scenes):
+            raise IndexError(f"scene {scene} out of range (batch of {len(self.scenes)})")
+        return self.scenes[scene]
+
     def state(self, scene: int = 0):
-        sc = self.scenes[scene]
+        sc = self._scene(scene)
         x = np.zeros((sc.n, 2)); v = np.zeros((sc.n, 2)); rho = np.zeros(sc.n)
         self._check(lib().uph_get_state(self.h, scene, _ptr(x), _ptr(v), _ptr(rho)))
         return dict(x=x, v=v, rho=rho)
 
     def rates(self, scene: int = 0):
-        nf = self.scenes[scene].n_fluid
+        nf = self._scene(scene).n_fluid
         R = np.zeros(nf, np.float32); Rt = np.zeros(nf, np.float32)
         a = np.zeros((nf, 2), np.float32)
         self._check(lib().uph_get_rates(self.h, scene, _ptr(R, C.c_float), _ptr(Rt, C.c_float),
