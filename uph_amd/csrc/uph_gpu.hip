@@ -31,6 +31,7 @@
 #define SCAN_ITEMS 16
 #define SCAN_BLK 256
 #define SCAN_TILE (SCAN_ITEMS * SCAN_BLK)
+#define FIX_GRID 512   // blocks of the normally empty capped-path launches
 
 // ------------------------------------------------------------------ sort
 
@@ -39,15 +40,31 @@ __global__ __launch_bounds__(BLK) void k_bin(int n, const double2* __restrict__ 
                                              uint32_t* __restrict__ tagB, int* __restrict__ keys,
                                              int* __restrict__ rank, int* __restrict__ count) {
   const int i = blockIdx.x * BLK + threadIdx.x;
-  if (i >= n) return;
-  const DevScene& S = sc[sceneOf[i]];
-  const double2 p = pos[i];
-  int cx, cy;
-  cell_of(S, p.x, p.y, cx, cy);
-  const int key = S.coff + cy * S.ncx + cx;
-  keys[i] = key;
-  rank[i] = atomicAdd(&count[key], 1);
-  tagB[i] = tagA[i];
+  const bool valid = i < n;
+  int key = -1;
+  if (valid) {
+    const DevScene& S = sc[sceneOf[i]];
+    const double2 p = pos[i];
+    int cx, cy;
+    cell_of(S, p.x, p.y, cx, cy);
+    key = S.coff + cy * S.ncx + cx;
+    keys[i] = key;
+    tagB[i] = tagA[i];
+  }
+  // Slots arrive sorted by the previous step's cells, so equal keys come in runs
+  // of consecutive lanes: one atomic per run reserves the run's ranks.
+  const int lane = threadIdx.x & 63;
+  const int prev = __shfl_up(key, 1, 64);
+  const bool head = !valid || lane == 0 || prev != key;
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long upto = heads & ((2ull << lane) - 1ull);   // bits 0..lane
+  const int rstart = 63 - __clzll(upto);
+  const unsigned long long above = lane == 63 ? 0ull : (heads & (~0ull << (lane + 1)));
+  const int rend = above ? __ffsll(above) - 1 : 64;
+  int base = 0;
+  if (valid && lane == rstart) base = atomicAdd(&count[key], rend - rstart);
+  base = __shfl(base, rstart, 64);
+  if (valid) rank[i] = base + (lane - rstart);
 }
 
 // Block-wide exclusive scan of SCAN_TILE ints; returns the tile total.
@@ -90,11 +107,11 @@ __device__ int block_scan_tile(int* vals /* SCAN_ITEMS per thread, in registers 
 
 __global__ __launch_bounds__(SCAN_BLK) void k_scan_reduce(int m, const int* __restrict__ in, int* __restrict__ partial) {
   __shared__ int lds[SCAN_BLK / 64 + 1];
-  const long base = (long)blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+  const long base = (long)blockIdx.x * SCAN_TILE + threadIdx.x;
   int s = 0;
 #pragma unroll
   for (int k = 0; k < SCAN_ITEMS; ++k) {
-    const long i = base + k;
+    const long i = base + (long)k * SCAN_BLK;
     if (i < m) s += in[i];
   }
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -149,27 +166,48 @@ __global__ __launch_bounds__(1024) void k_scan_top(int nb, int* __restrict__ par
   }
 }
 
-// Exclusive scan of `count` into `start` (m+1 entries), zeroing `count`.
+// Exclusive scan of `count` into `start` (m+1 entries), zeroing `count`.  The
+// tile goes through LDS (padded by one word per 16) so global accesses are
+// coalesced and each thread scans 16 consecutive entries without bank conflicts.
+__device__ __forceinline__ int tile_ix(int e) { return e + (e >> 4); }
+
 __global__ __launch_bounds__(SCAN_BLK) void k_scan_down(int m, int* __restrict__ count, const int* __restrict__ partial,
                                                         int* __restrict__ start) {
+  __shared__ int tile[SCAN_TILE + SCAN_TILE / 16];
   __shared__ int lds[SCAN_BLK / 64 + 1];
-  const long base = (long)blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
-  int v[SCAN_ITEMS];
+  const long base = (long)blockIdx.x * SCAN_TILE;
+  const int tid = threadIdx.x;
 #pragma unroll
   for (int k = 0; k < SCAN_ITEMS; ++k) {
-    const long i = base + k;
-    v[k] = i < m ? count[i] : 0;
+    const int e = k * SCAN_BLK + tid;
+    const long i = base + e;
+    int v = 0;
+    if (i < m) {
+      v = count[i];
+      count[i] = 0;
+    }
+    tile[tile_ix(e)] = v;
   }
+  __syncthreads();
+  int v[SCAN_ITEMS];
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) v[k] = tile[tile_ix(tid * SCAN_ITEMS + k)];
   block_scan_tile(v, lds);
   const int off = partial[blockIdx.x];
 #pragma unroll
   for (int k = 0; k < SCAN_ITEMS; ++k) {
-    const long i = base + k;
-    if (i < m) {
-      start[i] = v[k] + off;
-      if (i == m - 1) start[m] = v[k] + off + count[i];
-      count[i] = 0;
-    }
+    const int e = tid * SCAN_ITEMS + k;
+    const int orig = tile[tile_ix(e)];
+    const int ex = v[k] + off;
+    if (base + e == (long)m - 1) start[m] = ex + orig;
+    tile[tile_ix(e)] = ex;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    const int e = k * SCAN_BLK + tid;
+    const long i = base + e;
+    if (i < m) start[i] = tile[tile_ix(e)];
   }
 }
 
@@ -315,24 +353,27 @@ __device__ void kth_key(const DevScene& S, int p, const double2* __restrict__ po
 
 __global__ __launch_bounds__(BLK) void k_kth(int n, StepBufs b, const DevScene* __restrict__ sc) {
   if (b.ctl[1] == 0) return;
-  const int p = xcd_block() * BLK + threadIdx.x;
-  if (p == 0) atomicAdd(&b.ctl[2], 1);
-  if (p >= n) return;
-  if (tag_type(b.tagA[p]) != 1) return;
-  const DevScene& S = sc[b.sceneOf[p]];
-  int cnt;
-  double kd2;
-  uint32_t kid;
-  kth_key(S, p, b.posA, b.tagA, b.cellStart, cnt, kd2, kid);
-  b.kthD2[p] = kd2;
-  b.kthId[p] = kid;
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&b.ctl[2], 1);
+  const int nlog = (n + BLK - 1) / BLK;
+  for (int lb = xcd_block(); lb < nlog; lb += gridDim.x) {
+    const int p = lb * BLK + threadIdx.x;
+    if (p >= n) continue;
+    if (tag_type(b.tagA[p]) != 1) continue;
+    const DevScene& S = sc[b.sceneOf[p]];
+    int cnt;
+    double kd2;
+    uint32_t kid;
+    kth_key(S, p, b.posA, b.tagA, b.cellStart, cnt, kd2, kid);
+    b.kthD2[p] = kd2;
+    b.kthId[p] = kid;
+  }
 }
 
 // State of one fluid particle p during its pair sums.
 struct PairCtx {
   uint32_t idp;
   bool pb_ok;
-  float vpx, vpy, lpx, lpy, rhop, prp, mp;
+  float vpx, vpy, lpx, lpy, rhop, prp, mp, inv_mp;
   float h, invh, walpha, h2eps;
   CSum sR, sRt, sAx, sAy;
 
@@ -341,6 +382,7 @@ struct PairCtx {
     pb_ok = (int)idp < S.pb_gid;
     vpx = ap.x; vpy = ap.y; lpx = ap.z; lpy = ap.w;
     rhop = bp.x; prp = bp.y; mp = bp.z;
+    inv_mp = 1.0f / mp;
     h = S.fh; invh = S.finvh; walpha = S.fwalpha;
     h2eps = 0.01f * h * h;
   }
@@ -361,12 +403,15 @@ struct PairCtx {
   }
 
   // One admitted pair: x_p - x_q = (ex, ey), |.|^2 = d2 (FP64 geometry, FP32 sums).
+  // FP32 arithmetic may contract to FMA here; 1/r and 1/x use v_rsq_f32 / v_rcp_f32.
   __device__ __forceinline__ void add(const DevScene& S, double ex, double ey, double d2, uint32_t tq, float4 aq,
                                       float4 bq) {
+#pragma clang fp contract(fast)
     const bool p_is_i = tag_id(tq) > idp;   // pair (i=p, j=q) or (i=q, j=p)
     const float dx = (float)ex, dy = (float)ey;
     const float r2 = (float)d2;
-    const float r = sqrtf(r2);
+    const float rinv = __builtin_amdgcn_rsqf(r2);
+    const float r = r2 * rinv;
     const float qq = r * invh;
     // cubic spline gradient factor (derivatives.c:44-66): gradW = f * (x_p - x_q)
     float f;
@@ -374,7 +419,7 @@ struct PairCtx {
       f = walpha * (-3.0f + 2.25f * qq) * invh * invh;
     } else {
       const float t = 2.0f - qq;
-      f = -walpha * 0.75f * t * t * invh / r;
+      f = -walpha * 0.75f * t * t * invh * rinv;
     }
     const float gx = f * dx, gy = f * dy;
     const float mq = bq.z;
@@ -390,8 +435,8 @@ struct PairCtx {
     const float vr = dvx * dx + dvy * dy;
     float pij = prp + bq.y;
     if (vr < 0.0f) {
-      const float mu = h * vr / (r2 + h2eps);
-      pij += -S.fav * S.fc0 * mu / (0.5f * (rhop + bq.x));
+      const float mu = h * vr * __builtin_amdgcn_rcpf(r2 + h2eps);
+      pij += -S.fav * S.fc0 * mu * (2.0f * __builtin_amdgcn_rcpf(rhop + bq.x));
     }
     float ax = -mq * pij * gx;
     float ay = -mq * pij * gy;
@@ -399,8 +444,8 @@ struct PairCtx {
     // r0, so the range test is the reference's FP64 sqrt(d2) < r0.
     const uint32_t tyq = tag_type(tq);
     if (p_is_i && (tyq == 2 || tyq == 3 || tyq == 5) && sqrt(d2) < S.r0) {
-      const float kr = S.fk * (S.fr0 - r) / (r * mp);
-      const float cr = S.fcd * vr / (r2 * mp);
+      const float kr = S.fk * (S.fr0 - r) * rinv * inv_mp;
+      const float cr = S.fcd * vr * (rinv * rinv) * inv_mp;
       ax -= kr * dx + cr * dx;
       ay -= kr * dy + cr * dy;
     }
@@ -449,10 +494,7 @@ __device__ __forceinline__ void correct(const DevScene& S, const StepBufs& b, in
 // Direct form: one pass over the candidates, physics inside the distance test.
 // EXACT applies the K-nearest cap (run only when some query overflowed).
 template <bool EXACT>
-__global__ __launch_bounds__(BLK) void k_pair(int n, StepBufs b, const DevScene* __restrict__ sc) {
-  if (EXACT && b.ctl[1] == 0) return;
-  const int p = xcd_block() * BLK + threadIdx.x;
-  if (p >= n) return;
+__device__ __forceinline__ void pair_direct(int p, const StepBufs& b, const DevScene* __restrict__ sc) {
   const uint32_t tg = b.tagA[p];
   const DevScene& S = sc[b.sceneOf[p]];
   const int itime = b.ctl[0];
@@ -487,6 +529,18 @@ __global__ __launch_bounds__(BLK) void k_pair(int n, StepBufs b, const DevScene*
   });
   if (!EXACT && cnt > S.K) atomicOr(&b.ctl[1], 1);
   correct(S, b, p, itime, xp, vn, rt, c);
+}
+
+// EXACT (the capped path) runs as a small grid-stride launch that returns at
+// once unless the step's overflow flag is set.
+template <bool EXACT>
+__global__ __launch_bounds__(BLK) void k_pair(int n, StepBufs b, const DevScene* __restrict__ sc) {
+  if (EXACT && b.ctl[1] == 0) return;
+  const int nlog = (n + BLK - 1) / BLK;
+  for (int lb = xcd_block(); lb < nlog; lb += gridDim.x) {
+    const int p = lb * BLK + threadIdx.x;
+    if (p < n) pair_direct<EXACT>(p, b, sc);
+  }
 }
 
 // Compacted form (default): pass 1 runs only the FP64 distance test over the
@@ -765,8 +819,9 @@ static int launch_step(uph_handle* h, hipEvent_t pa, hipEvent_t pb) {
   else
     hipLaunchKernelGGL(k_pair_compact, dim3(grid8_of(h->n)), dim3(BLK), 0, st, h->n, h->b, h->dsc);
   if (pb) HIPCHK(hipEventRecord(pb, st), h->err);
-  hipLaunchKernelGGL(k_kth, dim3(grid8_of(h->n)), dim3(BLK), 0, st, h->n, h->b, h->dsc);
-  hipLaunchKernelGGL(k_pair<true>, dim3(grid8_of(h->n)), dim3(BLK), 0, st, h->n, h->b, h->dsc);
+  const int gfix = std::min(grid8_of(h->n), FIX_GRID);
+  hipLaunchKernelGGL(k_kth, dim3(gfix), dim3(BLK), 0, st, h->n, h->b, h->dsc);
+  hipLaunchKernelGGL(k_pair<true>, dim3(gfix), dim3(BLK), 0, st, h->n, h->b, h->dsc);
   HIPCHK(hipGetLastError(), h->err);
   return 0;
 }
