@@ -31,6 +31,7 @@
 #define SCAN_ITEMS 16
 #define SCAN_BLK 256
 #define SCAN_TILE (SCAN_ITEMS * SCAN_BLK)
+#define POS_PAD 128    // zeroed slots past the end of posA (read, never used, by k_pair_compact)
 #define FIX_GRID 512   // blocks of the normally empty capped-path launches
 
 // ------------------------------------------------------------------ sort
@@ -404,12 +405,11 @@ struct PairCtx {
 
   // One admitted pair: x_p - x_q = (ex, ey), |.|^2 = d2 (FP64 geometry, FP32 sums).
   // FP32 arithmetic may contract to FMA here; 1/r and 1/x use v_rsq_f32 / v_rcp_f32.
-  __device__ __forceinline__ void add(const DevScene& S, double ex, double ey, double d2, uint32_t tq, float4 aq,
-                                      float4 bq) {
+  __device__ __forceinline__ void add(const DevScene& S, double ex, double ey, uint32_t tq, float4 aq, float4 bq) {
 #pragma clang fp contract(fast)
     const bool p_is_i = tag_id(tq) > idp;   // pair (i=p, j=q) or (i=q, j=p)
     const float dx = (float)ex, dy = (float)ey;
-    const float r2 = (float)d2;
+    const float r2 = dx * dx + dy * dy;
     const float rinv = __builtin_amdgcn_rsqf(r2);
     const float r = r2 * rinv;
     const float qq = r * invh;
@@ -443,7 +443,8 @@ struct PairCtx {
     // wall spring (derivatives.c:2396-2449).  The damping part does not vanish at
     // r0, so the range test is the reference's FP64 sqrt(d2) < r0.
     const uint32_t tyq = tag_type(tq);
-    if (p_is_i && (tyq == 2 || tyq == 3 || tyq == 5) && sqrt(d2) < S.r0) {
+    if (p_is_i && (tyq == 2 || tyq == 3 || tyq == 5) &&
+        sqrt(__dadd_rn(__dmul_rn(ex, ex), __dmul_rn(ey, ey))) < S.r0) {
       const float kr = S.fk * (S.fr0 - r) * rinv * inv_mp;
       const float cr = S.fcd * vr * (rinv * rinv) * inv_mp;
       ax -= kr * dx + cr * dx;
@@ -525,7 +526,7 @@ __device__ __forceinline__ void pair_direct(int p, const StepBufs& b, const DevS
     const float4 bq = b.mB[q];
     const uint32_t tq = __float_as_uint(bq.w);
     if (!c.template admit<EXACT>(S, tag_id(tq), d2, kpd2, kpid, b, q)) return;
-    c.add(S, ex, ey, d2, tq, b.mA[q], bq);
+    c.add(S, ex, ey, tq, b.mA[q], bq);
   });
   if (!EXACT && cnt > S.K) atomicOr(&b.ctl[1], 1);
   correct(S, b, p, itime, xp, vn, rt, c);
@@ -543,76 +544,116 @@ __global__ __launch_bounds__(BLK) void k_pair(int n, StepBufs b, const DevScene*
   }
 }
 
-// Compacted form (default): pass 1 runs only the FP64 distance test over the
-// three candidate row ranges and records the in-range slots in LDS (16-bit
-// offsets into the row ranges); pass 2 runs the physics over that list, so the
-// lanes of a wavefront stay busy on in-range pairs.  A query with more than
-// min(K, UPH_NL) candidates in range, or an offset beyond 14 bits, raises the
-// overflow flag: the exact direct kernel then recomputes the step.
-#define UPH_NL 48
+// Compacted form (default).  Pass 1 runs only the FP64 distance test over the
+// three candidate row ranges, in a branch-free loop whose counter is uniform
+// across the wavefront (each lane walks its own row range by pointer), and
+// records in-range candidates as bits of one 64-bit mask per row.  Pass 2 walks
+// the set bits, so every lane spends its physics on in-range pairs only, with the
+// next pair's loads issued before the current pair's math.  A row range longer
+// than 64 slots, or more than K candidates in range, raises the step's overflow
+// flag and the exact direct kernel recomputes the step.
 __global__ __launch_bounds__(BLK) void k_pair_compact(int n, StepBufs b, const DevScene* __restrict__ sc) {
-  __shared__ uint16_t nl[UPH_NL][BLK];
   const int p = xcd_block() * BLK + threadIdx.x;
-  if (p >= n) return;
-  const uint32_t tg = b.tagA[p];
-  const DevScene& S = sc[b.sceneOf[p]];
+  const bool live = p < n;
+  const int pp = live ? p : n - 1;
+  const uint32_t tg = b.tagA[pp];
+  const bool fluid = live && tag_type(tg) == 1;
+  const DevScene& S = sc[b.sceneOf[pp]];
   const int itime = b.ctl[0];
-  const double2 xp = b.posA[p];
-  const double2 vn = b.velA[p];
-  const float4 rt = b.rtA[p];
-  if (tag_type(tg) != 1) {
-    wall_update(S, b, p, tg, itime, rt);
-    return;
-  }
+  const double2 xp = b.posA[pp];
   const double cut = S.cutoff;
-  const int tid = threadIdx.x;
   int cx, cy;
   cell_of(S, xp.x, xp.y, cx, cy);
   const int x0 = cx > 0 ? cx - 1 : 0;
   const int x1 = cx < S.ncx - 1 ? cx + 1 : S.ncx - 1;
-  int rs[3];
-  int cnt = 0;
+  int rb[3], rl[3];
+  unsigned long long m[3];
   bool bad = false;
 #pragma unroll
   for (int r = 0; r < 3; ++r) {
     const int row = cy - 1 + r;
-    rs[r] = 0;
-    if (row < 0 || row >= S.ncy) continue;
-    const int base = S.coff + row * S.ncx;
-    const int qb = b.cellStart[base + x0];
-    const int qe = b.cellStart[base + x1 + 1];
-    rs[r] = qb;
+    rb[r] = 0;
+    rl[r] = 0;
+    m[r] = 0ull;
+    if (fluid && row >= 0 && row < S.ncy) {
+      const int base = S.coff + row * S.ncx;
+      rb[r] = b.cellStart[base + x0];
+      rl[r] = b.cellStart[base + x1 + 1] - rb[r];
+    }
+    if (rl[r] > 64) {
+      bad = true;
+      rl[r] = 0;
+    }
+    const int lmax = wave_max(rl[r]);
+    const double2* rp = b.posA + rb[r];
 #pragma unroll 1
-    for (int q = qb; q < qe; ++q) {
-      if (q == p) continue;
-      const double2 xq = b.posA[q];
-      const double ex = xp.x - xq.x, ey = xp.y - xq.y;
-      const double d2 = __dadd_rn(__dmul_rn(ex, ex), __dmul_rn(ey, ey));
-      if (!(d2 < cut)) continue;
-      const int off = q - qb;
-      if (cnt < UPH_NL && off < (1 << 14)) nl[cnt][tid] = (uint16_t)((r << 14) | off);
-      else bad = true;
-      ++cnt;
+    for (int k = 0; k < lmax; k += 4) {
+      double2 xs[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) xs[u] = rp[u];   // posA has POS_PAD zeroed slots past n
+      rp += 4;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const double ex = xp.x - xs[u].x, ey = xp.y - xs[u].y;
+        const double d2 = __dadd_rn(__dmul_rn(ex, ex), __dmul_rn(ey, ey));
+        const bool in = (k + u < rl[r]) && (d2 < cut);
+        m[r] |= in ? (1ull << ((k + u) & 63)) : 0ull;
+      }
     }
   }
+  if (!live) return;
+  const double2 vn = b.velA[p];
+  const float4 rt = b.rtA[p];
+  if (!fluid) {
+    wall_update(S, b, p, tg, itime, rt);
+    return;
+  }
+  // drop the particle itself (always in range, in its own row)
+  const int self = p - rb[1];
+  if (self >= 0 && self < 64) m[1] &= ~(1ull << self);
+  const int cnt = __popcll(m[0]) + __popcll(m[1]) + __popcll(m[2]);
   if (bad || cnt > S.K) {
     atomicOr(&b.ctl[1], 1);
     return;   // the exact kernel recomputes this step
   }
   PairCtx c;
   c.init(S, tg, b.mA[p], b.mB[p]);
+  // walk the set bits row by row; next slot found (and loaded) one pair ahead
+  int rr = 0;
+  unsigned long long cm = m[0];
+  int cb = rb[0];
+  auto next = [&]() -> int {
+    while (cm == 0ull && rr < 2) {
+      ++rr;
+      cm = rr == 1 ? m[1] : m[2];
+      cb = rr == 1 ? rb[1] : rb[2];
+    }
+    if (cm == 0ull) return -1;
+    const int k = __ffsll((long long)cm) - 1;
+    cm &= cm - 1ull;
+    return cb + k;
+  };
+  int q = next();
+  double2 xq;
+  float4 aq, bq;
+  if (q >= 0) {
+    xq = ld_off(b.posA, (uint32_t)q);
+    aq = ld_off(b.mA, (uint32_t)q);
+    bq = ld_off(b.mB, (uint32_t)q);
+  }
 #pragma unroll 1
-  for (int k = 0; k < cnt; ++k) {
-    const uint32_t e = nl[k][tid];
-    const int r = (int)(e >> 14);
-    const int q = (r == 0 ? rs[0] : (r == 1 ? rs[1] : rs[2])) + (int)(e & 0x3fffu);
-    const double2 xq = b.posA[q];
-    const double ex = xp.x - xq.x, ey = xp.y - xq.y;
-    const double d2 = __dadd_rn(__dmul_rn(ex, ex), __dmul_rn(ey, ey));
-    const float4 bq = b.mB[q];
-    const uint32_t tq = __float_as_uint(bq.w);
-    if (!c.template admit<false>(S, tag_id(tq), d2, 0.0, 0u, b, q)) continue;
-    c.add(S, ex, ey, d2, tq, b.mA[q], bq);
+  while (q >= 0) {
+    const double2 xc = xq;
+    const float4 ac = aq, bc = bq;
+    q = next();
+    if (q >= 0) {
+      xq = ld_off(b.posA, (uint32_t)q);
+      aq = ld_off(b.mA, (uint32_t)q);
+      bq = ld_off(b.mB, (uint32_t)q);
+    }
+    const uint32_t tq = __float_as_uint(bc.w);
+    if (!c.template admit<false>(S, tag_id(tq), 0.0, 0.0, 0u, b, 0)) continue;
+    c.add(S, xp.x - xc.x, xp.y - xc.y, tq, ac, bc);
   }
   correct(S, b, p, itime, xp, vn, rt, c);
 }
@@ -770,7 +811,7 @@ static void fill_scene(DevScene& S, const uph_scene& u) {
 
 static int alloc_bufs(StepBufs& b, int n, int ncells, int nblocks_scan, Err& err, bool search_only) {
   memset(&b, 0, sizeof b);
-  if (dalloc(&b.posA, n, err) || dalloc(&b.posB, n, err) || dalloc(&b.tagA, n, err) ||
+  if (dalloc(&b.posA, (size_t)n + POS_PAD, err) || dalloc(&b.posB, n, err) || dalloc(&b.tagA, n, err) ||
       dalloc(&b.tagB, n, err) || dalloc(&b.keys, n, err) || dalloc(&b.rank, n, err) ||
       dalloc(&b.tmpIdx, n, err) || dalloc(&b.cellStart, (size_t)ncells + 1, err) ||
       dalloc(&b.cellCount, ncells, err) || dalloc(&b.partial, nblocks_scan, err) ||
@@ -954,6 +995,7 @@ extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts*
   }
   // cell counts start at zero; every scan leaves them zero again
   if (hipMemset(h->b.cellCount, 0, (size_t)h->ncells * sizeof(int)) != hipSuccess ||
+      hipMemset(h->b.posA + h->n, 0, POS_PAD * sizeof(double2)) != hipSuccess ||
       hipDeviceSynchronize() != hipSuccess) {
     err.msg = "uph_create: init failed";
     return fail();

@@ -76,17 +76,32 @@ __device__ __forceinline__ void for_candidates(const DevScene& S, double2 p, con
   }
 }
 
-// Neumaier-compensated FP32 sum.
+// Kahan-compensated FP32 sum (c carries the rounding error of the last add).
 struct CSum {
   float s, c;
   __device__ __forceinline__ CSum() : s(0.f), c(0.f) {}
   __device__ __forceinline__ void add(float x) {
-    const float t = s + x;
-    c += (fabsf(s) >= fabsf(x)) ? ((s - t) + x) : ((x - t) + s);
+    const float y = x - c;
+    const float t = s + y;
+    c = (t - s) - y;
     s = t;
   }
-  __device__ __forceinline__ float val() const { return s + c; }
+  __device__ __forceinline__ float val() const { return s - c; }
 };
+
+// Load of base[idx] through a 32-bit byte offset from a uniform base (lets the
+// compiler use the scalar-base + vector-offset addressing form).
+template <class T>
+__device__ __forceinline__ T ld_off(const T* __restrict__ base, uint32_t idx) {
+  return *(const T*)((const char*)base + (size_t)(idx * (uint32_t)sizeof(T)));
+}
+
+// Wavefront maximum of an int.
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  return v;
+}
 
 // Tait equation of state in FP64 (derivatives.c:1674-1682); integer gamma by
 // repeated squaring, otherwise pow().
