@@ -623,10 +623,15 @@ __global__ __launch_bounds__(BLK) void k_pair_compact(int n, StepBufs b, const D
   unsigned long long cm = m[0];
   int cb = rb[0];
   auto next = [&]() -> int {
-    while (cm == 0ull && rr < 2) {
-      ++rr;
-      cm = rr == 1 ? m[1] : m[2];
-      cb = rr == 1 ? rb[1] : rb[2];
+    if (cm == 0ull && rr == 0) {
+      rr = 1;
+      cm = m[1];
+      cb = rb[1];
+    }
+    if (cm == 0ull && rr == 1) {
+      rr = 2;
+      cm = m[2];
+      cb = rb[2];
     }
     if (cm == 0ull) return -1;
     const int k = __ffsll((long long)cm) - 1;
