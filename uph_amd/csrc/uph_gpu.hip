@@ -3,17 +3,21 @@
 // One step of sph.cc:90-168 (the predictor, derivatives() of
 // derivatives.c:3137 in the scheme's configuration, the corrector) is, for all
 // scenes of a batch at once:
-//   k_bin        cell key of every particle, rank inside its cell (atomic)
-//   k_scan_*     exclusive scan of cell counts -> cell start offsets
-//   k_scatter    provisional sorted slot of every particle
-//   k_reorder    final slot by (cell, original index), gather of the FP64
-//                state, predictor and FP64 Tait pressure -> FP32 neighbour mirror
-//   k_pair       fused neighbour search + W/gradW + continuity (both forms) +
-//                pressure gradient + artificial viscosity + gravity + wall
-//                springs, FP32 compensated sums, FP64 leapfrog corrector
-//   k_kth/k_pair<exact>  only when some fluid particle has more than K
-//                neighbours in range: the capped (K-nearest) neighbour sets
-//                of Search.cc:27-40, then the pair sums again.
+//   k_bin         cell key of every particle, rank inside its cell
+//                 (wavefront-aggregated atomics)
+//   k_scan_reduce per-tile cell totals
+//   k_scan_top    exclusive scan of the tile totals; opens the step
+//   k_scan_down   exclusive scan of cell counts -> cell start offsets
+//   k_scatter     provisional sorted slot of every particle
+//   k_reorder     final slot by (cell, original index), gather of the FP64
+//                 state, predictor and FP64 Tait pressure -> FP32 neighbour mirror
+//   k_pair_compact fused neighbour search + W/gradW + continuity (both forms) +
+//                 pressure gradient + artificial viscosity + gravity + wall
+//                 springs, FP32 compensated sums, FP64 leapfrog corrector; a
+//                 query with more than K neighbours in range records its K-th
+//                 key and flags the step
+//   k_pair<exact> returns at once unless the step was flagged; then applies the
+//                 capped (K-nearest) neighbour sets of Search.cc:27-40.
 #include "uph_kernels.h"
 #include "../../include/uph.h"
 
@@ -223,7 +227,6 @@ struct StepBufs {
   // A: sorted input of the pair kernel
   double2 *posA, *velA;
   double* rhoA;
-  float4* rtA;
   uint32_t* tagA;
   float4 *mA, *mB;
   // B: pair-kernel output, same slot order as A
@@ -234,9 +237,10 @@ struct StepBufs {
   // sort scratch
   int *keys, *rank, *tmpIdx, *cellStart, *cellCount, *partial;
   int* sceneOf;
-  // capped search
+  // capped search: K-th key of overflowing queries, valid when kthStep == itime
   double* kthD2;
   uint32_t* kthId;
+  int* kthStep;
   // per original index (global id)
   double2 *deckPos, *deckVel, *deckLvel;
   // control: [0] itime, [1] overflow flag of the step, [2] overflowed steps
@@ -269,7 +273,6 @@ __global__ __launch_bounds__(BLK) void k_reorder(int n, StepBufs b, const DevSce
   const float4 rt = b.rtB[i];
   b.velA[dst] = v;
   b.rhoA[dst] = rho;
-  b.rtA[dst] = rt;
   const int itime = b.ctl[0];
   const double dt = S.dt;
   double vx = v.x, vy = v.y, lx, ly, rp = rho;
@@ -352,21 +355,28 @@ __device__ void kth_key(const DevScene& S, int p, const double2* __restrict__ po
   kid = ti;
 }
 
-__global__ __launch_bounds__(BLK) void k_kth(int n, StepBufs b, const DevScene* __restrict__ sc) {
-  if (b.ctl[1] == 0) return;
-  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&b.ctl[2], 1);
-  const int nlog = (n + BLK - 1) / BLK;
-  for (int lb = xcd_block(); lb < nlog; lb += gridDim.x) {
-    const int p = lb * BLK + threadIdx.x;
-    if (p >= n) continue;
-    if (tag_type(b.tagA[p]) != 1) continue;
-    const DevScene& S = sc[b.sceneOf[p]];
-    int cnt;
-    double kd2;
-    uint32_t kid;
-    kth_key(S, p, b.posA, b.tagA, b.cellStart, cnt, kd2, kid);
-    b.kthD2[p] = kd2;
-    b.kthId[p] = kid;
+// Overflowing query (more than K candidates in range, or a row range the
+// compacted kernel cannot hold): its K-th (d2, id) key, stamped with the step,
+// and the step's flag that makes the exact kernel run.
+__device__ void mark_overflow(const DevScene& S, const StepBufs& b, int p, int itime) {
+  int cnt;
+  double kd2;
+  uint32_t kid;
+  kth_key(S, p, b.posA, b.tagA, b.cellStart, cnt, kd2, kid);
+  b.kthD2[p] = kd2;
+  b.kthId[p] = kid;
+  b.kthStep[p] = itime;
+  atomicOr(&b.ctl[1], 1);
+}
+
+// K-th key of slot q in this step, or +inf when q did not overflow.
+__device__ __forceinline__ void kth_of(const StepBufs& b, int q, int itime, double& kd2, uint32_t& kid) {
+  if (b.kthStep[q] == itime) {
+    kd2 = b.kthD2[q];
+    kid = b.kthId[q];
+  } else {
+    kd2 = INFINITY;
+    kid = 0xffffffffu;
   }
 }
 
@@ -398,7 +408,12 @@ struct PairCtx {
       if (EXACT && !key_le(d2, idq, kpd2, kpid)) return false;
     } else {
       if (!pb_ok) return false;
-      if (EXACT && !key_le(d2, idp, b.kthD2[q], b.kthId[q])) return false;
+      if (EXACT) {
+        double kd2;
+        uint32_t kid;
+        kth_of(b, q, b.ctl[0], kd2, kid);
+        if (!key_le(d2, idp, kd2, kid)) return false;
+      }
     }
     return true;
   }
@@ -457,19 +472,19 @@ struct PairCtx {
 
 // Boundary slot: wall state for the next step (deck + kinematics at t = (itime+1) dt).
 __device__ __forceinline__ void wall_update(const DevScene& S, const StepBufs& b, int p, uint32_t tg, int itime,
-                                            float4 rt) {
+                                            float mass) {
   const uint32_t id = tag_id(tg);
   double2 x = b.deckPos[id], v = b.deckVel[id];
   if (tag_type(tg) == 3 && S.motion != 0) wall_motion(S, (itime + 1) * S.dt, x.x, x.x, v.x);
   b.posB[p] = x;
   b.velB[p] = v;
   b.rhoB[p] = b.rhoA[p];
-  b.rtB[p] = rt;
+  b.rtB[p] = make_float4(0.f, 0.f, 0.f, mass);
 }
 
 // Gravity and the FP64 leapfrog corrector (sph.cc:137-168) for fluid slot p.
 __device__ __forceinline__ void correct(const DevScene& S, const StepBufs& b, int p, int itime, double2 xp,
-                                        double2 vn, float4 rt, const PairCtx& c) {
+                                        double2 vn, float mass, const PairCtx& c) {
   const double dt = S.dt;
   const float R = c.sR.val(), Rt = c.sRt.val();
   double ax = (double)c.sAx.val(), ay = (double)c.sAy.val();
@@ -488,7 +503,7 @@ __device__ __forceinline__ void correct(const DevScene& S, const StepBufs& b, in
   b.posB[p] = make_double2(xp.x + dt * v1x, xp.y + dt * v1y);
   b.velB[p] = make_double2(v1x, v1y);
   b.rhoB[p] = rho1;
-  b.rtB[p] = make_float4(Rt, (float)ax, (float)ay, rt.w);
+  b.rtB[p] = make_float4(Rt, (float)ax, (float)ay, mass);
   if (b.dbgRates) b.dbgRates[p] = make_float4(R, Rt, (float)ax, (float)ay);
 }
 
@@ -501,9 +516,9 @@ __device__ __forceinline__ void pair_direct(int p, const StepBufs& b, const DevS
   const int itime = b.ctl[0];
   const double2 xp = b.posA[p];
   const double2 vn = b.velA[p];
-  const float4 rt = b.rtA[p];
+  const float mass = b.mB[p].z;
   if (tag_type(tg) != 1) {
-    wall_update(S, b, p, tg, itime, rt);
+    wall_update(S, b, p, tg, itime, mass);
     return;
   }
   PairCtx c;
@@ -511,10 +526,7 @@ __device__ __forceinline__ void pair_direct(int p, const StepBufs& b, const DevS
   const double cut = S.cutoff;
   double kpd2 = INFINITY;
   uint32_t kpid = 0xffffffffu;
-  if (EXACT) {
-    kpd2 = b.kthD2[p];
-    kpid = b.kthId[p];
-  }
+  if (EXACT) kth_of(b, p, itime, kpd2, kpid);
   int cnt = 0;
   for_candidates(S, xp, b.cellStart, [&](int q) {
     if (q == p) return;
@@ -528,8 +540,11 @@ __device__ __forceinline__ void pair_direct(int p, const StepBufs& b, const DevS
     if (!c.template admit<EXACT>(S, tag_id(tq), d2, kpd2, kpid, b, q)) return;
     c.add(S, ex, ey, tq, b.mA[q], bq);
   });
-  if (!EXACT && cnt > S.K) atomicOr(&b.ctl[1], 1);
-  correct(S, b, p, itime, xp, vn, rt, c);
+  if (!EXACT && cnt > S.K) {
+    mark_overflow(S, b, p, itime);
+    return;
+  }
+  correct(S, b, p, itime, xp, vn, mass, c);
 }
 
 // EXACT (the capped path) runs as a small grid-stride launch that returns at
@@ -537,6 +552,7 @@ __device__ __forceinline__ void pair_direct(int p, const StepBufs& b, const DevS
 template <bool EXACT>
 __global__ __launch_bounds__(BLK) void k_pair(int n, StepBufs b, const DevScene* __restrict__ sc) {
   if (EXACT && b.ctl[1] == 0) return;
+  if (EXACT && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&b.ctl[2], 1);
   const int nlog = (n + BLK - 1) / BLK;
   for (int lb = xcd_block(); lb < nlog; lb += gridDim.x) {
     const int p = lb * BLK + threadIdx.x;
@@ -603,9 +619,9 @@ __global__ __launch_bounds__(BLK) void k_pair_compact(int n, StepBufs b, const D
   }
   if (!live) return;
   const double2 vn = b.velA[p];
-  const float4 rt = b.rtA[p];
+  const float mass = b.mB[p].z;
   if (!fluid) {
-    wall_update(S, b, p, tg, itime, rt);
+    wall_update(S, b, p, tg, itime, mass);
     return;
   }
   // drop the particle itself (always in range, in its own row)
@@ -613,7 +629,7 @@ __global__ __launch_bounds__(BLK) void k_pair_compact(int n, StepBufs b, const D
   if (self >= 0 && self < 64) m[1] &= ~(1ull << self);
   const int cnt = __popcll(m[0]) + __popcll(m[1]) + __popcll(m[2]);
   if (bad || cnt > S.K) {
-    atomicOr(&b.ctl[1], 1);
+    mark_overflow(S, b, p, itime);
     return;   // the exact kernel recomputes this step
   }
   PairCtx c;
@@ -660,7 +676,7 @@ __global__ __launch_bounds__(BLK) void k_pair_compact(int n, StepBufs b, const D
     if (!c.template admit<false>(S, tag_id(tq), 0.0, 0.0, 0u, b, 0)) continue;
     c.add(S, xp.x - xc.x, xp.y - xc.y, tq, ac, bc);
   }
-  correct(S, b, p, itime, xp, vn, rt, c);
+  correct(S, b, p, itime, xp, vn, mass, c);
 }
 
 // ---------------------------------------------------------------- search
@@ -825,7 +841,7 @@ static int alloc_bufs(StepBufs& b, int n, int ncells, int nblocks_scan, Err& err
     return -1;
   if (search_only) return 0;
   if (dalloc(&b.velA, n, err) || dalloc(&b.velB, n, err) || dalloc(&b.rhoA, n, err) ||
-      dalloc(&b.rhoB, n, err) || dalloc(&b.rtA, n, err) || dalloc(&b.rtB, n, err) ||
+      dalloc(&b.rhoB, n, err) || dalloc(&b.rtB, n, err) || dalloc(&b.kthStep, n, err) ||
       dalloc(&b.mA, n, err) || dalloc(&b.mB, n, err) || dalloc(&b.deckPos, n, err) ||
       dalloc(&b.deckVel, n, err) || dalloc(&b.deckLvel, n, err))
     return -1;
@@ -833,7 +849,7 @@ static int alloc_bufs(StepBufs& b, int n, int ncells, int nblocks_scan, Err& err
 }
 
 static void free_bufs(StepBufs& b) {
-  void* ptrs[] = {b.posA, b.velA, b.rhoA, b.rtA, b.tagA, b.mA, b.mB, b.posB, b.velB, b.rhoB,
+  void* ptrs[] = {b.posA, b.velA, b.rhoA, b.kthStep, b.tagA, b.mA, b.mB, b.posB, b.velB, b.rhoB,
                   b.rtB, b.tagB, b.keys, b.rank, b.tmpIdx, b.cellStart, b.cellCount, b.partial,
                   b.sceneOf, b.kthD2, b.kthId, b.deckPos, b.deckVel, b.deckLvel, b.ctl,
                   b.dbgRates};
@@ -866,7 +882,6 @@ static int launch_step(uph_handle* h, hipEvent_t pa, hipEvent_t pb) {
     hipLaunchKernelGGL(k_pair_compact, dim3(grid8_of(h->n)), dim3(BLK), 0, st, h->n, h->b, h->dsc);
   if (pb) HIPCHK(hipEventRecord(pb, st), h->err);
   const int gfix = std::min(grid8_of(h->n), FIX_GRID);
-  hipLaunchKernelGGL(k_kth, dim3(gfix), dim3(BLK), 0, st, h->n, h->b, h->dsc);
   hipLaunchKernelGGL(k_pair<true>, dim3(gfix), dim3(BLK), 0, st, h->n, h->b, h->dsc);
   HIPCHK(hipGetLastError(), h->err);
   return 0;
@@ -1001,6 +1016,8 @@ extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts*
   // cell counts start at zero; every scan leaves them zero again
   if (hipMemset(h->b.cellCount, 0, (size_t)h->ncells * sizeof(int)) != hipSuccess ||
       hipMemset(h->b.posA + h->n, 0, POS_PAD * sizeof(double2)) != hipSuccess ||
+      hipMemset(h->b.kthStep, 0, (size_t)h->n * sizeof(int)) != hipSuccess ||
+      hipMemset(h->b.partial, 0, (size_t)h->cs.nb * sizeof(int)) != hipSuccess ||
       hipDeviceSynchronize() != hipSuccess) {
     err.msg = "uph_create: init failed";
     return fail();
@@ -1180,6 +1197,7 @@ int search_reserve(SearchCtx& c, int n, int ncells, int nb) {
     const int nbb = std::max(std::max(nb, c.cap_nb), (nn + SCAN_TILE - 1) / SCAN_TILE + 1);
     if (alloc_bufs(c.b, nn, nc, nbb, c.err, true)) return -1;
     HIPCHK(hipMemset(c.b.cellCount, 0, (size_t)nc * sizeof(int)), c.err);
+    HIPCHK(hipMemset(c.b.partial, 0, (size_t)nbb * sizeof(int)), c.err);
     HIPCHK(hipMalloc((void**)&c.qcount, ((size_t)nn + 1) * sizeof(int)), c.err);
     HIPCHK(hipMalloc((void**)&c.qoff, ((size_t)nn + 1) * sizeof(int)), c.err);
     HIPCHK(hipMalloc((void**)&c.qpartial, (size_t)nbb * sizeof(int)), c.err);
