@@ -96,7 +96,9 @@ typedef struct uph_scene {
 typedef struct uph_opts {
   int use_graph;        /* 1: replay each step as one captured hipGraph     */
   int keep_rates;       /* 1: keep the last step's FP32 rates for uph_get_rates */
-  int pair_variant;     /* 0: compacted neighbour list in LDS (default), 1: direct loop */
+  int pair_variant;     /* 0: compacted two-pass pair kernel (default), 1: direct loop */
+  int pair_block;       /* work-group size of the compacted pair kernel: 128, 256 (default), 512 */
+  double cell_factor;   /* cell edge = cell_factor x search radius, 1 <= f <= 4 (default 1) */
 } uph_opts;
 
 typedef struct uph_handle uph_t;

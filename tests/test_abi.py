@@ -41,7 +41,7 @@ def test_struct_layout_matches_header():
     # uph_scene: 3 ints, 6 pointers, then doubles/ints as declared; ctypes pads like C
     s = solver.UphScene
     assert s.pos.offset == 16 and s.box.offset + 32 == ctypes.sizeof(s)
-    assert ctypes.sizeof(solver.UphOpts) == 12
+    assert ctypes.sizeof(solver.UphOpts) == 24
 
 
 def test_missing_library_fails_loudly(monkeypatch, tmp_path):

@@ -1,0 +1,157 @@
+"""Workloads of the other configurations, the autotuner and the C host driver, on the GPU."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT, load_golden_scene
+from uph_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(f"[gpu-configs] {msg}", flush=True)
+
+
+def test_config3_wave_tank_hydrostatic_profile():
+    """6.0 x 1.0 m layer at dx = 5 mm (~240K fluid), sinusoidal piston, 1:10 beach, 5000 steps."""
+    from oracle.oracle import Sim
+    from uph_amd.solver import GpuSolver
+    sc = scenes.wave_tank()
+    _say(f"config-3: {sc.n_fluid} fluid + {sc.n_boundary} wall, 3 steps vs oracle")
+    g = GpuSolver(sc)
+    g.step(3)
+    o = Sim(sc, nthreads=0)
+    o.step(3)
+    nf = sc.n_fluid
+    sg, so = g.state(), o.state()
+    drho = float(np.max(np.abs(sg["rho"][:nf] - so["rho"][:nf]) / so["rho"][:nf]))
+    _say(f"config-3 3 steps: max rel rho err {drho:.2e}")
+    assert drho < 1e-6
+    g.step(4997)
+    assert g.sync() == 0
+    st = g.state()
+    x, y, rho = st["x"][:nf, 0], st["x"][:nf, 1], st["rho"][:nf]
+    assert np.all(np.isfinite(st["x"])) and np.all(np.isfinite(rho))
+    P = sc.B * ((rho / sc.rho0) ** 7 - 1.0)
+    H = 1.0
+    m = (x > 2.0) & (x < 4.0)
+    depth_bins = np.linspace(0.05, 0.95, 10)
+    worst = 0.0
+    for lo, hi in zip(depth_bins[:-1], depth_bins[1:]):
+        sel = m & (H - y > lo) & (H - y <= hi)
+        if sel.sum() < 50:
+            continue
+        expect = sc.rho0 * sc.g * (H - y[sel]).mean()
+        worst = max(worst, abs(P[sel].mean() - expect) / (sc.rho0 * sc.g * H))
+    _say(f"config-3 after 5000 steps: worst hydrostatic deviation {worst:.3e} of rho0 g H")
+    assert worst < 0.03
+
+
+def test_config4_batch_of_512_scenes():
+    """512 dam breaks with seeded width/height/dx in one batch: every scene equals its own
+    single-scene run, and two small scenes match the oracle."""
+    from oracle.oracle import Sim
+    from uph_amd.solver import GpuSolver
+    scs = scenes.batch_dam_breaks()
+    nf = [s.n_fluid for s in scs]
+    _say(f"config-4: {len(scs)} scenes, {sum(nf)} fluid (min {min(nf)}, max {max(nf)})")
+    gb = GpuSolver(scs, use_graph=True)
+    gb.step(20)
+    assert gb.sync() == 0
+    rng = np.random.default_rng(0)
+    for k in rng.choice(len(scs), 4, replace=False):
+        gs = GpuSolver(scs[k])
+        gs.step(20)
+        a, b = gb.state(int(k)), gs.state(0)
+        for key in ("x", "v", "rho"):
+            assert np.array_equal(a[key], b[key]), (k, key)
+    small = np.argsort(nf)[:2]
+    for k in small:
+        o = Sim(scs[k])
+        o.step(20)
+        so, sg = o.state(), gb.state(int(k))
+        n = scs[k].n_fluid
+        drho = float(np.max(np.abs(sg["rho"][:n] - so["rho"][:n]) / so["rho"][:n]))
+        assert drho < 1e-6, (k, drho)
+    tot, _ = gb.time_steps(20)
+    _say(f"config-4: {tot / 20:.3f} ms per batch step, "
+         f"{sum(nf) * 20 / (tot * 1e-3):.3e} particle-updates/s")
+    gb.step(460)                       # 500 steps in all
+    assert gb.sync() == 0
+    for k in (0, 100, 511):
+        st = gb.state(k)
+        n = scs[k].n_fluid
+        assert np.all(np.isfinite(st["x"])) and 950 < st["rho"][:n].min() <= st["rho"][:n].max() < 1100
+
+
+def test_autotune_on_gpu():
+    from oracle.oracle import Sim
+    from uph_amd import autotune
+    from uph_amd.solver import GpuSolver
+    sc = scenes.dam_break(dx=0.004)
+    res = autotune.autotune(sc, warmup=2, steps=5, log=_say)
+    best = res["best"]
+    _say(f"autotune best {best} at {res['ms_per_step']:.4f} ms/step")
+    g = GpuSolver(sc, **best)
+    g.step(50)
+    o = Sim(sc, nthreads=0)
+    o.step(50)
+    n = sc.n_fluid
+    sg, so = g.state(), o.state()
+    assert float(np.max(np.abs(sg["rho"][:n] - so["rho"][:n]) / so["rho"][:n])) < 1e-6
+
+
+@pytest.mark.parametrize("cf,blk", [(1.3, 128), (1.15, 512)])
+def test_cell_factor_and_block_match_oracle(cf, blk):
+    from oracle.oracle import Sim
+    from uph_amd.solver import GpuSolver
+    sc = scenes.dam_break(jitter=0.1, seed=3)
+    g = GpuSolver(sc, cell_factor=cf, pair_block=blk)
+    g.step(200)
+    o = Sim(sc, nthreads=0)
+    o.step(200)
+    n = sc.n_fluid
+    sg, so = g.state(), o.state()
+    assert float(np.max(np.abs(sg["rho"][:n] - so["rho"][:n]) / so["rho"][:n])) < 1e-6
+
+
+def test_c_host_driver(tmp_path):
+    """uph_run (C, links only the C ABI) on the reference hydrostatic deck."""
+    from uph_amd import solver
+    from uph_amd.scenes import write_deck
+    drv = os.path.join(ROOT, "uph_amd", "uph_run")
+    assert os.path.exists(drv), "uph_run not built"
+    z = np.load(os.path.join(GOLDEN, "hydrostatic_init.npz"), allow_pickle=False)
+    vals = z["param_values"]
+    sc = load_golden_scene("hydrostatic_init.npz", alpha_av=0.1)
+    with open(tmp_path / "parameters.dat", "w") as f:
+        f.write("\n".join(repr(float(v)) if not float(v).is_integer() else str(int(v))
+                          for v in vals) + "\n")
+    write_deck(str(tmp_path / "FluidParticle.dat"), sc, "fluid")
+    write_deck(str(tmp_path / "FixedBoundaryParticle.dat"), sc, "fixed")
+    write_deck(str(tmp_path / "MovingBoundaryParticle.dat"), sc, "moving")
+    (tmp_path / "SolidParticle.dat").write_text("0\n")
+    _say("uph_run 100 steps")
+    out = subprocess.run([drv, "100", "--alpha", "0.1"], cwd=tmp_path, capture_output=True,
+                         text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    assert "particle-updates/s" in out.stdout
+    rows = np.loadtxt(tmp_path / "FluidParticleFinal.dat", skiprows=1)
+    # the same run through the Python mirror, on the same decks and the driver's cell box
+    from uph_amd.scenes import load_deck_dir
+    sc2 = load_deck_dir(str(tmp_path), alpha_av=0.1)
+    lo, hi = sc2.pos.min(0), sc2.pos.max(0)
+    mg = 10 * sc2.h
+    sc2.box = (lo[0] - mg, lo[1] - mg, hi[0] + mg, hi[1] + 2 * (hi[1] - lo[1]))
+    g = solver.GpuSolver(sc2)
+    g.step(100)
+    st = g.state()
+    n = sc2.n_fluid
+    ref = np.loadtxt(tmp_path / "FluidParticle.dat", skiprows=1)
+    assert not np.allclose(rows[:, 0:2], ref[:, 0:2], rtol=0, atol=1e-12)   # it moved
+    # the driver prints with 10 decimals
+    assert np.allclose(rows[:, 0:2], st["x"][:n], rtol=0, atol=6e-11)
+    assert np.allclose(rows[:, 4], st["rho"][:n], rtol=0, atol=6e-11)

@@ -568,8 +568,9 @@ __global__ __launch_bounds__(BLK) void k_pair(int n, StepBufs b, const DevScene*
 // next pair's loads issued before the current pair's math.  A row range longer
 // than 64 slots, or more than K candidates in range, raises the step's overflow
 // flag and the exact direct kernel recomputes the step.
-__global__ __launch_bounds__(BLK) void k_pair_compact(int n, StepBufs b, const DevScene* __restrict__ sc) {
-  const int p = xcd_block() * BLK + threadIdx.x;
+template <int PB>
+__global__ __launch_bounds__(PB) void k_pair_compact(int n, StepBufs b, const DevScene* __restrict__ sc) {
+  const int p = xcd_block() * PB + threadIdx.x;
   const bool live = p < n;
   const int pp = live ? p : n - 1;
   const uint32_t tg = b.tagA[pp];
@@ -795,10 +796,11 @@ struct uph_handle {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
-static void fill_scene(DevScene& S, const uph_scene& u) {
+static void fill_scene(DevScene& S, const uph_scene& u, double cell_factor) {
   memset(&S, 0, sizeof S);
   S.cutoff = u.h * u.h * 4;
-  const double cs = sqrt(S.cutoff) * (1.0 + 1e-9);
+  // a cell is at least the search radius, so the 3x3 stencil holds every pair
+  const double cs = sqrt(S.cutoff) * (1.0 + 1e-9) * (cell_factor > 1.0 ? cell_factor : 1.0);
   S.cell_inv = 1.0 / cs;
   S.xmin = u.box[0];
   S.ymin = u.box[1];
@@ -876,10 +878,18 @@ static int launch_step(uph_handle* h, hipEvent_t pa, hipEvent_t pb) {
   hipStream_t st = h->stream;
   if (launch_sort(st, h->n, h->ncells, h->cs.nb, h->b, h->dsc, 0, true, h->err)) return -1;
   if (pa) HIPCHK(hipEventRecord(pa, st), h->err);
-  if (h->opts.pair_variant == 1)
+  if (h->opts.pair_variant == 1) {
     hipLaunchKernelGGL(k_pair<false>, dim3(grid8_of(h->n)), dim3(BLK), 0, st, h->n, h->b, h->dsc);
-  else
-    hipLaunchKernelGGL(k_pair_compact, dim3(grid8_of(h->n)), dim3(BLK), 0, st, h->n, h->b, h->dsc);
+  } else {
+    const int pbk = h->opts.pair_block;
+    const int g = (((h->n + pbk - 1) / pbk + 7) / 8) * 8;
+    if (pbk == 128)
+      hipLaunchKernelGGL(k_pair_compact<128>, dim3(g), dim3(128), 0, st, h->n, h->b, h->dsc);
+    else if (pbk == 512)
+      hipLaunchKernelGGL(k_pair_compact<512>, dim3(g), dim3(512), 0, st, h->n, h->b, h->dsc);
+    else
+      hipLaunchKernelGGL(k_pair_compact<256>, dim3(g), dim3(256), 0, st, h->n, h->b, h->dsc);
+  }
   if (pb) HIPCHK(hipEventRecord(pb, st), h->err);
   const int gfix = std::min(grid8_of(h->n), FIX_GRID);
   hipLaunchKernelGGL(k_pair<true>, dim3(gfix), dim3(BLK), 0, st, h->n, h->b, h->dsc);
@@ -897,6 +907,8 @@ extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts*
   }
   uph_handle* h = new uph_handle();
   if (opts) h->opts = *opts;
+  if (h->opts.pair_block != 128 && h->opts.pair_block != 512) h->opts.pair_block = 256;
+  if (!(h->opts.cell_factor >= 1.0) || h->opts.cell_factor > 4.0) h->opts.cell_factor = 1.0;
   h->nscenes = n_scenes;
   h->hs.resize(n_scenes);
   long n = 0, nc = 0;
@@ -915,7 +927,7 @@ extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts*
         delete h;
         return -1;
       }
-    fill_scene(h->hs[s], u);
+    fill_scene(h->hs[s], u, h->opts.cell_factor);
     DevScene& S = h->hs[s];
     S.poff = (int)n;
     S.coff = (int)nc;

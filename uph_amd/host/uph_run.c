@@ -167,7 +167,7 @@ int main(int argc, char **argv) {
   const double margin = 10 * s.h + (moving == 1 ? stroke : 0.0);
   s.box[0] = lo[0] - margin; s.box[1] = lo[1] - margin;
   s.box[2] = hi[0] + margin; s.box[3] = hi[1] + 2 * (hi[1] - lo[1]);
-  uph_opts opts = {use_graph, 0};
+  uph_opts opts = {use_graph, 0, 0, 256, 1.0};
   uph_t *h = NULL;
   if (uph_create(&s, 1, &opts, &h) != 0) {
     fprintf(stderr, "uph_run: %s\n", uph_last_error(NULL));

@@ -34,7 +34,8 @@ class UphScene(C.Structure):
 
 
 class UphOpts(C.Structure):
-    _fields_ = [("use_graph", C.c_int), ("keep_rates", C.c_int), ("pair_variant", C.c_int)]
+    _fields_ = [("use_graph", C.c_int), ("keep_rates", C.c_int), ("pair_variant", C.c_int),
+                ("pair_block", C.c_int), ("cell_factor", C.c_double)]
 
 
 # every entry point of include/uph.h: name -> (restype, argtypes)
@@ -114,7 +115,8 @@ class GpuSolver:
     """Device-resident leapfrog over a batch of independent scenes."""
 
     def __init__(self, scenes: Sequence[Scene] | Scene, use_graph: bool = False,
-                 keep_rates: bool = False, pair_variant: int = 0):
+                 keep_rates: bool = False, pair_variant: int = 0, pair_block: int = 256,
+                 cell_factor: float = 1.0):
         if isinstance(scenes, Scene):
             scenes = [scenes]
         self.scenes = list(scenes)
@@ -140,7 +142,10 @@ class GpuSolver:
                                                        sc.m_amp, sc.m_omega)
             for q in range(4):
                 u.box[q] = float(sc.box[q])
-        opts = UphOpts(int(bool(use_graph)), int(bool(keep_rates)), int(pair_variant))
+        opts = UphOpts(int(bool(use_graph)), int(bool(keep_rates)), int(pair_variant),
+                       int(pair_block), float(cell_factor))
+        self.opts = dict(use_graph=bool(use_graph), pair_variant=int(pair_variant),
+                         pair_block=int(pair_block), cell_factor=float(cell_factor))
         h = C.c_void_p()
         rc = lib().uph_create(arr, len(self.scenes), C.byref(opts), C.byref(h))
         if rc != 0:
