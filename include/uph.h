@@ -75,7 +75,7 @@ typedef struct uph_scene {
   int n_fluid, n_fixed, n_moving;
   const double *pos;    /* 2n */
   const double *vel;    /* 2n */
-  const double *rho;    /* n  : density (boundary rows must equal rho0)   */
+  const double *rho;    /* n  : density (walls keep their deck density)   */
   const double *mass;   /* n  */
   const int *itype;     /* n  : 1 fluid, 2 fixed, 3 moving, 5 spring wall   */
   const double *lvel;   /* 2n : localVelocity deck columns                  */
@@ -117,8 +117,8 @@ int uph_sync(uph_t *h);
 int uph_get_state(uph_t *h, int scene, double *pos, double *vel, double *rho);
 
 /* Rates of the last step (needs opts.keep_rates): R = ddensitydt, Rt = ddensitydt_approx,
- * a = dvdt (= dvdt_approx in this scheme), fluid rows only (n_fluid, n_fluid, 2 n_fluid). */
-int uph_get_rates(uph_t *h, int scene, float *R, float *Rt, float *a);
+ * a = dvdt, at = dvdt_approx; fluid rows only (n_fluid, n_fluid, 2 n_fluid, 2 n_fluid). */
+int uph_get_rates(uph_t *h, int scene, float *R, float *Rt, float *a, float *at);
 
 /* Completed steps. */
 int uph_itime(const uph_t *h);

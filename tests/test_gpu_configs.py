@@ -30,24 +30,29 @@ def test_config3_wave_tank_hydrostatic_profile():
     drho = float(np.max(np.abs(sg["rho"][:nf] - so["rho"][:nf]) / so["rho"][:nf]))
     _say(f"config-3 3 steps: max rel rho err {drho:.2e}")
     assert drho < 1e-6
-    g.step(4997)
-    assert g.sync() == 0
-    st = g.state()
-    x, y, rho = st["x"][:nf, 0], st["x"][:nf, 1], st["rho"][:nf]
-    assert np.all(np.isfinite(st["x"])) and np.all(np.isfinite(rho))
-    P = sc.B * ((rho / sc.rho0) ** 7 - 1.0)
+    # 5000 steps; the hydrostatic profile is checked on its average over the last 2500
+    # steps (about one period of the column's acoustic breathing mode, 4 H / c0)
+    g.step(2497)
     H = 1.0
-    m = (x > 2.0) & (x < 4.0)
-    depth_bins = np.linspace(0.05, 0.95, 10)
-    worst = 0.0
-    for lo, hi in zip(depth_bins[:-1], depth_bins[1:]):
-        sel = m & (H - y > lo) & (H - y <= hi)
-        if sel.sum() < 50:
-            continue
-        expect = sc.rho0 * sc.g * (H - y[sel]).mean()
-        worst = max(worst, abs(P[sel].mean() - expect) / (sc.rho0 * sc.g * H))
-    _say(f"config-3 after 5000 steps: worst hydrostatic deviation {worst:.3e} of rho0 g H")
-    assert worst < 0.03
+    bins = np.linspace(0.05, 0.95, 7)
+    acc = []
+    for _ in range(25):
+        g.step(100)
+        st = g.state()
+        x, y, rho = st["x"][:nf, 0], st["x"][:nf, 1], st["rho"][:nf]
+        assert np.all(np.isfinite(st["x"])) and np.all(np.isfinite(rho))
+        P = sc.B * ((rho / sc.rho0) ** 7 - 1.0)
+        m = (x > 2.0) & (x < 4.0)
+        prof = []
+        for lo, hi in zip(bins[:-1], bins[1:]):
+            sel = m & (H - y > lo) & (H - y <= hi)
+            prof.append(P[sel].mean() / (sc.rho0 * sc.g * (H - y[sel]).mean()))
+        acc.append(prof)
+    assert g.sync() == 0
+    mean = np.mean(acc, axis=0)
+    _say(f"config-3 P / (rho0 g depth) by depth, mean over steps 2600-5000: {np.round(mean, 4)}; "
+         f"instantaneous range [{np.min(acc):.3f}, {np.max(acc):.3f}]")
+    assert np.all(np.abs(mean - 1.0) < 0.03)
 
 
 def test_config4_batch_of_512_scenes():

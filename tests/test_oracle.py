@@ -332,6 +332,7 @@ def test_leapfrog_free_fall_exact():
 def test_gather_form_agrees_with_scatter(oracle_lib):
     """Per-particle gather sums over an independently found pair set equal the oracle."""
     sc = scenes.dam_break(width=0.3, height=0.3, dx=0.02, tank=(0.6, 0.5), jitter=0.1, seed=2)
+    scenes.hydrostatic_walls(sc, 0.3)         # walls with density != rho0 = localDensity
     rng = np.random.default_rng(9)
     sc.vel[:sc.n_fluid] = rng.normal(0, 0.3, (sc.n_fluid, 2))
     sc.lvel[:sc.n_fluid] = sc.vel[:sc.n_fluid] + rng.normal(0, 0.05, (sc.n_fluid, 2))
@@ -340,11 +341,12 @@ def test_gather_form_agrees_with_scatter(oracle_lib):
     s.derivatives(sc.dt)
     st, r = s.state(), s.rates()
     x, v, vl, rho, m = st["x"], st["v"], st["vloc"], st["rho"], sc.mass
+    lam = st["rholoc"] / rho                  # 1 for fluid, rho0/rho for walls
     ip, jp, r2 = np_search(x, sc.n_fluid, sc.n, sc.particle_boundary, 40, sc.cutoff2)
     h = sc.h
     alpha = 10.0 / (7 * math.pi * h * h)
     P = sc.B * ((rho / sc.rho0) ** 7 - 1)
-    R = np.zeros(sc.n); Rt = np.zeros(sc.n); A = np.zeros((sc.n, 2))
+    R = np.zeros(sc.n); Rt = np.zeros(sc.n); A = np.zeros((sc.n, 2)); At = np.zeros((sc.n, 2))
     for i, j, d2 in zip(ip, jp, r2):
         for p, q in ((i, j), (j, i)):
             dxp = x[p] - x[q]
@@ -352,25 +354,30 @@ def test_gather_form_agrees_with_scatter(oracle_lib):
             qq = rr / h
             dwdq = alpha * (-3 * qq + 2.25 * qq * qq) if qq < 1 else -0.75 * alpha * (2 - qq) ** 2
             g = dwdq / h * dxp / rr                      # grad_p W_pq from dW/dq
-            R[p] += m[q] * np.dot(v[p] - vl[q], g)       # continuity, gather form
+            R[p] += m[q] * lam[q] * np.dot(v[p] - vl[q], g)   # continuity, gather form
             Rt[p] += (m[q] * np.dot(v[p] - vl[q], g) if p == i
                       else m[p] * np.dot(vl[p] - v[q], g))
             vr = np.dot(v[p] - v[q], dxp)
             Pi = 0.0
             if vr < 0:
                 Pi = -sc.alpha_av * sc.c0 * (h * vr / (d2 + 0.01 * h * h)) / (0.5 * (rho[p] + rho[q]))
-            A[p] -= m[q] * (P[p] / rho[p] ** 2 + P[q] / rho[q] ** 2 + Pi) * g
+            A[p] -= m[q] * (P[p] / rho[p] ** 2 * lam[q] + P[q] / rho[q] ** 2 * lam[p] + Pi) * g
+            At[p] -= m[q] * (P[p] / rho[p] ** 2 + P[q] / rho[q] ** 2 + Pi) * g
             if p == i and sc.itype[q] in (2, 3, 5) and rr < 3 * sc.dx:
-                A[p] -= sc.k_spring * (3 * sc.dx - rr) * dxp / (rr * m[p])
-                A[p] -= sc.c_damp * np.dot(dxp, v[p] - v[q]) * dxp / (d2 * m[p])
+                for acc in (A, At):
+                    acc[p] -= sc.k_spring * (3 * sc.dx - rr) * dxp / (rr * m[p])
+                    acc[p] -= sc.c_damp * np.dot(dxp, v[p] - v[q]) * dxp / (d2 * m[p])
     A[:, 1] -= sc.g
+    At[:, 1] -= sc.g
+    assert np.abs(lam[sc.n_fluid:] - 1).max() > 1e-4      # the wall factor is exercised
     nf = sc.n_fluid
     sR = np.abs(R[:nf]).max()
     assert np.abs(r["R"][:nf] - R[:nf]).max() < 1e-11 * sR
     assert np.abs(r["Rt"][:nf] - Rt[:nf]).max() < 1e-11 * sR
     sA = np.abs(A[:nf]).max()
     assert np.abs(r["a"][:nf] - A[:nf]).max() < 1e-11 * sA
-    assert np.abs(r["at"][:nf] - A[:nf]).max() < 1e-11 * sA
+    assert np.abs(r["at"][:nf] - At[:nf]).max() < 1e-11 * sA
+    assert np.abs(A[:nf] - At[:nf]).max() > 1e-6 * sA       # and the two forms differ
 
 
 def test_multistep_dam_break_stays_physical(oracle_lib):
@@ -384,3 +391,22 @@ def test_multistep_dam_break_stays_physical(oracle_lib):
     assert st["rho"][:nf].min() > 950 and st["rho"][:nf].max() < 1100
     assert st["x"][:nf, 1].min() > -sc.dx       # no particle through the floor
     assert np.abs(st["v"][:nf]).max() < 2 * math.sqrt(2 * 9.81 * 1.0)
+
+
+def test_hydrostatic_walls_support_the_bottom_row(oracle_lib):
+    """Walls are frozen to their deck state each step: at rho0 they carry no pressure and the
+    bottom fluid row is pulled into the floor; at the hydrostatic density it is supported."""
+    from oracle.oracle import Sim
+    res = {}
+    for hyd in (False, True):
+        sc = scenes.wave_tank(dx=0.02, amp=0.0)
+        if not hyd:
+            sc.rho[sc.n_fluid:] = sc.rho0
+        s = Sim(sc)
+        s.derivatives(sc.dt)
+        a = s.rates()["a"][:sc.n_fluid]
+        x = sc.pos[:sc.n_fluid]
+        bottom = (x[:, 0] > 1) & (x[:, 0] < 4) & (x[:, 1] < 0.015)
+        res[hyd] = a[bottom, 1].mean()
+    assert res[False] < -10 * 9.81
+    assert abs(res[True]) < 0.2 * 9.81

@@ -386,7 +386,8 @@ struct PairCtx {
   bool pb_ok;
   float vpx, vpy, lpx, lpy, rhop, prp, mp, inv_mp;
   float h, invh, walpha, h2eps;
-  CSum sR, sRt, sAx, sAy;
+  CSum sR, sRt, sAx, sAy;   // sAx/sAy: the predicting acceleration (dvdt_approx)
+  float dAx = 0.f, dAy = 0.f;   // dvdt - dvdt_approx: wall pairs with localDensity != density
 
   __device__ __forceinline__ void init(const DevScene& S, uint32_t tg, float4 ap, float4 bp) {
     idp = tag_id(tg);
@@ -438,8 +439,12 @@ struct PairCtx {
     }
     const float gx = f * dx, gy = f * dy;
     const float mq = bq.z;
+    const uint32_t tyq = tag_type(tq);
+    // localDensity/density of q: 1 for fluid (the leapfrog keeps them equal), rho0/rho
+    // for walls (inputbp.c:288 resets their localDensity to rho0)
+    const float lam = tyq == 1 ? 1.0f : S.frho0 * __builtin_amdgcn_rcpf(bq.x);
     // continuity (derivatives.c:1520-1567), symmetric in the pair order
-    sR.add(mq * ((vpx - aq.z) * gx + (vpy - aq.w) * gy));
+    sR.add(mq * lam * ((vpx - aq.z) * gx + (vpy - aq.w) * gy));
     // predicting continuity (derivatives.c:1217-1252), depends on the pair order
     if (p_is_i)
       sRt.add(mq * ((vpx - aq.z) * gx + (vpy - aq.w) * gy));
@@ -455,9 +460,15 @@ struct PairCtx {
     }
     float ax = -mq * pij * gx;
     float ay = -mq * pij * gy;
+    // PressureGradientForce weighs P_p by lam_q (derivatives.c:1126-1128); the
+    // predicting form (derivatives.c:1354-1356) does not
+    if (tyq != 1) {
+      const float w = -mq * prp * (lam - 1.0f);
+      dAx += w * gx;
+      dAy += w * gy;
+    }
     // wall spring (derivatives.c:2396-2449).  The damping part does not vanish at
     // r0, so the range test is the reference's FP64 sqrt(d2) < r0.
-    const uint32_t tyq = tag_type(tq);
     if (p_is_i && (tyq == 2 || tyq == 3 || tyq == 5) &&
         sqrt(__dadd_rn(__dmul_rn(ex, ex), __dmul_rn(ey, ey))) < S.r0) {
       const float kr = S.fk * (S.fr0 - r) * rinv * inv_mp;
@@ -487,8 +498,9 @@ __device__ __forceinline__ void correct(const DevScene& S, const StepBufs& b, in
                                         double2 vn, float mass, const PairCtx& c) {
   const double dt = S.dt;
   const float R = c.sR.val(), Rt = c.sRt.val();
-  double ax = (double)c.sAx.val(), ay = (double)c.sAy.val();
-  if (S.gravity == 1) ay -= S.g;     // BodyForce (derivatives.c:1167-1178)
+  double atx = (double)c.sAx.val(), aty = (double)c.sAy.val();
+  if (S.gravity == 1) aty -= S.g;    // BodyForce / PredictingBodyForce (derivatives.c:1167-1178, 1379-1390)
+  const double ax = atx + (double)c.dAx, ay = aty + (double)c.dAy;
   const double rho_n = b.rhoA[p];
   double rho1, v1x, v1y;
   if (itime == 1) {
@@ -503,8 +515,11 @@ __device__ __forceinline__ void correct(const DevScene& S, const StepBufs& b, in
   b.posB[p] = make_double2(xp.x + dt * v1x, xp.y + dt * v1y);
   b.velB[p] = make_double2(v1x, v1y);
   b.rhoB[p] = rho1;
-  b.rtB[p] = make_float4(Rt, (float)ax, (float)ay, mass);
-  if (b.dbgRates) b.dbgRates[p] = make_float4(R, Rt, (float)ax, (float)ay);
+  b.rtB[p] = make_float4(Rt, (float)atx, (float)aty, mass);
+  if (b.dbgRates) {
+    b.dbgRates[2 * p] = make_float4(R, Rt, (float)ax, (float)ay);
+    b.dbgRates[2 * p + 1] = make_float4((float)atx, (float)aty, 0.f, 0.f);
+  }
 }
 
 // Direct form: one pass over the candidates, physics inside the distance test.
@@ -830,6 +845,7 @@ static void fill_scene(DevScene& S, const uph_scene& u, double cell_factor) {
   S.fk = (float)u.k_spring;
   S.fcd = (float)u.c_damp;
   S.fr0 = (float)(3.0 * u.dx);
+  S.frho0 = (float)u.rho0;
 }
 
 static int alloc_bufs(StepBufs& b, int n, int ncells, int nblocks_scan, Err& err, bool search_only) {
@@ -922,8 +938,8 @@ extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts*
       return -1;
     }
     for (int i = u.n_fluid; i < np; ++i)
-      if (u.rho[i] != u.rho0) {
-        g_create_err = "uph_create: boundary density must equal rho0 (localDensity = rho0, inputbp.c:288)";
+      if (!(u.rho[i] > 0)) {
+        g_create_err = "uph_create: wall densities must be positive";
         delete h;
         return -1;
       }
@@ -960,7 +976,7 @@ extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts*
     return fail();
   }
   if (alloc_bufs(h->b, h->n, h->ncells, h->cs.nb, err, false)) return fail();
-  if (h->opts.keep_rates && dalloc(&h->b.dbgRates, h->n, err)) return fail();
+  if (h->opts.keep_rates && dalloc(&h->b.dbgRates, 2 * (size_t)h->n, err)) return fail();
   if (dalloc(&h->dsc, n_scenes, err)) return fail();
   // host staging in original order (global index = slot at start)
   std::vector<double2> pos(n), vel(n), lv(n);
@@ -1147,7 +1163,7 @@ extern "C" int uph_get_state(uph_t* h, int scene, double* pos, double* vel, doub
   return 0;
 }
 
-extern "C" int uph_get_rates(uph_t* h, int scene, float* R, float* Rt, float* a) {
+extern "C" int uph_get_rates(uph_t* h, int scene, float* R, float* Rt, float* a, float* at) {
   if (!h) return -1;
   if (!h->b.dbgRates) {
     h->err.msg = "uph_get_rates: handle created without keep_rates";
@@ -1159,17 +1175,19 @@ extern "C" int uph_get_rates(uph_t* h, int scene, float* R, float* Rt, float* a)
   }
   HIPCHK(hipStreamSynchronize(h->stream), h->err);
   const DevScene& S = h->hs[scene];
-  std::vector<float4> d(S.np);
+  std::vector<float4> d(2 * (size_t)S.np);
   std::vector<uint32_t> t(S.np);
-  HIPCHK(hipMemcpy(d.data(), h->b.dbgRates + S.poff, S.np * sizeof(float4), hipMemcpyDeviceToHost), h->err);
+  HIPCHK(hipMemcpy(d.data(), h->b.dbgRates + 2 * (size_t)S.poff, 2 * S.np * sizeof(float4), hipMemcpyDeviceToHost),
+         h->err);
   HIPCHK(hipMemcpy(t.data(), h->b.tagA + S.poff, S.np * sizeof(uint32_t), hipMemcpyDeviceToHost), h->err);
   for (int k = 0; k < S.np; ++k) {
     const int i = (int)((t[k] & UPH_ID_MASK) - (uint32_t)S.poff);
     if (i < 0 || i >= S.np) continue;
     if (i >= S.nf) continue;
-    if (R) R[i] = d[k].x;
-    if (Rt) Rt[i] = d[k].y;
-    if (a) { a[2 * i] = d[k].z; a[2 * i + 1] = d[k].w; }
+    if (R) R[i] = d[2 * k].x;
+    if (Rt) Rt[i] = d[2 * k].y;
+    if (a) { a[2 * i] = d[2 * k].z; a[2 * i + 1] = d[2 * k].w; }
+    if (at) { at[2 * i] = d[2 * k + 1].x; at[2 * i + 1] = d[2 * k + 1].y; }
   }
   return 0;
 }

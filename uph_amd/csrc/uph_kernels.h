@@ -29,7 +29,7 @@ struct DevScene {
   double h, dt, rho0, gamma, B, c0, g, r0;
   double m_amp, m_omega;
   // FP32 copies for the pair sums
-  float fh, finvh, fwalpha, fav, fc0, fk, fcd, fr0;
+  float fh, finvh, fwalpha, fav, fc0, fk, fcd, fr0, frho0;
 };
 
 __device__ __forceinline__ uint32_t tag_id(uint32_t t) { return t & UPH_ID_MASK; }

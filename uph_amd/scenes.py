@@ -205,7 +205,7 @@ def wave_tank(length=6.0, depth=1.0, dx=0.005, beach_toe=5.0, beach_slope=0.1,
     ny = int(math.floor(depth / dx + 1e-9))
     fl = _lattice(nx, ny, dx)
     bed = np.where(fl[:, 0] > beach_toe, (fl[:, 0] - beach_toe) * beach_slope, 0.0)
-    fl = fl[fl[:, 1] > bed + 0.5 * dx]
+    fl = fl[fl[:, 1] >= bed + 0.5 * dx - 1e-9 * dx]     # lattice rows start at y = dx/2
     H = ny * dx
     c0, B = tait_constants(rho0, gamma, g, H)
     rho, p = hydrostatic_density(fl[:, 1], H, rho0, gamma, g, B)
@@ -248,7 +248,23 @@ def wave_tank(length=6.0, depth=1.0, dx=0.005, beach_toe=5.0, beach_slope=0.1,
                    m_omega=2 * math.pi / period,
                    box=(-margin - amp, -margin, length + margin, wall_height + margin),
                    name=name or f"wave_tank_{length}x{depth}_dx{dx:g}")
+    hydrostatic_walls(sc, H)
     return sc
+
+
+def hydrostatic_walls(sc: Scene, H: float) -> None:
+    """Give wall particles below the free surface y = H the hydrostatic density.
+
+    Walls are reset to their deck state every step (inputbp.c:117-150), so a wall
+    at rho0 carries zero pressure and supports the fluid next to it with only
+    the fluid's own pressure; at the hydrostatic density it carries the pressure
+    of the water column at its depth.
+    """
+    w = slice(sc.n_fluid, sc.n)
+    y = sc.pos[w, 1]
+    rho_h, p_h = hydrostatic_density(np.minimum(y, H), H, sc.rho0, sc.gamma, sc.g, sc.B)
+    sc.rho[w] = np.where(y < H, rho_h, sc.rho0)
+    sc.pressure[w] = np.where(y < H, p_h, 0.0)
 
 
 def batch_dam_breaks(n_scenes=512, seed=20260512, width=(0.3, 1.5), height=(0.5, 2.0),

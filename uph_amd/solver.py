@@ -52,7 +52,8 @@ SIGNATURES = {
     "uph_sync": (C.c_int, [C.c_void_p]),
     "uph_get_state": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _dp]),
     "uph_get_rates": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_float),
-                                C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+                                C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                C.POINTER(C.c_float)]),
     "uph_itime": (C.c_int, [C.c_void_p]),
     "uph_time_steps": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp]),
     "uph_last_error": (C.c_char_p, [C.c_void_p]),
@@ -191,9 +192,10 @@ class GpuSolver:
         nf = self._scene(scene).n_fluid
         R = np.zeros(nf, np.float32); Rt = np.zeros(nf, np.float32)
         a = np.zeros((nf, 2), np.float32)
+        at = np.zeros((nf, 2), np.float32)
         self._check(lib().uph_get_rates(self.h, scene, _ptr(R, C.c_float), _ptr(Rt, C.c_float),
-                                        _ptr(a, C.c_float)))
-        return dict(R=R, Rt=Rt, a=a)
+                                        _ptr(a, C.c_float), _ptr(at, C.c_float)))
+        return dict(R=R, Rt=Rt, a=a, at=at)
 
     def time_steps(self, n: int):
         """(ms for the whole loop, mean ms of the fused pair kernel), HIP events."""
