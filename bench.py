@@ -126,7 +126,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic_from_profiles(),
-                     "kernel": "k_pair<false>", "kernel_ms": ms_pair},
+                     "kernel": "k_pair_compact<256>", "kernel_ms": ms_pair},
     }
     if args.graph:
         gg = GpuSolver(sc, use_graph=True)
