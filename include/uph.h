@@ -31,6 +31,8 @@
 #ifndef UPH_H_
 #define UPH_H_
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -128,6 +130,45 @@ int uph_itime(const uph_t *h);
 int uph_time_steps(uph_t *h, int n_steps, double *ms_total, double *ms_pair);
 
 const char *uph_last_error(const uph_t *h);
+
+/* ------------------------------------------ surrogate (graph network) kernels
+ *
+ * No counterpart in the reference: the learned acceleration surrogate that runs
+ * beside the plain solver.  Device buffers are raw pointers from uph_dev_alloc;
+ * `stream` is a hipStream_t (NULL: the default stream).  Matrices are row-major;
+ * BF16 arrays hold __bf16, and N, K (unless trans_a) and every leading
+ * dimension must be multiples of 8.
+ */
+int uph_dev_alloc(void **p, size_t bytes);
+int uph_dev_free(void *p);
+int uph_dev_upload(void *dst, const void *src, size_t bytes);
+int uph_dev_download(void *dst, const void *src, size_t bytes);
+int uph_dev_copy(void *dst, const void *src, size_t bytes);
+int uph_dev_zero(void *p, size_t bytes);
+int uph_dev_sync(void);
+const char *uph_gnn_last_error(void);
+
+/* C = act(op(A) B + bias) with BF16 MFMA and FP32 accumulation.  op(A) is A
+ * (M x K) or, when trans_a, the transpose of the K x M matrix A.  splits > 1
+ * divides K among work-groups (FP32 partials in `work`, M*N*splits floats);
+ * accumulate adds into an FP32 C (needs `work`, M*N floats).  act: 0 none, 1 ReLU. */
+int uph_gemm_bf16(int M, int N, int K, const void *A, int lda, int trans_a, const void *B, int ldb,
+                  const float *bias, void *C, int ldc, int out_bf16, int act, int splits, float *work,
+                  int accumulate, void *stream);
+int uph_cast_f32_bf16(long n, const float *x, void *y, void *stream);
+int uph_cast_bf16_f32(long n, const void *x, float *y, void *stream);
+int uph_relu_bwd(long n, const float *dy, const void *y, void *dz, void *stream);
+int uph_add_f32(long n, const float *x, float *y, void *stream);
+int uph_colsum_f32(int M, int N, const float *x, int ldx, float *out, int accumulate, void *stream);
+/* out[i] = sum_{e in [off[i], off[i+1])} x[idx ? idx[e] : e] (rows of F values) */
+int uph_segsum(int nseg, int F, const int *off, const int *idx, const void *x, int ldx, int x_bf16,
+               float *out, int ldo, int accumulate, void *stream);
+int uph_gather_rows(int n, int F, const int *rows, const void *x, int ldx, void *y, int ldy, int esize,
+                    void *stream);
+int uph_adam(long n, float *w, const float *g, float *m, float *v, void *w_bf16, float lr, float b1,
+             float b2, float eps, int t, void *stream);
+int uph_mse(int M, int C, const float *y, int ldy, const float *t, int ldt, const int *mask, int count,
+            float *dy, float *loss_part, void *stream);
 
 #ifdef __cplusplus
 }

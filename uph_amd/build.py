@@ -6,7 +6,7 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = os.path.join(HERE, "csrc", "uph_gpu.hip")
+SRCS = [os.path.join(HERE, "csrc", "uph_gpu.hip"), os.path.join(HERE, "csrc", "uph_gnn.hip")]
 OUT = os.path.join(HERE, "libuph_gpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -23,7 +23,7 @@ DRIVER = os.path.join(HERE, "uph_run")
 
 def build(verbose: bool = False) -> str:
     tmp = OUT + ".tmp"
-    cmd = [HIPCC, *FLAGS, "-I", os.path.join(HERE, "..", "include"), "-o", tmp, SRC]
+    cmd = [HIPCC, *FLAGS, "-I", os.path.join(HERE, "..", "include"), "-o", tmp, *SRCS]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

@@ -1,0 +1,425 @@
+// uph_gnn.hip -- kernels of the learned acceleration surrogate (graph network).
+//
+// The network's dense layers run as hand-written BF16 MFMA GEMMs
+// (v_mfma_f32_16x16x32_bf16, FP32 accumulation) with fused bias/activation;
+// weight gradients use the same GEMM over a transposed operand with a split-K
+// reduction so every output tile gets many work-groups.  The graph is the SPH
+// neighbour graph of the solver's cell list (radius = the search radius), kept
+// in CSR order by receiver so every aggregation is a deterministic segment
+// sum; the backward scatter to senders goes through a sender-ordered
+// permutation instead of atomics.
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+#include <stdint.h>
+#include <string.h>
+#include <stdio.h>
+#include <string>
+#include <mutex>
+
+#include "../../include/uph.h"
+
+typedef __bf16 bf16_t;
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+namespace {
+std::string g_gnn_err;
+std::mutex g_gnn_mu;
+
+int fail(const char* what, hipError_t e) {
+  char b[256];
+  snprintf(b, sizeof b, "%s: %s", what, hipGetErrorString(e));
+  std::lock_guard<std::mutex> lk(g_gnn_mu);
+  g_gnn_err = b;
+  return -1;
+}
+int fail(const char* what) {
+  std::lock_guard<std::mutex> lk(g_gnn_mu);
+  g_gnn_err = what;
+  return -1;
+}
+}  // namespace
+
+#define CHK(expr)                          \
+  do {                                     \
+    hipError_t e_ = (expr);                \
+    if (e_ != hipSuccess) return fail(#expr, e_); \
+  } while (0)
+
+// ------------------------------------------------------------ buffers
+
+extern "C" int uph_dev_alloc(void** p, size_t bytes) {
+  *p = nullptr;
+  CHK(hipMalloc(p, bytes ? bytes : 16));
+  return 0;
+}
+extern "C" int uph_dev_free(void* p) {
+  if (p) CHK(hipFree(p));
+  return 0;
+}
+extern "C" int uph_dev_upload(void* dst, const void* src, size_t bytes) {
+  CHK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return 0;
+}
+extern "C" int uph_dev_download(void* dst, const void* src, size_t bytes) {
+  CHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return 0;
+}
+extern "C" int uph_dev_copy(void* dst, const void* src, size_t bytes) {
+  CHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+  return 0;
+}
+extern "C" int uph_dev_zero(void* p, size_t bytes) {
+  CHK(hipMemset(p, 0, bytes));
+  return 0;
+}
+extern "C" int uph_dev_sync(void) {
+  CHK(hipDeviceSynchronize());
+  return 0;
+}
+extern "C" const char* uph_gnn_last_error(void) {
+  std::lock_guard<std::mutex> lk(g_gnn_mu);
+  return g_gnn_err.c_str();
+}
+
+// ------------------------------------------------------------ GEMM
+
+#define GBM 64
+#define GBN 64
+#define GBK 32
+#define GPAD 8
+
+__device__ __forceinline__ float act_fwd(float x, int act) { return act == 1 ? fmaxf(x, 0.f) : x; }
+
+// C[M x N] = act(op(A) . B + bias), op(A) = A (M x K, row stride lda) or, with TA,
+// A^T where A is stored K x M (row stride lda).  B is K x N (row stride ldb).
+// With split-K (gridDim.z > 1) each z-slice writes its FP32 partial product to
+// part + z*M*N (no bias/activation); k_gemm_reduce finishes.
+// Requires: N, lda, ldb, ldc multiples of 8; K a multiple of 8 unless TA.
+template <bool TA>
+__global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const bf16_t* __restrict__ A, int lda,
+                                              const bf16_t* __restrict__ B, int ldb, const float* __restrict__ bias,
+                                              void* __restrict__ C, int ldc, int out_bf16, int act, int kchunk,
+                                              float* __restrict__ part) {
+  __shared__ bf16_t As[GBM][GBK + GPAD];
+  __shared__ bf16_t Bs[GBN][GBK + GPAD];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int m0 = blockIdx.y * GBM, n0 = blockIdx.x * GBN;
+  const int kb = blockIdx.z * kchunk;
+  const int ke = min(K, kb + kchunk);
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  v4f acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = (v4f){0.f, 0.f, 0.f, 0.f};
+  const bf16_t zero = (bf16_t)0.0f;
+  for (int k0 = kb; k0 < ke; k0 += GBK) {
+    // stage A (64 x 32) as As[m][k]
+    if (!TA) {
+      const int r = tid >> 2, k8 = (tid & 3) * 8;
+      const int gm = m0 + r, gk = k0 + k8;
+      if (gm < M && gk + 7 < ke) {
+        const v8bf v = *(const v8bf*)(A + (size_t)gm * lda + gk);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) As[r][k8 + e] = v[e];
+      } else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) As[r][k8 + e] = (gm < M && gk + e < ke) ? A[(size_t)gm * lda + gk + e] : zero;
+      }
+    } else {
+      const int kk = tid >> 3, m8 = (tid & 7) * 8;
+      const int gk = k0 + kk, gm = m0 + m8;
+      if (gk < ke && gm + 7 < M) {
+        const v8bf v = *(const v8bf*)(A + (size_t)gk * lda + gm);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) As[m8 + e][kk] = v[e];
+      } else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) As[m8 + e][kk] = (gk < ke && gm + e < M) ? A[(size_t)gk * lda + gm + e] : zero;
+      }
+    }
+    // stage B (32 x 64) transposed as Bs[n][k]
+    {
+      const int kk = tid >> 3, n8 = (tid & 7) * 8;
+      const int gk = k0 + kk, gn = n0 + n8;
+      if (gk < ke && gn + 7 < N) {
+        const v8bf v = *(const v8bf*)(B + (size_t)gk * ldb + gn);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) Bs[n8 + e][kk] = v[e];
+      } else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) Bs[n8 + e][kk] = (gk < ke && gn + e < N) ? B[(size_t)gk * ldb + gn + e] : zero;
+      }
+    }
+    __syncthreads();
+    // one 16x16x32 MFMA per (mi, ni): lane holds row/col (lane % 16), k = (lane / 16) * 8 + 0..7
+    v8bf a[2], b[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      a[i] = *(const v8bf*)&As[wr + i * 16 + (lane & 15)][(lane >> 4) * 8];
+      b[i] = *(const v8bf*)&Bs[wc + i * 16 + (lane & 15)][(lane >> 4) * 8];
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    __syncthreads();
+  }
+  // D layout: lane holds column (lane % 16), rows (lane / 16) * 4 + 0..3
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gm = m0 + wr + i * 16 + (lane >> 4) * 4 + r;
+        const int gn = n0 + wc + j * 16 + (lane & 15);
+        if (gm >= M || gn >= N) continue;
+        float v = acc[i][j][r];
+        if (gridDim.z > 1) {
+          part[(size_t)blockIdx.z * M * N + (size_t)gm * N + gn] = v;
+          continue;
+        }
+        if (bias) v += bias[gn];
+        v = act_fwd(v, act);
+        if (out_bf16)
+          ((bf16_t*)C)[(size_t)gm * ldc + gn] = (bf16_t)v;
+        else
+          ((float*)C)[(size_t)gm * ldc + gn] = v;
+      }
+}
+
+__global__ void k_gemm_reduce(int M, int N, int splits, const float* __restrict__ part, const float* __restrict__ bias,
+                              void* __restrict__ C, int ldc, int out_bf16, int act, int accumulate) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)M * N) return;
+  const int gm = (int)(i / N), gn = (int)(i % N);
+  float v = 0.f;
+  for (int z = 0; z < splits; ++z) v += part[(size_t)z * M * N + i];
+  if (bias) v += bias[gn];
+  v = act_fwd(v, act);
+  if (out_bf16) {
+    ((bf16_t*)C)[(size_t)gm * ldc + gn] = (bf16_t)v;
+  } else {
+    float* c = (float*)C + (size_t)gm * ldc + gn;
+    *c = accumulate ? *c + v : v;
+  }
+}
+
+extern "C" int uph_gemm_bf16(int M, int N, int K, const void* A, int lda, int trans_a, const void* B, int ldb,
+                             const float* bias, void* C, int ldc, int out_bf16, int act, int splits, float* work,
+                             int accumulate, void* stream) {
+  if (M <= 0 || N <= 0 || K <= 0) return 0;
+  if ((N & 7) || (lda & 7) || (ldb & 7) || (ldc & 7) || (!trans_a && (K & 7)))
+    return fail("uph_gemm_bf16: N, K (unless trans_a) and leading dimensions must be multiples of 8");
+  if (splits < 1) splits = 1;
+  if ((splits > 1 || accumulate) && !work) return fail("uph_gemm_bf16: split-K or accumulate needs a workspace");
+  if (accumulate && out_bf16) return fail("uph_gemm_bf16: accumulate needs an FP32 output");
+  if (accumulate && splits == 1) splits = 1;
+  hipStream_t st = (hipStream_t)stream;
+  int kchunk = (K + splits - 1) / splits;
+  kchunk = ((kchunk + GBK - 1) / GBK) * GBK;
+  splits = (K + kchunk - 1) / kchunk;
+  const bool via_work = splits > 1 || accumulate;
+  dim3 grid((N + GBN - 1) / GBN, (M + GBM - 1) / GBM, via_work ? (splits > 1 ? splits : 1) : 1);
+  // a single-slice accumulate still goes through the workspace
+  if (via_work && splits == 1) grid.z = 1;
+  const bool part_path = via_work;
+  if (trans_a)
+    hipLaunchKernelGGL(k_gemm<true>, grid, dim3(256), 0, st, M, N, K, (const bf16_t*)A, lda, (const bf16_t*)B, ldb,
+                       part_path ? nullptr : bias, part_path ? (void*)work : C, part_path ? N : ldc,
+                       part_path ? 0 : out_bf16, part_path ? 0 : act, kchunk, work);
+  else
+    hipLaunchKernelGGL(k_gemm<false>, grid, dim3(256), 0, st, M, N, K, (const bf16_t*)A, lda, (const bf16_t*)B, ldb,
+                       part_path ? nullptr : bias, part_path ? (void*)work : C, part_path ? N : ldc,
+                       part_path ? 0 : out_bf16, part_path ? 0 : act, kchunk, work);
+  if (part_path) {
+    const long mn = (long)M * N;
+    hipLaunchKernelGGL(k_gemm_reduce, dim3((unsigned)((mn + 255) / 256)), dim3(256), 0, st, M, N, (int)grid.z, work,
+                       bias, C, ldc, out_bf16, act, accumulate);
+  }
+  CHK(hipGetLastError());
+  return 0;
+}
+
+// ------------------------------------------------------------ elementwise
+
+__global__ void k_f32_to_bf16(long n, const float* __restrict__ x, bf16_t* __restrict__ y) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = (bf16_t)x[i];
+}
+__global__ void k_bf16_to_f32(long n, const bf16_t* __restrict__ x, float* __restrict__ y) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = (float)x[i];
+}
+// dz = dy * (y > 0) (ReLU backward through its output), FP32 in, BF16 out
+__global__ void k_relu_bwd(long n, const float* __restrict__ dy, const bf16_t* __restrict__ y,
+                           bf16_t* __restrict__ dz) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dz[i] = (bf16_t)((float)y[i] > 0.f ? dy[i] : 0.f);
+}
+// y[r, 0:n] += x[r, 0:n] for FP32 rows (residual connections, gradient sums)
+__global__ void k_add_f32(long n, const float* __restrict__ x, float* __restrict__ y) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] += x[i];
+}
+// column sums of an FP32 M x N matrix (bias gradients), one thread per column
+__global__ void k_colsum(int M, int N, const float* __restrict__ x, int ldx, float* __restrict__ out, int accumulate) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  float s = 0.f;
+  for (int r = 0; r < M; ++r) s += x[(size_t)r * ldx + c];
+  out[c] = accumulate ? out[c] + s : s;
+}
+
+static inline unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
+
+extern "C" int uph_cast_f32_bf16(long n, const float* x, void* y, void* stream) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_f32_to_bf16, dim3(nblk(n)), dim3(256), 0, (hipStream_t)stream, n, x, (bf16_t*)y);
+  CHK(hipGetLastError());
+  return 0;
+}
+extern "C" int uph_cast_bf16_f32(long n, const void* x, float* y, void* stream) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_bf16_to_f32, dim3(nblk(n)), dim3(256), 0, (hipStream_t)stream, n, (const bf16_t*)x, y);
+  CHK(hipGetLastError());
+  return 0;
+}
+extern "C" int uph_relu_bwd(long n, const float* dy, const void* y, void* dz, void* stream) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_relu_bwd, dim3(nblk(n)), dim3(256), 0, (hipStream_t)stream, n, dy, (const bf16_t*)y,
+                     (bf16_t*)dz);
+  CHK(hipGetLastError());
+  return 0;
+}
+extern "C" int uph_add_f32(long n, const float* x, float* y, void* stream) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_add_f32, dim3(nblk(n)), dim3(256), 0, (hipStream_t)stream, n, x, y);
+  CHK(hipGetLastError());
+  return 0;
+}
+extern "C" int uph_colsum_f32(int M, int N, const float* x, int ldx, float* out, int accumulate, void* stream) {
+  if (N <= 0) return 0;
+  hipLaunchKernelGGL(k_colsum, dim3(nblk(N)), dim3(256), 0, (hipStream_t)stream, M, N, x, ldx, out, accumulate);
+  CHK(hipGetLastError());
+  return 0;
+}
+
+// ------------------------------------------------------------ graph ops
+
+// out[i, :] = sum over e in [off[i], off[i+1]) of x[idx ? idx[e] : e, :]  (FP32 or BF16 in, FP32 out)
+template <bool XBF>
+__global__ void k_segsum(int nseg, int F, const int* __restrict__ off, const int* __restrict__ idx,
+                         const void* __restrict__ x, int ldx, float* __restrict__ out, int ldo, int accumulate) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)nseg * F) return;
+  const int i = (int)(t / F), f = (int)(t % F);
+  float s = 0.f;
+  for (int e = off[i]; e < off[i + 1]; ++e) {
+    const int r = idx ? idx[e] : e;
+    s += XBF ? (float)((const bf16_t*)x)[(size_t)r * ldx + f] : ((const float*)x)[(size_t)r * ldx + f];
+  }
+  float* o = out + (size_t)i * ldo + f;
+  *o = accumulate ? *o + s : s;
+}
+
+extern "C" int uph_segsum(int nseg, int F, const int* off, const int* idx, const void* x, int ldx, int x_bf16,
+                          float* out, int ldo, int accumulate, void* stream) {
+  if (nseg <= 0 || F <= 0) return 0;
+  const long n = (long)nseg * F;
+  if (x_bf16)
+    hipLaunchKernelGGL(k_segsum<true>, dim3(nblk(n)), dim3(256), 0, (hipStream_t)stream, nseg, F, off, idx, x, ldx,
+                       out, ldo, accumulate);
+  else
+    hipLaunchKernelGGL(k_segsum<false>, dim3(nblk(n)), dim3(256), 0, (hipStream_t)stream, nseg, F, off, idx, x, ldx,
+                       out, ldo, accumulate);
+  CHK(hipGetLastError());
+  return 0;
+}
+
+// y[e, :] = x[rows[e], :]  (row gather; BF16 or FP32 by element size)
+__global__ void k_gather_rows(int n, int F, const int* __restrict__ rows, const char* __restrict__ x, int ldx_bytes,
+                              char* __restrict__ y, int ldy_bytes, int esize) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)n * F) return;
+  const int e = (int)(t / F), f = (int)(t % F);
+  const char* s = x + (size_t)rows[e] * ldx_bytes + (size_t)f * esize;
+  char* d = y + (size_t)e * ldy_bytes + (size_t)f * esize;
+  if (esize == 2)
+    *(uint16_t*)d = *(const uint16_t*)s;
+  else
+    *(uint32_t*)d = *(const uint32_t*)s;
+}
+
+extern "C" int uph_gather_rows(int n, int F, const int* rows, const void* x, int ldx, void* y, int ldy, int esize,
+                               void* stream) {
+  if (n <= 0 || F <= 0) return 0;
+  if (esize != 2 && esize != 4) return fail("uph_gather_rows: element size must be 2 or 4");
+  const long t = (long)n * F;
+  hipLaunchKernelGGL(k_gather_rows, dim3(nblk(t)), dim3(256), 0, (hipStream_t)stream, n, F, rows, (const char*)x,
+                     ldx * esize, (char*)y, ldy * esize, esize);
+  CHK(hipGetLastError());
+  return 0;
+}
+
+// ------------------------------------------------------------ optimiser / loss
+
+// Adam on FP32 master weights; writes the BF16 copy used by the GEMMs.
+__global__ void k_adam(long n, float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, bf16_t* __restrict__ wb, float lr, float b1, float b2, float eps,
+                       float c1, float c2) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  w[i] = wi;
+  if (wb) wb[i] = (bf16_t)wi;
+}
+
+extern "C" int uph_adam(long n, float* w, const float* g, float* m, float* v, void* w_bf16, float lr, float b1,
+                        float b2, float eps, int t, void* stream) {
+  if (n <= 0) return 0;
+  const float c1 = 1.f - powf(b1, (float)t), c2 = 1.f - powf(b2, (float)t);
+  hipLaunchKernelGGL(k_adam, dim3(nblk(n)), dim3(256), 0, (hipStream_t)stream, n, w, g, m, v, (bf16_t*)w_bf16, lr,
+                     b1, b2, eps, c1, c2);
+  CHK(hipGetLastError());
+  return 0;
+}
+
+// Masked MSE: d[i, c] = 2 (y - t) / count over rows with mask; per-block partial losses.
+__global__ void k_mse(int M, int C, const float* __restrict__ y, int ldy, const float* __restrict__ t, int ldt,
+                      const int* __restrict__ mask, float inv_count, float* __restrict__ dy, float* __restrict__ lpart) {
+  __shared__ float red[256];
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  float l = 0.f;
+  if (i < (long)M * C) {
+    const int r = (int)(i / C), c = (int)(i % C);
+    const bool on = !mask || mask[r];
+    const float d = on ? y[(size_t)r * ldy + c] - t[(size_t)r * ldt + c] : 0.f;
+    l = d * d * inv_count;
+    dy[(size_t)r * ldy + c] = 2.f * d * inv_count;
+  }
+  red[threadIdx.x] = l;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) lpart[blockIdx.x] = red[0];
+}
+
+extern "C" int uph_mse(int M, int C, const float* y, int ldy, const float* t, int ldt, const int* mask, int count,
+                       float* dy, float* loss_part, void* stream) {
+  if (M <= 0 || C <= 0 || count <= 0) return fail("uph_mse: empty input");
+  const long n = (long)M * C;
+  hipLaunchKernelGGL(k_mse, dim3(nblk(n)), dim3(256), 0, (hipStream_t)stream, M, C, y, ldy, t, ldt, mask,
+                     1.0f / (float)count, dy, loss_part);
+  CHK(hipGetLastError());
+  return 0;
+}

@@ -57,6 +57,32 @@ SIGNATURES = {
     "uph_itime": (C.c_int, [C.c_void_p]),
     "uph_time_steps": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp]),
     "uph_last_error": (C.c_char_p, [C.c_void_p]),
+    # surrogate kernels (pointers are device addresses passed as integers)
+    "uph_dev_alloc": (C.c_int, [C.POINTER(C.c_void_p), C.c_size_t]),
+    "uph_dev_free": (C.c_int, [C.c_void_p]),
+    "uph_dev_upload": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "uph_dev_download": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "uph_dev_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "uph_dev_zero": (C.c_int, [C.c_void_p, C.c_size_t]),
+    "uph_dev_sync": (C.c_int, []),
+    "uph_gnn_last_error": (C.c_char_p, []),
+    "uph_gemm_bf16": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
+                                C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int,
+                                C.c_void_p, C.c_int, C.c_void_p]),
+    "uph_cast_f32_bf16": (C.c_int, [C.c_long, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "uph_cast_bf16_f32": (C.c_int, [C.c_long, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "uph_relu_bwd": (C.c_int, [C.c_long, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "uph_add_f32": (C.c_int, [C.c_long, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "uph_colsum_f32": (C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                                 C.c_void_p]),
+    "uph_segsum": (C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int,
+                             C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
+    "uph_gather_rows": (C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                  C.c_int, C.c_int, C.c_void_p]),
+    "uph_adam": (C.c_int, [C.c_long, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                           C.c_float, C.c_float, C.c_float, C.c_float, C.c_int, C.c_void_p]),
+    "uph_mse": (C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p,
+                          C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
 }
 
 _lib = None
