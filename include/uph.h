@@ -131,6 +131,27 @@ int uph_time_steps(uph_t *h, int n_steps, double *ms_total, double *ms_pair);
 
 const char *uph_last_error(const uph_t *h);
 
+/* Device-side state exchange with the surrogate (no counterpart in the reference).
+ * Arrays are device pointers in the reference's order (pos/vel: 2n doubles).
+ * uph_state_from_device loads the fluid rows (walls keep their state); with
+ * restart != 0 the next step takes the first-step form of the leapfrog
+ * (sph.cc:154-167) with localVelocity = velocity, and uph_itime restarts at 0.
+ * Both calls, and uph_graph_build, synchronise the device before reading caller
+ * buffers (which may have been written on any stream) and return with their
+ * results complete. */
+int uph_state_to_device(uph_t *h, int scene, double *pos, double *vel, double *rho);
+int uph_state_from_device(uph_t *h, int scene, const double *pos, const double *vel, const double *rho,
+                          int restart);
+
+/* Radius graph of n device positions (2n doubles): receivers are the first n_recv
+ * particles; an edge (i <- j) exists for every j != i with squared distance < dist.
+ * off (n+1): CSR by receiver (0 edges for non-receivers); send/recv (E): edge ends in
+ * candidate order; soff (n+1), sperm (E): for node j, the ids of its edges as a sender.
+ * Returns -2 (and *n_edges) when E exceeds capacity. */
+int uph_graph_build(const double *pos, int n, int n_recv, double dist, int *off, int *send, int *recv,
+                    int *soff, int *sperm, long capacity, long *n_edges);
+const char *uph_graph_last_error(void);
+
 /* ------------------------------------------ surrogate (graph network) kernels
  *
  * No counterpart in the reference: the learned acceleration surrogate that runs
@@ -169,6 +190,25 @@ int uph_adam(long n, float *w, const float *g, float *m, float *v, void *w_bf16,
              float b2, float eps, int t, void *stream);
 int uph_mse(int M, int C, const float *y, int ldy, const float *t, int ldt, const int *mask, int count,
             float *dy, float *loss_part, void *stream);
+int uph_transpose_bf16(int K, int N, const float *w, void *wt, void *stream);
+int uph_gather_add2(long E, int F, const int *ra, const float *a, const int *rb, const float *b, int ldab,
+                    float *z, int ldz, void *stream);
+int uph_relu_to_bf16(long M, int F, const float *z, int ldz, void *y, int ldy, void *stream);
+int uph_rows_to_bf16(long M, int F, const float *x, int ldx, void *y, int ldy, void *stream);
+int uph_fill_col_bf16(long M, int ld, int col, float v, void *y, void *stream);
+int uph_relu_bwd_rows(long M, int F, const float *dy, int lddy, const void *y, int ldy, void *dz, int ldz,
+                      void *stream);
+int uph_add_rows_f32(long M, int F, const float *x, int ldx, float *y, int ldy, void *stream);
+/* Surrogate inputs and update: node features from 5 macro-step velocities (hist: 5 x n x 2,
+ * newest first) and a wall flag; edge features (x_recv - x_send)/R, |r|/R; semi-implicit
+ * Euler with the predicted acceleration; SPH summation density over the graph. */
+int uph_gns_node_feats(int n, int n_recv, const double *hist, float inv_vscale, void *out, void *stream);
+int uph_gns_edge_feats(long E, const int *send, const int *recv, const double *pos, double invR, void *out,
+                       void *stream);
+int uph_gns_integrate(int n_recv, double *pos, double *vel, const float *acc, int lda, double dt,
+                      double ascale, void *stream);
+int uph_sph_density(int n_recv, const int *off, const int *send, const double *pos, const double *mass,
+                    double h, double *rho, void *stream);
 
 #ifdef __cplusplus
 }

@@ -98,12 +98,14 @@ def test_segsum_and_gather():
     dx, doff, didx = DevBuf.of(x), DevBuf.of(off), DevBuf.of(idx)
     out = DevBuf(nseg * F * 4)
     check(lib().uph_segsum(nseg, F, doff.ptr, didx.ptr, dx.ptr, F, 0, out.ptr, F, 0, None))
+    check(lib().uph_dev_sync())
     got = out.download((nseg, F), np.float32)
     ref = np.stack([x[idx[off[i]:off[i + 1]]].sum(0) for i in range(nseg)])
     assert np.allclose(got, ref, atol=1e-5)
     rows = rng.integers(0, E, 123).astype(np.int32)
     y = DevBuf(123 * F * 4)
-    check(lib().uph_gather_rows(123, F, DevBuf.of(rows).ptr, dx.ptr, F, y.ptr, F, 4, None))
+    drows = DevBuf.of(rows)            # keep every device buffer alive across the launch
+    check(lib().uph_gather_rows(123, F, drows.ptr, dx.ptr, F, y.ptr, F, 4, None))
     assert np.array_equal(y.download((123, F), np.float32), x[rows])
 
 
@@ -115,6 +117,7 @@ def test_adam_and_mse():
     dw, dg, dm, dv = DevBuf.of(w), DevBuf.of(g), DevBuf.of(m), DevBuf.of(v)
     wb = DevBuf(2 * n)
     check(lib().uph_adam(n, dw.ptr, dg.ptr, dm.ptr, dv.ptr, wb.ptr, 1e-3, 0.9, 0.999, 1e-8, 1, None))
+    check(lib().uph_dev_sync())
     mr = 0.1 * g
     vr = 0.001 * g * g
     wr = w - 1e-3 * (mr / 0.1) / (np.sqrt(vr / 0.001) + 1e-8)
@@ -125,8 +128,9 @@ def test_adam_and_mse():
     mask = (rng.random(M) < 0.7).astype(np.int32)
     cnt = int(mask.sum()) * Cc
     dy, lp = DevBuf(M * Cc * 4), DevBuf(((M * Cc + 255) // 256) * 4)
-    check(lib().uph_mse(M, Cc, DevBuf.of(y).ptr, Cc, DevBuf.of(t).ptr, Cc, DevBuf.of(mask).ptr, cnt,
-                        dy.ptr, lp.ptr, None))
+    yd, td, md = DevBuf.of(y), DevBuf.of(t), DevBuf.of(mask)
+    check(lib().uph_mse(M, Cc, yd.ptr, Cc, td.ptr, Cc, md.ptr, cnt, dy.ptr, lp.ptr, None))
+    check(lib().uph_dev_sync())
     loss = lp.download((M * Cc + 255) // 256, np.float32).sum()
     d = (y - t) * mask[:, None]
     assert loss == pytest.approx((d * d).sum() / cnt, rel=1e-5)

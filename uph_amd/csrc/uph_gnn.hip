@@ -423,3 +423,222 @@ extern "C" int uph_mse(int M, int C, const float* y, int ldy, const float* t, in
   CHK(hipGetLastError());
   return 0;
 }
+
+// ------------------------------------------------------------ surrogate features
+
+// Node features (BF16, n x 16): columns 0-9 are the velocities at the last five
+// macro steps divided by vscale (hist holds 5 x n x 2 doubles, newest first),
+// 10 the wall flag (i >= n_recv), 11 the constant 1 that carries the first
+// layer's bias, 12-15 zero.
+__global__ void k_gns_node_feats(int n, int n_recv, const double* __restrict__ hist, float inv_vscale,
+                                 bf16_t* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  bf16_t* o = out + (size_t)i * 16;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    o[2 * k] = (bf16_t)((float)hist[((size_t)k * n + i) * 2] * inv_vscale);
+    o[2 * k + 1] = (bf16_t)((float)hist[((size_t)k * n + i) * 2 + 1] * inv_vscale);
+  }
+  o[10] = (bf16_t)(i >= n_recv ? 1.0f : 0.0f);
+  o[11] = (bf16_t)1.0f;
+#pragma unroll
+  for (int k = 12; k < 16; ++k) o[k] = (bf16_t)0.0f;
+}
+
+// Edge features (BF16, E x 8): (x_recv - x_send) / R, |r| / R, the constant 1, zeros.
+__global__ void k_gns_edge_feats(long E, const int* __restrict__ send, const int* __restrict__ recv,
+                                 const double* __restrict__ pos, double invR, bf16_t* __restrict__ out) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  const int i = recv[e], j = send[e];
+  const double dx = (pos[2 * i] - pos[2 * j]) * invR, dy = (pos[2 * i + 1] - pos[2 * j + 1]) * invR;
+  bf16_t* o = out + (size_t)e * 8;
+  o[0] = (bf16_t)(float)dx;
+  o[1] = (bf16_t)(float)dy;
+  o[2] = (bf16_t)(float)sqrt(dx * dx + dy * dy);
+  o[3] = (bf16_t)1.0f;
+#pragma unroll
+  for (int k = 4; k < 8; ++k) o[k] = (bf16_t)0.0f;
+}
+
+// Semi-implicit Euler on the predicted acceleration (FP64 state):
+// v += dt * ascale * a;  x += dt * v   for the receivers (fluid) only.
+__global__ void k_gns_integrate(int n_recv, double* __restrict__ pos, double* __restrict__ vel,
+                                const float* __restrict__ acc, int lda, double dt, double ascale) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_recv) return;
+  const double vx = vel[2 * i] + dt * ascale * (double)acc[(size_t)i * lda];
+  const double vy = vel[2 * i + 1] + dt * ascale * (double)acc[(size_t)i * lda + 1];
+  vel[2 * i] = vx;
+  vel[2 * i + 1] = vy;
+  pos[2 * i] += dt * vx;
+  pos[2 * i + 1] += dt * vy;
+}
+
+// SPH summation density of the receivers over the radius graph (cubic spline,
+// derivatives.c:44-66): rho_i = m_i W(0) + sum_j m_j W(r_ij).
+__global__ void k_sph_density(int n_recv, const int* __restrict__ off, const int* __restrict__ send,
+                              const double* __restrict__ pos, const double* __restrict__ mass, double h,
+                              double* __restrict__ rho) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_recv) return;
+  const double alpha = 10.0 / (7.0 * M_PI * h * h);
+  double s = mass[i] * alpha;
+  for (int e = off[i]; e < off[i + 1]; ++e) {
+    const int j = send[e];
+    const double dx = pos[2 * i] - pos[2 * j], dy = pos[2 * i + 1] - pos[2 * j + 1];
+    const double q = sqrt(dx * dx + dy * dy) / h;
+    double w = 0.0;
+    if (q < 1.0)
+      w = alpha * (1.0 - 1.5 * q * q + 0.75 * q * q * q);
+    else if (q < 2.0)
+      w = alpha * 0.25 * (2.0 - q) * (2.0 - q) * (2.0 - q);
+    s += mass[j] * w;
+  }
+  rho[i] = s;
+}
+
+extern "C" int uph_gns_node_feats(int n, int n_recv, const double* hist, float inv_vscale, void* out, void* stream) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_gns_node_feats, dim3(nblk(n)), dim3(256), 0, (hipStream_t)stream, n, n_recv, hist,
+                     inv_vscale, (bf16_t*)out);
+  CHK(hipGetLastError());
+  return 0;
+}
+extern "C" int uph_gns_edge_feats(long E, const int* send, const int* recv, const double* pos, double invR, void* out,
+                                  void* stream) {
+  if (E <= 0) return 0;
+  hipLaunchKernelGGL(k_gns_edge_feats, dim3(nblk(E)), dim3(256), 0, (hipStream_t)stream, E, send, recv, pos, invR,
+                     (bf16_t*)out);
+  CHK(hipGetLastError());
+  return 0;
+}
+extern "C" int uph_gns_integrate(int n_recv, double* pos, double* vel, const float* acc, int lda, double dt,
+                                 double ascale, void* stream) {
+  if (n_recv <= 0) return 0;
+  hipLaunchKernelGGL(k_gns_integrate, dim3(nblk(n_recv)), dim3(256), 0, (hipStream_t)stream, n_recv, pos, vel, acc,
+                     lda, dt, ascale);
+  CHK(hipGetLastError());
+  return 0;
+}
+extern "C" int uph_sph_density(int n_recv, const int* off, const int* send, const double* pos, const double* mass,
+                               double h, double* rho, void* stream) {
+  if (n_recv <= 0) return 0;
+  hipLaunchKernelGGL(k_sph_density, dim3(nblk(n_recv)), dim3(256), 0, (hipStream_t)stream, n_recv, off, send, pos,
+                     mass, h, rho);
+  CHK(hipGetLastError());
+  return 0;
+}
+
+// W^T as BF16 (K x N FP32 -> N x K BF16), for the input-gradient GEMMs.
+__global__ void k_transpose_bf16(int K, int N, const float* __restrict__ w, bf16_t* __restrict__ wt) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)K * N) return;
+  const int k = (int)(t / N), n = (int)(t % N);
+  wt[(size_t)n * K + k] = (bf16_t)w[t];
+}
+extern "C" int uph_transpose_bf16(int K, int N, const float* w, void* wt, void* stream) {
+  if (K <= 0 || N <= 0) return 0;
+  hipLaunchKernelGGL(k_transpose_bf16, dim3(nblk((long)K * N)), dim3(256), 0, (hipStream_t)stream, K, N, w,
+                     (bf16_t*)wt);
+  CHK(hipGetLastError());
+  return 0;
+}
+
+// z[e, :] += a[ra[e], :] + b[rb[e], :]   (FP32 rows; the per-node projections of
+// the edge MLP's first layer added to its per-edge part)
+__global__ void k_gather_add2(long E, int F, const int* __restrict__ ra, const float* __restrict__ a,
+                              const int* __restrict__ rb, const float* __restrict__ b, int ldab,
+                              float* __restrict__ z, int ldz) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= E * F) return;
+  const long e = t / F;
+  const int f = (int)(t % F);
+  z[e * ldz + f] += a[(size_t)ra[e] * ldab + f] + b[(size_t)rb[e] * ldab + f];
+}
+extern "C" int uph_gather_add2(long E, int F, const int* ra, const float* a, const int* rb, const float* b, int ldab,
+                               float* z, int ldz, void* stream) {
+  if (E <= 0 || F <= 0) return 0;
+  hipLaunchKernelGGL(k_gather_add2, dim3(nblk(E * F)), dim3(256), 0, (hipStream_t)stream, E, F, ra, a, rb, b, ldab, z,
+                     ldz);
+  CHK(hipGetLastError());
+  return 0;
+}
+
+// y = relu(z) as BF16 into a row of width ld (columns F..ld-1 untouched)
+__global__ void k_relu_to_bf16(long M, int F, const float* __restrict__ z, int ldz, bf16_t* __restrict__ y, int ldy) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= M * F) return;
+  const long r = t / F;
+  const int f = (int)(t % F);
+  y[r * ldy + f] = (bf16_t)fmaxf(z[r * ldz + f], 0.f);
+}
+extern "C" int uph_relu_to_bf16(long M, int F, const float* z, int ldz, void* y, int ldy, void* stream) {
+  if (M <= 0 || F <= 0) return 0;
+  hipLaunchKernelGGL(k_relu_to_bf16, dim3(nblk(M * F)), dim3(256), 0, (hipStream_t)stream, M, F, z, ldz, (bf16_t*)y,
+                     ldy);
+  CHK(hipGetLastError());
+  return 0;
+}
+
+// FP32 rows -> BF16 rows of width ldy (the latent streams' GEMM inputs)
+__global__ void k_rows_to_bf16(long M, int F, const float* __restrict__ x, int ldx, bf16_t* __restrict__ y, int ldy) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= M * F) return;
+  const long r = t / F;
+  const int f = (int)(t % F);
+  y[r * ldy + f] = (bf16_t)x[r * ldx + f];
+}
+extern "C" int uph_rows_to_bf16(long M, int F, const float* x, int ldx, void* y, int ldy, void* stream) {
+  if (M <= 0 || F <= 0) return 0;
+  hipLaunchKernelGGL(k_rows_to_bf16, dim3(nblk(M * F)), dim3(256), 0, (hipStream_t)stream, M, F, x, ldx, (bf16_t*)y,
+                     ldy);
+  CHK(hipGetLastError());
+  return 0;
+}
+
+// Fill column `col` of an M x ld BF16 matrix with `v` (the constant-1 bias feature)
+__global__ void k_fill_col(long M, int ld, int col, float v, bf16_t* __restrict__ y) {
+  const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < M) y[r * ld + col] = (bf16_t)v;
+}
+extern "C" int uph_fill_col_bf16(long M, int ld, int col, float v, void* y, void* stream) {
+  if (M <= 0) return 0;
+  hipLaunchKernelGGL(k_fill_col, dim3(nblk(M)), dim3(256), 0, (hipStream_t)stream, M, ld, col, v, (bf16_t*)y);
+  CHK(hipGetLastError());
+  return 0;
+}
+
+// dz[r, f] = dy[r, f] * (y[r, f] > 0) on the first F columns of strided rows
+__global__ void k_relu_bwd_rows(long M, int F, const float* __restrict__ dy, int lddy, const bf16_t* __restrict__ y,
+                                int ldy, bf16_t* __restrict__ dz, int ldz) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= M * F) return;
+  const long r = t / F;
+  const int f = (int)(t % F);
+  dz[r * ldz + f] = (bf16_t)((float)y[r * ldy + f] > 0.f ? dy[r * lddy + f] : 0.f);
+}
+extern "C" int uph_relu_bwd_rows(long M, int F, const float* dy, int lddy, const void* y, int ldy, void* dz, int ldz,
+                                 void* stream) {
+  if (M <= 0 || F <= 0) return 0;
+  hipLaunchKernelGGL(k_relu_bwd_rows, dim3(nblk(M * F)), dim3(256), 0, (hipStream_t)stream, M, F, dy, lddy,
+                     (const bf16_t*)y, ldy, (bf16_t*)dz, ldz);
+  CHK(hipGetLastError());
+  return 0;
+}
+
+// y[r, 0:F] += x[r, 0:F] on strided FP32 rows
+__global__ void k_add_rows(long M, int F, const float* __restrict__ x, int ldx, float* __restrict__ y, int ldy) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= M * F) return;
+  const long r = t / F;
+  const int f = (int)(t % F);
+  y[r * ldy + f] += x[r * ldx + f];
+}
+extern "C" int uph_add_rows_f32(long M, int F, const float* x, int ldx, float* y, int ldy, void* stream) {
+  if (M <= 0 || F <= 0) return 0;
+  hipLaunchKernelGGL(k_add_rows, dim3(nblk(M * F)), dim3(256), 0, (hipStream_t)stream, M, F, x, ldx, y, ldy);
+  CHK(hipGetLastError());
+  return 0;
+}

@@ -83,6 +83,32 @@ SIGNATURES = {
                            C.c_float, C.c_float, C.c_float, C.c_float, C.c_int, C.c_void_p]),
     "uph_mse": (C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p,
                           C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "uph_transpose_bf16": (C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "uph_gather_add2": (C.c_int, [C.c_long, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                  C.c_int, C.c_void_p, C.c_int, C.c_void_p]),
+    "uph_relu_to_bf16": (C.c_int, [C.c_long, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                                   C.c_void_p]),
+    "uph_rows_to_bf16": (C.c_int, [C.c_long, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                                   C.c_void_p]),
+    "uph_fill_col_bf16": (C.c_int, [C.c_long, C.c_int, C.c_int, C.c_float, C.c_void_p, C.c_void_p]),
+    "uph_relu_bwd_rows": (C.c_int, [C.c_long, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                                    C.c_void_p, C.c_int, C.c_void_p]),
+    "uph_add_rows_f32": (C.c_int, [C.c_long, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                                   C.c_void_p]),
+    "uph_gns_node_feats": (C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_float, C.c_void_p, C.c_void_p]),
+    "uph_gns_edge_feats": (C.c_int, [C.c_long, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
+                                     C.c_void_p, C.c_void_p]),
+    "uph_gns_integrate": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double,
+                                    C.c_double, C.c_void_p]),
+    "uph_sph_density": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
+                                  C.c_void_p, C.c_void_p]),
+    "uph_state_to_device": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "uph_state_from_device": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                        C.c_int]),
+    "uph_graph_build": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_void_p, C.c_void_p,
+                                  C.c_void_p, C.c_void_p, C.c_void_p, C.c_long,
+                                  C.POINTER(C.c_long)]),
+    "uph_graph_last_error": (C.c_char_p, []),
 }
 
 _lib = None
