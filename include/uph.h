@@ -136,9 +136,9 @@ const char *uph_last_error(const uph_t *h);
  * uph_state_from_device loads the fluid rows (walls keep their state); with
  * restart != 0 the next step takes the first-step form of the leapfrog
  * (sph.cc:154-167) with localVelocity = velocity, and uph_itime restarts at 0.
- * Both calls, and uph_graph_build, synchronise the device before reading caller
- * buffers (which may have been written on any stream) and return with their
- * results complete. */
+ * Both calls, and uph_graph_build, synchronise the device before touching caller
+ * buffers (which may be in use on any stream) and return with their results
+ * complete. */
 int uph_state_to_device(uph_t *h, int scene, double *pos, double *vel, double *rho);
 int uph_state_from_device(uph_t *h, int scene, const double *pos, const double *vel, const double *rho,
                           int restart);
@@ -209,6 +209,11 @@ int uph_gns_integrate(int n_recv, double *pos, double *vel, const float *acc, in
                       double ascale, void *stream);
 int uph_sph_density(int n_recv, const int *off, const int *send, const double *pos, const double *mass,
                     double h, double *rho, void *stream);
+/* FP64 elementwise helpers of the rollout: out = a x + b y (macro-step velocities from
+ * positions); out = x * num / den (density carried across a surrogate block). */
+int uph_lincomb_f64(long n, double a, const double *x, double b, const double *y, double *out, void *stream);
+int uph_scale_ratio_f64(long n, const double *x, const double *num, const double *den, double *out,
+                        void *stream);
 
 #ifdef __cplusplus
 }

@@ -642,3 +642,31 @@ extern "C" int uph_add_rows_f32(long M, int F, const float* x, int ldx, float* y
   CHK(hipGetLastError());
   return 0;
 }
+
+// out = a x + b y over FP64 arrays
+__global__ void k_lincomb_f64(long n, double a, const double* __restrict__ x, double b, const double* __restrict__ y,
+                              double* __restrict__ out) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = a * x[i] + b * y[i];
+}
+extern "C" int uph_lincomb_f64(long n, double a, const double* x, double b, const double* y, double* out,
+                               void* stream) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_lincomb_f64, dim3(nblk(n)), dim3(256), 0, (hipStream_t)stream, n, a, x, b, y, out);
+  CHK(hipGetLastError());
+  return 0;
+}
+
+// out = x * num / den over FP64 arrays
+__global__ void k_scale_ratio_f64(long n, const double* __restrict__ x, const double* __restrict__ num,
+                                  const double* __restrict__ den, double* __restrict__ out) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = x[i] * (num[i] / den[i]);
+}
+extern "C" int uph_scale_ratio_f64(long n, const double* x, const double* num, const double* den, double* out,
+                                   void* stream) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_scale_ratio_f64, dim3(nblk(n)), dim3(256), 0, (hipStream_t)stream, n, x, num, den, out);
+  CHK(hipGetLastError());
+  return 0;
+}

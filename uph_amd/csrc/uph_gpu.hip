@@ -1388,6 +1388,8 @@ extern "C" int uph_state_to_device(uph_t* h, int scene, double* pos, double* vel
     return -1;
   }
   const DevScene& S = h->hs[scene];
+  // the caller's work on pos/vel/rho, on any stream, completes before they are overwritten
+  HIPCHK(hipDeviceSynchronize(), h->err);
   hipLaunchKernelGGL(k_state_out, dim3(grid_of(S.np)), dim3(BLK), 0, h->stream, S.poff, S.np, h->b.tagA, h->b.posB,
                      h->b.velB, h->b.rhoB, (double2*)pos, (double2*)vel, rho);
   HIPCHK(hipGetLastError(), h->err);

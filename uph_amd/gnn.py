@@ -170,7 +170,10 @@ class GraphNet:
         cn, cE, ct = self._cap
         if n <= cn and E <= cE and (ct or not train):
             return
-        n, E, train = max(n, cn), max(E, cE), bool(train or ct)
+        # grow with headroom: rollouts see the edge count fluctuate step to step
+        n = max(n if n <= cn else int(n * 1.1) + 64, cn)
+        E = max(E if E <= cE else int(E * 1.25) + 1024, cE)
+        train = bool(train or ct)
         W, Wl = self.W, self.W + 8
         nb = lambda rows, cols, es: DevBuf(rows * cols * es)
         self.b = b = {}

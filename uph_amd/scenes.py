@@ -164,12 +164,14 @@ def _tank_walls(dx, tank_w, tank_h, layers=3):
 
 def dam_break(width=1.0, height=2.0, dx=0.02, tank=(4.0, 3.0), layers=3, h_factor=1.3,
               alpha_av=0.1, cfl=0.25, rho0=1000.0, gamma=7.0, g=9.81, jitter=0.0, seed=0,
-              name=None) -> Scene:
+              eos_height=None, name=None) -> Scene:
     """Dam break: fluid column width x height at the left of a tank, hydrostatic density.
 
     Fluid at ((i+1/2)dx, (j+1/2)dx); walls of `layers` fixed boundary rows at the
     same spacing; m = rho0 dx^2; h = h_factor dx; c0 = 10 sqrt(2 g H);
     dt = cfl h / c0.  `jitter` displaces fluid particles uniformly by +-jitter*dx.
+    `eos_height` (default: the column height) is the height c0 is derived from, so
+    columns of different heights can share one sound speed and time step.
     """
     nx = int(math.floor(width / dx + 1e-9))
     ny = int(math.floor(height / dx + 1e-9))
@@ -178,7 +180,7 @@ def dam_break(width=1.0, height=2.0, dx=0.02, tank=(4.0, 3.0), layers=3, h_facto
         rng = np.random.default_rng(seed)
         fl = fl + rng.uniform(-jitter * dx, jitter * dx, fl.shape)
     H = ny * dx
-    c0, B = tait_constants(rho0, gamma, g, H)
+    c0, B = tait_constants(rho0, gamma, g, H if eos_height is None else eos_height)
     rho, p = hydrostatic_density(fl[:, 1], H, rho0, gamma, g, B)
     walls = _tank_walls(dx, tank[0], tank[1], layers)
     mass = rho0 * dx * dx

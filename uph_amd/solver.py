@@ -95,6 +95,10 @@ SIGNATURES = {
                                     C.c_void_p, C.c_int, C.c_void_p]),
     "uph_add_rows_f32": (C.c_int, [C.c_long, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
                                    C.c_void_p]),
+    "uph_lincomb_f64": (C.c_int, [C.c_long, C.c_double, C.c_void_p, C.c_double, C.c_void_p,
+                                  C.c_void_p, C.c_void_p]),
+    "uph_scale_ratio_f64": (C.c_int, [C.c_long, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_void_p]),
     "uph_gns_node_feats": (C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_float, C.c_void_p, C.c_void_p]),
     "uph_gns_edge_feats": (C.c_int, [C.c_long, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
                                      C.c_void_p, C.c_void_p]),
@@ -248,6 +252,17 @@ class GpuSolver:
         self._check(lib().uph_get_rates(self.h, scene, _ptr(R, C.c_float), _ptr(Rt, C.c_float),
                                         _ptr(a, C.c_float), _ptr(at, C.c_float)))
         return dict(R=R, Rt=Rt, a=a, at=at)
+
+    def state_to_device(self, pos: int, vel: int, rho: int, scene: int = 0):
+        """Copies the scene's state, in reference order, into device buffers (pointers)."""
+        self._scene(scene)
+        self._check(lib().uph_state_to_device(self.h, scene, pos, vel, rho))
+
+    def state_from_device(self, pos: int, vel: int, rho: int, restart: bool = True,
+                          scene: int = 0):
+        """Loads the fluid rows of (pos, vel, rho) from device buffers (pointers)."""
+        self._scene(scene)
+        self._check(lib().uph_state_from_device(self.h, scene, pos, vel, rho, int(restart)))
 
     def time_steps(self, n: int):
         """(ms for the whole loop, mean ms of the fused pair kernel), HIP events."""
