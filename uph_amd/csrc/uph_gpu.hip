@@ -460,7 +460,7 @@ struct PairCtx {
     }
     float ax = -mq * pij * gx;
     float ay = -mq * pij * gy;
-    // PressureGradientForce weighs P_p by lam_q (derivatives.c:1126-1128); the
+    // PressureGradientForce weighs P_p by lam_q (derivatives.c:1129-1131); the
     // predicting form (derivatives.c:1354-1356) does not
     if (tyq != 1) {
       const float w = -mq * prp * (lam - 1.0f);
