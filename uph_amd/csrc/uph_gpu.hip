@@ -385,7 +385,7 @@ struct PairCtx {
   uint32_t idp;
   bool pb_ok;
   float vpx, vpy, lpx, lpy, rhop, prp, mp, inv_mp;
-  float h, invh, walpha, h2eps;
+  float h, invh, walpha, h2eps, cav;
   CSum sR, sRt, sAx, sAy;   // sAx/sAy: the predicting acceleration (dvdt_approx)
   float dAx = 0.f, dAy = 0.f;   // dvdt - dvdt_approx: wall pairs with localDensity != density
 
@@ -397,6 +397,7 @@ struct PairCtx {
     inv_mp = 1.0f / mp;
     h = S.fh; invh = S.finvh; walpha = S.fwalpha;
     h2eps = 0.01f * h * h;
+    cav = -2.0f * S.fav * S.fc0 * h;
   }
 
   // Admission of an in-range pair by the rules of Search.cc:33 (original index
@@ -440,33 +441,34 @@ struct PairCtx {
     const float gx = f * dx, gy = f * dy;
     const float mq = bq.z;
     const uint32_t tyq = tag_type(tq);
-    // localDensity/density of q: 1 for fluid (the leapfrog keeps them equal), rho0/rho
-    // for walls (inputbp.c:288 resets their localDensity to rho0)
-    const float lam = tyq == 1 ? 1.0f : S.frho0 * __builtin_amdgcn_rcpf(bq.x);
-    // continuity (derivatives.c:1520-1567), symmetric in the pair order
-    sR.add(mq * lam * ((vpx - aq.z) * gx + (vpy - aq.w) * gy));
-    // predicting continuity (derivatives.c:1217-1252), depends on the pair order
-    if (p_is_i)
-      sRt.add(mq * ((vpx - aq.z) * gx + (vpy - aq.w) * gy));
-    else
-      sRt.add(mp * ((lpx - aq.x) * gx + (lpy - aq.y) * gy));
-    // pressure gradient (derivatives.c:1109-1163) + Monaghan viscosity
-    const float dvx = vpx - aq.x, dvy = vpy - aq.y;
-    const float vr = dvx * dx + dvy * dy;
-    float pij = prp + bq.y;
-    if (vr < 0.0f) {
-      const float mu = h * vr * __builtin_amdgcn_rcpf(r2 + h2eps);
-      pij += -S.fav * S.fc0 * mu * (2.0f * __builtin_amdgcn_rcpf(rhop + bq.x));
-    }
-    float ax = -mq * pij * gx;
-    float ay = -mq * pij * gy;
-    // PressureGradientForce weighs P_p by lam_q (derivatives.c:1129-1131); the
-    // predicting form (derivatives.c:1354-1356) does not
+    // m_q times localDensity/density of q: the factor is 1 for fluid (the leapfrog
+    // keeps them equal) and rho0/rho for walls (inputbp.c:288 resets their
+    // localDensity to rho0).  PressureGradientForce weighs P_p by that factor
+    // (derivatives.c:1129-1131), the predicting form (derivatives.c:1353-1355) does
+    // not: the difference is kept apart in dA.
+    float mql = mq;
     if (tyq != 1) {
+      const float lam = S.frho0 * __builtin_amdgcn_rcpf(bq.x);
+      mql = mq * lam;
       const float w = -mq * prp * (lam - 1.0f);
       dAx += w * gx;
       dAy += w * gy;
     }
+    // continuity (derivatives.c:1520-1567), symmetric in the pair order
+    const float cont = (vpx - aq.z) * gx + (vpy - aq.w) * gy;
+    sR.add(mql * cont);
+    // predicting continuity (derivatives.c:1217-1252), depends on the pair order
+    const float alt = (lpx - aq.x) * gx + (lpy - aq.y) * gy;
+    sRt.add(p_is_i ? mq * cont : mp * alt);
+    // pressure gradient (derivatives.c:1109-1163) + Monaghan viscosity
+    // -alpha c0 mu / rho_bar with mu = h v.r / (r^2 + 0.01 h^2), rho_bar = (rho_p + rho_q) / 2,
+    // through one reciprocal of the product of the two denominators
+    const float dvx = vpx - aq.x, dvy = vpy - aq.y;
+    const float vr = dvx * dx + dvy * dy;
+    float pij = prp + bq.y;
+    if (vr < 0.0f) pij += cav * vr * __builtin_amdgcn_rcpf((r2 + h2eps) * (rhop + bq.x));
+    float ax = -mq * pij * gx;
+    float ay = -mq * pij * gy;
     // wall spring (derivatives.c:2396-2449).  The damping part does not vanish at
     // r0, so the range test is the reference's FP64 sqrt(d2) < r0.
     if (p_is_i && (tyq == 2 || tyq == 3 || tyq == 5) &&
@@ -616,22 +618,38 @@ __global__ __launch_bounds__(PB) void k_pair_compact(int n, StepBufs b, const De
       bad = true;
       rl[r] = 0;
     }
-    const int lmax = wave_max(rl[r]);
+    // the trip count is the wave's maximum row length (uniform, in an SGPR); slots
+    // past this lane's row end are tested too and their bits cleared below
+    const int lmax = __builtin_amdgcn_readfirstlane(wave_max(rl[r]));
     const double2* rp = b.posA + rb[r];
+    uint32_t mw[2] = {0u, 0u};
+#pragma unroll
+    for (int hw = 0; hw < 2; ++hw) {
+      const int k0 = 32 * hw;
+      const int kend = hw == 0 ? (lmax < 32 ? lmax : 32) : lmax;
+      // candidate k0 + j enters as the low bit and is shifted up by each later one:
+      // one add-with-carry per candidate; bit-reversed into place after the loop
+      uint32_t bits = 0u;
 #pragma unroll 1
-    for (int k = 0; k < lmax; k += 4) {
-      double2 xs[4];
+      for (int k = k0; k < kend; k += 4) {
+        double2 xs[4];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) xs[u] = rp[u];   // posA has POS_PAD zeroed slots past n
-      rp += 4;
+        for (int u = 0; u < 4; ++u) xs[u] = rp[u];   // posA has POS_PAD zeroed slots past n
+        rp += 4;
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const double ex = xp.x - xs[u].x, ey = xp.y - xs[u].y;
-        const double d2 = __dadd_rn(__dmul_rn(ex, ex), __dmul_rn(ey, ey));
-        const bool in = (k + u < rl[r]) && (d2 < cut);
-        m[r] |= in ? (1ull << ((k + u) & 63)) : 0ull;
+        for (int u = 0; u < 4; ++u) {
+          const double ex = xp.x - xs[u].x, ey = xp.y - xs[u].y;
+          const double d2 = __dadd_rn(__dmul_rn(ex, ex), __dmul_rn(ey, ey));
+          bits = bits + bits + (d2 < cut ? 1u : 0u);
+        }
       }
+      const int nb = (kend - k0 + 3) & ~3;   // candidates tested in this half (uniform)
+      if (nb > 0) mw[hw] = __builtin_bitreverse32(bits) >> (32 - nb);
     }
+    const int l = rl[r];
+    mw[0] &= l >= 32 ? 0xffffffffu : ((1u << l) - 1u);
+    mw[1] &= l >= 64 ? 0xffffffffu : (l > 32 ? ((1u << (l - 32)) - 1u) : 0u);
+    m[r] = ((unsigned long long)mw[1] << 32) | mw[0];
   }
   if (!live) return;
   const double2 vn = b.velA[p];
@@ -650,47 +668,54 @@ __global__ __launch_bounds__(PB) void k_pair_compact(int n, StepBufs b, const De
   }
   PairCtx c;
   c.init(S, tg, b.mA[p], b.mB[p]);
-  // walk the set bits row by row; next slot found (and loaded) one pair ahead
+  // the set bits of the three rows as one stream, advanced without branches;
+  // -1 when exhausted
   int rr = 0;
   unsigned long long cm = m[0];
   int cb = rb[0];
   auto next = [&]() -> int {
-    if (cm == 0ull && rr == 0) {
-      rr = 1;
-      cm = m[1];
-      cb = rb[1];
-    }
-    if (cm == 0ull && rr == 1) {
-      rr = 2;
-      cm = m[2];
-      cb = rb[2];
-    }
-    if (cm == 0ull) return -1;
+    const bool a0 = (cm == 0ull) & (rr == 0);
+    cm = a0 ? m[1] : cm;
+    cb = a0 ? rb[1] : cb;
+    rr = a0 ? 1 : rr;
+    const bool a1 = (cm == 0ull) & (rr == 1);
+    cm = a1 ? m[2] : cm;
+    cb = a1 ? rb[2] : cb;
+    rr = a1 ? 2 : rr;
+    const bool none = cm == 0ull;
     const int k = __ffsll((long long)cm) - 1;
     cm &= cm - 1ull;
-    return cb + k;
+    return none ? -1 : cb + k;
   };
-  int q = next();
-  double2 xq;
-  float4 aq, bq;
-  if (q >= 0) {
-    xq = ld_off(b.posA, (uint32_t)q);
-    aq = ld_off(b.mA, (uint32_t)q);
-    bq = ld_off(b.mB, (uint32_t)q);
-  }
+  // two register sets in ping-pong: the next pair's loads are in flight while the
+  // current pair's physics runs (an exhausted stream loads slot p, unused)
+  struct Nb {
+    double2 x;
+    float4 a, b;
+  };
+  auto load = [&](Nb& o, int q) {
+    const uint32_t s = (uint32_t)(q >= 0 ? q : p);
+    o.x = ld_off(b.posA, s);
+    o.a = ld_off(b.mA, s);
+    o.b = ld_off(b.mB, s);
+  };
+  auto pair = [&](const Nb& o) {
+    const uint32_t tq = __float_as_uint(o.b.w);
+    if (c.template admit<false>(S, tag_id(tq), 0.0, 0.0, 0u, b, 0))
+      c.add(S, xp.x - o.x.x, xp.y - o.x.y, tq, o.a, o.b);
+  };
+  Nb n0, n1;
+  int q0 = next();
+  load(n0, q0);
 #pragma unroll 1
-  while (q >= 0) {
-    const double2 xc = xq;
-    const float4 ac = aq, bc = bq;
-    q = next();
-    if (q >= 0) {
-      xq = ld_off(b.posA, (uint32_t)q);
-      aq = ld_off(b.mA, (uint32_t)q);
-      bq = ld_off(b.mB, (uint32_t)q);
-    }
-    const uint32_t tq = __float_as_uint(bc.w);
-    if (!c.template admit<false>(S, tag_id(tq), 0.0, 0.0, 0u, b, 0)) continue;
-    c.add(S, xp.x - xc.x, xp.y - xc.y, tq, ac, bc);
+  while (q0 >= 0) {
+    const int q1 = next();
+    load(n1, q1);
+    pair(n0);
+    if (q1 < 0) break;
+    q0 = next();
+    load(n0, q0);
+    pair(n1);
   }
   correct(S, b, p, itime, xp, vn, mass, c);
 }
