@@ -97,7 +97,7 @@ typedef struct uph_scene {
 
 typedef struct uph_opts {
   int use_graph;        /* 1: replay each step as one captured hipGraph     */
-  int keep_rates;       /* 1: keep the last step's FP32 rates for uph_get_rates */
+  int keep_rates;       /* 1: keep the last step's rates for uph_get_rates */
   int pair_variant;     /* 0: compacted two-pass pair kernel (default), 1: direct loop */
   int pair_block;       /* work-group size of the compacted pair kernel: 128, 256 (default), 512 */
   double cell_factor;   /* cell edge = cell_factor x search radius, 1 <= f <= 4 (default 1) */
@@ -135,7 +135,8 @@ const char *uph_last_error(const uph_t *h);
  * Arrays are device pointers in the reference's order (pos/vel: 2n doubles).
  * uph_state_from_device loads the fluid rows (walls keep their state); with
  * restart != 0 the next step takes the first-step form of the leapfrog
- * (sph.cc:154-167) with localVelocity = velocity, and uph_itime restarts at 0.
+ * (sph.cc:154-167) with localVelocity = velocity, and uph_itime restarts at 0
+ * (the wall kinematics keep their clock).
  * Both calls, and uph_graph_build, synchronise the device before touching caller
  * buffers (which may be in use on any stream) and return with their results
  * complete. */

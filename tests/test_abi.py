@@ -12,8 +12,11 @@ from uph_amd import solver
 
 
 def _declared():
-    with open(os.path.join(ROOT, "include", "uph.h")) as f:
-        text = f.read()
+    text = ""
+    for h in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if h.endswith(".h"):
+            with open(os.path.join(ROOT, "include", h)) as f:
+                text += f.read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     names = re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", text)
     skip = {"if", "for", "while", "return", "sizeof"}
@@ -49,3 +52,45 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(solver, "_lib", None)
     with pytest.raises(RuntimeError):
         solver.lib()
+
+
+def _system_members():
+    """Member names of uph_system in declaration order."""
+    with open(os.path.join(ROOT, "include", "uph_system.h")) as f:
+        text = f.read()
+    body = text[text.index("typedef struct uph_system {"):text.index("} uph_system;")]
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S).split("{", 1)[1]
+    names = []
+    for decl in body.split(";"):
+        decl = decl.strip()
+        if not decl:
+            continue
+        decl = re.sub(r"^(const\s+)?(int|double)\s+", "", decl)
+        names += [p.strip().lstrip("*").strip() for p in decl.split(",")]
+    return names
+
+
+REF_SYSTEM_H = "/root/reference/src/lib/System.h"
+
+
+@pytest.mark.skipif(not os.path.exists(REF_SYSTEM_H), reason="reference tree not present")
+def test_system_view_is_binary_compatible_with_the_reference(tmp_path):
+    """Every member of uph_system sits at the offset the reference's System has."""
+    import subprocess
+    names = _system_members()
+    assert len(names) > 150 and names[0] == "dim" and names[-1] == "HowManyNeighbors"
+    src = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{REF_SYSTEM_H}"',
+           f'#include "{os.path.join(ROOT, "include", "uph_system.h")}"', "int main(void) {",
+           '  printf("%zu %zu\\n", sizeof(System), sizeof(uph_system));']
+    for n in names:
+        src.append(f'  printf("{n} %zu %zu\\n", offsetof(System, {n}), offsetof(uph_system, {n}));')
+    src.append("  return 0; }")
+    (tmp_path / "abi.c").write_text("\n".join(src) + "\n")
+    exe = tmp_path / "abi"
+    subprocess.run(["gcc", "-o", str(exe), str(tmp_path / "abi.c")], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    a, b = out[0].split()
+    assert a == b, ("sizeof", a, b)
+    bad = [l for l in out[1:] if l and l.split()[1] != l.split()[2]]
+    assert not bad, bad
+    assert len([l for l in out[1:] if l]) == len(names)

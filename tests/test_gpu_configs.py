@@ -160,3 +160,58 @@ def test_c_host_driver(tmp_path):
     # the driver prints with 10 decimals
     assert np.allclose(rows[:, 0:2], st["x"][:n], rtol=0, atol=6e-11)
     assert np.allclose(rows[:, 4], st["rho"][:n], rtol=0, atol=6e-11)
+
+
+def _deck_dir(tmp_path, fixture):
+    from uph_amd.scenes import write_deck
+    z = np.load(os.path.join(GOLDEN, fixture), allow_pickle=False)
+    vals = z["param_values"]
+    sc = load_golden_scene(fixture, alpha_av=0.1)
+    with open(tmp_path / "parameters.dat", "w") as f:
+        f.write("\n".join(repr(float(v)) if not float(v).is_integer() else str(int(v))
+                          for v in vals) + "\n")
+    write_deck(str(tmp_path / "FluidParticle.dat"), sc, "fluid")
+    write_deck(str(tmp_path / "FixedBoundaryParticle.dat"), sc, "fixed")
+    write_deck(str(tmp_path / "MovingBoundaryParticle.dat"), sc, "moving")
+    (tmp_path / "SolidParticle.dat").write_text("0\n")
+    return sc
+
+
+@pytest.mark.parametrize("fixture", ["hydrostatic_init.npz", "waves.npz"])
+def test_c_host_driver_compat_derivatives(tmp_path, fixture):
+    """The reference's host loop (sph.cc:88-168) in C over a System object, calling the
+    library's derivatives() each step, against the device-resident loop and the oracle."""
+    from oracle.oracle import Sim
+    from uph_amd.scenes import load_deck_dir
+    drv = os.path.join(ROOT, "uph_amd", "uph_run")
+    assert os.path.exists(drv), "uph_run not built"
+    _deck_dir(tmp_path, fixture)
+    steps = 100
+    _say(f"uph_run --compat {steps} on {fixture}")
+    out = subprocess.run([drv, str(steps), "--alpha", "0.1", "--compat", "--out", "compat.dat"],
+                         cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    _say(out.stdout.strip())
+    out = subprocess.run([drv, str(steps), "--alpha", "0.1", "--out", "device.dat"], cwd=tmp_path,
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    a = np.loadtxt(tmp_path / "compat.dat", skiprows=1)
+    b = np.loadtxt(tmp_path / "device.dat", skiprows=1)
+    sc = load_deck_dir(str(tmp_path), alpha_av=0.1)
+    dx = sc.dx
+    dpos = float(np.abs(a[:, 0:2] - b[:, 0:2]).max()) / dx
+    drho = float(np.max(np.abs(a[:, 4] - b[:, 4]) / b[:, 4]))
+    _say(f"compat vs device loop: max |dx|/dx {dpos:.2e}, max rel rho {drho:.2e}")
+    # same arithmetic (the System receives the FP64 acceleration the device corrector
+    # uses); the compat context picks its own cell box, so FP32 sums may run in
+    # another order
+    assert dpos < 1e-8 and drho < 1e-9
+    # and against the FP64 oracle on the same decks
+    o = Sim(sc, nthreads=0)
+    o.step(steps)
+    so = o.state()
+    n = sc.n_fluid
+    drho_o = float(np.max(np.abs(a[:, 4] - so["rho"][:n]) / so["rho"][:n]))
+    dpos_o = float(np.abs(a[:, 0:2] - so["x"][:n]).max()) / dx
+    _say(f"compat vs oracle: max rel rho {drho_o:.2e}, max |dx|/dx {dpos_o:.2e}")
+    assert drho_o < 1e-6 and dpos_o < 1e-4

@@ -168,6 +168,7 @@ __global__ __launch_bounds__(1024) void k_scan_top(int nb, int* __restrict__ par
   if (step_ctl && threadIdx.x == 0) {
     step_ctl[0] += 1;   // itime of the step being computed
     step_ctl[1] = 0;    // overflow flag of this step
+    step_ctl[3] += 1;   // stamp of this step's K-th keys
   }
 }
 
@@ -237,15 +238,16 @@ struct StepBufs {
   // sort scratch
   int *keys, *rank, *tmpIdx, *cellStart, *cellCount, *partial;
   int* sceneOf;
-  // capped search: K-th key of overflowing queries, valid when kthStep == itime
+  // capped search: K-th key of overflowing queries, valid when kthStep equals the
+  // step stamp ctl[3] (a counter that no restart resets, so a stamp never repeats)
   double* kthD2;
   uint32_t* kthId;
   int* kthStep;
   // per original index (global id)
   double2 *deckPos, *deckVel, *deckLvel;
-  // control: [0] itime, [1] overflow flag of the step, [2] overflowed steps
+  // control: [0] itime, [1] overflow flag of the step, [2] overflowed steps, [3] step stamp
   int* ctl;
-  float4* dbgRates;
+  float4* dbgRates;   // keep_rates: per slot {R, R~, a~x, a~y} (FP32), {ax, ay} (FP64)
 };
 
 // Final slot = cell start + number of same-cell particles with a smaller
@@ -294,7 +296,7 @@ __global__ __launch_bounds__(BLK) void k_reorder(int n, StepBufs b, const DevSce
     ly = l0.y;
     if (tag_type(tg) == 3 && S.motion != 0) {
       double xx = p.x, vv = v.x;
-      wall_motion(S, itime * dt, b.deckPos[my].x, xx, vv);
+      wall_motion(S, S.t0 + itime * dt, b.deckPos[my].x, xx, vv);
       lx = vv;
     }
   }
@@ -358,20 +360,20 @@ __device__ void kth_key(const DevScene& S, int p, const double2* __restrict__ po
 // Overflowing query (more than K candidates in range, or a row range the
 // compacted kernel cannot hold): its K-th (d2, id) key, stamped with the step,
 // and the step's flag that makes the exact kernel run.
-__device__ void mark_overflow(const DevScene& S, const StepBufs& b, int p, int itime) {
+__device__ void mark_overflow(const DevScene& S, const StepBufs& b, int p) {
   int cnt;
   double kd2;
   uint32_t kid;
   kth_key(S, p, b.posA, b.tagA, b.cellStart, cnt, kd2, kid);
   b.kthD2[p] = kd2;
   b.kthId[p] = kid;
-  b.kthStep[p] = itime;
+  b.kthStep[p] = b.ctl[3];
   atomicOr(&b.ctl[1], 1);
 }
 
 // K-th key of slot q in this step, or +inf when q did not overflow.
-__device__ __forceinline__ void kth_of(const StepBufs& b, int q, int itime, double& kd2, uint32_t& kid) {
-  if (b.kthStep[q] == itime) {
+__device__ __forceinline__ void kth_of(const StepBufs& b, int q, double& kd2, uint32_t& kid) {
+  if (b.kthStep[q] == b.ctl[3]) {
     kd2 = b.kthD2[q];
     kid = b.kthId[q];
   } else {
@@ -413,7 +415,7 @@ struct PairCtx {
       if (EXACT) {
         double kd2;
         uint32_t kid;
-        kth_of(b, q, b.ctl[0], kd2, kid);
+        kth_of(b, q, kd2, kid);
         if (!key_le(d2, idp, kd2, kid)) return false;
       }
     }
@@ -488,7 +490,7 @@ __device__ __forceinline__ void wall_update(const DevScene& S, const StepBufs& b
                                             float mass) {
   const uint32_t id = tag_id(tg);
   double2 x = b.deckPos[id], v = b.deckVel[id];
-  if (tag_type(tg) == 3 && S.motion != 0) wall_motion(S, (itime + 1) * S.dt, x.x, x.x, v.x);
+  if (tag_type(tg) == 3 && S.motion != 0) wall_motion(S, S.t0 + (itime + 1) * S.dt, x.x, x.x, v.x);
   b.posB[p] = x;
   b.velB[p] = v;
   b.rhoB[p] = b.rhoA[p];
@@ -519,8 +521,10 @@ __device__ __forceinline__ void correct(const DevScene& S, const StepBufs& b, in
   b.rhoB[p] = rho1;
   b.rtB[p] = make_float4(Rt, (float)atx, (float)aty, mass);
   if (b.dbgRates) {
-    b.dbgRates[2 * p] = make_float4(R, Rt, (float)ax, (float)ay);
-    b.dbgRates[2 * p + 1] = make_float4((float)atx, (float)aty, 0.f, 0.f);
+    // {R, R~, a~ (as the predictor reads them)} then the FP64 acceleration the
+    // corrector used, in the same 32 bytes
+    b.dbgRates[2 * p] = make_float4(R, Rt, (float)atx, (float)aty);
+    *reinterpret_cast<double2*>(b.dbgRates + 2 * p + 1) = make_double2(ax, ay);
   }
 }
 
@@ -543,7 +547,7 @@ __device__ __forceinline__ void pair_direct(int p, const StepBufs& b, const DevS
   const double cut = S.cutoff;
   double kpd2 = INFINITY;
   uint32_t kpid = 0xffffffffu;
-  if (EXACT) kth_of(b, p, itime, kpd2, kpid);
+  if (EXACT) kth_of(b, p, kpd2, kpid);
   int cnt = 0;
   for_candidates(S, xp, b.cellStart, [&](int q) {
     if (q == p) return;
@@ -558,7 +562,7 @@ __device__ __forceinline__ void pair_direct(int p, const StepBufs& b, const DevS
     c.add(S, ex, ey, tq, b.mA[q], bq);
   });
   if (!EXACT && cnt > S.K) {
-    mark_overflow(S, b, p, itime);
+    mark_overflow(S, b, p);
     return;
   }
   correct(S, b, p, itime, xp, vn, mass, c);
@@ -663,7 +667,7 @@ __global__ __launch_bounds__(PB) void k_pair_compact(int n, StepBufs b, const De
   if (self >= 0 && self < 64) m[1] &= ~(1ull << self);
   const int cnt = __popcll(m[0]) + __popcll(m[1]) + __popcll(m[2]);
   if (bad || cnt > S.K) {
-    mark_overflow(S, b, p, itime);
+    mark_overflow(S, b, p);
     return;   // the exact kernel recomputes this step
   }
   PairCtx c;
@@ -1188,7 +1192,8 @@ extern "C" int uph_get_state(uph_t* h, int scene, double* pos, double* vel, doub
   return 0;
 }
 
-extern "C" int uph_get_rates(uph_t* h, int scene, float* R, float* Rt, float* a, float* at) {
+// Rates of one scene's fluid rows in reference order; the acceleration in FP64.
+static int get_rates_f64(uph_t* h, int scene, float* R, float* Rt, double* a, float* at) {
   if (!h) return -1;
   if (!h->b.dbgRates) {
     h->err.msg = "uph_get_rates: handle created without keep_rates";
@@ -1211,9 +1216,22 @@ extern "C" int uph_get_rates(uph_t* h, int scene, float* R, float* Rt, float* a,
     if (i >= S.nf) continue;
     if (R) R[i] = d[2 * k].x;
     if (Rt) Rt[i] = d[2 * k].y;
-    if (a) { a[2 * i] = d[2 * k].z; a[2 * i + 1] = d[2 * k].w; }
-    if (at) { at[2 * i] = d[2 * k + 1].x; at[2 * i + 1] = d[2 * k + 1].y; }
+    if (at) { at[2 * i] = d[2 * k].z; at[2 * i + 1] = d[2 * k].w; }
+    if (a) {
+      double2 ad;
+      memcpy(&ad, &d[2 * k + 1], sizeof ad);
+      a[2 * i] = ad.x;
+      a[2 * i + 1] = ad.y;
+    }
   }
+  return 0;
+}
+
+extern "C" int uph_get_rates(uph_t* h, int scene, float* R, float* Rt, float* a, float* at) {
+  std::vector<double> ad;
+  if (a && h && scene >= 0 && scene < h->nscenes) ad.resize(2 * (size_t)h->hs[scene].nf);
+  if (get_rates_f64(h, scene, R, Rt, a ? ad.data() : nullptr, at)) return -1;
+  for (size_t k = 0; k < ad.size(); ++k) a[k] = (float)ad[k];
   return 0;
 }
 
@@ -1393,7 +1411,7 @@ __global__ void k_state_out(int poff, int np, const uint32_t* __restrict__ tag, 
 __global__ void k_state_in(int poff, int np, int nf, const uint32_t* __restrict__ tag, double2* __restrict__ pos,
                            double2* __restrict__ vel, double* __restrict__ rho, const double2* __restrict__ ipos,
                            const double2* __restrict__ ivel, const double* __restrict__ irho,
-                           double2* __restrict__ deckLvel, int restart) {
+                           const double2* __restrict__ ilvel, double2* __restrict__ deckLvel, int restart) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= np) return;
   const int s = poff + k;
@@ -1403,7 +1421,26 @@ __global__ void k_state_in(int poff, int np, int nf, const uint32_t* __restrict_
   if (ipos) pos[s] = ipos[i];
   if (ivel) vel[s] = ivel[i];
   if (irho) rho[s] = irho[i];
-  if (restart && ivel) deckLvel[g] = ivel[i];
+  if (restart && ilvel) deckLvel[g] = ilvel[i];
+  else if (restart && ivel) deckLvel[g] = ivel[i];
+}
+
+// Boundary slots of one scene at time t: the deck state plus the wall kinematics.
+__global__ void k_walls_at(int poff, int np, int nf, const uint32_t* __restrict__ tag, double2* __restrict__ pos,
+                           double2* __restrict__ vel, const double2* __restrict__ deckPos,
+                           const double2* __restrict__ deckVel, const DevScene* __restrict__ sc, int scene, double t) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= np) return;
+  const int s = poff + k;
+  const uint32_t tg = tag[s];
+  const uint32_t g = tag_id(tg);
+  const int i = (int)g - poff;
+  if (i < nf || i >= np) return;
+  const DevScene& S = sc[scene];
+  double2 x = deckPos[g], v = deckVel[g];
+  if (tag_type(tg) == 3 && S.motion != 0) wall_motion(S, t, x.x, x.x, v.x);
+  pos[s] = x;
+  vel[s] = v;
 }
 
 extern "C" int uph_state_to_device(uph_t* h, int scene, double* pos, double* vel, double* rho) {
@@ -1433,10 +1470,14 @@ extern "C" int uph_state_from_device(uph_t* h, int scene, const double* pos, con
   // the caller may have written pos/vel/rho on any stream
   HIPCHK(hipDeviceSynchronize(), h->err);
   hipLaunchKernelGGL(k_state_in, dim3(grid_of(S.np)), dim3(BLK), 0, h->stream, S.poff, S.np, S.nf, h->b.tagA,
-                     h->b.posB, h->b.velB, h->b.rhoB, (const double2*)pos, (const double2*)vel, rho, h->b.deckLvel,
-                     restart);
+                     h->b.posB, h->b.velB, h->b.rhoB, (const double2*)pos, (const double2*)vel, rho,
+                     (const double2*)nullptr, h->b.deckLvel, restart);
   HIPCHK(hipGetLastError(), h->err);
   if (restart) {
+    // the step counter restarts; the clock of the wall kinematics continues
+    for (auto& T : h->hs) T.t0 += h->itime * T.dt;
+    HIPCHK(hipMemcpyAsync(h->dsc, h->hs.data(), h->hs.size() * sizeof(DevScene), hipMemcpyHostToDevice, h->stream),
+           h->err);
     const int zero = 0;
     HIPCHK(hipMemcpyAsync(h->b.ctl, &zero, sizeof(int), hipMemcpyHostToDevice, h->stream), h->err);
     h->itime = 0;
@@ -1665,3 +1706,287 @@ extern "C" int uph_search_neighbors_cap(const double* pos_xy, int* i_pair, int* 
 }
 
 extern "C" const char* uph_search_last_error(void) { return g_search.err.msg.c_str(); }
+
+// ------------------------------------------------------ derivatives() compat
+//
+// The reference's rate routine (derivatives.c:3137) over its System object:
+// the fluid rows go to a device context (one scene), one step of the first-step
+// form runs with the caller's localVelocity, and the pair kernel's rates come
+// back into the System.  The boundary rows are reset to their state at the
+// first call (inputbp.c re-reads them from the decks every step) with the
+// piston kinematics of inputbp.c:194-207.
+
+#include "../../include/uph_system.h"
+
+namespace {
+struct CompatCtx {
+  std::mutex mu;
+  std::string err;
+  uph_t* h = nullptr;
+  const uph_system* sys = nullptr;
+  int nf = -1, nfix = -1, nmov = -1;
+  double hsml = 0.0, dt = 0.0;
+  std::vector<double> key;                         // constants the context was built with
+  std::vector<double2> deck_x, deck_v, deck_lv;   // boundary rows at the first call
+  std::vector<double> deck_rho, deck_p;
+  double2* dstage = nullptr;                       // pos, vel, lvel (nf each), then rho (nf doubles)
+  long stage_n = -1;
+  std::vector<double2> hpos, hvel, hlv;
+  std::vector<double> hrho;
+  std::vector<float> R, Rt, at;
+  std::vector<double> a;
+};
+CompatCtx g_compat;
+
+int compat_fail(const std::string& m) {
+  g_compat.err = m;
+  return -1;
+}
+
+double compat_tait(const uph_system* s, double rho) {
+  return s->CompressionFactor * (pow(rho / s->density0, s->AdiabaticConstant) - 1.0);
+}
+
+// Piston wall of inputbp.c:194-207 (movingBoundary 1); other modes keep the deck.
+void compat_wall(const uph_system* s, bool moving, double t, double2& x, double2& v, double2& lv) {
+  if (moving && s->movingBoundary == 1) {
+    x.x = 0.5 * s->stroke * (1.0 - cos(s->omega * t));
+    v.x = 0.5 * s->stroke * s->omega * sin(s->omega * t);
+    lv.x = v.x;
+  }
+}
+
+void compat_release() {
+  CompatCtx& c = g_compat;
+  if (c.h) uph_destroy(c.h);
+  if (c.dstage) (void)hipFree(c.dstage);
+  c.h = nullptr;
+  c.dstage = nullptr;
+  c.stage_n = -1;
+  c.sys = nullptr;
+}
+
+// Everything the device context depends on besides the particle rows.
+std::vector<double> compat_key(const uph_system* s) {
+  return {(double)s->NumberOfFluidParticles, (double)s->NumberOfFixedBoundaryParticles,
+          (double)s->NumberOfMovingBoundaryParticles, s->hsml[0], s->timeStep, s->dx, s->density0,
+          s->AdiabaticConstant, s->SpeedOfSound, s->CompressionFactor, s->Gravity, (double)s->gravityFlag,
+          s->alpha, s->springConstant0, s->dampingConstant0, (double)s->HowManyNeighbors,
+          (double)s->movingBoundary, s->stroke, s->omega};
+}
+
+int compat_build(const uph_system* s) {
+  CompatCtx& c = g_compat;
+  compat_release();
+  const int nf = s->NumberOfFluidParticles, nfix = s->NumberOfFixedBoundaryParticles;
+  const int nmov = s->NumberOfMovingBoundaryParticles, np = nf + nfix + nmov;
+  std::vector<double> pos(2 * (size_t)np), vel(2 * (size_t)np), lv(2 * (size_t)np), rho(np), mass(np);
+  std::vector<int> ity(np);
+  double lo[2] = {INFINITY, INFINITY}, hi[2] = {-INFINITY, -INFINITY};
+  for (int i = 0; i < np; ++i) {
+    for (int d = 0; d < 2; ++d) {
+      pos[2 * i + d] = s->Position[i][d];
+      vel[2 * i + d] = s->Velocity[i][d];
+      lv[2 * i + d] = s->localVelocity[i][d];
+      lo[d] = std::min(lo[d], pos[2 * i + d]);
+      hi[d] = std::max(hi[d], pos[2 * i + d]);
+    }
+    rho[i] = s->density[i];
+    mass[i] = s->mass[i];
+    ity[i] = s->itype[i];
+  }
+  if (!std::isfinite(lo[0]) || !std::isfinite(lo[1]) || !std::isfinite(hi[0]) || !std::isfinite(hi[1]))
+    return compat_fail("derivatives: non-finite positions");
+  uph_scene u;
+  memset(&u, 0, sizeof u);
+  u.n_fluid = nf;
+  u.n_fixed = nfix;
+  u.n_moving = nmov;
+  u.pos = pos.data();
+  u.vel = vel.data();
+  u.rho = rho.data();
+  u.mass = mass.data();
+  u.itype = ity.data();
+  u.lvel = lv.data();
+  u.dx = s->dx;
+  u.h = s->hsml[0];
+  u.rho0 = s->density0;
+  u.gamma = s->AdiabaticConstant;
+  u.c0 = s->SpeedOfSound;
+  u.B = s->CompressionFactor;
+  u.g = s->Gravity;
+  u.gravity = s->gravityFlag;
+  u.alpha_av = s->alpha;
+  u.k_spring = s->springConstant0;
+  u.c_damp = s->dampingConstant0;
+  u.dt = s->timeStep;
+  u.K = s->HowManyNeighbors > 0 ? s->HowManyNeighbors : 40;
+  u.motion = s->movingBoundary == 1 ? UPH_MOTION_PISTON : UPH_MOTION_NONE;
+  u.m_amp = s->stroke;
+  u.m_omega = s->omega;
+  // cell grid: the particles' box with a 10 h margin, twice the height above
+  const double mg = 10.0 * u.h;
+  u.box[0] = lo[0] - mg;
+  u.box[1] = lo[1] - mg;
+  u.box[2] = hi[0] + mg;
+  u.box[3] = hi[1] + 2.0 * (hi[1] - lo[1]) + mg;
+  uph_opts o;
+  memset(&o, 0, sizeof o);
+  o.keep_rates = 1;
+  o.pair_block = 256;
+  o.cell_factor = 1.0;
+  if (uph_create(&u, 1, &o, &c.h) != 0) return compat_fail(std::string("derivatives: ") + uph_last_error(nullptr));
+  if (hipMalloc((void**)&c.dstage, ((size_t)3 * nf + (nf + 1) / 2 + 1) * sizeof(double2)) != hipSuccess) {
+    compat_release();
+    return compat_fail("derivatives: device staging allocation failed");
+  }
+  c.stage_n = nf;
+  c.sys = s;
+  c.nf = nf;
+  c.nfix = nfix;
+  c.nmov = nmov;
+  c.hsml = u.h;
+  c.dt = u.dt;
+  c.key = compat_key(s);
+  const int nb = nfix + nmov;
+  c.deck_x.resize(nb);
+  c.deck_v.resize(nb);
+  c.deck_lv.resize(nb);
+  c.deck_rho.resize(nb);
+  c.deck_p.resize(nb);
+  for (int k = 0; k < nb; ++k) {
+    const int i = nf + k;
+    c.deck_x[k] = make_double2(s->Position[i][0], s->Position[i][1]);
+    c.deck_v[k] = make_double2(s->Velocity[i][0], s->Velocity[i][1]);
+    c.deck_lv[k] = make_double2(s->localVelocity[i][0], s->localVelocity[i][1]);
+    c.deck_rho[k] = s->density[i];
+    c.deck_p[k] = s->DynamicPressure ? s->DynamicPressure[i] : 0.0;
+  }
+  c.hpos.resize(nf);
+  c.hvel.resize(nf);
+  c.hlv.resize(nf);
+  c.hrho.resize(nf);
+  c.R.resize(nf);
+  c.Rt.resize(nf);
+  c.a.resize(2 * (size_t)nf);
+  c.at.resize(2 * (size_t)nf);
+  return 0;
+}
+}  // namespace
+
+extern "C" int uph_derivatives(uph_system* s, double t) {
+  CompatCtx& c = g_compat;
+  std::lock_guard<std::mutex> lk(c.mu);
+  c.err.clear();
+  if (!s) return compat_fail("derivatives: null System");
+  if (s->NumberOfSolidParticles != 0 || s->NumberOfFrozenParticles != 0 || s->NumberOfVirtualParticles != 0)
+    return compat_fail("derivatives: solid, frozen and virtual particles are not supported");
+  if (s->dim != 2) return compat_fail("derivatives: only dim = 2 is supported");
+  const int nf = s->NumberOfFluidParticles, nfix = s->NumberOfFixedBoundaryParticles;
+  const int nmov = s->NumberOfMovingBoundaryParticles;
+  if (nf < 0 || nfix < 0 || nmov < 0 || nf + nfix + nmov <= 0)
+    return compat_fail("derivatives: invalid particle counts");
+  if (!s->Position || !s->Velocity || !s->localVelocity || !s->density || !s->mass || !s->itype || !s->hsml ||
+      !s->ddensitydt || !s->ddensitydt_approx || !s->dvdt || !s->dvdt_approx)
+    return compat_fail("derivatives: a required System array is not allocated");
+  if (!(s->timeStep > 0) || !(s->hsml[0] > 0) || !(s->density0 > 0))
+    return compat_fail("derivatives: timeStep, hsml and density0 must be positive");
+  if (!c.h || c.sys != s || c.key != compat_key(s)) {
+    if (compat_build(s)) return -1;
+  }
+  uph_handle* h = c.h;
+  DevScene& S = h->hs[0];
+  // fluid rows in, with the caller's localVelocity; the step counter restarts so
+  // that the step takes the first-step form, at time t
+  for (int i = 0; i < nf; ++i) {
+    c.hpos[i] = make_double2(s->Position[i][0], s->Position[i][1]);
+    c.hvel[i] = make_double2(s->Velocity[i][0], s->Velocity[i][1]);
+    c.hlv[i] = make_double2(s->localVelocity[i][0], s->localVelocity[i][1]);
+    c.hrho[i] = s->density[i];
+  }
+  double2* dpos = c.dstage;
+  double2* dvel = c.dstage + nf;
+  double2* dlv = c.dstage + 2 * (size_t)nf;
+  double* drho = (double*)(c.dstage + 3 * (size_t)nf);
+  Err& err = h->err;
+  auto chk = [&](hipError_t e, const char* what) {
+    if (e != hipSuccess) {
+      c.err = std::string("derivatives: ") + what + ": " + hipGetErrorString(e);
+      return true;
+    }
+    return false;
+  };
+  if (nf > 0 && (chk(hipMemcpy(dpos, c.hpos.data(), nf * sizeof(double2), hipMemcpyHostToDevice), "upload") ||
+                 chk(hipMemcpy(dvel, c.hvel.data(), nf * sizeof(double2), hipMemcpyHostToDevice), "upload") ||
+                 chk(hipMemcpy(dlv, c.hlv.data(), nf * sizeof(double2), hipMemcpyHostToDevice), "upload") ||
+                 chk(hipMemcpy(drho, c.hrho.data(), nf * sizeof(double), hipMemcpyHostToDevice), "upload")))
+    return -1;
+  S.t0 = t - S.dt;
+  const int zero = 0;
+  if (chk(hipMemcpy(h->dsc, h->hs.data(), sizeof(DevScene), hipMemcpyHostToDevice), "scene upload") ||
+      chk(hipMemcpy(h->b.ctl, &zero, sizeof(int), hipMemcpyHostToDevice), "control upload"))
+    return -1;
+  h->itime = 0;
+  hipLaunchKernelGGL(k_state_in, dim3(grid_of(S.np)), dim3(BLK), 0, h->stream, S.poff, S.np, S.nf, h->b.tagA,
+                     h->b.posB, h->b.velB, h->b.rhoB, (const double2*)dpos, (const double2*)dvel,
+                     (const double*)drho, (const double2*)dlv, h->b.deckLvel, 1);
+  hipLaunchKernelGGL(k_walls_at, dim3(grid_of(S.np)), dim3(BLK), 0, h->stream, S.poff, S.np, S.nf, h->b.tagA,
+                     h->b.posB, h->b.velB, h->b.deckPos, h->b.deckVel, h->dsc, 0, t);
+  if (chk(hipGetLastError(), "launch") || launch_step(h, nullptr, nullptr)) {
+    if (c.err.empty()) c.err = "derivatives: " + err.msg;
+    return -1;
+  }
+  if (get_rates_f64(h, 0, c.R.data(), c.Rt.data(), c.a.data(), c.at.data()) != 0)
+    return compat_fail("derivatives: " + err.msg);
+  // rates and the equation of state for the fluid rows
+  const double B = s->CompressionFactor, gm = s->AdiabaticConstant, r0 = s->density0;
+  for (int i = 0; i < nf; ++i) {
+    s->ddensitydt[i] = c.R[i];
+    s->ddensitydt_approx[i] = c.Rt[i];
+    s->dvdt[i][0] = c.a[2 * i];
+    s->dvdt[i][1] = c.a[2 * i + 1];
+    s->dvdt_approx[i][0] = c.at[2 * i];
+    s->dvdt_approx[i][1] = c.at[2 * i + 1];
+    const double rho = s->density[i];
+    const double P = compat_tait(s, rho);
+    if (s->Pressure) s->Pressure[i] = P;
+    if (s->DynamicPressure) s->DynamicPressure[i] = P;
+    const double dPdrho = B * gm / r0 * pow(rho / r0, gm - 1.0);
+    if (s->dpdt) s->dpdt[i] = dPdrho * c.R[i];
+    if (s->dpdt_approx) s->dpdt_approx[i] = dPdrho * c.Rt[i];
+    if (s->densityFluctuation) s->densityFluctuation[i] = 0.0;
+    if (s->pressureFluctuation) s->pressureFluctuation[i] = 0.0;
+    if (s->velocityFluctuation) s->velocityFluctuation[i][0] = s->velocityFluctuation[i][1] = 0.0;
+  }
+  // boundary rows: the deck at time t (inputbp.c:117-150, 194-207, 288)
+  for (int k = 0; k < nfix + nmov; ++k) {
+    const int i = nf + k;
+    double2 x = c.deck_x[k], v = c.deck_v[k], lv = c.deck_lv[k];
+    compat_wall(s, k >= nfix, t, x, v, lv);
+    s->Position[i][0] = x.x;
+    s->Position[i][1] = x.y;
+    s->Velocity[i][0] = v.x;
+    s->Velocity[i][1] = v.y;
+    s->localVelocity[i][0] = lv.x;
+    s->localVelocity[i][1] = lv.y;
+    s->density[i] = c.deck_rho[k];
+    if (s->DynamicPressure) s->DynamicPressure[i] = c.deck_p[k];
+    if (s->Pressure) s->Pressure[i] = compat_tait(s, c.deck_rho[k]);
+    if (s->localDensity) s->localDensity[i] = r0;
+  }
+  return 0;
+}
+
+extern "C" void derivatives(uph_system* s, double t) {
+  if (uph_derivatives(s, t) == 0) return;
+  fprintf(stderr, "%s\n", uph_derivatives_last_error());
+  if (!s || !s->ddensitydt || !s->dvdt) return;
+  for (int i = 0; i < s->NumberOfFluidParticles; ++i) {
+    s->ddensitydt[i] = NAN;
+    s->dvdt[i][0] = s->dvdt[i][1] = NAN;
+    if (s->ddensitydt_approx) s->ddensitydt_approx[i] = NAN;
+    if (s->dvdt_approx) s->dvdt_approx[i][0] = s->dvdt_approx[i][1] = NAN;
+  }
+}
+
+extern "C" const char* uph_derivatives_last_error(void) { return g_compat.err.c_str(); }

@@ -28,6 +28,7 @@ struct DevScene {
   double cutoff;            // squared search radius, (h*h)*4
   double h, dt, rho0, gamma, B, c0, g, r0;
   double m_amp, m_omega;
+  double t0;                // time of step 0: wall kinematics run at t0 + itime * dt
   // FP32 copies for the pair sums
   float fh, finvh, fwalpha, fav, fc0, fk, fcd, fr0, frho0;
 };

@@ -7,8 +7,12 @@
  *   FixedBoundaryParticle.dat, MovingBoundaryParticle.dat, SolidParticle.dat
  * (12-column rows, input.c:57-72 / inputbp.c:88-150).
  *
- *   uph_run <nsteps> [--fluid FILE] [--h-factor F] [--alpha A] [--graph] [--out FILE]
+ *   uph_run <nsteps> [--fluid FILE] [--h-factor F] [--alpha A] [--graph] [--compat] [--out FILE]
  *
+ * --compat runs the time loop on the host in the shape of sph.cc:88-168 over a
+ * System object (include/uph_system.h) and calls the library's derivatives()
+ * once per step, as the reference's own main does; without it the whole loop
+ * runs on the device through uph_step.
  * Constants follow System.c:292-313: c = sqrt(g H)/Mach (sqrt(2 g H)/Mach when
  * movingBoundary == 2), B = rho0 c^2 / gamma, eta = sqrt(Nn / 4 pi), hsml = eta dx
  * (input.c:107) unless --h-factor sets h = F dx.  The scheme is the library's:
@@ -24,6 +28,7 @@
 #include <time.h>
 
 #include "../../include/uph.h"
+#include "../../include/uph_system.h"
 
 #ifndef M_PI
 #define M_PI 3.14159265358979323846
@@ -75,6 +80,92 @@ static void free_deck(deck_t *d) {
   free(d->itype);
 }
 
+/* n rows of 2 doubles over one zeroed block (the reference's CreateMatrix layout) */
+static double **rows2(int n) {
+  size_t m = n > 0 ? (size_t)n : 1;
+  double **r = malloc(m * sizeof(double *));
+  double *b = calloc(2 * m, sizeof(double));
+  for (size_t i = 0; i < m; i++) r[i] = b + 2 * i;
+  return r;
+}
+
+static void free_rows2(double **r) {
+  if (r) free(r[0]);
+  free(r);
+}
+
+static double *vec(int n) { return calloc(n > 0 ? (size_t)n : 1, sizeof(double)); }
+
+/* The host time loop of sph.cc:88-168 with the fluctuation fields zero (the
+ * config-mode scheme) and derivatives() from the library.  On return s holds
+ * the state after nsteps. */
+static int run_compat(uph_system *s, int nsteps) {
+  const int nf = s->NumberOfFluidParticles, np = s->MaxNumberOfParticles;
+  const double dt = s->timeStep;
+  for (int itime = 1; itime <= nsteps; itime++) {
+    const double t = itime * dt;
+    if (itime != 1) {
+      /* predictor (sph.cc:91-126) */
+      memcpy(s->density_min, s->density, sizeof(double) * nf);
+      memcpy(s->DynamicPressure_min, s->DynamicPressure, sizeof(double) * nf);
+      memcpy(s->localDensity, s->density, sizeof(double) * nf);
+      memcpy(s->localPressure, s->DynamicPressure, sizeof(double) * nf);
+      memcpy(s->localVelocity[0], s->Velocity[0], sizeof(double) * 2 * np);
+      memcpy(s->Velocity_min[0], s->Velocity[0], sizeof(double) * 2 * np);
+      for (int i = 0; i < nf; i++) {
+        s->density[i] += 0.5 * dt * s->ddensitydt_approx[i];
+        s->localDensity[i] = s->localDensity[i] + 0.5 * dt * s->ddensitydt_approx[i];
+        s->DynamicPressure[i] += 0.5 * dt * s->dpdt_approx[i];
+        s->localPressure[i] = s->localPressure[i] + 0.5 * dt * s->dpdt_approx[i];
+      }
+      for (int i = 0; i < nf; i++)
+        for (int d = 0; d < 2; d++) {
+          s->Velocity_min[i][d] = s->Velocity[i][d];
+          s->Velocity_xsph[i][d] += 0.5 * dt * s->dvdt[i][d];
+          s->Velocity[i][d] += 0.5 * dt * s->dvdt_approx[i][d];
+          s->localVelocity[i][d] = s->Velocity[i][d] + 0.5 * dt * s->dvdt_approx[i][d];
+        }
+    } else {
+      memcpy(s->Velocity_xsph[0], s->Velocity[0], sizeof(double) * 2 * np);
+    }
+    if (uph_derivatives(s, t) != 0) {
+      fprintf(stderr, "uph_run: %s\n", uph_derivatives_last_error());
+      return -1;
+    }
+    /* corrector (sph.cc:137-168) */
+    if (itime == 1) {
+      for (int i = 0; i < nf; i++) {
+        s->localDensity[i] = s->density[i] + 0.5 * dt * s->ddensitydt[i];
+        s->density[i] += 0.5 * dt * s->ddensitydt[i];
+        s->localPressure[i] = s->DynamicPressure[i] + 0.5 * dt * s->dpdt[i];
+        s->DynamicPressure[i] += 0.5 * dt * s->dpdt[i];
+      }
+      for (int i = 0; i < nf; i++)
+        for (int d = 0; d < 2; d++) {
+          s->Velocity_xsph[i][d] = s->Velocity[i][d] + 0.5 * dt * s->dvdt[i][d];
+          s->localVelocity[i][d] = s->Velocity[i][d] + 0.5 * dt * s->dvdt[i][d];
+          s->Velocity[i][d] += 0.5 * dt * s->dvdt[i][d];
+          s->Position[i][d] += dt * s->Velocity[i][d];
+        }
+    } else {
+      for (int i = 0; i < nf; i++) {
+        s->localDensity[i] = s->density_min[i] + dt * s->ddensitydt[i];
+        s->density[i] = s->density_min[i] + dt * s->ddensitydt[i];
+        s->localPressure[i] = s->DynamicPressure_min[i] + dt * s->dpdt[i];
+        s->DynamicPressure[i] = s->DynamicPressure_min[i] + dt * s->dpdt[i];
+      }
+      for (int i = 0; i < nf; i++)
+        for (int d = 0; d < 2; d++) {
+          s->Velocity_xsph[i][d] = s->Velocity_min[i][d] + dt * s->dvdt[i][d];
+          s->localVelocity[i][d] = s->Velocity_min[i][d] + dt * s->dvdt[i][d];
+          s->Velocity[i][d] = s->Velocity_min[i][d] + dt * s->dvdt[i][d];
+          s->Position[i][d] += dt * s->Velocity[i][d];
+        }
+    }
+  }
+  return 0;
+}
+
 static double now_s(void) {
   struct timespec ts;
   clock_gettime(CLOCK_MONOTONIC, &ts);
@@ -89,13 +180,14 @@ int main(int argc, char **argv) {
   int nsteps = atoi(argv[1]);
   const char *fluid_file = "FluidParticle.dat", *out_file = "FluidParticleFinal.dat";
   double h_factor = 0.0, alpha = -1.0;
-  int use_graph = 0;
+  int use_graph = 0, compat = 0;
   for (int a = 2; a < argc; a++) {
     if (!strcmp(argv[a], "--fluid") && a + 1 < argc) fluid_file = argv[++a];
     else if (!strcmp(argv[a], "--h-factor") && a + 1 < argc) h_factor = atof(argv[++a]);
     else if (!strcmp(argv[a], "--alpha") && a + 1 < argc) alpha = atof(argv[++a]);
     else if (!strcmp(argv[a], "--out") && a + 1 < argc) out_file = argv[++a];
     else if (!strcmp(argv[a], "--graph")) use_graph = 1;
+    else if (!strcmp(argv[a], "--compat")) compat = 1;
     else {
       fprintf(stderr, "uph_run: unknown option %s\n", argv[a]);
       return 1;
@@ -167,6 +259,72 @@ int main(int argc, char **argv) {
   const double margin = 10 * s.h + (moving == 1 ? stroke : 0.0);
   s.box[0] = lo[0] - margin; s.box[1] = lo[1] - margin;
   s.box[2] = hi[0] + margin; s.box[3] = hi[1] + 2 * (hi[1] - lo[1]);
+  if (compat) {
+    /* a System as the reference's input() leaves it (System.c, input.c, inputbp.c) */
+    uph_system sy;
+    memset(&sy, 0, sizeof sy);
+    sy.dim = 2;
+    sy.NumberOfFluidParticles = fl.n;
+    sy.NumberOfFixedBoundaryParticles = fx.n;
+    sy.NumberOfMovingBoundaryParticles = mv.n;
+    sy.NumberOfBoundaryParticles = fx.n + mv.n;
+    sy.MaxNumberOfParticles = n;
+    sy.Position = rows2(n); sy.Velocity = rows2(n); sy.localVelocity = rows2(n);
+    sy.Velocity_min = rows2(n); sy.Velocity_xsph = rows2(n);
+    sy.dvdt = rows2(n); sy.dvdt_approx = rows2(n); sy.velocityFluctuation = rows2(n);
+    sy.mass = vec(n); sy.density = vec(n); sy.density_min = vec(n); sy.localDensity = vec(n);
+    sy.Pressure = vec(n); sy.DynamicPressure = vec(n); sy.DynamicPressure_min = vec(n);
+    sy.localPressure = vec(n); sy.dpdt = vec(n); sy.dpdt_approx = vec(n); sy.hsml = vec(n);
+    sy.ddensitydt = vec(n); sy.ddensitydt_approx = vec(n);
+    sy.densityFluctuation = vec(n); sy.pressureFluctuation = vec(n);
+    sy.itype = calloc(n, sizeof(int));
+    o = 0;
+    for (int p = 0; p < 3; p++)
+      for (int i = 0; i < parts[p]->n; i++, o++) {
+        for (int d = 0; d < 2; d++) {
+          sy.Position[o][d] = parts[p]->pos[2 * i + d];
+          sy.Velocity[o][d] = parts[p]->vel[2 * i + d];
+          sy.localVelocity[o][d] = parts[p]->lvel[2 * i + d];
+        }
+        sy.density[o] = parts[p]->rho[i];
+        sy.localDensity[o] = parts[p]->rho[i];
+        sy.DynamicPressure[o] = parts[p]->pres[i];
+        sy.mass[o] = parts[p]->mass[i];
+        sy.itype[o] = parts[p]->itype[i];
+        sy.hsml[o] = s.h;
+      }
+    sy.dx = dx;
+    sy.density0 = rho0; sy.AdiabaticConstant = gamma; sy.Gravity = g; sy.gravityFlag = gravity;
+    sy.SpeedOfSound = c; sy.CompressionFactor = s.B; sy.alpha = s.alpha_av;
+    sy.springConstant0 = k0; sy.dampingConstant0 = c0d; sy.timeStep = dt; sy.HowManyNeighbors = 40;
+    sy.movingBoundary = moving; sy.stroke = stroke; sy.wavePeriod = period; sy.omega = 2.0 * M_PI / period;
+    double tc = now_s();
+    if (run_compat(&sy, nsteps) != 0) return 1;
+    double elc = now_s() - tc;
+    printf("%d fluid + %d wall particles, %d steps in %.3f s (host loop, derivatives() on the GPU): "
+           "%.4g particle-updates/s\n", fl.n, fx.n + mv.n, nsteps, elc, elc > 0 ? (double)fl.n * nsteps / elc : 0.0);
+    FILE *fc = fopen(out_file, "w");
+    if (!fc) {
+      fprintf(stderr, "uph_run: cannot write %s\n", out_file);
+      return 1;
+    }
+    fprintf(fc, "%d\n", fl.n);
+    for (int i = 0; i < fl.n; i++)
+      fprintf(fc, "%.10lf %.10lf %.10lf %.10lf %.10lf %.10lf %.10lf %.10lf %.10lf %d %.10lf %.10lf\n",
+              sy.Position[i][0], sy.Position[i][1], sy.Velocity[i][0], sy.Velocity[i][1], sy.density[i],
+              sy.DynamicPressure[i], sy.mass[i], 0.0, 0.0, 1, sy.localVelocity[i][0], sy.localVelocity[i][1]);
+    fclose(fc);
+    free_rows2(sy.Position); free_rows2(sy.Velocity); free_rows2(sy.localVelocity);
+    free_rows2(sy.Velocity_min); free_rows2(sy.Velocity_xsph); free_rows2(sy.dvdt);
+    free_rows2(sy.dvdt_approx); free_rows2(sy.velocityFluctuation);
+    free(sy.mass); free(sy.density); free(sy.density_min); free(sy.localDensity); free(sy.Pressure);
+    free(sy.DynamicPressure); free(sy.DynamicPressure_min); free(sy.localPressure); free(sy.dpdt);
+    free(sy.dpdt_approx); free(sy.hsml); free(sy.ddensitydt); free(sy.ddensitydt_approx);
+    free(sy.densityFluctuation); free(sy.pressureFluctuation); free(sy.itype);
+    free(all.pos); free(all.vel); free(all.lvel); free(all.rho); free(all.mass); free(all.itype);
+    free_deck(&fl); free_deck(&fx); free_deck(&mv); free_deck(&so);
+    return 0;
+  }
   uph_opts opts = {use_graph, 0, 0, 256, 1.0};
   uph_t *h = NULL;
   if (uph_create(&s, 1, &opts, &h) != 0) {

@@ -113,6 +113,9 @@ SIGNATURES = {
                                   C.c_void_p, C.c_void_p, C.c_void_p, C.c_long,
                                   C.POINTER(C.c_long)]),
     "uph_graph_last_error": (C.c_char_p, []),
+    "uph_derivatives": (C.c_int, [C.c_void_p, C.c_double]),
+    "derivatives": (None, [C.c_void_p, C.c_double]),
+    "uph_derivatives_last_error": (C.c_char_p, []),
 }
 
 _lib = None
