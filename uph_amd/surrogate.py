@@ -23,9 +23,10 @@ target is the second difference of positions, (x[t+1] - 2 x[t] + x[t-1]) / Dt^2,
 with Gaussian noise on the input velocities (the target is corrected so that
 the noisy state still lands on the true next position).
 
-Scope: scenes with static walls (the correction restarts the plain solver's
-clock, so moving walls would lose their phase).  No counterpart in the
-reference; the plain path it wraps follows sph.cc:90-168.
+Scope: scenes with static walls (the network advances fluid particles only,
+and the velocity history of a wall would have to come from its kinematics).
+No counterpart in the reference; the plain path it wraps follows
+sph.cc:90-168.
 """
 from __future__ import annotations
 
