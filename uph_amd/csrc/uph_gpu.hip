@@ -5,9 +5,9 @@
 // scenes of a batch at once:
 //   k_bin         cell key of every particle, rank inside its cell
 //                 (wavefront-aggregated atomics)
-//   k_scan_reduce per-tile cell totals
-//   k_scan_top    exclusive scan of the tile totals; opens the step
-//   k_scan_down   exclusive scan of cell counts -> cell start offsets
+//   k_scan_reduce per-tile cell totals; opens the step
+//   k_scan_down   exclusive scan of cell counts -> cell start offsets (each tile
+//                 adds up the totals of the tiles before it)
 //   k_scatter     provisional sorted slot of every particle
 //   k_reorder     final slot by (cell, original index), gather of the FP64
 //                 state, predictor and FP64 Tait pressure -> FP32 neighbour mirror
@@ -110,8 +110,16 @@ __device__ int block_scan_tile(int* vals /* SCAN_ITEMS per thread, in registers 
   return total;
 }
 
-__global__ __launch_bounds__(SCAN_BLK) void k_scan_reduce(int m, const int* __restrict__ in, int* __restrict__ partial) {
+// Per-tile totals of `in`.  With `step_ctl` it also opens a step: itime += 1,
+// the overflow flag cleared, the K-th key stamp advanced.
+__global__ __launch_bounds__(SCAN_BLK) void k_scan_reduce(int m, const int* __restrict__ in, int* __restrict__ partial,
+                                                          int* __restrict__ step_ctl) {
   __shared__ int lds[SCAN_BLK / 64 + 1];
+  if (step_ctl && blockIdx.x == 0 && threadIdx.x == 0) {
+    step_ctl[0] += 1;   // itime of the step being computed
+    step_ctl[1] = 0;    // overflow flag of this step
+    step_ctl[3] += 1;   // stamp of this step's K-th keys
+  }
   const long base = (long)blockIdx.x * SCAN_TILE + threadIdx.x;
   int s = 0;
 #pragma unroll
@@ -128,47 +136,6 @@ __global__ __launch_bounds__(SCAN_BLK) void k_scan_reduce(int m, const int* __re
     int t = 0;
     for (int w = 0; w < SCAN_BLK / 64; ++w) t += lds[w];
     partial[blockIdx.x] = t;
-  }
-}
-
-// Single block: exclusive scan of the per-tile partials.  Also opens a step
-// (itime += 1, overflow flag cleared) when `step_ctl` is given.
-__global__ __launch_bounds__(1024) void k_scan_top(int nb, int* __restrict__ partial, int* __restrict__ step_ctl) {
-  __shared__ int lds[1024 / 64 + 1];
-  __shared__ int carry;
-  if (threadIdx.x == 0) carry = 0;
-  __syncthreads();
-  for (int b0 = 0; b0 < nb; b0 += 1024) {
-    const int i = b0 + threadIdx.x;
-    const int v = i < nb ? partial[i] : 0;
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    int x = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const int y = __shfl_up(x, o, 64);
-      if (lane >= o) x += y;
-    }
-    if (lane == 63) lds[wid] = x;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      int acc = 0;
-      for (int w = 0; w < 16; ++w) {
-        const int t = lds[w];
-        lds[w] = acc;
-        acc += t;
-      }
-      lds[16] = acc;
-    }
-    __syncthreads();
-    if (i < nb) partial[i] = carry + lds[wid] + x - v;
-    __syncthreads();
-    if (threadIdx.x == 0) carry += lds[16];
-    __syncthreads();
-  }
-  if (step_ctl && threadIdx.x == 0) {
-    step_ctl[0] += 1;   // itime of the step being computed
-    step_ctl[1] = 0;    // overflow flag of this step
-    step_ctl[3] += 1;   // stamp of this step's K-th keys
   }
 }
 
@@ -199,7 +166,16 @@ __global__ __launch_bounds__(SCAN_BLK) void k_scan_down(int m, int* __restrict__
 #pragma unroll
   for (int k = 0; k < SCAN_ITEMS; ++k) v[k] = tile[tile_ix(tid * SCAN_ITEMS + k)];
   block_scan_tile(v, lds);
-  const int off = partial[blockIdx.x];
+  // offset of this tile: the sum of the preceding tiles' totals
+  int pre = 0;
+  for (int t = tid; t < (int)blockIdx.x; t += SCAN_BLK) pre += partial[t];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) pre += __shfl_down(pre, o, 64);
+  if ((tid & 63) == 0) lds[tid >> 6] = pre;
+  __syncthreads();
+  int off = 0;
+#pragma unroll
+  for (int w = 0; w < SCAN_BLK / 64; ++w) off += lds[w];
 #pragma unroll
   for (int k = 0; k < SCAN_ITEMS; ++k) {
     const int e = tid * SCAN_ITEMS + k;
@@ -910,8 +886,8 @@ static int launch_sort(hipStream_t st, int n, int ncells, int nb, StepBufs& b, c
                        bool step_ctl, Err& err) {
   hipLaunchKernelGGL(k_bin, dim3(grid_of(n)), dim3(BLK), 0, st, n, b.posB, b.sceneOf, dsc, b.tagA, b.tagB, b.keys,
                      b.rank, b.cellCount);
-  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_BLK), 0, st, ncells, b.cellCount, b.partial);
-  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, nb, b.partial, step_ctl ? b.ctl : (int*)nullptr);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_BLK), 0, st, ncells, b.cellCount, b.partial,
+                     step_ctl ? b.ctl : (int*)nullptr);
   hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(SCAN_BLK), 0, st, ncells, b.cellCount, b.partial, b.cellStart);
   hipLaunchKernelGGL(k_scatter, dim3(grid_of(n)), dim3(BLK), 0, st, n, b.keys, b.rank, b.cellStart, b.tmpIdx);
   hipLaunchKernelGGL(k_reorder, dim3(grid_of(n)), dim3(BLK), 0, st, n, b, dsc, search_only);
@@ -1350,8 +1326,7 @@ int search_run(const double* const* rows, const double* flat, int* i_pair, int* 
   hipLaunchKernelGGL(k_search_count, dim3(grid8_of(n)), dim3(BLK), 0, st, nf, n, c.b, c.dsc, c.qcount);
   // exclusive scan of per-query counts (nf entries) into qoff
   const int qnb = (nf + SCAN_TILE - 1) / SCAN_TILE;
-  hipLaunchKernelGGL(k_scan_reduce, dim3(qnb), dim3(SCAN_BLK), 0, st, nf, c.qcount, c.qpartial);
-  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, qnb, c.qpartial, (int*)nullptr);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(qnb), dim3(SCAN_BLK), 0, st, nf, c.qcount, c.qpartial, (int*)nullptr);
   hipLaunchKernelGGL(k_scan_down, dim3(qnb), dim3(SCAN_BLK), 0, st, nf, c.qcount, c.qpartial, c.qoff);
   HIPCHK(hipGetLastError(), c.err);
   int niac = 0;
@@ -1667,8 +1642,7 @@ extern "C" int uph_graph_build(const double* pos, int n, int n_recv, double dist
   const int qnb = (n + SCAN_TILE - 1) / SCAN_TILE;
   // receivers
   hipLaunchKernelGGL(k_graph_count, dim3(grid8_of(n)), dim3(BLK), 0, st, n, c.b, c.dsc, n_recv, c.cnt, 0);
-  hipLaunchKernelGGL(k_scan_reduce, dim3(qnb), dim3(SCAN_BLK), 0, st, n, c.cnt, c.qpartial);
-  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, qnb, c.qpartial, (int*)nullptr);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(qnb), dim3(SCAN_BLK), 0, st, n, c.cnt, c.qpartial, (int*)nullptr);
   hipLaunchKernelGGL(k_scan_down, dim3(qnb), dim3(SCAN_BLK), 0, st, n, c.cnt, c.qpartial, off);
   int ne = 0;
   HIPCHK(hipMemcpyAsync(&ne, off + n, sizeof(int), hipMemcpyDeviceToHost, st), c.err);
@@ -1681,8 +1655,7 @@ extern "C" int uph_graph_build(const double* pos, int n, int n_recv, double dist
   hipLaunchKernelGGL(k_graph_emit, dim3(grid8_of(n)), dim3(BLK), 0, st, n, c.b, c.dsc, n_recv, off, send, recv);
   // senders
   hipLaunchKernelGGL(k_graph_count, dim3(grid8_of(n)), dim3(BLK), 0, st, n, c.b, c.dsc, n_recv, c.cnt, 1);
-  hipLaunchKernelGGL(k_scan_reduce, dim3(qnb), dim3(SCAN_BLK), 0, st, n, c.cnt, c.qpartial);
-  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, qnb, c.qpartial, (int*)nullptr);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(qnb), dim3(SCAN_BLK), 0, st, n, c.cnt, c.qpartial, (int*)nullptr);
   hipLaunchKernelGGL(k_scan_down, dim3(qnb), dim3(SCAN_BLK), 0, st, n, c.cnt, c.qpartial, soff);
   hipLaunchKernelGGL(k_graph_sender_perm, dim3(grid8_of(n)), dim3(BLK), 0, st, n, c.b, c.dsc, n_recv, off, send,
                      soff, sperm);
