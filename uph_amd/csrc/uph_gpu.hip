@@ -363,7 +363,9 @@ struct PairCtx {
   uint32_t idp;
   bool pb_ok;
   float vpx, vpy, lpx, lpy, rhop, prp, mp, inv_mp;
-  float h, invh, walpha, h2eps, cav;
+  float invh, h2eps, cav;
+  float fa, fb, fw;     // cubic-spline gradient factor: fa + fb q (q < 1), fw (2 - q)^2 / r (1 <= q < 2)
+  int pb;               // ParticleBoundary as a global index
   CSum sR, sRt, sAx, sAy;   // sAx/sAy: the predicting acceleration (dvdt_approx)
   float dAx = 0.f, dAy = 0.f;   // dvdt - dvdt_approx: wall pairs with localDensity != density
 
@@ -373,9 +375,14 @@ struct PairCtx {
     vpx = ap.x; vpy = ap.y; lpx = ap.z; lpy = ap.w;
     rhop = bp.x; prp = bp.y; mp = bp.z;
     inv_mp = 1.0f / mp;
-    h = S.fh; invh = S.finvh; walpha = S.fwalpha;
+    const float h = S.fh, walpha = S.fwalpha;
+    invh = S.finvh;
     h2eps = 0.01f * h * h;
     cav = -2.0f * S.fav * S.fc0 * h;
+    fa = -3.0f * walpha * invh * invh;
+    fb = 2.25f * walpha * invh * invh;
+    fw = -0.75f * walpha * invh;
+    pb = S.pb_gid;
   }
 
   // Admission of an in-range pair by the rules of Search.cc:33 (original index
@@ -383,6 +390,7 @@ struct PairCtx {
   template <bool EXACT>
   __device__ __forceinline__ bool admit(const DevScene& S, uint32_t idq, double d2, double kpd2, uint32_t kpid,
                                         const StepBufs& b, int q) const {
+    if (!EXACT) return idq > idp ? (int)idq < pb : pb_ok;   // a select, no branch
     if (idq > idp) {
       if ((int)idq >= S.pb_gid) return false;
       if (EXACT && !key_le(d2, idq, kpd2, kpid)) return false;
@@ -408,14 +416,12 @@ struct PairCtx {
     const float rinv = __builtin_amdgcn_rsqf(r2);
     const float r = r2 * rinv;
     const float qq = r * invh;
-    // cubic spline gradient factor (derivatives.c:44-66): gradW = f * (x_p - x_q)
-    float f;
-    if (qq < 1.0f) {
-      f = walpha * (-3.0f + 2.25f * qq) * invh * invh;
-    } else {
-      const float t = 2.0f - qq;
-      f = -walpha * 0.75f * t * t * invh * rinv;
-    }
+    // cubic spline gradient factor (derivatives.c:44-66): gradW = f * (x_p - x_q);
+    // both branches evaluated and selected (lanes of a wavefront take both anyway)
+    const float t = 2.0f - qq;
+    const float f_far = fw * t * t * rinv;
+    const float f_near = fa + fb * qq;
+    const float f = qq < 1.0f ? f_near : f_far;
     const float gx = f * dx, gy = f * dy;
     const float mq = bq.z;
     const uint32_t tyq = tag_type(tq);
@@ -437,7 +443,11 @@ struct PairCtx {
     sR.add(mql * cont);
     // predicting continuity (derivatives.c:1217-1252), depends on the pair order
     const float alt = (lpx - aq.x) * gx + (lpy - aq.y) * gy;
-    sRt.add(p_is_i ? mq * cont : mp * alt);
+    float rt_q = mq * cont, rt_p = mp * alt;
+    // both forms are live in every wavefront; keep them as values so the choice is a
+    // select rather than two masked branches
+    asm volatile("" : "+v"(rt_q), "+v"(rt_p));
+    sRt.add(p_is_i ? rt_q : rt_p);
     // pressure gradient (derivatives.c:1109-1163) + Monaghan viscosity
     // -alpha c0 mu / rho_bar with mu = h v.r / (r^2 + 0.01 h^2), rho_bar = (rho_p + rho_q) / 2,
     // through one reciprocal of the product of the two denominators
