@@ -210,6 +210,14 @@ int uph_gns_integrate(int n_recv, double *pos, double *vel, const float *acc, in
                       double ascale, void *stream);
 int uph_sph_density(int n_recv, const int *off, const int *send, const double *pos, const double *mass,
                     double h, double *rho, void *stream);
+/* One message-passing step on the edges at width 128, fused: z = [e,1] A + zb[recv] +
+ * zc[send], u = relu(z), m = [u,1] D, e += m, eb_out = bf16(e), m_out = m.  eb_in/eb_out
+ * (may alias), u1_out: E x 136 BF16 rows (latent, the constant 1, zeros); e, m_out: E x 128
+ * FP32; zb, zc: n x 128 FP32; At, Dt: the weights transposed, 128 x 136 BF16.  u1_out
+ * (optional) receives u for the backward pass. */
+int uph_gns_edge_step(long E, const void *eb_in, void *eb_out, float *e, const float *zb, const float *zc,
+                      const int *recv, const int *send, const void *At, const void *Dt, float *m_out,
+                      void *u1_out, void *stream);
 /* FP64 elementwise helpers of the rollout: out = a x + b y (macro-step velocities from
  * positions); out = x * num / den (density carried across a surrogate block). */
 int uph_lincomb_f64(long n, double a, const double *x, double b, const double *y, double *out, void *stream);

@@ -15,6 +15,7 @@
 #include <stdio.h>
 #include <string>
 #include <mutex>
+#include <algorithm>
 
 #include "../../include/uph.h"
 
@@ -667,6 +668,161 @@ extern "C" int uph_scale_ratio_f64(long n, const double* x, const double* num, c
                                    void* stream) {
   if (n <= 0) return 0;
   hipLaunchKernelGGL(k_scale_ratio_f64, dim3(nblk(n)), dim3(256), 0, (hipStream_t)stream, n, x, num, den, out);
+  CHK(hipGetLastError());
+  return 0;
+}
+
+// ------------------------------------------------------------ fused edge step
+//
+// One message-passing step on the edges, width 128, in one kernel:
+//   z  = [e,1] A + zb[recv] + zc[send];  u = relu(z)            (GEMM 1 + gathers)
+//   m  = [u,1] D                                                (GEMM 2)
+//   e += m;  eb_out = bf16(e);  m_out = m                        (residual update)
+// A tile of 192 edges stays in LDS between the two GEMMs (u never goes to HBM
+// unless u1_out asks for it, as training does), the two weight matrices stay
+// in LDS for the whole persistent loop, and the MFMA tiles are those of k_gemm
+// (16x16x32 BF16, FP32 accumulation).  Per edge the HBM traffic is the BF16
+// input row, the FP32 latent read and written, the BF16 output row and the
+// FP32 message.
+#define EW 128            // latent width
+#define EKP 136           // row width of the BF16 inputs: latent, the constant 1, zeros
+#define EKC 160           // K extent of the MFMA loop (5 chunks of 32); columns >= EKP are zero
+#define ELD 168           // LDS row stride (bf16)
+#define ETILE 192         // edges per tile: 12 wavefronts of 16 rows
+#define ETHREADS 768
+
+__global__ __launch_bounds__(ETHREADS) void k_edge_step(long E, const bf16_t* __restrict__ eb_in,
+                                                        bf16_t* __restrict__ eb_out, float* __restrict__ e,
+                                                        const float* __restrict__ zb, const float* __restrict__ zc,
+                                                        const int* __restrict__ recv, const int* __restrict__ send,
+                                                        const bf16_t* __restrict__ At, const bf16_t* __restrict__ Dt,
+                                                        float* __restrict__ m_out, bf16_t* __restrict__ u1_out) {
+  extern __shared__ __align__(16) unsigned char esm[];
+  bf16_t* sA = (bf16_t*)esm;
+  bf16_t* sD = sA + EW * ELD;
+  bf16_t* sX = sD + EW * ELD;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const bf16_t zero = (bf16_t)0.0f;
+  v8bf z8;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) z8[q] = zero;
+  // transposed weights (row n = output column n), zero past column EKP
+  for (int idx = tid; idx < EW * (ELD / 8); idx += ETHREADS) {
+    const int n = idx / (ELD / 8), k8 = (idx % (ELD / 8)) * 8;
+    v8bf va = z8, vd = z8;
+    if (k8 + 8 <= EKP) {
+      va = *(const v8bf*)(At + (size_t)n * EKP + k8);
+      vd = *(const v8bf*)(Dt + (size_t)n * EKP + k8);
+    }
+    *(v8bf*)(sA + n * ELD + k8) = va;
+    *(v8bf*)(sD + n * ELD + k8) = vd;
+  }
+  for (int idx = tid; idx < ETILE * ((ELD - EKP) / 8); idx += ETHREADS) {
+    const int r = idx / ((ELD - EKP) / 8), k8 = EKP + (idx % ((ELD - EKP) / 8)) * 8;
+    *(v8bf*)(sX + r * ELD + k8) = z8;
+  }
+  __syncthreads();
+  const long ntiles = (E + ETILE - 1) / ETILE;
+  const int ar = 16 * w + (lane & 15), kq = (lane >> 4) * 8;
+  for (long t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const long e0 = t * ETILE;
+    for (int idx = tid; idx < ETILE * (EKP / 8); idx += ETHREADS) {
+      const int r = idx / (EKP / 8), k8 = (idx % (EKP / 8)) * 8;
+      const long er = e0 + r;
+      *(v8bf*)(sX + r * ELD + k8) = er < E ? *(const v8bf*)(eb_in + er * EKP + k8) : z8;
+    }
+    __syncthreads();
+    // the latent rows this lane will update are loaded first, so their HBM latency
+    // overlaps both GEMMs
+    float ev[8][4];
+    int rcv[4], snd[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const long er = e0 + 16 * w + (lane >> 4) * 4 + i;
+      const bool live = er < E;
+      rcv[i] = live ? recv[er] : 0;
+      snd[i] = live ? send[er] : 0;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) ev[j][i] = live ? e[er * EW + 16 * j + (lane & 15)] : 0.f;
+    }
+    v4f acc[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) acc[j] = (v4f){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kc = 0; kc < EKC; kc += 32) {
+      const v8bf a = *(const v8bf*)(sX + ar * ELD + kc + kq);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const v8bf bb = *(const v8bf*)(sA + (16 * j + (lane & 15)) * ELD + kc + kq);
+        acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bb, acc[j], 0, 0, 0);
+      }
+    }
+    // D layout: lane holds column (lane % 16) of each 16x16 tile, rows (lane / 16) * 4 + i
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int col = 16 * j + (lane & 15);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int r = 16 * w + (lane >> 4) * 4 + i;
+        const long er = e0 + r;
+        const float z = acc[j][i] + zb[(size_t)rcv[i] * EW + col] + zc[(size_t)snd[i] * EW + col];
+        const bf16_t u = (bf16_t)fmaxf(z, 0.f);
+        sX[r * ELD + col] = er < E ? u : zero;
+        if (u1_out && er < E) u1_out[er * EKP + col] = u;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 8; ++j) acc[j] = (v4f){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kc = 0; kc < EKC; kc += 32) {
+      const v8bf a = *(const v8bf*)(sX + ar * ELD + kc + kq);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const v8bf bb = *(const v8bf*)(sD + (16 * j + (lane & 15)) * ELD + kc + kq);
+        acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bb, acc[j], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int col = 16 * j + (lane & 15);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const long er = e0 + 16 * w + (lane >> 4) * 4 + i;
+        if (er >= E) continue;
+        const float mv = acc[j][i];
+        const float en = ev[j][i] + mv;
+        e[er * EW + col] = en;
+        eb_out[er * EKP + col] = (bf16_t)en;
+        m_out[er * EW + col] = mv;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+extern "C" int uph_gns_edge_step(long E, const void* eb_in, void* eb_out, float* e, const float* zb, const float* zc,
+                                 const int* recv, const int* send, const void* At, const void* Dt, float* m_out,
+                                 void* u1_out, void* stream) {
+  if (E <= 0) return 0;
+  const size_t lds = (size_t)(2 * EW + ETILE) * ELD * sizeof(bf16_t);
+  static int ready = 0;
+  if (!ready) {
+    int dev = 0, maxlds = 0;
+    CHK(hipGetDevice(&dev));
+    CHK(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    if ((size_t)maxlds < lds) return fail("uph_gns_edge_step: the device has too little LDS per work-group");
+    CHK(hipFuncSetAttribute((const void*)k_edge_step, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    ready = 1;
+  }
+  int dev = 0, ncu = 0;
+  CHK(hipGetDevice(&dev));
+  CHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  const long ntiles = (E + ETILE - 1) / ETILE;
+  const int grid = (int)std::min<long>(ntiles, (long)ncu);
+  hipLaunchKernelGGL(k_edge_step, dim3(grid), dim3(ETHREADS), lds, (hipStream_t)stream, E, (const bf16_t*)eb_in,
+                     (bf16_t*)eb_out, e, zb, zc, recv, send, (const bf16_t*)At, (const bf16_t*)Dt, m_out,
+                     (bf16_t*)u1_out);
   CHK(hipGetLastError());
   return 0;
 }

@@ -19,6 +19,7 @@ import numpy as np
 from .devbuf import DevBuf, check, from_bf16, lib, to_bf16
 
 NODE_IN, EDGE_IN, OUT = 16, 8, 2
+FUSED_WIDTH = 128          # width the fused edge kernel (uph_gns_edge_step) is built for
 
 
 def _pad8(x: int) -> int:
@@ -231,8 +232,13 @@ class GraphNet:
 
     # ------------------------------------------------------------ forward
 
-    def forward(self, g: Graph, xn: DevBuf, xe: DevBuf, train: bool = False) -> DevBuf:
+    def forward(self, g: Graph, xn: DevBuf, xe: DevBuf, train: bool = False,
+                fused: Optional[bool] = None) -> DevBuf:
+        """fused (default: width 128): the edge MLP of every step as one kernel."""
         n, E, W, Wl = g.n, g.E, self.W, self.W + 8
+        fused = (W == FUSED_WIDTH) if fused is None else bool(fused)
+        if fused and W != FUSED_WIDTH:
+            raise ValueError(f"the fused edge kernel is built for width {FUSED_WIDTH}")
         self._ensure(n, E, train)
         b, L = self.b, self.lin
         Lb = lib()
@@ -252,12 +258,19 @@ class GraphNet:
             hb, eb = b["hb"][s].ptr, b["eb"][s].ptr
             self._lin_fwd(P("B"), hb, Wl, n, b["zb"].ptr, W)
             self._lin_fwd(P("C"), hb, Wl, n, b["zc"].ptr, W)
-            self._lin_fwd(P("A"), eb, Wl, E, b["ze"].ptr, Wl)
-            check(Lb.uph_gather_add2(E, W, g.recv.ptr, b["zb"].ptr, g.send.ptr, b["zc"].ptr, W,
-                                     b["ze"].ptr, Wl, None))
             u1 = b["u1"][s].ptr
-            check(Lb.uph_relu_to_bf16(E, W, b["ze"].ptr, Wl, u1, Wl, None))
-            self._lin_fwd(P("D"), u1, Wl, E, b["m"].ptr, W)
+            if fused:
+                # both edge GEMMs, the gathers and the residual update in one kernel
+                check(Lb.uph_gns_edge_step(E, eb, b["eb"][s1].ptr, b["e"].ptr, b["zb"].ptr,
+                                           b["zc"].ptr, g.recv.ptr, g.send.ptr, P("A").wt.ptr,
+                                           P("D").wt.ptr, b["m"].ptr, u1 if train else None,
+                                           None))
+            else:
+                self._lin_fwd(P("A"), eb, Wl, E, b["ze"].ptr, Wl)
+                check(Lb.uph_gather_add2(E, W, g.recv.ptr, b["zb"].ptr, g.send.ptr, b["zc"].ptr,
+                                         W, b["ze"].ptr, Wl, None))
+                check(Lb.uph_relu_to_bf16(E, W, b["ze"].ptr, Wl, u1, Wl, None))
+                self._lin_fwd(P("D"), u1, Wl, E, b["m"].ptr, W)
             check(Lb.uph_segsum(n, W, g.off.ptr, None, b["m"].ptr, W, 0, b["a"].ptr, W, 0, None))
             ab = b["abs"][s].ptr
             check(Lb.uph_rows_to_bf16(n, W, b["a"].ptr, W, ab, W, None))
@@ -266,10 +279,11 @@ class GraphNet:
             u2 = b["u2"][s].ptr
             check(Lb.uph_relu_to_bf16(n, W, b["zn"].ptr, Wl, u2, Wl, None))
             self._lin_fwd(P("G"), u2, Wl, n, b["zb"].ptr, W)
-            check(Lb.uph_add_f32(E * W, b["m"].ptr, b["e"].ptr, None))
+            if not fused:
+                check(Lb.uph_add_f32(E * W, b["m"].ptr, b["e"].ptr, None))
+                check(Lb.uph_rows_to_bf16(E, W, b["e"].ptr, W, b["eb"][s1].ptr, Wl, None))
             check(Lb.uph_add_f32(n * W, b["zb"].ptr, b["h"].ptr, None))
             check(Lb.uph_rows_to_bf16(n, W, b["h"].ptr, W, b["hb"][s1].ptr, Wl, None))
-            check(Lb.uph_rows_to_bf16(E, W, b["e"].ptr, W, b["eb"][s1].ptr, Wl, None))
         hbf = b["hb"][self.S if train else 0].ptr
         self._lin_fwd(L["dec1"], hbf, Wl, n, b["zn"].ptr, Wl)
         check(Lb.uph_relu_to_bf16(n, W, b["zn"].ptr, Wl, b["up"][0].ptr, Wl, None))

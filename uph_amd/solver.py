@@ -95,6 +95,9 @@ SIGNATURES = {
                                     C.c_void_p, C.c_int, C.c_void_p]),
     "uph_add_rows_f32": (C.c_int, [C.c_long, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
                                    C.c_void_p]),
+    "uph_gns_edge_step": (C.c_int, [C.c_long, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.c_void_p, C.c_void_p, C.c_void_p]),
     "uph_lincomb_f64": (C.c_int, [C.c_long, C.c_double, C.c_void_p, C.c_double, C.c_void_p,
                                   C.c_void_p, C.c_void_p]),
     "uph_scale_ratio_f64": (C.c_int, [C.c_long, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
