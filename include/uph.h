@@ -170,11 +170,13 @@ int uph_dev_zero(void *p, size_t bytes);
 int uph_dev_sync(void);
 const char *uph_gnn_last_error(void);
 
-/* C = act(op(A) B + bias) with BF16 MFMA and FP32 accumulation.  op(A) is A
- * (M x K) or, when trans_a, the transpose of the K x M matrix A.  splits > 1
- * divides K among work-groups (FP32 partials in `work`, M*N*splits floats);
- * accumulate adds into an FP32 C (needs `work`, M*N floats).  act: 0 none, 1 ReLU. */
-int uph_gemm_bf16(int M, int N, int K, const void *A, int lda, int trans_a, const void *B, int ldb,
+/* C = act(op(A) B + bias) with BF16 MFMA and FP32 accumulation.  trans bit 0:
+ * op(A) is the transpose of the K x M matrix A (else A is M x K); bit 1: B is
+ * given as its N x K transpose (row stride ldb; K a multiple of 8), else K x N.
+ * splits > 1 divides K among work-groups (FP32 partials in `work`,
+ * M*N*splits floats); accumulate adds into an FP32 C (needs `work`, M*N
+ * floats).  act: 0 none, 1 ReLU. */
+int uph_gemm_bf16(int M, int N, int K, const void *A, int lda, int trans, const void *B, int ldb,
                   const float *bias, void *C, int ldc, int out_bf16, int act, int splits, float *work,
                   int accumulate, void *stream);
 int uph_cast_f32_bf16(long n, const float *x, void *y, void *stream);

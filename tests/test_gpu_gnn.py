@@ -135,3 +135,17 @@ def test_adam_and_mse():
     d = (y - t) * mask[:, None]
     assert loss == pytest.approx((d * d).sum() / cnt, rel=1e-5)
     assert np.allclose(dy.download((M, Cc), np.float32), 2 * d / cnt, atol=1e-7)
+
+
+@pytest.mark.parametrize("ta", [0, 1])
+def test_gemm_b_given_transposed(ta):
+    """trans bit 1: B passed as its N x K transpose (with and without A^T)."""
+    rng = np.random.default_rng(21 + ta)
+    M, N, K = 200, 136, 144
+    A = rng.normal(size=(M, K)).astype(np.float32)
+    B = rng.normal(size=(K, N)).astype(np.float32)
+    Ain = np.ascontiguousarray(A.T) if ta else A
+    lda = M if ta else K
+    C = _gemm(M, N, K, Ain, lda, ta | 2, np.ascontiguousarray(B.T), K)
+    R = _ref(A, B)
+    assert np.abs(C - R).max() < 1e-4 * np.abs(R).max() + 1e-5

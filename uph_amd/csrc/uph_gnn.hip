@@ -96,8 +96,10 @@ __device__ __forceinline__ float act_fwd(float x, int act) { return act == 1 ? f
 // A^T where A is stored K x M (row stride lda).  B is K x N (row stride ldb).
 // With split-K (gridDim.z > 1) each z-slice writes its FP32 partial product to
 // part + z*M*N (no bias/activation); k_gemm_reduce finishes.
+// With TB, B is given transposed: stored N x K (row stride ldb), as the weight
+// copies W^T are, so it stages with vector loads and stores.
 // Requires: N, lda, ldb, ldc multiples of 8; K a multiple of 8 unless TA.
-template <bool TA>
+template <bool TA, bool TB>
 __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const bf16_t* __restrict__ A, int lda,
                                               const bf16_t* __restrict__ B, int ldb, const float* __restrict__ bias,
                                               void* __restrict__ C, int ldc, int out_bf16, int act, int kchunk,
@@ -121,9 +123,7 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const bf16_t*
       const int r = tid >> 2, k8 = (tid & 3) * 8;
       const int gm = m0 + r, gk = k0 + k8;
       if (gm < M && gk + 7 < ke) {
-        const v8bf v = *(const v8bf*)(A + (size_t)gm * lda + gk);
-#pragma unroll
-        for (int e = 0; e < 8; ++e) As[r][k8 + e] = v[e];
+        *(v8bf*)&As[r][k8] = *(const v8bf*)(A + (size_t)gm * lda + gk);
       } else {
 #pragma unroll
         for (int e = 0; e < 8; ++e) As[r][k8 + e] = (gm < M && gk + e < ke) ? A[(size_t)gm * lda + gk + e] : zero;
@@ -141,7 +141,16 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const bf16_t*
       }
     }
     // stage B (32 x 64) transposed as Bs[n][k]
-    {
+    if (TB) {
+      const int r = tid >> 2, k8 = (tid & 3) * 8;
+      const int gn = n0 + r, gk = k0 + k8;
+      if (gn < N && gk + 7 < ke) {
+        *(v8bf*)&Bs[r][k8] = *(const v8bf*)(B + (size_t)gn * ldb + gk);
+      } else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) Bs[r][k8 + e] = (gn < N && gk + e < ke) ? B[(size_t)gn * ldb + gk + e] : zero;
+      }
+    } else {
       const int kk = tid >> 3, n8 = (tid & 7) * 8;
       const int gk = k0 + kk, gn = n0 + n8;
       if (gk < ke && gn + 7 < N) {
@@ -208,10 +217,13 @@ __global__ void k_gemm_reduce(int M, int N, int splits, const float* __restrict_
   }
 }
 
-extern "C" int uph_gemm_bf16(int M, int N, int K, const void* A, int lda, int trans_a, const void* B, int ldb,
+extern "C" int uph_gemm_bf16(int M, int N, int K, const void* A, int lda, int trans, const void* B, int ldb,
                              const float* bias, void* C, int ldc, int out_bf16, int act, int splits, float* work,
                              int accumulate, void* stream) {
   if (M <= 0 || N <= 0 || K <= 0) return 0;
+  const int trans_a = trans & 1, trans_b = (trans >> 1) & 1;
+  if (trans & ~3) return fail("uph_gemm_bf16: trans takes bit 0 (A^T) and bit 1 (B given as N x K)");
+  if (trans_b && (K & 7)) return fail("uph_gemm_bf16: K must be a multiple of 8 when B is given as N x K");
   if ((N & 7) || (lda & 7) || (ldb & 7) || (ldc & 7) || (!trans_a && (K & 7)))
     return fail("uph_gemm_bf16: N, K (unless trans_a) and leading dimensions must be multiples of 8");
   if (splits < 1) splits = 1;
@@ -227,14 +239,19 @@ extern "C" int uph_gemm_bf16(int M, int N, int K, const void* A, int lda, int tr
   // a single-slice accumulate still goes through the workspace
   if (via_work && splits == 1) grid.z = 1;
   const bool part_path = via_work;
-  if (trans_a)
-    hipLaunchKernelGGL(k_gemm<true>, grid, dim3(256), 0, st, M, N, K, (const bf16_t*)A, lda, (const bf16_t*)B, ldb,
-                       part_path ? nullptr : bias, part_path ? (void*)work : C, part_path ? N : ldc,
-                       part_path ? 0 : out_bf16, part_path ? 0 : act, kchunk, work);
+#define UPH_GEMM(TA_, TB_)                                                                                        \
+  hipLaunchKernelGGL((k_gemm<TA_, TB_>), grid, dim3(256), 0, st, M, N, K, (const bf16_t*)A, lda, (const bf16_t*)B, \
+                     ldb, part_path ? nullptr : bias, part_path ? (void*)work : C, part_path ? N : ldc,             \
+                     part_path ? 0 : out_bf16, part_path ? 0 : act, kchunk, work)
+  if (trans_a && trans_b)
+    UPH_GEMM(true, true);
+  else if (trans_a)
+    UPH_GEMM(true, false);
+  else if (trans_b)
+    UPH_GEMM(false, true);
   else
-    hipLaunchKernelGGL(k_gemm<false>, grid, dim3(256), 0, st, M, N, K, (const bf16_t*)A, lda, (const bf16_t*)B, ldb,
-                       part_path ? nullptr : bias, part_path ? (void*)work : C, part_path ? N : ldc,
-                       part_path ? 0 : out_bf16, part_path ? 0 : act, kchunk, work);
+    UPH_GEMM(false, false);
+#undef UPH_GEMM
   if (part_path) {
     const long mn = (long)M * N;
     hipLaunchKernelGGL(k_gemm_reduce, dim3((unsigned)((mn + 255) / 256)), dim3(256), 0, st, M, N, (int)grid.z, work,

@@ -209,26 +209,27 @@ class GraphNet:
 
     # ------------------------------------------------------------ GEMM helpers
 
-    def _gemm(self, M, N, K, A, lda, B, ldb, C, ldc, trans_a=0, out_bf16=0, act=0,
+    def _gemm(self, M, N, K, A, lda, B, ldb, C, ldc, trans_a=0, trans_b=0, out_bf16=0, act=0,
               accumulate=0):
+        """trans_b: B is given as its N x K transpose (the weight copies kept for that)."""
         splits = 1
         if trans_a:
             splits = max(1, min(64, K // 4096))
         work = self.work.ptr if (splits > 1 or accumulate) else None
-        check(lib().uph_gemm_bf16(M, N, K, A, lda, trans_a, B, ldb, None, C, ldc, out_bf16, act,
-                                  splits, work, accumulate, None))
+        check(lib().uph_gemm_bf16(M, N, K, A, lda, int(trans_a) | (int(trans_b) << 1), B, ldb, None,
+                                  C, ldc, out_bf16, act, splits, work, accumulate, None))
 
     def _lin_fwd(self, L: Lin, X, ldx, rows, out, ldo):
-        """out (FP32) = X (BF16, rows x L.Kp) . W"""
-        self._gemm(rows, L.Np, L.Kp, X, ldx, L.wb.ptr, L.Np, out, ldo)
+        """out (FP32) = X (BF16, rows x L.Kp) . W   (W read through its transpose)"""
+        self._gemm(rows, L.Np, L.Kp, X, ldx, L.wt.ptr, L.Kp, out, ldo, trans_b=1)
 
     def _lin_dw(self, L: Lin, X, ldx, rows, dZb, ldz):
         """grad += X^T dZ  (X BF16 rows x Kp, dZ BF16 rows x Np)"""
         self._gemm(L.Kp, L.Np, rows, X, ldx, dZb, ldz, L.g.ptr, L.Np, trans_a=1, accumulate=1)
 
     def _lin_dx(self, L: Lin, dZb, ldz, rows, out, ldo):
-        """out (FP32, rows x Kp) = dZ W^T"""
-        self._gemm(rows, L.Kp, L.Np, dZb, ldz, L.wt.ptr, L.Kp, out, ldo)
+        """out (FP32, rows x Kp) = dZ W^T   (W^T given through W, its N x K form)"""
+        self._gemm(rows, L.Kp, L.Np, dZb, ldz, L.wb.ptr, L.Np, out, ldo, trans_b=1)
 
     # ------------------------------------------------------------ forward
 
@@ -275,7 +276,8 @@ class GraphNet:
             ab = b["abs"][s].ptr
             check(Lb.uph_rows_to_bf16(n, W, b["a"].ptr, W, ab, W, None))
             self._lin_fwd(P("E"), hb, Wl, n, b["zn"].ptr, Wl)
-            self._gemm(n, W, W, ab, W, P("F").wb.ptr, P("F").Np, b["zn"].ptr, Wl, accumulate=1)
+            self._gemm(n, W, W, ab, W, P("F").wt.ptr, P("F").Kp, b["zn"].ptr, Wl, trans_b=1,
+                       accumulate=1)
             u2 = b["u2"][s].ptr
             check(Lb.uph_relu_to_bf16(n, W, b["zn"].ptr, Wl, u2, Wl, None))
             self._lin_fwd(P("G"), u2, Wl, n, b["zb"].ptr, W)
@@ -337,7 +339,7 @@ class GraphNet:
             self._lin_dw(P("F"), ab, W, n, dy, Wl)
             self._lin_dx(P("E"), dy, Wl, n, b["t1"].ptr, Wl)
             check(Lb.uph_add_rows_f32(n, W, b["t1"].ptr, Wl, b["dh"].ptr, Wl, None))
-            self._gemm(n, W, W, dy, Wl, P("F").wt.ptr, W, b["da"].ptr, W)
+            self._gemm(n, W, W, dy, Wl, P("F").wb.ptr, P("F").Np, b["da"].ptr, W, trans_b=1)
             # edge update: dm = de + da[recv]
             check(Lb.uph_gather_rows(E, W, g.recv.ptr, b["da"].ptr, W, b["dm"].ptr, W, 4, None))
             check(Lb.uph_add_f32(E * W, b["de"].ptr, b["dm"].ptr, None))
