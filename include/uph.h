@@ -131,6 +131,20 @@ int uph_time_steps(uph_t *h, int n_steps, double *ms_total, double *ms_pair);
 
 const char *uph_last_error(const uph_t *h);
 
+/* Diagnostics that sph.cc's output reports, over a scene's current fluid state:
+ * TotalKineticEnergy (derivatives.c:2254-2263), NeighborsToPressurePoint
+ * (:2290-2338; the pressure is the Tait pressure of each particle) and
+ * DensityConsistencyFunctions (:3090-3117, whose first value the reference
+ * always reports as 0).  probe_pressure is 0 and probe_norm 1 when no particle
+ * is within 3 h of the probe. */
+typedef struct uph_diag {
+  double kinetic_energy, potential_energy;
+  double probe_pressure, probe_norm;
+  int probe_neighbors;
+  double consistency_one, consistency_two, consistency_three, density_uncertainty;
+} uph_diag;
+int uph_diagnostics(uph_t *h, int scene, double probe_x, double probe_y, uph_diag *out);
+
 /* Device-side state exchange with the surrogate (no counterpart in the reference).
  * Arrays are device pointers in the reference's order (pos/vel: 2n doubles).
  * uph_state_from_device loads the fluid rows (walls keep their state); with

@@ -169,7 +169,10 @@ typedef struct uph_system {
  * omega.  Writes, for fluid rows: ddensitydt, ddensitydt_approx, dvdt,
  * dvdt_approx; Pressure and DynamicPressure = Tait(density); dpdt and
  * dpdt_approx = (dP/drho) times the two density rates; the fluctuation fields
- * (where allocated) = 0.  Boundary rows are reset to the state they had at the
+ * (where allocated) = 0; the diagnostics of derivatives.c:4317-4324
+ * (totalKineticEnergy, potentialEnergy, pointNeighbors, pointPressure,
+ * pressureNorm, consistencyDensity*, densityUncertainty) of the state it was
+ * called with, probe at pressureProbeX/Y.  Boundary rows are reset to the state they had at the
  * first call (the deck, as inputbp re-reads it every step) with the piston
  * kinematics of inputbp.c:194-207 at time t.  Solid, frozen and virtual
  * particles are not supported (error).

@@ -192,6 +192,8 @@ def test_c_host_driver_compat_derivatives(tmp_path, fixture):
                          cwd=tmp_path, capture_output=True, text=True, timeout=300)
     assert out.returncode == 0, out.stderr
     _say(out.stdout.strip())
+    diag = [l for l in out.stdout.splitlines() if l.startswith("diagnostics:")]
+    assert diag and "nan" not in diag[0].lower()
     out = subprocess.run([drv, str(steps), "--alpha", "0.1", "--out", "device.dat"], cwd=tmp_path,
                          capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stderr

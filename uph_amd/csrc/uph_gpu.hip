@@ -221,6 +221,7 @@ struct StepBufs {
   int* kthStep;
   // per original index (global id)
   double2 *deckPos, *deckVel, *deckLvel;
+  double* massD;   // FP64 mass per original index (diagnostics)
   // control: [0] itime, [1] overflow flag of the step, [2] overflowed steps, [3] step stamp
   int* ctl;
   float4* dbgRates;   // keep_rates: per slot {R, R~, a~x, a~y} (FP32), {ax, ay} (FP64)
@@ -824,6 +825,8 @@ struct uph_handle {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double* diag = nullptr;   // per-block partial sums of uph_diagnostics
+  int diag_cap = 0;
 };
 
 static void fill_scene(DevScene& S, const uph_scene& u, double cell_factor) {
@@ -876,7 +879,7 @@ static int alloc_bufs(StepBufs& b, int n, int ncells, int nblocks_scan, Err& err
   if (dalloc(&b.velA, n, err) || dalloc(&b.velB, n, err) || dalloc(&b.rhoA, n, err) ||
       dalloc(&b.rhoB, n, err) || dalloc(&b.rtB, n, err) || dalloc(&b.kthStep, n, err) ||
       dalloc(&b.mA, n, err) || dalloc(&b.mB, n, err) || dalloc(&b.deckPos, n, err) ||
-      dalloc(&b.deckVel, n, err) || dalloc(&b.deckLvel, n, err))
+      dalloc(&b.deckVel, n, err) || dalloc(&b.deckLvel, n, err) || dalloc(&b.massD, n, err))
     return -1;
   return 0;
 }
@@ -884,7 +887,7 @@ static int alloc_bufs(StepBufs& b, int n, int ncells, int nblocks_scan, Err& err
 static void free_bufs(StepBufs& b) {
   void* ptrs[] = {b.posA, b.velA, b.rhoA, b.kthStep, b.tagA, b.mA, b.mB, b.posB, b.velB, b.rhoB,
                   b.rtB, b.tagB, b.keys, b.rank, b.tmpIdx, b.cellStart, b.cellCount, b.partial,
-                  b.sceneOf, b.kthD2, b.kthId, b.deckPos, b.deckVel, b.deckLvel, b.ctl,
+                  b.sceneOf, b.kthD2, b.kthId, b.deckPos, b.deckVel, b.deckLvel, b.massD, b.ctl,
                   b.dbgRates};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -995,7 +998,7 @@ extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts*
   if (dalloc(&h->dsc, n_scenes, err)) return fail();
   // host staging in original order (global index = slot at start)
   std::vector<double2> pos(n), vel(n), lv(n);
-  std::vector<double> rho(n);
+  std::vector<double> rho(n), massv(n);
   std::vector<float4> rt(n);
   std::vector<uint32_t> tag(n);
   std::vector<int> sceneOf(n);
@@ -1009,6 +1012,7 @@ extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts*
       lv[g] = make_double2(u.lvel[2 * i], u.lvel[2 * i + 1]);
       rho[g] = u.rho[i];
       rt[g] = make_float4(0.f, 0.f, 0.f, (float)u.mass[i]);
+      massv[g] = u.mass[i];
       int ty = u.itype[i];
       if (i < u.n_fluid) ty = 1;
       else if (ty == 1 || ty < 0 || ty > 7) ty = 2;
@@ -1045,6 +1049,7 @@ extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts*
   int ctl[4] = {0, 0, 0, 0};
   if (hipMemcpy(h->b.deckVel, vdeck.data(), n * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(h->b.deckLvel, lv.data(), n * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(h->b.massD, massv.data(), n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(h->b.posB, pos.data(), n * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(h->b.velB, vel.data(), n * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(h->b.rhoB, rho.data(), n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
@@ -1080,6 +1085,7 @@ extern "C" void uph_destroy(uph_t* h) {
   if (h->graph) (void)hipGraphDestroy(h->graph);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
+  if (h->diag) (void)hipFree(h->diag);
   free_bufs(h->b);
   if (h->dsc) (void)hipFree(h->dsc);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -1222,6 +1228,102 @@ extern "C" int uph_get_rates(uph_t* h, int scene, float* R, float* Rt, float* a,
 }
 
 extern "C" const char* uph_last_error(const uph_t* h) { return h ? h->err.msg.c_str() : g_create_err.c_str(); }
+
+// ------------------------------------------------------ diagnostics
+//
+// The reductions sph.cc's output reports, over one scene's current fluid state:
+// TotalKineticEnergy (derivatives.c:2254-2263), NeighborsToPressurePoint
+// (:2290-2338, pressure from the Tait equation of state) and
+// DensityConsistencyFunctions (:3090-3117).  Per-block partial sums in FP64 are
+// added on the host in block order, so the result does not depend on timing.
+#define DIAG_Q 7
+#define DIAG_BLK 256
+
+__global__ __launch_bounds__(DIAG_BLK) void k_diag(int poff, int np, const uint32_t* __restrict__ tag,
+                                                    const double2* __restrict__ pos, const double2* __restrict__ vel,
+                                                    const double* __restrict__ rho, const double* __restrict__ massD,
+                                                    const DevScene* __restrict__ sc, int scene, double px, double py,
+                                                    double inv_nf, double* __restrict__ part) {
+  __shared__ double red[DIAG_Q][DIAG_BLK];
+  const int k = blockIdx.x * DIAG_BLK + threadIdx.x;
+  double q[DIAG_Q];
+#pragma unroll
+  for (int j = 0; j < DIAG_Q; ++j) q[j] = 0.0;
+  if (k < np) {
+    const int s = poff + k;
+    const uint32_t tg = tag[s];
+    if (tag_type(tg) == 1) {
+      const DevScene& S = sc[scene];
+      const double m = massD[tag_id(tg)];
+      const double2 x = pos[s], v = vel[s];
+      const double rh = rho[s];
+      q[0] = 0.5 * m * (v.x * v.x + v.y * v.y);
+      q[1] = m * S.g * x.y;
+      const double dx = x.x - px, dy = x.y - py;
+      const double r = sqrt(dx * dx + dy * dy), hh = S.h;
+      const double P = tait(S, rh);
+      if (r <= 3.0 * hh && P > 0.0) {
+        const double coeff1 = 0.5 - 5.0 * exp(-9.0);
+        const double w = (exp(-r * r / (hh * hh)) - exp(-9.0)) * m / (2.0 * M_PI * hh * hh * coeff1 * rh);
+        q[2] = P * w;
+        q[3] = w;
+        q[4] = 1.0;
+      }
+      q[5] = rh * inv_nf;   // localDensity / Nf: localDensity equals density for fluid
+      q[6] = rh * inv_nf;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < DIAG_Q; ++j) red[j][threadIdx.x] = q[j];
+  __syncthreads();
+  for (int st = DIAG_BLK / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st)
+#pragma unroll
+      for (int j = 0; j < DIAG_Q; ++j) red[j][threadIdx.x] += red[j][threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x < DIAG_Q) part[(size_t)blockIdx.x * DIAG_Q + threadIdx.x] = red[threadIdx.x][0];
+}
+
+extern "C" int uph_diagnostics(uph_t* h, int scene, double probe_x, double probe_y, uph_diag* out) {
+  if (!h || !out) return -1;
+  if (scene < 0 || scene >= h->nscenes) {
+    h->err.msg = "uph_diagnostics: scene out of range";
+    return -1;
+  }
+  const DevScene& S = h->hs[scene];
+  const int nb = (S.np + DIAG_BLK - 1) / DIAG_BLK;
+  if (nb > h->diag_cap) {
+    if (h->diag) HIPCHK(hipFree(h->diag), h->err);
+    h->diag = nullptr;
+    HIPCHK(hipMalloc((void**)&h->diag, (size_t)nb * DIAG_Q * sizeof(double)), h->err);
+    h->diag_cap = nb;
+  }
+  hipLaunchKernelGGL(k_diag, dim3(nb), dim3(DIAG_BLK), 0, h->stream, S.poff, S.np, h->b.tagA, h->b.posB,
+                     h->b.velB, h->b.rhoB, h->b.massD, h->dsc, scene, probe_x, probe_y,
+                     S.nf > 0 ? 1.0 / S.nf : 0.0, h->diag);
+  HIPCHK(hipGetLastError(), h->err);
+  std::vector<double> part((size_t)nb * DIAG_Q);
+  HIPCHK(hipMemcpyAsync(part.data(), h->diag, part.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream),
+         h->err);
+  HIPCHK(hipStreamSynchronize(h->stream), h->err);
+  double t[DIAG_Q] = {0, 0, 0, 0, 0, 0, 0};
+  for (int b = 0; b < nb; ++b)
+    for (int j = 0; j < DIAG_Q; ++j) t[j] += part[(size_t)b * DIAG_Q + j];
+  memset(out, 0, sizeof *out);
+  out->kinetic_energy = t[0];
+  out->potential_energy = t[1];
+  out->probe_neighbors = (int)t[4];
+  out->probe_norm = out->probe_neighbors ? t[3] : 1.0;
+  out->probe_pressure = out->probe_neighbors ? t[2] / t[3] : 0.0;
+  // derivatives.c:3096-3099 accumulates outside its reduction and then overwrites
+  // the result with the reduction's (empty) total: the reported value is 0
+  out->consistency_one = 0.0;
+  out->consistency_two = t[5];
+  out->consistency_three = t[6];
+  out->density_uncertainty = out->consistency_one - t[5] * t[5] / t[6];
+  return 0;
+}
 
 // ------------------------------------------------------ SearchNeighbors
 
@@ -1915,7 +2017,22 @@ extern "C" int uph_derivatives(uph_system* s, double t) {
                      (const double*)drho, (const double2*)dlv, h->b.deckLvel, 1);
   hipLaunchKernelGGL(k_walls_at, dim3(grid_of(S.np)), dim3(BLK), 0, h->stream, S.poff, S.np, S.nf, h->b.tagA,
                      h->b.posB, h->b.velB, h->b.deckPos, h->b.deckVel, h->dsc, 0, t);
-  if (chk(hipGetLastError(), "launch") || launch_step(h, nullptr, nullptr)) {
+  if (chk(hipGetLastError(), "launch")) return -1;
+  // the diagnostics derivatives() leaves in the System (derivatives.c:4317-4324), of
+  // the state it was called with
+  uph_diag dg;
+  if (uph_diagnostics(h, 0, s->pressureProbeX, s->pressureProbeY, &dg) != 0)
+    return compat_fail("derivatives: " + err.msg);
+  s->totalKineticEnergy = dg.kinetic_energy;
+  s->potentialEnergy = dg.potential_energy;
+  s->pointNeighbors = dg.probe_neighbors;
+  s->pressureNorm = dg.probe_norm;
+  if (dg.probe_neighbors) s->pointPressure = dg.probe_pressure;
+  s->consistencyDensityOne = dg.consistency_one;
+  s->consistencyDensityTwo = dg.consistency_two;
+  s->consistencyDensityThree = dg.consistency_three;
+  s->densityUncertainty = dg.density_uncertainty;
+  if (launch_step(h, nullptr, nullptr)) {
     if (c.err.empty()) c.err = "derivatives: " + err.msg;
     return -1;
   }

@@ -298,11 +298,15 @@ int main(int argc, char **argv) {
     sy.SpeedOfSound = c; sy.CompressionFactor = s.B; sy.alpha = s.alpha_av;
     sy.springConstant0 = k0; sy.dampingConstant0 = c0d; sy.timeStep = dt; sy.HowManyNeighbors = 40;
     sy.movingBoundary = moving; sy.stroke = stroke; sy.wavePeriod = period; sy.omega = 2.0 * M_PI / period;
+    sy.pressureProbeX = v[30]; sy.pressureProbeY = v[31];
     double tc = now_s();
     if (run_compat(&sy, nsteps) != 0) return 1;
     double elc = now_s() - tc;
     printf("%d fluid + %d wall particles, %d steps in %.3f s (host loop, derivatives() on the GPU): "
            "%.4g particle-updates/s\n", fl.n, fx.n + mv.n, nsteps, elc, elc > 0 ? (double)fl.n * nsteps / elc : 0.0);
+    /* the diagnostics of the last derivatives() call (the state before the last corrector) */
+    printf("diagnostics: kinetic %.12e potential %.12e probe %.12e (%d neighbours)\n", sy.totalKineticEnergy,
+           sy.potentialEnergy, sy.pointPressure, sy.pointNeighbors);
     FILE *fc = fopen(out_file, "w");
     if (!fc) {
       fprintf(stderr, "uph_run: cannot write %s\n", out_file);
@@ -340,6 +344,10 @@ int main(int argc, char **argv) {
   double el = now_s() - t0;
   printf("%d fluid + %d wall particles, %d steps in %.3f s: %.4g particle-updates/s\n", fl.n,
          fx.n + mv.n, nsteps, el, el > 0 ? (double)fl.n * nsteps / el : 0.0);
+  uph_diag dg;
+  if (uph_diagnostics(h, 0, v[30], v[31], &dg) == 0)
+    printf("diagnostics: kinetic %.12e potential %.12e probe %.12e (%d neighbours)\n", dg.kinetic_energy,
+           dg.potential_energy, dg.probe_pressure, dg.probe_neighbors);
   double *pos = malloc(sizeof(double) * 2 * n), *vel = malloc(sizeof(double) * 2 * n);
   double *rho = malloc(sizeof(double) * n);
   if (uph_get_state(h, 0, pos, vel, rho) != 0) {

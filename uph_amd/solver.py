@@ -33,6 +33,14 @@ class UphScene(C.Structure):
                 ("m_amp", C.c_double), ("m_omega", C.c_double), ("box", C.c_double * 4)]
 
 
+class UphDiag(C.Structure):
+    _fields_ = [("kinetic_energy", C.c_double), ("potential_energy", C.c_double),
+                ("probe_pressure", C.c_double), ("probe_norm", C.c_double),
+                ("probe_neighbors", C.c_int), ("consistency_one", C.c_double),
+                ("consistency_two", C.c_double), ("consistency_three", C.c_double),
+                ("density_uncertainty", C.c_double)]
+
+
 class UphOpts(C.Structure):
     _fields_ = [("use_graph", C.c_int), ("keep_rates", C.c_int), ("pair_variant", C.c_int),
                 ("pair_block", C.c_int), ("cell_factor", C.c_double)]
@@ -117,6 +125,7 @@ SIGNATURES = {
                                   C.POINTER(C.c_long)]),
     "uph_graph_last_error": (C.c_char_p, []),
     "uph_derivatives": (C.c_int, [C.c_void_p, C.c_double]),
+    "uph_diagnostics": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_void_p]),
     "derivatives": (None, [C.c_void_p, C.c_double]),
     "uph_derivatives_last_error": (C.c_char_p, []),
 }
@@ -258,6 +267,15 @@ class GpuSolver:
         self._check(lib().uph_get_rates(self.h, scene, _ptr(R, C.c_float), _ptr(Rt, C.c_float),
                                         _ptr(a, C.c_float), _ptr(at, C.c_float)))
         return dict(R=R, Rt=Rt, a=a, at=at)
+
+    def diagnostics(self, probe=(0.0, 0.0), scene: int = 0) -> dict:
+        """Kinetic/potential energy, the pressure probe and the density consistency values
+        of the scene's current fluid state (derivatives.c:2254-2338, 3090-3117)."""
+        self._scene(scene)
+        d = UphDiag()
+        self._check(lib().uph_diagnostics(self.h, scene, float(probe[0]), float(probe[1]),
+                                          C.byref(d)))
+        return {k: getattr(d, k) for k, _ in UphDiag._fields_}
 
     def state_to_device(self, pos: int, vel: int, rho: int, scene: int = 0):
         """Copies the scene's state, in reference order, into device buffers (pointers)."""
