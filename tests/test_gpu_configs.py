@@ -162,10 +162,12 @@ def test_c_host_driver(tmp_path):
     assert np.allclose(rows[:, 4], st["rho"][:n], rtol=0, atol=6e-11)
 
 
-def _deck_dir(tmp_path, fixture):
+def _deck_dir(tmp_path, fixture, print_step=None):
     from uph_amd.scenes import write_deck
     z = np.load(os.path.join(GOLDEN, fixture), allow_pickle=False)
-    vals = z["param_values"]
+    vals = z["param_values"].copy()
+    if print_step is not None:
+        vals[16] = print_step           # printStep (System.c:60-94 field 17)
     sc = load_golden_scene(fixture, alpha_av=0.1)
     with open(tmp_path / "parameters.dat", "w") as f:
         f.write("\n".join(repr(float(v)) if not float(v).is_integer() else str(int(v))
@@ -217,3 +219,36 @@ def test_c_host_driver_compat_derivatives(tmp_path, fixture):
     dpos_o = float(np.abs(a[:, 0:2] - so["x"][:n]).max()) / dx
     _say(f"compat vs oracle: max rel rho {drho_o:.2e}, max |dx|/dx {dpos_o:.2e}")
     assert drho_o < 1e-6 and dpos_o < 1e-4
+
+
+def test_c_host_driver_writes_the_reference_output(tmp_path):
+    """printStep output (output.c:19-130) from both driver loops: out<itime>.dat with the
+    reference's 14 columns for every particle, one line per output in
+    TotalKineticEnergy.dat and PointPressure.dat; the two loops' snapshots agree."""
+    drv = os.path.join(ROOT, "uph_amd", "uph_run")
+    sc = _deck_dir(tmp_path, "hydrostatic_init.npz", print_step=25)
+    for mode in ("device", "compat"):
+        d = tmp_path / mode
+        d.mkdir()
+        for f in ("parameters.dat", "FluidParticle.dat", "FixedBoundaryParticle.dat",
+                  "MovingBoundaryParticle.dat", "SolidParticle.dat"):
+            (d / f).write_text((tmp_path / f).read_text())
+        args = [drv, "100", "--alpha", "0.1"] + (["--compat"] if mode == "compat" else [])
+        out = subprocess.run(args, cwd=d, capture_output=True, text=True, timeout=300)
+        assert out.returncode == 0, out.stderr
+        for it in (25, 50, 75, 100):
+            rows = np.loadtxt(d / f"out{it}.dat")
+            assert rows.shape == (sc.n, 14), rows.shape
+            assert np.array_equal(rows[:, 0], np.arange(sc.n))
+        ke = np.loadtxt(d / "TotalKineticEnergy.dat")
+        pp = np.loadtxt(d / "PointPressure.dat")
+        assert ke.shape == (4, 3) and pp.shape == (4, 5)
+        assert np.allclose(ke[:, 0], np.array([25, 50, 75, 100]) * sc.dt, rtol=1e-9)
+        final = np.loadtxt(d / "FluidParticleFinal.dat", skiprows=1)
+        snap = np.loadtxt(d / "out100.dat")[:sc.n_fluid]
+        assert np.allclose(snap[:, 1:3], final[:, 0:2], rtol=0, atol=2e-10)
+    a = np.loadtxt(tmp_path / "device" / "out100.dat")
+    b = np.loadtxt(tmp_path / "compat" / "out100.dat")
+    nf = sc.n_fluid
+    _say(f"out100.dat device vs compat: max |dx| {np.abs(a[:nf, 1:3] - b[:nf, 1:3]).max():.2e}")
+    assert np.allclose(a[:nf, 1:6], b[:nf, 1:6], rtol=0, atol=1e-9)

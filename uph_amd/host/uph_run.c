@@ -80,6 +80,18 @@ static void free_deck(deck_t *d) {
   free(d->itype);
 }
 
+/* One printStep output (see write_output). */
+typedef struct snap {
+  int n, nf;
+  const double *pos, *vel, *rho, *mass, *pres, *rholoc;
+  const int *itype;
+  double h, t;
+  double ke, pe, probe_p, probe_norm;
+  int probe_n;
+} snap_t;
+static int write_output(const snap_t *o, int itime);
+static int write_init(const snap_t *o, const double *lvel);
+
 /* n rows of 2 doubles over one zeroed block (the reference's CreateMatrix layout) */
 static double **rows2(int n) {
   size_t m = n > 0 ? (size_t)n : 1;
@@ -99,7 +111,7 @@ static double *vec(int n) { return calloc(n > 0 ? (size_t)n : 1, sizeof(double))
 /* The host time loop of sph.cc:88-168 with the fluctuation fields zero (the
  * config-mode scheme) and derivatives() from the library.  On return s holds
  * the state after nsteps. */
-static int run_compat(uph_system *s, int nsteps) {
+static int run_compat(uph_system *s, int nsteps, int print_step, int init_steps) {
   const int nf = s->NumberOfFluidParticles, np = s->MaxNumberOfParticles;
   const double dt = s->timeStep;
   for (int itime = 1; itime <= nsteps; itime++) {
@@ -162,6 +174,14 @@ static int run_compat(uph_system *s, int nsteps) {
           s->Position[i][d] += dt * s->Velocity[i][d];
         }
     }
+    if (print_step > 0 && itime % print_step == 0) {
+      /* sph.cc:168-180: the rows are contiguous (rows2), as CreateMatrix makes them */
+      snap_t o = {np, nf, s->Position[0], s->Velocity[0], s->density, s->mass, s->DynamicPressure,
+                  s->localDensity, s->itype, s->hsml[0], itime * dt, s->totalKineticEnergy,
+                  s->potentialEnergy, s->pointPressure, s->pressureNorm, s->pointNeighbors};
+      if (write_output(&o, itime) != 0) return -1;
+      if (itime == init_steps && write_init(&o, s->localVelocity[0]) != 0) return -1;
+    }
   }
   return 0;
 }
@@ -172,6 +192,60 @@ static double now_s(void) {
   return ts.tv_sec + 1e-9 * ts.tv_nsec;
 }
 
+/* Output of sph.cc's printStep (output.c:19-130): the snapshot out<itime>.dat of
+ * every particle, and one line appended to TotalKineticEnergy.dat and
+ * PointPressure.dat.  Columns the config-mode scheme does not compute
+ * (damGateFlag, approxVorticity, specificDissipationRate, the second probe) are
+ * written as 0; the series of the reference's other diagnostics files are not
+ * written. */
+
+static int write_output(const snap_t *o, int itime) {
+  char name[64];
+  snprintf(name, sizeof name, "out%d.dat", itime);
+  FILE *f = fopen(name, "w");
+  FILE *fke = fopen("TotalKineticEnergy.dat", "a"), *fpp = fopen("PointPressure.dat", "a");
+  if (!f || !fke || !fpp) {
+    fprintf(stderr, "uph_run: cannot write the output of step %d\n", itime);
+    if (f) fclose(f);
+    if (fke) fclose(fke);
+    if (fpp) fclose(fpp);
+    return -1;
+  }
+  for (int i = 0; i < o->n; i++)
+    fprintf(f, "%d \t%.10lf\t%.10lf\t%.10lf\t%.10lf\t%.10lf\t%.10lf\t%.10lf\t%.10lf\t%d\t%.10lf\t%d\t%.10lf\t%.10lf\n",
+            i, o->pos[2 * i], o->pos[2 * i + 1], o->vel[2 * i], o->vel[2 * i + 1], o->rho[i], o->mass[i], o->h,
+            o->pres[i], 0, 0.0, o->itype[i], 0.0, o->rholoc[i]);
+  fprintf(fke, "%.10lf\t%.10lf %.10lf\n", o->t, o->ke, o->pe);
+  fprintf(fpp, "%.10lf\t%.10lf\t%d\t%.10lf\t%.10lf\n", o->t, o->probe_p, o->probe_n, o->probe_norm, 0.0);
+  fclose(f);
+  fclose(fke);
+  fclose(fpp);
+  return 0;
+}
+
+/* FluidParticleInit.dat at step NumberOfInitializationSteps (output.c:116-128): the
+ * fluid rows in the deck format, a restart point. */
+static int write_init(const snap_t *o, const double *lvel) {
+  FILE *f = fopen("FluidParticleInit.dat", "a");
+  if (!f) return -1;
+  fprintf(f, "%d\n", o->nf);
+  for (int i = 0; i < o->nf; i++)
+    fprintf(f, "%.10lf\t%.10lf\t%.10lf\t%.10lf\t%.10lf\t%.10lf\t%.10lf\t%.10lf\t%.10lf\t%d\t%.10lf\t%.10lf\n",
+            o->pos[2 * i], o->pos[2 * i + 1], o->vel[2 * i], o->vel[2 * i + 1], o->rho[i], o->pres[i], o->mass[i],
+            0.0, 0.0, o->itype[i], lvel[2 * i], lvel[2 * i + 1]);
+  fclose(f);
+  return 0;
+}
+
+/* sph.cc:54-80 empties the series files at start */
+static void truncate_series(void) {
+  const char *names[] = {"TotalKineticEnergy.dat", "PointPressure.dat", "FluidParticleInit.dat"};
+  for (int k = 0; k < 3; k++) {
+    FILE *f = fopen(names[k], "w");
+    if (f) fclose(f);
+  }
+}
+
 int main(int argc, char **argv) {
   if (argc < 2) {
     fprintf(stderr, "Give a maximum number of time step\n");
@@ -180,7 +254,7 @@ int main(int argc, char **argv) {
   int nsteps = atoi(argv[1]);
   const char *fluid_file = "FluidParticle.dat", *out_file = "FluidParticleFinal.dat";
   double h_factor = 0.0, alpha = -1.0;
-  int use_graph = 0, compat = 0;
+  int use_graph = 0, compat = 0, write = 1;
   for (int a = 2; a < argc; a++) {
     if (!strcmp(argv[a], "--fluid") && a + 1 < argc) fluid_file = argv[++a];
     else if (!strcmp(argv[a], "--h-factor") && a + 1 < argc) h_factor = atof(argv[++a]);
@@ -188,6 +262,7 @@ int main(int argc, char **argv) {
     else if (!strcmp(argv[a], "--out") && a + 1 < argc) out_file = argv[++a];
     else if (!strcmp(argv[a], "--graph")) use_graph = 1;
     else if (!strcmp(argv[a], "--compat")) compat = 1;
+    else if (!strcmp(argv[a], "--no-output")) write = 0;
     else {
       fprintf(stderr, "uph_run: unknown option %s\n", argv[a]);
       return 1;
@@ -211,6 +286,7 @@ int main(int argc, char **argv) {
   const double dx = v[7], mach = v[11], alpha_deck = v[14], dt = v[18], nn = v[20];
   const double k0 = v[23], c0d = v[24];
   const int gravity = (int)v[26], moving = (int)v[28];
+  const int print_step = write ? (int)v[16] : 0, init_steps = (int)v[29];
 
   deck_t fl, fx, mv, so;
   if (read_deck(fluid_file, &fl) || read_deck("FixedBoundaryParticle.dat", &fx) ||
@@ -299,8 +375,9 @@ int main(int argc, char **argv) {
     sy.springConstant0 = k0; sy.dampingConstant0 = c0d; sy.timeStep = dt; sy.HowManyNeighbors = 40;
     sy.movingBoundary = moving; sy.stroke = stroke; sy.wavePeriod = period; sy.omega = 2.0 * M_PI / period;
     sy.pressureProbeX = v[30]; sy.pressureProbeY = v[31];
+    if (print_step > 0) truncate_series();
     double tc = now_s();
-    if (run_compat(&sy, nsteps) != 0) return 1;
+    if (run_compat(&sy, nsteps, print_step, init_steps) != 0) return 1;
     double elc = now_s() - tc;
     printf("%d fluid + %d wall particles, %d steps in %.3f s (host loop, derivatives() on the GPU): "
            "%.4g particle-updates/s\n", fl.n, fx.n + mv.n, nsteps, elc, elc > 0 ? (double)fl.n * nsteps / elc : 0.0);
@@ -335,11 +412,57 @@ int main(int argc, char **argv) {
     fprintf(stderr, "uph_run: %s\n", uph_last_error(NULL));
     return 1;
   }
+  double *pos = malloc(sizeof(double) * 2 * n), *vel = malloc(sizeof(double) * 2 * n);
+  double *rho = malloc(sizeof(double) * n);
   double t0 = now_s();
-  if (uph_step(h, nsteps) != 0 || uph_sync(h) < 0) {
-    fprintf(stderr, "uph_run: %s\n", uph_last_error(h));
-    uph_destroy(h);
-    return 1;
+  if (print_step <= 0 || print_step > nsteps) {
+    if (uph_step(h, nsteps) != 0 || uph_sync(h) < 0) {
+      fprintf(stderr, "uph_run: %s\n", uph_last_error(h));
+      uph_destroy(h);
+      return 1;
+    }
+  } else {
+    /* printStep output: the next interval is queued before the host writes the
+     * files of this one, so the device keeps stepping during the I/O */
+    truncate_series();
+    double *pres = malloc(sizeof(double) * n), *rhol = malloc(sizeof(double) * n);
+    int done = 0, first = print_step < nsteps ? print_step : nsteps;
+    if (uph_step(h, first) != 0) {
+      fprintf(stderr, "uph_run: %s\n", uph_last_error(h));
+      return 1;
+    }
+    while (done < nsteps) {
+      const int chunk = print_step < nsteps - done ? print_step : nsteps - done;
+      uph_diag dg;
+      if (uph_get_state(h, 0, pos, vel, rho) != 0 || uph_diagnostics(h, 0, v[30], v[31], &dg) != 0) {
+        fprintf(stderr, "uph_run: %s\n", uph_last_error(h));
+        return 1;
+      }
+      done += chunk;
+      if (done < nsteps) {
+        const int next = print_step < nsteps - done ? print_step : nsteps - done;
+        if (uph_step(h, next) != 0) {
+          fprintf(stderr, "uph_run: %s\n", uph_last_error(h));
+          return 1;
+        }
+      }
+      if (done % print_step == 0) {
+        for (int i = 0; i < n; i++) {
+          pres[i] = i < fl.n ? s.B * (pow(rho[i] / rho0, gamma) - 1.0) : (i < fl.n + fx.n ? fx.pres[i - fl.n] : mv.pres[i - fl.n - fx.n]);
+          rhol[i] = i < fl.n ? rho[i] : rho0;
+        }
+        snap_t o = {n, fl.n, pos, vel, rho, all.mass, pres, rhol, all.itype, s.h, done * dt, dg.kinetic_energy,
+                    dg.potential_energy, dg.probe_pressure, dg.probe_norm, dg.probe_neighbors};
+        if (write_output(&o, done) != 0) return 1;
+        if (done == init_steps && write_init(&o, vel) != 0) return 1;
+      }
+    }
+    if (uph_sync(h) < 0) {
+      fprintf(stderr, "uph_run: %s\n", uph_last_error(h));
+      return 1;
+    }
+    free(pres);
+    free(rhol);
   }
   double el = now_s() - t0;
   printf("%d fluid + %d wall particles, %d steps in %.3f s: %.4g particle-updates/s\n", fl.n,
@@ -348,8 +471,6 @@ int main(int argc, char **argv) {
   if (uph_diagnostics(h, 0, v[30], v[31], &dg) == 0)
     printf("diagnostics: kinetic %.12e potential %.12e probe %.12e (%d neighbours)\n", dg.kinetic_energy,
            dg.potential_energy, dg.probe_pressure, dg.probe_neighbors);
-  double *pos = malloc(sizeof(double) * 2 * n), *vel = malloc(sizeof(double) * 2 * n);
-  double *rho = malloc(sizeof(double) * n);
   if (uph_get_state(h, 0, pos, vel, rho) != 0) {
     fprintf(stderr, "uph_run: %s\n", uph_last_error(h));
     uph_destroy(h);
