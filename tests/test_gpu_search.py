@@ -30,12 +30,12 @@ def _check(sc, name, K=40):
     return got
 
 
-def test_lattice_config1():
-    _check(scenes.dam_break(), "config-1 lattice")
+def test_dam_break_5k_lattice():
+    _check(scenes.dam_break(), "dam break 5K lattice")
 
 
 def test_jittered_lattice():
-    _check(scenes.dam_break(jitter=0.05, seed=11), "config-1 jittered +-0.05dx")
+    _check(scenes.dam_break(jitter=0.05, seed=11), "dam break 5K jittered +-0.05dx")
 
 
 def test_reference_hydrostatic_deck():
@@ -90,8 +90,8 @@ def test_edge_cases():
     _check(ref_sc, "spread-out positions (coarsened grid)")
 
 
-def test_config2_one_million():
+def test_dam_break_1m_lattice():
     sc = scenes.dam_break(dx=2 ** 0.5 * 1e-3)
-    got = _check(sc, "config-2 1M lattice")
+    got = _check(sc, "dam break 1M lattice")
     # 20 neighbours per interior particle: 10 pairs each
     assert len(got[0]) > 9.5 * sc.n_fluid

@@ -4,7 +4,7 @@ Pair sums run in FP32 with compensated accumulation on the GPU and in FP64 in
 the oracle; positions, velocities and densities are integrated in FP64 on
 both.  Tolerances (stated per test) follow the precision plan: rates to 1e-5
 of their largest magnitude, density to 1e-5 relative after 1000 steps of the
-config-1 dam break.
+5,000-particle dam break.
 """
 import numpy as np
 import pytest
@@ -54,7 +54,7 @@ def _compare_rates(sc, name):
 
 def test_rates_jittered_moving_fluid():
     sc = _moving_scene(scenes.dam_break(jitter=0.1, seed=4))
-    err, _ = _compare_rates(sc, "config-1 jittered, random v")
+    err, _ = _compare_rates(sc, "dam break 5K jittered, random v")
     for k, e in err.items():
         assert e < 1e-5, (k, e)
 
@@ -94,10 +94,10 @@ def _run_both(sc, steps, name, nthreads=0, **gopts):
     return drho, dx, sg, so, g
 
 
-def test_dam_break_config1_1000_steps():
-    """BASELINE config 1 geometry: 5,000 fluid, dx = 0.02, 1000 steps."""
+def test_dam_break_5k_1000_steps():
+    """Dam break 1.0 x 2.0 m, 5,000 fluid, dx = 0.02, 1000 steps."""
     sc = scenes.dam_break()
-    drho, dx, sg, so, g = _run_both(sc, 1000, "config-1 1000 steps")
+    drho, dx, sg, so, g = _run_both(sc, 1000, "dam break 5K 1000 steps")
     assert g.sync() == 0
     assert drho < 1e-5
     assert dx < 1e-3
@@ -199,18 +199,18 @@ def test_errors_are_reported():
         g.rates()                           # created without keep_rates
 
 
-def test_config2_one_million_steps():
-    """BASELINE config 2 (999,698 fluid at dx = sqrt(2) mm): GPU vs oracle for 3 steps,
+def test_dam_break_1m_steps():
+    """Dam break 1.0 x 2.0 m at dx = sqrt(2) mm (999,698 fluid): GPU vs oracle for 3 steps,
     then 200 GPU steps checked for mass/finiteness/sortedness-independent properties."""
     sc = scenes.dam_break(dx=2 ** 0.5 * 1e-3)
-    drho, dx, sg, so, g = _run_both(sc, 3, "config-2 1M, 3 steps")
+    drho, dx, sg, so, g = _run_both(sc, 3, "dam break 1M, 3 steps")
     assert drho < 1e-6
     assert dx < 1e-4
     g.step(197)
     overflow = g.sync()
     st = g.state()
     nf = sc.n_fluid
-    _say(f"config-2 after 200 steps: rho [{st['rho'][:nf].min():.3f}, {st['rho'][:nf].max():.3f}]"
+    _say(f"dam break 1M after 200 steps: rho [{st['rho'][:nf].min():.3f}, {st['rho'][:nf].max():.3f}]"
          f" overflow steps {overflow}")
     assert np.all(np.isfinite(st["x"])) and np.all(np.isfinite(st["rho"]))
     assert 990 < st["rho"][:nf].min() and st["rho"][:nf].max() < 1100

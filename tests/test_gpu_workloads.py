@@ -1,4 +1,4 @@
-"""Workloads of the other configurations, the autotuner and the C host driver, on the GPU."""
+"""The wave tank, the 512-scene batch, the autotuner and the C host driver, on the GPU."""
 import os
 import subprocess
 
@@ -12,15 +12,15 @@ pytestmark = pytest.mark.gpu
 
 
 def _say(msg):
-    print(f"[gpu-configs] {msg}", flush=True)
+    print(f"[gpu-workloads] {msg}", flush=True)
 
 
-def test_config3_wave_tank_hydrostatic_profile():
+def test_wave_tank_240k_hydrostatic_profile():
     """6.0 x 1.0 m layer at dx = 5 mm (~240K fluid), sinusoidal piston, 1:10 beach, 5000 steps."""
     from oracle.oracle import Sim
     from uph_amd.solver import GpuSolver
     sc = scenes.wave_tank()
-    _say(f"config-3: {sc.n_fluid} fluid + {sc.n_boundary} wall, 3 steps vs oracle")
+    _say(f"wave tank: {sc.n_fluid} fluid + {sc.n_boundary} wall, 3 steps vs oracle")
     g = GpuSolver(sc)
     g.step(3)
     o = Sim(sc, nthreads=0)
@@ -28,7 +28,7 @@ def test_config3_wave_tank_hydrostatic_profile():
     nf = sc.n_fluid
     sg, so = g.state(), o.state()
     drho = float(np.max(np.abs(sg["rho"][:nf] - so["rho"][:nf]) / so["rho"][:nf]))
-    _say(f"config-3 3 steps: max rel rho err {drho:.2e}")
+    _say(f"wave tank 3 steps: max rel rho err {drho:.2e}")
     assert drho < 1e-6
     # 5000 steps; the hydrostatic profile is checked on its average over the last 2500
     # steps (about one period of the column's acoustic breathing mode, 4 H / c0)
@@ -50,19 +50,19 @@ def test_config3_wave_tank_hydrostatic_profile():
         acc.append(prof)
     assert g.sync() == 0
     mean = np.mean(acc, axis=0)
-    _say(f"config-3 P / (rho0 g depth) by depth, mean over steps 2600-5000: {np.round(mean, 4)}; "
+    _say(f"wave tank P / (rho0 g depth) by depth, mean over steps 2600-5000: {np.round(mean, 4)}; "
          f"instantaneous range [{np.min(acc):.3f}, {np.max(acc):.3f}]")
     assert np.all(np.abs(mean - 1.0) < 0.03)
 
 
-def test_config4_batch_of_512_scenes():
+def test_batch_of_512_dam_breaks():
     """512 dam breaks with seeded width/height/dx in one batch: every scene equals its own
     single-scene run, and two small scenes match the oracle."""
     from oracle.oracle import Sim
     from uph_amd.solver import GpuSolver
     scs = scenes.batch_dam_breaks()
     nf = [s.n_fluid for s in scs]
-    _say(f"config-4: {len(scs)} scenes, {sum(nf)} fluid (min {min(nf)}, max {max(nf)})")
+    _say(f"batch: {len(scs)} scenes, {sum(nf)} fluid (min {min(nf)}, max {max(nf)})")
     gb = GpuSolver(scs, use_graph=True)
     gb.step(20)
     assert gb.sync() == 0
@@ -82,7 +82,7 @@ def test_config4_batch_of_512_scenes():
         drho = float(np.max(np.abs(sg["rho"][:n] - so["rho"][:n]) / so["rho"][:n]))
         assert drho < 1e-6, (k, drho)
     tot, _ = gb.time_steps(20)
-    _say(f"config-4: {tot / 20:.3f} ms per batch step, "
+    _say(f"batch: {tot / 20:.3f} ms per batch step, "
          f"{sum(nf) * 20 / (tot * 1e-3):.3e} particle-updates/s")
     gb.step(460)                       # 500 steps in all
     assert gb.sync() == 0
