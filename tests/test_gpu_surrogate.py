@@ -113,3 +113,35 @@ def test_trained_network_beats_zero_network():
         st = ro.state()
         assert np.all(np.isfinite(st["x"])) and np.all(np.isfinite(st["rho"]))
     assert res["trained"]["pos_err_mean_dx"] < 0.7 * res["zero"]["pos_err_mean_dx"]
+
+
+def test_restart_does_not_reuse_k_th_keys_of_earlier_steps():
+    """After a restart the step counter starts again at 1.  The K-th keys that capped
+    queries recorded before it must not pass for the new steps' keys: with K = 15 the
+    dam-break lattice caps every interior particle (20 in range); the restart state
+    spreads the right half of the column (spacing 1.35 dx, about 12 in range), so its
+    particles no longer cap while the left half still does.  The restarted solver
+    equals a fresh one built from the restart state."""
+    from uph_amd.solver import GpuSolver
+    sc = scenes.dam_break()
+    sc.K = 15
+    n, nf = sc.n, sc.n_fluid
+    a = GpuSolver(sc)
+    a.step(2)
+    assert a.sync() > 0, "the cap should bind before the restart"
+    cur = a.state()
+    x = cur["x"].copy()
+    right = np.arange(n) < nf
+    right &= x[:, 0] > 0.5
+    x[right] = np.array([0.5, 0.0]) + (x[right] - np.array([0.5, 0.0])) * 1.35
+    pos, vel, rho = DevBuf.of(x), DevBuf.of(cur["v"]), DevBuf.of(cur["rho"])
+    a.state_from_device(pos.ptr, vel.ptr, rho.ptr, restart=True)
+    s2 = sc.copy()
+    s2.pos[:nf], s2.vel[:nf], s2.rho[:nf] = x[:nf], cur["v"][:nf], cur["rho"][:nf]
+    s2.lvel[:nf] = cur["v"][:nf]
+    b = GpuSolver(s2)
+    a.step(2)
+    b.step(2)
+    sa, sb = a.state(), b.state()
+    for k in sa:
+        assert np.array_equal(sa[k], sb[k]), k
