@@ -72,6 +72,14 @@ enum {
   UPH_MOTION_SINE_OFFSET = 2  /* x = x_deck + A sin wt, vx = A w cos wt */
 };
 
+/* Smoothing kernel of the pair sums. */
+enum {
+  UPH_KERNEL_CUBIC = 0,       /* cubic spline, Kernel (derivatives.c:44-66) */
+  UPH_KERNEL_POLY7 = 1        /* PolyOrderSeven W and its gradient as CalculateKernel evaluates them
+                                 (derivatives.c:767-823, 1099-1101): grad W = -a0 (1 - q^2/4)^7
+                                 (2048 - 3584 q^2 + ... + 305 q^12) dx / h^2, a0 = 2145/(192512 pi h^2) */
+};
+
 /* One independent scene.  Arrays hold n = n_fluid + n_fixed + n_moving rows. */
 typedef struct uph_scene {
   int n_fluid, n_fixed, n_moving;
@@ -93,6 +101,7 @@ typedef struct uph_scene {
   int motion;           /* UPH_MOTION_*                                     */
   double m_amp, m_omega;
   double box[4];        /* xmin, ymin, xmax, ymax covered by the cell grid  */
+  int kernel;           /* UPH_KERNEL_*; the same for every scene of a batch */
 } uph_scene;
 
 typedef struct uph_opts {

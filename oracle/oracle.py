@@ -25,7 +25,7 @@ class Params(C.Structure):
                 ("g", C.c_double), ("gravity", C.c_int), ("alpha_av", C.c_double),
                 ("k_spring", C.c_double), ("c_damp", C.c_double), ("dt", C.c_double),
                 ("K", C.c_int), ("motion", C.c_int), ("m_amp", C.c_double),
-                ("m_omega", C.c_double)]
+                ("m_omega", C.c_double), ("kernel", C.c_int)]
 
 
 def build() -> None:
@@ -49,6 +49,8 @@ def lib():
         L.orc_search_brute.restype = C.c_int
         L.orc_cubic.argtypes = [_dp, C.c_double, C.c_double, _dp, _dp]
         L.orc_cubic.restype = None
+        L.orc_poly7.argtypes = [_dp, C.c_double, C.c_double, _dp, _dp]
+        L.orc_poly7.restype = None
         L.orc_create.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(Params), _dp, _dp, _dp,
                                  _dp, _dp, _ip, _dp, C.c_int]
         L.orc_create.restype = C.c_void_p
@@ -77,6 +79,15 @@ def cubic(dx, dy, r, h):
     return w[0], dw[0], dw[1]
 
 
+def poly7(dx, dy, r, h):
+    """(W, dW/dx, dW/dy) of the PolyOrderSeven kernel pair the reference evaluates
+    (derivatives.c:767-823) at separation (dx, dy), |.| = r."""
+    w = np.zeros(1)
+    dw = np.zeros(2)
+    lib().orc_poly7(np.array([dx, dy], dtype=np.float64), float(r), float(h), w, dw)
+    return w[0], dw[0], dw[1]
+
+
 def search(pos, nf, n, pb, K, dist, brute=False, cap=None, nthreads=0):
     """The SearchNeighbors contract: returns (i_pair, j_pair, rij2) in emission order."""
     pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1)
@@ -96,7 +107,7 @@ def search(pos, nf, n, pb, K, dist, brute=False, cap=None, nthreads=0):
 def params_of(sc) -> Params:
     return Params(sc.dx, sc.h, sc.rho0, sc.gamma, sc.c0, sc.B, sc.g, int(sc.gravity),
                   sc.alpha_av, sc.k_spring, sc.c_damp, sc.dt, int(sc.K), int(sc.motion),
-                  sc.m_amp, sc.m_omega)
+                  sc.m_amp, sc.m_omega, int(getattr(sc, "kernel", 0)))
 
 
 class Sim:

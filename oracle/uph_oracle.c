@@ -47,6 +47,7 @@ typedef struct {
   double alpha_av, k_spring, c_damp, dt;
   int K, motion;
   double m_amp, m_omega;
+  int kernel;                           /* 0: cubic spline (derivatives.c:44-66), 1: PolyOrderSeven (:767-823) */
 } orc_params;
 
 typedef struct {
@@ -94,6 +95,40 @@ void orc_cubic(const double *dx, double r, double h, double *w, double *dw) {
     *w = 0.0;
     dw[0] = dw[1] = 0.0;
   }
+}
+
+/* PolyOrderSeven W (PolyOrderSevenConvolutionKernel, derivatives.c:767-784):
+ * W = a (1 - q^2/4)^8 (2048 - 3072 q^2 + 2816 q^4 - 1984 q^6 + 1184 q^8 - 628 q^10 + 305 q^12),
+ * a = 2145/(1347584 pi h^2), for q <= 2. */
+double orc_poly7_w(double r, double h) {
+  const double a = 2145.0 / (1347584.0 * M_PI * h * h);
+  const double q = r / h;
+  if (!(q >= 0.0 && q <= 2.0)) return 0.0;
+  const double u = q * q, s = 1 - 0.25 * u;
+  const double s2 = s * s, s4 = s2 * s2, s8 = s4 * s4;
+  const double poly = 2048 - 3072 * u + 2816 * u * u - 1984 * u * u * u + 1184 * u * u * u * u -
+                      628 * u * u * u * u * u + 305 * u * u * u * u * u * u;
+  return a * s8 * poly;
+}
+
+/* PolyOrderSeven grad W (PolyOrderSevenDeconvolutionKernel, derivatives.c:786-823).
+ * Its correction coefficients -131/329 and 150882/1840097 (:792-793) are integer
+ * divisions and evaluate to 0, so only the first term is left:
+ * grad W = -a0 (1 - q^2/4)^7 (2048 - 3584 q^2 + 3712 q^4 - 2912 q^6 + 1912 q^8 - 1106 q^10 + 305 q^12) dx / h^2,
+ * a0 = 2145/(192512 pi h^2).  CalculateKernel (:1099-1101) pairs it with the W above. */
+void orc_poly7(const double *dx, double r, double h, double *w, double *dw) {
+  const double a0 = 2145.0 / (192512.0 * M_PI * h * h);
+  const double q = r / h;
+  *w = orc_poly7_w(r, h);
+  if (!(q >= 0.0 && q <= 2.0)) {
+    dw[0] = dw[1] = 0.0;
+    return;
+  }
+  const double u = q * q, s = 1 - 0.25 * u;
+  const double s2 = s * s, s4 = s2 * s2, s7 = s4 * s2 * s;
+  const double poly = 2048 - 3584 * u + 3712 * u * u - 2912 * u * u * u + 1912 * u * u * u * u -
+                      1106 * u * u * u * u * u + 305 * u * u * u * u * u * u;
+  for (int d = 0; d < 2; d++) dw[d] = -a0 * s7 * poly * dx[d] / (h * h);
 }
 
 /* ------------------------------------------------------- neighbour search */
@@ -433,7 +468,10 @@ int orc_derivatives(orc_sim *s, double t) {
     const double h = 0.5 * (s->hsml[i] + s->hsml[j]);
     for (int d = 0; d < 2; d++) dx[d] = s->x[2 * i + d] - s->x[2 * j + d];
     const double r = sqrt(s->rij2[k]);
-    orc_cubic(dx, r, h, &w, &s->dwdx[2 * k]);
+    if (p->kernel == 1)
+      orc_poly7(dx, r, h, &w, &s->dwdx[2 * k]);
+    else
+      orc_cubic(dx, r, h, &w, &s->dwdx[2 * k]);
   }
 
   /* EquationOfState (derivatives.c:1674-1682); config mode feeds it to the momentum terms */

@@ -40,11 +40,30 @@ def test_library_exports_every_declared_symbol():
     assert sorted(solver.SIGNATURES) == _declared()
 
 
-def test_struct_layout_matches_header():
-    # uph_scene: 3 ints, 6 pointers, then doubles/ints as declared; ctypes pads like C
-    s = solver.UphScene
-    assert s.pos.offset == 16 and s.box.offset + 32 == ctypes.sizeof(s)
-    assert ctypes.sizeof(solver.UphOpts) == 24
+def test_struct_layout_matches_header(tmp_path):
+    """Every field of the ctypes mirrors sits at the offset gcc gives the C struct
+    of include/uph.h, and the sizes agree."""
+    import subprocess
+    mirrors = {"uph_scene": solver.UphScene, "uph_opts": solver.UphOpts, "uph_diag": solver.UphDiag}
+    src = ['#include <stdio.h>', '#include <stddef.h>',
+           f'#include "{os.path.join(ROOT, "include", "uph.h")}"', "int main(void) {"]
+    for cname, py in mirrors.items():
+        src.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            src.append(f'  printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    src.append("  return 0; }")
+    (tmp_path / "layout.c").write_text("\n".join(src) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-o", str(exe), str(tmp_path / "layout.c")], check=True)
+    got = {}
+    for line in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n"):
+        if line:
+            c, f, v = line.split()
+            got[(c, f)] = int(v)
+    for cname, py in mirrors.items():
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
 
 
 def test_missing_library_fails_loudly(monkeypatch, tmp_path):

@@ -242,3 +242,59 @@ def test_rates_hydrostatic_walls():
     err, _ = _compare_rates(sc, "wave tank, hydrostatic walls")
     for k, e in err.items():
         assert e < 1e-5, (k, e)
+
+
+# ---------------------------------------------------- PolyOrderSeven kernel
+
+def _poly7(sc):
+    sc.kernel = scenes.KERNEL_POLY7
+    return sc
+
+
+def test_rates_poly7_jittered_moving_fluid():
+    """The PolyOrderSeven pair (derivatives.c:767-823) in the FP32 pair sums."""
+    sc = _poly7(_moving_scene(scenes.dam_break(jitter=0.1, seed=4)))
+    err, _ = _compare_rates(sc, "PolyOrderSeven, dam break 5K jittered, random v")
+    for k, e in err.items():
+        assert e < 1e-5, (k, e)
+
+
+def test_rates_poly7_reference_dambreak_deck():
+    """The reference's own kernel on its tests/dambreak deck (h = 2.691 dx, K = 40 binds)."""
+    sc = _poly7(_moving_scene(load_golden_scene("dambreak.npz"), amp=0.1))
+    err, g = _compare_rates(sc, "PolyOrderSeven, tests/dambreak (K=40 binds)")
+    assert g.sync() == 1
+    for k, e in err.items():
+        assert e < 1e-5, (k, e)
+
+
+def test_poly7_dam_break_5k_500_steps():
+    sc = _poly7(scenes.dam_break())
+    drho, dx, sg, so, g = _run_both(sc, 500, "PolyOrderSeven dam break 5K 500 steps")
+    assert drho < 1e-5
+    assert dx < 1e-3
+
+
+def test_poly7_compact_variant_equals_direct():
+    from uph_amd.solver import GpuSolver
+    sc = _poly7(_moving_scene(scenes.dam_break(width=0.5, height=0.8, jitter=0.1, seed=12)))
+    _say("PolyOrderSeven compact vs direct")
+    a = GpuSolver(sc, pair_variant=0)
+    b = GpuSolver(sc, pair_variant=1)
+    a.step(20)
+    b.step(20)
+    sa, sb = a.state(), b.state()
+    for key in ("x", "v", "rho"):
+        assert np.array_equal(sa[key], sb[key]), key
+
+
+def test_kernel_must_be_uniform_across_a_batch():
+    from uph_amd.solver import GpuSolver
+    a = scenes.dam_break(width=0.2, height=0.2)
+    b = _poly7(scenes.dam_break(width=0.2, height=0.3))
+    with pytest.raises(RuntimeError, match="kernel"):
+        GpuSolver([a, b])
+    bad = scenes.dam_break(width=0.2, height=0.2)
+    bad.kernel = 7
+    with pytest.raises(RuntimeError, match="kernel"):
+        GpuSolver(bad)

@@ -252,3 +252,34 @@ def test_c_host_driver_writes_the_reference_output(tmp_path):
     nf = sc.n_fluid
     _say(f"out100.dat device vs compat: max |dx| {np.abs(a[:nf, 1:3] - b[:nf, 1:3]).max():.2e}")
     assert np.allclose(a[:nf, 1:6], b[:nf, 1:6], rtol=0, atol=1e-9)
+
+
+def test_c_host_driver_poly7_on_the_waves_deck(tmp_path):
+    """uph_run --kernel poly7: the reference's own smoothing kernel (derivatives.c:767-823),
+    its waves deck, hsml = eta dx and piston, against the FP64 oracle with that kernel."""
+    from oracle.oracle import Sim
+    from uph_amd import scenes as S
+    from uph_amd.scenes import load_deck_dir
+    drv = os.path.join(ROOT, "uph_amd", "uph_run")
+    assert os.path.exists(drv), "uph_run not built"
+    _deck_dir(tmp_path, "waves.npz")
+    steps = 100
+    _say(f"uph_run --kernel poly7 {steps} on waves.npz")
+    out = subprocess.run([drv, str(steps), "--alpha", "0.1", "--kernel", "poly7", "--no-output",
+                          "--out", "poly7.dat"], cwd=tmp_path, capture_output=True, text=True,
+                         timeout=120)
+    assert out.returncode == 0, out.stderr
+    a = np.loadtxt(tmp_path / "poly7.dat", skiprows=1)
+    sc = load_deck_dir(str(tmp_path), alpha_av=0.1)
+    sc.kernel = S.KERNEL_POLY7
+    o = Sim(sc, nthreads=0)
+    o.step(steps)
+    so = o.state()
+    n = sc.n_fluid
+    drho = float(np.max(np.abs(a[:, 4] - so["rho"][:n]) / so["rho"][:n]))
+    dpos = float(np.abs(a[:, 0:2] - so["x"][:n]).max()) / sc.dx
+    _say(f"poly7 driver vs oracle: max rel rho {drho:.2e}, max |dx|/dx {dpos:.2e}")
+    assert drho < 1e-6 and dpos < 1e-4
+    bad = subprocess.run([drv, "1", "--kernel", "poly7", "--compat"], cwd=tmp_path,
+                         capture_output=True, text=True, timeout=60)
+    assert bad.returncode != 0 and "cubic" in bad.stderr

@@ -2,7 +2,9 @@
 
 The reference's build needs <mkl.h> and libnabo, absent here, so every oracle
 function is pinned by mathematics and independent brute force instead:
-closed forms of the cubic spline, an independent numpy statement of the
+closed forms of the cubic spline and of the PolyOrderSeven kernel (its
+gradient against the derivative of its W, taken by numpy's polynomial algebra
+and by finite differences), an independent numpy statement of the
 SearchNeighbors contract, the SPH identities the pair sums must satisfy
 (divergence of a linear velocity field, hydrostatic balance, Galilean
 invariance, momentum conservation), the Tait inverse, and the exact
@@ -74,6 +76,76 @@ def test_cubic_gradient_is_derivative_of_w(oracle_lib):
         wm = oracle_lib.cubic(dx - e, dy, math.hypot(dx - e, dy), H)[0]
         assert gx == pytest.approx((wp - wm) / (2 * e), rel=1e-5, abs=1e-3)
         _, gx2, gy2 = oracle_lib.cubic(-dx, -dy, r, H)
+        assert gx2 == -gx and gy2 == -gy
+
+
+def _poly7_factors(h):
+    """PolyOrderSevenConvolutionKernel (derivatives.c:767-784) as a * s(q)^8 * Q(q)
+    with the numpy polynomials s = 1 - q^2/4 and Q = 2048 - 3072 q^2 + ... + 305 q^12."""
+    from numpy.polynomial import Polynomial as Pl
+    c = np.zeros(13)
+    c[0::2] = [2048, -3072, 2816, -1984, 1184, -628, 305]
+    return 2145.0 / (1347584.0 * math.pi * h * h), Pl([1.0, 0.0, -0.25]), Pl(c)
+
+
+def _poly7_w_poly(h):
+    """W(q) of PolyOrderSeven expanded into one numpy polynomial in q."""
+    a, s, Q = _poly7_factors(h)
+    return a * s ** 8 * Q
+
+
+def dwdq(kernel, qq, h):
+    """dW/dq of the cubic spline (closed form) or of the PolyOrderSeven W (product
+    rule on its factors); independent of the oracle's gradient code."""
+    if kernel == scenes.KERNEL_POLY7:
+        a, s, Q = _poly7_factors(h)
+        sv = s(qq)
+        return float(a * (8 * sv ** 7 * s.deriv()(qq) * Q(qq) + sv ** 8 * Q.deriv()(qq)))
+    alpha = 10.0 / (7 * math.pi * h * h)
+    return alpha * (-3 * qq + 2.25 * qq * qq) if qq < 1 else -0.75 * alpha * (2 - qq) ** 2
+
+
+def test_poly7_w0_and_support(oracle_lib):
+    w, gx, gy = oracle_lib.poly7(0.0, 0.0, 0.0, H)
+    assert w == pytest.approx(2145.0 * 2048 / (1347584.0 * math.pi * H * H), rel=1e-15)
+    assert gx == 0.0 and gy == 0.0
+    w, gx, gy = oracle_lib.poly7(2.0 * H, 0.0, 2.0 * H, H)
+    assert w == 0.0 and gx == 0.0 and gy == 0.0
+    assert oracle_lib.poly7(2.0 * H * (1 + 1e-9), 0.0, 2.0 * H * (1 + 1e-9), H)[0] == 0.0
+
+
+def test_poly7_normalised(oracle_lib):
+    f = lambda r: 2 * math.pi * r * oracle_lib.poly7(r, 0.0, r, H)[0]
+    val = integrate.quad(f, 0, 2 * H, epsabs=1e-14, epsrel=1e-13)[0]
+    assert val == pytest.approx(1.0, abs=1e-11)
+
+
+def test_poly7_gradient_is_derivative_of_w(oracle_lib):
+    """The reference's gradient (PolyOrderSevenDeconvolutionKernel with its integer-
+    division coefficients -131/329 = 150882/1840097 = 0, derivatives.c:792-793) is
+    the gradient of its W: checked against finite differences of the oracle's W and
+    against the derivative of the W polynomial.  Reading the two coefficients as
+    real divisions would add a term and fail both."""
+    rng = np.random.default_rng(4)
+    wp = _poly7_w_poly(H)
+    # the monomial expansion of W cancels digits near q = 2, where W is small:
+    # compare on the scale of W(0) and of the steepest gradient
+    w0 = float(wp(0.0))
+    gmax = max(abs(dwdq(scenes.KERNEL_POLY7, q, H)) for q in np.linspace(0, 2, 201)) / H
+    for _ in range(50):
+        r = rng.uniform(0.02, 1.98) * H
+        th = rng.uniform(0, 2 * math.pi)
+        dx, dy = r * math.cos(th), r * math.sin(th)
+        w, gx, gy = oracle_lib.poly7(dx, dy, r, H)
+        assert w == pytest.approx(float(wp(r / H)), rel=1e-12, abs=1e-11 * w0)
+        e = 1e-7 * H
+        wpe = oracle_lib.poly7(dx + e, dy, math.hypot(dx + e, dy), H)[0]
+        wme = oracle_lib.poly7(dx - e, dy, math.hypot(dx - e, dy), H)[0]
+        assert gx == pytest.approx((wpe - wme) / (2 * e), rel=1e-5, abs=1e-3)
+        g_an = dwdq(scenes.KERNEL_POLY7, r / H, H) / H / r
+        assert gx == pytest.approx(g_an * dx, rel=1e-10, abs=1e-10 * gmax)
+        assert gy == pytest.approx(g_an * dy, rel=1e-10, abs=1e-10 * gmax)
+        _, gx2, gy2 = oracle_lib.poly7(-dx, -dy, r, H)
         assert gx2 == -gx and gy2 == -gy
 
 
@@ -180,9 +252,14 @@ def _interior(sc, margin):
     return np.all((x > lo) & (x < hi), axis=1)
 
 
-def test_continuity_divergence_of_linear_field(oracle_lib):
+KERNELS = [scenes.KERNEL_CUBIC, scenes.KERNEL_POLY7]
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_continuity_divergence_of_linear_field(oracle_lib, kernel):
     """v = a (x - xc): SPH continuity gives R = -rho div v = -2 a rho in the interior."""
     sc = _fluid_patch(jitter=0.0)
+    sc.kernel = kernel
     a = 0.7
     xc = sc.pos.mean(0)
     sc.vel = a * (sc.pos - xc)
@@ -194,19 +271,17 @@ def test_continuity_divergence_of_linear_field(oracle_lib):
     m = _interior(sc, 3 * sc.h)
     # exact lattice sum: (v_i - v_j).grad_i W = a r W'(r) for every lattice offset in range
     h, dx = sc.h, sc.dx
-    alpha = 10.0 / (7 * math.pi * h * h)
     lat = 0.0
     for di in range(-3, 4):
         for dj in range(-3, 4):
             rr = dx * math.hypot(di, dj)
             if 0 < rr and rr * rr < 4 * h * h:
-                qq = rr / h
-                dwdq = alpha * (-3 * qq + 2.25 * qq * qq) if qq < 1 else -0.75 * alpha * (2 - qq) ** 2
-                lat += sc.mass[0] * a * rr * dwdq / h
+                lat += sc.mass[0] * a * rr * dwdq(kernel, rr / h, h) / h
     assert np.allclose(r["R"][m], lat, rtol=1e-12)
     assert np.allclose(r["Rt"][m], lat, rtol=1e-12)
     # and the continuum value -rho div v within the lattice's discretisation error
-    assert np.allclose(r["R"][m], -2 * a * 1000.0, rtol=2e-2)
+    # (PolyOrderSeven is the narrower kernel: 4 % at h = 1.3 dx, 1 % for the spline)
+    assert np.allclose(r["R"][m], -2 * a * 1000.0, rtol=2e-2 if kernel == scenes.KERNEL_CUBIC else 5e-2)
 
 
 def test_hydrostatic_balance_interior(oracle_lib):
@@ -225,8 +300,10 @@ def test_hydrostatic_balance_interior(oracle_lib):
     assert np.all(r["P"][sc.n_fluid:] == 0.0)          # boundary at rho0: P(rho0) = 0
 
 
-def test_galilean_invariance(oracle_lib):
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_galilean_invariance(oracle_lib, kernel):
     sc = _fluid_patch()
+    sc.kernel = kernel
     sc.vel[:] = (0.3, -0.2)
     sc.lvel = sc.vel.copy()
     from oracle.oracle import Sim
@@ -238,9 +315,11 @@ def test_galilean_invariance(oracle_lib):
     assert np.abs(r["Rt"]).max() < 1e-9 * scale
 
 
-def test_momentum_conserved(oracle_lib):
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_momentum_conserved(oracle_lib, kernel):
     """No gravity, no walls: pairwise antisymmetry gives sum m a = 0 (AV included)."""
     sc = _fluid_patch(jitter=0.2)
+    sc.kernel = kernel
     rng = np.random.default_rng(5)
     sc.vel = rng.normal(0, 0.5, sc.vel.shape)
     sc.lvel = sc.vel.copy()
@@ -329,9 +408,11 @@ def test_leapfrog_free_fall_exact():
     assert np.all(st["rho"] == 1000.0)
 
 
-def test_gather_form_agrees_with_scatter(oracle_lib):
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_gather_form_agrees_with_scatter(oracle_lib, kernel):
     """Per-particle gather sums over an independently found pair set equal the oracle."""
     sc = scenes.dam_break(width=0.3, height=0.3, dx=0.02, tank=(0.6, 0.5), jitter=0.1, seed=2)
+    sc.kernel = kernel
     scenes.hydrostatic_walls(sc, 0.3)         # walls with density != rho0 = localDensity
     rng = np.random.default_rng(9)
     sc.vel[:sc.n_fluid] = rng.normal(0, 0.3, (sc.n_fluid, 2))
@@ -344,16 +425,13 @@ def test_gather_form_agrees_with_scatter(oracle_lib):
     lam = st["rholoc"] / rho                  # 1 for fluid, rho0/rho for walls
     ip, jp, r2 = np_search(x, sc.n_fluid, sc.n, sc.particle_boundary, 40, sc.cutoff2)
     h = sc.h
-    alpha = 10.0 / (7 * math.pi * h * h)
     P = sc.B * ((rho / sc.rho0) ** 7 - 1)
     R = np.zeros(sc.n); Rt = np.zeros(sc.n); A = np.zeros((sc.n, 2)); At = np.zeros((sc.n, 2))
     for i, j, d2 in zip(ip, jp, r2):
         for p, q in ((i, j), (j, i)):
             dxp = x[p] - x[q]
             rr = math.sqrt(d2)
-            qq = rr / h
-            dwdq = alpha * (-3 * qq + 2.25 * qq * qq) if qq < 1 else -0.75 * alpha * (2 - qq) ** 2
-            g = dwdq / h * dxp / rr                      # grad_p W_pq from dW/dq
+            g = dwdq(kernel, rr / h, h) / h * dxp / rr   # grad_p W_pq from dW/dq
             R[p] += m[q] * lam[q] * np.dot(v[p] - vl[q], g)   # continuity, gather form
             Rt[p] += (m[q] * np.dot(v[p] - vl[q], g) if p == i
                       else m[p] * np.dot(vl[p] - v[q], g))

@@ -366,6 +366,7 @@ struct PairCtx {
   float vpx, vpy, lpx, lpy, rhop, prp, mp, inv_mp;
   float invh, h2eps, cav;
   float fa, fb, fw;     // cubic-spline gradient factor: fa + fb q (q < 1), fw (2 - q)^2 / r (1 <= q < 2)
+  float k7, invh2;      // PolyOrderSeven gradient factor: k7 (1 - u/4)^7 P(u), u = r^2 / h^2
   int pb;               // ParticleBoundary as a global index
   CSum sR, sRt, sAx, sAy;   // sAx/sAy: the predicting acceleration (dvdt_approx)
   float dAx = 0.f, dAy = 0.f;   // dvdt - dvdt_approx: wall pairs with localDensity != density
@@ -383,7 +384,40 @@ struct PairCtx {
     fa = -3.0f * walpha * invh * invh;
     fb = 2.25f * walpha * invh * invh;
     fw = -0.75f * walpha * invh;
+    k7 = S.fk7;
+    invh2 = invh * invh;
     pb = S.pb_gid;
+  }
+
+  // grad W = f (x_p - x_q) for the kernel KT at |x_p - x_q|^2 = r2, 1/|.| = rinv
+  template <int KT>
+  __device__ __forceinline__ float grad_factor(float r2, float r, float rinv) const {
+#pragma clang fp contract(fast)
+    if (KT == UPH_KERNEL_POLY7) {
+      // PolyOrderSevenDeconvolutionKernel (derivatives.c:786-823); its correction
+      // terms carry the integer quotients -131/329 and 150882/1840097, both 0, so
+      // grad W = -a0 (1 - u/4)^7 (2048 - 3584 u + 3712 u^2 - 2912 u^3 + 1912 u^4
+      // - 1106 u^5 + 305 u^6) dx / h^2 with u = q^2, evaluated by Horner in u
+      const float u = r2 * invh2;
+      const float s = 1.0f - 0.25f * u;
+      const float s2 = s * s;
+      const float s7 = s2 * s2 * s2 * s;
+      float pu = 305.0f;
+      pu = pu * u - 1106.0f;
+      pu = pu * u + 1912.0f;
+      pu = pu * u - 2912.0f;
+      pu = pu * u + 3712.0f;
+      pu = pu * u - 3584.0f;
+      pu = pu * u + 2048.0f;
+      return k7 * s7 * pu;
+    }
+    // cubic spline gradient factor (derivatives.c:44-66); both branches evaluated
+    // and selected (lanes of a wavefront take both anyway)
+    const float qq = r * invh;
+    const float t = 2.0f - qq;
+    const float f_far = fw * t * t * rinv;
+    const float f_near = fa + fb * qq;
+    return qq < 1.0f ? f_near : f_far;
   }
 
   // Admission of an in-range pair by the rules of Search.cc:33 (original index
@@ -409,6 +443,7 @@ struct PairCtx {
 
   // One admitted pair: x_p - x_q = (ex, ey), |.|^2 = d2 (FP64 geometry, FP32 sums).
   // FP32 arithmetic may contract to FMA here; 1/r and 1/x use v_rsq_f32 / v_rcp_f32.
+  template <int KT>
   __device__ __forceinline__ void add(const DevScene& S, double ex, double ey, uint32_t tq, float4 aq, float4 bq) {
 #pragma clang fp contract(fast)
     const bool p_is_i = tag_id(tq) > idp;   // pair (i=p, j=q) or (i=q, j=p)
@@ -416,13 +451,7 @@ struct PairCtx {
     const float r2 = dx * dx + dy * dy;
     const float rinv = __builtin_amdgcn_rsqf(r2);
     const float r = r2 * rinv;
-    const float qq = r * invh;
-    // cubic spline gradient factor (derivatives.c:44-66): gradW = f * (x_p - x_q);
-    // both branches evaluated and selected (lanes of a wavefront take both anyway)
-    const float t = 2.0f - qq;
-    const float f_far = fw * t * t * rinv;
-    const float f_near = fa + fb * qq;
-    const float f = qq < 1.0f ? f_near : f_far;
+    const float f = grad_factor<KT>(r2, r, rinv);
     const float gx = f * dx, gy = f * dy;
     const float mq = bq.z;
     const uint32_t tyq = tag_type(tq);
@@ -517,7 +546,7 @@ __device__ __forceinline__ void correct(const DevScene& S, const StepBufs& b, in
 
 // Direct form: one pass over the candidates, physics inside the distance test.
 // EXACT applies the K-nearest cap (run only when some query overflowed).
-template <bool EXACT>
+template <bool EXACT, int KT>
 __device__ __forceinline__ void pair_direct(int p, const StepBufs& b, const DevScene* __restrict__ sc) {
   const uint32_t tg = b.tagA[p];
   const DevScene& S = sc[b.sceneOf[p]];
@@ -546,7 +575,7 @@ __device__ __forceinline__ void pair_direct(int p, const StepBufs& b, const DevS
     const float4 bq = b.mB[q];
     const uint32_t tq = __float_as_uint(bq.w);
     if (!c.template admit<EXACT>(S, tag_id(tq), d2, kpd2, kpid, b, q)) return;
-    c.add(S, ex, ey, tq, b.mA[q], bq);
+    c.template add<KT>(S, ex, ey, tq, b.mA[q], bq);
   });
   if (!EXACT && cnt > S.K) {
     mark_overflow(S, b, p);
@@ -557,14 +586,14 @@ __device__ __forceinline__ void pair_direct(int p, const StepBufs& b, const DevS
 
 // EXACT (the capped path) runs as a small grid-stride launch that returns at
 // once unless the step's overflow flag is set.
-template <bool EXACT>
+template <bool EXACT, int KT>
 __global__ __launch_bounds__(BLK) void k_pair(int n, StepBufs b, const DevScene* __restrict__ sc) {
   if (EXACT && b.ctl[1] == 0) return;
   if (EXACT && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&b.ctl[2], 1);
   const int nlog = (n + BLK - 1) / BLK;
   for (int lb = xcd_block(); lb < nlog; lb += gridDim.x) {
     const int p = lb * BLK + threadIdx.x;
-    if (p < n) pair_direct<EXACT>(p, b, sc);
+    if (p < n) pair_direct<EXACT, KT>(p, b, sc);
   }
 }
 
@@ -576,7 +605,7 @@ __global__ __launch_bounds__(BLK) void k_pair(int n, StepBufs b, const DevScene*
 // next pair's loads issued before the current pair's math.  A row range longer
 // than 64 slots, or more than K candidates in range, raises the step's overflow
 // flag and the exact direct kernel recomputes the step.
-template <int PB>
+template <int PB, int KT>
 __global__ __launch_bounds__(PB) void k_pair_compact(int n, StepBufs b, const DevScene* __restrict__ sc) {
   const int p = xcd_block() * PB + threadIdx.x;
   const bool live = p < n;
@@ -693,7 +722,7 @@ __global__ __launch_bounds__(PB) void k_pair_compact(int n, StepBufs b, const De
   auto pair = [&](const Nb& o) {
     const uint32_t tq = __float_as_uint(o.b.w);
     if (c.template admit<false>(S, tag_id(tq), 0.0, 0.0, 0u, b, 0))
-      c.add(S, xp.x - o.x.x, xp.y - o.x.y, tq, o.a, o.b);
+      c.template add<KT>(S, xp.x - o.x.x, xp.y - o.x.y, tq, o.a, o.b);
   };
   Nb n0, n1;
   int q0 = next();
@@ -816,6 +845,7 @@ struct uph_handle {
   hipStream_t stream = nullptr;
   int n = 0, ncells = 0, nscenes = 0;
   int itime = 0;
+  int kernel = UPH_KERNEL_CUBIC;   // smoothing kernel, the same for every scene
   std::vector<DevScene> hs;
   std::vector<int> fluid_n, tot_n;
   DevScene* dsc = nullptr;
@@ -864,6 +894,8 @@ static void fill_scene(DevScene& S, const uph_scene& u, double cell_factor) {
   S.fcd = (float)u.c_damp;
   S.fr0 = (float)(3.0 * u.dx);
   S.frho0 = (float)u.rho0;
+  S.fk7 = (float)(-2145.0 / (192512.0 * M_PI * u.h * u.h * u.h * u.h));
+  S.kernel = u.kernel;
 }
 
 static int alloc_bufs(StepBufs& b, int n, int ncells, int nblocks_scan, Err& err, bool search_only) {
@@ -908,25 +940,35 @@ static int launch_sort(hipStream_t st, int n, int ncells, int nb, StepBufs& b, c
   return 0;
 }
 
-static int launch_step(uph_handle* h, hipEvent_t pa, hipEvent_t pb) {
-  hipStream_t st = h->stream;
-  if (launch_sort(st, h->n, h->ncells, h->cs.nb, h->b, h->dsc, 0, true, h->err)) return -1;
-  if (pa) HIPCHK(hipEventRecord(pa, st), h->err);
+// The pair kernels of one smoothing kernel KT: the selected pass, then the exact
+// capped fixup (which returns at once unless the step overflowed).
+template <int KT>
+static int launch_pairs(uph_handle* h, hipStream_t st, hipEvent_t pb) {
   if (h->opts.pair_variant == 1) {
-    hipLaunchKernelGGL(k_pair<false>, dim3(grid8_of(h->n)), dim3(BLK), 0, st, h->n, h->b, h->dsc);
+    hipLaunchKernelGGL((k_pair<false, KT>), dim3(grid8_of(h->n)), dim3(BLK), 0, st, h->n, h->b, h->dsc);
   } else {
     const int pbk = h->opts.pair_block;
     const int g = (((h->n + pbk - 1) / pbk + 7) / 8) * 8;
     if (pbk == 128)
-      hipLaunchKernelGGL(k_pair_compact<128>, dim3(g), dim3(128), 0, st, h->n, h->b, h->dsc);
+      hipLaunchKernelGGL((k_pair_compact<128, KT>), dim3(g), dim3(128), 0, st, h->n, h->b, h->dsc);
     else if (pbk == 512)
-      hipLaunchKernelGGL(k_pair_compact<512>, dim3(g), dim3(512), 0, st, h->n, h->b, h->dsc);
+      hipLaunchKernelGGL((k_pair_compact<512, KT>), dim3(g), dim3(512), 0, st, h->n, h->b, h->dsc);
     else
-      hipLaunchKernelGGL(k_pair_compact<256>, dim3(g), dim3(256), 0, st, h->n, h->b, h->dsc);
+      hipLaunchKernelGGL((k_pair_compact<256, KT>), dim3(g), dim3(256), 0, st, h->n, h->b, h->dsc);
   }
   if (pb) HIPCHK(hipEventRecord(pb, st), h->err);
   const int gfix = std::min(grid8_of(h->n), FIX_GRID);
-  hipLaunchKernelGGL(k_pair<true>, dim3(gfix), dim3(BLK), 0, st, h->n, h->b, h->dsc);
+  hipLaunchKernelGGL((k_pair<true, KT>), dim3(gfix), dim3(BLK), 0, st, h->n, h->b, h->dsc);
+  return 0;
+}
+
+static int launch_step(uph_handle* h, hipEvent_t pa, hipEvent_t pb) {
+  hipStream_t st = h->stream;
+  if (launch_sort(st, h->n, h->ncells, h->cs.nb, h->b, h->dsc, 0, true, h->err)) return -1;
+  if (pa) HIPCHK(hipEventRecord(pa, st), h->err);
+  if ((h->kernel == UPH_KERNEL_POLY7 ? launch_pairs<UPH_KERNEL_POLY7>(h, st, pb)
+                                      : launch_pairs<UPH_KERNEL_CUBIC>(h, st, pb)) != 0)
+    return -1;
   HIPCHK(hipGetLastError(), h->err);
   return 0;
 }
@@ -955,6 +997,12 @@ extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts*
       delete h;
       return -1;
     }
+    if ((u.kernel != UPH_KERNEL_CUBIC && u.kernel != UPH_KERNEL_POLY7) || u.kernel != scenes[0].kernel) {
+      g_create_err = "uph_create: scene " + std::to_string(s) +
+                     " has an unknown smoothing kernel or not the kernel of scene 0";
+      delete h;
+      return -1;
+    }
     for (int i = u.n_fluid; i < np; ++i)
       if (!(u.rho[i] > 0)) {
         g_create_err = "uph_create: wall densities must be positive";
@@ -977,6 +1025,7 @@ extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts*
     return -1;
   }
   h->n = (int)n;
+  h->kernel = scenes[0].kernel;
   h->ncells = (int)nc;
   h->cs.m = (int)nc;
   h->cs.nb = (int)((nc + SCAN_TILE - 1) / SCAN_TILE);

@@ -31,6 +31,8 @@ struct DevScene {
   double t0;                // time of step 0: wall kinematics run at t0 + itime * dt
   // FP32 copies for the pair sums
   float fh, finvh, fwalpha, fav, fc0, fk, fcd, fr0, frho0;
+  float fk7;                // PolyOrderSeven gradient factor -2145 / (192512 pi h^4)
+  int kernel;               // UPH_KERNEL_*
 };
 
 __device__ __forceinline__ uint32_t tag_id(uint32_t t) { return t & UPH_ID_MASK; }

@@ -7,7 +7,8 @@
  *   FixedBoundaryParticle.dat, MovingBoundaryParticle.dat, SolidParticle.dat
  * (12-column rows, input.c:57-72 / inputbp.c:88-150).
  *
- *   uph_run <nsteps> [--fluid FILE] [--h-factor F] [--alpha A] [--graph] [--compat] [--out FILE]
+ *   uph_run <nsteps> [--fluid FILE] [--h-factor F] [--alpha A] [--kernel cubic|poly7] [--graph]
+ *           [--compat] [--out FILE] [--no-output]
  *
  * --compat runs the time loop on the host in the shape of sph.cc:88-168 over a
  * System object (include/uph_system.h) and calls the library's derivatives()
@@ -16,7 +17,8 @@
  * Constants follow System.c:292-313: c = sqrt(g H)/Mach (sqrt(2 g H)/Mach when
  * movingBoundary == 2), B = rho0 c^2 / gamma, eta = sqrt(Nn / 4 pi), hsml = eta dx
  * (input.c:107) unless --h-factor sets h = F dx.  The scheme is the library's:
- * cubic spline, Tait pressure in the momentum equation, artificial viscosity
+ * cubic spline (--kernel poly7: the PolyOrderSeven pair of CalculateKernel,
+ * derivatives.c:767-823, device loop only), Tait pressure in the momentum equation, artificial viscosity
  * `alpha` (deck field 15 unless --alpha), walls reset from the decks, the
  * piston of inputbp.c:194-207 when movingBoundary == 1.  After the run the
  * fluid state is written in the deck format (default FluidParticleFinal.dat).
@@ -254,12 +256,21 @@ int main(int argc, char **argv) {
   int nsteps = atoi(argv[1]);
   const char *fluid_file = "FluidParticle.dat", *out_file = "FluidParticleFinal.dat";
   double h_factor = 0.0, alpha = -1.0;
-  int use_graph = 0, compat = 0, write = 1;
+  int use_graph = 0, compat = 0, write = 1, kernel = UPH_KERNEL_CUBIC;
   for (int a = 2; a < argc; a++) {
     if (!strcmp(argv[a], "--fluid") && a + 1 < argc) fluid_file = argv[++a];
     else if (!strcmp(argv[a], "--h-factor") && a + 1 < argc) h_factor = atof(argv[++a]);
     else if (!strcmp(argv[a], "--alpha") && a + 1 < argc) alpha = atof(argv[++a]);
     else if (!strcmp(argv[a], "--out") && a + 1 < argc) out_file = argv[++a];
+    else if (!strcmp(argv[a], "--kernel") && a + 1 < argc) {
+      const char *k = argv[++a];
+      if (!strcmp(k, "cubic")) kernel = UPH_KERNEL_CUBIC;
+      else if (!strcmp(k, "poly7")) kernel = UPH_KERNEL_POLY7;
+      else {
+        fprintf(stderr, "uph_run: unknown kernel %s (cubic, poly7)\n", k);
+        return 1;
+      }
+    }
     else if (!strcmp(argv[a], "--graph")) use_graph = 1;
     else if (!strcmp(argv[a], "--compat")) compat = 1;
     else if (!strcmp(argv[a], "--no-output")) write = 0;
@@ -332,9 +343,14 @@ int main(int argc, char **argv) {
   s.k_spring = k0; s.c_damp = c0d; s.dt = dt; s.K = 40;
   s.motion = moving == 1 ? UPH_MOTION_PISTON : UPH_MOTION_NONE;
   s.m_amp = stroke; s.m_omega = 2.0 * M_PI / period;
+  s.kernel = kernel;
   const double margin = 10 * s.h + (moving == 1 ? stroke : 0.0);
   s.box[0] = lo[0] - margin; s.box[1] = lo[1] - margin;
   s.box[2] = hi[0] + margin; s.box[3] = hi[1] + 2 * (hi[1] - lo[1]);
+  if (compat && kernel != UPH_KERNEL_CUBIC) {
+    fprintf(stderr, "uph_run: --compat evaluates the cubic spline only\n");
+    return 1;
+  }
   if (compat) {
     /* a System as the reference's input() leaves it (System.c, input.c, inputbp.c) */
     uph_system sy;

@@ -33,6 +33,10 @@ MOTION_NONE = 0
 MOTION_PISTON_REF = 1     # x = S/2 (1 - cos wt), v = S w/2 sin wt, absolute (inputbp.c:194-207)
 MOTION_SINE_OFFSET = 2    # x = x_deck + A sin wt, v = A w cos wt
 
+# smoothing kernel of the pair sums
+KERNEL_CUBIC = 0          # cubic spline, Kernel (derivatives.c:44-66)
+KERNEL_POLY7 = 1          # PolyOrderSeven pair of CalculateKernel (derivatives.c:767-823, 1099-1101)
+
 
 @dataclasses.dataclass
 class Scene:
@@ -64,6 +68,7 @@ class Scene:
     m_amp: float = 0.0
     m_omega: float = 0.0
     box: tuple = (0.0, 0.0, 1.0, 1.0)     # (xmin, ymin, xmax, ymax) of the cell grid
+    kernel: int = KERNEL_CUBIC
     name: str = "scene"
 
     @property

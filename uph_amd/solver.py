@@ -30,7 +30,8 @@ class UphScene(C.Structure):
                 ("gamma", C.c_double), ("c0", C.c_double), ("B", C.c_double), ("g", C.c_double),
                 ("gravity", C.c_int), ("alpha_av", C.c_double), ("k_spring", C.c_double),
                 ("c_damp", C.c_double), ("dt", C.c_double), ("K", C.c_int), ("motion", C.c_int),
-                ("m_amp", C.c_double), ("m_omega", C.c_double), ("box", C.c_double * 4)]
+                ("m_amp", C.c_double), ("m_omega", C.c_double), ("box", C.c_double * 4),
+                ("kernel", C.c_int)]
 
 
 class UphDiag(C.Structure):
@@ -214,6 +215,7 @@ class GpuSolver:
                                                        sc.m_amp, sc.m_omega)
             for q in range(4):
                 u.box[q] = float(sc.box[q])
+            u.kernel = int(sc.kernel)
         opts = UphOpts(int(bool(use_graph)), int(bool(keep_rates)), int(pair_variant),
                        int(pair_block), float(cell_factor))
         self.opts = dict(use_graph=bool(use_graph), pair_variant=int(pair_variant),
