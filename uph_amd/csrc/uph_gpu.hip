@@ -36,7 +36,10 @@
 #define SCAN_BLK 256
 #define SCAN_TILE (SCAN_ITEMS * SCAN_BLK)
 #define POS_PAD 128    // zeroed slots past the end of posA (read, never used, by k_pair_compact)
-#define FIX_GRID 512   // blocks of the normally empty capped-path launches
+// Blocks of the capped-path launch: when it has work it walks every slot, so it
+// fills the chip (4 blocks of 256 per CU on 256 CUs); when it has none it returns
+// at once (measured no slower than a 512-block launch).
+#define FIX_GRID 1024
 
 // ------------------------------------------------------------------ sort
 
@@ -334,18 +337,286 @@ __device__ void kth_key(const DevScene& S, int p, const double2* __restrict__ po
   kid = ti;
 }
 
-// Overflowing query (more than K candidates in range, or a row range the
-// compacted kernel cannot hold): its K-th (d2, id) key, stamped with the step,
-// and the step's flag that makes the exact kernel run.
-__device__ void mark_overflow(const DevScene& S, const StepBufs& b, int p) {
-  int cnt;
-  double kd2;
-  uint32_t kid;
-  kth_key(S, p, b.posA, b.tagA, b.cellStart, cnt, kd2, kid);
-  b.kthD2[p] = kd2;
-  b.kthId[p] = kid;
-  b.kthStep[p] = b.ctl[3];
-  atomicOr(&b.ctl[1], 1);
+// ------------------------------------------- K-th key, one wavefront per query
+//
+// A query with more than K candidates in range needs its K-th smallest (d2, id)
+// key.  The 64 lanes of a wavefront find it together: the candidate slots are
+// spread over the lanes (consecutive slots, so the loads coalesce), the keys in
+// range are compacted into a per-wavefront LDS list, and each lane counts, for
+// the keys it owns, how many listed keys are smaller; the key of rank K - 1 is
+// the K-th.  Every candidate is read once, where a lane working alone re-reads
+// all of them once per rank.
+
+#define KTH_CAP 256   // keys in range one wavefront can list; beyond it one lane searches alone
+#ifndef GATHER_CHUNKS
+#define GATHER_CHUNKS 6   // 64-slot chunks of candidates loaded per pass
+#endif
+
+struct KthLds {
+  double d2[KTH_CAP];
+  uint32_t id[KTH_CAP];
+};
+
+// LDS written by some lanes of a wavefront and then read by others: the
+// wavefront's LDS operations complete in issue order; this keeps the compiler
+// from moving them across.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ int lanes_below(unsigned long long m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// (d2, id) < (e2, jd)
+__device__ __forceinline__ bool key_lt(double d2, uint32_t id, double e2, uint32_t jd) {
+  return d2 < e2 || (d2 == e2 && id < jd);
+}
+
+// The three candidate row ranges of a position: slots [rb[r], rb[r] + rl[r]).
+__device__ __forceinline__ void candidate_rows(const DevScene& S, double2 x, const int* __restrict__ cellStart,
+                                               int* rb, int* rl) {
+  int cx, cy;
+  cell_of(S, x.x, x.y, cx, cy);
+  const int x0 = cx > 0 ? cx - 1 : 0;
+  const int x1 = cx < S.ncx - 1 ? cx + 1 : S.ncx - 1;
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const int row = cy - 1 + r;
+    rb[r] = 0;
+    rl[r] = 0;
+    if (row >= 0 && row < S.ncy) {
+      const int base = S.coff + row * S.ncx;
+      rb[r] = cellStart[base + x0];
+      rl[r] = cellStart[base + x1 + 1] - rb[r];
+    }
+  }
+}
+
+// Lists the in-range candidates (d2 < cutoff, not q) of slot q at xq, row ranges
+// rb/rl, in L; all arguments are the same on every lane.  Returns how many are in
+// range (only the first KTH_CAP are listed).
+__device__ int wave_list(const DevScene& S, int q, double2 xq, const int* rb, const int* rl,
+                         const double2* __restrict__ pos, const uint32_t* __restrict__ tag, KthLds& L) {
+  const int lane = threadIdx.x & 63;
+  const int total = rl[0] + rl[1] + rl[2];
+  int n = 0;
+  // up to 6 chunks of 64 candidates per pass, all loads issued before any is used
+  // (a chunk past the end is skipped; slots past the end of the last chunk load
+  // the query's own row, unused)
+#pragma unroll 1
+  for (int c0 = 0; c0 < total; c0 += 64 * GATHER_CHUNKS) {
+    double2 xs[GATHER_CHUNKS];
+    uint32_t ts[GATHER_CHUNKS];
+    int sl[GATHER_CHUNKS];
+#pragma unroll
+    for (int u = 0; u < GATHER_CHUNKS; ++u) {
+      if (c0 + 64 * u >= total) break;
+      const int c = c0 + 64 * u + lane;
+      sl[u] = c >= total ? q
+              : c < rl[0] ? rb[0] + c
+              : c < rl[0] + rl[1] ? rb[1] + (c - rl[0]) : rb[2] + (c - rl[0] - rl[1]);
+      xs[u] = pos[sl[u]];
+      ts[u] = tag[sl[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < GATHER_CHUNKS; ++u) {
+      if (c0 + 64 * u >= total) break;
+      const double d2 = dist2(xq, xs[u]);
+      const bool in = sl[u] != q && d2 < S.cutoff;
+      const unsigned long long m = __ballot(in);
+      const int at = n + lanes_below(m);
+      if (in && at < KTH_CAP) {
+        L.d2[at] = d2;
+        L.id[at] = tag_id(ts[u]);
+      }
+      n += __popcll(m);
+    }
+  }
+  wave_lds_sync();
+  return n;
+}
+
+// wave_list for slot q, its position and rows read here.
+__device__ int wave_gather(const DevScene& S, int q, const double2* __restrict__ pos,
+                           const uint32_t* __restrict__ tag, const int* __restrict__ cellStart, KthLds& L) {
+  const double2 xq = pos[q];
+  int rb[3], rl[3];
+  candidate_rows(S, xq, cellStart, rb, rl);
+  return wave_list(S, q, xq, rb, rl, pos, tag, L);
+}
+
+// 1 if (d2, id) < (e2, jd), else 0, for d2, e2 >= 0 given by their bit patterns
+// (which order as integers): the borrow out of the 96-bit difference.
+__device__ __forceinline__ int key_lt_bits(uint64_t d2, uint32_t id, uint64_t e2, uint32_t jd) {
+  const int64_t t = (int64_t)d2 - (int64_t)e2 - (int64_t)(id < jd ? 1 : 0);
+  return (int)((uint64_t)t >> 63);
+}
+
+// Value of lane l (the same l on every lane) on every lane.
+__device__ __forceinline__ int bcast(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ uint64_t bcast64(uint64_t v, int l) {
+  return ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(v >> 32), l) << 32) |
+         (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
+}
+__device__ __forceinline__ double bcastd(double v, int l) {
+  return __longlong_as_double((long long)bcast64((uint64_t)__double_as_longlong(v), l));
+}
+
+// The key of rank K - 1 among the n <= 64 NS listed keys (n > K), on every lane,
+// by quickselect over the wavefront: a lane owns keys lane + 64 s, s < NS; each
+// round takes one still-possible key as the pivot, counts the keys below it (one
+// comparison per owned key and a ballot), and keeps the side that holds rank
+// K - 1.  Every round removes at least the pivot, so it ends within n rounds;
+// the pivot position is drawn from a fixed sequence, so the result and the work
+// are the same on every run.
+template <int NS>
+__device__ void wave_kth_listed(const KthLds& L, int n, int K, uint32_t seed, double& kd2, uint32_t& kid) {
+  const int lane = threadIdx.x & 63;
+  uint64_t kd[NS];
+  uint32_t ki[NS];
+  bool act[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    const int j = lane + 64 * s;
+    act[s] = j < n;
+    kd[s] = act[s] ? (uint64_t)__double_as_longlong(L.d2[j]) : 0ull;
+    ki[s] = act[s] ? L.id[j] : 0u;
+  }
+  int below = 0;   // keys ranked below every key still possible
+  uint32_t rnd = seed * 2654435761u + 1u;
+#pragma unroll 1
+  for (int round = 0; round < n; ++round) {
+    unsigned long long am[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) am[s] = __ballot(act[s]);
+    // pivot: the first possible key at or after a drawn (slot, lane) position
+    rnd = rnd * 1664525u + 1013904223u;
+    const int s0 = (int)(rnd >> 28) & (NS - 1), l0 = (int)(rnd >> 22) & 63;
+    int ps = -1, pl = 0;
+#pragma unroll
+    for (int t = 0; t < NS; ++t) {
+      const int s = (s0 + t) & (NS - 1);
+      unsigned long long msk = 0ull;
+#pragma unroll
+      for (int u = 0; u < NS; ++u)
+        if (u == s) msk = am[u];
+      if (ps < 0 && msk) {
+        const unsigned long long r = l0 ? ((msk >> l0) | (msk << (64 - l0))) : msk;
+        pl = (__ffsll((long long)r) - 1 + l0) & 63;
+        ps = s;
+      }
+    }
+    uint64_t pd = 0ull;
+    uint32_t pi = 0u;
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+      if (s == ps) {
+        pd = bcast64(kd[s], pl);
+        pi = (uint32_t)bcast((int)ki[s], pl);
+      }
+    bool lt[NS];
+    int c = 0;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      lt[s] = act[s] && key_lt_bits(kd[s], ki[s], pd, pi);
+      c += __popcll(__ballot(lt[s]));
+    }
+    const int r = below + c;   // rank of the pivot
+    if (r == K - 1) {
+      kd2 = __longlong_as_double((long long)pd);
+      kid = pi;
+      break;
+    }
+    if (r > K - 1) {
+#pragma unroll
+      for (int s = 0; s < NS; ++s) act[s] = lt[s];
+    } else {
+      below = r + 1;
+#pragma unroll
+      for (int s = 0; s < NS; ++s) act[s] = act[s] && !lt[s] && ki[s] != pi;
+    }
+  }
+  wave_lds_sync();
+}
+
+// K-th key of slot q on every lane (+inf when at most K are in range), from its
+// n in-range candidates listed in L.
+__device__ void wave_kth_of_list(const DevScene& S, int q, const double2* __restrict__ pos,
+                                 const uint32_t* __restrict__ tag, const int* __restrict__ cellStart, KthLds& L,
+                                 int n, double& kd2, uint32_t& kid) {
+  kd2 = INFINITY;
+  kid = 0xffffffffu;
+  if (n <= S.K) return;
+  if (n <= KTH_CAP) {
+    // as few owned-key slots as the list needs (the count is the same on every lane)
+    if (n <= 128)
+      wave_kth_listed<2>(L, n, S.K, (uint32_t)q, kd2, kid);
+    else
+      wave_kth_listed<KTH_CAP / 64>(L, n, S.K, (uint32_t)q, kd2, kid);
+    return;
+  }
+  // more in range than the list holds: one lane searches alone
+  if ((threadIdx.x & 63) == 0) {
+    int c;
+    kth_key(S, q, pos, tag, cellStart, c, kd2, kid);
+  }
+  kd2 = bcastd(kd2, 0);
+  kid = (uint32_t)bcast((int)kid, 0);
+}
+
+// K-th key of slot q on every lane; n: the number in range.
+__device__ void wave_kth(const DevScene& S, int q, const double2* __restrict__ pos, const uint32_t* __restrict__ tag,
+                         const int* __restrict__ cellStart, KthLds& L, int& n, double& kd2, uint32_t& kid) {
+  n = wave_gather(S, q, pos, tag, cellStart, L);
+  wave_kth_of_list(S, q, pos, tag, cellStart, L, n, kd2, kid);
+}
+
+// The overflowing queries of this wavefront (more than K candidates in range,
+// or a row range the compacted kernel cannot hold; ovf set on their lanes): each
+// gets its K-th key, stamped with the step, and the step's flag makes the exact
+// kernel run.  Every lane of the wavefront calls this.  With ROWS, each lane
+// passes its own scene index, position and candidate rows, and the wavefront takes
+// them from the query's lane instead of reading them again.
+template <bool ROWS>
+__device__ void wave_overflow(const StepBufs& b, const DevScene* __restrict__ sc, int p, bool ovf, KthLds& L,
+                              int sidx = 0, double2 xp = double2{}, const int* rbp = nullptr,
+                              const int* rlp = nullptr) {
+  unsigned long long m = __ballot(ovf);
+  if (m == 0ull) return;
+  const bool lead = (threadIdx.x & 63) == 0;
+  if (lead) atomicOr(&b.ctl[1], 1);
+#pragma unroll 1
+  while (m) {
+    const int l = __ffsll((long long)m) - 1;
+    m &= m - 1ull;
+    const int q = bcast(p, l);
+    int n;
+    double kd2;
+    uint32_t kid;
+    if (ROWS) {
+      const DevScene& S = sc[bcast(sidx, l)];
+      const double2 xq = make_double2(bcastd(xp.x, l), bcastd(xp.y, l));
+      int rb[3], rl[3];
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        rb[r] = bcast(rbp[r], l);
+        rl[r] = bcast(rlp[r], l);
+      }
+      n = wave_list(S, q, xq, rb, rl, b.posA, b.tagA, L);
+      wave_kth_of_list(S, q, b.posA, b.tagA, b.cellStart, L, n, kd2, kid);
+    } else {
+      const DevScene& S = sc[b.sceneOf[q]];
+      wave_kth(S, q, b.posA, b.tagA, b.cellStart, L, n, kd2, kid);
+    }
+    if (lead) {
+      b.kthD2[q] = kd2;
+      b.kthId[q] = kid;
+      b.kthStep[q] = b.ctl[3];
+    }
+  }
 }
 
 // K-th key of slot q in this step, or +inf when q did not overflow.
@@ -546,8 +817,10 @@ __device__ __forceinline__ void correct(const DevScene& S, const StepBufs& b, in
 
 // Direct form: one pass over the candidates, physics inside the distance test.
 // EXACT applies the K-nearest cap (run only when some query overflowed).
+// Returns true for a query with more than K candidates in range when !EXACT
+// (nothing written; its K-th key is left to wave_overflow).
 template <bool EXACT, int KT>
-__device__ __forceinline__ void pair_direct(int p, const StepBufs& b, const DevScene* __restrict__ sc) {
+__device__ __forceinline__ bool pair_direct(int p, const StepBufs& b, const DevScene* __restrict__ sc) {
   const uint32_t tg = b.tagA[p];
   const DevScene& S = sc[b.sceneOf[p]];
   const int itime = b.ctl[0];
@@ -556,7 +829,7 @@ __device__ __forceinline__ void pair_direct(int p, const StepBufs& b, const DevS
   const float mass = b.mB[p].z;
   if (tag_type(tg) != 1) {
     wall_update(S, b, p, tg, itime, mass);
-    return;
+    return false;
   }
   PairCtx c;
   c.init(S, tg, b.mA[p], b.mB[p]);
@@ -577,11 +850,9 @@ __device__ __forceinline__ void pair_direct(int p, const StepBufs& b, const DevS
     if (!c.template admit<EXACT>(S, tag_id(tq), d2, kpd2, kpid, b, q)) return;
     c.template add<KT>(S, ex, ey, tq, b.mA[q], bq);
   });
-  if (!EXACT && cnt > S.K) {
-    mark_overflow(S, b, p);
-    return;
-  }
+  if (!EXACT && cnt > S.K) return true;
   correct(S, b, p, itime, xp, vn, mass, c);
+  return false;
 }
 
 // EXACT (the capped path) runs as a small grid-stride launch that returns at
@@ -590,102 +861,24 @@ template <bool EXACT, int KT>
 __global__ __launch_bounds__(BLK) void k_pair(int n, StepBufs b, const DevScene* __restrict__ sc) {
   if (EXACT && b.ctl[1] == 0) return;
   if (EXACT && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&b.ctl[2], 1);
+  __shared__ KthLds lds[EXACT ? 1 : BLK / 64];
   const int nlog = (n + BLK - 1) / BLK;
+  // the trip count is the same for the whole block, so every wavefront reaches
+  // wave_overflow with all of its lanes
   for (int lb = xcd_block(); lb < nlog; lb += gridDim.x) {
     const int p = lb * BLK + threadIdx.x;
-    if (p < n) pair_direct<EXACT, KT>(p, b, sc);
+    bool ovf = false;
+    if (p < n) ovf = pair_direct<EXACT, KT>(p, b, sc);
+    if (!EXACT) wave_overflow<false>(b, sc, p, ovf, lds[threadIdx.x >> 6]);
   }
 }
 
-// Compacted form (default).  Pass 1 runs only the FP64 distance test over the
-// three candidate row ranges, in a branch-free loop whose counter is uniform
-// across the wavefront (each lane walks its own row range by pointer), and
-// records in-range candidates as bits of one 64-bit mask per row.  Pass 2 walks
-// the set bits, so every lane spends its physics on in-range pairs only, with the
-// next pair's loads issued before the current pair's math.  A row range longer
-// than 64 slots, or more than K candidates in range, raises the step's overflow
-// flag and the exact direct kernel recomputes the step.
-template <int PB, int KT>
-__global__ __launch_bounds__(PB) void k_pair_compact(int n, StepBufs b, const DevScene* __restrict__ sc) {
-  const int p = xcd_block() * PB + threadIdx.x;
-  const bool live = p < n;
-  const int pp = live ? p : n - 1;
-  const uint32_t tg = b.tagA[pp];
-  const bool fluid = live && tag_type(tg) == 1;
-  const DevScene& S = sc[b.sceneOf[pp]];
-  const int itime = b.ctl[0];
-  const double2 xp = b.posA[pp];
-  const double cut = S.cutoff;
-  int cx, cy;
-  cell_of(S, xp.x, xp.y, cx, cy);
-  const int x0 = cx > 0 ? cx - 1 : 0;
-  const int x1 = cx < S.ncx - 1 ? cx + 1 : S.ncx - 1;
-  int rb[3], rl[3];
-  unsigned long long m[3];
-  bool bad = false;
-#pragma unroll
-  for (int r = 0; r < 3; ++r) {
-    const int row = cy - 1 + r;
-    rb[r] = 0;
-    rl[r] = 0;
-    m[r] = 0ull;
-    if (fluid && row >= 0 && row < S.ncy) {
-      const int base = S.coff + row * S.ncx;
-      rb[r] = b.cellStart[base + x0];
-      rl[r] = b.cellStart[base + x1 + 1] - rb[r];
-    }
-    if (rl[r] > 64) {
-      bad = true;
-      rl[r] = 0;
-    }
-    // the trip count is the wave's maximum row length (uniform, in an SGPR); slots
-    // past this lane's row end are tested too and their bits cleared below
-    const int lmax = __builtin_amdgcn_readfirstlane(wave_max(rl[r]));
-    const double2* rp = b.posA + rb[r];
-    uint32_t mw[2] = {0u, 0u};
-#pragma unroll
-    for (int hw = 0; hw < 2; ++hw) {
-      const int k0 = 32 * hw;
-      const int kend = hw == 0 ? (lmax < 32 ? lmax : 32) : lmax;
-      // candidate k0 + j enters as the low bit and is shifted up by each later one:
-      // one add-with-carry per candidate; bit-reversed into place after the loop
-      uint32_t bits = 0u;
-#pragma unroll 1
-      for (int k = k0; k < kend; k += 4) {
-        double2 xs[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) xs[u] = rp[u];   // posA has POS_PAD zeroed slots past n
-        rp += 4;
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const double ex = xp.x - xs[u].x, ey = xp.y - xs[u].y;
-          const double d2 = __dadd_rn(__dmul_rn(ex, ex), __dmul_rn(ey, ey));
-          bits = bits + bits + (d2 < cut ? 1u : 0u);
-        }
-      }
-      const int nb = (kend - k0 + 3) & ~3;   // candidates tested in this half (uniform)
-      if (nb > 0) mw[hw] = __builtin_bitreverse32(bits) >> (32 - nb);
-    }
-    const int l = rl[r];
-    mw[0] &= l >= 32 ? 0xffffffffu : ((1u << l) - 1u);
-    mw[1] &= l >= 64 ? 0xffffffffu : (l > 32 ? ((1u << (l - 32)) - 1u) : 0u);
-    m[r] = ((unsigned long long)mw[1] << 32) | mw[0];
-  }
-  if (!live) return;
-  const double2 vn = b.velA[p];
-  const float mass = b.mB[p].z;
-  if (!fluid) {
-    wall_update(S, b, p, tg, itime, mass);
-    return;
-  }
-  // drop the particle itself (always in range, in its own row)
-  const int self = p - rb[1];
-  if (self >= 0 && self < 64) m[1] &= ~(1ull << self);
-  const int cnt = __popcll(m[0]) + __popcll(m[1]) + __popcll(m[2]);
-  if (bad || cnt > S.K) {
-    mark_overflow(S, b, p);
-    return;   // the exact kernel recomputes this step
-  }
+// Pass 2 of k_pair_compact for one fluid particle: the pair sums over the set bits
+// of m[0..2] (row starts rb[0..2]) and the corrector.
+template <int KT>
+__device__ __forceinline__ void compact_pairs(const DevScene& S, const StepBufs& b, int p, uint32_t tg, int itime,
+                                              double2 xp, double2 vn, float mass, const int* rb,
+                                              const unsigned long long* m) {
   PairCtx c;
   c.init(S, tg, b.mA[p], b.mB[p]);
   // the set bits of the three rows as one stream, advanced without branches;
@@ -740,67 +933,279 @@ __global__ __launch_bounds__(PB) void k_pair_compact(int n, StepBufs b, const De
   correct(S, b, p, itime, xp, vn, mass, c);
 }
 
+// Compacted form (default).  Pass 1 runs only the FP64 distance test over the
+// three candidate row ranges, in a branch-free loop whose counter is uniform
+// across the wavefront (each lane walks its own row range by pointer), and
+// records in-range candidates as bits of one 64-bit mask per row.  Pass 2 walks
+// the set bits, so every lane spends its physics on in-range pairs only, with the
+// next pair's loads issued before the current pair's math.  A row range longer
+// than 64 slots, or more than K candidates in range, raises the step's overflow
+// flag and the exact direct kernel recomputes the step.
+template <int PB, int KT>
+__global__ __launch_bounds__(PB) void k_pair_compact(int n, StepBufs b, const DevScene* __restrict__ sc) {
+  __shared__ KthLds lds[PB / 64];
+  const int p = xcd_block() * PB + threadIdx.x;
+  const bool live = p < n;
+  const int pp = live ? p : n - 1;
+  const uint32_t tg = b.tagA[pp];
+  const bool fluid = live && tag_type(tg) == 1;
+  const int sidx = b.sceneOf[pp];
+  const DevScene& S = sc[sidx];
+  const int itime = b.ctl[0];
+  const double2 xp = b.posA[pp];
+  const double cut = S.cutoff;
+  int cx, cy;
+  cell_of(S, xp.x, xp.y, cx, cy);
+  const int x0 = cx > 0 ? cx - 1 : 0;
+  const int x1 = cx < S.ncx - 1 ? cx + 1 : S.ncx - 1;
+  int rb[3], rl[3], rlf[3];   // rlf: the row lengths before the 64-slot limit
+  unsigned long long m[3];
+  bool bad = false;
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const int row = cy - 1 + r;
+    rb[r] = 0;
+    rl[r] = 0;
+    m[r] = 0ull;
+    if (fluid && row >= 0 && row < S.ncy) {
+      const int base = S.coff + row * S.ncx;
+      rb[r] = b.cellStart[base + x0];
+      rl[r] = b.cellStart[base + x1 + 1] - rb[r];
+    }
+    rlf[r] = rl[r];
+    if (rl[r] > 64) {
+      bad = true;
+      rl[r] = 0;
+    }
+    // the trip count is the wave's maximum row length (uniform, in an SGPR); slots
+    // past this lane's row end are tested too and their bits cleared below
+    const int lmax = __builtin_amdgcn_readfirstlane(wave_max(rl[r]));
+    const double2* rp = b.posA + rb[r];
+    uint32_t mw[2] = {0u, 0u};
+#pragma unroll
+    for (int hw = 0; hw < 2; ++hw) {
+      const int k0 = 32 * hw;
+      const int kend = hw == 0 ? (lmax < 32 ? lmax : 32) : lmax;
+      // candidate k0 + j enters as the low bit and is shifted up by each later one:
+      // one add-with-carry per candidate; bit-reversed into place after the loop
+      uint32_t bits = 0u;
+#pragma unroll 1
+      for (int k = k0; k < kend; k += 4) {
+        double2 xs[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) xs[u] = rp[u];   // posA has POS_PAD zeroed slots past n
+        rp += 4;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const double ex = xp.x - xs[u].x, ey = xp.y - xs[u].y;
+          const double d2 = __dadd_rn(__dmul_rn(ex, ex), __dmul_rn(ey, ey));
+          bits = bits + bits + (d2 < cut ? 1u : 0u);
+        }
+      }
+      const int nb = (kend - k0 + 3) & ~3;   // candidates tested in this half (uniform)
+      if (nb > 0) mw[hw] = __builtin_bitreverse32(bits) >> (32 - nb);
+    }
+    const int l = rl[r];
+    mw[0] &= l >= 32 ? 0xffffffffu : ((1u << l) - 1u);
+    mw[1] &= l >= 64 ? 0xffffffffu : (l > 32 ? ((1u << (l - 32)) - 1u) : 0u);
+    m[r] = ((unsigned long long)mw[1] << 32) | mw[0];
+  }
+  // every lane reaches wave_overflow: the work of a lane is a call of this
+  bool ovf = false;
+  auto run = [&]() {
+    if (!live) return;
+    const double2 vn = b.velA[p];
+    const float mass = b.mB[p].z;
+    if (!fluid) {
+      wall_update(S, b, p, tg, itime, mass);
+      return;
+    }
+    // drop the particle itself (always in range, in its own row)
+    const int self = p - rb[1];
+    if (self >= 0 && self < 64) m[1] &= ~(1ull << self);
+    const int cnt = __popcll(m[0]) + __popcll(m[1]) + __popcll(m[2]);
+    if (bad || cnt > S.K) {
+      ovf = true;   // the exact kernel recomputes this step
+      return;
+    }
+    compact_pairs<KT>(S, b, p, tg, itime, xp, vn, mass, rb, m);
+  };
+  run();
+  wave_overflow<true>(b, sc, p, ovf, lds[threadIdx.x >> 6], sidx, xp, rb, rlf);
+}
+
 // ---------------------------------------------------------------- search
 
+// Pairs of query p that SearchNeighbors emits (Search.cc:27-40): in range,
+// j > i, j < ParticleBoundary, key <= the K-th key (kd2, kid).
+__device__ __forceinline__ bool search_keeps(const DevScene& S, uint32_t idp, double d2, uint32_t idq, double kd2,
+                                             uint32_t kid) {
+  return idq > idp && (int)idq < S.pb_gid && key_le(d2, idq, kd2, kid);
+}
+
+// Per query: the K-th key (when more than K are in range) and the number of
+// pairs it emits.  A lane counts its own query in one pass; queries with more
+// than K in range are left to the whole wavefront.
 __global__ __launch_bounds__(BLK) void k_search_count(int nf_slots_unused, int n, StepBufs b, const DevScene* __restrict__ sc,
                                                       int* __restrict__ qcount) {
+  __shared__ KthLds lds[BLK / 64];
   const int p = xcd_block() * BLK + threadIdx.x;
-  if (p >= n) return;
-  const uint32_t tg = b.tagA[p];
-  if (tag_type(tg) != 1) return;
   const DevScene& S = sc[0];
-  int cnt;
-  double kd2;
-  uint32_t kid;
-  kth_key(S, p, b.posA, b.tagA, b.cellStart, cnt, kd2, kid);
-  b.kthD2[p] = kd2;
-  b.kthId[p] = kid;
-  const uint32_t idp = tag_id(tg);
-  const double2 pp = b.posA[p];
-  int m = 0;
-  for_candidates(S, pp, b.cellStart, [&](int q) {
-    if (q == p) return;
-    const double d2 = dist2(pp, b.posA[q]);
-    if (!(d2 < S.cutoff)) return;
+  bool capped = false;
+  if (p < n && tag_type(b.tagA[p]) == 1) {
+    const uint32_t idp = tag_id(b.tagA[p]);
+    const double2 pp = b.posA[p];
+    int c = 0, m = 0;
+    for_candidates(S, pp, b.cellStart, [&](int q) {
+      if (q == p) return;
+      const double d2 = dist2(pp, b.posA[q]);
+      if (!(d2 < S.cutoff)) return;
+      ++c;
+      const uint32_t idq = tag_id(b.tagA[q]);
+      if (idq > idp && (int)idq < S.pb_gid) ++m;
+    });
+    if (c > S.K) {
+      capped = true;
+    } else {
+      b.kthD2[p] = INFINITY;
+      b.kthId[p] = 0xffffffffu;
+      qcount[idp] = m;
+    }
+  }
+  unsigned long long w = __ballot(capped);
+  const int lane = threadIdx.x & 63;
+  KthLds& L = lds[threadIdx.x >> 6];
+#pragma unroll 1
+  while (w) {
+    const int l = __ffsll((long long)w) - 1;
+    w &= w - 1ull;
+    const int q = bcast(p, l);
     const uint32_t idq = tag_id(b.tagA[q]);
-    if (idq > idp && (int)idq < S.pb_gid && key_le(d2, idq, kd2, kid)) ++m;
-  });
-  qcount[idp] = m;
+    int nin;
+    double kd2;
+    uint32_t kid;
+    wave_kth(S, q, b.posA, b.tagA, b.cellStart, L, nin, kd2, kid);
+    int m = 0;
+    if (nin <= KTH_CAP) {
+      // the listed keys this lane owns that the query keeps
+      for (int j = lane; j < nin; j += 64) m += search_keeps(S, idq, L.d2[j], L.id[j], kd2, kid) ? 1 : 0;
+      for (int o = 32; o > 0; o >>= 1) m += __shfl_xor(m, o, 64);
+    } else if (lane == 0) {
+      const double2 pq = b.posA[q];
+      for_candidates(S, pq, b.cellStart, [&](int u) {
+        if (u == q) return;
+        const double d2 = dist2(pq, b.posA[u]);
+        if (d2 < S.cutoff && search_keeps(S, idq, d2, tag_id(b.tagA[u]), kd2, kid)) ++m;
+      });
+    }
+    if (lane == 0) {
+      b.kthD2[q] = kd2;
+      b.kthId[q] = kid;
+      qcount[idq] = m;
+    }
+    wave_lds_sync();
+  }
 }
+
+// Writes the pairs of query p in (d2, id) order from qoff[id_p].  A lane writes
+// its own query's pairs when they are few (each pair is ranked by a pass over
+// the candidates); longer lists are ranked by the whole wavefront.
+#define EMIT_LANE_MAX 16
 
 __global__ __launch_bounds__(BLK) void k_search_emit(int n, StepBufs b, const DevScene* __restrict__ sc,
                                                      const int* __restrict__ qoff, int* __restrict__ ip,
                                                      int* __restrict__ jp, double* __restrict__ r2) {
+  __shared__ KthLds lds[BLK / 64];
   const int p = xcd_block() * BLK + threadIdx.x;
-  if (p >= n) return;
-  const uint32_t tg = b.tagA[p];
-  if (tag_type(tg) != 1) return;
   const DevScene& S = sc[0];
-  const uint32_t idp = tag_id(tg);
-  const double kd2 = b.kthD2[p];
-  const uint32_t kid = b.kthId[p];
-  const double2 pp = b.posA[p];
-  const int o = qoff[idp];
-  for_candidates(S, pp, b.cellStart, [&](int q) {
-    if (q == p) return;
-    const double d2 = dist2(pp, b.posA[q]);
-    if (!(d2 < S.cutoff)) return;
+  bool wide = false;
+  if (p < n && tag_type(b.tagA[p]) == 1) {
+    const uint32_t idp = tag_id(b.tagA[p]);
+    const int o = qoff[idp];
+    if (qoff[idp + 1] - o > EMIT_LANE_MAX) {
+      wide = true;
+    } else {
+      const double kd2 = b.kthD2[p];
+      const uint32_t kid = b.kthId[p];
+      const double2 pp = b.posA[p];
+      for_candidates(S, pp, b.cellStart, [&](int q) {
+        if (q == p) return;
+        const double d2 = dist2(pp, b.posA[q]);
+        if (!(d2 < S.cutoff)) return;
+        const uint32_t idq = tag_id(b.tagA[q]);
+        if (!search_keeps(S, idp, d2, idq, kd2, kid)) return;
+        // rank among the kept candidates by (d2, id)
+        int rnk = 0;
+        for_candidates(S, pp, b.cellStart, [&](int u) {
+          if (u == p) return;
+          const double e2 = dist2(pp, b.posA[u]);
+          if (!(e2 < S.cutoff)) return;
+          const uint32_t idu = tag_id(b.tagA[u]);
+          if (search_keeps(S, idp, e2, idu, kd2, kid) && key_lt(e2, idu, d2, idq)) ++rnk;
+        });
+        ip[o + rnk] = (int)idp;
+        jp[o + rnk] = (int)idq;
+        r2[o + rnk] = d2;
+      });
+    }
+  }
+  unsigned long long w = __ballot(wide);
+  const int lane = threadIdx.x & 63;
+  KthLds& L = lds[threadIdx.x >> 6];
+#pragma unroll 1
+  while (w) {
+    const int l = __ffsll((long long)w) - 1;
+    w &= w - 1ull;
+    const int q = bcast(p, l);
     const uint32_t idq = tag_id(b.tagA[q]);
-    if (!(idq > idp && (int)idq < S.pb_gid && key_le(d2, idq, kd2, kid))) return;
-    // rank among the qualifying candidates by (d2, id)
-    int rnk = 0;
-    for_candidates(S, pp, b.cellStart, [&](int u) {
-      if (u == p) return;
-      const double e2 = dist2(pp, b.posA[u]);
-      if (!(e2 < S.cutoff)) return;
-      const uint32_t idu = tag_id(b.tagA[u]);
-      if (!(idu > idp && (int)idu < S.pb_gid && key_le(e2, idu, kd2, kid))) return;
-      if (e2 < d2 || (e2 == d2 && idu < idq)) ++rnk;
-    });
-    ip[o + rnk] = (int)idp;
-    jp[o + rnk] = (int)idq;
-    r2[o + rnk] = d2;
-  });
+    const double kd2 = b.kthD2[q];
+    const uint32_t kid = b.kthId[q];
+    const int o = qoff[idq];
+    const int nin = wave_gather(S, q, b.posA, b.tagA, b.cellStart, L);
+    if (nin <= KTH_CAP) {
+      // each kept key this lane owns, ranked among the kept keys
+      for (int j0 = 0; j0 < nin; j0 += 64) {
+        const int j = j0 + lane;
+        const bool own = j < nin;
+        const double dj = own ? L.d2[j] : 0.0;
+        const uint32_t jd = own ? L.id[j] : 0u;
+        const bool keep = own && search_keeps(S, idq, dj, jd, kd2, kid);
+        int rnk = 0;
+#pragma unroll 2
+        for (int i = 0; i < nin; ++i) {
+          const double di = L.d2[i];
+          const uint32_t id = L.id[i];
+          rnk += (search_keeps(S, idq, di, id, kd2, kid) && key_lt(di, id, dj, jd)) ? 1 : 0;
+        }
+        if (keep) {
+          ip[o + rnk] = (int)idq;
+          jp[o + rnk] = (int)jd;
+          r2[o + rnk] = dj;
+        }
+      }
+    } else if (lane == 0) {
+      const double2 pq = b.posA[q];
+      for_candidates(S, pq, b.cellStart, [&](int u) {
+        if (u == q) return;
+        const double d2 = dist2(pq, b.posA[u]);
+        if (!(d2 < S.cutoff)) return;
+        const uint32_t idu = tag_id(b.tagA[u]);
+        if (!search_keeps(S, idq, d2, idu, kd2, kid)) return;
+        int rnk = 0;
+        for_candidates(S, pq, b.cellStart, [&](int v) {
+          if (v == q) return;
+          const double e2 = dist2(pq, b.posA[v]);
+          if (!(e2 < S.cutoff)) return;
+          const uint32_t idv = tag_id(b.tagA[v]);
+          if (search_keeps(S, idq, e2, idv, kd2, kid) && key_lt(e2, idv, d2, idu)) ++rnk;
+        });
+        ip[o + rnk] = (int)idq;
+        jp[o + rnk] = (int)idu;
+        r2[o + rnk] = d2;
+      });
+    }
+    wave_lds_sync();
+  }
 }
 
 // ================================================================== host
