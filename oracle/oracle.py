@@ -25,7 +25,8 @@ class Params(C.Structure):
                 ("g", C.c_double), ("gravity", C.c_int), ("alpha_av", C.c_double),
                 ("k_spring", C.c_double), ("c_damp", C.c_double), ("dt", C.c_double),
                 ("K", C.c_int), ("motion", C.c_int), ("m_amp", C.c_double),
-                ("m_omega", C.c_double), ("kernel", C.c_int)]
+                ("m_omega", C.c_double), ("kernel", C.c_int), ("mode", C.c_int),
+                ("kappa", C.c_double), ("mu", C.c_double), ("eps", C.c_double)]
 
 
 def build() -> None:
@@ -63,6 +64,8 @@ def lib():
         L.orc_derivatives.restype = C.c_int
         L.orc_get_state.argtypes = [C.c_void_p] + [C.c_void_p] * 5
         L.orc_get_rates.argtypes = [C.c_void_p] + [C.c_void_p] * 5
+        L.orc_get_ref.argtypes = [C.c_void_p] + [C.c_void_p] * 7
+        L.orc_set_local.argtypes = [C.c_void_p] + [C.c_void_p] * 3
         L.orc_niac.argtypes = [C.c_void_p]
         L.orc_niac.restype = C.c_int
         L.orc_itime.argtypes = [C.c_void_p]
@@ -107,7 +110,9 @@ def search(pos, nf, n, pb, K, dist, brute=False, cap=None, nthreads=0):
 def params_of(sc) -> Params:
     return Params(sc.dx, sc.h, sc.rho0, sc.gamma, sc.c0, sc.B, sc.g, int(sc.gravity),
                   sc.alpha_av, sc.k_spring, sc.c_damp, sc.dt, int(sc.K), int(sc.motion),
-                  sc.m_amp, sc.m_omega, int(getattr(sc, "kernel", 0)))
+                  sc.m_amp, sc.m_omega, int(getattr(sc, "kernel", 0)), int(getattr(sc, "mode", 0)),
+                  float(getattr(sc, "kappa", 0.0)), float(getattr(sc, "mu", 0.0)),
+                  float(getattr(sc, "eps", 0.0)))
 
 
 class Sim:
@@ -159,3 +164,19 @@ class Sim:
         lib().orc_get_rates(self.h, R.ctypes.data, Rt.ctypes.data, a.ctypes.data,
                             at.ctypes.data, P.ctypes.data)
         return dict(R=R, Rt=Rt, a=a, at=at, P=P)
+
+    def ref(self):
+        """Ref-mode fields: DynamicPressure P, localPressure Ploc, dpdt RP, dpdt_approx
+        RPt, fluctuations drho, dP (n) and dv (n, 2)."""
+        n = self.n
+        out = {k: np.zeros(n) for k in ("P", "Ploc", "RP", "RPt", "drho", "dP")}
+        out["dv"] = np.zeros((n, 2))
+        lib().orc_get_ref(self.h, *[out[k].ctypes.data for k in
+                                    ("P", "Ploc", "RP", "RPt", "drho", "dP", "dv")])
+        return out
+
+    def set_local(self, rholoc=None, Ploc=None, vloc=None):
+        """Sets localDensity / localPressure / localVelocity before a derivatives() call."""
+        f = lambda a, m: None if a is None else np.ascontiguousarray(a, np.float64).reshape(-1)
+        self._loc = [f(rholoc, 1), f(Ploc, 1), f(vloc, 2)]
+        lib().orc_set_local(self.h, *[None if a is None else a.ctypes.data for a in self._loc])

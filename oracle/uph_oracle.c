@@ -48,6 +48,8 @@ typedef struct {
   int K, motion;
   double m_amp, m_omega;
   int kernel;                           /* 0: cubic spline (derivatives.c:44-66), 1: PolyOrderSeven (:767-823) */
+  int mode;                             /* 0: config mode, 1: the reference's own scheme (ref mode) */
+  double kappa, mu, eps;                /* ref mode: thermalDiffusivity0, viscosity0, epsilon */
 } orc_params;
 
 typedef struct {
@@ -56,6 +58,9 @@ typedef struct {
   double *x, *v, *vmin, *vloc, *rho, *rhomin, *rholoc, *P, *mass, *hsml;
   int *itype;
   double *R, *Rt, *a, *at;              /* ddensitydt, ddensitydt_approx, dvdt, dvdt_approx */
+  /* ref mode: DynamicPressure_min, localPressure, dpdt, dpdt_approx, the fluctuation
+   * fields, the pair kernel values w, spongeFactor */
+  double *Pmin, *Ploc, *RP, *RPt, *drho, *dP, *dv, *w, *sponge;
   double *deck_x, *deck_v, *deck_rho, *deck_P, *deck_lvel;
   int *ip, *jp;
   double *rij2, *dwdx;
@@ -334,6 +339,7 @@ orc_sim *orc_create(int nf, int nfixed, int nmoving, const orc_params *p, const 
 #define A(f, c) s->f = calloc(n1 * (c), sizeof(double))
   A(x, 2); A(v, 2); A(vmin, 2); A(vloc, 2); A(rho, 1); A(rhomin, 1); A(rholoc, 1); A(P, 1);
   A(mass, 1); A(hsml, 1); A(R, 1); A(Rt, 1); A(a, 2); A(at, 2);
+  A(Pmin, 1); A(Ploc, 1); A(RP, 1); A(RPt, 1); A(drho, 1); A(dP, 1); A(dv, 2); A(sponge, 1);
   A(deck_x, 2); A(deck_v, 2); A(deck_rho, 1); A(deck_P, 1); A(deck_lvel, 2);
 #undef A
   s->itype = calloc(n1, sizeof(int));
@@ -351,8 +357,17 @@ orc_sim *orc_create(int nf, int nfixed, int nmoving, const orc_params *p, const 
   memcpy(s->deck_lvel, lvel, sizeof(double) * 2 * n);
   for (int i = 0; i < n; i++) {
     s->hsml[i] = p->h;
-    /* config-mode choice: localDensity starts equal to density (input.c:109 sets rho0) */
-    s->rholoc[i] = i < nf ? rho[i] : p->rho0;
+    if (p->mode == 1) {
+      /* input.c:109-110: localDensity = rho0, localPressure = 0 */
+      s->rholoc[i] = p->rho0;
+      s->Ploc[i] = 0.0;
+      /* input.c:137-143 with System.c:333-335 (spongeFactor0 = 0.2, spongePosition0 = 7,
+       * waveAbsorption = 1); SpongeLayerWaveReflectionAbsorption acts for movingBoundary == 1 */
+      s->sponge[i] = (i < nf && p->motion == 1 && pos[2 * i] >= 7.0) ? 0.2 : 0.0;
+    } else {
+      /* config-mode choice: localDensity starts equal to density (input.c:109 sets rho0) */
+      s->rholoc[i] = i < nf ? rho[i] : p->rho0;
+    }
   }
   memcpy(s->rhomin, rho, sizeof(double) * n);
   memcpy(s->vmin, vel, sizeof(double) * 2 * n);
@@ -361,6 +376,7 @@ orc_sim *orc_create(int nf, int nfixed, int nmoving, const orc_params *p, const 
   s->jp = malloc(sizeof(int) * s->cap);
   s->rij2 = malloc(sizeof(double) * s->cap);
   s->dwdx = malloc(sizeof(double) * 2 * s->cap);
+  s->w = malloc(sizeof(double) * s->cap);
   s->tmp = malloc(sizeof(double) * 2 * n1 * s->nthreads);
   s->itime = 0;
   return s;
@@ -372,7 +388,8 @@ void orc_destroy(orc_sim *s) {
   free(s->rholoc); free(s->P); free(s->mass); free(s->hsml); free(s->R); free(s->Rt);
   free(s->a); free(s->at); free(s->deck_x); free(s->deck_v); free(s->deck_rho); free(s->deck_P);
   free(s->deck_lvel); free(s->itype); free(s->ip); free(s->jp); free(s->rij2); free(s->dwdx);
-  free(s->tmp); free(s);
+  free(s->tmp); free(s->Pmin); free(s->Ploc); free(s->RP); free(s->RPt); free(s->drho);
+  free(s->dP); free(s->dv); free(s->w); free(s->sponge); free(s);
 }
 
 /* Boundary reset and wall kinematics (inputbp.c:117-150, 194-207, 285-290). */
@@ -386,6 +403,7 @@ static void boundary_reset(orc_sim *s, double t) {
     s->P[i] = s->deck_P[i];
     s->hsml[i] = p->h;
     s->rholoc[i] = p->rho0;
+    s->Ploc[i] = 0.0;
   }
   if (p->motion == 1 || p->motion == 2) {
     double X, V;
@@ -442,6 +460,178 @@ static inline double av_pi(const orc_sim *s, int i, int j, const double *dx, dou
 #define SCATTER_TID (void)0
 #endif
 
+/* ------------------------------------------------------------ ref mode */
+
+/* Bulk modulus of the pressure transport, rho0 c^2 + gamma P (derivatives.c:1290-1291, 1607-1608). */
+static inline double ref_pp(const orc_sim *s, int i) {
+  return s->p.rho0 * s->p.c0 * s->p.c0 + s->p.gamma * s->P[i];
+}
+
+/* PredictingDynamicPressureEquation (derivatives.c:1254-1331) -> RPt.  Note the
+ * reference's volumes: volumei = mass[j]/density[i], volumej = mass[j]/density[j]. */
+static void ref_predicting_pressure(orc_sim *s) {
+  const orc_params *p = &s->p;
+  SCATTER_BEGIN(1)
+    const int i = s->ip[k], j = s->jp[k];
+    const double *dw = &s->dwdx[2 * k];
+    double dx[2], dv[2];
+    for (int d = 0; d < 2; d++) {
+      dv[d] = s->v[2 * i + d] - s->v[2 * j + d];
+      dx[d] = s->x[2 * i + d] - s->x[2 * j + d];
+    }
+    double vdw = 0.0, rdwij = 0.0;
+    for (int d = 0; d < 2; d++) {
+      vdw += dv[d] * dw[d];
+      rdwij += dx[d] * dw[d];
+    }
+    const double pi = ref_pp(s, i), pj = ref_pp(s, j);
+    const double volumei = s->mass[j] / s->rho[i];
+    const double volumej = s->mass[j] / s->rho[j];
+    const double vj = s->mass[j] / s->rho[i];
+    const double vi = s->mass[i] / s->rho[j];
+    tb[i] += vj * pi * vdw;
+    tb[j] += vi * pj * vdw;
+    const double h = 0.5 * (s->hsml[i] + s->hsml[j]);
+    const double eta2 = 0.01 * h * h;
+    const double nu = rdwij / (s->rij2[k] + eta2);
+    const double epj = 4.0 * pi / (pi + pj);
+    const double epi = 4.0 * pj / (pi + pj);
+    tb[i] += p->gamma * p->kappa * volumej * (s->P[i] - s->P[j]) * nu * epj;
+    tb[j] += p->gamma * p->kappa * volumei * (s->P[j] - s->P[i]) * nu * epi;
+    const double rho = (s->rho[i] + s->rho[j]) / (s->rho[i] * s->rho[j]);
+    const double rhoi = (s->rho[i] - s->rho[j]) * rho;
+    const double rhoj = (s->rho[j] - s->rho[i]) * rho;
+    tb[i] -= p->kappa * volumej * pi * rhoi * nu;
+    tb[j] -= p->kappa * pj * volumei * rhoj * nu;
+  SCATTER_END(s->RPt)
+}
+
+/* LocalDensityFluctuation (derivatives.c:2100-2135) and LocalPressureFluctuation
+ * (:2137-2171): f_i -= V_j (f_i - f_j) w, f_j += V_i (f_i - f_j) w. */
+static void ref_scalar_fluctuation(orc_sim *s, const double *f, double *dst) {
+  for (int i = 0; i < s->n; i++) dst[i] = 0.0;
+  SCATTER_BEGIN(1)
+    const int i = s->ip[k], j = s->jp[k];
+    const double volumei = s->mass[i] / s->rho[i];
+    const double volumej = s->mass[j] / s->rho[j];
+    const double avgpart = (f[i] - f[j]) * s->w[k];
+    tb[i] -= volumej * avgpart;
+    tb[j] += volumei * avgpart;
+  SCATTER_END(dst)
+}
+
+/* LocalVelocityFluctuation (derivatives.c:2060-2098).  Both of the reference's
+ * updates go to particle i (the first of the pair); j receives nothing. */
+static void ref_velocity_fluctuation(orc_sim *s) {
+  for (int i = 0; i < 2 * s->n; i++) s->dv[i] = 0.0;
+  SCATTER_BEGIN(2)
+    const int i = s->ip[k], j = s->jp[k];
+    for (int d = 0; d < 2; d++) {
+      const double avgpart = (s->v[2 * i + d] - s->v[2 * j + d]) * s->w[k];
+      tb[2 * i + d] -= s->mass[j] * avgpart / s->rholoc[i];
+      tb[2 * i + d] += s->mass[i] * avgpart / s->rholoc[j];
+    }
+  SCATTER_END(s->dv)
+}
+
+/* DynamicPressureEquation (derivatives.c:1569-1672) -> RP. */
+static void ref_pressure(orc_sim *s) {
+  const orc_params *p = &s->p;
+  SCATTER_BEGIN(1)
+    const int i = s->ip[k], j = s->jp[k];
+    const double *dw = &s->dwdx[2 * k];
+    double dx[2], dvij[2], dvji[2];
+    for (int d = 0; d < 2; d++) {
+      dvij[d] = s->v[2 * i + d] - s->vloc[2 * j + d];
+      dvji[d] = s->v[2 * j + d] - s->vloc[2 * i + d];
+      dx[d] = s->x[2 * i + d] - s->x[2 * j + d];
+    }
+    double vdwij = 0.0, vdwji = 0.0, rdwij = 0.0;
+    for (int d = 0; d < 2; d++) {
+      vdwij += dvij[d] * dw[d];
+      vdwji += dvji[d] * dw[d];
+      rdwij += dx[d] * dw[d];
+    }
+    const double pi = ref_pp(s, i), pj = ref_pp(s, j);
+    const double volumei = s->mass[i] / s->rho[i];
+    const double volumej = s->mass[j] / s->rho[j];
+    const double reducedVolumei = volumei * s->rholoc[i] / s->rho[j];
+    const double reducedVolumej = volumej * s->rholoc[j] / s->rho[i];
+    tb[i] += reducedVolumej * pi * vdwij;
+    tb[j] -= reducedVolumei * pj * vdwji;
+    const double h = 0.5 * (s->hsml[i] + s->hsml[j]);
+    const double eta2 = 0.01 * h * h;
+    const double nu = rdwij / (s->rij2[k] + eta2);
+    const double epj = 4.0 * pi / (pi + pj);
+    const double epi = 4.0 * pj / (pi + pj);
+    tb[i] += p->gamma * p->kappa * volumej * (s->P[i] - s->Ploc[j]) * nu * epj;
+    tb[j] += p->gamma * p->kappa * volumei * (s->P[j] - s->Ploc[i]) * nu * epi;
+    const double rho = (s->rho[i] + s->rho[j]) / (s->rho[i] * s->rho[j]);
+    const double rhoi = (s->rho[i] - s->rholoc[j]) * rho;
+    const double rhoj = (s->rho[j] - s->rholoc[i]) * rho;
+    tb[i] -= p->kappa * volumej * pi * rhoi * nu;
+    tb[j] -= p->kappa * pj * volumei * rhoj * nu;
+  SCATTER_END(s->RP)
+}
+
+/* ViscousForcesSingleStep (derivatives.c:1770-1839) -> a, and
+ * PredictingViscousForcesSingleStep (:1448-1515) -> at.  dx = x_j - x_i,
+ * dv = v_i - v_j; viscosity[] = viscosity0 (input.c:108, inputbp.c:288). */
+static void ref_viscous(orc_sim *s, int predicting) {
+  const orc_params *p = &s->p;
+  SCATTER_BEGIN(2)
+    const int i = s->ip[k], j = s->jp[k];
+    const double *dw = &s->dwdx[2 * k];
+    double dx[2], dvij[2], dvji[2];
+    for (int d = 0; d < 2; d++) {
+      dvij[d] = s->v[2 * i + d] - s->v[2 * j + d];
+      dvji[d] = predicting ? dvij[d] : s->v[2 * j + d] - s->v[2 * i + d];
+      dx[d] = s->x[2 * j + d] - s->x[2 * i + d];
+    }
+    const double h = 0.5 * (s->hsml[i] + s->hsml[j]);
+    const double eta2 = 0.01 * h * h;
+    const double invR2 = 1.0 / (s->rij2[k] + eta2);
+    double rhoi, rhoj;
+    if (predicting) {
+      rhoi = 1.0 / (s->rho[i] * s->rho[i]);
+      rhoj = 1.0 / (s->rho[j] * s->rho[j]);
+    } else {
+      rhoi = s->rholoc[j] / (s->rho[i] * s->rho[i] * s->rho[j]);
+      rhoj = s->rholoc[i] / (s->rho[j] * s->rho[j] * s->rho[i]);
+    }
+    const double ei = s->rho[i] * p->mu, ej = s->rho[j] * p->mu;
+    const double eff = 2.0 * ei * ej / (ei + ej);
+    const double iS00 = eff * (dx[0] * dvij[0] - dx[1] * dvij[1]) * invR2;
+    const double iS01 = eff * (dx[0] * dvij[1] + dx[1] * dvij[0]) * invR2;
+    const double iS10 = eff * (dx[1] * dvij[0] + dx[0] * dvij[1]) * invR2;
+    const double iS11 = eff * (dx[1] * dvij[1] - dx[0] * dvij[0]) * invR2;
+    const double jS00 = eff * (dx[0] * dvji[0] - dx[1] * dvji[1]) * invR2;
+    const double jS01 = eff * (dx[0] * dvji[1] + dx[1] * dvji[0]) * invR2;
+    const double jS10 = eff * (dx[1] * dvji[0] + dx[0] * dvji[1]) * invR2;
+    const double jS11 = eff * (dx[1] * dvji[1] - dx[0] * dvji[0]) * invR2;
+    double ax = (iS00 * rhoi + jS00 * rhoj) * dw[0];
+    ax += (iS01 * rhoi + jS01 * rhoj) * dw[1];
+    double ay = (iS10 * rhoi + jS10 * rhoj) * dw[0];
+    ay += (iS11 * rhoi + jS11 * rhoj) * dw[1];
+    tb[2 * i] += s->mass[j] * ax;
+    tb[2 * i + 1] += s->mass[j] * ay;
+    tb[2 * j] -= s->mass[i] * ax;
+    tb[2 * j + 1] -= s->mass[i] * ay;
+  SCATTER_END(predicting ? s->at : s->a)
+}
+
+/* Ref-mode sums that do not touch dvdt: the pressure rates and the fluctuation
+ * fields (derivatives.c:4204-4237). */
+static void ref_pressure_and_fluctuations(orc_sim *s) {
+  memset(s->RP, 0, sizeof(double) * s->n);
+  memset(s->RPt, 0, sizeof(double) * s->n);
+  ref_predicting_pressure(s);
+  ref_scalar_fluctuation(s, s->rho, s->drho);
+  ref_scalar_fluctuation(s, s->P, s->dP);
+  ref_velocity_fluctuation(s);
+  ref_pressure(s);
+}
+
 /*
  * One call of derivatives(sys, t) in config mode (derivatives.c:3137-4315 with
  * the scheme's substitutions): rates R, Rt, a, at are recomputed from scratch.
@@ -472,13 +662,17 @@ int orc_derivatives(orc_sim *s, double t) {
       orc_poly7(dx, r, h, &w, &s->dwdx[2 * k]);
     else
       orc_cubic(dx, r, h, &w, &s->dwdx[2 * k]);
+    s->w[k] = w;
   }
 
-  /* EquationOfState (derivatives.c:1674-1682); config mode feeds it to the momentum terms */
-  for (int i = 0; i < n; i++) {
-    const double density = s->rho[i] / p->rho0;
-    s->P[i] = p->B * (pow(density, p->gamma) - 1.0);
-  }
+  /* EquationOfState (derivatives.c:1674-1682); config mode feeds it to the momentum
+   * terms, ref mode integrates DynamicPressure instead (its Tait Pressure is output only) */
+  if (p->mode == 0)
+    for (int i = 0; i < n; i++) {
+      const double density = s->rho[i] / p->rho0;
+      s->P[i] = p->B * (pow(density, p->gamma) - 1.0);
+    }
+  if (p->mode == 1) ref_pressure_and_fluctuations(s);
 
   /* PredictingDensityEquation (derivatives.c:1217-1252) */
   SCATTER_BEGIN(1)
@@ -544,12 +738,22 @@ int orc_derivatives(orc_sim *s, double t) {
     }
   SCATTER_END(s->a)
 
+  /* ref mode, stressType 1 (derivatives.c:4243-4247) */
+  if (p->mode == 1) {
+    ref_viscous(s, 0);
+    ref_viscous(s, 1);
+  }
+
   /* BodyForce / PredictingBodyForce (derivatives.c:1167-1178, 1379-1390) */
   if (p->gravity == 1)
     for (int i = 0; i < s->nf; i++) {
       s->a[2 * i] -= 0.0; s->a[2 * i + 1] -= p->g;
       s->at[2 * i] -= 0.0; s->at[2 * i + 1] -= p->g;
     }
+
+  /* SpongeLayerWaveReflectionAbsorption (derivatives.c:2827-2838) */
+  if (p->mode == 1)
+    for (int i = 0; i < s->nf; i++) s->a[2 * i] -= s->sponge[i] * s->v[2 * i];
 
   /* SpringBoundaryForces + PredictingSpringBoundaryForces (derivatives.c:2396-2449,
    * 1392-1446): identical formulas accumulated into a and at. */
@@ -580,10 +784,14 @@ int orc_derivatives(orc_sim *s, double t) {
   return 0;
 }
 
-/* Leapfrog time step (sph.cc:90-168), fluctuation fields zero. */
+/* Leapfrog time step (sph.cc:90-168).  Config mode: fluctuation fields zero and
+ * DynamicPressure from the EOS.  Ref mode: DynamicPressure integrated with dpdt,
+ * local fields carry the fluctuations (epsilon weighs the velocity's). */
 int orc_step(orc_sim *s) {
   const double dt = s->p.dt;
   const int nf = s->nf, n = s->n;
+  const int ref = s->p.mode == 1;
+  const double eps = ref ? s->p.eps : 0.0;
   int itime = ++s->itime;
   double t = itime * dt;
   if (itime != 1) {
@@ -591,37 +799,60 @@ int orc_step(orc_sim *s) {
     memcpy(s->rholoc, s->rho, sizeof(double) * nf);
     memcpy(s->vloc, s->v, sizeof(double) * 2 * n);
     memcpy(s->vmin, s->v, sizeof(double) * 2 * n);
+    if (ref) {
+      memcpy(s->Pmin, s->P, sizeof(double) * nf);
+      memcpy(s->Ploc, s->P, sizeof(double) * nf);
+    }
     for (int i = 0; i < nf; i++) {
       s->rho[i] += 0.5 * dt * s->Rt[i];
-      s->rholoc[i] = s->rholoc[i] + 0.5 * dt * s->Rt[i];
+      if (ref) {
+        s->rholoc[i] = s->rholoc[i] + 0.5 * dt * s->Rt[i] + s->drho[i];
+        s->P[i] += 0.5 * dt * s->RPt[i];
+        s->Ploc[i] = s->Ploc[i] + 0.5 * dt * s->RPt[i] + s->dP[i];
+      } else {
+        s->rholoc[i] = s->rholoc[i] + 0.5 * dt * s->Rt[i];
+      }
     }
     for (int i = 0; i < nf; i++)
       for (int d = 0; d < 2; d++) {
         s->vmin[2 * i + d] = s->v[2 * i + d];
         s->v[2 * i + d] += 0.5 * dt * s->at[2 * i + d];
         s->vloc[2 * i + d] = s->v[2 * i + d] + 0.5 * dt * s->at[2 * i + d];
+        if (ref) s->vloc[2 * i + d] += eps * s->dv[2 * i + d];
       }
   }
   if (orc_derivatives(s, t) != 0) return -1;
   if (itime == 1) {
     for (int i = 0; i < nf; i++) {
       s->rholoc[i] = s->rho[i] + 0.5 * dt * s->R[i];
+      if (ref) s->rholoc[i] += s->drho[i];
       s->rho[i] += 0.5 * dt * s->R[i];
+      if (ref) {
+        s->Ploc[i] = s->P[i] + 0.5 * dt * s->RP[i] + s->dP[i];
+        s->P[i] += 0.5 * dt * s->RP[i];
+      }
     }
     for (int i = 0; i < nf; i++)
       for (int d = 0; d < 2; d++) {
         s->vloc[2 * i + d] = s->v[2 * i + d] + 0.5 * dt * s->a[2 * i + d];
+        if (ref) s->vloc[2 * i + d] += eps * s->dv[2 * i + d];
         s->v[2 * i + d] += 0.5 * dt * s->a[2 * i + d];
         s->x[2 * i + d] += dt * s->v[2 * i + d];
       }
   } else {
     for (int i = 0; i < nf; i++) {
       s->rholoc[i] = s->rhomin[i] + dt * s->R[i];
+      if (ref) s->rholoc[i] += s->drho[i];
       s->rho[i] = s->rhomin[i] + dt * s->R[i];
+      if (ref) {
+        s->Ploc[i] = s->Pmin[i] + dt * s->RP[i] + s->dP[i];
+        s->P[i] = s->Pmin[i] + dt * s->RP[i];
+      }
     }
     for (int i = 0; i < nf; i++)
       for (int d = 0; d < 2; d++) {
         s->vloc[2 * i + d] = s->vmin[2 * i + d] + dt * s->a[2 * i + d];
+        if (ref) s->vloc[2 * i + d] += eps * s->dv[2 * i + d];
         s->v[2 * i + d] = s->vmin[2 * i + d] + dt * s->a[2 * i + d];
         s->x[2 * i + d] += dt * s->v[2 * i + d];
       }
@@ -651,6 +882,27 @@ void orc_get_rates(const orc_sim *s, double *R, double *Rt, double *a, double *a
   if (a) memcpy(a, s->a, sizeof(double) * 2 * s->n);
   if (at) memcpy(at, s->at, sizeof(double) * 2 * s->n);
   if (P) memcpy(P, s->P, sizeof(double) * s->n);
+}
+
+/* Ref-mode fields: DynamicPressure, localPressure, dpdt, dpdt_approx, density and
+ * pressure fluctuations (n each), velocity fluctuation (2n). */
+void orc_get_ref(const orc_sim *s, double *P, double *Ploc, double *RP, double *RPt, double *drho,
+                 double *dP, double *dv) {
+  if (P) memcpy(P, s->P, sizeof(double) * s->n);
+  if (Ploc) memcpy(Ploc, s->Ploc, sizeof(double) * s->n);
+  if (RP) memcpy(RP, s->RP, sizeof(double) * s->n);
+  if (RPt) memcpy(RPt, s->RPt, sizeof(double) * s->n);
+  if (drho) memcpy(drho, s->drho, sizeof(double) * s->n);
+  if (dP) memcpy(dP, s->dP, sizeof(double) * s->n);
+  if (dv) memcpy(dv, s->dv, sizeof(double) * 2 * s->n);
+}
+
+/* Sets localDensity, localPressure and localVelocity (n, n, 2n; NULL keeps a field),
+ * for tests that evaluate derivatives() at a chosen state. */
+void orc_set_local(orc_sim *s, const double *rholoc, const double *Ploc, const double *vloc) {
+  if (rholoc) memcpy(s->rholoc, rholoc, sizeof(double) * s->n);
+  if (Ploc) memcpy(s->Ploc, Ploc, sizeof(double) * s->n);
+  if (vloc) memcpy(s->vloc, vloc, sizeof(double) * 2 * s->n);
 }
 
 int orc_niac(const orc_sim *s) { return s->niac; }

@@ -37,6 +37,10 @@ MOTION_SINE_OFFSET = 2    # x = x_deck + A sin wt, v = A w cos wt
 KERNEL_CUBIC = 0          # cubic spline, Kernel (derivatives.c:44-66)
 KERNEL_POLY7 = 1          # PolyOrderSeven pair of CalculateKernel (derivatives.c:767-823, 1099-1101)
 
+# scheme
+MODE_CONFIG = 0           # Tait pressure in the momentum terms, artificial viscosity, no fluctuations
+MODE_REF = 1              # the reference's run phase: pressure transport, viscous forces, fluctuations
+
 
 @dataclasses.dataclass
 class Scene:
@@ -69,6 +73,10 @@ class Scene:
     m_omega: float = 0.0
     box: tuple = (0.0, 0.0, 1.0, 1.0)     # (xmin, ymin, xmax, ymax) of the cell grid
     kernel: int = KERNEL_CUBIC
+    mode: int = MODE_CONFIG
+    kappa: float = 0.0       # thermalDiffusivity0 (ref mode)
+    mu: float = 0.0          # viscosity0 (ref mode)
+    eps: float = 0.0         # epsilon, weight of the velocity fluctuation (ref mode)
     name: str = "scene"
 
     @property
@@ -361,6 +369,7 @@ def scene_from_rows(fluid: np.ndarray, fixed: np.ndarray, moving: np.ndarray,
               dt=prm["timeStep"], motion=(MOTION_PISTON_REF if int(prm["movingBoundary"]) == 1
                                           else MOTION_NONE),
               m_amp=prm["stroke"], m_omega=2.0 * math.pi / prm["wavePeriod"],
+              kappa=prm["thermalDiffusivity0"], mu=prm["viscosity0"], eps=prm["epsilon"],
               box=(lo[0], lo[1], hi[0], hi[1]), name="deck")
     kw.update(over)
     return Scene(pos=pos, vel=rows[:, 2:4].copy(), rho=rows[:, 4].copy(),
