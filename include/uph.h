@@ -80,6 +80,16 @@ enum {
                                  (2048 - 3584 q^2 + ... + 305 q^12) dx / h^2, a0 = 2145/(192512 pi h^2) */
 };
 
+/* Scheme of the pair sums and the leapfrog. */
+enum {
+  UPH_MODE_CONFIG = 0,        /* Tait pressure in the momentum terms, artificial viscosity alpha_av */
+  UPH_MODE_REF = 1            /* the reference's run phase (derivatives.c:4196-4315, flagInitSimul = 1,
+                                 stressType = 1): DynamicPressure integrated by the pressure transport
+                                 equation (:1569-1672, :1254-1331), viscous forces (:1770-1839,
+                                 :1448-1515), density / pressure / velocity fluctuations
+                                 (:2060-2171) in the local fields of the leapfrog (sph.cc:90-168) */
+};
+
 /* One independent scene.  Arrays hold n = n_fluid + n_fixed + n_moving rows. */
 typedef struct uph_scene {
   int n_fluid, n_fixed, n_moving;
@@ -102,6 +112,9 @@ typedef struct uph_scene {
   double m_amp, m_omega;
   double box[4];        /* xmin, ymin, xmax, ymax covered by the cell grid  */
   int kernel;           /* UPH_KERNEL_*; the same for every scene of a batch */
+  const double *pres;   /* n  : DynamicPressure deck column (UPH_MODE_REF; NULL: zero) */
+  int mode;             /* UPH_MODE_*; the same for every scene of a batch */
+  double kappa, mu, eps;      /* thermalDiffusivity0, viscosity0, epsilon (UPH_MODE_REF) */
 } uph_scene;
 
 typedef struct uph_opts {
@@ -130,6 +143,16 @@ int uph_get_state(uph_t *h, int scene, double *pos, double *vel, double *rho);
 /* Rates of the last step (needs opts.keep_rates): R = ddensitydt, Rt = ddensitydt_approx,
  * a = dvdt, at = dvdt_approx; fluid rows only (n_fluid, n_fluid, 2 n_fluid, 2 n_fluid). */
 int uph_get_rates(uph_t *h, int scene, float *R, float *Rt, float *a, float *at);
+
+/* UPH_MODE_REF: DynamicPressure of one scene in the reference's order (n values;
+ * walls hold their deck value).  UPH_MODE_CONFIG: the Tait pressure of the
+ * density. */
+int uph_get_pressure(uph_t *h, int scene, double *P);
+
+/* UPH_MODE_REF rates of the last step (needs opts.keep_rates), fluid rows:
+ * dpdt, dpdt_approx, densityFluctuation, pressureFluctuation (n_fluid each),
+ * velocityFluctuation (2 n_fluid).  Any pointer may be NULL. */
+int uph_get_ref_rates(uph_t *h, int scene, float *RP, float *RPt, float *drho, float *dP, float *dv);
 
 /* Completed steps. */
 int uph_itime(const uph_t *h);

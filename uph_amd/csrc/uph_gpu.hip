@@ -30,6 +30,7 @@
 #include <mutex>
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 #define BLK 256
 #define SCAN_ITEMS 16
@@ -225,15 +226,27 @@ struct StepBufs {
   // per original index (global id)
   double2 *deckPos, *deckVel, *deckLvel;
   double* massD;   // FP64 mass per original index (diagnostics)
+  // UPH_MODE_REF: DynamicPressure (A: at the start of the step, B: after it), its
+  // deck value per original index, the predicting pressure rate and the density and
+  // pressure fluctuations of the last step {dpdt_approx, drho, dP, 0}, the velocity
+  // fluctuation, and the neighbour mirror {localDensity, localPressure}
+  double *presA, *presB, *deckP;
+  float4* rt2B;
+  float2 *dvB, *mC;
   // control: [0] itime, [1] overflow flag of the step, [2] overflowed steps, [3] step stamp
   int* ctl;
-  float4* dbgRates;   // keep_rates: per slot {R, R~, a~x, a~y} (FP32), {ax, ay} (FP64)
+  float4* dbgRates;   // keep_rates: per slot {R, R~, a~x, a~y} (FP32), {ax, ay} (FP64);
+                      // UPH_MODE_REF adds {dpdt, dpdt_approx, drho, dP}, {dvx, dvy, 0, 0}
 };
 
 // Final slot = cell start + number of same-cell particles with a smaller
 // original index (deterministic order); gather the state and build the
 // predictor mirror (sph.cc:93-126: rho += dt/2 Rt, v += dt/2 at,
-// vloc = v + dt/2 at; Tait pressure of the predicted density).
+// vloc = v + dt/2 at; Tait pressure of the predicted density).  UPH_MODE_REF
+// also predicts DynamicPressure (P += dt/2 dpdt_approx) and adds the last step's
+// fluctuations to the local fields: localDensity = rho + drho, localPressure =
+// P + dP, localVelocity = v + dt/2 at + eps dv; at the first step they are
+// rho0, 0 and the deck's localVelocity (input.c:109-110).
 __global__ __launch_bounds__(BLK) void k_reorder(int n, StepBufs b, const DevScene* __restrict__ sc, int search_only) {
   const int d = blockIdx.x * BLK + threadIdx.x;
   if (d >= n) return;
@@ -279,6 +292,29 @@ __global__ __launch_bounds__(BLK) void k_reorder(int n, StepBufs b, const DevSce
       wall_motion(S, S.t0 + itime * dt, b.deckPos[my].x, xx, vv);
       lx = vv;
     }
+  }
+  if (S.mode == 1) {
+    // walls: DynamicPressure from the deck, localDensity rho0, localPressure 0
+    // (inputbp.c:117-150, 289-290)
+    double P = tag_type(tg) == 1 ? b.presB[i] : b.deckP[my];
+    double Pp = P, rl = S.rho0, pl = 0.0;
+    if (tag_type(tg) == 1 && itime != 1) {
+      const float4 r2 = b.rt2B[i];
+      const float2 dv = b.dvB[i];
+      Pp = P + 0.5 * dt * (double)r2.x;
+      rl = rp + (double)r2.y;
+      pl = Pp + (double)r2.z;
+      lx += S.eps * (double)dv.x;
+      ly += S.eps * (double)dv.y;
+    }
+    // densities enter the mirror as deviations from rho0 (formed in FP64), so the
+    // differences the fluctuations and diffusion terms take keep FP32's relative
+    // precision instead of losing the leading digits of rho0
+    b.presA[dst] = P;
+    b.mA[dst] = make_float4((float)vx, (float)vy, (float)lx, (float)ly);
+    b.mB[dst] = make_float4((float)(rp - S.rho0), (float)Pp, rt.w, __uint_as_float(tg));
+    b.mC[dst] = make_float2((float)(rl - S.rho0), (float)pl);
+    return;
   }
   const double P = tait(S, rp);
   const double pr = P / (rp * rp);
@@ -630,24 +666,19 @@ __device__ __forceinline__ void kth_of(const StepBufs& b, int q, double& kd2, ui
   }
 }
 
-// State of one fluid particle p during its pair sums.
-struct PairCtx {
+// Kernel factors and pair admission of one fluid particle p, shared by both schemes.
+struct PairBase {
   uint32_t idp;
   bool pb_ok;
-  float vpx, vpy, lpx, lpy, rhop, prp, mp, inv_mp;
+  int pb;               // ParticleBoundary as a global index
   float invh, h2eps, cav;
   float fa, fb, fw;     // cubic-spline gradient factor: fa + fb q (q < 1), fw (2 - q)^2 / r (1 <= q < 2)
   float k7, invh2;      // PolyOrderSeven gradient factor: k7 (1 - u/4)^7 P(u), u = r^2 / h^2
-  int pb;               // ParticleBoundary as a global index
-  CSum sR, sRt, sAx, sAy;   // sAx/sAy: the predicting acceleration (dvdt_approx)
-  float dAx = 0.f, dAy = 0.f;   // dvdt - dvdt_approx: wall pairs with localDensity != density
+  float wa, w7;         // W factors: cubic alpha = 10/(7 pi h^2), PolyOrderSeven 2145/(1347584 pi h^2)
 
-  __device__ __forceinline__ void init(const DevScene& S, uint32_t tg, float4 ap, float4 bp) {
+  __device__ __forceinline__ void init_base(const DevScene& S, uint32_t tg) {
     idp = tag_id(tg);
     pb_ok = (int)idp < S.pb_gid;
-    vpx = ap.x; vpy = ap.y; lpx = ap.z; lpy = ap.w;
-    rhop = bp.x; prp = bp.y; mp = bp.z;
-    inv_mp = 1.0f / mp;
     const float h = S.fh, walpha = S.fwalpha;
     invh = S.finvh;
     h2eps = 0.01f * h * h;
@@ -657,6 +688,8 @@ struct PairCtx {
     fw = -0.75f * walpha * invh;
     k7 = S.fk7;
     invh2 = invh * invh;
+    wa = walpha;
+    w7 = S.fw7;
     pb = S.pb_gid;
   }
 
@@ -691,6 +724,33 @@ struct PairCtx {
     return qq < 1.0f ? f_near : f_far;
   }
 
+  // W at |x_p - x_q|^2 = r2, |.| = r for the kernel KT
+  template <int KT>
+  __device__ __forceinline__ float w_value(float r2, float r) const {
+#pragma clang fp contract(fast)
+    if (KT == UPH_KERNEL_POLY7) {
+      // PolyOrderSevenConvolutionKernel (derivatives.c:767-784): a (1 - u/4)^8
+      // (2048 - 3072 u + 2816 u^2 - 1984 u^3 + 1184 u^4 - 628 u^5 + 305 u^6)
+      const float u = r2 * invh2;
+      const float s = 1.0f - 0.25f * u;
+      const float s2 = s * s, s4 = s2 * s2;
+      float pu = 305.0f;
+      pu = pu * u - 628.0f;
+      pu = pu * u + 1184.0f;
+      pu = pu * u - 1984.0f;
+      pu = pu * u + 2816.0f;
+      pu = pu * u - 3072.0f;
+      pu = pu * u + 2048.0f;
+      return w7 * (s4 * s4) * pu;
+    }
+    // cubic spline W (derivatives.c:44-66)
+    const float qq = r * invh;
+    const float t = 2.0f - qq;
+    const float w_far = 0.25f * wa * t * t * t;
+    const float w_near = wa * (1.0f + qq * qq * (-1.5f + 0.75f * qq));
+    return qq < 1.0f ? w_near : w_far;
+  }
+
   // Admission of an in-range pair by the rules of Search.cc:33 (original index
   // order, ParticleBoundary bound) and, when EXACT, the K-nearest cap.
   template <bool EXACT>
@@ -712,10 +772,26 @@ struct PairCtx {
     return true;
   }
 
+};
+
+// Config-mode sums of one fluid particle p.
+struct PairCtx : PairBase {
+  float vpx, vpy, lpx, lpy, rhop, prp, mp, inv_mp;
+  CSum sR, sRt, sAx, sAy;   // sAx/sAy: the predicting acceleration (dvdt_approx)
+  float dAx = 0.f, dAy = 0.f;   // dvdt - dvdt_approx: wall pairs with localDensity != density
+
+  __device__ __forceinline__ void init(const DevScene& S, uint32_t tg, float4 ap, float4 bp, float2) {
+    init_base(S, tg);
+    vpx = ap.x; vpy = ap.y; lpx = ap.z; lpy = ap.w;
+    rhop = bp.x; prp = bp.y; mp = bp.z;
+    inv_mp = 1.0f / mp;
+  }
+
   // One admitted pair: x_p - x_q = (ex, ey), |.|^2 = d2 (FP64 geometry, FP32 sums).
   // FP32 arithmetic may contract to FMA here; 1/r and 1/x use v_rsq_f32 / v_rcp_f32.
   template <int KT>
-  __device__ __forceinline__ void add(const DevScene& S, double ex, double ey, uint32_t tq, float4 aq, float4 bq) {
+  __device__ __forceinline__ void add(const DevScene& S, double ex, double ey, uint32_t tq, float4 aq, float4 bq,
+                                      float2) {
 #pragma clang fp contract(fast)
     const bool p_is_i = tag_id(tq) > idp;   // pair (i=p, j=q) or (i=q, j=p)
     const float dx = (float)ex, dy = (float)ey;
@@ -772,6 +848,150 @@ struct PairCtx {
   }
 };
 
+// Ref-mode sums of one fluid particle p: the reference's run-phase rates in gather
+// form (every neighbour q of p; terms whose reference form depends on which of p, q
+// is the pair's first index branch on the original ids, as a5 does).
+struct RefCtx : PairBase {
+  float vpx, vpy, lpx, lpy;   // predicted velocity, localVelocity
+  float rhop, Pp, mp, rlp;    // predicted density, DynamicPressure, mass, localDensity
+  float drp, drlp, r0;        // density and localDensity minus rho0; rho0
+  float inv_mp, irp, irp2, Kp, gk, kap, mu;
+  CSum sR, sRt;               // ddensitydt, ddensitydt_approx
+  CSum sL, sC2, sLt;          // dpdt diffusion part; sum m_q (v_p - v_q).grad W; dpdt_approx diffusion part
+  CSum sAx, sAy, sTx, sTy;    // dvdt, dvdt_approx
+  float dr = 0.f, dP = 0.f, dvx = 0.f, dvy = 0.f;   // fluctuations
+
+  __device__ __forceinline__ void init(const DevScene& S, uint32_t tg, float4 ap, float4 bp, float2 cp) {
+    init_base(S, tg);
+    vpx = ap.x; vpy = ap.y; lpx = ap.z; lpy = ap.w;
+    r0 = S.frho0;
+    drp = bp.x; Pp = bp.y; mp = bp.z;
+    drlp = cp.x;
+    rhop = r0 + drp;
+    rlp = r0 + drlp;
+    inv_mp = 1.0f / mp;
+    irp = 1.0f / rhop;
+    irp2 = irp * irp;
+    Kp = S.fK0 + S.fgamma * Pp;
+    gk = S.fgk;
+    kap = S.fkap;
+    mu = S.fmu;
+  }
+
+  template <int KT>
+  __device__ __forceinline__ void add(const DevScene& S, double ex, double ey, uint32_t tq, float4 aq, float4 bq,
+                                      float2 cq) {
+    // no contraction: the compacted and the direct kernel inline this body into
+    // different code, and only unfused arithmetic rounds the same in both
+#pragma clang fp contract(off)
+    const bool p_is_i = tag_id(tq) > idp;   // pair (i=p, j=q) or (i=q, j=p)
+    const float dx = (float)ex, dy = (float)ey;
+    const float r2 = dx * dx + dy * dy;
+    const float rinv = __builtin_amdgcn_rsqf(r2);
+    const float r = r2 * rinv;
+    const float f = grad_factor<KT>(r2, r, rinv);
+    const float gx = f * dx, gy = f * dy;   // grad_p W_pq
+    const float w = w_value<KT>(r2, r);
+    const float drq = bq.x, Pq = bq.y, mq = bq.z;
+    const float drlq = cq.x, Plq = cq.y;
+    const float rhoq = r0 + drq, rlq = r0 + drlq;
+    const float irq = __builtin_amdgcn_rcpf(rhoq);
+    const float Vq = mq * irq;
+    const uint32_t tyq = tag_type(tq);
+    // DensityEquation (derivatives.c:1520-1567)
+    const float cont = (vpx - aq.z) * gx + (vpy - aq.w) * gy;
+    sR.add(mq * rlq * irq * cont);
+    // PredictingDensityEquation (derivatives.c:1217-1252), depends on the pair order
+    const float dvx_ = vpx - aq.x, dvy_ = vpy - aq.y;
+    const float std_ = dvx_ * gx + dvy_ * gy;          // (v_p - v_q).grad W
+    const float alt = (lpx - aq.x) * gx + (lpy - aq.y) * gy;
+    float rt_q = mq * cont, rt_p = mp * alt;
+    asm volatile("" : "+v"(rt_q), "+v"(rt_p));
+    sRt.add(p_is_i ? rt_q : rt_p);
+    // DynamicPressureEquation (:1569-1672): its compression term is K_p/rho_p times
+    // the continuity sum (added at the end); the diffusion terms of P and rho
+    const float ir2 = __builtin_amdgcn_rcpf(r2 + h2eps);
+    const float nu = (dx * gx + dy * gy) * ir2;
+    const float Kq = S.fK0 + S.fgamma * Pq;
+    const float ep = 4.0f * Kp * __builtin_amdgcn_rcpf(Kp + Kq);
+    const float rb = (rhop + rhoq) * irp * irq;
+    sL.add(Vq * (gk * (Pp - Plq) * ep - kap * Kp * (drp - drlq) * rb) * nu);
+    // PredictingDynamicPressureEquation (:1254-1331): the reference's volume of the
+    // second particle of a pair is mass[j]/density[i]
+    sC2.add(mq * std_);
+    const float Vt = p_is_i ? Vq : mp * irq;
+    sLt.add(Vt * (gk * (Pp - Pq) * ep - kap * Kp * (drp - drq) * rb) * nu);
+    // Local{Density,Pressure}Fluctuation (:2100-2171); LocalVelocityFluctuation
+    // (:2060-2098) adds both of its terms to the pair's first particle only
+    dr += Vq * (drq - drp) * w;
+    dP += Vq * (Pq - Pp) * w;
+    if (p_is_i) {
+      // m_p/rholoc_q - m_q/rholoc_p over the common denominator, its numerator
+      // from the deviations
+      const float num = r0 * (mp - mq) + (mp * drlp - mq * drlq);
+      const float cv = num * __builtin_amdgcn_rcpf(rlp * rlq) * w;
+      dvx += cv * dvx_;
+      dvy += cv * dvy_;
+    }
+    // PressureGradientForce (:1109-1163) with DynamicPressure, its predicting form
+    // (:1333-1377), and Monaghan viscosity when alpha_av > 0
+    const float vr = dvx_ * dx + dvy_ * dy;
+    float pij = Pp * rlq * irp2 * irq + Pq * rlp * irq * irq * irp;
+    float ptij = Pp * irp2 + Pq * irq * irq;
+    if (vr < 0.0f) {
+      const float av = cav * vr * __builtin_amdgcn_rcpf((r2 + h2eps) * (rhop + rhoq));
+      pij += av;
+      ptij += av;
+    }
+    float ax = -mq * pij * gx, ay = -mq * pij * gy;
+    float tx = -mq * ptij * gx, ty = -mq * ptij * gy;
+    // ViscousForcesSingleStep (:1770-1839) and its predicting form (:1448-1515):
+    // X = x_q - x_p, D = v_p - v_q, sigma = mu_eff (X D deviatoric) / (r^2 + 0.01 h^2)
+    {
+      const float X0 = -dx, X1 = -dy;
+      const float eff = 2.0f * mu * rhop * rhoq * __builtin_amdgcn_rcpf(rhop + rhoq);
+      const float s00 = eff * ir2 * (X0 * dvx_ - X1 * dvy_);
+      const float s01 = eff * ir2 * (X0 * dvy_ + X1 * dvx_);
+      const float sgx = s00 * gx + s01 * gy, sgy = s01 * gx - s00 * gy;
+      // rholoc_q/(rho_p^2 rho_q) - rholoc_p/(rho_q^2 rho_p) = (rholoc_q rho_q -
+      // rholoc_p rho_p) / (rho_p rho_q)^2, the numerator from the deviations
+      const float num = r0 * ((drlq + drq) - (drlp + drp)) + (drlq * drq - drlp * drp);
+      const float dA = num * irp2 * irq * irq;
+      const float sg = p_is_i ? dA : -dA;
+      ax += mq * sg * sgx;
+      ay += mq * sg * sgy;
+      const float st = mq * (irp2 + irq * irq);
+      tx += st * sgx;
+      ty += st * sgy;
+    }
+    // wall spring (derivatives.c:2396-2449, 1392-1446), into both accelerations
+    if (p_is_i && (tyq == 2 || tyq == 3 || tyq == 5) &&
+        sqrt(__dadd_rn(__dmul_rn(ex, ex), __dmul_rn(ey, ey))) < S.r0) {
+      const float kr = S.fk * (S.fr0 - r) * rinv * inv_mp;
+      const float cr = S.fcd * vr * (rinv * rinv) * inv_mp;
+      const float sx = kr * dx + cr * dx, sy = kr * dy + cr * dy;
+      ax -= sx;
+      ay -= sy;
+      tx -= sx;
+      ty -= sy;
+    }
+    sAx.add(ax);
+    sAy.add(ay);
+    sTx.add(tx);
+    sTy.add(ty);
+  }
+};
+
+// The pair context of a scheme.
+template <int MODE>
+using CtxOf = typename std::conditional<MODE == UPH_MODE_REF, RefCtx, PairCtx>::type;
+
+// The {localDensity, localPressure} mirror of slot q, read by the ref-mode scheme only.
+template <int MODE>
+__device__ __forceinline__ float2 mirror_c(const StepBufs& b, uint32_t q) {
+  return MODE == UPH_MODE_REF ? ld_off(b.mC, q) : make_float2(0.f, 0.f);
+}
+
 // Boundary slot: wall state for the next step (deck + kinematics at t = (itime+1) dt).
 __device__ __forceinline__ void wall_update(const DevScene& S, const StepBufs& b, int p, uint32_t tg, int itime,
                                             float mass) {
@@ -782,6 +1002,7 @@ __device__ __forceinline__ void wall_update(const DevScene& S, const StepBufs& b
   b.velB[p] = v;
   b.rhoB[p] = b.rhoA[p];
   b.rtB[p] = make_float4(0.f, 0.f, 0.f, mass);
+  if (S.mode == 1) b.presB[p] = b.deckP[id];
 }
 
 // Gravity and the FP64 leapfrog corrector (sph.cc:137-168) for fluid slot p.
@@ -815,11 +1036,49 @@ __device__ __forceinline__ void correct(const DevScene& S, const StepBufs& b, in
   }
 }
 
+// UPH_MODE_REF: gravity, the sponge layer and the FP64 leapfrog corrector
+// (sph.cc:137-168) with DynamicPressure integrated like the density.
+__device__ __forceinline__ void correct(const DevScene& S, const StepBufs& b, int p, int itime, double2 xp,
+                                        double2 vn, float mass, const RefCtx& c) {
+  const double dt = S.dt;
+  const float R = c.sR.val(), Rt = c.sRt.val();
+  const float RP = c.Kp * c.irp * R + c.sL.val();      // compression + diffusion
+  const float RPt = c.Kp * c.irp * c.sC2.val() + c.sLt.val();
+  double ax = (double)c.sAx.val(), ay = (double)c.sAy.val();
+  double atx = (double)c.sTx.val(), aty = (double)c.sTy.val();
+  if (S.gravity == 1) {   // BodyForce / PredictingBodyForce (derivatives.c:1167-1178, 1379-1390)
+    ay -= S.g;
+    aty -= S.g;
+  }
+  // SpongeLayerWaveReflectionAbsorption (derivatives.c:2827-2838): spongeFactor0 = 0.2
+  // beyond spongePosition0 = 7 (System.c:333-335) by the particle's initial x
+  // (input.c:137-143), for the reference piston only
+  if (S.motion == 1 && b.deckPos[tag_id(b.tagA[p])].x >= 7.0) ax -= 0.2 * (double)c.vpx;
+  const double rho_n = b.rhoA[p], P_n = b.presA[p];
+  const double hdt = itime == 1 ? 0.5 * dt : dt;
+  const double rho1 = rho_n + hdt * (double)R;
+  const double P1 = P_n + hdt * (double)RP;
+  const double v1x = vn.x + hdt * ax, v1y = vn.y + hdt * ay;
+  b.posB[p] = make_double2(xp.x + dt * v1x, xp.y + dt * v1y);
+  b.velB[p] = make_double2(v1x, v1y);
+  b.rhoB[p] = rho1;
+  b.presB[p] = P1;
+  b.rtB[p] = make_float4(Rt, (float)atx, (float)aty, mass);
+  b.rt2B[p] = make_float4(RPt, c.dr, c.dP, 0.f);
+  b.dvB[p] = make_float2(c.dvx, c.dvy);
+  if (b.dbgRates) {
+    b.dbgRates[4 * p] = make_float4(R, Rt, (float)atx, (float)aty);
+    *reinterpret_cast<double2*>(b.dbgRates + 4 * p + 1) = make_double2(ax, ay);
+    b.dbgRates[4 * p + 2] = make_float4(RP, RPt, c.dr, c.dP);
+    b.dbgRates[4 * p + 3] = make_float4(c.dvx, c.dvy, 0.f, 0.f);
+  }
+}
+
 // Direct form: one pass over the candidates, physics inside the distance test.
 // EXACT applies the K-nearest cap (run only when some query overflowed).
 // Returns true for a query with more than K candidates in range when !EXACT
 // (nothing written; its K-th key is left to wave_overflow).
-template <bool EXACT, int KT>
+template <bool EXACT, int KT, int MODE>
 __device__ __forceinline__ bool pair_direct(int p, const StepBufs& b, const DevScene* __restrict__ sc) {
   const uint32_t tg = b.tagA[p];
   const DevScene& S = sc[b.sceneOf[p]];
@@ -831,8 +1090,8 @@ __device__ __forceinline__ bool pair_direct(int p, const StepBufs& b, const DevS
     wall_update(S, b, p, tg, itime, mass);
     return false;
   }
-  PairCtx c;
-  c.init(S, tg, b.mA[p], b.mB[p]);
+  CtxOf<MODE> c;
+  c.init(S, tg, b.mA[p], b.mB[p], mirror_c<MODE>(b, (uint32_t)p));
   const double cut = S.cutoff;
   double kpd2 = INFINITY;
   uint32_t kpid = 0xffffffffu;
@@ -848,7 +1107,7 @@ __device__ __forceinline__ bool pair_direct(int p, const StepBufs& b, const DevS
     const float4 bq = b.mB[q];
     const uint32_t tq = __float_as_uint(bq.w);
     if (!c.template admit<EXACT>(S, tag_id(tq), d2, kpd2, kpid, b, q)) return;
-    c.template add<KT>(S, ex, ey, tq, b.mA[q], bq);
+    c.template add<KT>(S, ex, ey, tq, b.mA[q], bq, mirror_c<MODE>(b, (uint32_t)q));
   });
   if (!EXACT && cnt > S.K) return true;
   correct(S, b, p, itime, xp, vn, mass, c);
@@ -857,7 +1116,7 @@ __device__ __forceinline__ bool pair_direct(int p, const StepBufs& b, const DevS
 
 // EXACT (the capped path) runs as a small grid-stride launch that returns at
 // once unless the step's overflow flag is set.
-template <bool EXACT, int KT>
+template <bool EXACT, int KT, int MODE>
 __global__ __launch_bounds__(BLK) void k_pair(int n, StepBufs b, const DevScene* __restrict__ sc) {
   if (EXACT && b.ctl[1] == 0) return;
   if (EXACT && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&b.ctl[2], 1);
@@ -868,19 +1127,19 @@ __global__ __launch_bounds__(BLK) void k_pair(int n, StepBufs b, const DevScene*
   for (int lb = xcd_block(); lb < nlog; lb += gridDim.x) {
     const int p = lb * BLK + threadIdx.x;
     bool ovf = false;
-    if (p < n) ovf = pair_direct<EXACT, KT>(p, b, sc);
+    if (p < n) ovf = pair_direct<EXACT, KT, MODE>(p, b, sc);
     if (!EXACT) wave_overflow<false>(b, sc, p, ovf, lds[threadIdx.x >> 6]);
   }
 }
 
 // Pass 2 of k_pair_compact for one fluid particle: the pair sums over the set bits
 // of m[0..2] (row starts rb[0..2]) and the corrector.
-template <int KT>
+template <int KT, int MODE>
 __device__ __forceinline__ void compact_pairs(const DevScene& S, const StepBufs& b, int p, uint32_t tg, int itime,
                                               double2 xp, double2 vn, float mass, const int* rb,
                                               const unsigned long long* m) {
-  PairCtx c;
-  c.init(S, tg, b.mA[p], b.mB[p]);
+  CtxOf<MODE> c;
+  c.init(S, tg, b.mA[p], b.mB[p], mirror_c<MODE>(b, (uint32_t)p));
   // the set bits of the three rows as one stream, advanced without branches;
   // -1 when exhausted
   int rr = 0;
@@ -905,17 +1164,19 @@ __device__ __forceinline__ void compact_pairs(const DevScene& S, const StepBufs&
   struct Nb {
     double2 x;
     float4 a, b;
+    float2 c;
   };
   auto load = [&](Nb& o, int q) {
     const uint32_t s = (uint32_t)(q >= 0 ? q : p);
     o.x = ld_off(b.posA, s);
     o.a = ld_off(b.mA, s);
     o.b = ld_off(b.mB, s);
+    o.c = mirror_c<MODE>(b, s);
   };
   auto pair = [&](const Nb& o) {
     const uint32_t tq = __float_as_uint(o.b.w);
     if (c.template admit<false>(S, tag_id(tq), 0.0, 0.0, 0u, b, 0))
-      c.template add<KT>(S, xp.x - o.x.x, xp.y - o.x.y, tq, o.a, o.b);
+      c.template add<KT>(S, xp.x - o.x.x, xp.y - o.x.y, tq, o.a, o.b, o.c);
   };
   Nb n0, n1;
   int q0 = next();
@@ -941,7 +1202,7 @@ __device__ __forceinline__ void compact_pairs(const DevScene& S, const StepBufs&
 // next pair's loads issued before the current pair's math.  A row range longer
 // than 64 slots, or more than K candidates in range, raises the step's overflow
 // flag and the exact direct kernel recomputes the step.
-template <int PB, int KT>
+template <int PB, int KT, int MODE>
 __global__ __launch_bounds__(PB) void k_pair_compact(int n, StepBufs b, const DevScene* __restrict__ sc) {
   __shared__ KthLds lds[PB / 64];
   const int p = xcd_block() * PB + threadIdx.x;
@@ -1028,7 +1289,7 @@ __global__ __launch_bounds__(PB) void k_pair_compact(int n, StepBufs b, const De
       ovf = true;   // the exact kernel recomputes this step
       return;
     }
-    compact_pairs<KT>(S, b, p, tg, itime, xp, vn, mass, rb, m);
+    compact_pairs<KT, MODE>(S, b, p, tg, itime, xp, vn, mass, rb, m);
   };
   run();
   wave_overflow<true>(b, sc, p, ovf, lds[threadIdx.x >> 6], sidx, xp, rb, rlf);
@@ -1251,6 +1512,7 @@ struct uph_handle {
   int n = 0, ncells = 0, nscenes = 0;
   int itime = 0;
   int kernel = UPH_KERNEL_CUBIC;   // smoothing kernel, the same for every scene
+  int mode = UPH_MODE_CONFIG;      // scheme, the same for every scene
   std::vector<DevScene> hs;
   std::vector<int> fluid_n, tot_n;
   DevScene* dsc = nullptr;
@@ -1300,7 +1562,15 @@ static void fill_scene(DevScene& S, const uph_scene& u, double cell_factor) {
   S.fr0 = (float)(3.0 * u.dx);
   S.frho0 = (float)u.rho0;
   S.fk7 = (float)(-2145.0 / (192512.0 * M_PI * u.h * u.h * u.h * u.h));
+  S.fw7 = (float)(2145.0 / (1347584.0 * M_PI * u.h * u.h));
   S.kernel = u.kernel;
+  S.mode = u.mode;
+  S.eps = u.eps;
+  S.fK0 = (float)(u.rho0 * u.c0 * u.c0);
+  S.fgamma = (float)u.gamma;
+  S.fgk = (float)(u.gamma * u.kappa);
+  S.fkap = (float)u.kappa;
+  S.fmu = (float)u.mu;
 }
 
 static int alloc_bufs(StepBufs& b, int n, int ncells, int nblocks_scan, Err& err, bool search_only) {
@@ -1316,7 +1586,9 @@ static int alloc_bufs(StepBufs& b, int n, int ncells, int nblocks_scan, Err& err
   if (dalloc(&b.velA, n, err) || dalloc(&b.velB, n, err) || dalloc(&b.rhoA, n, err) ||
       dalloc(&b.rhoB, n, err) || dalloc(&b.rtB, n, err) || dalloc(&b.kthStep, n, err) ||
       dalloc(&b.mA, n, err) || dalloc(&b.mB, n, err) || dalloc(&b.deckPos, n, err) ||
-      dalloc(&b.deckVel, n, err) || dalloc(&b.deckLvel, n, err) || dalloc(&b.massD, n, err))
+      dalloc(&b.deckVel, n, err) || dalloc(&b.deckLvel, n, err) || dalloc(&b.massD, n, err) ||
+      dalloc(&b.presA, n, err) || dalloc(&b.presB, n, err) || dalloc(&b.deckP, n, err) ||
+      dalloc(&b.rt2B, n, err) || dalloc(&b.dvB, n, err) || dalloc(&b.mC, n, err))
     return -1;
   return 0;
 }
@@ -1325,7 +1597,7 @@ static void free_bufs(StepBufs& b) {
   void* ptrs[] = {b.posA, b.velA, b.rhoA, b.kthStep, b.tagA, b.mA, b.mB, b.posB, b.velB, b.rhoB,
                   b.rtB, b.tagB, b.keys, b.rank, b.tmpIdx, b.cellStart, b.cellCount, b.partial,
                   b.sceneOf, b.kthD2, b.kthId, b.deckPos, b.deckVel, b.deckLvel, b.massD, b.ctl,
-                  b.dbgRates};
+                  b.dbgRates, b.presA, b.presB, b.deckP, b.rt2B, b.dvB, b.mC};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   memset(&b, 0, sizeof b);
@@ -1345,25 +1617,25 @@ static int launch_sort(hipStream_t st, int n, int ncells, int nb, StepBufs& b, c
   return 0;
 }
 
-// The pair kernels of one smoothing kernel KT: the selected pass, then the exact
-// capped fixup (which returns at once unless the step overflowed).
-template <int KT>
+// The pair kernels of one smoothing kernel KT and scheme MODE: the selected pass,
+// then the exact capped fixup (which returns at once unless the step overflowed).
+template <int KT, int MODE>
 static int launch_pairs(uph_handle* h, hipStream_t st, hipEvent_t pb) {
   if (h->opts.pair_variant == 1) {
-    hipLaunchKernelGGL((k_pair<false, KT>), dim3(grid8_of(h->n)), dim3(BLK), 0, st, h->n, h->b, h->dsc);
+    hipLaunchKernelGGL((k_pair<false, KT, MODE>), dim3(grid8_of(h->n)), dim3(BLK), 0, st, h->n, h->b, h->dsc);
   } else {
     const int pbk = h->opts.pair_block;
     const int g = (((h->n + pbk - 1) / pbk + 7) / 8) * 8;
     if (pbk == 128)
-      hipLaunchKernelGGL((k_pair_compact<128, KT>), dim3(g), dim3(128), 0, st, h->n, h->b, h->dsc);
+      hipLaunchKernelGGL((k_pair_compact<128, KT, MODE>), dim3(g), dim3(128), 0, st, h->n, h->b, h->dsc);
     else if (pbk == 512)
-      hipLaunchKernelGGL((k_pair_compact<512, KT>), dim3(g), dim3(512), 0, st, h->n, h->b, h->dsc);
+      hipLaunchKernelGGL((k_pair_compact<512, KT, MODE>), dim3(g), dim3(512), 0, st, h->n, h->b, h->dsc);
     else
-      hipLaunchKernelGGL((k_pair_compact<256, KT>), dim3(g), dim3(256), 0, st, h->n, h->b, h->dsc);
+      hipLaunchKernelGGL((k_pair_compact<256, KT, MODE>), dim3(g), dim3(256), 0, st, h->n, h->b, h->dsc);
   }
   if (pb) HIPCHK(hipEventRecord(pb, st), h->err);
   const int gfix = std::min(grid8_of(h->n), FIX_GRID);
-  hipLaunchKernelGGL((k_pair<true, KT>), dim3(gfix), dim3(BLK), 0, st, h->n, h->b, h->dsc);
+  hipLaunchKernelGGL((k_pair<true, KT, MODE>), dim3(gfix), dim3(BLK), 0, st, h->n, h->b, h->dsc);
   return 0;
 }
 
@@ -1371,9 +1643,14 @@ static int launch_step(uph_handle* h, hipEvent_t pa, hipEvent_t pb) {
   hipStream_t st = h->stream;
   if (launch_sort(st, h->n, h->ncells, h->cs.nb, h->b, h->dsc, 0, true, h->err)) return -1;
   if (pa) HIPCHK(hipEventRecord(pa, st), h->err);
-  if ((h->kernel == UPH_KERNEL_POLY7 ? launch_pairs<UPH_KERNEL_POLY7>(h, st, pb)
-                                      : launch_pairs<UPH_KERNEL_CUBIC>(h, st, pb)) != 0)
-    return -1;
+  int rc;
+  if (h->mode == UPH_MODE_REF)
+    rc = h->kernel == UPH_KERNEL_POLY7 ? launch_pairs<UPH_KERNEL_POLY7, UPH_MODE_REF>(h, st, pb)
+                                       : launch_pairs<UPH_KERNEL_CUBIC, UPH_MODE_REF>(h, st, pb);
+  else
+    rc = h->kernel == UPH_KERNEL_POLY7 ? launch_pairs<UPH_KERNEL_POLY7, UPH_MODE_CONFIG>(h, st, pb)
+                                       : launch_pairs<UPH_KERNEL_CUBIC, UPH_MODE_CONFIG>(h, st, pb);
+  if (rc != 0) return -1;
   HIPCHK(hipGetLastError(), h->err);
   return 0;
 }
@@ -1408,6 +1685,12 @@ extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts*
       delete h;
       return -1;
     }
+    if ((u.mode != UPH_MODE_CONFIG && u.mode != UPH_MODE_REF) || u.mode != scenes[0].mode) {
+      g_create_err = "uph_create: scene " + std::to_string(s) +
+                     " has an unknown scheme (mode) or not the scheme of scene 0";
+      delete h;
+      return -1;
+    }
     for (int i = u.n_fluid; i < np; ++i)
       if (!(u.rho[i] > 0)) {
         g_create_err = "uph_create: wall densities must be positive";
@@ -1431,6 +1714,7 @@ extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts*
   }
   h->n = (int)n;
   h->kernel = scenes[0].kernel;
+  h->mode = scenes[0].mode;
   h->ncells = (int)nc;
   h->cs.m = (int)nc;
   h->cs.nb = (int)((nc + SCAN_TILE - 1) / SCAN_TILE);
@@ -1448,11 +1732,12 @@ extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts*
     return fail();
   }
   if (alloc_bufs(h->b, h->n, h->ncells, h->cs.nb, err, false)) return fail();
-  if (h->opts.keep_rates && dalloc(&h->b.dbgRates, 2 * (size_t)h->n, err)) return fail();
+  if (h->opts.keep_rates && dalloc(&h->b.dbgRates, (h->mode == UPH_MODE_REF ? 4 : 2) * (size_t)h->n, err))
+    return fail();
   if (dalloc(&h->dsc, n_scenes, err)) return fail();
   // host staging in original order (global index = slot at start)
   std::vector<double2> pos(n), vel(n), lv(n);
-  std::vector<double> rho(n), massv(n);
+  std::vector<double> rho(n), massv(n), pres(n);
   std::vector<float4> rt(n);
   std::vector<uint32_t> tag(n);
   std::vector<int> sceneOf(n);
@@ -1465,6 +1750,7 @@ extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts*
       vel[g] = make_double2(u.vel[2 * i], u.vel[2 * i + 1]);
       lv[g] = make_double2(u.lvel[2 * i], u.lvel[2 * i + 1]);
       rho[g] = u.rho[i];
+      pres[g] = u.pres ? u.pres[i] : 0.0;
       rt[g] = make_float4(0.f, 0.f, 0.f, (float)u.mass[i]);
       massv[g] = u.mass[i];
       int ty = u.itype[i];
@@ -1504,6 +1790,8 @@ extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts*
   if (hipMemcpy(h->b.deckVel, vdeck.data(), n * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(h->b.deckLvel, lv.data(), n * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(h->b.massD, massv.data(), n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(h->b.presB, pres.data(), n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(h->b.deckP, pres.data(), n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(h->b.posB, pos.data(), n * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(h->b.velB, vel.data(), n * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(h->b.rhoB, rho.data(), n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
@@ -1519,6 +1807,8 @@ extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts*
   if (hipMemset(h->b.cellCount, 0, (size_t)h->ncells * sizeof(int)) != hipSuccess ||
       hipMemset(h->b.posA + h->n, 0, POS_PAD * sizeof(double2)) != hipSuccess ||
       hipMemset(h->b.kthStep, 0, (size_t)h->n * sizeof(int)) != hipSuccess ||
+      hipMemset(h->b.rt2B, 0, (size_t)h->n * sizeof(float4)) != hipSuccess ||
+      hipMemset(h->b.dvB, 0, (size_t)h->n * sizeof(float2)) != hipSuccess ||
       hipMemset(h->b.partial, 0, (size_t)h->cs.nb * sizeof(int)) != hipSuccess ||
       hipDeviceSynchronize() != hipSuccess) {
     err.msg = "uph_create: init failed";
@@ -1651,23 +1941,85 @@ static int get_rates_f64(uph_t* h, int scene, float* R, float* Rt, double* a, fl
   }
   HIPCHK(hipStreamSynchronize(h->stream), h->err);
   const DevScene& S = h->hs[scene];
-  std::vector<float4> d(2 * (size_t)S.np);
+  const int st = h->mode == UPH_MODE_REF ? 4 : 2;   // float4s per slot
+  std::vector<float4> d(st * (size_t)S.np);
   std::vector<uint32_t> t(S.np);
-  HIPCHK(hipMemcpy(d.data(), h->b.dbgRates + 2 * (size_t)S.poff, 2 * S.np * sizeof(float4), hipMemcpyDeviceToHost),
+  HIPCHK(hipMemcpy(d.data(), h->b.dbgRates + st * (size_t)S.poff, st * S.np * sizeof(float4),
+                   hipMemcpyDeviceToHost),
          h->err);
   HIPCHK(hipMemcpy(t.data(), h->b.tagA + S.poff, S.np * sizeof(uint32_t), hipMemcpyDeviceToHost), h->err);
   for (int k = 0; k < S.np; ++k) {
     const int i = (int)((t[k] & UPH_ID_MASK) - (uint32_t)S.poff);
     if (i < 0 || i >= S.np) continue;
     if (i >= S.nf) continue;
-    if (R) R[i] = d[2 * k].x;
-    if (Rt) Rt[i] = d[2 * k].y;
-    if (at) { at[2 * i] = d[2 * k].z; at[2 * i + 1] = d[2 * k].w; }
+    if (R) R[i] = d[st * k].x;
+    if (Rt) Rt[i] = d[st * k].y;
+    if (at) { at[2 * i] = d[st * k].z; at[2 * i + 1] = d[st * k].w; }
     if (a) {
       double2 ad;
-      memcpy(&ad, &d[2 * k + 1], sizeof ad);
+      memcpy(&ad, &d[st * k + 1], sizeof ad);
       a[2 * i] = ad.x;
       a[2 * i + 1] = ad.y;
+    }
+  }
+  return 0;
+}
+
+extern "C" int uph_get_ref_rates(uph_t* h, int scene, float* RP, float* RPt, float* drho, float* dP, float* dv) {
+  if (!h) return -1;
+  if (h->mode != UPH_MODE_REF || !h->b.dbgRates) {
+    h->err.msg = "uph_get_ref_rates: handle not in UPH_MODE_REF or created without keep_rates";
+    return -1;
+  }
+  if (scene < 0 || scene >= h->nscenes) {
+    h->err.msg = "uph_get_ref_rates: scene out of range";
+    return -1;
+  }
+  HIPCHK(hipStreamSynchronize(h->stream), h->err);
+  const DevScene& S = h->hs[scene];
+  std::vector<float4> d(4 * (size_t)S.np);
+  std::vector<uint32_t> t(S.np);
+  HIPCHK(hipMemcpy(d.data(), h->b.dbgRates + 4 * (size_t)S.poff, 4 * S.np * sizeof(float4), hipMemcpyDeviceToHost),
+         h->err);
+  HIPCHK(hipMemcpy(t.data(), h->b.tagA + S.poff, S.np * sizeof(uint32_t), hipMemcpyDeviceToHost), h->err);
+  for (int k = 0; k < S.np; ++k) {
+    const int i = (int)((t[k] & UPH_ID_MASK) - (uint32_t)S.poff);
+    if (i < 0 || i >= S.nf) continue;
+    const float4 e = d[4 * k + 2], f = d[4 * k + 3];
+    if (RP) RP[i] = e.x;
+    if (RPt) RPt[i] = e.y;
+    if (drho) drho[i] = e.z;
+    if (dP) dP[i] = e.w;
+    if (dv) { dv[2 * i] = f.x; dv[2 * i + 1] = f.y; }
+  }
+  return 0;
+}
+
+extern "C" int uph_get_pressure(uph_t* h, int scene, double* P) {
+  if (!h || !P) return -1;
+  if (scene < 0 || scene >= h->nscenes) {
+    h->err.msg = "uph_get_pressure: scene out of range";
+    return -1;
+  }
+  HIPCHK(hipStreamSynchronize(h->stream), h->err);
+  const DevScene& S = h->hs[scene];
+  std::vector<double> v(S.np);
+  std::vector<uint32_t> t(S.np);
+  HIPCHK(hipMemcpy(v.data(), (h->mode == UPH_MODE_REF ? h->b.presB : h->b.rhoB) + S.poff, S.np * sizeof(double),
+                   hipMemcpyDeviceToHost),
+         h->err);
+  HIPCHK(hipMemcpy(t.data(), h->b.tagA + S.poff, S.np * sizeof(uint32_t), hipMemcpyDeviceToHost), h->err);
+  for (int k = 0; k < S.np; ++k) {
+    const int i = (int)((t[k] & UPH_ID_MASK) - (uint32_t)S.poff);
+    if (i < 0 || i >= S.np) {
+      h->err.msg = "uph_get_pressure: corrupt particle tags";
+      return -1;
+    }
+    if (h->mode == UPH_MODE_REF) {
+      P[i] = v[k];
+    } else {   // EquationOfState (derivatives.c:1674-1682) of the density
+      const double x = v[k] / S.rho0;
+      P[i] = S.B * (pow(x, S.gamma) - 1.0);
     }
   }
   return 0;
@@ -1696,6 +2048,7 @@ extern "C" const char* uph_last_error(const uph_t* h) { return h ? h->err.msg.c_
 __global__ __launch_bounds__(DIAG_BLK) void k_diag(int poff, int np, const uint32_t* __restrict__ tag,
                                                     const double2* __restrict__ pos, const double2* __restrict__ vel,
                                                     const double* __restrict__ rho, const double* __restrict__ massD,
+                                                    const double* __restrict__ pres,
                                                     const DevScene* __restrict__ sc, int scene, double px, double py,
                                                     double inv_nf, double* __restrict__ part) {
   __shared__ double red[DIAG_Q][DIAG_BLK];
@@ -1715,7 +2068,9 @@ __global__ __launch_bounds__(DIAG_BLK) void k_diag(int poff, int np, const uint3
       q[1] = m * S.g * x.y;
       const double dx = x.x - px, dy = x.y - py;
       const double r = sqrt(dx * dx + dy * dy), hh = S.h;
-      const double P = tait(S, rh);
+      // DynamicPressure: the Tait pressure of the density in config mode, the
+      // integrated state in ref mode
+      const double P = pres ? pres[s] : tait(S, rh);
       if (r <= 3.0 * hh && P > 0.0) {
         const double coeff1 = 0.5 - 5.0 * exp(-9.0);
         const double w = (exp(-r * r / (hh * hh)) - exp(-9.0)) * m / (2.0 * M_PI * hh * hh * coeff1 * rh);
@@ -1754,7 +2109,8 @@ extern "C" int uph_diagnostics(uph_t* h, int scene, double probe_x, double probe
     h->diag_cap = nb;
   }
   hipLaunchKernelGGL(k_diag, dim3(nb), dim3(DIAG_BLK), 0, h->stream, S.poff, S.np, h->b.tagA, h->b.posB,
-                     h->b.velB, h->b.rhoB, h->b.massD, h->dsc, scene, probe_x, probe_y,
+                     h->b.velB, h->b.rhoB, h->b.massD, h->mode == UPH_MODE_REF ? h->b.presB : nullptr, h->dsc,
+                     scene, probe_x, probe_y,
                      S.nf > 0 ? 1.0 / S.nf : 0.0, h->diag);
   HIPCHK(hipGetLastError(), h->err);
   std::vector<double> part((size_t)nb * DIAG_Q);

@@ -32,7 +32,12 @@ struct DevScene {
   // FP32 copies for the pair sums
   float fh, finvh, fwalpha, fav, fc0, fk, fcd, fr0, frho0;
   float fk7;                // PolyOrderSeven gradient factor -2145 / (192512 pi h^4)
+  float fw7;                // PolyOrderSeven W factor 2145 / (1347584 pi h^2)
   int kernel;               // UPH_KERNEL_*
+  // UPH_MODE_REF
+  int mode;
+  double eps;               // weight of the velocity fluctuation in localVelocity
+  float fK0, fgamma, fgk, fkap, fmu;  // rho0 c^2, gamma, gamma kappa, kappa, viscosity0
 };
 
 __device__ __forceinline__ uint32_t tag_id(uint32_t t) { return t & UPH_ID_MASK; }

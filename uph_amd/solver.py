@@ -31,7 +31,8 @@ class UphScene(C.Structure):
                 ("gravity", C.c_int), ("alpha_av", C.c_double), ("k_spring", C.c_double),
                 ("c_damp", C.c_double), ("dt", C.c_double), ("K", C.c_int), ("motion", C.c_int),
                 ("m_amp", C.c_double), ("m_omega", C.c_double), ("box", C.c_double * 4),
-                ("kernel", C.c_int)]
+                ("kernel", C.c_int), ("pres", _dp), ("mode", C.c_int), ("kappa", C.c_double),
+                ("mu", C.c_double), ("eps", C.c_double)]
 
 
 class UphDiag(C.Structure):
@@ -63,6 +64,8 @@ SIGNATURES = {
     "uph_get_rates": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_float),
                                 C.POINTER(C.c_float), C.POINTER(C.c_float),
                                 C.POINTER(C.c_float)]),
+    "uph_get_ref_rates": (C.c_int, [C.c_void_p, C.c_int] + [C.POINTER(C.c_float)] * 5),
+    "uph_get_pressure": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
     "uph_itime": (C.c_int, [C.c_void_p]),
     "uph_time_steps": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp]),
     "uph_last_error": (C.c_char_p, [C.c_void_p]),
@@ -202,7 +205,8 @@ class GpuSolver:
             mass = np.ascontiguousarray(sc.mass, dtype=np.float64)
             ity = np.ascontiguousarray(sc.itype, dtype=np.int32)
             lvel = np.ascontiguousarray(sc.lvel, dtype=np.float64).reshape(-1)
-            self._keep += [pos, vel, rho, mass, ity, lvel]
+            pres = np.ascontiguousarray(sc.pressure, dtype=np.float64)
+            self._keep += [pos, vel, rho, mass, ity, lvel, pres]
             u = arr[k]
             u.n_fluid, u.n_fixed, u.n_moving = sc.n_fluid, sc.n_fixed, sc.n_moving
             u.pos, u.vel, u.rho, u.mass = _ptr(pos), _ptr(vel), _ptr(rho), _ptr(mass)
@@ -216,6 +220,9 @@ class GpuSolver:
             for q in range(4):
                 u.box[q] = float(sc.box[q])
             u.kernel = int(sc.kernel)
+            u.pres = _ptr(pres)
+            u.mode = int(sc.mode)
+            u.kappa, u.mu, u.eps = float(sc.kappa), float(sc.mu), float(sc.eps)
         opts = UphOpts(int(bool(use_graph)), int(bool(keep_rates)), int(pair_variant),
                        int(pair_block), float(cell_factor))
         self.opts = dict(use_graph=bool(use_graph), pair_variant=int(pair_variant),
@@ -269,6 +276,23 @@ class GpuSolver:
         self._check(lib().uph_get_rates(self.h, scene, _ptr(R, C.c_float), _ptr(Rt, C.c_float),
                                         _ptr(a, C.c_float), _ptr(at, C.c_float)))
         return dict(R=R, Rt=Rt, a=a, at=at)
+
+    def ref_rates(self, scene: int = 0):
+        """Ref-mode rates of the last step: dpdt, dpdt_approx, the density, pressure and
+        velocity fluctuations (fluid rows, FP32)."""
+        nf = self._scene(scene).n_fluid
+        out = {k: np.zeros(nf, np.float32) for k in ("RP", "RPt", "drho", "dP")}
+        out["dv"] = np.zeros((nf, 2), np.float32)
+        self._check(lib().uph_get_ref_rates(self.h, scene, *[_ptr(out[k], C.c_float) for k in
+                                                             ("RP", "RPt", "drho", "dP", "dv")]))
+        return out
+
+    def pressure(self, scene: int = 0):
+        """DynamicPressure of every particle in reference order (ref mode: the integrated
+        state; config mode: the Tait pressure of the density)."""
+        P = np.zeros(self._scene(scene).n)
+        self._check(lib().uph_get_pressure(self.h, scene, _ptr(P)))
+        return P
 
     def diagnostics(self, probe=(0.0, 0.0), scene: int = 0) -> dict:
         """Kinetic/potential energy, the pressure probe and the density consistency values
