@@ -83,6 +83,11 @@ def main():
     ap.add_argument("--graph", action="store_true", help="also time hipGraph replay")
     ap.add_argument("--kernel", choices=["cubic", "poly7"], default="cubic",
                     help="smoothing kernel: cubic spline (default) or the reference's PolyOrderSeven")
+    ap.add_argument("--scheme", choices=["config", "ref"], default="config",
+                    help="config (default): Tait pressure and artificial viscosity; ref: the reference's "
+                         "run-phase scheme (pressure transport, viscous forces, fluctuations) with its "
+                         "decks' viscosity0 1e-6 and epsilon 1 and thermalDiffusivity0 0.0015 (the decks' "
+                         "0.015 is explicitly unstable at this resolution)")
     ap.add_argument("--h-over-dx", type=float, default=1.3,
                     help="smoothing length in lattice spacings (1.3 default; the reference's decks "
                          "use 2.691, where the 40-neighbour cap binds for every particle)")
@@ -103,8 +108,13 @@ def main():
     if args.h_over_dx != 1.3:
         sc.h = args.h_over_dx * sc.dx
         sc.dt = 0.25 * sc.h / sc.c0
+    if args.scheme == "ref":
+        sc.mode = scenes.MODE_REF
+        # the pressure diffusion is explicit: with the decks' 0.015 m^2/s its SPH Laplacian
+        # needs dt < h^2 / (10 gamma kappa) = 3.2 us at h = 1.84 mm, below this dt of 7.3 us
+        sc.kappa, sc.mu, sc.eps, sc.alpha_av = 0.0015, 1e-6, 1.0, 0.0
     log(f"workload: {sc.n_fluid} fluid + {sc.n_boundary} wall particles, {args.kernel} kernel, "
-        f"h = {args.h_over_dx} dx")
+        f"h = {args.h_over_dx} dx, {args.scheme} scheme")
     g = GpuSolver(sc)
     g.step(args.warmup)
     g.sync()
@@ -132,14 +142,15 @@ def main():
         "data": "synthetic (generated dam-break lattice)",
         "config": {"workload": "dam break 1.0x2.0 m column, dx=sqrt(2) mm, 4x3 m tank",
                    "fluid_particles": sc.n_fluid, "wall_particles": sc.n_boundary,
-                   "kernel": args.kernel, "h_over_dx": args.h_over_dx, "dt": sc.dt,
+                   "kernel": args.kernel, "scheme": args.scheme, "h_over_dx": args.h_over_dx, "dt": sc.dt,
                    "capped_steps": overflow},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      # the committed counter run covers the default workload only
-                     "traffic": (traffic_from_profiles()
-                                 if args.kernel == "cubic" and args.h_over_dx == 1.3 else None),
-                     "kernel": f"k_pair_compact<256, {int(sc.kernel)}> ({args.kernel})",
+                     "traffic": (traffic_from_profiles() if args.kernel == "cubic" and
+                                 args.h_over_dx == 1.3 and args.scheme == "config" else None),
+                     "kernel": f"k_pair_compact<256, {int(sc.kernel)}, {int(sc.mode)}> ({args.kernel}, "
+                               f"{args.scheme})",
                      "kernel_ms": ms_pair},
     }
     if args.graph:

@@ -298,3 +298,28 @@ def test_kernel_must_be_uniform_across_a_batch():
     bad.kernel = 7
     with pytest.raises(RuntimeError, match="kernel"):
         GpuSolver(bad)
+
+
+def test_broken_state_is_reported_not_ground_through():
+    """70,000 fluid particles stacked inside one cell (as escaped particles of a run that
+    has blown up pile into the clamped edge cells): every query has more than 65,536
+    candidates, so the step flags the state instead of running an O(N) search per
+    particle, and sync() raises."""
+    import time
+    from uph_amd.solver import GpuSolver
+    rng = np.random.default_rng(0)
+    n = 70_000
+    pos = np.vstack([rng.uniform(0.5, 0.5 + 1e-4, (n, 2)), [[0.0, 0.0], [1.0, 1.0]]])
+    N = len(pos)
+    c0, B = scenes.tait_constants(1000.0, 7.0, 9.81, 1.0)
+    sc = scenes.Scene(pos=pos, vel=np.zeros((N, 2)), rho=np.full(N, 1000.0), pressure=np.zeros(N),
+                      mass=np.full(N, 0.01), itype=np.array([1] * n + [2, 2], np.int32),
+                      lvel=np.zeros((N, 2)), n_fluid=n, n_fixed=2, n_moving=0, dx=0.01, h=0.013,
+                      c0=c0, B=B, dt=1e-5, box=(0.0, 0.0, 1.0, 1.0))
+    _say("broken state: 70,000 particles in one cell")
+    g = GpuSolver(sc)
+    t = time.perf_counter()
+    g.step(1)
+    with pytest.raises(RuntimeError, match="blown up"):
+        g.sync()
+    assert time.perf_counter() - t < 10.0

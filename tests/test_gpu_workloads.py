@@ -283,3 +283,37 @@ def test_c_host_driver_poly7_on_the_waves_deck(tmp_path):
     bad = subprocess.run([drv, "1", "--kernel", "poly7", "--compat"], cwd=tmp_path,
                          capture_output=True, text=True, timeout=60)
     assert bad.returncode != 0 and "cubic" in bad.stderr
+
+
+def test_c_host_driver_reference_scheme_on_its_dambreak_deck(tmp_path):
+    """uph_run --scheme ref --kernel poly7: the reference's own configuration of its
+    dambreak deck (PolyOrderSeven at h = 2.691 dx with the 40-neighbour cap binding,
+    pressure transport from the deck's DynamicPressure column, viscosity0, epsilon,
+    thermalDiffusivity0, springs) against the oracle; the written DynamicPressure
+    column is the integrated state."""
+    from oracle.oracle import Sim
+    from uph_amd import scenes as S
+    from uph_amd.scenes import load_deck_dir
+    drv = os.path.join(ROOT, "uph_amd", "uph_run")
+    assert os.path.exists(drv), "uph_run not built"
+    _deck_dir(tmp_path, "dambreak.npz")
+    steps = 60
+    _say(f"uph_run --scheme ref --kernel poly7 {steps} on dambreak.npz")
+    out = subprocess.run([drv, str(steps), "--scheme", "ref", "--kernel", "poly7", "--no-output",
+                          "--out", "ref.dat"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    _say(out.stdout.strip())
+    a = np.loadtxt(tmp_path / "ref.dat", skiprows=1)
+    sc = load_deck_dir(str(tmp_path), mode=S.MODE_REF, kernel=S.KERNEL_POLY7)
+    o = Sim(sc, nthreads=0)
+    o.step(steps)
+    so, Po = o.state(), o.ref()["P"]
+    n = sc.n_fluid
+    drho = float(np.max(np.abs(a[:, 4] - so["rho"][:n]) / so["rho"][:n]))
+    dP = float(np.abs(a[:, 5] - Po[:n]).max() / np.abs(Po[:n]).max())
+    dpos = float(np.abs(a[:, 0:2] - so["x"][:n]).max()) / sc.dx
+    _say(f"reference scheme driver vs oracle: rho {drho:.2e}, P rel-to-max {dP:.2e}, |dx|/dx {dpos:.2e}")
+    assert drho < 1e-6 and dP < 1e-5 and dpos < 1e-4
+    bad = subprocess.run([drv, "1", "--scheme", "ref", "--compat"], cwd=tmp_path,
+                         capture_output=True, text=True, timeout=60)
+    assert bad.returncode != 0

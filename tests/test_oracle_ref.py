@@ -423,3 +423,29 @@ def test_ref_mode_hydrostatic_deck_in_balance(oracle_lib):
     assert np.abs(st["rho"][:nf] / sc.rho[:nf] - 1).max() < 0.03
     assert np.abs(st["x"][:nf] - sc.pos[:nf]).max() < 0.1 * sc.dx
     assert ref["P"][:nf].min() > -100 and ref["P"][:nf].max() < 1.5 * sc.pressure[:nf].max()
+
+
+def test_decks_diffusivity_is_unstable_at_fine_resolution(oracle_lib):
+    """The pressure diffusion is explicit.  The decks' thermalDiffusivity0 = 0.015 m^2/s
+    (set for dx = 6-10 mm) blows up within 50 steps at dx = sqrt(2) mm with the 2 m
+    column's c0 and dt = 0.25 h / c0 (its SPH Laplacian needs dt < h^2/(10 gamma kappa)
+    = 3.2 us; dt is 7.3 us), while 0.0015 stays physical: the reason the 1M ref-mode
+    benchmark uses the smaller value."""
+    from oracle.oracle import Sim
+    dx = 2 ** 0.5 * 1e-3
+    sc = scenes.dam_break(width=0.04, height=0.08, dx=dx, tank=(0.2, 0.2))
+    sc.c0, sc.B = scenes.tait_constants(1000.0, 7.0, 9.81, 2.0)
+    sc.dt = 0.25 * sc.h / sc.c0
+    sc.mode = scenes.MODE_REF
+    sc.mu, sc.eps, sc.alpha_av = 1e-6, 1.0, 0.0
+    nf = sc.n_fluid
+    out = {}
+    for kappa in (0.015, 0.0015):
+        sc.kappa = kappa
+        s = Sim(sc)
+        try:
+            s.step(50)
+            out[kappa] = float(np.abs(s.state()["rho"][:nf] - 1000.0).max())
+        except RuntimeError:          # the pair list overflows once particles pile up
+            out[kappa] = float("inf")
+    assert out[0.015] > 1e3 and out[0.0015] < 10.0, out

@@ -41,6 +41,12 @@
 // fills the chip (4 blocks of 256 per CU on 256 CUs); when it has none it returns
 // at once (measured no slower than a 512-block launch).
 #define FIX_GRID 1024
+// Candidates (slots of a query's 3x3 cells, or of one cell) beyond which a step
+// gives up on the query and flags the state as broken: no physical particle distribution comes
+// near it (a smoothing length of 6 lattice spacings with 4-fold coarsened cells
+// gives about 21,000), while a run that has blown up piles its escaped particles
+// into the clamped edge cells, where an exact search would cost O(N) per query.
+#define CAND_MAX 65536
 
 // ------------------------------------------------------------------ sort
 
@@ -233,7 +239,8 @@ struct StepBufs {
   double *presA, *presB, *deckP;
   float4* rt2B;
   float2 *dvB, *mC;
-  // control: [0] itime, [1] overflow flag of the step, [2] overflowed steps, [3] step stamp
+  // control: [0] itime, [1] overflow flag of the step, [2] overflowed steps, [3] step stamp,
+  // [4] broken state (a query with more than CAND_MAX candidates; sticky)
   int* ctl;
   float4* dbgRates;   // keep_rates: per slot {R, R~, a~x, a~y} (FP32), {ax, ay} (FP64);
                       // UPH_MODE_REF adds {dpdt, dpdt_approx, drho, dP}, {dvx, dvy, 0, 0}
@@ -255,9 +262,14 @@ __global__ __launch_bounds__(BLK) void k_reorder(int n, StepBufs b, const DevSce
   const int c0 = b.cellStart[c], c1 = b.cellStart[c + 1];
   const uint32_t tg = b.tagB[i];
   const uint32_t my = tag_id(tg);
-  int r = 0;
-  for (int t = c0; t < c1; ++t) r += (tag_id(b.tagB[b.tmpIdx[t]]) < my) ? 1 : 0;
-  const int dst = c0 + r;
+  int dst = d;   // the provisional slot: a cell beyond CAND_MAX marks a broken state
+  if (c1 - c0 <= CAND_MAX) {
+    int r = 0;
+    for (int t = c0; t < c1; ++t) r += (tag_id(b.tagB[b.tmpIdx[t]]) < my) ? 1 : 0;
+    dst = c0 + r;
+  } else if (!search_only) {
+    atomicOr(&b.ctl[4], 1);
+  }
   const double2 p = b.posB[i];
   b.posA[dst] = p;
   b.tagA[dst] = tg;
@@ -433,11 +445,13 @@ __device__ __forceinline__ void candidate_rows(const DevScene& S, double2 x, con
 
 // Lists the in-range candidates (d2 < cutoff, not q) of slot q at xq, row ranges
 // rb/rl, in L; all arguments are the same on every lane.  Returns how many are in
-// range (only the first KTH_CAP are listed).
+// range (only the first KTH_CAP are listed), or -1 for more than CAND_MAX
+// candidates (nothing listed).
 __device__ int wave_list(const DevScene& S, int q, double2 xq, const int* rb, const int* rl,
                          const double2* __restrict__ pos, const uint32_t* __restrict__ tag, KthLds& L) {
   const int lane = threadIdx.x & 63;
   const int total = rl[0] + rl[1] + rl[2];
+  if (total > CAND_MAX) return -1;
   int n = 0;
   // up to 6 chunks of 64 candidates per pass, all loads issued before any is used
   // (a chunk past the end is skipped; slots past the end of the last chunk load
@@ -579,14 +593,16 @@ __device__ void wave_kth_listed(const KthLds& L, int n, int K, uint32_t seed, do
 }
 
 // K-th key of slot q on every lane (+inf when at most K are in range), from its
-// n in-range candidates listed in L.
+// n in-range candidates listed in L.  n = -1 (more than CAND_MAX candidates):
+// +inf, unless `exact`, when one lane searches the candidates alone.
 __device__ void wave_kth_of_list(const DevScene& S, int q, const double2* __restrict__ pos,
                                  const uint32_t* __restrict__ tag, const int* __restrict__ cellStart, KthLds& L,
-                                 int n, double& kd2, uint32_t& kid) {
+                                 int n, double& kd2, uint32_t& kid, bool exact = false) {
   kd2 = INFINITY;
   kid = 0xffffffffu;
-  if (n <= S.K) return;
-  if (n <= KTH_CAP) {
+  if (n < 0 && !exact) return;
+  if (n >= 0 && n <= S.K) return;
+  if (n >= 0 && n <= KTH_CAP) {
     // as few owned-key slots as the list needs (the count is the same on every lane)
     if (n <= 128)
       wave_kth_listed<2>(L, n, S.K, (uint32_t)q, kd2, kid);
@@ -603,11 +619,13 @@ __device__ void wave_kth_of_list(const DevScene& S, int q, const double2* __rest
   kid = (uint32_t)bcast((int)kid, 0);
 }
 
-// K-th key of slot q on every lane; n: the number in range.
+// K-th key of slot q on every lane; n: the number in range (-1: more than
+// CAND_MAX candidates, see wave_kth_of_list for `exact`).
 __device__ void wave_kth(const DevScene& S, int q, const double2* __restrict__ pos, const uint32_t* __restrict__ tag,
-                         const int* __restrict__ cellStart, KthLds& L, int& n, double& kd2, uint32_t& kid) {
+                         const int* __restrict__ cellStart, KthLds& L, int& n, double& kd2, uint32_t& kid,
+                         bool exact = false) {
   n = wave_gather(S, q, pos, tag, cellStart, L);
-  wave_kth_of_list(S, q, pos, tag, cellStart, L, n, kd2, kid);
+  wave_kth_of_list(S, q, pos, tag, cellStart, L, n, kd2, kid, exact);
 }
 
 // The overflowing queries of this wavefront (more than K candidates in range,
@@ -651,6 +669,7 @@ __device__ void wave_overflow(const StepBufs& b, const DevScene* __restrict__ sc
       b.kthD2[q] = kd2;
       b.kthId[q] = kid;
       b.kthStep[q] = b.ctl[3];
+      if (n < 0) atomicOr(&b.ctl[4], 1);
     }
   }
 }
@@ -1090,6 +1109,14 @@ __device__ __forceinline__ bool pair_direct(int p, const StepBufs& b, const DevS
     wall_update(S, b, p, tg, itime, mass);
     return false;
   }
+  {
+    int rb[3], rl[3];
+    candidate_rows(S, xp, b.cellStart, rb, rl);
+    if (rl[0] + rl[1] + rl[2] > CAND_MAX) {   // a broken state: flag it, skip the query
+      atomicOr(&b.ctl[4], 1);
+      return false;
+    }
+  }
   CtxOf<MODE> c;
   c.init(S, tg, b.mA[p], b.mB[p], mirror_c<MODE>(b, (uint32_t)p));
   const double cut = S.cutoff;
@@ -1118,7 +1145,7 @@ __device__ __forceinline__ bool pair_direct(int p, const StepBufs& b, const DevS
 // once unless the step's overflow flag is set.
 template <bool EXACT, int KT, int MODE>
 __global__ __launch_bounds__(BLK) void k_pair(int n, StepBufs b, const DevScene* __restrict__ sc) {
-  if (EXACT && b.ctl[1] == 0) return;
+  if (EXACT && (b.ctl[1] == 0 || b.ctl[4] != 0)) return;
   if (EXACT && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&b.ctl[2], 1);
   __shared__ KthLds lds[EXACT ? 1 : BLK / 64];
   const int nlog = (n + BLK - 1) / BLK;
@@ -1345,9 +1372,9 @@ __global__ __launch_bounds__(BLK) void k_search_count(int nf_slots_unused, int n
     int nin;
     double kd2;
     uint32_t kid;
-    wave_kth(S, q, b.posA, b.tagA, b.cellStart, L, nin, kd2, kid);
+    wave_kth(S, q, b.posA, b.tagA, b.cellStart, L, nin, kd2, kid, true);
     int m = 0;
-    if (nin <= KTH_CAP) {
+    if (nin >= 0 && nin <= KTH_CAP) {
       // the listed keys this lane owns that the query keeps
       for (int j = lane; j < nin; j += 64) m += search_keeps(S, idq, L.d2[j], L.id[j], kd2, kid) ? 1 : 0;
       for (int o = 32; o > 0; o >>= 1) m += __shfl_xor(m, o, 64);
@@ -1423,7 +1450,7 @@ __global__ __launch_bounds__(BLK) void k_search_emit(int n, StepBufs b, const De
     const uint32_t kid = b.kthId[q];
     const int o = qoff[idq];
     const int nin = wave_gather(S, q, b.posA, b.tagA, b.cellStart, L);
-    if (nin <= KTH_CAP) {
+    if (nin >= 0 && nin <= KTH_CAP) {
       // each kept key this lane owns, ranked among the kept keys
       for (int j0 = 0; j0 < nin; j0 += 64) {
         const int j = j0 + lane;
@@ -1580,7 +1607,7 @@ static int alloc_bufs(StepBufs& b, int n, int ncells, int nblocks_scan, Err& err
       dalloc(&b.tmpIdx, n, err) || dalloc(&b.cellStart, (size_t)ncells + 1, err) ||
       dalloc(&b.cellCount, ncells, err) || dalloc(&b.partial, nblocks_scan, err) ||
       dalloc(&b.sceneOf, n, err) || dalloc(&b.kthD2, n, err) || dalloc(&b.kthId, n, err) ||
-      dalloc(&b.ctl, 4, err))
+      dalloc(&b.ctl, 8, err))
     return -1;
   if (search_only) return 0;
   if (dalloc(&b.velA, n, err) || dalloc(&b.velB, n, err) || dalloc(&b.rhoA, n, err) ||
@@ -1786,7 +1813,7 @@ extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts*
     const uph_scene& u = scenes[s];
     for (int i = 0; i < h->hs[s].np; ++i) vdeck[h->hs[s].poff + i] = make_double2(u.vel[2 * i], u.vel[2 * i + 1]);
   }
-  int ctl[4] = {0, 0, 0, 0};
+  int ctl[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   if (hipMemcpy(h->b.deckVel, vdeck.data(), n * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(h->b.deckLvel, lv.data(), n * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(h->b.massD, massv.data(), n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
@@ -1868,8 +1895,14 @@ extern "C" int uph_step(uph_t* h, int n_steps) {
 extern "C" int uph_sync(uph_t* h) {
   if (!h) return -1;
   HIPCHK(hipStreamSynchronize(h->stream), h->err);
-  int ctl[4];
+  int ctl[8];
   HIPCHK(hipMemcpy(ctl, h->b.ctl, sizeof ctl, hipMemcpyDeviceToHost), h->err);
+  if (ctl[4]) {
+    h->err.msg = "uph_sync: a particle had more than " + std::to_string(CAND_MAX) +
+                 " candidates in its 3x3 cells: the state has blown up (particles escaped or collapsed "
+                 "into one cell); the step's results are not valid";
+    return -1;
+  }
   return ctl[2];
 }
 
@@ -2848,6 +2881,7 @@ extern "C" int uph_derivatives(uph_system* s, double t) {
   }
   if (get_rates_f64(h, 0, c.R.data(), c.Rt.data(), c.a.data(), c.at.data()) != 0)
     return compat_fail("derivatives: " + err.msg);
+  if (uph_sync(h) < 0) return compat_fail("derivatives: " + err.msg);
   // rates and the equation of state for the fluid rows
   const double B = s->CompressionFactor, gm = s->AdiabaticConstant, r0 = s->density0;
   for (int i = 0; i < nf; ++i) {

@@ -7,8 +7,8 @@
  *   FixedBoundaryParticle.dat, MovingBoundaryParticle.dat, SolidParticle.dat
  * (12-column rows, input.c:57-72 / inputbp.c:88-150).
  *
- *   uph_run <nsteps> [--fluid FILE] [--h-factor F] [--alpha A] [--kernel cubic|poly7] [--graph]
- *           [--compat] [--out FILE] [--no-output]
+ *   uph_run <nsteps> [--fluid FILE] [--h-factor F] [--alpha A] [--kernel cubic|poly7]
+ *           [--scheme config|ref] [--graph] [--compat] [--out FILE] [--no-output]
  *
  * --compat runs the time loop on the host in the shape of sph.cc:88-168 over a
  * System object (include/uph_system.h) and calls the library's derivatives()
@@ -20,7 +20,11 @@
  * cubic spline (--kernel poly7: the PolyOrderSeven pair of CalculateKernel,
  * derivatives.c:767-823, device loop only), Tait pressure in the momentum equation, artificial viscosity
  * `alpha` (deck field 15 unless --alpha), walls reset from the decks, the
- * piston of inputbp.c:194-207 when movingBoundary == 1.  After the run the
+ * piston of inputbp.c:194-207 when movingBoundary == 1.  --scheme ref runs the
+ * reference's own run-phase scheme instead (DynamicPressure integrated from the
+ * deck column, viscous forces with viscosity0, fluctuations weighted by epsilon,
+ * diffusion thermalDiffusivity0; with --kernel poly7 this is the reference's
+ * configuration as its decks set it), device loop only.  After the run the
  * fluid state is written in the deck format (default FluidParticleFinal.dat).
  */
 #include <math.h>
@@ -256,7 +260,7 @@ int main(int argc, char **argv) {
   int nsteps = atoi(argv[1]);
   const char *fluid_file = "FluidParticle.dat", *out_file = "FluidParticleFinal.dat";
   double h_factor = 0.0, alpha = -1.0;
-  int use_graph = 0, compat = 0, write = 1, kernel = UPH_KERNEL_CUBIC;
+  int use_graph = 0, compat = 0, write = 1, kernel = UPH_KERNEL_CUBIC, mode = UPH_MODE_CONFIG;
   for (int a = 2; a < argc; a++) {
     if (!strcmp(argv[a], "--fluid") && a + 1 < argc) fluid_file = argv[++a];
     else if (!strcmp(argv[a], "--h-factor") && a + 1 < argc) h_factor = atof(argv[++a]);
@@ -268,6 +272,15 @@ int main(int argc, char **argv) {
       else if (!strcmp(k, "poly7")) kernel = UPH_KERNEL_POLY7;
       else {
         fprintf(stderr, "uph_run: unknown kernel %s (cubic, poly7)\n", k);
+        return 1;
+      }
+    }
+    else if (!strcmp(argv[a], "--scheme") && a + 1 < argc) {
+      const char *k = argv[++a];
+      if (!strcmp(k, "config")) mode = UPH_MODE_CONFIG;
+      else if (!strcmp(k, "ref")) mode = UPH_MODE_REF;
+      else {
+        fprintf(stderr, "uph_run: unknown scheme %s (config, ref)\n", k);
         return 1;
       }
     }
@@ -313,6 +326,7 @@ int main(int argc, char **argv) {
   all.pos = malloc(sizeof(double) * 2 * n); all.vel = malloc(sizeof(double) * 2 * n);
   all.lvel = malloc(sizeof(double) * 2 * n); all.rho = malloc(sizeof(double) * n);
   all.mass = malloc(sizeof(double) * n); all.itype = malloc(sizeof(int) * n);
+  all.pres = malloc(sizeof(double) * n);
   deck_t *parts[3] = {&fl, &fx, &mv};
   int o = 0;
   double lo[2] = {1e300, 1e300}, hi[2] = {-1e300, -1e300};
@@ -326,6 +340,7 @@ int main(int argc, char **argv) {
         if (all.pos[2 * o + d] > hi[d]) hi[d] = all.pos[2 * o + d];
       }
       all.rho[o] = parts[p]->rho[i];
+      all.pres[o] = parts[p]->pres[i];
       all.mass[o] = parts[p]->mass[i];
       all.itype[o] = parts[p]->itype[i];
     }
@@ -344,11 +359,14 @@ int main(int argc, char **argv) {
   s.motion = moving == 1 ? UPH_MOTION_PISTON : UPH_MOTION_NONE;
   s.m_amp = stroke; s.m_omega = 2.0 * M_PI / period;
   s.kernel = kernel;
+  s.mode = mode;
+  s.pres = all.pres;
+  s.kappa = v[27]; s.mu = v[10]; s.eps = v[13];   /* thermalDiffusivity0, viscosity0, epsilon */
   const double margin = 10 * s.h + (moving == 1 ? stroke : 0.0);
   s.box[0] = lo[0] - margin; s.box[1] = lo[1] - margin;
   s.box[2] = hi[0] + margin; s.box[3] = hi[1] + 2 * (hi[1] - lo[1]);
-  if (compat && kernel != UPH_KERNEL_CUBIC) {
-    fprintf(stderr, "uph_run: --compat evaluates the cubic spline only\n");
+  if (compat && (kernel != UPH_KERNEL_CUBIC || mode != UPH_MODE_CONFIG)) {
+    fprintf(stderr, "uph_run: --compat evaluates the cubic spline in config mode only\n");
     return 1;
   }
   if (compat) {
@@ -418,7 +436,7 @@ int main(int argc, char **argv) {
     free(sy.DynamicPressure); free(sy.DynamicPressure_min); free(sy.localPressure); free(sy.dpdt);
     free(sy.dpdt_approx); free(sy.hsml); free(sy.ddensitydt); free(sy.ddensitydt_approx);
     free(sy.densityFluctuation); free(sy.pressureFluctuation); free(sy.itype);
-    free(all.pos); free(all.vel); free(all.lvel); free(all.rho); free(all.mass); free(all.itype);
+    free(all.pos); free(all.vel); free(all.lvel); free(all.rho); free(all.mass); free(all.itype); free(all.pres);
     free_deck(&fl); free_deck(&fx); free_deck(&mv); free_deck(&so);
     return 0;
   }
@@ -467,6 +485,10 @@ int main(int argc, char **argv) {
           pres[i] = i < fl.n ? s.B * (pow(rho[i] / rho0, gamma) - 1.0) : (i < fl.n + fx.n ? fx.pres[i - fl.n] : mv.pres[i - fl.n - fx.n]);
           rhol[i] = i < fl.n ? rho[i] : rho0;
         }
+        if (mode == UPH_MODE_REF && uph_get_pressure(h, 0, pres) != 0) {   /* DynamicPressure */
+          fprintf(stderr, "uph_run: %s\n", uph_last_error(h));
+          return 1;
+        }
         snap_t o = {n, fl.n, pos, vel, rho, all.mass, pres, rhol, all.itype, s.h, done * dt, dg.kinetic_energy,
                     dg.potential_energy, dg.probe_pressure, dg.probe_norm, dg.probe_neighbors};
         if (write_output(&o, done) != 0) return 1;
@@ -498,17 +520,23 @@ int main(int argc, char **argv) {
     uph_destroy(h);
     return 1;
   }
+  double *pfin = malloc(sizeof(double) * n);
+  if (mode == UPH_MODE_REF && uph_get_pressure(h, 0, pfin) != 0) {
+    fprintf(stderr, "uph_run: %s\n", uph_last_error(h));
+    uph_destroy(h);
+    return 1;
+  }
   fprintf(fo, "%d\n", fl.n);
   for (int i = 0; i < fl.n; i++) {
-    const double P = s.B * (pow(rho[i] / rho0, gamma) - 1.0);
+    const double P = mode == UPH_MODE_REF ? pfin[i] : s.B * (pow(rho[i] / rho0, gamma) - 1.0);
     fprintf(fo, "%.10lf %.10lf %.10lf %.10lf %.10lf %.10lf %.10lf %.10lf %.10lf %d %.10lf %.10lf\n",
             pos[2 * i], pos[2 * i + 1], vel[2 * i], vel[2 * i + 1], rho[i], P, all.mass[i], 0.0, 0.0, 1,
             vel[2 * i], vel[2 * i + 1]);
   }
   fclose(fo);
   uph_destroy(h);
-  free(pos); free(vel); free(rho);
-  free(all.pos); free(all.vel); free(all.lvel); free(all.rho); free(all.mass); free(all.itype);
+  free(pos); free(vel); free(rho); free(pfin);
+  free(all.pos); free(all.vel); free(all.lvel); free(all.rho); free(all.mass); free(all.itype); free(all.pres);
   free_deck(&fl); free_deck(&fx); free_deck(&mv); free_deck(&so);
   return 0;
 }
