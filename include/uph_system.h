@@ -189,6 +189,23 @@ int uph_derivatives(uph_system *sys, double t);
 void derivatives(uph_system *sys, double t);
 const char *uph_derivatives_last_error(void);
 
+/* The scheme derivatives() evaluates, for the whole process (no counterpart in the
+ * reference, whose derivatives() is its run-phase scheme): mode UPH_MODE_CONFIG
+ * (default: the description above) or UPH_MODE_REF, kernel UPH_KERNEL_CUBIC or
+ * UPH_KERNEL_POLY7 (include/uph.h).  Without a call, the environment variable
+ * UPH_DERIVATIVES_SCHEME selects it at the first derivatives() call: "ref" (the
+ * reference's scheme with its PolyOrderSeven kernel), "ref-cubic" or "config".
+ *
+ * UPH_MODE_REF evaluates derivatives.c:4196-4315 as the reference's decks run it
+ * (flagInitSimul 1, stressType 1, spring walls) at the System's state: it reads
+ * the fluid rows' DynamicPressure, localDensity, localPressure and localVelocity
+ * as the caller's host loop (sph.cc:90-168) left them, plus thermalDiffusivity0,
+ * viscosity0 and epsilon, and writes ddensitydt, ddensitydt_approx, dvdt,
+ * dvdt_approx, dpdt, dpdt_approx, densityFluctuation, pressureFluctuation,
+ * velocityFluctuation and Pressure (Tait, output only); DynamicPressure stays the
+ * caller's.  Returns 0, or -1 for an unknown mode or kernel. */
+int uph_derivatives_scheme(int mode, int kernel);
+
 #ifdef __cplusplus
 }
 #endif

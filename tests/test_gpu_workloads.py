@@ -280,9 +280,9 @@ def test_c_host_driver_poly7_on_the_waves_deck(tmp_path):
     dpos = float(np.abs(a[:, 0:2] - so["x"][:n]).max()) / sc.dx
     _say(f"poly7 driver vs oracle: max rel rho {drho:.2e}, max |dx|/dx {dpos:.2e}")
     assert drho < 1e-6 and dpos < 1e-4
-    bad = subprocess.run([drv, "1", "--kernel", "poly7", "--compat"], cwd=tmp_path,
+    bad = subprocess.run([drv, "1", "--kernel", "poly9"], cwd=tmp_path,
                          capture_output=True, text=True, timeout=60)
-    assert bad.returncode != 0 and "cubic" in bad.stderr
+    assert bad.returncode != 0 and "unknown kernel" in bad.stderr
 
 
 def test_c_host_driver_reference_scheme_on_its_dambreak_deck(tmp_path):
@@ -314,6 +314,46 @@ def test_c_host_driver_reference_scheme_on_its_dambreak_deck(tmp_path):
     dpos = float(np.abs(a[:, 0:2] - so["x"][:n]).max()) / sc.dx
     _say(f"reference scheme driver vs oracle: rho {drho:.2e}, P rel-to-max {dP:.2e}, |dx|/dx {dpos:.2e}")
     assert drho < 1e-6 and dP < 1e-5 and dpos < 1e-4
-    bad = subprocess.run([drv, "1", "--scheme", "ref", "--compat"], cwd=tmp_path,
+    bad = subprocess.run([drv, "1", "--scheme", "reference"], cwd=tmp_path,
                          capture_output=True, text=True, timeout=60)
-    assert bad.returncode != 0
+    assert bad.returncode != 0 and "unknown scheme" in bad.stderr
+
+
+def test_c_host_driver_compat_reference_scheme(tmp_path):
+    """The reference's host loop (sph.cc:88-168, with its fluctuation terms) over a
+    System object, calling the library's derivatives() with the reference's own scheme
+    and kernel (uph_derivatives_scheme(UPH_MODE_REF, UPH_KERNEL_POLY7)), on the
+    reference's dambreak deck: against the device-resident loop and the oracle."""
+    from oracle.oracle import Sim
+    from uph_amd import scenes as S
+    from uph_amd.scenes import load_deck_dir
+    drv = os.path.join(ROOT, "uph_amd", "uph_run")
+    assert os.path.exists(drv), "uph_run not built"
+    _deck_dir(tmp_path, "dambreak.npz")
+    steps = 40
+    args = [drv, str(steps), "--scheme", "ref", "--kernel", "poly7", "--no-output"]
+    _say(f"uph_run --compat --scheme ref --kernel poly7 {steps} on dambreak.npz")
+    out = subprocess.run(args + ["--compat", "--out", "compat.dat"], cwd=tmp_path, capture_output=True,
+                         text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    _say(out.stdout.strip())
+    out = subprocess.run(args + ["--out", "device.dat"], cwd=tmp_path, capture_output=True, text=True,
+                         timeout=120)
+    assert out.returncode == 0, out.stderr
+    a = np.loadtxt(tmp_path / "compat.dat", skiprows=1)
+    b = np.loadtxt(tmp_path / "device.dat", skiprows=1)
+    sc = load_deck_dir(str(tmp_path), mode=S.MODE_REF, kernel=S.KERNEL_POLY7)
+    dx = sc.dx
+    dpos = float(np.abs(a[:, 0:2] - b[:, 0:2]).max()) / dx
+    drho = float(np.max(np.abs(a[:, 4] - b[:, 4]) / b[:, 4]))
+    dP = float(np.abs(a[:, 5] - b[:, 5]).max() / np.abs(b[:, 5]).max())
+    _say(f"compat vs device loop: |dx|/dx {dpos:.2e}, rho {drho:.2e}, P {dP:.2e}")
+    assert dpos < 1e-7 and drho < 1e-8 and dP < 1e-6
+    o = Sim(sc, nthreads=0)
+    o.step(steps)
+    so, Po = o.state(), o.ref()["P"]
+    n = sc.n_fluid
+    drho_o = float(np.max(np.abs(a[:, 4] - so["rho"][:n]) / so["rho"][:n]))
+    dP_o = float(np.abs(a[:, 5] - Po[:n]).max() / np.abs(Po[:n]).max())
+    _say(f"compat vs oracle: rho {drho_o:.2e}, P {dP_o:.2e}")
+    assert drho_o < 1e-6 and dP_o < 1e-5

@@ -237,6 +237,7 @@ struct StepBufs {
   // pressure fluctuations of the last step {dpdt_approx, drho, dP, 0}, the velocity
   // fluctuation, and the neighbour mirror {localDensity, localPressure}
   double *presA, *presB, *deckP;
+  double *deckRl, *deckPl;   // localDensity, localPressure of the first step, per original index
   float4* rt2B;
   float2 *dvB, *mC;
   // control: [0] itime, [1] overflow flag of the step, [2] overflowed steps, [3] step stamp,
@@ -253,7 +254,8 @@ struct StepBufs {
 // also predicts DynamicPressure (P += dt/2 dpdt_approx) and adds the last step's
 // fluctuations to the local fields: localDensity = rho + drho, localPressure =
 // P + dP, localVelocity = v + dt/2 at + eps dv; at the first step they are
-// rho0, 0 and the deck's localVelocity (input.c:109-110).
+// deckRl, deckPl (rho0 and 0 from uph_create, input.c:109-110) and the deck's
+// localVelocity.
 __global__ __launch_bounds__(BLK) void k_reorder(int n, StepBufs b, const DevScene* __restrict__ sc, int search_only) {
   const int d = blockIdx.x * BLK + threadIdx.x;
   if (d >= n) return;
@@ -310,7 +312,10 @@ __global__ __launch_bounds__(BLK) void k_reorder(int n, StepBufs b, const DevSce
     // (inputbp.c:117-150, 289-290)
     double P = tag_type(tg) == 1 ? b.presB[i] : b.deckP[my];
     double Pp = P, rl = S.rho0, pl = 0.0;
-    if (tag_type(tg) == 1 && itime != 1) {
+    if (tag_type(tg) == 1 && itime == 1) {
+      rl = b.deckRl[my];
+      pl = b.deckPl[my];
+    } else if (tag_type(tg) == 1) {
       const float4 r2 = b.rt2B[i];
       const float2 dv = b.dvB[i];
       Pp = P + 0.5 * dt * (double)r2.x;
@@ -1615,7 +1620,8 @@ static int alloc_bufs(StepBufs& b, int n, int ncells, int nblocks_scan, Err& err
       dalloc(&b.mA, n, err) || dalloc(&b.mB, n, err) || dalloc(&b.deckPos, n, err) ||
       dalloc(&b.deckVel, n, err) || dalloc(&b.deckLvel, n, err) || dalloc(&b.massD, n, err) ||
       dalloc(&b.presA, n, err) || dalloc(&b.presB, n, err) || dalloc(&b.deckP, n, err) ||
-      dalloc(&b.rt2B, n, err) || dalloc(&b.dvB, n, err) || dalloc(&b.mC, n, err))
+      dalloc(&b.rt2B, n, err) || dalloc(&b.dvB, n, err) || dalloc(&b.mC, n, err) ||
+      dalloc(&b.deckRl, n, err) || dalloc(&b.deckPl, n, err))
     return -1;
   return 0;
 }
@@ -1624,7 +1630,7 @@ static void free_bufs(StepBufs& b) {
   void* ptrs[] = {b.posA, b.velA, b.rhoA, b.kthStep, b.tagA, b.mA, b.mB, b.posB, b.velB, b.rhoB,
                   b.rtB, b.tagB, b.keys, b.rank, b.tmpIdx, b.cellStart, b.cellCount, b.partial,
                   b.sceneOf, b.kthD2, b.kthId, b.deckPos, b.deckVel, b.deckLvel, b.massD, b.ctl,
-                  b.dbgRates, b.presA, b.presB, b.deckP, b.rt2B, b.dvB, b.mC};
+                  b.dbgRates, b.presA, b.presB, b.deckP, b.rt2B, b.dvB, b.mC, b.deckRl, b.deckPl};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   memset(&b, 0, sizeof b);
@@ -1764,7 +1770,7 @@ extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts*
   if (dalloc(&h->dsc, n_scenes, err)) return fail();
   // host staging in original order (global index = slot at start)
   std::vector<double2> pos(n), vel(n), lv(n);
-  std::vector<double> rho(n), massv(n), pres(n);
+  std::vector<double> rho(n), massv(n), pres(n), rl0(n);
   std::vector<float4> rt(n);
   std::vector<uint32_t> tag(n);
   std::vector<int> sceneOf(n);
@@ -1778,6 +1784,7 @@ extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts*
       lv[g] = make_double2(u.lvel[2 * i], u.lvel[2 * i + 1]);
       rho[g] = u.rho[i];
       pres[g] = u.pres ? u.pres[i] : 0.0;
+      rl0[g] = u.rho0;
       rt[g] = make_float4(0.f, 0.f, 0.f, (float)u.mass[i]);
       massv[g] = u.mass[i];
       int ty = u.itype[i];
@@ -1819,6 +1826,8 @@ extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts*
       hipMemcpy(h->b.massD, massv.data(), n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(h->b.presB, pres.data(), n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(h->b.deckP, pres.data(), n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(h->b.deckRl, rl0.data(), n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(h->b.deckPl, 0, n * sizeof(double)) != hipSuccess ||
       hipMemcpy(h->b.posB, pos.data(), n * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(h->b.velB, vel.data(), n * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(h->b.rhoB, rho.data(), n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
@@ -2355,6 +2364,23 @@ __global__ void k_state_in(int poff, int np, int nf, const uint32_t* __restrict_
   else if (restart && ivel) deckLvel[g] = ivel[i];
 }
 
+// UPH_MODE_REF fluid rows from device arrays in the reference's order:
+// DynamicPressure into the state, localDensity and localPressure into the fields
+// the first-step form of the next step reads.
+__global__ void k_ref_in(int poff, int np, int nf, const uint32_t* __restrict__ tag, double* __restrict__ pres,
+                         double* __restrict__ deckRl, double* __restrict__ deckPl, const double* __restrict__ iP,
+                         const double* __restrict__ irl, const double* __restrict__ ipl) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= np) return;
+  const int s = poff + k;
+  const uint32_t g = tag_id(tag[s]);
+  const int i = (int)g - poff;
+  if (i < 0 || i >= nf) return;
+  pres[s] = iP[i];
+  deckRl[g] = irl[i];
+  deckPl[g] = ipl[i];
+}
+
 // Boundary slots of one scene at time t: the deck state plus the wall kinematics.
 __global__ void k_walls_at(int poff, int np, int nf, const uint32_t* __restrict__ tag, double2* __restrict__ pos,
                            double2* __restrict__ vel, const double2* __restrict__ deckPos,
@@ -2663,8 +2689,31 @@ struct CompatCtx {
   std::vector<double> hrho;
   std::vector<float> R, Rt, at;
   std::vector<double> a;
+  // scheme of the seam: config mode (default) or the reference's run-phase scheme
+  int mode = UPH_MODE_CONFIG, kernel = UPH_KERNEL_CUBIC;
+  bool scheme_set = false;
+  std::vector<double> hP, hrl, hpl;
+  std::vector<float> RP, RPt, fr, fP, fv;
 };
 CompatCtx g_compat;
+
+// The seam's scheme: set by uph_derivatives_scheme, else by the environment
+// variable UPH_DERIVATIVES_SCHEME ("config", "ref" = the reference's scheme with
+// its PolyOrderSeven kernel, "ref-cubic"), else config mode.
+void compat_scheme_from_env() {
+  CompatCtx& c = g_compat;
+  if (c.scheme_set) return;
+  c.scheme_set = true;
+  const char* e = getenv("UPH_DERIVATIVES_SCHEME");
+  if (!e) return;
+  if (!strcmp(e, "ref")) {
+    c.mode = UPH_MODE_REF;
+    c.kernel = UPH_KERNEL_POLY7;
+  } else if (!strcmp(e, "ref-cubic")) {
+    c.mode = UPH_MODE_REF;
+    c.kernel = UPH_KERNEL_CUBIC;
+  }
+}
 
 int compat_fail(const std::string& m) {
   g_compat.err = m;
@@ -2696,11 +2745,13 @@ void compat_release() {
 
 // Everything the device context depends on besides the particle rows.
 std::vector<double> compat_key(const uph_system* s) {
+  const CompatCtx& c = g_compat;
   return {(double)s->NumberOfFluidParticles, (double)s->NumberOfFixedBoundaryParticles,
           (double)s->NumberOfMovingBoundaryParticles, s->hsml[0], s->timeStep, s->dx, s->density0,
           s->AdiabaticConstant, s->SpeedOfSound, s->CompressionFactor, s->Gravity, (double)s->gravityFlag,
           s->alpha, s->springConstant0, s->dampingConstant0, (double)s->HowManyNeighbors,
-          (double)s->movingBoundary, s->stroke, s->omega};
+          (double)s->movingBoundary, s->stroke, s->omega, (double)c.mode, (double)c.kernel,
+          s->thermalDiffusivity0, s->viscosity0, s->epsilon};
 }
 
 int compat_build(const uph_system* s) {
@@ -2752,6 +2803,15 @@ int compat_build(const uph_system* s) {
   u.motion = s->movingBoundary == 1 ? UPH_MOTION_PISTON : UPH_MOTION_NONE;
   u.m_amp = s->stroke;
   u.m_omega = s->omega;
+  u.kernel = c.kernel;
+  u.mode = c.mode;
+  u.kappa = s->thermalDiffusivity0;
+  u.mu = s->viscosity0;
+  u.eps = s->epsilon;
+  std::vector<double> pres(np, 0.0);
+  if (s->DynamicPressure)
+    for (int i = 0; i < np; ++i) pres[i] = s->DynamicPressure[i];
+  u.pres = pres.data();
   // cell grid: the particles' box with a 10 h margin, twice the height above
   const double mg = 10.0 * u.h;
   u.box[0] = lo[0] - mg;
@@ -2764,7 +2824,8 @@ int compat_build(const uph_system* s) {
   o.pair_block = 256;
   o.cell_factor = 1.0;
   if (uph_create(&u, 1, &o, &c.h) != 0) return compat_fail(std::string("derivatives: ") + uph_last_error(nullptr));
-  if (hipMalloc((void**)&c.dstage, ((size_t)3 * nf + (nf + 1) / 2 + 1) * sizeof(double2)) != hipSuccess) {
+  // staging: pos, vel, lvel (nf double2 each), then rho, P, localDensity, localPressure (nf doubles each)
+  if (hipMalloc((void**)&c.dstage, ((size_t)3 * nf + 2 * (size_t)nf + 1) * sizeof(double2)) != hipSuccess) {
     compat_release();
     return compat_fail("derivatives: device staging allocation failed");
   }
@@ -2798,6 +2859,14 @@ int compat_build(const uph_system* s) {
   c.Rt.resize(nf);
   c.a.resize(2 * (size_t)nf);
   c.at.resize(2 * (size_t)nf);
+  c.hP.resize(nf);
+  c.hrl.resize(nf);
+  c.hpl.resize(nf);
+  c.RP.resize(nf);
+  c.RPt.resize(nf);
+  c.fr.resize(nf);
+  c.fP.resize(nf);
+  c.fv.resize(2 * (size_t)nf);
   return 0;
 }
 }  // namespace
@@ -2819,6 +2888,11 @@ extern "C" int uph_derivatives(uph_system* s, double t) {
     return compat_fail("derivatives: a required System array is not allocated");
   if (!(s->timeStep > 0) || !(s->hsml[0] > 0) || !(s->density0 > 0))
     return compat_fail("derivatives: timeStep, hsml and density0 must be positive");
+  compat_scheme_from_env();
+  const bool ref = c.mode == UPH_MODE_REF;
+  if (ref && (!s->DynamicPressure || !s->localDensity || !s->localPressure || !s->dpdt || !s->dpdt_approx))
+    return compat_fail("derivatives: the reference scheme needs DynamicPressure, localDensity, localPressure, "
+                       "dpdt and dpdt_approx");
   if (!c.h || c.sys != s || c.key != compat_key(s)) {
     if (compat_build(s)) return -1;
   }
@@ -2849,6 +2923,21 @@ extern "C" int uph_derivatives(uph_system* s, double t) {
                  chk(hipMemcpy(dlv, c.hlv.data(), nf * sizeof(double2), hipMemcpyHostToDevice), "upload") ||
                  chk(hipMemcpy(drho, c.hrho.data(), nf * sizeof(double), hipMemcpyHostToDevice), "upload")))
     return -1;
+  double* dP = drho + nf;
+  double* drl = dP + nf;
+  double* dpl = drl + nf;
+  if (ref && nf > 0) {
+    // the reference's state: DynamicPressure and the local fields the host loop built
+    for (int i = 0; i < nf; ++i) {
+      c.hP[i] = s->DynamicPressure[i];
+      c.hrl[i] = s->localDensity[i];
+      c.hpl[i] = s->localPressure[i];
+    }
+    if (chk(hipMemcpy(dP, c.hP.data(), nf * sizeof(double), hipMemcpyHostToDevice), "upload") ||
+        chk(hipMemcpy(drl, c.hrl.data(), nf * sizeof(double), hipMemcpyHostToDevice), "upload") ||
+        chk(hipMemcpy(dpl, c.hpl.data(), nf * sizeof(double), hipMemcpyHostToDevice), "upload"))
+      return -1;
+  }
   S.t0 = t - S.dt;
   const int zero = 0;
   if (chk(hipMemcpy(h->dsc, h->hs.data(), sizeof(DevScene), hipMemcpyHostToDevice), "scene upload") ||
@@ -2860,6 +2949,10 @@ extern "C" int uph_derivatives(uph_system* s, double t) {
                      (const double*)drho, (const double2*)dlv, h->b.deckLvel, 1);
   hipLaunchKernelGGL(k_walls_at, dim3(grid_of(S.np)), dim3(BLK), 0, h->stream, S.poff, S.np, S.nf, h->b.tagA,
                      h->b.posB, h->b.velB, h->b.deckPos, h->b.deckVel, h->dsc, 0, t);
+  if (ref)
+    hipLaunchKernelGGL(k_ref_in, dim3(grid_of(S.np)), dim3(BLK), 0, h->stream, S.poff, S.np, S.nf, h->b.tagA,
+                       h->b.presB, h->b.deckRl, h->b.deckPl, (const double*)dP, (const double*)drl,
+                       (const double*)dpl);
   if (chk(hipGetLastError(), "launch")) return -1;
   // the diagnostics derivatives() leaves in the System (derivatives.c:4317-4324), of
   // the state it was called with
@@ -2881,6 +2974,8 @@ extern "C" int uph_derivatives(uph_system* s, double t) {
   }
   if (get_rates_f64(h, 0, c.R.data(), c.Rt.data(), c.a.data(), c.at.data()) != 0)
     return compat_fail("derivatives: " + err.msg);
+  if (ref && uph_get_ref_rates(h, 0, c.RP.data(), c.RPt.data(), c.fr.data(), c.fP.data(), c.fv.data()) != 0)
+    return compat_fail("derivatives: " + err.msg);
   if (uph_sync(h) < 0) return compat_fail("derivatives: " + err.msg);
   // rates and the equation of state for the fluid rows
   const double B = s->CompressionFactor, gm = s->AdiabaticConstant, r0 = s->density0;
@@ -2893,7 +2988,20 @@ extern "C" int uph_derivatives(uph_system* s, double t) {
     s->dvdt_approx[i][1] = c.at[2 * i + 1];
     const double rho = s->density[i];
     const double P = compat_tait(s, rho);
-    if (s->Pressure) s->Pressure[i] = P;
+    if (s->Pressure) s->Pressure[i] = P;   // EquationOfState (derivatives.c:1674-1682)
+    if (ref) {
+      // DynamicPressure is the caller's state; the pressure rates and the
+      // fluctuation fields come from the device
+      s->dpdt[i] = c.RP[i];
+      s->dpdt_approx[i] = c.RPt[i];
+      if (s->densityFluctuation) s->densityFluctuation[i] = c.fr[i];
+      if (s->pressureFluctuation) s->pressureFluctuation[i] = c.fP[i];
+      if (s->velocityFluctuation) {
+        s->velocityFluctuation[i][0] = c.fv[2 * i];
+        s->velocityFluctuation[i][1] = c.fv[2 * i + 1];
+      }
+      continue;
+    }
     if (s->DynamicPressure) s->DynamicPressure[i] = P;
     const double dPdrho = B * gm / r0 * pow(rho / r0, gm - 1.0);
     if (s->dpdt) s->dpdt[i] = dPdrho * c.R[i];
@@ -2917,7 +3025,21 @@ extern "C" int uph_derivatives(uph_system* s, double t) {
     if (s->DynamicPressure) s->DynamicPressure[i] = c.deck_p[k];
     if (s->Pressure) s->Pressure[i] = compat_tait(s, c.deck_rho[k]);
     if (s->localDensity) s->localDensity[i] = r0;
+    if (s->localPressure) s->localPressure[i] = 0.0;   // inputbp.c:290
   }
+  return 0;
+}
+
+extern "C" int uph_derivatives_scheme(int mode, int kernel) {
+  CompatCtx& c = g_compat;
+  std::lock_guard<std::mutex> lk(c.mu);
+  if ((mode != UPH_MODE_CONFIG && mode != UPH_MODE_REF) || (kernel != UPH_KERNEL_CUBIC && kernel != UPH_KERNEL_POLY7)) {
+    c.err = "uph_derivatives_scheme: unknown mode or kernel";
+    return -1;
+  }
+  c.mode = mode;
+  c.kernel = kernel;
+  c.scheme_set = true;
   return 0;
 }
 

@@ -132,16 +132,17 @@ static int run_compat(uph_system *s, int nsteps, int print_step, int init_steps)
       memcpy(s->Velocity_min[0], s->Velocity[0], sizeof(double) * 2 * np);
       for (int i = 0; i < nf; i++) {
         s->density[i] += 0.5 * dt * s->ddensitydt_approx[i];
-        s->localDensity[i] = s->localDensity[i] + 0.5 * dt * s->ddensitydt_approx[i];
+        s->localDensity[i] = s->localDensity[i] + 0.5 * dt * s->ddensitydt_approx[i] + s->densityFluctuation[i];
         s->DynamicPressure[i] += 0.5 * dt * s->dpdt_approx[i];
-        s->localPressure[i] = s->localPressure[i] + 0.5 * dt * s->dpdt_approx[i];
+        s->localPressure[i] = s->localPressure[i] + 0.5 * dt * s->dpdt_approx[i] + s->pressureFluctuation[i];
       }
       for (int i = 0; i < nf; i++)
         for (int d = 0; d < 2; d++) {
           s->Velocity_min[i][d] = s->Velocity[i][d];
           s->Velocity_xsph[i][d] += 0.5 * dt * s->dvdt[i][d];
           s->Velocity[i][d] += 0.5 * dt * s->dvdt_approx[i][d];
-          s->localVelocity[i][d] = s->Velocity[i][d] + 0.5 * dt * s->dvdt_approx[i][d];
+          s->localVelocity[i][d] = s->Velocity[i][d] + 0.5 * dt * s->dvdt_approx[i][d] +
+                                   s->epsilon * s->velocityFluctuation[i][d];
         }
     } else {
       memcpy(s->Velocity_xsph[0], s->Velocity[0], sizeof(double) * 2 * np);
@@ -153,29 +154,33 @@ static int run_compat(uph_system *s, int nsteps, int print_step, int init_steps)
     /* corrector (sph.cc:137-168) */
     if (itime == 1) {
       for (int i = 0; i < nf; i++) {
-        s->localDensity[i] = s->density[i] + 0.5 * dt * s->ddensitydt[i];
+        s->localDensity[i] = s->density[i] + 0.5 * dt * s->ddensitydt[i] + s->densityFluctuation[i];
         s->density[i] += 0.5 * dt * s->ddensitydt[i];
-        s->localPressure[i] = s->DynamicPressure[i] + 0.5 * dt * s->dpdt[i];
+        s->localPressure[i] = s->DynamicPressure[i] + 0.5 * dt * s->dpdt[i] + s->pressureFluctuation[i];
         s->DynamicPressure[i] += 0.5 * dt * s->dpdt[i];
       }
       for (int i = 0; i < nf; i++)
         for (int d = 0; d < 2; d++) {
-          s->Velocity_xsph[i][d] = s->Velocity[i][d] + 0.5 * dt * s->dvdt[i][d];
-          s->localVelocity[i][d] = s->Velocity[i][d] + 0.5 * dt * s->dvdt[i][d];
+          s->Velocity_xsph[i][d] = s->Velocity[i][d] + 0.5 * dt * s->dvdt[i][d] +
+                                   s->epsilon * s->velocityFluctuation[i][d];
+          s->localVelocity[i][d] = s->Velocity[i][d] + 0.5 * dt * s->dvdt[i][d] +
+                                   s->epsilon * s->velocityFluctuation[i][d];
           s->Velocity[i][d] += 0.5 * dt * s->dvdt[i][d];
           s->Position[i][d] += dt * s->Velocity[i][d];
         }
     } else {
       for (int i = 0; i < nf; i++) {
-        s->localDensity[i] = s->density_min[i] + dt * s->ddensitydt[i];
+        s->localDensity[i] = s->density_min[i] + dt * s->ddensitydt[i] + s->densityFluctuation[i];
         s->density[i] = s->density_min[i] + dt * s->ddensitydt[i];
-        s->localPressure[i] = s->DynamicPressure_min[i] + dt * s->dpdt[i];
+        s->localPressure[i] = s->DynamicPressure_min[i] + dt * s->dpdt[i] + s->pressureFluctuation[i];
         s->DynamicPressure[i] = s->DynamicPressure_min[i] + dt * s->dpdt[i];
       }
       for (int i = 0; i < nf; i++)
         for (int d = 0; d < 2; d++) {
-          s->Velocity_xsph[i][d] = s->Velocity_min[i][d] + dt * s->dvdt[i][d];
-          s->localVelocity[i][d] = s->Velocity_min[i][d] + dt * s->dvdt[i][d];
+          s->Velocity_xsph[i][d] = s->Velocity_min[i][d] + dt * s->dvdt[i][d] +
+                                   s->epsilon * s->velocityFluctuation[i][d];
+          s->localVelocity[i][d] = s->Velocity_min[i][d] + dt * s->dvdt[i][d] +
+                                   s->epsilon * s->velocityFluctuation[i][d];
           s->Velocity[i][d] = s->Velocity_min[i][d] + dt * s->dvdt[i][d];
           s->Position[i][d] += dt * s->Velocity[i][d];
         }
@@ -365,8 +370,8 @@ int main(int argc, char **argv) {
   const double margin = 10 * s.h + (moving == 1 ? stroke : 0.0);
   s.box[0] = lo[0] - margin; s.box[1] = lo[1] - margin;
   s.box[2] = hi[0] + margin; s.box[3] = hi[1] + 2 * (hi[1] - lo[1]);
-  if (compat && (kernel != UPH_KERNEL_CUBIC || mode != UPH_MODE_CONFIG)) {
-    fprintf(stderr, "uph_run: --compat evaluates the cubic spline in config mode only\n");
+  if (compat && uph_derivatives_scheme(mode, kernel) != 0) {
+    fprintf(stderr, "uph_run: %s\n", uph_derivatives_last_error());
     return 1;
   }
   if (compat) {
@@ -397,7 +402,10 @@ int main(int argc, char **argv) {
           sy.localVelocity[o][d] = parts[p]->lvel[2 * i + d];
         }
         sy.density[o] = parts[p]->rho[i];
-        sy.localDensity[o] = parts[p]->rho[i];
+        /* input.c:109-110 sets localDensity = rho0 and localPressure = 0; config mode
+         * starts localDensity at the density */
+        sy.localDensity[o] = mode == UPH_MODE_REF ? rho0 : parts[p]->rho[i];
+        sy.localPressure[o] = 0.0;
         sy.DynamicPressure[o] = parts[p]->pres[i];
         sy.mass[o] = parts[p]->mass[i];
         sy.itype[o] = parts[p]->itype[i];
@@ -409,6 +417,7 @@ int main(int argc, char **argv) {
     sy.springConstant0 = k0; sy.dampingConstant0 = c0d; sy.timeStep = dt; sy.HowManyNeighbors = 40;
     sy.movingBoundary = moving; sy.stroke = stroke; sy.wavePeriod = period; sy.omega = 2.0 * M_PI / period;
     sy.pressureProbeX = v[30]; sy.pressureProbeY = v[31];
+    sy.viscosity0 = v[10]; sy.epsilon = v[13]; sy.thermalDiffusivity0 = v[27];
     if (print_step > 0) truncate_series();
     double tc = now_s();
     if (run_compat(&sy, nsteps, print_step, init_steps) != 0) return 1;

@@ -66,6 +66,7 @@ SIGNATURES = {
                                 C.POINTER(C.c_float)]),
     "uph_get_ref_rates": (C.c_int, [C.c_void_p, C.c_int] + [C.POINTER(C.c_float)] * 5),
     "uph_get_pressure": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
+    "uph_derivatives_scheme": (C.c_int, [C.c_int, C.c_int]),
     "uph_itime": (C.c_int, [C.c_void_p]),
     "uph_time_steps": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp]),
     "uph_last_error": (C.c_char_p, [C.c_void_p]),
