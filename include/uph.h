@@ -65,11 +65,17 @@ const char *uph_search_last_error(void);
 
 /* ---------------------------------------------------------- stepping API */
 
-/* Moving-boundary kinematics of itype-3 particles (inputbp.c:194-207). */
+/* Kinematics of the moving boundary particles (the n_moving rows; inputbp.c:192-283). */
 enum {
   UPH_MOTION_NONE = 0,
-  UPH_MOTION_PISTON = 1,      /* x = S/2 (1 - cos wt), vx = S w/2 sin wt (reference piston) */
-  UPH_MOTION_SINE_OFFSET = 2  /* x = x_deck + A sin wt, vx = A w cos wt */
+  UPH_MOTION_PISTON = 1,      /* movingBoundary 1: x = S/2 (1 - cos wt), vx = S w/2 sin wt, localVelocity x = vx */
+  UPH_MOTION_SINE_OFFSET = 2, /* x = x_deck + A sin wt, vx = A w cos wt (no reference counterpart) */
+  UPH_MOTION_LID = 3,         /* movingBoundary 3: x = x_deck + U t, vx = U, localVelocity x = U */
+  UPH_MOTION_GATE = 4,        /* movingBoundary 2: y = U t + gate_y for 0 < t < t_stop, then U t_stop + gate_y,
+                                 vy = U then 0; t_stop = 1.5 (depth - 15 dx) / U */
+  UPH_MOTION_SLOSH = 5        /* virtualType 5: with L = S/2 (1 - cos wt): itype 2 rows at x = L, itype 3 rows at
+                                 x = flat_bottom + L, itype 1 rows (the k-th moving row) at x = k dx/2 + L;
+                                 vx = S w/2 sin wt */
 };
 
 /* Smoothing kernel of the pair sums. */
@@ -115,6 +121,10 @@ typedef struct uph_scene {
   const double *pres;   /* n  : DynamicPressure deck column (UPH_MODE_REF; NULL: zero) */
   int mode;             /* UPH_MODE_*; the same for every scene of a batch */
   double kappa, mu, eps;      /* thermalDiffusivity0, viscosity0, epsilon (UPH_MODE_REF) */
+  double lid_u;         /* lidVelocity U (UPH_MOTION_LID, UPH_MOTION_GATE)        */
+  double depth;         /* LocalDepth (UPH_MOTION_GATE's stop time)               */
+  double flat_bottom;   /* FlatBottomLength (UPH_MOTION_SLOSH's right wall)       */
+  const double *gate_y; /* n: deck column 9, tempPositionGate (UPH_MOTION_GATE; NULL: zero) */
 } uph_scene;
 
 typedef struct uph_opts {
@@ -137,7 +147,9 @@ int uph_step(uph_t *h, int n_steps);
  * particle had more than K neighbours in range (exact capped path taken). */
 int uph_sync(uph_t *h);
 
-/* Current state of one scene in the reference's order (any pointer may be NULL). */
+/* Current state of one scene in the reference's order (any pointer may be NULL):
+ * fluid rows after the last step; moving boundary rows where the last step's
+ * derivatives(t) placed them (inputbp.c:192-283), at the deck before the first step. */
 int uph_get_state(uph_t *h, int scene, double *pos, double *vel, double *rho);
 
 /* Rates of the last step (needs opts.keep_rates): R = ddensitydt, Rt = ddensitydt_approx,
@@ -178,7 +190,8 @@ typedef struct uph_diag {
 int uph_diagnostics(uph_t *h, int scene, double probe_x, double probe_y, uph_diag *out);
 
 /* Device-side state exchange with the surrogate (no counterpart in the reference).
- * Arrays are device pointers in the reference's order (pos/vel: 2n doubles).
+ * Arrays are device pointers in the reference's order (pos/vel: 2n doubles);
+ * uph_state_to_device writes what uph_get_state returns.
  * uph_state_from_device loads the fluid rows (walls keep their state); with
  * restart != 0 the next step takes the first-step form of the leapfrog
  * (sph.cc:154-167) with localVelocity = velocity, and uph_itime restarts at 0

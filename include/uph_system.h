@@ -165,17 +165,21 @@ typedef struct uph_system {
  * scheme keeps localDensity equal to density for fluid), mass, itype, hsml[0]
  * and the constants density0, AdiabaticConstant, SpeedOfSound,
  * CompressionFactor, Gravity, gravityFlag, alpha, springConstant0,
- * dampingConstant0, dx, timeStep, HowManyNeighbors, movingBoundary, stroke,
- * omega.  Writes, for fluid rows: ddensitydt, ddensitydt_approx, dvdt,
+ * dampingConstant0, dx, timeStep, HowManyNeighbors, flagInitSimul (must be 1,
+ * the run phase), movingBoundary, virtualType, stroke, omega, lidVelocity,
+ * LocalDepth, FlatBottomLength and tempPositionGate.  Writes, for fluid rows: ddensitydt, ddensitydt_approx, dvdt,
  * dvdt_approx; Pressure and DynamicPressure = Tait(density); dpdt and
  * dpdt_approx = (dP/drho) times the two density rates; the fluctuation fields
  * (where allocated) = 0; the diagnostics of derivatives.c:4317-4324
  * (totalKineticEnergy, potentialEnergy, pointNeighbors, pointPressure,
  * pressureNorm, consistencyDensity*, densityUncertainty) of the state it was
  * called with, probe at pressureProbeX/Y.  Boundary rows are reset to the state they had at the
- * first call (the deck, as inputbp re-reads it every step) with the piston
- * kinematics of inputbp.c:194-207 at time t.  Solid, frozen and virtual
- * particles are not supported (error).
+ * first call (the deck, as inputbp re-reads it every step) with the moving rows'
+ * kinematics of inputbp.c:192-283 at time t: movingBoundary 1 (piston), 3 (lid
+ * at lidVelocity), 2 (gate rising at lidVelocity from tempPositionGate until
+ * 1.5 (LocalDepth - 15 dx) / lidVelocity), else virtualType 5 (sloshing tank:
+ * walls of itype 2, 3 and bottom of itype 1, FlatBottomLength apart).  Solid,
+ * frozen and virtual particles are not supported (error).
  *
  * The first call builds a device context for the System's sizes and
  * constants; later calls with the same System, sizes and constants reuse it

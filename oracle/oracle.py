@@ -26,7 +26,8 @@ class Params(C.Structure):
                 ("k_spring", C.c_double), ("c_damp", C.c_double), ("dt", C.c_double),
                 ("K", C.c_int), ("motion", C.c_int), ("m_amp", C.c_double),
                 ("m_omega", C.c_double), ("kernel", C.c_int), ("mode", C.c_int),
-                ("kappa", C.c_double), ("mu", C.c_double), ("eps", C.c_double)]
+                ("kappa", C.c_double), ("mu", C.c_double), ("eps", C.c_double),
+                ("lid_u", C.c_double), ("depth", C.c_double), ("flat_bottom", C.c_double)]
 
 
 def build() -> None:
@@ -65,6 +66,7 @@ def lib():
         L.orc_get_state.argtypes = [C.c_void_p] + [C.c_void_p] * 5
         L.orc_get_rates.argtypes = [C.c_void_p] + [C.c_void_p] * 5
         L.orc_get_ref.argtypes = [C.c_void_p] + [C.c_void_p] * 7
+        L.orc_set_gate_y.argtypes = [C.c_void_p, _dp]
         L.orc_set_local.argtypes = [C.c_void_p] + [C.c_void_p] * 3
         L.orc_niac.argtypes = [C.c_void_p]
         L.orc_niac.restype = C.c_int
@@ -112,7 +114,8 @@ def params_of(sc) -> Params:
                   sc.alpha_av, sc.k_spring, sc.c_damp, sc.dt, int(sc.K), int(sc.motion),
                   sc.m_amp, sc.m_omega, int(getattr(sc, "kernel", 0)), int(getattr(sc, "mode", 0)),
                   float(getattr(sc, "kappa", 0.0)), float(getattr(sc, "mu", 0.0)),
-                  float(getattr(sc, "eps", 0.0)))
+                  float(getattr(sc, "eps", 0.0)), float(getattr(sc, "lid_u", 0.0)),
+                  float(getattr(sc, "depth", 0.0)), float(getattr(sc, "flat_bottom", 0.0)))
 
 
 class Sim:
@@ -127,6 +130,9 @@ class Sim:
                       np.ascontiguousarray(sc.itype, dtype=np.int32), f(sc.lvel)]
         self.h = lib().orc_create(sc.n_fluid, sc.n_fixed, sc.n_moving, C.byref(self._p),
                                   *self._keep, int(nthreads))
+        if getattr(sc, "gate_y", None) is not None:
+            self._gate = f(sc.gate_y)
+            lib().orc_set_gate_y(self.h, self._gate)
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -50,6 +50,7 @@ typedef struct {
   int kernel;                           /* 0: cubic spline (derivatives.c:44-66), 1: PolyOrderSeven (:767-823) */
   int mode;                             /* 0: config mode, 1: the reference's own scheme (ref mode) */
   double kappa, mu, eps;                /* ref mode: thermalDiffusivity0, viscosity0, epsilon */
+  double lid_u, depth, flat_bottom;     /* lidVelocity, LocalDepth, FlatBottomLength (motions 3, 4, 5) */
 } orc_params;
 
 typedef struct {
@@ -61,6 +62,7 @@ typedef struct {
   /* ref mode: DynamicPressure_min, localPressure, dpdt, dpdt_approx, the fluctuation
    * fields, the pair kernel values w, spongeFactor */
   double *Pmin, *Ploc, *RP, *RPt, *drho, *dP, *dv, *w, *sponge;
+  double *gate_y;                       /* tempPositionGate (deck column 9) */
   double *deck_x, *deck_v, *deck_rho, *deck_P, *deck_lvel;
   int *ip, *jp;
   double *rij2, *dwdx;
@@ -340,6 +342,7 @@ orc_sim *orc_create(int nf, int nfixed, int nmoving, const orc_params *p, const 
   A(x, 2); A(v, 2); A(vmin, 2); A(vloc, 2); A(rho, 1); A(rhomin, 1); A(rholoc, 1); A(P, 1);
   A(mass, 1); A(hsml, 1); A(R, 1); A(Rt, 1); A(a, 2); A(at, 2);
   A(Pmin, 1); A(Ploc, 1); A(RP, 1); A(RPt, 1); A(drho, 1); A(dP, 1); A(dv, 2); A(sponge, 1);
+  A(gate_y, 1);
   A(deck_x, 2); A(deck_v, 2); A(deck_rho, 1); A(deck_P, 1); A(deck_lvel, 2);
 #undef A
   s->itype = calloc(n1, sizeof(int));
@@ -389,7 +392,7 @@ void orc_destroy(orc_sim *s) {
   free(s->a); free(s->at); free(s->deck_x); free(s->deck_v); free(s->deck_rho); free(s->deck_P);
   free(s->deck_lvel); free(s->itype); free(s->ip); free(s->jp); free(s->rij2); free(s->dwdx);
   free(s->tmp); free(s->Pmin); free(s->Ploc); free(s->RP); free(s->RPt); free(s->drho);
-  free(s->dP); free(s->dv); free(s->w); free(s->sponge); free(s);
+  free(s->dP); free(s->dv); free(s->w); free(s->sponge); free(s->gate_y); free(s);
 }
 
 /* Boundary reset and wall kinematics (inputbp.c:117-150, 194-207, 285-290). */
@@ -419,7 +422,49 @@ static void boundary_reset(orc_sim *s, double t) {
       s->v[2 * i] = V;
       s->vloc[2 * i] = V;
     }
+  } else if (p->motion == 3) {     /* lid (inputbp.c:209-223): tempPosition is the deck x */
+    for (int i = s->nf + s->nfixed; i < s->n; i++) {
+      s->x[2 * i] = p->lid_u * t + s->deck_x[2 * i];
+      s->v[2 * i] = p->lid_u;
+      s->vloc[2 * i] = p->lid_u;
+    }
+  } else if (p->motion == 4) {     /* gate (inputbp.c:225-243) */
+    const double stop = 1.5 * (p->depth - 15.0 * p->dx) / p->lid_u;
+    for (int i = s->nf + s->nfixed; i < s->n; i++) {
+      if (t > 0.0 && t < stop) {
+        s->x[2 * i + 1] = p->lid_u * t + s->gate_y[i];
+        s->v[2 * i + 1] = p->lid_u;
+      } else {
+        s->x[2 * i + 1] = p->lid_u * stop + s->gate_y[i];
+        s->v[2 * i + 1] = 0.0;
+      }
+    }
+  } else if (p->motion == 5) {     /* sloshing tank (inputbp.c:244-276), by deck itype */
+    double L = 0.5 * p->m_amp * (1.0 - cos(p->m_omega * t));
+    double V = 0.5 * p->m_amp * p->m_omega * sin(p->m_omega * t);
+    const double R = p->flat_bottom + L;
+    const int first = s->nf + s->nfixed;
+    for (int i = first; i < s->n; i++) {
+      if (s->itype[i] == 2) {
+        s->x[2 * i] = L;
+        s->v[2 * i] = V;
+      } else if (s->itype[i] == 3) {
+        s->x[2 * i] = R;
+        s->v[2 * i] = V;
+      } else if (s->itype[i] == 1) {
+        s->x[2 * i] = (i - first) * p->dx * 0.5 + L;
+        s->v[2 * i] = V;
+      } else {                     /* the loop zeroes the left wall's motion for later rows */
+        L = 0.0;
+        V = 0.0;
+      }
+    }
   }
+}
+
+/* tempPositionGate of every particle (deck column 9), for the gate motion. */
+void orc_set_gate_y(orc_sim *s, const double *gate_y) {
+  memcpy(s->gate_y, gate_y, sizeof(double) * s->n);
 }
 
 /* Monaghan (1992) viscous term Pi_ij, beta = 0, mean sound speed c0. */

@@ -116,9 +116,11 @@ def test_piston_wave_tank():
     drho, dx, sg, so, _ = _run_both(sc, 300, "wave tank piston 300 steps")
     assert drho < 1e-5
     assert dx < 1e-3
-    # the piston particles are where the kinematics put them (t = (itime + 1) dt)
+    # the piston particles are where the 300th derivatives() call put them
     mv = slice(sc.n_fluid + sc.n_fixed, sc.n)
     assert np.allclose(sg["x"][mv, 1], sc.pos[mv, 1])
+    assert float(np.abs(sg["x"][mv] - so["x"][mv]).max()) < 1e-12
+    assert float(np.abs(sg["v"][mv] - so["v"][mv]).max()) < 1e-12
 
 
 def test_reference_waves_deck_piston():

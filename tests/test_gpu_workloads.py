@@ -357,3 +357,54 @@ def test_c_host_driver_compat_reference_scheme(tmp_path):
     dP_o = float(np.abs(a[:, 5] - Po[:n]).max() / np.abs(Po[:n]).max())
     _say(f"compat vs oracle: rho {drho_o:.2e}, P {dP_o:.2e}")
     assert drho_o < 1e-6 and dP_o < 1e-5
+
+
+def _motion_scene(motion):
+    if motion == "lid":
+        return scenes.lid_cavity(size=0.15, dx=0.005, lid_u=1.0)
+    if motion == "gate":
+        sc = scenes.gate_dam_break(width=0.3, height=0.6, dx=0.01, gate_u=2.0)
+        sc.depth = 15 * sc.dx + 2.0 * 60.5 * sc.dt / 1.5      # the gate stops after 60 steps
+        return sc
+    return scenes.sloshing_tank(width=0.2, depth=0.06, dx=0.004, wall_height=0.12, stroke=0.01, period=0.1)
+
+
+@pytest.mark.parametrize("motion", ["lid", "gate", "slosh"])
+def test_c_host_driver_moving_boundaries(tmp_path, motion):
+    """A reference input directory with movingBoundary 3 (lid), 2 (gate, heights in
+    deck column 9) or virtualType 5 (sloshing tank): the device loop and the host loop
+    over derivatives(System*) against each other and the oracle, fluid and walls."""
+    from oracle.oracle import Sim
+    drv = os.path.join(ROOT, "uph_amd", "uph_run")
+    assert os.path.exists(drv), "uph_run not built"
+    sc = _motion_scene(motion)
+    scenes.write_deck_dir(str(tmp_path), sc, printStep=120)
+    steps = 120
+    for mode in ("device", "compat"):
+        d = tmp_path / mode
+        d.mkdir()
+        for f in ("parameters.dat", "FluidParticle.dat", "FixedBoundaryParticle.dat",
+                  "MovingBoundaryParticle.dat", "SolidParticle.dat"):
+            (d / f).write_text((tmp_path / f).read_text())
+        _say(f"uph_run {mode} {steps} steps, {motion} deck, n={sc.n}")
+        out = subprocess.run([drv, str(steps)] + (["--compat"] if mode == "compat" else []), cwd=d,
+                             capture_output=True, text=True, timeout=300)
+        assert out.returncode == 0, out.stderr
+    a = np.loadtxt(tmp_path / "device" / f"out{steps}.dat")
+    b = np.loadtxt(tmp_path / "compat" / f"out{steps}.dat")
+    back = scenes.load_deck_dir(str(tmp_path))
+    assert back.motion == sc.motion
+    nf, dx = back.n_fluid, back.dx
+    dpos = float(np.abs(a[:, 1:3] - b[:, 1:3]).max()) / dx
+    drho = float(np.max(np.abs(a[:nf, 5] - b[:nf, 5]) / b[:nf, 5]))
+    _say(f"{motion}: device vs compat |dx|/dx {dpos:.2e}, rho {drho:.2e}")
+    assert dpos < 1e-8 and drho < 1e-9
+    o = Sim(back, nthreads=0)
+    o.step(steps)
+    so = o.state()
+    drho_o = float(np.max(np.abs(b[:nf, 5] - so["rho"][:nf]) / so["rho"][:nf]))
+    dpos_o = float(np.abs(b[:nf, 1:3] - so["x"][:nf]).max()) / dx
+    # the snapshot's boundary rows are where the last derivatives() call put them
+    dwall = float(np.abs(b[nf:, 1:3] - so["x"][nf:]).max())
+    _say(f"{motion}: compat vs oracle rho {drho_o:.2e}, |dx|/dx {dpos_o:.2e}, walls {dwall:.2e} m")
+    assert drho_o < 1e-6 and dpos_o < 1e-4 and dwall < 2e-10

@@ -232,6 +232,8 @@ struct StepBufs {
   // per original index (global id)
   double2 *deckPos, *deckVel, *deckLvel;
   double* massD;   // FP64 mass per original index (diagnostics)
+  int* wallKind;   // motion kind of each boundary particle (wall_class), per original index
+  double* wallRef; // its gate height or sloshing-bottom offset
   // UPH_MODE_REF: DynamicPressure (A: at the start of the step, B: after it), its
   // deck value per original index, the predicting pressure rate and the density and
   // pressure fluctuations of the last step {dpdt_approx, drho, dP, 0}, the velocity
@@ -301,10 +303,10 @@ __global__ __launch_bounds__(BLK) void k_reorder(int n, StepBufs b, const DevSce
     const double2 l0 = b.deckLvel[my];
     lx = l0.x;
     ly = l0.y;
-    if (tag_type(tg) == 3 && S.motion != 0) {
-      double xx = p.x, vv = v.x;
-      wall_motion(S, S.t0 + itime * dt, b.deckPos[my].x, xx, vv);
-      lx = vv;
+    if (S.motion != 0) {
+      double2 xx = p, vv = v;
+      double lvx;
+      if (wall_kin(S, b.wallKind[my], b.wallRef[my], S.t0 + itime * dt, b.deckPos[my], xx, vv, &lvx)) lx = lvx;
     }
   }
   if (S.mode == 1) {
@@ -1021,7 +1023,10 @@ __device__ __forceinline__ void wall_update(const DevScene& S, const StepBufs& b
                                             float mass) {
   const uint32_t id = tag_id(tg);
   double2 x = b.deckPos[id], v = b.deckVel[id];
-  if (tag_type(tg) == 3 && S.motion != 0) wall_motion(S, S.t0 + (itime + 1) * S.dt, x.x, x.x, v.x);
+  if (S.motion != 0) {
+    double lvx;
+    (void)wall_kin(S, b.wallKind[id], b.wallRef[id], S.t0 + (itime + 1) * S.dt, b.deckPos[id], x, v, &lvx);
+  }
   b.posB[p] = x;
   b.velB[p] = v;
   b.rhoB[p] = b.rhoA[p];
@@ -1556,6 +1561,11 @@ struct uph_handle {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double* diag = nullptr;   // per-block partial sums of uph_diagnostics
   int diag_cap = 0;
+  // boundary rows' deck and kinematics (host copies, by global index), from which the
+  // state getters place them at the last step's time
+  std::vector<double2> deck_pos, deck_vel;
+  std::vector<int> wall_kind;
+  std::vector<double> wall_ref;
 };
 
 static void fill_scene(DevScene& S, const uph_scene& u, double cell_factor) {
@@ -1598,6 +1608,9 @@ static void fill_scene(DevScene& S, const uph_scene& u, double cell_factor) {
   S.kernel = u.kernel;
   S.mode = u.mode;
   S.eps = u.eps;
+  S.lid_u = u.lid_u;
+  S.t_stop = u.lid_u != 0.0 ? 1.5 * (u.depth - 15.0 * u.dx) / u.lid_u : 0.0;   // inputbp.c:227
+  S.flat_bottom = u.flat_bottom;
   S.fK0 = (float)(u.rho0 * u.c0 * u.c0);
   S.fgamma = (float)u.gamma;
   S.fgk = (float)(u.gamma * u.kappa);
@@ -1621,7 +1634,8 @@ static int alloc_bufs(StepBufs& b, int n, int ncells, int nblocks_scan, Err& err
       dalloc(&b.deckVel, n, err) || dalloc(&b.deckLvel, n, err) || dalloc(&b.massD, n, err) ||
       dalloc(&b.presA, n, err) || dalloc(&b.presB, n, err) || dalloc(&b.deckP, n, err) ||
       dalloc(&b.rt2B, n, err) || dalloc(&b.dvB, n, err) || dalloc(&b.mC, n, err) ||
-      dalloc(&b.deckRl, n, err) || dalloc(&b.deckPl, n, err))
+      dalloc(&b.deckRl, n, err) || dalloc(&b.deckPl, n, err) || dalloc(&b.wallKind, n, err) ||
+      dalloc(&b.wallRef, n, err))
     return -1;
   return 0;
 }
@@ -1630,7 +1644,8 @@ static void free_bufs(StepBufs& b) {
   void* ptrs[] = {b.posA, b.velA, b.rhoA, b.kthStep, b.tagA, b.mA, b.mB, b.posB, b.velB, b.rhoB,
                   b.rtB, b.tagB, b.keys, b.rank, b.tmpIdx, b.cellStart, b.cellCount, b.partial,
                   b.sceneOf, b.kthD2, b.kthId, b.deckPos, b.deckVel, b.deckLvel, b.massD, b.ctl,
-                  b.dbgRates, b.presA, b.presB, b.deckP, b.rt2B, b.dvB, b.mC, b.deckRl, b.deckPl};
+                  b.dbgRates, b.presA, b.presB, b.deckP, b.rt2B, b.dvB, b.mC, b.deckRl, b.deckPl,
+                  b.wallKind, b.wallRef};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   memset(&b, 0, sizeof b);
@@ -1718,6 +1733,12 @@ extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts*
       delete h;
       return -1;
     }
+    if (u.motion < UPH_MOTION_NONE || u.motion > UPH_MOTION_SLOSH ||
+        (u.motion == UPH_MOTION_GATE && !(u.lid_u != 0.0))) {
+      g_create_err = "uph_create: scene " + std::to_string(s) + " has an unknown motion (or a gate with no speed)";
+      delete h;
+      return -1;
+    }
     if ((u.mode != UPH_MODE_CONFIG && u.mode != UPH_MODE_REF) || u.mode != scenes[0].mode) {
       g_create_err = "uph_create: scene " + std::to_string(s) +
                      " has an unknown scheme (mode) or not the scheme of scene 0";
@@ -1770,13 +1791,15 @@ extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts*
   if (dalloc(&h->dsc, n_scenes, err)) return fail();
   // host staging in original order (global index = slot at start)
   std::vector<double2> pos(n), vel(n), lv(n);
-  std::vector<double> rho(n), massv(n), pres(n), rl0(n);
+  std::vector<double> rho(n), massv(n), pres(n), rl0(n), wref(n, 0.0);
+  std::vector<int> wkind(n, 0);
   std::vector<float4> rt(n);
   std::vector<uint32_t> tag(n);
   std::vector<int> sceneOf(n);
   for (int s = 0; s < n_scenes; ++s) {
     const uph_scene& u = scenes[s];
     const DevScene& S = h->hs[s];
+    int halted = 0;
     for (int i = 0; i < S.np; ++i) {
       const long g = S.poff + i;
       pos[g] = make_double2(u.pos[2 * i], u.pos[2 * i + 1]);
@@ -1788,8 +1811,13 @@ extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts*
       rt[g] = make_float4(0.f, 0.f, 0.f, (float)u.mass[i]);
       massv[g] = u.mass[i];
       int ty = u.itype[i];
+      if (i >= u.n_fluid) wkind[g] = wall_class(u.motion, i, u.n_fluid, u.n_fixed, ty, u.dx,
+                                                u.gate_y ? u.gate_y[i] : 0.0, &wref[g], &halted);
+      // walls keep their itype (2, 3, 5 take the spring force, derivatives.c:2406);
+      // a wall of itype 1 (the sloshing tank's bottom) becomes 4, a wall without it
       if (i < u.n_fluid) ty = 1;
-      else if (ty == 1 || ty < 0 || ty > 7) ty = 2;
+      else if (ty == 1) ty = 4;
+      else if (ty < 0 || ty > 7) ty = 2;
       tag[g] = ((uint32_t)ty << UPH_ID_BITS) | (uint32_t)g;
       sceneOf[g] = s;
     }
@@ -1798,21 +1826,18 @@ extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts*
     err.msg = "uph_create: upload failed";
     return fail();
   }
+  h->deck_pos = pos;
+  h->wall_kind = wkind;
+  h->wall_ref = wref;
   // boundary particles start at their step-1 positions (derivatives is called with t = dt)
   for (int s = 0; s < n_scenes; ++s) {
     const DevScene& S = h->hs[s];
     if (S.motion == 0) continue;
     for (int i = S.nf; i < S.np; ++i) {
       const long g = S.poff + i;
-      if ((tag[g] >> UPH_ID_BITS) != 3) continue;
-      const double t = 1 * S.dt;
-      if (S.motion == 1) {
-        pos[g].x = 0.5 * S.m_amp * (1.0 - cos(S.m_omega * t));
-        vel[g].x = 0.5 * S.m_amp * S.m_omega * sin(S.m_omega * t);
-      } else {
-        pos[g].x = pos[g].x + S.m_amp * sin(S.m_omega * t);
-        vel[g].x = S.m_amp * S.m_omega * cos(S.m_omega * t);
-      }
+      double lvx;
+      const double2 xd = pos[g];
+      (void)wall_kin(S, wkind[g], wref[g], 1 * S.dt, xd, pos[g], vel[g], &lvx);
     }
   }
   std::vector<double2> vdeck(n);
@@ -1820,6 +1845,7 @@ extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts*
     const uph_scene& u = scenes[s];
     for (int i = 0; i < h->hs[s].np; ++i) vdeck[h->hs[s].poff + i] = make_double2(u.vel[2 * i], u.vel[2 * i + 1]);
   }
+  h->deck_vel = vdeck;
   int ctl[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   if (hipMemcpy(h->b.deckVel, vdeck.data(), n * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(h->b.deckLvel, lv.data(), n * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess ||
@@ -1827,6 +1853,8 @@ extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts*
       hipMemcpy(h->b.presB, pres.data(), n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(h->b.deckP, pres.data(), n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(h->b.deckRl, rl0.data(), n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(h->b.wallKind, wkind.data(), n * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(h->b.wallRef, wref.data(), n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemset(h->b.deckPl, 0, n * sizeof(double)) != hipSuccess ||
       hipMemcpy(h->b.posB, pos.data(), n * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(h->b.velB, vel.data(), n * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess ||
@@ -1966,6 +1994,21 @@ extern "C" int uph_get_state(uph_t* h, int scene, double* pos, double* vel, doub
     if (pos) { pos[2 * i] = p[k].x; pos[2 * i + 1] = p[k].y; }
     if (vel) { vel[2 * i] = v[k].x; vel[2 * i + 1] = v[k].y; }
     if (rho) rho[i] = r[k];
+  }
+  // the device holds the boundary rows at the next step's time; the reference's
+  // state after step n has them where derivatives(t = n dt) put them (inputbp.c:192-283),
+  // and before the first step at the deck
+  if (S.motion != 0 && (pos || vel)) {
+    const bool at_deck = h->itime == 0 && S.t0 == 0.0;
+    const double t = S.t0 + h->itime * S.dt;
+    for (int i = S.nf; i < S.np; ++i) {
+      const long g = S.poff + i;
+      double2 x = h->deck_pos[g], w = h->deck_vel[g];
+      double lvx;
+      if (!at_deck) (void)wall_kin(S, h->wall_kind[g], h->wall_ref[g], t, h->deck_pos[g], x, w, &lvx);
+      if (pos) { pos[2 * i] = x.x; pos[2 * i + 1] = x.y; }
+      if (vel) { vel[2 * i] = w.x; vel[2 * i + 1] = w.y; }
+    }
   }
   return 0;
 }
@@ -2384,7 +2427,8 @@ __global__ void k_ref_in(int poff, int np, int nf, const uint32_t* __restrict__ 
 // Boundary slots of one scene at time t: the deck state plus the wall kinematics.
 __global__ void k_walls_at(int poff, int np, int nf, const uint32_t* __restrict__ tag, double2* __restrict__ pos,
                            double2* __restrict__ vel, const double2* __restrict__ deckPos,
-                           const double2* __restrict__ deckVel, const DevScene* __restrict__ sc, int scene, double t) {
+                           const double2* __restrict__ deckVel, const int* __restrict__ wallKind,
+                           const double* __restrict__ wallRef, const DevScene* __restrict__ sc, int scene, double t) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= np) return;
   const int s = poff + k;
@@ -2394,9 +2438,30 @@ __global__ void k_walls_at(int poff, int np, int nf, const uint32_t* __restrict_
   if (i < nf || i >= np) return;
   const DevScene& S = sc[scene];
   double2 x = deckPos[g], v = deckVel[g];
-  if (tag_type(tg) == 3 && S.motion != 0) wall_motion(S, t, x.x, x.x, v.x);
+  if (S.motion != 0) {
+    double lvx;
+    (void)wall_kin(S, wallKind[g], wallRef[g], t, deckPos[g], x, v, &lvx);
+  }
   pos[s] = x;
   vel[s] = v;
+}
+
+// Boundary rows of a state in reference order at time t, or at the deck (at_deck).
+__global__ void k_walls_out(int poff, int np, int nf, const double2* __restrict__ deckPos,
+                            const double2* __restrict__ deckVel, const int* __restrict__ wallKind,
+                            const double* __restrict__ wallRef, const DevScene* __restrict__ sc, int scene, double t,
+                            int at_deck, double2* __restrict__ pos, double2* __restrict__ vel) {
+  const int i = nf + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= np) return;
+  const int g = poff + i;
+  double2 x = deckPos[g], v = deckVel[g];
+  const DevScene& S = sc[scene];
+  if (!at_deck) {
+    double lvx;
+    (void)wall_kin(S, wallKind[g], wallRef[g], t, deckPos[g], x, v, &lvx);
+  }
+  if (pos) pos[i] = x;
+  if (vel) vel[i] = v;
 }
 
 extern "C" int uph_state_to_device(uph_t* h, int scene, double* pos, double* vel, double* rho) {
@@ -2411,6 +2476,12 @@ extern "C" int uph_state_to_device(uph_t* h, int scene, double* pos, double* vel
   hipLaunchKernelGGL(k_state_out, dim3(grid_of(S.np)), dim3(BLK), 0, h->stream, S.poff, S.np, h->b.tagA, h->b.posB,
                      h->b.velB, h->b.rhoB, (double2*)pos, (double2*)vel, rho);
   HIPCHK(hipGetLastError(), h->err);
+  if (S.motion != 0 && S.np > S.nf) {   // boundary rows at the last step's time, as uph_get_state
+    hipLaunchKernelGGL(k_walls_out, dim3(grid_of(S.np - S.nf)), dim3(BLK), 0, h->stream, S.poff, S.np, S.nf,
+                       h->b.deckPos, h->b.deckVel, h->b.wallKind, h->b.wallRef, h->dsc, scene,
+                       S.t0 + h->itime * S.dt, (int)(h->itime == 0 && S.t0 == 0.0), (double2*)pos, (double2*)vel);
+    HIPCHK(hipGetLastError(), h->err);
+  }
   HIPCHK(hipStreamSynchronize(h->stream), h->err);
   return 0;
 }
@@ -2689,6 +2760,8 @@ struct CompatCtx {
   std::vector<double> hrho;
   std::vector<float> R, Rt, at;
   std::vector<double> a;
+  std::vector<int> wkind;      // boundary rows' motion kind and reference (wall_class)
+  std::vector<double> wref;
   // scheme of the seam: config mode (default) or the reference's run-phase scheme
   int mode = UPH_MODE_CONFIG, kernel = UPH_KERNEL_CUBIC;
   bool scheme_set = false;
@@ -2724,13 +2797,14 @@ double compat_tait(const uph_system* s, double rho) {
   return s->CompressionFactor * (pow(rho / s->density0, s->AdiabaticConstant) - 1.0);
 }
 
-// Piston wall of inputbp.c:194-207 (movingBoundary 1); other modes keep the deck.
-void compat_wall(const uph_system* s, bool moving, double t, double2& x, double2& v, double2& lv) {
-  if (moving && s->movingBoundary == 1) {
-    x.x = 0.5 * s->stroke * (1.0 - cos(s->omega * t));
-    v.x = 0.5 * s->stroke * s->omega * sin(s->omega * t);
-    lv.x = v.x;
-  }
+// The motion of the System's moving rows (inputbp.c:192-283): movingBoundary 1,
+// 3, 2, else virtualType 5, else none.
+int compat_motion(const uph_system* s) {
+  if (s->movingBoundary == 1) return UPH_MOTION_PISTON;
+  if (s->movingBoundary == 3) return UPH_MOTION_LID;
+  if (s->movingBoundary == 2) return UPH_MOTION_GATE;
+  if (s->virtualType == 5) return UPH_MOTION_SLOSH;
+  return UPH_MOTION_NONE;
 }
 
 void compat_release() {
@@ -2751,7 +2825,8 @@ std::vector<double> compat_key(const uph_system* s) {
           s->AdiabaticConstant, s->SpeedOfSound, s->CompressionFactor, s->Gravity, (double)s->gravityFlag,
           s->alpha, s->springConstant0, s->dampingConstant0, (double)s->HowManyNeighbors,
           (double)s->movingBoundary, s->stroke, s->omega, (double)c.mode, (double)c.kernel,
-          s->thermalDiffusivity0, s->viscosity0, s->epsilon};
+          s->thermalDiffusivity0, s->viscosity0, s->epsilon, (double)s->virtualType, s->lidVelocity,
+          s->LocalDepth, s->FlatBottomLength};
 }
 
 int compat_build(const uph_system* s) {
@@ -2800,9 +2875,13 @@ int compat_build(const uph_system* s) {
   u.c_damp = s->dampingConstant0;
   u.dt = s->timeStep;
   u.K = s->HowManyNeighbors > 0 ? s->HowManyNeighbors : 40;
-  u.motion = s->movingBoundary == 1 ? UPH_MOTION_PISTON : UPH_MOTION_NONE;
+  u.motion = compat_motion(s);
   u.m_amp = s->stroke;
   u.m_omega = s->omega;
+  u.lid_u = s->lidVelocity;
+  u.depth = s->LocalDepth;
+  u.flat_bottom = s->FlatBottomLength;
+  u.gate_y = s->tempPositionGate;
   u.kernel = c.kernel;
   u.mode = c.mode;
   u.kappa = s->thermalDiffusivity0;
@@ -2838,6 +2917,12 @@ int compat_build(const uph_system* s) {
   c.dt = u.dt;
   c.key = compat_key(s);
   const int nb = nfix + nmov;
+  c.wkind.assign(nb, 0);
+  c.wref.assign(nb, 0.0);
+  int halted = 0;
+  for (int k = 0; k < nb; ++k)
+    c.wkind[k] = wall_class(u.motion, nf + k, nf, nfix, s->itype[nf + k], s->dx,
+                            s->tempPositionGate ? s->tempPositionGate[nf + k] : 0.0, &c.wref[k], &halted);
   c.deck_x.resize(nb);
   c.deck_v.resize(nb);
   c.deck_lv.resize(nb);
@@ -2878,6 +2963,8 @@ extern "C" int uph_derivatives(uph_system* s, double t) {
   if (!s) return compat_fail("derivatives: null System");
   if (s->NumberOfSolidParticles != 0 || s->NumberOfFrozenParticles != 0 || s->NumberOfVirtualParticles != 0)
     return compat_fail("derivatives: solid, frozen and virtual particles are not supported");
+  if (s->flagInitSimul != 1)   // derivatives.c:4197: 0 is the initialization phase
+    return compat_fail("derivatives: flagInitSimul must be 1 (the run phase; the initialization phase is not supported)");
   if (s->dim != 2) return compat_fail("derivatives: only dim = 2 is supported");
   const int nf = s->NumberOfFluidParticles, nfix = s->NumberOfFixedBoundaryParticles;
   const int nmov = s->NumberOfMovingBoundaryParticles;
@@ -2948,7 +3035,7 @@ extern "C" int uph_derivatives(uph_system* s, double t) {
                      h->b.posB, h->b.velB, h->b.rhoB, (const double2*)dpos, (const double2*)dvel,
                      (const double*)drho, (const double2*)dlv, h->b.deckLvel, 1);
   hipLaunchKernelGGL(k_walls_at, dim3(grid_of(S.np)), dim3(BLK), 0, h->stream, S.poff, S.np, S.nf, h->b.tagA,
-                     h->b.posB, h->b.velB, h->b.deckPos, h->b.deckVel, h->dsc, 0, t);
+                     h->b.posB, h->b.velB, h->b.deckPos, h->b.deckVel, h->b.wallKind, h->b.wallRef, h->dsc, 0, t);
   if (ref)
     hipLaunchKernelGGL(k_ref_in, dim3(grid_of(S.np)), dim3(BLK), 0, h->stream, S.poff, S.np, S.nf, h->b.tagA,
                        h->b.presB, h->b.deckRl, h->b.deckPl, (const double*)dP, (const double*)drl,
@@ -3014,7 +3101,8 @@ extern "C" int uph_derivatives(uph_system* s, double t) {
   for (int k = 0; k < nfix + nmov; ++k) {
     const int i = nf + k;
     double2 x = c.deck_x[k], v = c.deck_v[k], lv = c.deck_lv[k];
-    compat_wall(s, k >= nfix, t, x, v, lv);
+    double lvx;
+    if (wall_kin(h->hs[0], c.wkind[k], c.wref[k], t, c.deck_x[k], x, v, &lvx)) lv.x = lvx;
     s->Position[i][0] = x.x;
     s->Position[i][1] = x.y;
     s->Velocity[i][0] = v.x;

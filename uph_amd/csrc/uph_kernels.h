@@ -38,6 +38,8 @@ struct DevScene {
   int mode;
   double eps;               // weight of the velocity fluctuation in localVelocity
   float fK0, fgamma, fgk, fkap, fmu;  // rho0 c^2, gamma, gamma kappa, kappa, viscosity0
+  // lid / gate speed, the gate's stop time, the sloshing tank's right-wall offset
+  double lid_u, t_stop, flat_bottom;
 };
 
 __device__ __forceinline__ uint32_t tag_id(uint32_t t) { return t & UPH_ID_MASK; }
@@ -131,13 +133,71 @@ __device__ __forceinline__ double tait(const DevScene& S, double rho) {
   return S.B * (xp - 1.0);
 }
 
-// Wall kinematics at time t for a moving (itype 3) particle (inputbp.c:194-207).
-__device__ __forceinline__ void wall_motion(const DevScene& S, double t, double xdeck, double& x, double& vx) {
-  if (S.motion == 1) {
-    x = 0.5 * S.m_amp * (1.0 - cos(S.m_omega * t));
-    vx = 0.5 * S.m_amp * S.m_omega * sin(S.m_omega * t);
-  } else if (S.motion == 2) {
-    x = xdeck + S.m_amp * sin(S.m_omega * t);
-    vx = S.m_amp * S.m_omega * cos(S.m_omega * t);
+// Kinematics of a moving-set boundary particle at time t (inputbp.c:192-283):
+// kind 0 is static, 1 follows the scene's motion, 2 / 3 / 4 are the sloshing
+// tank's left wall, right wall and bottom (deck itype 2, 3, 1), and 5 / 6 / 7 the
+// same rows after a moving row of another itype, past which the reference's loop
+// has zeroed LeftBoundaryPosition and LeftBoundaryVelocity (inputbp.c:270-275;
+// RightBoundaryPosition was taken before the loop and keeps moving).  ref is the
+// particle's gate height (tempPositionGate) or its sloshing-bottom offset.  Sets
+// x and v from the deck position xd, and *lvx where the reference assigns the
+// localVelocity x component (returns true then).
+__host__ __device__ inline bool wall_kin(const DevScene& S, int kind, double ref, double t, double2 xd, double2& x,
+                                         double2& v, double* lvx) {
+  if (kind == 0) return false;
+  switch (S.motion) {
+    case 1:   // piston, absolute position (inputbp.c:194-207)
+      x.x = 0.5 * S.m_amp * (1.0 - cos(S.m_omega * t));
+      v.x = 0.5 * S.m_amp * S.m_omega * sin(S.m_omega * t);
+      *lvx = v.x;
+      return true;
+    case 2:   // sinusoidal offset of the deck position
+      x.x = xd.x + S.m_amp * sin(S.m_omega * t);
+      v.x = S.m_amp * S.m_omega * cos(S.m_omega * t);
+      *lvx = v.x;
+      return true;
+    case 3:   // lid (inputbp.c:209-223)
+      x.x = S.lid_u * t + xd.x;
+      v.x = S.lid_u;
+      *lvx = S.lid_u;
+      return true;
+    case 4: {   // gate (inputbp.c:225-243)
+      const bool on = t > 0.0 && t < S.t_stop;
+      x.y = S.lid_u * (on ? t : S.t_stop) + ref;
+      v.y = on ? S.lid_u : 0.0;
+      return false;
+    }
+    case 5: {   // sloshing tank (inputbp.c:244-276)
+      const bool halted = kind >= 5;
+      const int k = halted ? kind - 3 : kind;
+      const double Lt = 0.5 * S.m_amp * (1.0 - cos(S.m_omega * t));
+      const double L = halted ? 0.0 : Lt;
+      x.x = k == 2 ? L : (k == 3 ? S.flat_bottom + Lt : ref + L);
+      v.x = halted ? 0.0 : 0.5 * S.m_amp * S.m_omega * sin(S.m_omega * t);
+      return false;
+    }
+    default:
+      return false;
   }
+}
+
+// Motion kind and reference value of boundary row i of a scene (n_fluid fluid
+// rows, then n_fixed fixed, then n_moving moving rows), from its deck itype.
+// Called for the rows in order: *halted (0 before the first row) records that a
+// sloshing-tank moving row of an itype other than 1, 2, 3 has been passed.
+__host__ __device__ inline int wall_class(int motion, int i, int nf, int nfix, int ity, double dx, double gate_y,
+                                          double* ref, int* halted) {
+  *ref = 0.0;
+  if (motion == 0 || i < nf + nfix) return 0;
+  if (motion == 4) *ref = gate_y;
+  if (motion != 5) return 1;
+  const int h = *halted ? 3 : 0;
+  if (ity == 2) return 2 + h;
+  if (ity == 3) return 3 + h;
+  if (ity == 1) {
+    *ref = (double)(i - nf - nfix) * dx * 0.5;
+    return 4 + h;
+  }
+  *halted = 1;
+  return 0;
 }

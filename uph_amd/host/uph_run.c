@@ -42,7 +42,7 @@
 
 typedef struct {
   int n;
-  double *pos, *vel, *rho, *pres, *mass, *lvel;
+  double *pos, *vel, *rho, *pres, *mass, *lvel, *gate;   /* gate: column 9, tempPositionGate */
   int *itype;
 } deck_t;
 
@@ -65,12 +65,13 @@ static int read_deck(const char *path, deck_t *d) {
   d->rho = calloc(n, sizeof(double));
   d->pres = calloc(n, sizeof(double));
   d->mass = calloc(n, sizeof(double));
+  d->gate = calloc(n, sizeof(double));
   d->itype = calloc(n, sizeof(int));
   for (int i = 0; i < d->n; i++) {
-    double t0, t1;
+    double t0;
     int got = fscanf(f, "%lf %lf %lf %lf %lf %lf %lf %lf %lf %d %lf %lf", &d->pos[2 * i],
                      &d->pos[2 * i + 1], &d->vel[2 * i], &d->vel[2 * i + 1], &d->rho[i], &d->pres[i],
-                     &d->mass[i], &t0, &t1, &d->itype[i], &d->lvel[2 * i], &d->lvel[2 * i + 1]);
+                     &d->mass[i], &t0, &d->gate[i], &d->itype[i], &d->lvel[2 * i], &d->lvel[2 * i + 1]);
     if (got != 12) {
       fprintf(stderr, "uph_run: %s row %d: expected 12 columns\n", path, i + 1);
       fclose(f);
@@ -82,7 +83,7 @@ static int read_deck(const char *path, deck_t *d) {
 }
 
 static void free_deck(deck_t *d) {
-  free(d->pos); free(d->vel); free(d->lvel); free(d->rho); free(d->pres); free(d->mass);
+  free(d->pos); free(d->vel); free(d->lvel); free(d->rho); free(d->pres); free(d->mass); free(d->gate);
   free(d->itype);
 }
 
@@ -310,6 +311,12 @@ int main(int argc, char **argv) {
       return 1;
     }
   fclose(fp);
+  if ((int)v[25] != 1) {
+    /* flagInitSimul 0 selects the initialization phase (frozen particles, inputbp.c:75,
+       derivatives.c:4197) */
+    fprintf(stderr, "uph_run: flagInitSimul %g: only the run phase (flagInitSimul 1) is supported\n", v[25]);
+    return 1;
+  }
   /* field map of System.c:60-94 (0-based) */
   const double stroke = v[0], period = v[1], depth = v[2], rho0 = v[4], gamma = v[5], g = v[6];
   const double dx = v[7], mach = v[11], alpha_deck = v[14], dt = v[18], nn = v[20];
@@ -331,7 +338,7 @@ int main(int argc, char **argv) {
   all.pos = malloc(sizeof(double) * 2 * n); all.vel = malloc(sizeof(double) * 2 * n);
   all.lvel = malloc(sizeof(double) * 2 * n); all.rho = malloc(sizeof(double) * n);
   all.mass = malloc(sizeof(double) * n); all.itype = malloc(sizeof(int) * n);
-  all.pres = malloc(sizeof(double) * n);
+  all.pres = malloc(sizeof(double) * n); all.gate = malloc(sizeof(double) * n);
   deck_t *parts[3] = {&fl, &fx, &mv};
   int o = 0;
   double lo[2] = {1e300, 1e300}, hi[2] = {-1e300, -1e300};
@@ -346,6 +353,7 @@ int main(int argc, char **argv) {
       }
       all.rho[o] = parts[p]->rho[i];
       all.pres[o] = parts[p]->pres[i];
+      all.gate[o] = parts[p]->gate[i];
       all.mass[o] = parts[p]->mass[i];
       all.itype[o] = parts[p]->itype[i];
     }
@@ -361,7 +369,10 @@ int main(int argc, char **argv) {
   s.rho0 = rho0; s.gamma = gamma; s.c0 = c; s.B = (1. / gamma) * rho0 * c * c; s.g = g;
   s.gravity = gravity; s.alpha_av = alpha >= 0 ? alpha : alpha_deck;
   s.k_spring = k0; s.c_damp = c0d; s.dt = dt; s.K = 40;
-  s.motion = moving == 1 ? UPH_MOTION_PISTON : UPH_MOTION_NONE;
+  /* inputbp.c:192-283: movingBoundary 1, 3, 2, else virtualType 5 */
+  s.motion = moving == 1 ? UPH_MOTION_PISTON : moving == 3 ? UPH_MOTION_LID : moving == 2 ? UPH_MOTION_GATE
+           : (int)v[15] == 5 ? UPH_MOTION_SLOSH : UPH_MOTION_NONE;
+  s.lid_u = v[19]; s.depth = v[2]; s.flat_bottom = v[3]; s.gate_y = all.gate;
   s.m_amp = stroke; s.m_omega = 2.0 * M_PI / period;
   s.kernel = kernel;
   s.mode = mode;
@@ -415,7 +426,10 @@ int main(int argc, char **argv) {
     sy.density0 = rho0; sy.AdiabaticConstant = gamma; sy.Gravity = g; sy.gravityFlag = gravity;
     sy.SpeedOfSound = c; sy.CompressionFactor = s.B; sy.alpha = s.alpha_av;
     sy.springConstant0 = k0; sy.dampingConstant0 = c0d; sy.timeStep = dt; sy.HowManyNeighbors = 40;
+    sy.flagInitSimul = 1;
     sy.movingBoundary = moving; sy.stroke = stroke; sy.wavePeriod = period; sy.omega = 2.0 * M_PI / period;
+    sy.virtualType = (int)v[15]; sy.lidVelocity = v[19]; sy.LocalDepth = v[2]; sy.FlatBottomLength = v[3];
+    sy.tempPositionGate = all.gate;
     sy.pressureProbeX = v[30]; sy.pressureProbeY = v[31];
     sy.viscosity0 = v[10]; sy.epsilon = v[13]; sy.thermalDiffusivity0 = v[27];
     if (print_step > 0) truncate_series();
@@ -445,7 +459,7 @@ int main(int argc, char **argv) {
     free(sy.DynamicPressure); free(sy.DynamicPressure_min); free(sy.localPressure); free(sy.dpdt);
     free(sy.dpdt_approx); free(sy.hsml); free(sy.ddensitydt); free(sy.ddensitydt_approx);
     free(sy.densityFluctuation); free(sy.pressureFluctuation); free(sy.itype);
-    free(all.pos); free(all.vel); free(all.lvel); free(all.rho); free(all.mass); free(all.itype); free(all.pres);
+    free(all.pos); free(all.vel); free(all.lvel); free(all.rho); free(all.mass); free(all.itype); free(all.pres); free(all.gate);
     free_deck(&fl); free_deck(&fx); free_deck(&mv); free_deck(&so);
     return 0;
   }
@@ -545,7 +559,7 @@ int main(int argc, char **argv) {
   fclose(fo);
   uph_destroy(h);
   free(pos); free(vel); free(rho); free(pfin);
-  free(all.pos); free(all.vel); free(all.lvel); free(all.rho); free(all.mass); free(all.itype); free(all.pres);
+  free(all.pos); free(all.vel); free(all.lvel); free(all.rho); free(all.mass); free(all.itype); free(all.pres); free(all.gate);
   free_deck(&fl); free_deck(&fx); free_deck(&mv); free_deck(&so);
   return 0;
 }

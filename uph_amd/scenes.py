@@ -32,6 +32,9 @@ FLUID, FIXED, MOVING, SPRING_WALL = 1, 2, 3, 5
 MOTION_NONE = 0
 MOTION_PISTON_REF = 1     # x = S/2 (1 - cos wt), v = S w/2 sin wt, absolute (inputbp.c:194-207)
 MOTION_SINE_OFFSET = 2    # x = x_deck + A sin wt, v = A w cos wt
+MOTION_LID = 3            # x = x_deck + U t (movingBoundary 3, inputbp.c:209-223)
+MOTION_GATE = 4           # y = gate_y + U min(t, t_stop) (movingBoundary 2, inputbp.c:225-243)
+MOTION_SLOSH = 5          # sloshing tank walls by deck itype (virtualType 5, inputbp.c:244-276)
 
 # smoothing kernel of the pair sums
 KERNEL_CUBIC = 0          # cubic spline, Kernel (derivatives.c:44-66)
@@ -77,6 +80,10 @@ class Scene:
     kappa: float = 0.0       # thermalDiffusivity0 (ref mode)
     mu: float = 0.0          # viscosity0 (ref mode)
     eps: float = 0.0         # epsilon, weight of the velocity fluctuation (ref mode)
+    lid_u: float = 0.0       # lidVelocity (lid and gate)
+    depth: float = 0.0       # LocalDepth (the gate's stop time 1.5 (depth - 15 dx) / lid_u)
+    flat_bottom: float = 0.0  # FlatBottomLength (the sloshing tank's right wall)
+    gate_y: Optional[np.ndarray] = None   # (N,) deck column 9, tempPositionGate
     name: str = "scene"
 
     @property
@@ -101,7 +108,7 @@ class Scene:
         return dataclasses.replace(
             self, pos=self.pos.copy(), vel=self.vel.copy(), rho=self.rho.copy(),
             pressure=self.pressure.copy(), mass=self.mass.copy(), itype=self.itype.copy(),
-            lvel=self.lvel.copy())
+            lvel=self.lvel.copy(), gate_y=None if self.gate_y is None else self.gate_y.copy())
 
 
 def tait_constants(rho0: float, gamma: float, g: float, H: float):
@@ -207,6 +214,89 @@ def dam_break(width=1.0, height=2.0, dx=0.02, tank=(4.0, 3.0), layers=3, h_facto
                      name=name or f"dam_break_{width}x{height}_dx{dx:g}")
 
 
+def lid_cavity(size=0.2, dx=0.005, lid_u=1.0, layers=3, h_factor=1.3, alpha_av=0.1, cfl=0.25,
+               rho0=1000.0, gamma=7.0, name=None) -> Scene:
+    """Lid-driven cavity: a square of fluid at rest in a closed box whose top `layers`
+    rows move at lid_u (UPH_MOTION_LID, movingBoundary 3); no gravity.  The sound
+    speed is 10 lid_u."""
+    n = int(math.floor(size / dx + 1e-9))
+    fl = _lattice(n, n, dx)
+    c0 = 10.0 * lid_u
+    B = rho0 * c0 * c0 / gamma
+    walls = _tank_walls(dx, size, size, layers)
+    nl = int(round((size + 2 * (layers - 0.5) * dx) / dx)) + 1
+    lid = np.concatenate([np.stack([-(layers - 0.5) * dx + np.arange(nl) * dx,
+                                    np.full(nl, size + (l + 0.5) * dx)], axis=1) for l in range(layers)])
+    mass = rho0 * dx * dx
+    h = h_factor * dx
+    k, c = _wall_params(mass, dx, 9.81)
+    margin = (layers + 2) * dx
+    return _assemble(fl, np.full(len(fl), rho0), np.zeros(len(fl)), walls, lid, mass, dx=dx, h=h, rho0=rho0,
+                     gamma=gamma, c0=c0, B=B, g=9.81, gravity=0, alpha_av=alpha_av, k_spring=k, c_damp=c,
+                     dt=cfl * h / c0, motion=MOTION_LID, lid_u=lid_u, depth=n * dx,
+                     box=(-margin, -margin, size + margin, size + margin + 1.0),
+                     name=name or f"lid_cavity_{size}_dx{dx:g}")
+
+
+def gate_dam_break(width=0.3, height=0.6, dx=0.01, tank=(1.2, 0.9), gate_u=1.0, layers=3, h_factor=1.3,
+                   alpha_av=0.1, cfl=0.25, rho0=1000.0, gamma=7.0, g=9.81, name=None) -> Scene:
+    """Dam break released by a gate: `layers` columns of moving boundary particles
+    just right of the fluid column rise at gate_u (UPH_MOTION_GATE, movingBoundary 2)
+    until 1.5 (height - 15 dx) / gate_u, then stop; their deck column 9 (gate_y) is
+    their initial height."""
+    sc = dam_break(width=width, height=height, dx=dx, tank=tank, layers=layers, h_factor=h_factor,
+                   alpha_av=alpha_av, cfl=cfl, rho0=rho0, gamma=gamma, g=g)
+    ny = int(math.floor(height / dx + 1e-9)) + 2
+    gate = np.concatenate([np.stack([np.full(ny, width + (l + 0.5) * dx), (np.arange(ny) + 0.5) * dx], axis=1)
+                           for l in range(layers)])
+    nf, nfx = sc.n_fluid, sc.n_fixed
+    out = _assemble(sc.pos[:nf], sc.rho[:nf], sc.pressure[:nf], sc.pos[nf:nf + nfx], gate, sc.mass[0],
+                    dx=dx, h=sc.h, rho0=rho0, gamma=gamma, c0=sc.c0, B=sc.B, g=g, gravity=1,
+                    alpha_av=alpha_av, k_spring=sc.k_spring, c_damp=sc.c_damp, dt=sc.dt,
+                    motion=MOTION_GATE, lid_u=gate_u, depth=height, box=sc.box,
+                    name=name or f"gate_dam_break_{width}x{height}_dx{dx:g}")
+    out.gate_y = out.pos[:, 1].copy()
+    return out
+
+
+def sloshing_tank(width=0.4, depth=0.1, dx=0.005, wall_height=0.3, stroke=0.02, period=0.5, h_factor=1.3,
+                  alpha_av=0.1, cfl=0.25, rho0=1000.0, gamma=7.0, g=9.81, name=None) -> Scene:
+    """Sloshing tank (UPH_MOTION_SLOSH, virtualType 5): every wall moves with
+    L(t) = stroke/2 (1 - cos 2 pi t / period).  The reference places each row by its
+    itype alone -- the k-th moving row of itype 1 at x = k dx/2 + L, every row of
+    itype 2 at x = L, of itype 3 at x = FlatBottomLength + L (inputbp.c:244-276) --
+    so the moving set is one bottom row (2 nx + 1 particles at dx/2 on y = 0),
+    then one left and one right column at x = 0 and x = width (dx/2 apart from
+    y = dx/2 up to wall_height); the fluid lattice fills [0, width] x [0, depth]."""
+    nx = int(math.floor(width / dx + 1e-9))
+    ny = int(math.floor(depth / dx + 1e-9))
+    fl = _lattice(nx, ny, dx)
+    W, H = nx * dx, ny * dx
+    c0, B = tait_constants(rho0, gamma, g, H)
+    rho, p = hydrostatic_density(fl[:, 1], H, rho0, gamma, g, B)
+    nb = 2 * nx + 1
+    bottom = np.stack([np.arange(nb) * 0.5 * dx, np.zeros(nb)], axis=1)
+    nw = int(math.floor(wall_height / (0.5 * dx) + 1e-9))
+    ys = np.arange(1, nw + 1) * 0.5 * dx
+    left = np.stack([np.zeros(nw), ys], axis=1)
+    right = np.stack([np.full(nw, W), ys], axis=1)
+    moving = np.concatenate([bottom, left, right])
+    mass = rho0 * dx * dx
+    h = h_factor * dx
+    k, c = _wall_params(mass, dx, g)
+    margin = 4 * dx + stroke
+    out = _assemble(fl, rho, p, np.zeros((0, 2)), moving, mass, dx=dx, h=h, rho0=rho0, gamma=gamma,
+                    c0=c0, B=B, g=g, gravity=1, alpha_av=alpha_av, k_spring=k, c_damp=c, dt=cfl * h / c0,
+                    motion=MOTION_SLOSH, m_amp=stroke, m_omega=2.0 * math.pi / period, flat_bottom=W, depth=H,
+                    box=(-margin, -margin, W + margin, wall_height + margin),
+                    name=name or f"sloshing_{width}x{depth}_dx{dx:g}")
+    nf = out.n_fluid
+    out.itype[nf:nf + nb] = 1
+    out.itype[nf + nb:nf + nb + nw] = 2
+    out.itype[nf + nb + nw:] = 3
+    return out
+
+
 def wave_tank(length=6.0, depth=1.0, dx=0.005, beach_toe=5.0, beach_slope=0.1,
               amp=0.05, period=1.2, layers=3, h_factor=1.3, alpha_av=0.1, cfl=0.25,
               rho0=1000.0, gamma=7.0, g=9.81, wall_height=1.5, name=None) -> Scene:
@@ -308,12 +398,59 @@ def write_deck(path: str, sc: Scene, which: str) -> None:
            "fixed": slice(sc.n_fluid, sc.n_fluid + sc.n_fixed),
            "moving": slice(sc.n_fluid + sc.n_fixed, sc.n)}[which]
     idx = np.arange(sc.n)[sel]
+    gate = sc.gate_y if sc.gate_y is not None else np.zeros(sc.n)
     with open(path, "w") as f:
         f.write(f"{len(idx)}\n")
         for i in idx:
             f.write("%.10f %.10f %.10f %.10f %.10f %.10f %.10f %.10f %.10f %d %.10f %.10f\n" % (
                 sc.pos[i, 0], sc.pos[i, 1], sc.vel[i, 0], sc.vel[i, 1], sc.rho[i],
-                sc.pressure[i], sc.mass[i], 0.0, 0.0, sc.itype[i], sc.lvel[i, 0], sc.lvel[i, 1]))
+                sc.pressure[i], sc.mass[i], 0.0, gate[i], sc.itype[i], sc.lvel[i, 0], sc.lvel[i, 1]))
+
+
+PARAMETER_NAMES = ["stroke", "wavePeriod", "LocalDepth", "FlatBottomLength", "density0",
+                   "AdiabaticConstant", "Gravity", "dx", "dy", "dim", "viscosity0", "machNumber",
+                   "pecletNumber", "epsilon", "alpha", "virtualType", "printStep", "screenStep",
+                   "timeStep", "lidVelocity", "nearNeighbor", "selectBoundaryForce", "lennardSelect",
+                   "springConstant0", "dampingConstant0", "flagInitSimul", "gravityFlag",
+                   "thermalDiffusivity0", "movingBoundary", "NumberOfInitializationSteps",
+                   "pressureProbeX", "pressureProbeY", "stressType", "disspationControlVolume"]
+
+
+def scene_parameters(sc: Scene, **over) -> dict:
+    """parameters.dat values (System.c:60-94) that reproduce a Scene's constants when
+    read back by scene_from_rows: machNumber from c0 (sqrt(2 g H) / c0 for the gate,
+    System.c:292-297), nearNeighbor = 4 pi (h / dx)^2, the motion as movingBoundary
+    (1 piston, 3 lid, 2 gate) or virtualType 5 (sloshing)."""
+    H = sc.depth if sc.depth > 0 else 1.0
+    mb = {MOTION_PISTON_REF: 1, MOTION_LID: 3, MOTION_GATE: 2}.get(sc.motion, 0)
+    if sc.motion not in (MOTION_NONE, MOTION_PISTON_REF, MOTION_LID, MOTION_GATE, MOTION_SLOSH):
+        raise ValueError(f"motion {sc.motion} has no parameters.dat form")
+    mach = math.sqrt((2.0 if mb == 2 else 1.0) * sc.g * H) / sc.c0
+    prm = dict(stroke=sc.m_amp, wavePeriod=2.0 * math.pi / sc.m_omega if sc.m_omega > 0 else 1.0,
+               LocalDepth=H, FlatBottomLength=sc.flat_bottom, density0=sc.rho0, AdiabaticConstant=sc.gamma,
+               Gravity=sc.g, dx=sc.dx, dy=sc.dx, dim=2, viscosity0=sc.mu, machNumber=mach, pecletNumber=1,
+               epsilon=sc.eps, alpha=sc.alpha_av, virtualType=5 if sc.motion == MOTION_SLOSH else 0,
+               printStep=1000000, screenStep=1000000, timeStep=sc.dt, lidVelocity=sc.lid_u,
+               nearNeighbor=4.0 * math.pi * (sc.h / sc.dx) ** 2, selectBoundaryForce=3, lennardSelect=0,
+               springConstant0=sc.k_spring, dampingConstant0=sc.c_damp, flagInitSimul=1,
+               gravityFlag=sc.gravity, thermalDiffusivity0=sc.kappa, movingBoundary=mb,
+               NumberOfInitializationSteps=0, pressureProbeX=0.0, pressureProbeY=0.0, stressType=1,
+               disspationControlVolume=0)
+    prm.update(over)
+    return prm
+
+
+def write_deck_dir(d: str, sc: Scene, **over) -> None:
+    """A reference input directory for a Scene: parameters.dat (scene_parameters) and
+    the fluid, fixed, moving and (empty) solid particle decks."""
+    prm = scene_parameters(sc, **over)
+    with open(os.path.join(d, "parameters.dat"), "w") as f:
+        f.write("\n".join(repr(float(prm[k])) for k in PARAMETER_NAMES) + "\n")
+    write_deck(os.path.join(d, "FluidParticle.dat"), sc, "fluid")
+    write_deck(os.path.join(d, "FixedBoundaryParticle.dat"), sc, "fixed")
+    write_deck(os.path.join(d, "MovingBoundaryParticle.dat"), sc, "moving")
+    with open(os.path.join(d, "SolidParticle.dat"), "w") as f:
+        f.write("0\n")
 
 
 def read_deck(path: str) -> np.ndarray:
@@ -330,16 +467,24 @@ def read_deck(path: str) -> np.ndarray:
 
 def read_parameters(path: str) -> dict:
     """The 34 positional values of parameters.dat (System.c:60-94)."""
-    names = ["stroke", "wavePeriod", "LocalDepth", "FlatBottomLength", "density0",
-             "AdiabaticConstant", "Gravity", "dx", "dy", "dim", "viscosity0", "machNumber",
-             "pecletNumber", "epsilon", "alpha", "virtualType", "printStep", "screenStep",
-             "timeStep", "lidVelocity", "nearNeighbor", "selectBoundaryForce", "lennardSelect",
-             "springConstant0", "dampingConstant0", "flagInitSimul", "gravityFlag",
-             "thermalDiffusivity0", "movingBoundary", "NumberOfInitializationSteps",
-             "pressureProbeX", "pressureProbeY", "stressType", "disspationControlVolume"]
     with open(path) as f:
         vals = f.read().split()
-    return {k: float(v) for k, v in zip(names, vals)}
+    return {k: float(v) for k, v in zip(PARAMETER_NAMES, vals)}
+
+
+def deck_motion(prm: dict) -> int:
+    """The moving rows' kinematics a parameters.dat selects (inputbp.c:192-283):
+    movingBoundary 1, 3, 2, else virtualType 5."""
+    mb = int(prm["movingBoundary"])
+    if mb == 1:
+        return MOTION_PISTON_REF
+    if mb == 3:
+        return MOTION_LID
+    if mb == 2:
+        return MOTION_GATE
+    if int(prm["virtualType"]) == 5:
+        return MOTION_SLOSH
+    return MOTION_NONE
 
 
 def scene_from_rows(fluid: np.ndarray, fixed: np.ndarray, moving: np.ndarray,
@@ -366,14 +511,14 @@ def scene_from_rows(fluid: np.ndarray, fixed: np.ndarray, moving: np.ndarray,
     kw = dict(dx=dx, h=eta * dx, rho0=prm["density0"], gamma=prm["AdiabaticConstant"],
               c0=c, B=B, g=g, gravity=int(prm["gravityFlag"]), alpha_av=prm["alpha"],
               k_spring=prm["springConstant0"], c_damp=prm["dampingConstant0"],
-              dt=prm["timeStep"], motion=(MOTION_PISTON_REF if int(prm["movingBoundary"]) == 1
-                                          else MOTION_NONE),
+              dt=prm["timeStep"], motion=deck_motion(prm),
+              lid_u=prm["lidVelocity"], depth=prm["LocalDepth"], flat_bottom=prm["FlatBottomLength"],
               m_amp=prm["stroke"], m_omega=2.0 * math.pi / prm["wavePeriod"],
               kappa=prm["thermalDiffusivity0"], mu=prm["viscosity0"], eps=prm["epsilon"],
               box=(lo[0], lo[1], hi[0], hi[1]), name="deck")
     kw.update(over)
     return Scene(pos=pos, vel=rows[:, 2:4].copy(), rho=rows[:, 4].copy(),
-                 pressure=rows[:, 5].copy(), mass=rows[:, 6].copy(),
+                 pressure=rows[:, 5].copy(), mass=rows[:, 6].copy(), gate_y=rows[:, 8].copy(),
                  itype=rows[:, 9].astype(np.int32), lvel=rows[:, 10:12].copy(),
                  n_fluid=len(fluid), n_fixed=len(fixed), n_moving=len(moving), **kw)
 

@@ -32,7 +32,8 @@ class UphScene(C.Structure):
                 ("c_damp", C.c_double), ("dt", C.c_double), ("K", C.c_int), ("motion", C.c_int),
                 ("m_amp", C.c_double), ("m_omega", C.c_double), ("box", C.c_double * 4),
                 ("kernel", C.c_int), ("pres", _dp), ("mode", C.c_int), ("kappa", C.c_double),
-                ("mu", C.c_double), ("eps", C.c_double)]
+                ("mu", C.c_double), ("eps", C.c_double), ("lid_u", C.c_double), ("depth", C.c_double),
+                ("flat_bottom", C.c_double), ("gate_y", _dp)]
 
 
 class UphDiag(C.Structure):
@@ -224,6 +225,11 @@ class GpuSolver:
             u.pres = _ptr(pres)
             u.mode = int(sc.mode)
             u.kappa, u.mu, u.eps = float(sc.kappa), float(sc.mu), float(sc.eps)
+            u.lid_u, u.depth, u.flat_bottom = float(sc.lid_u), float(sc.depth), float(sc.flat_bottom)
+            if sc.gate_y is not None:
+                gy = np.ascontiguousarray(sc.gate_y, dtype=np.float64)
+                self._keep.append(gy)
+                u.gate_y = _ptr(gy)
         opts = UphOpts(int(bool(use_graph)), int(bool(keep_rates)), int(pair_variant),
                        int(pair_block), float(cell_factor))
         self.opts = dict(use_graph=bool(use_graph), pair_variant=int(pair_variant),
