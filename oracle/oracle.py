@@ -27,7 +27,8 @@ class Params(C.Structure):
                 ("K", C.c_int), ("motion", C.c_int), ("m_amp", C.c_double),
                 ("m_omega", C.c_double), ("kernel", C.c_int), ("mode", C.c_int),
                 ("kappa", C.c_double), ("mu", C.c_double), ("eps", C.c_double),
-                ("lid_u", C.c_double), ("depth", C.c_double), ("flat_bottom", C.c_double)]
+                ("lid_u", C.c_double), ("depth", C.c_double), ("flat_bottom", C.c_double),
+                ("vtype", C.c_int)]
 
 
 def build() -> None:
@@ -68,6 +69,12 @@ def lib():
         L.orc_get_ref.argtypes = [C.c_void_p] + [C.c_void_p] * 7
         L.orc_set_gate_y.argtypes = [C.c_void_p, _dp]
         L.orc_set_local.argtypes = [C.c_void_p] + [C.c_void_p] * 3
+        L.orc_get_ghosts.argtypes = [C.c_void_p] + [C.c_void_p] * 8
+        L.orc_get_ghosts.restype = C.c_int
+        L.orc_ghost_capacity.argtypes = [C.c_void_p]
+        L.orc_ghost_capacity.restype = C.c_int
+        L.orc_ghost_sections.argtypes = [C.c_int, C.c_void_p]
+        L.orc_ghost_sections.restype = C.c_int
         L.orc_niac.argtypes = [C.c_void_p]
         L.orc_niac.restype = C.c_int
         L.orc_itime.argtypes = [C.c_void_p]
@@ -115,7 +122,8 @@ def params_of(sc) -> Params:
                   sc.m_amp, sc.m_omega, int(getattr(sc, "kernel", 0)), int(getattr(sc, "mode", 0)),
                   float(getattr(sc, "kappa", 0.0)), float(getattr(sc, "mu", 0.0)),
                   float(getattr(sc, "eps", 0.0)), float(getattr(sc, "lid_u", 0.0)),
-                  float(getattr(sc, "depth", 0.0)), float(getattr(sc, "flat_bottom", 0.0)))
+                  float(getattr(sc, "depth", 0.0)), float(getattr(sc, "flat_bottom", 0.0)),
+                  int(getattr(sc, "vtype", 0)))
 
 
 class Sim:
@@ -180,6 +188,16 @@ class Sim:
         lib().orc_get_ref(self.h, *[out[k].ctypes.data for k in
                                     ("P", "Ploc", "RP", "RPt", "drho", "dP", "dv")])
         return out
+
+    def ghosts(self):
+        """Mirror rows of the last derivatives() call (derivatives.c:3193-4192): x, v,
+        vloc (ng, 2) and rho, rholoc, P, Ploc, mass (ng), in the reference's order."""
+        cap = max(lib().orc_ghost_capacity(self.h), 1)
+        out = {k: np.zeros((cap, 2)) for k in ("x", "v", "vloc")}
+        out.update({k: np.zeros(cap) for k in ("rho", "rholoc", "P", "Ploc", "mass")})
+        ng = lib().orc_get_ghosts(self.h, *[out[k].ctypes.data for k in
+                                            ("x", "v", "vloc", "rho", "rholoc", "P", "Ploc", "mass")])
+        return {k: a[:ng].copy() for k, a in out.items()}
 
     def set_local(self, rholoc=None, Ploc=None, vloc=None):
         """Sets localDensity / localPressure / localVelocity before a derivatives() call."""

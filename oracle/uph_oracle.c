@@ -19,7 +19,9 @@
  *     (absent from the reference: alpha is parsed at System.c:75 but unused)
  *   - gravity                                          derivatives.c:1167-1178, 1379-1390
  *   - spring boundary force + predicting variant       derivatives.c:2396-2449, 1392-1446
- *   - boundary reset and piston kinematics             inputbp.c:117-150, 194-207
+ *   - boundary reset and moving-set kinematics         inputbp.c:117-150, 192-283
+ *   - mirror ("virtual") particles by virtualType     derivatives.c:3193-4192
+ *     (ranked by the neighbour search, never paired)   derivatives.c:1069-1085
  *   - leapfrog predictor / corrector                   sph.cc:79-168
  * with the fluctuation fields and epsilon terms zero (they multiply fields the
  * scheme does not evolve).  The reference's own build cannot be compiled in
@@ -51,6 +53,7 @@ typedef struct {
   int mode;                             /* 0: config mode, 1: the reference's own scheme (ref mode) */
   double kappa, mu, eps;                /* ref mode: thermalDiffusivity0, viscosity0, epsilon */
   double lid_u, depth, flat_bottom;     /* lidVelocity, LocalDepth, FlatBottomLength (motions 3, 4, 5) */
+  int vtype;                            /* virtualType: mirror particles of derivatives.c:3193-4192 */
 } orc_params;
 
 typedef struct {
@@ -63,6 +66,11 @@ typedef struct {
    * fields, the pair kernel values w, spongeFactor */
   double *Pmin, *Ploc, *RP, *RPt, *drho, *dP, *dv, *w, *sponge;
   double *gate_y;                       /* tempPositionGate (deck column 9) */
+  /* mirror particles of the last derivatives() call: rows n .. n+ng-1 of x, and their
+   * Velocity, localVelocity, density, localDensity, DynamicPressure, localPressure, mass */
+  int ng, gcap;
+  double *gv, *glv, *grho, *grl, *gP, *gPl, *gmass;
+  double lbp, rbp;                      /* LeftBoundaryPosition, RightBoundaryPosition */
   double *deck_x, *deck_v, *deck_rho, *deck_P, *deck_lvel;
   int *ip, *jp;
   double *rij2, *dwdx;
@@ -72,6 +80,8 @@ typedef struct {
   int nthreads;
   double *tmp;                          /* nthreads x 2n scatter buffers */
 } orc_sim;
+
+int orc_ghost_sections(int vtype, int *sec);
 
 static int nthreads_of(int want) {
 #ifdef _OPENMP
@@ -338,8 +348,15 @@ orc_sim *orc_create(int nf, int nfixed, int nmoving, const orc_params *p, const 
   s->p = *p;
   s->nthreads = nthreads_of(nthreads);
   size_t n1 = (size_t)(n > 0 ? n : 1);
+  /* at most one mirror particle per fluid particle and section */
+  s->gcap = nf * orc_ghost_sections(p->vtype, NULL);
+  const size_t g1 = (size_t)(s->gcap > 0 ? s->gcap : 1);
+  s->x = calloc(2 * (n1 + g1), sizeof(double));
+#define G(f, c) s->f = calloc(g1 * (c), sizeof(double))
+  G(gv, 2); G(glv, 2); G(grho, 1); G(grl, 1); G(gP, 1); G(gPl, 1); G(gmass, 1);
+#undef G
 #define A(f, c) s->f = calloc(n1 * (c), sizeof(double))
-  A(x, 2); A(v, 2); A(vmin, 2); A(vloc, 2); A(rho, 1); A(rhomin, 1); A(rholoc, 1); A(P, 1);
+  A(v, 2); A(vmin, 2); A(vloc, 2); A(rho, 1); A(rhomin, 1); A(rholoc, 1); A(P, 1);
   A(mass, 1); A(hsml, 1); A(R, 1); A(Rt, 1); A(a, 2); A(at, 2);
   A(Pmin, 1); A(Ploc, 1); A(RP, 1); A(RPt, 1); A(drho, 1); A(dP, 1); A(dv, 2); A(sponge, 1);
   A(gate_y, 1);
@@ -392,7 +409,9 @@ void orc_destroy(orc_sim *s) {
   free(s->a); free(s->at); free(s->deck_x); free(s->deck_v); free(s->deck_rho); free(s->deck_P);
   free(s->deck_lvel); free(s->itype); free(s->ip); free(s->jp); free(s->rij2); free(s->dwdx);
   free(s->tmp); free(s->Pmin); free(s->Ploc); free(s->RP); free(s->RPt); free(s->drho);
-  free(s->dP); free(s->dv); free(s->w); free(s->sponge); free(s->gate_y); free(s);
+  free(s->dP); free(s->dv); free(s->w); free(s->sponge); free(s->gate_y);
+  free(s->gv); free(s->glv); free(s->grho); free(s->grl); free(s->gP); free(s->gPl); free(s->gmass);
+  free(s);
 }
 
 /* Boundary reset and wall kinematics (inputbp.c:117-150, 194-207, 285-290). */
@@ -408,6 +427,10 @@ static void boundary_reset(orc_sim *s, double t) {
     s->rholoc[i] = p->rho0;
     s->Ploc[i] = 0.0;
   }
+  /* LeftBoundaryPosition: the piston's or the sloshing tank's, 0 in inputbp's no-motion
+   * branch (inputbp.c:279-281); the lid and gate branches leave it (and
+   * RightBoundaryPosition) unset -- 0 here */
+  s->lbp = 0.0;
   if (p->motion == 1 || p->motion == 2) {
     double X, V;
     if (p->motion == 1) {          /* reference piston, absolute position */
@@ -422,6 +445,7 @@ static void boundary_reset(orc_sim *s, double t) {
       s->v[2 * i] = V;
       s->vloc[2 * i] = V;
     }
+    if (p->motion == 1) s->lbp = X;
   } else if (p->motion == 3) {     /* lid (inputbp.c:209-223): tempPosition is the deck x */
     for (int i = s->nf + s->nfixed; i < s->n; i++) {
       s->x[2 * i] = p->lid_u * t + s->deck_x[2 * i];
@@ -459,6 +483,8 @@ static void boundary_reset(orc_sim *s, double t) {
         V = 0.0;
       }
     }
+    s->lbp = L;
+    s->rbp = R;
   }
 }
 
@@ -677,6 +703,161 @@ static void ref_pressure_and_fluctuations(orc_sim *s) {
   ref_pressure(s);
 }
 
+/* ------------------------------------------------------- mirror particles */
+
+/* Sections of derivatives.c:3193-4192: each is one loop over the fluid rows in
+ * index order that appends the mirror image of every row meeting its test. */
+enum {
+  GS_LEFT0 = 1,    /* x about 0 (0.0 * FlatBottomLength), |x| < 2h                 :3474-3502 */
+  GS_LEFT,         /* x about LeftBoundaryPosition, |x - LBP| < 2h                 :3198-3226 */
+  GS_BOTTOM,       /* y about 0, y < 2h and x <= FlatBottomLength                  :3228-3257 */
+  GS_CORNER_L,     /* both, at (LBP, 0): |x - LBP| < 2h and y < 2h                 :3259-3288 */
+  GS_CORNER_R,     /* both, at (RightBoundaryPosition, 0)                          :3722-3751 */
+  GS_RIGHT_FB,     /* x about FlatBottomLength, |x - FBL| < 2h                     :3596-3624 */
+  GS_RIGHT,        /* x about RightBoundaryPosition                                :3753-3781 */
+  GS_TOP_DX,       /* y -> LocalDepth + 2 |y - LocalDepth - dx|                    :3534-3563 */
+  GS_TOP,          /* y -> LocalDepth + 2 |y - LocalDepth|                         :4095-4125 */
+  GS_SLOPE         /* reflection in y = (x - FBL)/15, distance < 3h, x > FBL       :3290-3318 */
+};
+
+/* The section list of a virtualType (sec may be NULL); returns its length.
+ * virtualType 2's sections are all commented out in the reference. */
+int orc_ghost_sections(int vtype, int *sec) {
+  static const int t1[] = {GS_LEFT, GS_BOTTOM, GS_CORNER_L, GS_SLOPE};                  /* :3195 */
+  static const int t3[] = {GS_LEFT0, GS_BOTTOM, GS_TOP_DX, GS_CORNER_L, GS_RIGHT_FB};   /* :3471 */
+  static const int t4[] = {GS_LEFT0, GS_BOTTOM, GS_CORNER_L, GS_RIGHT_FB};              /* :3783, :3907 */
+  static const int t5[] = {GS_LEFT, GS_CORNER_L, GS_CORNER_R, GS_RIGHT};                /* :3626 */
+  static const int t7[] = {GS_LEFT0, GS_BOTTOM, GS_TOP, GS_CORNER_L, GS_RIGHT_FB};      /* :4032 */
+  const int *t = NULL;
+  int m = 0;
+  switch (vtype) {
+    case 1: t = t1; m = 4; break;
+    case 3: t = t3; m = 5; break;
+    case 4: case 8: t = t4; m = 4; break;
+    case 5: t = t5; m = 4; break;
+    case 7: t = t7; m = 5; break;
+    default: break;
+  }
+  if (sec) for (int k = 0; k < m; k++) sec[k] = t[k];
+  return m;
+}
+
+/* Appends the mirror rows of the current fluid state after the n particle rows of
+ * x, in the reference's order, with the fields derivatives.c copies into them;
+ * sets s->ng (NumberOfVirtualParticles, :4192). */
+static void ghosts_generate(orc_sim *s) {
+  const orc_params *p = &s->p;
+  int sec[8];
+  const int m = orc_ghost_sections(p->vtype, sec);
+  const double slope = 1.0 / 15.0;   /* System.c:309 */
+  const double lbp = s->lbp, rbp = s->rbp, fbl = p->flat_bottom, H = p->depth;
+  int ng = 0;
+  for (int k = 0; k < m; k++) {
+    for (int i = 0; i < s->nf; i++) {
+      const double x = s->x[2 * i], y = s->x[2 * i + 1], h2 = 2.0 * s->hsml[i];
+      double gx, gy, sx = 1.0, sy = 1.0;    /* ghost position, velocity signs */
+      double vx = s->v[2 * i], vy = s->v[2 * i + 1], lx = s->vloc[2 * i], ly = s->vloc[2 * i + 1];
+      int on = 0;
+      switch (sec[k]) {
+        case GS_LEFT0: {
+          double d = fabs(x - 0.0 * fbl);
+          on = d < h2; gx = 0.0 - d; gy = y; sx = -1.0;
+          break;
+        }
+        case GS_LEFT: {
+          double d = fabs(x - lbp);
+          on = d < h2; gx = lbp - d; gy = y; sx = -1.0;
+          break;
+        }
+        case GS_BOTTOM:
+          on = (y < h2) && (x <= fbl); gx = x; gy = -y; sy = -1.0;
+          break;
+        case GS_CORNER_L: {
+          double d = fabs(x - lbp), dy = y;
+          on = (d < h2) && (dy < h2); gx = lbp - d; gy = -y; sx = sy = -1.0;
+          break;
+        }
+        case GS_CORNER_R: {
+          double d = fabs(x - rbp), dy = y;
+          on = (d < h2) && (dy < h2); gx = rbp + d; gy = -y; sx = sy = -1.0;
+          break;
+        }
+        case GS_RIGHT_FB: {
+          double d = fabs(x - fbl);
+          on = d < h2; gx = fbl + d; gy = y; sx = -1.0;
+          break;
+        }
+        case GS_RIGHT: {
+          double d = fabs(x - rbp);
+          on = d < h2; gx = rbp + d; gy = y; sx = -1.0;
+          break;
+        }
+        case GS_TOP_DX: {
+          double d = fabs(y - H - p->dx);
+          on = d < h2; gx = x; gy = H + 2 * d; sy = -1.0;
+          break;
+        }
+        case GS_TOP: {
+          double d = fabs(y - H);
+          on = d < h2; gx = x; gy = H + 2 * d; sy = -1.0;
+          break;
+        }
+        case GS_SLOPE: {
+          double denoms = 1 + slope * slope;
+          double nums = 1 - slope * slope;
+          double yIntercept = slope * fbl;
+          double d = (1.0 / sqrt(denoms)) * fabs(slope * x - y - yIntercept);
+          on = (d < 3.0 * s->hsml[i]) && (x > fbl);
+          gx = (nums * x + 2.0 * slope * y + 2.0 * slope * yIntercept) / denoms;
+          gy = (2.0 * slope * x - nums * y - 2.0 * yIntercept) / denoms;
+          if (on) {
+            const double rvx = (nums * vx + 2.0 * slope * vy) / denoms;
+            const double rvy = (2.0 * slope * vx - nums * vy) / denoms;
+            const double rlx = (nums * lx + 2.0 * slope * ly) / denoms;
+            const double rly = (2.0 * slope * lx - nums * ly) / denoms;
+            vx = rvx; vy = rvy; lx = rlx; ly = rly;
+          }
+          break;
+        }
+        default:
+          gx = gy = 0.0;
+          break;
+      }
+      if (!on) continue;
+      const int r = s->n + ng;
+      s->x[2 * r] = gx;
+      s->x[2 * r + 1] = gy;
+      s->gv[2 * ng] = sx * vx; s->gv[2 * ng + 1] = sy * vy;
+      s->glv[2 * ng] = sx * lx; s->glv[2 * ng + 1] = sy * ly;
+      s->grho[ng] = s->rho[i];
+      s->grl[ng] = s->rholoc[i];
+      s->gP[ng] = s->P[i];
+      s->gPl[ng] = s->Ploc[i];
+      s->gmass[ng] = s->mass[i];
+      ng++;
+    }
+  }
+  s->ng = ng;
+}
+
+/* The mirror rows of the last derivatives() call (any pointer may be NULL);
+ * returns their number. */
+int orc_get_ghosts(const orc_sim *s, double *pos, double *vel, double *lvel, double *rho,
+                   double *rholoc, double *P, double *Ploc, double *mass) {
+  const int ng = s->ng;
+  if (pos) memcpy(pos, s->x + 2 * (size_t)s->n, sizeof(double) * 2 * ng);
+  if (vel) memcpy(vel, s->gv, sizeof(double) * 2 * ng);
+  if (lvel) memcpy(lvel, s->glv, sizeof(double) * 2 * ng);
+  if (rho) memcpy(rho, s->grho, sizeof(double) * ng);
+  if (rholoc) memcpy(rholoc, s->grl, sizeof(double) * ng);
+  if (P) memcpy(P, s->gP, sizeof(double) * ng);
+  if (Ploc) memcpy(Ploc, s->gPl, sizeof(double) * ng);
+  if (mass) memcpy(mass, s->gmass, sizeof(double) * ng);
+  return ng;
+}
+
+int orc_ghost_capacity(const orc_sim *s) { return s->gcap; }
+
 /*
  * One call of derivatives(sys, t) in config mode (derivatives.c:3137-4315 with
  * the scheme's substitutions): rates R, Rt, a, at are recomputed from scratch.
@@ -689,10 +870,12 @@ int orc_derivatives(orc_sim *s, double t) {
   memset(s->a, 0, sizeof(double) * 2 * n);
   memset(s->at, 0, sizeof(double) * 2 * n);
   boundary_reset(s, t);
+  ghosts_generate(s);
 
-  /* CalculateKernel (derivatives.c:1063-1106) with the cubic spline */
+  /* CalculateKernel (derivatives.c:1063-1106) with the cubic spline: the search ranks
+   * the mirror rows too (:1069-1074), the pair filter drops them */
   const int pb = s->nf + s->nfixed + s->nmoving - 1;
-  int niac = orc_search(s->x, s->ip, s->jp, s->rij2, s->nf, n, pb, p->K,
+  int niac = orc_search(s->x, s->ip, s->jp, s->rij2, s->nf, n + s->ng, pb, p->K,
                         s->hsml[0] * s->hsml[0] * 4, s->cap, s->nthreads);
   if (niac < 0) return -1;
   s->niac = niac;

@@ -82,6 +82,7 @@ class Scene:
     eps: float = 0.0         # epsilon, weight of the velocity fluctuation (ref mode)
     lid_u: float = 0.0       # lidVelocity (lid and gate)
     depth: float = 0.0       # LocalDepth (the gate's stop time 1.5 (depth - 15 dx) / lid_u)
+    vtype: int = 0           # virtualType: mirror particles ranked by the search (derivatives.c:3193-4192)
     flat_bottom: float = 0.0  # FlatBottomLength (the sloshing tank's right wall)
     gate_y: Optional[np.ndarray] = None   # (N,) deck column 9, tempPositionGate
     name: str = "scene"
@@ -294,6 +295,7 @@ def sloshing_tank(width=0.4, depth=0.1, dx=0.005, wall_height=0.3, stroke=0.02, 
     out.itype[nf:nf + nb] = 1
     out.itype[nf + nb:nf + nb + nw] = 2
     out.itype[nf + nb + nw:] = 3
+    out.vtype = 5          # the sloshing deck's virtualType also selects its mirror particles
     return out
 
 
@@ -425,11 +427,14 @@ def scene_parameters(sc: Scene, **over) -> dict:
     mb = {MOTION_PISTON_REF: 1, MOTION_LID: 3, MOTION_GATE: 2}.get(sc.motion, 0)
     if sc.motion not in (MOTION_NONE, MOTION_PISTON_REF, MOTION_LID, MOTION_GATE, MOTION_SLOSH):
         raise ValueError(f"motion {sc.motion} has no parameters.dat form")
+    # virtualType 5 is the sloshing motion whenever movingBoundary is 0 (inputbp.c:244)
+    if (sc.motion == MOTION_SLOSH) != (sc.vtype == 5 and mb == 0):
+        raise ValueError(f"motion {sc.motion} with virtualType {sc.vtype} has no parameters.dat form")
     mach = math.sqrt((2.0 if mb == 2 else 1.0) * sc.g * H) / sc.c0
     prm = dict(stroke=sc.m_amp, wavePeriod=2.0 * math.pi / sc.m_omega if sc.m_omega > 0 else 1.0,
                LocalDepth=H, FlatBottomLength=sc.flat_bottom, density0=sc.rho0, AdiabaticConstant=sc.gamma,
                Gravity=sc.g, dx=sc.dx, dy=sc.dx, dim=2, viscosity0=sc.mu, machNumber=mach, pecletNumber=1,
-               epsilon=sc.eps, alpha=sc.alpha_av, virtualType=5 if sc.motion == MOTION_SLOSH else 0,
+               epsilon=sc.eps, alpha=sc.alpha_av, virtualType=sc.vtype,
                printStep=1000000, screenStep=1000000, timeStep=sc.dt, lidVelocity=sc.lid_u,
                nearNeighbor=4.0 * math.pi * (sc.h / sc.dx) ** 2, selectBoundaryForce=3, lennardSelect=0,
                springConstant0=sc.k_spring, dampingConstant0=sc.c_damp, flagInitSimul=1,
@@ -515,7 +520,7 @@ def scene_from_rows(fluid: np.ndarray, fixed: np.ndarray, moving: np.ndarray,
               lid_u=prm["lidVelocity"], depth=prm["LocalDepth"], flat_bottom=prm["FlatBottomLength"],
               m_amp=prm["stroke"], m_omega=2.0 * math.pi / prm["wavePeriod"],
               kappa=prm["thermalDiffusivity0"], mu=prm["viscosity0"], eps=prm["epsilon"],
-              box=(lo[0], lo[1], hi[0], hi[1]), name="deck")
+              vtype=int(prm["virtualType"]), box=(lo[0], lo[1], hi[0], hi[1]), name="deck")
     kw.update(over)
     return Scene(pos=pos, vel=rows[:, 2:4].copy(), rho=rows[:, 4].copy(),
                  pressure=rows[:, 5].copy(), mass=rows[:, 6].copy(), gate_y=rows[:, 8].copy(),
