@@ -125,6 +125,12 @@ typedef struct uph_scene {
   double depth;         /* LocalDepth (UPH_MOTION_GATE's stop time)               */
   double flat_bottom;   /* FlatBottomLength (UPH_MOTION_SLOSH's right wall)       */
   const double *gate_y; /* n: deck column 9, tempPositionGate (UPH_MOTION_GATE; NULL: zero) */
+  int vtype;            /* virtualType: mirror ("virtual") particles of derivatives.c:3193-4192,
+                           which the search ranks among the K nearest and never pairs;
+                           1, 3, 4, 5, 7, 8 have them, other values none */
+  double lbp, rbp;      /* LeftBoundaryPosition, RightBoundaryPosition where the motion does
+                           not set them (UPH_MOTION_LID, UPH_MOTION_GATE, and rbp without
+                           UPH_MOTION_SLOSH); the mirror sections read them */
 } uph_scene;
 
 typedef struct uph_opts {
@@ -151,6 +157,18 @@ int uph_sync(uph_t *h);
  * fluid rows after the last step; moving boundary rows where the last step's
  * derivatives(t) placed them (inputbp.c:192-283), at the deck before the first step. */
 int uph_get_state(uph_t *h, int scene, double *pos, double *vel, double *rho);
+
+/* Mirror rows of one scene from the last step's derivatives() (derivatives.c:3193-4192):
+ * the rows NumberOfVirtualParticles counts, in the reference's order (section by
+ * section, sources in index order), with the fields derivatives.c copies into them
+ * from their source at that call: Position, Velocity, localVelocity (2 per row),
+ * density, localDensity, DynamicPressure (ref mode; config mode: the Tait pressure of
+ * the density), localPressure, mass.  Returns the number of rows, or -1 when it
+ * exceeds cap (any pointer may be NULL).  uph_ghost_capacity: the most rows a step
+ * can make (one per fluid row and section). */
+int uph_get_ghosts(uph_t *h, int scene, int cap, double *pos, double *vel, double *lvel, double *rho,
+                   double *rholoc, double *P, double *Ploc, double *mass);
+int uph_ghost_capacity(uph_t *h, int scene);
 
 /* Rates of the last step (needs opts.keep_rates): R = ddensitydt, Rt = ddensitydt_approx,
  * a = dvdt, at = dvdt_approx; fluid rows only (n_fluid, n_fluid, 2 n_fluid, 2 n_fluid). */

@@ -870,6 +870,14 @@ int orc_derivatives(orc_sim *s, double t) {
   memset(s->a, 0, sizeof(double) * 2 * n);
   memset(s->at, 0, sizeof(double) * 2 * n);
   boundary_reset(s, t);
+  /* EquationOfState (derivatives.c:1674-1682); config mode feeds it to the momentum
+   * terms, ref mode integrates DynamicPressure instead (its Tait Pressure is output
+   * only).  Evaluated before the mirror rows copy the pressure of their sources. */
+  if (p->mode == 0)
+    for (int i = 0; i < n; i++) {
+      const double density = s->rho[i] / p->rho0;
+      s->P[i] = p->B * (pow(density, p->gamma) - 1.0);
+    }
   ghosts_generate(s);
 
   /* CalculateKernel (derivatives.c:1063-1106) with the cubic spline: the search ranks
@@ -893,13 +901,6 @@ int orc_derivatives(orc_sim *s, double t) {
     s->w[k] = w;
   }
 
-  /* EquationOfState (derivatives.c:1674-1682); config mode feeds it to the momentum
-   * terms, ref mode integrates DynamicPressure instead (its Tait Pressure is output only) */
-  if (p->mode == 0)
-    for (int i = 0; i < n; i++) {
-      const double density = s->rho[i] / p->rho0;
-      s->P[i] = p->B * (pow(density, p->gamma) - 1.0);
-    }
   if (p->mode == 1) ref_pressure_and_fluctuations(s);
 
   /* PredictingDensityEquation (derivatives.c:1217-1252) */

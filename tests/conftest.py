@@ -28,3 +28,18 @@ def oracle_lib():
     from oracle import oracle
     oracle.lib()
     return oracle
+
+
+def oracle_spread(sc, steps, so):
+    """Largest relative density and position (in dx) differences, after `steps`, of the
+    oracle from its own run `so` when the fluid positions start moved by up to
+    1e-12 dx: the conditioning of a trajectory whose capped neighbour sets change."""
+    from oracle.oracle import Sim
+    sp = sc.copy()
+    nf = sc.n_fluid
+    sp.pos[:nf] += np.random.default_rng(7).uniform(-1e-12, 1e-12, (nf, 2)) * sc.dx
+    o = Sim(sp, nthreads=0)
+    o.step(steps)
+    st = o.state()
+    return (float(np.max(np.abs(st["rho"][:nf] - so["rho"][:nf]) / so["rho"][:nf])),
+            float(np.abs(st["x"][:nf] - so["x"][:nf]).max()) / sc.dx)

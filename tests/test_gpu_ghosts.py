@@ -1,0 +1,201 @@
+"""Mirror ("virtual") particles on the GPU (derivatives.c:3193-4192) against the FP64
+oracle, on scenes where K = 40 binds, so that the mirror rows, ranked by the search and
+never paired (derivatives.c:1069-1085, Search.cc:33), change the neighbour sets.
+
+Tolerances: rates to 1e-5 of their largest magnitude, states to 1e-5 relative density
+and 1e-3 dx after the stated steps (as the other stepping tests).  Mirror rows of the
+last step: the same number and order; positions bit-exact (the same FP64 expressions,
+no contraction) up to the wall kinematics' cosine, velocities and densities to 1e-12
+relative; the pressure to 1e-9 (the oracle's pow against repeated squaring).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import oracle_spread
+from uph_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(f"[gpu-ghosts] {msg}", flush=True)
+
+
+def _tank(nx, ny, dx=0.01, vtype=8, h_factor=2.691, jitter=0.1, seed=0):
+    W = nx * dx
+    sc = scenes.dam_break(width=W, height=ny * dx, dx=dx, tank=(W, 2 * ny * dx), h_factor=h_factor,
+                          jitter=jitter, seed=seed)
+    sc.flat_bottom = W
+    sc.depth = ny * dx
+    sc.vtype = vtype
+    return sc
+
+
+def _rel(a, b):
+    return float(np.abs(np.asarray(a, np.float64) - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def _compare_ghosts(gg, go, name, x_atol=0.0):
+    assert len(gg["x"]) == len(go["x"]) > 0, (len(gg["x"]), len(go["x"]))
+    dx = float(np.abs(gg["x"] - go["x"]).max())
+    dv = _rel(gg["v"], go["v"])
+    drho = _rel(gg["rho"], go["rho"])
+    dP = _rel(gg["P"], go["P"]) if np.abs(go["P"]).max() > 0 else float(np.abs(gg["P"]).max())
+    _say(f"{name}: {len(go['x'])} mirror rows, |dx| {dx:.2e}, v {dv:.2e}, rho {drho:.2e}, P {dP:.2e}")
+    assert dx <= x_atol
+    assert dv < 1e-12 and _rel(gg["vloc"], go["vloc"]) < 1e-12
+    assert drho < 1e-12 and _rel(gg["rholoc"], go["rholoc"]) < 1e-12
+    assert dP < 1e-9
+    assert np.array_equal(gg["mass"], go["mass"])
+
+
+def _rates_and_ghosts(sc, name, steps):
+    from oracle.oracle import Sim
+    from uph_amd.solver import GpuSolver
+    _say(f"{name}: n={sc.n}, vtype {sc.vtype}, {steps} steps")
+    g = GpuSolver(sc, keep_rates=True)
+    g.step(steps)
+    assert g.sync() >= 1           # the capped path ran
+    rg, gg = g.rates(), g.ghosts()
+    o = Sim(sc, nthreads=0)
+    o.step(steps)
+    ro, go = o.rates(), o.ghosts()
+    nf = sc.n_fluid
+    err = {k: _rel(rg[k], ro[k][:nf]) for k in ("R", "Rt", "a", "at")}
+    _say(f"{name}: rates rel-to-max {err}")
+    return err, gg, go, rg, ro
+
+
+def test_dam_break_tank_mirror_rows_change_the_capped_sets():
+    sc = _tank(30, 24)
+    err, gg, go, rg, ro = _rates_and_ghosts(sc, "vtype 8 tank, h = 2.691 dx", 1)
+    for k, e in err.items():
+        assert e < 1e-5, (k, e)
+    _compare_ghosts(gg, go, "vtype 8 step 1")
+    # step 2: the fluid rows have moved by FP32-summed rates, their images with them
+    err, gg, go, *_ = _rates_and_ghosts(sc, "vtype 8 tank, h = 2.691 dx", 2)
+    for k, e in err.items():
+        assert e < 1e-5, (k, e)
+    assert len(gg["x"]) == len(go["x"])
+    assert float(np.abs(gg["x"] - go["x"]).max()) < 1e-9 * sc.dx
+    # without the mirror rows the GPU's rates are those of other neighbour sets
+    from uph_amd.solver import GpuSolver
+    plain = sc.copy()
+    plain.vtype = 0
+    g0 = GpuSolver(plain, keep_rates=True)
+    g0.step(2)
+    g0.sync()
+    e0 = _rel(g0.rates()["a"], ro["a"][:sc.n_fluid])
+    _say(f"vtype 0 on the same tank vs the oracle with mirror rows: a {e0:.2e}")
+    assert e0 > 1e-3
+
+
+def _run_both(sc, steps, name, conditioned=False, **gopts):
+    """GPU against oracle after `steps`; with `conditioned` (walls sweeping through
+    capped neighbour sets) to four times the oracle's own spread under a 1e-12 dx
+    perturbation of the fluid positions."""
+    from oracle.oracle import Sim
+    from uph_amd.solver import GpuSolver
+    _say(f"{name}: n={sc.n}, vtype {sc.vtype}, {steps} steps")
+    g = GpuSolver(sc, **gopts)
+    g.step(steps)
+    assert g.sync() >= 0
+    sg, gg = g.state(), g.ghosts()
+    o = Sim(sc, nthreads=0)
+    o.step(steps)
+    so, go = o.state(), o.ghosts()
+    nf = sc.n_fluid
+    drho = float(np.max(np.abs(sg["rho"][:nf] - so["rho"][:nf]) / so["rho"][:nf]))
+    dx = float(np.abs(sg["x"][:nf] - so["x"][:nf]).max()) / sc.dx
+    _say(f"{name}: max rel rho {drho:.2e}, |dx|/dx {dx:.2e}")
+    tr, tx = 1e-5, 1e-3
+    if conditioned:
+        sr, sx = oracle_spread(sc, steps, so)
+        _say(f"{name}: oracle spread under a 1e-12 dx perturbation: rho {sr:.2e}, |dx|/dx {sx:.2e}")
+        tr, tx = max(tr, 4 * sr), max(tx, 4 * sx)
+    assert drho < tr and dx < tx
+    # the mirror rows of the last step come from the fluid state of that step, which
+    # the GPU reached through FP32 sums: compare them to that tolerance
+    assert len(gg["x"]) > 0
+    if len(gg["x"]) == len(go["x"]):
+        assert float(np.abs(gg["x"] - go["x"]).max()) / sc.dx < tx
+    else:   # a row that crossed a section's reach in one run and not the other
+        assert conditioned and abs(len(gg["x"]) - len(go["x"])) <= 4
+    return g, sg, gg
+
+
+@pytest.mark.parametrize("vtype", [3, 4, 7, 8])
+def test_tank_trajectories_with_mirror_rows(vtype):
+    sc = _tank(24, 20, vtype=vtype, h_factor=2.0, seed=vtype)
+    _run_both(sc, 60, f"vtype {vtype} tank, h = 2 dx")
+
+
+def test_waves_on_a_slope_with_the_piston():
+    """virtualType 1: left images about the piston (LeftBoundaryPosition), bottom and
+    corner images, and the slope's reflections beyond FlatBottomLength.  The fluid is
+    jittered and the piston one column: on the bare lattice with stacked piston rows,
+    keys tie at the K-th rank so often that the oracle's own rates move by 10 % under
+    a 1e-14 dx perturbation; jittered, by 1e-12."""
+    sc = scenes.wave_tank(length=1.0, depth=0.2, dx=0.01, beach_toe=0.6, amp=0.03, period=0.4, h_factor=2.0,
+                          layers=1, jitter=0.1, seed=3)
+    sc.motion = scenes.MOTION_PISTON_REF
+    sc.m_amp, sc.m_omega = 0.03, 2 * math.pi / 0.4
+    sc.vtype = 1
+    sc.flat_bottom = 0.6
+    err, gg, go, *_ = _rates_and_ghosts(sc, "vtype 1 wave tank, rates of step 2", 2)
+    for k, e in err.items():
+        assert e < 1e-5, (k, e)
+    _run_both(sc, 60, "vtype 1 wave tank with piston and slope, h = 2 dx", conditioned=True)
+
+
+def test_sloshing_tank_with_its_mirror_rows_where_k_binds():
+    sc = scenes.sloshing_tank(width=0.2, depth=0.06, dx=0.004, wall_height=0.12, stroke=0.01, period=0.1,
+                              h_factor=2.0, jitter=0.1, seed=3)
+    assert sc.vtype == 5
+    err, gg, go, *_ = _rates_and_ghosts(sc, "vtype 5 sloshing tank, rates of step 2", 2)
+    for k, e in err.items():
+        assert e < 1e-5, (k, e)
+    _run_both(sc, 80, "vtype 5 sloshing tank, h = 2 dx", conditioned=True)
+
+
+def test_mirror_rows_of_one_step_match_exactly():
+    """One derivatives() call from the deck: the fluid state is the deck's on both, so
+    the mirror rows agree bit for bit in position."""
+    sc = _tank(20, 16, vtype=7, h_factor=2.0, seed=4)
+    _, gg, go, _, _ = _rates_and_ghosts(sc, "vtype 7 tank, one step", 1)
+    _compare_ghosts(gg, go, "vtype 7 step 1")
+
+
+def test_batch_with_and_without_mirror_rows_equals_single_scenes():
+    from uph_amd.solver import GpuSolver
+    a = _tank(16, 12, vtype=8, h_factor=2.0, seed=1)
+    b = scenes.dam_break(width=0.3, height=0.4, dx=0.02)
+    c = _tank(12, 14, vtype=7, h_factor=2.0, seed=2)
+    batch = GpuSolver([a, b, c])
+    batch.step(25)
+    for k, sc in enumerate((a, b, c)):
+        one = GpuSolver(sc)
+        one.step(25)
+        sb, so = batch.state(k), one.state()
+        for key in ("x", "v", "rho"):
+            assert np.array_equal(sb[key], so[key]), (k, key)
+        gb, go = batch.ghosts(k), one.ghosts()
+        assert all(np.array_equal(gb[key], go[key]) for key in gb), k
+
+
+def test_graph_replay_with_mirror_rows():
+    """The mirror kernels run inside the captured step: replays equal eager steps."""
+    from uph_amd.solver import GpuSolver
+    sc = _tank(20, 16, vtype=8, h_factor=2.0, seed=6)
+    eager = GpuSolver(sc)
+    graph = GpuSolver(sc, use_graph=True)
+    for _ in range(3):
+        eager.step(10)
+        graph.step(10)
+        se, sg = eager.state(), graph.state()
+        for key in ("x", "v", "rho"):
+            assert np.array_equal(se[key], sg[key]), key
+        ge, gg = eager.ghosts(), graph.ghosts()
+        assert all(np.array_equal(ge[key], gg[key]) for key in ge)

@@ -3,13 +3,18 @@ the lid-driven cavity, the dam break released by a gate, the sloshing tank (also
 a moving row of another itype, past which the reference's loop stops the left wall).
 
 Fluid states to the stepping tolerance of the other GPU tests (density 1e-5 relative,
-positions 1e-3 dx).  Boundary rows: the state after n steps has them where the n-th
+positions 1e-3 dx), or, where walls sweep through neighbour sets that K = 40 caps
+(the sloshing tank and its mirror rows), to four times the oracle's own spread
+under a 1e-12 dx perturbation of the fluid positions: a neighbour that enters the
+capped set one step earlier changes the rates by a kernel weight, so there the
+trajectories are only as close as the problem's conditioning.  Boundary rows: the state after n steps has them where the n-th
 derivatives() call put them, at n dt, in the oracle and through uph_get_state and
 uph_state_to_device; the kinematics are FP64 on both, so they agree to rounding.
 """
 import numpy as np
 import pytest
 
+from conftest import oracle_spread
 from uph_amd import scenes
 
 pytestmark = pytest.mark.gpu
@@ -19,7 +24,7 @@ def _say(msg):
     print(f"[gpu-motion] {msg}", flush=True)
 
 
-def _run(sc, steps, name):
+def _run(sc, steps, name, conditioned=False):
     from oracle.oracle import Sim
     from uph_amd.solver import GpuSolver
     _say(f"{name}: n={sc.n} steps={steps} motion={sc.motion}")
@@ -36,7 +41,12 @@ def _run(sc, steps, name):
     bx = float(np.abs(sg["x"][nf:] - walls["x"][nf:]).max())
     bv = float(np.abs(sg["v"][nf:] - walls["v"][nf:]).max())
     _say(f"{name}: fluid rho {drho:.2e}, |dx|/dx {dx:.2e}; walls |dx| {bx:.2e} m, |dv| {bv:.2e} m/s")
-    assert drho < 1e-5 and dx < 1e-3
+    tr, tx = 1e-5, 1e-3
+    if conditioned:
+        sr, sx = oracle_spread(sc, steps, so)
+        _say(f"{name}: oracle spread under a 1e-12 dx perturbation: rho {sr:.2e}, |dx|/dx {sx:.2e}")
+        tr, tx = max(tr, 4 * sr), max(tx, 4 * sx)
+    assert drho < tr and dx < tx
     assert bx < 1e-12 and bv < 1e-12
     # the device-side getter places them the same way
     from uph_amd.devbuf import DevBuf
@@ -68,7 +78,7 @@ def test_gate_dam_break_through_its_stop():
 
 def test_sloshing_tank():
     sc = scenes.sloshing_tank(width=0.2, depth=0.06, dx=0.004, wall_height=0.12, stroke=0.01, period=0.1)
-    _run(sc, 300, "sloshing tank 0.2 x 0.06, dx 4 mm, 300 steps")
+    _run(sc, 300, "sloshing tank 0.2 x 0.06, dx 4 mm, 300 steps", conditioned=True)
 
 
 def test_sloshing_tank_with_a_row_of_another_itype():
@@ -77,7 +87,7 @@ def test_sloshing_tank_with_a_row_of_another_itype():
     first = sc.n_fluid + sc.n_fixed + int((it[sc.n_fluid + sc.n_fixed:] == 1).sum())
     sc.itype = it.copy()
     sc.itype[first:first + 3] = 4
-    sg, _ = _run(sc, 100, "sloshing tank, itype-4 rows after the bottom, 100 steps")
+    sg, _ = _run(sc, 100, "sloshing tank, itype-4 rows after the bottom, 100 steps", conditioned=True)
     left = np.nonzero(sc.itype == 2)[0]
     assert np.all(sg["x"][left, 0] == 0.0) and np.all(sg["v"][left, 0] == 0.0)
 

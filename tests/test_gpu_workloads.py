@@ -401,10 +401,59 @@ def test_c_host_driver_moving_boundaries(tmp_path, motion):
     assert dpos < 1e-8 and drho < 1e-9
     o = Sim(back, nthreads=0)
     o.step(steps)
-    so = o.state()
+    so, go = o.state(), o.ghosts()
+    n = back.n
+    # the sloshing deck's virtualType 5 lists mirror rows after the particle rows
+    assert b.shape[0] == n + len(go["x"])
     drho_o = float(np.max(np.abs(b[:nf, 5] - so["rho"][:nf]) / so["rho"][:nf]))
     dpos_o = float(np.abs(b[:nf, 1:3] - so["x"][:nf]).max()) / dx
     # the snapshot's boundary rows are where the last derivatives() call put them
-    dwall = float(np.abs(b[nf:, 1:3] - so["x"][nf:]).max())
-    _say(f"{motion}: compat vs oracle rho {drho_o:.2e}, |dx|/dx {dpos_o:.2e}, walls {dwall:.2e} m")
+    dwall = float(np.abs(b[nf:n, 1:3] - so["x"][nf:]).max())
+    _say(f"{motion}: compat vs oracle rho {drho_o:.2e}, |dx|/dx {dpos_o:.2e}, walls {dwall:.2e} m, "
+         f"{len(go['x'])} mirror rows")
     assert drho_o < 1e-6 and dpos_o < 1e-4 and dwall < 2e-10
+    if len(go["x"]):
+        assert float(np.abs(b[n:, 1:3] - go["x"]).max()) / dx < 1e-4
+
+
+def test_c_host_driver_mirror_particles(tmp_path):
+    """A deck with virtualType 8 where K binds (h = 2.691 dx): both driver loops rank the
+    mirror rows in the search and list them after the particle rows of out<itime>.dat
+    (output.c:28, itype 4); against each other and the oracle."""
+    from oracle.oracle import Sim
+    drv = os.path.join(ROOT, "uph_amd", "uph_run")
+    W, H, dx = 0.24, 0.2, 0.01
+    sc = scenes.dam_break(width=W, height=H, dx=dx, tank=(W, 2 * H), h_factor=2.691, jitter=0.1, seed=3)
+    sc.flat_bottom, sc.depth, sc.vtype = W, H, 8
+    steps = 40
+    scenes.write_deck_dir(str(tmp_path), sc, printStep=steps)
+    for mode in ("device", "compat"):
+        d = tmp_path / mode
+        d.mkdir()
+        for f in ("parameters.dat", "FluidParticle.dat", "FixedBoundaryParticle.dat",
+                  "MovingBoundaryParticle.dat", "SolidParticle.dat"):
+            (d / f).write_text((tmp_path / f).read_text())
+        _say(f"uph_run {mode} {steps} steps, virtualType 8 deck, n={sc.n}")
+        out = subprocess.run([drv, str(steps)] + (["--compat"] if mode == "compat" else []), cwd=d,
+                             capture_output=True, text=True, timeout=300)
+        assert out.returncode == 0, out.stderr
+    a = np.loadtxt(tmp_path / "device" / f"out{steps}.dat")
+    b = np.loadtxt(tmp_path / "compat" / f"out{steps}.dat")
+    back = scenes.load_deck_dir(str(tmp_path))
+    assert back.vtype == 8
+    o = Sim(back, nthreads=0)
+    o.step(steps)
+    so, go = o.state(), o.ghosts()
+    n, nf = back.n, back.n_fluid
+    ng = len(go["x"])
+    assert ng > 0 and a.shape[0] == b.shape[0] == n + ng, (a.shape, b.shape, n, ng)
+    assert np.all(a[n:, 11] == 4) and np.all(b[n:, 11] == 4)
+    dpos = float(np.abs(a[:, 1:3] - b[:, 1:3]).max()) / dx
+    drho = float(np.max(np.abs(a[:, 5] - b[:, 5]) / b[:, 5]))
+    _say(f"device vs compat, particle and mirror rows: |dx|/dx {dpos:.2e}, rho {drho:.2e}")
+    assert dpos < 1e-8 and drho < 1e-9
+    drho_o = float(np.max(np.abs(b[:nf, 5] - so["rho"][:nf]) / so["rho"][:nf]))
+    dpos_o = float(np.abs(b[:nf, 1:3] - so["x"][:nf]).max()) / dx
+    dg_o = float(np.abs(b[n:, 1:3] - go["x"]).max()) / dx
+    _say(f"compat vs oracle: rho {drho_o:.2e}, |dx|/dx {dpos_o:.2e}, mirror rows |dx|/dx {dg_o:.2e}")
+    assert drho_o < 1e-6 and dpos_o < 1e-4 and dg_o < 1e-4

@@ -247,6 +247,24 @@ struct StepBufs {
   int* ctl;
   float4* dbgRates;   // keep_rates: per slot {R, R~, a~x, a~y} (FP32), {ax, ay} (FP64);
                       // UPH_MODE_REF adds {dpdt, dpdt_approx, drho, dP}, {dvx, dvy, 0, 0}
+  // mirror ("virtual") particles, allocated when some scene has them (gtot > 0):
+  // the section test of every (scene, section, fluid row), its exclusive scan (the
+  // mirror index of each passing test; gstart[gtot] is the step's total) and the scan's
+  // partials; per mirror index its position, key id, scene, cell and in-cell rank;
+  // the mirror rows sorted by cell (gcellStart per cell, ncells + 1); the fields
+  // derivatives.c copies into a mirror row; the predicted fields of fluid rows of
+  // scenes with mirrors, per A slot, from which they are copied
+  int gtot;
+  int *gflag, *gstart, *gpart;
+  double2* gpos;
+  uint32_t* gid;
+  int *gscene, *gkey, *grank, *gcellCount, *gcellStart;
+  double2* gposS;
+  uint32_t* gidS;
+  double2 *gvel, *glvel;
+  double *grho, *grl, *gP, *gPl, *gmass;
+  double2 *predV, *predVl;
+  double *predRho, *predRl, *predP, *predPl;
 };
 
 // Final slot = cell start + number of same-cell particles with a smaller
@@ -333,12 +351,113 @@ __global__ __launch_bounds__(BLK) void k_reorder(int n, StepBufs b, const DevSce
     b.mA[dst] = make_float4((float)vx, (float)vy, (float)lx, (float)ly);
     b.mB[dst] = make_float4((float)(rp - S.rho0), (float)Pp, rt.w, __uint_as_float(tg));
     b.mC[dst] = make_float2((float)(rl - S.rho0), (float)pl);
+    if (S.nsec > 0 && tag_type(tg) == 1) {   // the fields this row's mirror rows copy
+      b.predV[dst] = make_double2(vx, vy);
+      b.predVl[dst] = make_double2(lx, ly);
+      b.predRho[dst] = rp;
+      b.predRl[dst] = rl;
+      b.predP[dst] = Pp;
+      b.predPl[dst] = pl;
+    }
     return;
   }
   const double P = tait(S, rp);
   const double pr = P / (rp * rp);
   b.mA[dst] = make_float4((float)vx, (float)vy, (float)lx, (float)ly);
   b.mB[dst] = make_float4((float)rp, (float)pr, rt.w, __uint_as_float(tg));
+  if (S.nsec > 0 && tag_type(tg) == 1) {
+    b.predV[dst] = make_double2(vx, vy);
+    b.predVl[dst] = make_double2(lx, ly);
+    b.predRho[dst] = rp;
+    b.predRl[dst] = rp;   // config mode keeps localDensity equal to density
+    b.predP[dst] = P;
+    b.predPl[dst] = 0.0;
+  }
+}
+
+// ------------------------------------------------------ mirror particles
+//
+// derivatives.c:3193-4192 appends, after the particle rows, the mirror image of
+// every fluid row that passes a section's test, section by section and in index
+// order; the search ranks these rows among the K nearest (derivatives.c:1069-1085)
+// and the pair filter drops them (Search.cc:33).  Here they are rebuilt each step
+// after the sort: a test per (section, fluid row), a scan that gives each passing
+// test its mirror index in the reference's order, the rows written by index, and a
+// cell list of their own, which the pair kernels read only to count candidates
+// in range and to rank keys.
+
+// Section tests of the fluid rows at their positions of this step.
+__global__ __launch_bounds__(BLK) void k_ghost_flags(int n, StepBufs b, const DevScene* __restrict__ sc) {
+  const int d = blockIdx.x * BLK + threadIdx.x;
+  if (d >= n) return;
+  const uint32_t tg = b.tagA[d];
+  if (tag_type(tg) != 1) return;
+  const DevScene& S = sc[b.sceneOf[d]];
+  if (S.nsec == 0) return;
+  const int i = (int)tag_id(tg) - S.poff;
+  double lbp, rbp;
+  boundary_lines(S, S.t0 + b.ctl[0] * S.dt, lbp, rbp);
+  const double2 x = b.posA[d];
+  for (int k = 0; k < S.nsec; ++k) {
+    double2 g, v = make_double2(0.0, 0.0), lv = v;
+    b.gflag[S.goff + k * S.nf + i] =
+        mirror_row(S.sec[k], lbp, rbp, S.flat_bottom, S.depth, S.dx, S.h, x, g, v, lv) ? 1 : 0;
+  }
+}
+
+// The mirror rows of the passing tests, at their scanned indices.
+__global__ __launch_bounds__(BLK) void k_ghost_emit(int n, StepBufs b, const DevScene* __restrict__ sc) {
+  const int d = blockIdx.x * BLK + threadIdx.x;
+  if (d >= n) return;
+  const uint32_t tg = b.tagA[d];
+  if (tag_type(tg) != 1) return;
+  const int s = b.sceneOf[d];
+  const DevScene& S = sc[s];
+  if (S.nsec == 0) return;
+  const int i = (int)tag_id(tg) - S.poff;
+  double lbp, rbp;
+  boundary_lines(S, S.t0 + b.ctl[0] * S.dt, lbp, rbp);
+  const double2 x = b.posA[d];
+  const int first = b.gstart[S.goff];
+  for (int k = 0; k < S.nsec; ++k) {
+    const int e = S.goff + k * S.nf + i;
+    const int j = b.gstart[e];
+    if (b.gstart[e + 1] == j) continue;
+    double2 g = x, v = b.predV[d], lv = b.predVl[d];
+    (void)mirror_row(S.sec[k], lbp, rbp, S.flat_bottom, S.depth, S.dx, S.h, x, g, v, lv);
+    b.gpos[j] = g;
+    b.gid[j] = (uint32_t)(S.gbase + (j - first));
+    b.gscene[j] = s;
+    b.gvel[j] = v;
+    b.glvel[j] = lv;
+    b.grho[j] = b.predRho[d];
+    b.grl[j] = b.predRl[d];
+    b.gP[j] = b.predP[d];
+    b.gPl[j] = b.predPl[d];
+    b.gmass[j] = b.massD[tag_id(tg)];
+  }
+}
+
+// Cell and in-cell rank of each mirror row (order within a cell is free: the rows
+// only enter counts and key ranks).
+__global__ __launch_bounds__(BLK) void k_ghost_bin(StepBufs b, const DevScene* __restrict__ sc) {
+  const int j = blockIdx.x * BLK + threadIdx.x;
+  if (j >= b.gtot || j >= b.gstart[b.gtot]) return;
+  const DevScene& S = sc[b.gscene[j]];
+  const double2 g = b.gpos[j];
+  int cx, cy;
+  cell_of(S, g.x, g.y, cx, cy);
+  const int key = S.coff + cy * S.ncx + cx;
+  b.gkey[j] = key;
+  b.grank[j] = atomicAdd(&b.gcellCount[key], 1);
+}
+
+__global__ __launch_bounds__(BLK) void k_ghost_scatter(StepBufs b) {
+  const int j = blockIdx.x * BLK + threadIdx.x;
+  if (j >= b.gtot || j >= b.gstart[b.gtot]) return;
+  const int t = b.gcellStart[b.gkey[j]] + b.grank[j];
+  b.gposS[t] = b.gpos[j];
+  b.gidS[t] = b.gid[j];
 }
 
 // ------------------------------------------------------------- pair kernel
@@ -357,13 +476,18 @@ __device__ __forceinline__ int xcd_block() {
 // it exceeds K, the K-th smallest (d2, id) key: the neighbour set of
 // Search.cc:27-40 is then the in-range candidates with key <= that key.
 __device__ void kth_key(const DevScene& S, int p, const double2* __restrict__ pos, const uint32_t* __restrict__ tag,
-                        const int* __restrict__ cellStart, int& cnt, double& kd2, uint32_t& kid) {
+                        const int* __restrict__ cellStart, int& cnt, double& kd2, uint32_t& kid,
+                        const StepBufs* gb = nullptr) {
   const double2 pp = pos[p];
   const double cut = S.cutoff;
   int c = 0;
   for_candidates(S, pp, cellStart, [&](int q) {
     if (q != p && dist2(pp, pos[q]) < cut) ++c;
   });
+  if (gb)   // mirror rows rank too
+    for_candidates(S, pp, gb->gcellStart, [&](int t) {
+      if (dist2(pp, gb->gposS[t]) < cut) ++c;
+    });
   cnt = c;
   kd2 = INFINITY;
   kid = 0xffffffffu;
@@ -374,17 +498,23 @@ __device__ void kth_key(const DevScene& S, int p, const double2* __restrict__ po
   for (int k = 0; k < S.K; ++k) {
     double bd = INFINITY;
     uint32_t bi = 0xffffffffu;
-    for_candidates(S, pp, cellStart, [&](int q) {
-      if (q == p) return;
-      const double d2 = dist2(pp, pos[q]);
-      if (!(d2 < cut)) return;
-      const uint32_t id = tag_id(tag[q]);
+    auto take = [&](double d2, uint32_t id) {
       const bool above = d2 > td || (d2 == td && id > ti);
       if (above && (d2 < bd || (d2 == bd && id < bi))) {
         bd = d2;
         bi = id;
       }
+    };
+    for_candidates(S, pp, cellStart, [&](int q) {
+      if (q == p) return;
+      const double d2 = dist2(pp, pos[q]);
+      if (d2 < cut) take(d2, tag_id(tag[q]));
     });
+    if (gb)
+      for_candidates(S, pp, gb->gcellStart, [&](int t) {
+        const double d2 = dist2(pp, gb->gposS[t]);
+        if (d2 < cut) take(d2, gb->gidS[t]);
+      });
     td = bd;
     ti = bi;
   }
@@ -505,6 +635,53 @@ __device__ int wave_gather(const DevScene& S, int q, const double2* __restrict__
   return wave_list(S, q, xq, rb, rl, pos, tag, L);
 }
 
+// Mirror rows in range (d2 < cutoff) of a query at x; CAND_MAX + 1 when its cells
+// hold more than CAND_MAX of them.
+__device__ int ghost_count(const DevScene& S, double2 x, const StepBufs& b) {
+  int rb[3], rl[3];
+  candidate_rows(S, x, b.gcellStart, rb, rl);
+  if (rl[0] + rl[1] + rl[2] > CAND_MAX) return CAND_MAX + 1;
+  int c = 0;
+#pragma unroll 1
+  for (int r = 0; r < 3; ++r)
+#pragma unroll 1
+    for (int t = rb[r]; t < rb[r] + rl[r]; ++t) c += dist2(x, b.gposS[t]) < S.cutoff ? 1 : 0;
+  return c;
+}
+
+// Appends the mirror rows in range of a query at xq (the same on every lane) to the
+// wavefront's list L after its first n entries: the new count (entries past KTH_CAP
+// counted, not listed), or -1 beyond CAND_MAX candidates.
+__device__ int wave_list_ghosts(const DevScene& S, double2 xq, int n, const StepBufs& b, KthLds& L) {
+  const int lane = threadIdx.x & 63;
+  int rb[3], rl[3];
+  candidate_rows(S, xq, b.gcellStart, rb, rl);
+  const int total = rl[0] + rl[1] + rl[2];
+  if (total > CAND_MAX) return -1;
+#pragma unroll 1
+  for (int c0 = 0; c0 < total; c0 += 64) {
+    const int c = c0 + lane;
+    bool in = false;
+    double d2 = 0.0;
+    uint32_t id = 0u;
+    if (c < total) {
+      const int t = c < rl[0] ? rb[0] + c : c < rl[0] + rl[1] ? rb[1] + (c - rl[0]) : rb[2] + (c - rl[0] - rl[1]);
+      d2 = dist2(xq, b.gposS[t]);
+      in = d2 < S.cutoff;
+      id = b.gidS[t];
+    }
+    const unsigned long long m = __ballot(in);
+    const int at = n + lanes_below(m);
+    if (in && at < KTH_CAP) {
+      L.d2[at] = d2;
+      L.id[at] = id;
+    }
+    n += __popcll(m);
+  }
+  wave_lds_sync();
+  return n;
+}
+
 // 1 if (d2, id) < (e2, jd), else 0, for d2, e2 >= 0 given by their bit patterns
 // (which order as integers): the borrow out of the 96-bit difference.
 __device__ __forceinline__ int key_lt_bits(uint64_t d2, uint32_t id, uint64_t e2, uint32_t jd) {
@@ -604,7 +781,8 @@ __device__ void wave_kth_listed(const KthLds& L, int n, int K, uint32_t seed, do
 // +inf, unless `exact`, when one lane searches the candidates alone.
 __device__ void wave_kth_of_list(const DevScene& S, int q, const double2* __restrict__ pos,
                                  const uint32_t* __restrict__ tag, const int* __restrict__ cellStart, KthLds& L,
-                                 int n, double& kd2, uint32_t& kid, bool exact = false) {
+                                 int n, double& kd2, uint32_t& kid, bool exact = false,
+                                 const StepBufs* gb = nullptr) {
   kd2 = INFINITY;
   kid = 0xffffffffu;
   if (n < 0 && !exact) return;
@@ -620,7 +798,7 @@ __device__ void wave_kth_of_list(const DevScene& S, int q, const double2* __rest
   // more in range than the list holds: one lane searches alone
   if ((threadIdx.x & 63) == 0) {
     int c;
-    kth_key(S, q, pos, tag, cellStart, c, kd2, kid);
+    kth_key(S, q, pos, tag, cellStart, c, kd2, kid, gb);
   }
   kd2 = bcastd(kd2, 0);
   kid = (uint32_t)bcast((int)kid, 0);
@@ -667,10 +845,14 @@ __device__ void wave_overflow(const StepBufs& b, const DevScene* __restrict__ sc
         rl[r] = bcast(rlp[r], l);
       }
       n = wave_list(S, q, xq, rb, rl, b.posA, b.tagA, L);
-      wave_kth_of_list(S, q, b.posA, b.tagA, b.cellStart, L, n, kd2, kid);
+      if (S.nsec > 0 && n >= 0) n = wave_list_ghosts(S, xq, n, b, L);
+      wave_kth_of_list(S, q, b.posA, b.tagA, b.cellStart, L, n, kd2, kid, false, S.nsec > 0 ? &b : nullptr);
     } else {
       const DevScene& S = sc[b.sceneOf[q]];
-      wave_kth(S, q, b.posA, b.tagA, b.cellStart, L, n, kd2, kid);
+      const double2 xq = b.posA[q];
+      n = wave_gather(S, q, b.posA, b.tagA, b.cellStart, L);
+      if (S.nsec > 0 && n >= 0) n = wave_list_ghosts(S, xq, n, b, L);
+      wave_kth_of_list(S, q, b.posA, b.tagA, b.cellStart, L, n, kd2, kid, false, S.nsec > 0 ? &b : nullptr);
     }
     if (lead) {
       b.kthD2[q] = kd2;
@@ -1146,6 +1328,7 @@ __device__ __forceinline__ bool pair_direct(int p, const StepBufs& b, const DevS
     if (!c.template admit<EXACT>(S, tag_id(tq), d2, kpd2, kpid, b, q)) return;
     c.template add<KT>(S, ex, ey, tq, b.mA[q], bq, mirror_c<MODE>(b, (uint32_t)q));
   });
+  if (!EXACT && S.nsec > 0 && cnt <= S.K) cnt += ghost_count(S, xp, b);
   if (!EXACT && cnt > S.K) return true;
   correct(S, b, p, itime, xp, vn, mass, c);
   return false;
@@ -1321,7 +1504,8 @@ __global__ __launch_bounds__(PB) void k_pair_compact(int n, StepBufs b, const De
     // drop the particle itself (always in range, in its own row)
     const int self = p - rb[1];
     if (self >= 0 && self < 64) m[1] &= ~(1ull << self);
-    const int cnt = __popcll(m[0]) + __popcll(m[1]) + __popcll(m[2]);
+    int cnt = __popcll(m[0]) + __popcll(m[1]) + __popcll(m[2]);
+    if (S.nsec > 0 && !bad && cnt <= S.K) cnt += ghost_count(S, xp, b);   // mirror rows rank too
     if (bad || cnt > S.K) {
       ovf = true;   // the exact kernel recomputes this step
       return;
@@ -1616,6 +1800,10 @@ static void fill_scene(DevScene& S, const uph_scene& u, double cell_factor) {
   S.fgk = (float)(u.gamma * u.kappa);
   S.fkap = (float)u.kappa;
   S.fmu = (float)u.mu;
+  S.lbp0 = u.lbp;
+  S.rbp0 = u.rbp;
+  S.depth = u.depth;
+  S.dx = u.dx;
 }
 
 static int alloc_bufs(StepBufs& b, int n, int ncells, int nblocks_scan, Err& err, bool search_only) {
@@ -1640,12 +1828,37 @@ static int alloc_bufs(StepBufs& b, int n, int ncells, int nblocks_scan, Err& err
   return 0;
 }
 
+// Mirror-row buffers for gtot section tests (see StepBufs), zeroed.
+static int alloc_ghost_bufs(StepBufs& b, int n, int ncells, int gtot, int gnb, Err& err) {
+  b.gtot = gtot;
+  const size_t g = (size_t)gtot;
+  if (dalloc(&b.gflag, g, err) || dalloc(&b.gstart, g + 1, err) || dalloc(&b.gpart, (size_t)gnb, err) ||
+      dalloc(&b.gpos, g, err) || dalloc(&b.gid, g, err) || dalloc(&b.gscene, g, err) || dalloc(&b.gkey, g, err) ||
+      dalloc(&b.grank, g, err) || dalloc(&b.gcellCount, (size_t)ncells, err) ||
+      dalloc(&b.gcellStart, (size_t)ncells + 1, err) || dalloc(&b.gposS, g, err) || dalloc(&b.gidS, g, err) ||
+      dalloc(&b.gvel, g, err) || dalloc(&b.glvel, g, err) || dalloc(&b.grho, g, err) || dalloc(&b.grl, g, err) ||
+      dalloc(&b.gP, g, err) || dalloc(&b.gPl, g, err) || dalloc(&b.gmass, g, err) || dalloc(&b.predV, n, err) ||
+      dalloc(&b.predVl, n, err) || dalloc(&b.predRho, n, err) || dalloc(&b.predRl, n, err) ||
+      dalloc(&b.predP, n, err) || dalloc(&b.predPl, n, err))
+    return -1;
+  if (hipMemset(b.gflag, 0, g * sizeof(int)) != hipSuccess ||
+      hipMemset(b.gstart, 0, (g + 1) * sizeof(int)) != hipSuccess ||
+      hipMemset(b.gcellCount, 0, (size_t)ncells * sizeof(int)) != hipSuccess ||
+      hipMemset(b.gcellStart, 0, ((size_t)ncells + 1) * sizeof(int)) != hipSuccess) {
+    err.msg = "mirror buffers: init failed";
+    return -1;
+  }
+  return 0;
+}
+
 static void free_bufs(StepBufs& b) {
   void* ptrs[] = {b.posA, b.velA, b.rhoA, b.kthStep, b.tagA, b.mA, b.mB, b.posB, b.velB, b.rhoB,
                   b.rtB, b.tagB, b.keys, b.rank, b.tmpIdx, b.cellStart, b.cellCount, b.partial,
                   b.sceneOf, b.kthD2, b.kthId, b.deckPos, b.deckVel, b.deckLvel, b.massD, b.ctl,
                   b.dbgRates, b.presA, b.presB, b.deckP, b.rt2B, b.dvB, b.mC, b.deckRl, b.deckPl,
-                  b.wallKind, b.wallRef};
+                  b.wallKind, b.wallRef, b.gflag, b.gstart, b.gpart, b.gpos, b.gid, b.gscene, b.gkey,
+                  b.grank, b.gcellCount, b.gcellStart, b.gposS, b.gidS, b.gvel, b.glvel, b.grho, b.grl,
+                  b.gP, b.gPl, b.gmass, b.predV, b.predVl, b.predRho, b.predRl, b.predP, b.predPl};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   memset(&b, 0, sizeof b);
@@ -1687,9 +1900,29 @@ static int launch_pairs(uph_handle* h, hipStream_t st, hipEvent_t pb) {
   return 0;
 }
 
+// The mirror rows of this step and their cell list (scenes with a virtualType that
+// has sections only): tests, scan, rows, bin, scan, scatter.
+static int launch_ghosts(uph_handle* h, hipStream_t st) {
+  StepBufs& b = h->b;
+  const int gnb = (b.gtot + SCAN_TILE - 1) / SCAN_TILE;
+  hipLaunchKernelGGL(k_ghost_flags, dim3(grid_of(h->n)), dim3(BLK), 0, st, h->n, b, h->dsc);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(gnb), dim3(SCAN_BLK), 0, st, b.gtot, b.gflag, b.gpart, (int*)nullptr);
+  hipLaunchKernelGGL(k_scan_down, dim3(gnb), dim3(SCAN_BLK), 0, st, b.gtot, b.gflag, b.gpart, b.gstart);
+  hipLaunchKernelGGL(k_ghost_emit, dim3(grid_of(h->n)), dim3(BLK), 0, st, h->n, b, h->dsc);
+  hipLaunchKernelGGL(k_ghost_bin, dim3(grid_of(b.gtot)), dim3(BLK), 0, st, b, h->dsc);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(h->cs.nb), dim3(SCAN_BLK), 0, st, h->ncells, b.gcellCount, b.partial,
+                     (int*)nullptr);
+  hipLaunchKernelGGL(k_scan_down, dim3(h->cs.nb), dim3(SCAN_BLK), 0, st, h->ncells, b.gcellCount, b.partial,
+                     b.gcellStart);
+  hipLaunchKernelGGL(k_ghost_scatter, dim3(grid_of(b.gtot)), dim3(BLK), 0, st, b);
+  HIPCHK(hipGetLastError(), h->err);
+  return 0;
+}
+
 static int launch_step(uph_handle* h, hipEvent_t pa, hipEvent_t pb) {
   hipStream_t st = h->stream;
   if (launch_sort(st, h->n, h->ncells, h->cs.nb, h->b, h->dsc, 0, true, h->err)) return -1;
+  if (h->b.gtot > 0 && launch_ghosts(h, st)) return -1;
   if (pa) HIPCHK(hipEventRecord(pa, st), h->err);
   int rc;
   if (h->mode == UPH_MODE_REF)
@@ -1717,7 +1950,7 @@ extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts*
   if (!(h->opts.cell_factor >= 1.0) || h->opts.cell_factor > 4.0) h->opts.cell_factor = 1.0;
   h->nscenes = n_scenes;
   h->hs.resize(n_scenes);
-  long n = 0, nc = 0;
+  long n = 0, nc = 0, gtot = 0;
   for (int s = 0; s < n_scenes; ++s) {
     const uph_scene& u = scenes[s];
     const int np = u.n_fluid + u.n_fixed + u.n_moving;
@@ -1756,12 +1989,22 @@ extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts*
     S.poff = (int)n;
     S.coff = (int)nc;
     S.pb_gid = (int)n + (u.n_fluid + u.n_fixed + u.n_moving - 1);
+    // mirror rows: one test per section and fluid row, ids after the scene's rows
+    S.nsec = u.n_fluid > 0 ? mirror_sections(u.vtype, S.sec) : 0;
+    S.goff = (int)gtot;
+    S.gbase = (int)n + np;
+    gtot += (long)S.nsec * u.n_fluid;
+    if ((long)S.gbase + (long)S.nsec * u.n_fluid >= (long)UPH_ID_MASK) {
+      g_create_err = "uph_create: too many particles and mirror rows";
+      delete h;
+      return -1;
+    }
     n += np;
     nc += (long)S.ncx * S.ncy;
     h->fluid_n.push_back(u.n_fluid);
     h->tot_n.push_back(np);
   }
-  if (n >= (long)UPH_ID_MASK || nc >= (1L << 30)) {
+  if (n >= (long)UPH_ID_MASK || nc >= (1L << 30) || gtot >= (1L << 30)) {
     g_create_err = "uph_create: too many particles or cells";
     delete h;
     return -1;
@@ -1786,6 +2029,9 @@ extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts*
     return fail();
   }
   if (alloc_bufs(h->b, h->n, h->ncells, h->cs.nb, err, false)) return fail();
+  if (gtot > 0 &&
+      alloc_ghost_bufs(h->b, h->n, h->ncells, (int)gtot, (int)((gtot + SCAN_TILE - 1) / SCAN_TILE), err))
+    return fail();
   if (h->opts.keep_rates && dalloc(&h->b.dbgRates, (h->mode == UPH_MODE_REF ? 4 : 2) * (size_t)h->n, err))
     return fail();
   if (dalloc(&h->dsc, n_scenes, err)) return fail();
@@ -1821,6 +2067,7 @@ extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts*
       tag[g] = ((uint32_t)ty << UPH_ID_BITS) | (uint32_t)g;
       sceneOf[g] = s;
     }
+    h->hs[s].slosh_halt = halted;   // uploaded with the scene table below
   }
   if (hipMemcpy(h->b.deckPos, pos.data(), n * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess) {
     err.msg = "uph_create: upload failed";
@@ -1968,6 +2215,46 @@ extern "C" int uph_time_steps(uph_t* h, int n_steps, double* ms_total, double* m
   if (ms_total) *ms_total = tot;
   if (ms_pair) *ms_pair = pair / n_steps;
   return 0;
+}
+
+extern "C" int uph_ghost_capacity(uph_t* h, int scene) {
+  if (!h || scene < 0 || scene >= h->nscenes) return -1;
+  return h->hs[scene].nsec * h->hs[scene].nf;
+}
+
+extern "C" int uph_get_ghosts(uph_t* h, int scene, int cap, double* pos, double* vel, double* lvel, double* rho,
+                              double* rholoc, double* P, double* Ploc, double* mass) {
+  if (!h) return -1;
+  if (scene < 0 || scene >= h->nscenes) {
+    h->err.msg = "uph_get_ghosts: scene out of range";
+    return -1;
+  }
+  HIPCHK(hipStreamSynchronize(h->stream), h->err);
+  const DevScene& S = h->hs[scene];
+  if (S.nsec == 0) return 0;
+  int se[2];
+  HIPCHK(hipMemcpy(&se[0], h->b.gstart + S.goff, sizeof(int), hipMemcpyDeviceToHost), h->err);
+  HIPCHK(hipMemcpy(&se[1], h->b.gstart + S.goff + S.nsec * S.nf, sizeof(int), hipMemcpyDeviceToHost), h->err);
+  const int cnt = se[1] - se[0];
+  if (cnt < 0 || cnt > cap) {
+    h->err.msg = "uph_get_ghosts: " + std::to_string(cnt) + " mirror rows, capacity " + std::to_string(cap);
+    return -1;
+  }
+  auto get = [&](double* dst, const void* src, size_t width) -> int {
+    if (!dst || cnt == 0) return 0;
+    return hipMemcpy(dst, (const char*)src + (size_t)se[0] * width, (size_t)cnt * width, hipMemcpyDeviceToHost) ==
+                   hipSuccess
+               ? 0
+               : -1;
+  };
+  if (get(pos, h->b.gpos, sizeof(double2)) || get(vel, h->b.gvel, sizeof(double2)) ||
+      get(lvel, h->b.glvel, sizeof(double2)) || get(rho, h->b.grho, sizeof(double)) ||
+      get(rholoc, h->b.grl, sizeof(double)) || get(P, h->b.gP, sizeof(double)) ||
+      get(Ploc, h->b.gPl, sizeof(double)) || get(mass, h->b.gmass, sizeof(double))) {
+    h->err.msg = "uph_get_ghosts: download failed";
+    return -1;
+  }
+  return cnt;
 }
 
 extern "C" int uph_get_state(uph_t* h, int scene, double* pos, double* vel, double* rho) {
@@ -2826,7 +3113,10 @@ std::vector<double> compat_key(const uph_system* s) {
           s->alpha, s->springConstant0, s->dampingConstant0, (double)s->HowManyNeighbors,
           (double)s->movingBoundary, s->stroke, s->omega, (double)c.mode, (double)c.kernel,
           s->thermalDiffusivity0, s->viscosity0, s->epsilon, (double)s->virtualType, s->lidVelocity,
-          s->LocalDepth, s->FlatBottomLength};
+          s->LocalDepth, s->FlatBottomLength,
+          // the boundary positions the mirror sections read where no motion sets them
+          (s->movingBoundary == 2 || s->movingBoundary == 3) ? s->LeftBoundaryPosition : 0.0,
+          compat_motion(s) != UPH_MOTION_SLOSH ? s->RightBoundaryPosition : 0.0};
 }
 
 int compat_build(const uph_system* s) {
@@ -2887,6 +3177,9 @@ int compat_build(const uph_system* s) {
   u.kappa = s->thermalDiffusivity0;
   u.mu = s->viscosity0;
   u.eps = s->epsilon;
+  u.vtype = s->virtualType;
+  u.lbp = s->LeftBoundaryPosition;
+  u.rbp = s->RightBoundaryPosition;
   std::vector<double> pres(np, 0.0);
   if (s->DynamicPressure)
     for (int i = 0; i < np; ++i) pres[i] = s->DynamicPressure[i];
@@ -2961,8 +3254,9 @@ extern "C" int uph_derivatives(uph_system* s, double t) {
   std::lock_guard<std::mutex> lk(c.mu);
   c.err.clear();
   if (!s) return compat_fail("derivatives: null System");
-  if (s->NumberOfSolidParticles != 0 || s->NumberOfFrozenParticles != 0 || s->NumberOfVirtualParticles != 0)
-    return compat_fail("derivatives: solid, frozen and virtual particles are not supported");
+  // (NumberOfVirtualParticles is an output: each call makes the mirror rows anew)
+  if (s->NumberOfSolidParticles != 0 || s->NumberOfFrozenParticles != 0)
+    return compat_fail("derivatives: solid and frozen particles are not supported");
   if (s->flagInitSimul != 1)   // derivatives.c:4197: 0 is the initialization phase
     return compat_fail("derivatives: flagInitSimul must be 1 (the run phase; the initialization phase is not supported)");
   if (s->dim != 2) return compat_fail("derivatives: only dim = 2 is supported");
@@ -3114,6 +3408,38 @@ extern "C" int uph_derivatives(uph_system* s, double t) {
     if (s->Pressure) s->Pressure[i] = compat_tait(s, c.deck_rho[k]);
     if (s->localDensity) s->localDensity[i] = r0;
     if (s->localPressure) s->localPressure[i] = 0.0;   // inputbp.c:290
+  }
+  // mirror rows after the particle rows, with the fields derivatives.c:3193-4192
+  // copies from their sources, and their number (derivatives.c:4192)
+  s->NumberOfVirtualParticles = 0;
+  if (S.nsec > 0) {
+    const int cap = S.nsec * nf;
+    std::vector<double> gx(2 * (size_t)cap), gv(2 * (size_t)cap), gl(2 * (size_t)cap), gr(cap), grl(cap), gp(cap),
+        gpl(cap), gm(cap);
+    const int ng = uph_get_ghosts(h, 0, cap, gx.data(), gv.data(), gl.data(), gr.data(), grl.data(), gp.data(),
+                                  gpl.data(), gm.data());
+    if (ng < 0) return compat_fail("derivatives: " + err.msg);
+    const int first = nf + nfix + nmov;
+    if ((long)first + ng > (long)s->MaxNumberOfParticles)
+      return compat_fail("derivatives: the mirror rows need " + std::to_string(first + ng) +
+                         " rows, the System's MaxNumberOfParticles is " + std::to_string(s->MaxNumberOfParticles));
+    for (int k = 0; k < ng; ++k) {
+      const int i = first + k;
+      s->Position[i][0] = gx[2 * k];
+      s->Position[i][1] = gx[2 * k + 1];
+      s->Velocity[i][0] = gv[2 * k];
+      s->Velocity[i][1] = gv[2 * k + 1];
+      s->localVelocity[i][0] = gl[2 * k];
+      s->localVelocity[i][1] = gl[2 * k + 1];
+      s->mass[i] = gm[k];
+      s->density[i] = gr[k];
+      s->hsml[i] = s->hsml[0];
+      s->itype[i] = 4;
+      if (s->DynamicPressure) s->DynamicPressure[i] = gp[k];
+      if (s->localDensity) s->localDensity[i] = grl[k];
+      if (s->localPressure) s->localPressure[i] = gpl[k];
+    }
+    s->NumberOfVirtualParticles = ng;
   }
   return 0;
 }

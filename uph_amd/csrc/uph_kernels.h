@@ -40,7 +40,145 @@ struct DevScene {
   float fK0, fgamma, fgk, fkap, fmu;  // rho0 c^2, gamma, gamma kappa, kappa, viscosity0
   // lid / gate speed, the gate's stop time, the sloshing tank's right-wall offset
   double lid_u, t_stop, flat_bottom;
+  // mirror ("virtual") particles: the section codes of the scene's virtualType in
+  // order (UPH_GS_*), the first entry of its section tests in the mirror flag array
+  // (nsec x nf entries, section-major), the id of its first mirror row (after its
+  // np rows); a sloshing moving set whose loop zeroes LeftBoundaryPosition; the
+  // boundary positions where the motion does not set them; LocalDepth, dx
+  int nsec, sec[5], goff, gbase, slosh_halt;
+  double lbp0, rbp0, depth, dx;
 };
+
+// Sections of derivatives.c:3193-4192 (one loop over the fluid rows each).
+enum {
+  UPH_GS_LEFT0 = 1,  // x about 0: |x| < 2h, (-|x|, y)
+  UPH_GS_LEFT,       // x about LeftBoundaryPosition: |x - L| < 2h, (L - |x - L|, y)
+  UPH_GS_BOTTOM,     // y about 0: y < 2h and x <= FlatBottomLength, (x, -y)
+  UPH_GS_CORNER_L,   // |x - L| < 2h and y < 2h: (L - |x - L|, -y)
+  UPH_GS_CORNER_R,   // |x - R| < 2h and y < 2h: (R + |x - R|, -y)
+  UPH_GS_RIGHT_FB,   // |x - FBL| < 2h: (FBL + |x - FBL|, y)
+  UPH_GS_RIGHT,      // |x - R| < 2h: (R + |x - R|, y)
+  UPH_GS_TOP_DX,     // |y - H - dx| < 2h: (x, H + 2 |y - H - dx|)
+  UPH_GS_TOP,        // |y - H| < 2h: (x, H + 2 |y - H|)
+  UPH_GS_SLOPE       // reflection in y = (x - FBL) / 15 within 3h, x > FBL
+};
+
+// Section list of a virtualType (derivatives.c:3195, 3471, 3626, 3783, 3907, 4032;
+// virtualType 2's are commented out); returns its length.
+__host__ __device__ inline int mirror_sections(int vtype, int* sec) {
+  int m = 0;
+  auto put = [&](int a, int b, int c, int d, int e) {
+    const int v[5] = {a, b, c, d, e};
+    for (int k = 0; k < 5; ++k)
+      if (v[k]) sec[m++] = v[k];
+  };
+  switch (vtype) {
+    case 1: put(UPH_GS_LEFT, UPH_GS_BOTTOM, UPH_GS_CORNER_L, UPH_GS_SLOPE, 0); break;
+    case 3: put(UPH_GS_LEFT0, UPH_GS_BOTTOM, UPH_GS_TOP_DX, UPH_GS_CORNER_L, UPH_GS_RIGHT_FB); break;
+    case 4:
+    case 8: put(UPH_GS_LEFT0, UPH_GS_BOTTOM, UPH_GS_CORNER_L, UPH_GS_RIGHT_FB, 0); break;
+    case 5: put(UPH_GS_LEFT, UPH_GS_CORNER_L, UPH_GS_CORNER_R, UPH_GS_RIGHT, 0); break;
+    case 7: put(UPH_GS_LEFT0, UPH_GS_BOTTOM, UPH_GS_TOP, UPH_GS_CORNER_L, UPH_GS_RIGHT_FB); break;
+    default: break;
+  }
+  return m;
+}
+
+// Section `sec`'s test of a fluid row at x and, when it passes, its mirror
+// position g and mirrored velocity v and localVelocity lv, in FP64 with the
+// reference's expressions (no contraction, so the positions are bit-exact).
+__host__ __device__ inline bool mirror_row(int sec, double lbp, double rbp, double fbl, double H, double dx,
+                                           double h, double2 x, double2& g, double2& v, double2& lv) {
+#pragma clang fp contract(off)
+  const double h2 = 2.0 * h;
+  double sx = 1.0, sy = 1.0;
+  bool on = false;
+  switch (sec) {
+    case UPH_GS_LEFT0: {
+      const double d = fabs(x.x - 0.0 * fbl);
+      on = d < h2; g = make_double2(0.0 - d, x.y); sx = -1.0;
+      break;
+    }
+    case UPH_GS_LEFT: {
+      const double d = fabs(x.x - lbp);
+      on = d < h2; g = make_double2(lbp - d, x.y); sx = -1.0;
+      break;
+    }
+    case UPH_GS_BOTTOM:
+      on = (x.y < h2) && (x.x <= fbl); g = make_double2(x.x, -x.y); sy = -1.0;
+      break;
+    case UPH_GS_CORNER_L: {
+      const double d = fabs(x.x - lbp);
+      on = (d < h2) && (x.y < h2); g = make_double2(lbp - d, -x.y); sx = sy = -1.0;
+      break;
+    }
+    case UPH_GS_CORNER_R: {
+      const double d = fabs(x.x - rbp);
+      on = (d < h2) && (x.y < h2); g = make_double2(rbp + d, -x.y); sx = sy = -1.0;
+      break;
+    }
+    case UPH_GS_RIGHT_FB: {
+      const double d = fabs(x.x - fbl);
+      on = d < h2; g = make_double2(fbl + d, x.y); sx = -1.0;
+      break;
+    }
+    case UPH_GS_RIGHT: {
+      const double d = fabs(x.x - rbp);
+      on = d < h2; g = make_double2(rbp + d, x.y); sx = -1.0;
+      break;
+    }
+    case UPH_GS_TOP_DX: {
+      const double d = fabs(x.y - H - dx);
+      on = d < h2; g = make_double2(x.x, H + 2 * d); sy = -1.0;
+      break;
+    }
+    case UPH_GS_TOP: {
+      const double d = fabs(x.y - H);
+      on = d < h2; g = make_double2(x.x, H + 2 * d); sy = -1.0;
+      break;
+    }
+    case UPH_GS_SLOPE: {
+      const double slope = 1.0 / 15.0;   // System.c:309
+      const double denoms = 1 + slope * slope;
+      const double nums = 1 - slope * slope;
+      const double yIntercept = slope * fbl;
+      const double d = (1.0 / sqrt(denoms)) * fabs(slope * x.x - x.y - yIntercept);
+      on = (d < 3.0 * h) && (x.x > fbl);
+      g = make_double2((nums * x.x + 2.0 * slope * x.y + 2.0 * slope * yIntercept) / denoms,
+                       (2.0 * slope * x.x - nums * x.y - 2.0 * yIntercept) / denoms);
+      if (on) {
+        v = make_double2((nums * v.x + 2.0 * slope * v.y) / denoms, (2.0 * slope * v.x - nums * v.y) / denoms);
+        lv = make_double2((nums * lv.x + 2.0 * slope * lv.y) / denoms, (2.0 * slope * lv.x - nums * lv.y) / denoms);
+      }
+      return on;
+    }
+    default:
+      return false;
+  }
+  if (on) {
+    v = make_double2(sx * v.x, sy * v.y);
+    lv = make_double2(sx * lv.x, sy * lv.y);
+  }
+  return on;
+}
+
+// LeftBoundaryPosition and RightBoundaryPosition of a derivatives(t) call, as
+// inputbp leaves them (inputbp.c:192-283): the piston's position, the sloshing
+// tank's L(t) (0 past a moving row of another itype) and FlatBottomLength + L(t),
+// 0 in the no-motion branch, otherwise the values the scene was given.
+__host__ __device__ inline void boundary_lines(const DevScene& S, double t, double& lbp, double& rbp) {
+  lbp = S.lbp0;
+  rbp = S.rbp0;
+  const double L = 0.5 * S.m_amp * (1.0 - cos(S.m_omega * t));
+  if (S.motion == 1) {
+    lbp = L;
+  } else if (S.motion == 5) {
+    lbp = S.slosh_halt ? 0.0 : L;
+    rbp = S.flat_bottom + L;
+  } else if (S.motion == 0 || S.motion == 2) {
+    lbp = 0.0;
+  }
+}
 
 __device__ __forceinline__ uint32_t tag_id(uint32_t t) { return t & UPH_ID_MASK; }
 __device__ __forceinline__ uint32_t tag_type(uint32_t t) { return t >> UPH_ID_BITS; }

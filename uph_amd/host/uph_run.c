@@ -187,8 +187,10 @@ static int run_compat(uph_system *s, int nsteps, int print_step, int init_steps)
         }
     }
     if (print_step > 0 && itime % print_step == 0) {
-      /* sph.cc:168-180: the rows are contiguous (rows2), as CreateMatrix makes them */
-      snap_t o = {np, nf, s->Position[0], s->Velocity[0], s->density, s->mass, s->DynamicPressure,
+      /* sph.cc:168-180: the rows are contiguous (rows2), as CreateMatrix makes them;
+       * fluid, boundary and mirror rows (output.c:28) */
+      const int rows = nf + s->NumberOfBoundaryParticles + s->NumberOfVirtualParticles;
+      snap_t o = {rows, nf, s->Position[0], s->Velocity[0], s->density, s->mass, s->DynamicPressure,
                   s->localDensity, s->itype, s->hsml[0], itime * dt, s->totalKineticEnergy,
                   s->potentialEnergy, s->pointPressure, s->pressureNorm, s->pointNeighbors};
       if (write_output(&o, itime) != 0) return -1;
@@ -373,6 +375,7 @@ int main(int argc, char **argv) {
   s.motion = moving == 1 ? UPH_MOTION_PISTON : moving == 3 ? UPH_MOTION_LID : moving == 2 ? UPH_MOTION_GATE
            : (int)v[15] == 5 ? UPH_MOTION_SLOSH : UPH_MOTION_NONE;
   s.lid_u = v[19]; s.depth = v[2]; s.flat_bottom = v[3]; s.gate_y = all.gate;
+  s.vtype = (int)v[15];   /* virtualType: mirror particles (derivatives.c:3193-4192) */
   s.m_amp = stroke; s.m_omega = 2.0 * M_PI / period;
   s.kernel = kernel;
   s.mode = mode;
@@ -394,16 +397,19 @@ int main(int argc, char **argv) {
     sy.NumberOfFixedBoundaryParticles = fx.n;
     sy.NumberOfMovingBoundaryParticles = mv.n;
     sy.NumberOfBoundaryParticles = fx.n + mv.n;
-    sy.MaxNumberOfParticles = n;
-    sy.Position = rows2(n); sy.Velocity = rows2(n); sy.localVelocity = rows2(n);
-    sy.Velocity_min = rows2(n); sy.Velocity_xsph = rows2(n);
-    sy.dvdt = rows2(n); sy.dvdt_approx = rows2(n); sy.velocityFluctuation = rows2(n);
-    sy.mass = vec(n); sy.density = vec(n); sy.density_min = vec(n); sy.localDensity = vec(n);
-    sy.Pressure = vec(n); sy.DynamicPressure = vec(n); sy.DynamicPressure_min = vec(n);
-    sy.localPressure = vec(n); sy.dpdt = vec(n); sy.dpdt_approx = vec(n); sy.hsml = vec(n);
-    sy.ddensitydt = vec(n); sy.ddensitydt_approx = vec(n);
-    sy.densityFluctuation = vec(n); sy.pressureFluctuation = vec(n);
-    sy.itype = calloc(n, sizeof(int));
+    /* rows for the mirror particles derivatives() appends (at most 5 per fluid row;
+     * the reference reserves 100, System.c:195) */
+    const int nm = n + 5 * fl.n;
+    sy.MaxNumberOfParticles = nm;
+    sy.Position = rows2(nm); sy.Velocity = rows2(nm); sy.localVelocity = rows2(nm);
+    sy.Velocity_min = rows2(nm); sy.Velocity_xsph = rows2(nm);
+    sy.dvdt = rows2(nm); sy.dvdt_approx = rows2(nm); sy.velocityFluctuation = rows2(nm);
+    sy.mass = vec(nm); sy.density = vec(nm); sy.density_min = vec(nm); sy.localDensity = vec(nm);
+    sy.Pressure = vec(nm); sy.DynamicPressure = vec(nm); sy.DynamicPressure_min = vec(nm);
+    sy.localPressure = vec(nm); sy.dpdt = vec(nm); sy.dpdt_approx = vec(nm); sy.hsml = vec(nm);
+    sy.ddensitydt = vec(nm); sy.ddensitydt_approx = vec(nm);
+    sy.densityFluctuation = vec(nm); sy.pressureFluctuation = vec(nm);
+    sy.itype = calloc(nm, sizeof(int));
     o = 0;
     for (int p = 0; p < 3; p++)
       for (int i = 0; i < parts[p]->n; i++, o++) {
@@ -469,8 +475,11 @@ int main(int argc, char **argv) {
     fprintf(stderr, "uph_run: %s\n", uph_last_error(NULL));
     return 1;
   }
-  double *pos = malloc(sizeof(double) * 2 * n), *vel = malloc(sizeof(double) * 2 * n);
-  double *rho = malloc(sizeof(double) * n);
+  /* room for the mirror rows the snapshots list after the particle rows (output.c:28) */
+  const int gcap = uph_ghost_capacity(h, 0) > 0 ? uph_ghost_capacity(h, 0) : 0;
+  const int nx = n + gcap;
+  double *pos = malloc(sizeof(double) * 2 * nx), *vel = malloc(sizeof(double) * 2 * nx);
+  double *rho = malloc(sizeof(double) * nx);
   double t0 = now_s();
   if (print_step <= 0 || print_step > nsteps) {
     if (uph_step(h, nsteps) != 0 || uph_sync(h) < 0) {
@@ -482,7 +491,12 @@ int main(int argc, char **argv) {
     /* printStep output: the next interval is queued before the host writes the
      * files of this one, so the device keeps stepping during the I/O */
     truncate_series();
-    double *pres = malloc(sizeof(double) * n), *rhol = malloc(sizeof(double) * n);
+    double *pres = malloc(sizeof(double) * nx), *rhol = malloc(sizeof(double) * nx);
+    double *massx = malloc(sizeof(double) * nx);
+    int *itypex = malloc(sizeof(int) * nx);
+    memcpy(massx, all.mass, sizeof(double) * n);
+    memcpy(itypex, all.itype, sizeof(int) * n);
+    for (int i = n; i < nx; i++) itypex[i] = 4;   /* mirror rows (derivatives.c:3220) */
     int done = 0, first = print_step < nsteps ? print_step : nsteps;
     if (uph_step(h, first) != 0) {
       fprintf(stderr, "uph_run: %s\n", uph_last_error(h));
@@ -496,13 +510,8 @@ int main(int argc, char **argv) {
         return 1;
       }
       done += chunk;
-      if (done < nsteps) {
-        const int next = print_step < nsteps - done ? print_step : nsteps - done;
-        if (uph_step(h, next) != 0) {
-          fprintf(stderr, "uph_run: %s\n", uph_last_error(h));
-          return 1;
-        }
-      }
+      /* everything the snapshot lists is read before the next interval is queued */
+      int ng = 0;
       if (done % print_step == 0) {
         for (int i = 0; i < n; i++) {
           pres[i] = i < fl.n ? s.B * (pow(rho[i] / rho0, gamma) - 1.0) : (i < fl.n + fx.n ? fx.pres[i - fl.n] : mv.pres[i - fl.n - fx.n]);
@@ -512,12 +521,29 @@ int main(int argc, char **argv) {
           fprintf(stderr, "uph_run: %s\n", uph_last_error(h));
           return 1;
         }
-        snap_t o = {n, fl.n, pos, vel, rho, all.mass, pres, rhol, all.itype, s.h, done * dt, dg.kinetic_energy,
+        ng = uph_get_ghosts(h, 0, gcap, pos + 2 * (size_t)n, vel + 2 * (size_t)n, NULL, rho + n, rhol + n,
+                            pres + n, NULL, massx + n);
+        if (ng < 0) {
+          fprintf(stderr, "uph_run: %s\n", uph_last_error(h));
+          return 1;
+        }
+      }
+      if (done < nsteps) {
+        const int next = print_step < nsteps - done ? print_step : nsteps - done;
+        if (uph_step(h, next) != 0) {
+          fprintf(stderr, "uph_run: %s\n", uph_last_error(h));
+          return 1;
+        }
+      }
+      if (done % print_step == 0) {
+        snap_t o = {n + ng, fl.n, pos, vel, rho, massx, pres, rhol, itypex, s.h, done * dt, dg.kinetic_energy,
                     dg.potential_energy, dg.probe_pressure, dg.probe_norm, dg.probe_neighbors};
         if (write_output(&o, done) != 0) return 1;
         if (done == init_steps && write_init(&o, vel) != 0) return 1;
       }
     }
+    free(massx);
+    free(itypex);
     if (uph_sync(h) < 0) {
       fprintf(stderr, "uph_run: %s\n", uph_last_error(h));
       return 1;

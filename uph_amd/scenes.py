@@ -261,17 +261,21 @@ def gate_dam_break(width=0.3, height=0.6, dx=0.01, tank=(1.2, 0.9), gate_u=1.0, 
 
 
 def sloshing_tank(width=0.4, depth=0.1, dx=0.005, wall_height=0.3, stroke=0.02, period=0.5, h_factor=1.3,
-                  alpha_av=0.1, cfl=0.25, rho0=1000.0, gamma=7.0, g=9.81, name=None) -> Scene:
+                  alpha_av=0.1, cfl=0.25, rho0=1000.0, gamma=7.0, g=9.81, jitter=0.0, seed=0,
+                  name=None) -> Scene:
     """Sloshing tank (UPH_MOTION_SLOSH, virtualType 5): every wall moves with
     L(t) = stroke/2 (1 - cos 2 pi t / period).  The reference places each row by its
     itype alone -- the k-th moving row of itype 1 at x = k dx/2 + L, every row of
     itype 2 at x = L, of itype 3 at x = FlatBottomLength + L (inputbp.c:244-276) --
     so the moving set is one bottom row (2 nx + 1 particles at dx/2 on y = 0),
     then one left and one right column at x = 0 and x = width (dx/2 apart from
-    y = dx/2 up to wall_height); the fluid lattice fills [0, width] x [0, depth]."""
+    y = dx/2 up to wall_height); the fluid lattice fills [0, width] x [0, depth]
+    (`jitter`: fluid displaced uniformly by +-jitter*dx)."""
     nx = int(math.floor(width / dx + 1e-9))
     ny = int(math.floor(depth / dx + 1e-9))
     fl = _lattice(nx, ny, dx)
+    if jitter > 0:
+        fl = fl + np.random.default_rng(seed).uniform(-jitter * dx, jitter * dx, fl.shape)
     W, H = nx * dx, ny * dx
     c0, B = tait_constants(rho0, gamma, g, H)
     rho, p = hydrostatic_density(fl[:, 1], H, rho0, gamma, g, B)
@@ -301,18 +305,21 @@ def sloshing_tank(width=0.4, depth=0.1, dx=0.005, wall_height=0.3, stroke=0.02, 
 
 def wave_tank(length=6.0, depth=1.0, dx=0.005, beach_toe=5.0, beach_slope=0.1,
               amp=0.05, period=1.2, layers=3, h_factor=1.3, alpha_av=0.1, cfl=0.25,
-              rho0=1000.0, gamma=7.0, g=9.81, wall_height=1.5, name=None) -> Scene:
+              rho0=1000.0, gamma=7.0, g=9.81, wall_height=1.5, jitter=0.0, seed=0, name=None) -> Scene:
     """Water layer with a piston wavemaker on the left and a sloping beach on the right.
 
     The piston is `layers` rows of moving boundary particles displaced by
     amp sin(2 pi t / period); the beach rises at `beach_slope` from x = beach_toe
-    to the right wall at x = length.
+    to the right wall at x = length.  `jitter` displaces fluid particles uniformly by
+    +-jitter*dx.
     """
     nx = int(math.floor(length / dx + 1e-9))
     ny = int(math.floor(depth / dx + 1e-9))
     fl = _lattice(nx, ny, dx)
     bed = np.where(fl[:, 0] > beach_toe, (fl[:, 0] - beach_toe) * beach_slope, 0.0)
     fl = fl[fl[:, 1] >= bed + 0.5 * dx - 1e-9 * dx]     # lattice rows start at y = dx/2
+    if jitter > 0:
+        fl = fl + np.random.default_rng(seed).uniform(-jitter * dx, jitter * dx, fl.shape)
     H = ny * dx
     c0, B = tait_constants(rho0, gamma, g, H)
     rho, p = hydrostatic_density(fl[:, 1], H, rho0, gamma, g, B)

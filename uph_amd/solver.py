@@ -33,7 +33,8 @@ class UphScene(C.Structure):
                 ("m_amp", C.c_double), ("m_omega", C.c_double), ("box", C.c_double * 4),
                 ("kernel", C.c_int), ("pres", _dp), ("mode", C.c_int), ("kappa", C.c_double),
                 ("mu", C.c_double), ("eps", C.c_double), ("lid_u", C.c_double), ("depth", C.c_double),
-                ("flat_bottom", C.c_double), ("gate_y", _dp)]
+                ("flat_bottom", C.c_double), ("gate_y", _dp), ("vtype", C.c_int),
+                ("lbp", C.c_double), ("rbp", C.c_double)]
 
 
 class UphDiag(C.Structure):
@@ -67,6 +68,8 @@ SIGNATURES = {
                                 C.POINTER(C.c_float)]),
     "uph_get_ref_rates": (C.c_int, [C.c_void_p, C.c_int] + [C.POINTER(C.c_float)] * 5),
     "uph_get_pressure": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
+    "uph_get_ghosts": (C.c_int, [C.c_void_p, C.c_int, C.c_int] + [C.POINTER(C.c_double)] * 8),
+    "uph_ghost_capacity": (C.c_int, [C.c_void_p, C.c_int]),
     "uph_derivatives_scheme": (C.c_int, [C.c_int, C.c_int]),
     "uph_itime": (C.c_int, [C.c_void_p]),
     "uph_time_steps": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp]),
@@ -230,6 +233,8 @@ class GpuSolver:
                 gy = np.ascontiguousarray(sc.gate_y, dtype=np.float64)
                 self._keep.append(gy)
                 u.gate_y = _ptr(gy)
+            u.vtype = int(getattr(sc, "vtype", 0))
+            u.lbp, u.rbp = 0.0, 0.0
         opts = UphOpts(int(bool(use_graph)), int(bool(keep_rates)), int(pair_variant),
                        int(pair_block), float(cell_factor))
         self.opts = dict(use_graph=bool(use_graph), pair_variant=int(pair_variant),
@@ -300,6 +305,18 @@ class GpuSolver:
         P = np.zeros(self._scene(scene).n)
         self._check(lib().uph_get_pressure(self.h, scene, _ptr(P)))
         return P
+
+    def ghosts(self, scene: int = 0):
+        """Mirror rows of the last step (derivatives.c:3193-4192): x, v, vloc (ng, 2) and
+        rho, rholoc, P, Ploc, mass (ng), in the reference's order."""
+        self._scene(scene)
+        cap = max(int(lib().uph_ghost_capacity(self.h, scene)), 1)
+        out = {k: np.zeros((cap, 2)) for k in ("x", "v", "vloc")}
+        out.update({k: np.zeros(cap) for k in ("rho", "rholoc", "P", "Ploc", "mass")})
+        ng = lib().uph_get_ghosts(self.h, scene, cap, *[_ptr(out[k]) for k in
+                                                         ("x", "v", "vloc", "rho", "rholoc", "P", "Ploc", "mass")])
+        self._check(0 if ng >= 0 else -1)
+        return {k: a[:ng].copy() for k, a in out.items()}
 
     def diagnostics(self, probe=(0.0, 0.0), scene: int = 0) -> dict:
         """Kinetic/potential energy, the pressure probe and the density consistency values
