@@ -91,6 +91,9 @@ def main():
     ap.add_argument("--h-over-dx", type=float, default=1.3,
                     help="smoothing length in lattice spacings (1.3 default; the reference's decks "
                          "use 2.691, where the 40-neighbour cap binds for every particle)")
+    ap.add_argument("--vtype", type=int, default=0,
+                    help="virtualType of the tank (mirror particles of derivatives.c:3193-4192; 0: none; "
+                         "8: the dam break's left, bottom, corner and right images)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run one D2D copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -113,8 +116,10 @@ def main():
         # the pressure diffusion is explicit: with the decks' 0.015 m^2/s its SPH Laplacian
         # needs dt < h^2 / (10 gamma kappa) = 3.2 us at h = 1.84 mm, below this dt of 7.3 us
         sc.kappa, sc.mu, sc.eps, sc.alpha_av = 0.0015, 1e-6, 1.0, 0.0
+    if args.vtype:
+        sc.vtype, sc.flat_bottom, sc.depth = args.vtype, 4.0, 2.0   # the tank's width, the column's height
     log(f"workload: {sc.n_fluid} fluid + {sc.n_boundary} wall particles, {args.kernel} kernel, "
-        f"h = {args.h_over_dx} dx, {args.scheme} scheme")
+        f"h = {args.h_over_dx} dx, {args.scheme} scheme, virtualType {args.vtype}")
     g = GpuSolver(sc)
     g.step(args.warmup)
     g.sync()
@@ -143,12 +148,13 @@ def main():
         "config": {"workload": "dam break 1.0x2.0 m column, dx=sqrt(2) mm, 4x3 m tank",
                    "fluid_particles": sc.n_fluid, "wall_particles": sc.n_boundary,
                    "kernel": args.kernel, "scheme": args.scheme, "h_over_dx": args.h_over_dx, "dt": sc.dt,
-                   "capped_steps": overflow},
+                   "virtual_type": args.vtype, "capped_steps": overflow},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      # the committed counter run covers the default workload only
                      "traffic": (traffic_from_profiles() if args.kernel == "cubic" and
-                                 args.h_over_dx == 1.3 and args.scheme == "config" else None),
+                                 args.h_over_dx == 1.3 and args.scheme == "config" and not args.vtype
+                                 else None),
                      "kernel": f"k_pair_compact<256, {int(sc.kernel)}, {int(sc.mode)}> ({args.kernel}, "
                                f"{args.scheme})",
                      "kernel_ms": ms_pair},

@@ -247,15 +247,18 @@ struct StepBufs {
   int* ctl;
   float4* dbgRates;   // keep_rates: per slot {R, R~, a~x, a~y} (FP32), {ax, ay} (FP64);
                       // UPH_MODE_REF adds {dpdt, dpdt_approx, drho, dP}, {dvx, dvy, 0, 0}
-  // mirror ("virtual") particles, allocated when some scene has them (gtot > 0):
-  // the section test of every (scene, section, fluid row), its exclusive scan (the
-  // mirror index of each passing test; gstart[gtot] is the step's total) and the scan's
-  // partials; per mirror index its position, key id, scene, cell and in-cell rank;
-  // the mirror rows sorted by cell (gcellStart per cell, ncells + 1); the fields
-  // derivatives.c copies into a mirror row; the predicted fields of fluid rows of
-  // scenes with mirrors, per A slot, from which they are copied
-  int gtot;
-  int *gflag, *gstart, *gpart;
+  // mirror ("virtual") particles, allocated when some scene has them (gtot > 0, the
+  // most mirror rows a step can make; wtot words of section-test bits): the bits the
+  // sort sets (cleared again by k_ghost_count), the step's copy of them, their
+  // popcounts and exclusive scan (gwstart[w] + the set bits below a row's in word w is
+  // its mirror index; gwstart[wtot] is the step's total) with the scan's partials; the
+  // slots of the fluid rows that have mirror rows and their number; per mirror index
+  // its position, key id, scene, cell and in-cell rank; the mirror rows sorted by cell
+  // (gcellStart per cell, ncells + 1); the fields derivatives.c copies into a mirror
+  // row; the predicted fields of the listed rows, per A slot, from which they are copied
+  int gtot, wtot;
+  uint32_t *gbits, *gbits2;
+  int *gwcnt, *gwstart, *gpart, *gcand, *gcandN;
   double2* gpos;
   uint32_t* gid;
   int *gscene, *gkey, *grank, *gcellCount, *gcellStart;
@@ -266,6 +269,36 @@ struct StepBufs {
   double2 *predV, *predVl;
   double *predRho, *predRl, *predP, *predPl;
 };
+
+// The section tests of fluid row `id` (global) at position x in this step
+// (derivatives.c:3193-4192): each passing test sets the row's bit in its section's
+// words; a row with any mirror row is listed (slot d).  Returns whether it was.
+__device__ __forceinline__ bool mark_mirrors(const DevScene& S, const StepBufs& b, int itime, uint32_t id, double2 x,
+                                             int d) {
+  double lbp, rbp;
+  boundary_lines(S, S.t0 + itime * S.dt, lbp, rbp);
+  const int i = (int)id - S.poff;
+  bool any = false;
+  for (int k = 0; k < S.nsec; ++k) {
+    double2 g, v = make_double2(0.0, 0.0), lv = v;
+    if (mirror_row(S.sec[k], lbp, rbp, S.flat_bottom, S.depth, S.dx, S.h, x, g, v, lv)) {
+      atomicOr(&b.gbits[S.gw + k * S.nwords + (i >> 5)], 1u << (i & 31));
+      any = true;
+    }
+  }
+  // one atomic per wavefront for the list slots of its listed rows (the lanes here
+  // are the wavefront's fluid rows of scenes with mirror rows)
+  const unsigned long long m = __ballot(any);
+  if (any) {
+    const int leader = __ffsll((long long)m) - 1;
+    int base = 0;
+    if ((int)(threadIdx.x & 63) == leader) base = atomicAdd(b.gcandN, __popcll(m));
+    base = __shfl(base, leader, 64);
+    const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+    b.gcand[base + below] = d;
+  }
+  return any;
+}
 
 // Final slot = cell start + number of same-cell particles with a smaller
 // original index (deterministic order); gather the state and build the
@@ -351,8 +384,8 @@ __global__ __launch_bounds__(BLK) void k_reorder(int n, StepBufs b, const DevSce
     b.mA[dst] = make_float4((float)vx, (float)vy, (float)lx, (float)ly);
     b.mB[dst] = make_float4((float)(rp - S.rho0), (float)Pp, rt.w, __uint_as_float(tg));
     b.mC[dst] = make_float2((float)(rl - S.rho0), (float)pl);
-    if (S.nsec > 0 && tag_type(tg) == 1) {   // the fields this row's mirror rows copy
-      b.predV[dst] = make_double2(vx, vy);
+    if (S.nsec > 0 && tag_type(tg) == 1 && mark_mirrors(S, b, itime, my, p, dst)) {
+      b.predV[dst] = make_double2(vx, vy);   // the fields this row's mirror rows copy
       b.predVl[dst] = make_double2(lx, ly);
       b.predRho[dst] = rp;
       b.predRl[dst] = rl;
@@ -365,7 +398,7 @@ __global__ __launch_bounds__(BLK) void k_reorder(int n, StepBufs b, const DevSce
   const double pr = P / (rp * rp);
   b.mA[dst] = make_float4((float)vx, (float)vy, (float)lx, (float)ly);
   b.mB[dst] = make_float4((float)rp, (float)pr, rt.w, __uint_as_float(tg));
-  if (S.nsec > 0 && tag_type(tg) == 1) {
+  if (S.nsec > 0 && tag_type(tg) == 1 && mark_mirrors(S, b, itime, my, p, dst)) {
     b.predV[dst] = make_double2(vx, vy);
     b.predVl[dst] = make_double2(lx, ly);
     b.predRho[dst] = rp;
@@ -380,49 +413,42 @@ __global__ __launch_bounds__(BLK) void k_reorder(int n, StepBufs b, const DevSce
 // derivatives.c:3193-4192 appends, after the particle rows, the mirror image of
 // every fluid row that passes a section's test, section by section and in index
 // order; the search ranks these rows among the K nearest (derivatives.c:1069-1085)
-// and the pair filter drops them (Search.cc:33).  Here they are rebuilt each step
-// after the sort: a test per (section, fluid row), a scan that gives each passing
-// test its mirror index in the reference's order, the rows written by index, and a
-// cell list of their own, which the pair kernels read only to count candidates
-// in range and to rank keys.
+// and the pair filter drops them (Search.cc:33).  Here they are rebuilt each step:
+// the sort (k_reorder) sets one bit per passing test, in words per section and 32
+// fluid rows, and lists the rows with any; a scan over the words' popcounts gives
+// each set bit its mirror index in the reference's order; the listed rows write
+// their mirror rows by index; the mirror rows get a cell list of their own, which
+// the pair kernels read only to count candidates in range and to rank keys.
 
-// Section tests of the fluid rows at their positions of this step.
-__global__ __launch_bounds__(BLK) void k_ghost_flags(int n, StepBufs b, const DevScene* __restrict__ sc) {
-  const int d = blockIdx.x * BLK + threadIdx.x;
-  if (d >= n) return;
-  const uint32_t tg = b.tagA[d];
-  if (tag_type(tg) != 1) return;
-  const DevScene& S = sc[b.sceneOf[d]];
-  if (S.nsec == 0) return;
-  const int i = (int)tag_id(tg) - S.poff;
-  double lbp, rbp;
-  boundary_lines(S, S.t0 + b.ctl[0] * S.dt, lbp, rbp);
-  const double2 x = b.posA[d];
-  for (int k = 0; k < S.nsec; ++k) {
-    double2 g, v = make_double2(0.0, 0.0), lv = v;
-    b.gflag[S.goff + k * S.nf + i] =
-        mirror_row(S.sec[k], lbp, rbp, S.flat_bottom, S.depth, S.dx, S.h, x, g, v, lv) ? 1 : 0;
-  }
+// Popcount of each word of section-test bits (the scan's input); the words move to
+// the step's copy and are cleared for the next step's sort.
+__global__ __launch_bounds__(BLK) void k_ghost_count(StepBufs b) {
+  const int w = blockIdx.x * BLK + threadIdx.x;
+  if (w >= b.wtot) return;
+  const uint32_t v = b.gbits[w];
+  b.gbits2[w] = v;
+  b.gbits[w] = 0u;
+  b.gwcnt[w] = __popc(v);
 }
 
-// The mirror rows of the passing tests, at their scanned indices.
-__global__ __launch_bounds__(BLK) void k_ghost_emit(int n, StepBufs b, const DevScene* __restrict__ sc) {
-  const int d = blockIdx.x * BLK + threadIdx.x;
-  if (d >= n) return;
+// The mirror rows of the listed fluid rows, at their indices in the reference's
+// order: section k's rows come after those of sections < k, and within a section
+// in fluid index order (the set bits below the row's).
+__device__ void emit_mirrors(const StepBufs& b, const DevScene* __restrict__ sc, int d) {
   const uint32_t tg = b.tagA[d];
-  if (tag_type(tg) != 1) return;
   const int s = b.sceneOf[d];
   const DevScene& S = sc[s];
-  if (S.nsec == 0) return;
   const int i = (int)tag_id(tg) - S.poff;
   double lbp, rbp;
   boundary_lines(S, S.t0 + b.ctl[0] * S.dt, lbp, rbp);
   const double2 x = b.posA[d];
-  const int first = b.gstart[S.goff];
+  const int first = b.gwstart[S.gw];
+  const uint32_t below = (1u << (i & 31)) - 1u;
   for (int k = 0; k < S.nsec; ++k) {
-    const int e = S.goff + k * S.nf + i;
-    const int j = b.gstart[e];
-    if (b.gstart[e + 1] == j) continue;
+    const int w = S.gw + k * S.nwords + (i >> 5);
+    const uint32_t bits = b.gbits2[w];
+    if (!((bits >> (i & 31)) & 1u)) continue;
+    const int j = b.gwstart[w] + __popc(bits & below);
     double2 g = x, v = b.predV[d], lv = b.predVl[d];
     (void)mirror_row(S.sec[k], lbp, rbp, S.flat_bottom, S.depth, S.dx, S.h, x, g, v, lv);
     b.gpos[j] = g;
@@ -438,11 +464,18 @@ __global__ __launch_bounds__(BLK) void k_ghost_emit(int n, StepBufs b, const Dev
   }
 }
 
+// The listed rows' mirror rows (a grid-stride walk over the list).
+__global__ __launch_bounds__(BLK) void k_ghost_emit(StepBufs b, const DevScene* __restrict__ sc) {
+  const int nl = *b.gcandN;
+  for (int c = blockIdx.x * BLK + threadIdx.x; c < nl; c += gridDim.x * BLK) emit_mirrors(b, sc, b.gcand[c]);
+}
+
 // Cell and in-cell rank of each mirror row (order within a cell is free: the rows
 // only enter counts and key ranks).
 __global__ __launch_bounds__(BLK) void k_ghost_bin(StepBufs b, const DevScene* __restrict__ sc) {
   const int j = blockIdx.x * BLK + threadIdx.x;
-  if (j >= b.gtot || j >= b.gstart[b.gtot]) return;
+  if (j == 0) *b.gcandN = 0;   // the list is read; the next sort starts it again
+  if (j >= b.gtot || j >= b.gwstart[b.wtot]) return;
   const DevScene& S = sc[b.gscene[j]];
   const double2 g = b.gpos[j];
   int cx, cy;
@@ -454,7 +487,7 @@ __global__ __launch_bounds__(BLK) void k_ghost_bin(StepBufs b, const DevScene* _
 
 __global__ __launch_bounds__(BLK) void k_ghost_scatter(StepBufs b) {
   const int j = blockIdx.x * BLK + threadIdx.x;
-  if (j >= b.gtot || j >= b.gstart[b.gtot]) return;
+  if (j >= b.gtot || j >= b.gwstart[b.wtot]) return;
   const int t = b.gcellStart[b.gkey[j]] + b.grank[j];
   b.gposS[t] = b.gpos[j];
   b.gidS[t] = b.gid[j];
@@ -1750,6 +1783,7 @@ struct uph_handle {
   std::vector<double2> deck_pos, deck_vel;
   std::vector<int> wall_kind;
   std::vector<double> wall_ref;
+  int gfluid = 0;   // fluid rows of the scenes with mirror rows (the most rows the sort lists)
 };
 
 static void fill_scene(DevScene& S, const uph_scene& u, double cell_factor) {
@@ -1828,12 +1862,16 @@ static int alloc_bufs(StepBufs& b, int n, int ncells, int nblocks_scan, Err& err
   return 0;
 }
 
-// Mirror-row buffers for gtot section tests (see StepBufs), zeroed.
-static int alloc_ghost_bufs(StepBufs& b, int n, int ncells, int gtot, int gnb, Err& err) {
+// Mirror-row buffers for at most gtot rows and wtot words of test bits (see
+// StepBufs), zeroed.
+static int alloc_ghost_bufs(StepBufs& b, int n, int ncells, int gtot, int wtot, Err& err) {
   b.gtot = gtot;
-  const size_t g = (size_t)gtot;
-  if (dalloc(&b.gflag, g, err) || dalloc(&b.gstart, g + 1, err) || dalloc(&b.gpart, (size_t)gnb, err) ||
-      dalloc(&b.gpos, g, err) || dalloc(&b.gid, g, err) || dalloc(&b.gscene, g, err) || dalloc(&b.gkey, g, err) ||
+  b.wtot = wtot;
+  const size_t g = (size_t)gtot, w = (size_t)wtot;
+  const size_t wnb = (w + SCAN_TILE - 1) / SCAN_TILE;
+  if (dalloc(&b.gbits, w, err) || dalloc(&b.gbits2, w, err) || dalloc(&b.gwcnt, w, err) ||
+      dalloc(&b.gwstart, w + 1, err) || dalloc(&b.gpart, wnb, err) || dalloc(&b.gcand, (size_t)n, err) ||
+      dalloc(&b.gcandN, 1, err) || dalloc(&b.gpos, g, err) || dalloc(&b.gid, g, err) || dalloc(&b.gscene, g, err) || dalloc(&b.gkey, g, err) ||
       dalloc(&b.grank, g, err) || dalloc(&b.gcellCount, (size_t)ncells, err) ||
       dalloc(&b.gcellStart, (size_t)ncells + 1, err) || dalloc(&b.gposS, g, err) || dalloc(&b.gidS, g, err) ||
       dalloc(&b.gvel, g, err) || dalloc(&b.glvel, g, err) || dalloc(&b.grho, g, err) || dalloc(&b.grl, g, err) ||
@@ -1841,8 +1879,9 @@ static int alloc_ghost_bufs(StepBufs& b, int n, int ncells, int gtot, int gnb, E
       dalloc(&b.predVl, n, err) || dalloc(&b.predRho, n, err) || dalloc(&b.predRl, n, err) ||
       dalloc(&b.predP, n, err) || dalloc(&b.predPl, n, err))
     return -1;
-  if (hipMemset(b.gflag, 0, g * sizeof(int)) != hipSuccess ||
-      hipMemset(b.gstart, 0, (g + 1) * sizeof(int)) != hipSuccess ||
+  if (hipMemset(b.gbits, 0, w * sizeof(uint32_t)) != hipSuccess ||
+      hipMemset(b.gwstart, 0, (w + 1) * sizeof(int)) != hipSuccess ||
+      hipMemset(b.gcandN, 0, sizeof(int)) != hipSuccess ||
       hipMemset(b.gcellCount, 0, (size_t)ncells * sizeof(int)) != hipSuccess ||
       hipMemset(b.gcellStart, 0, ((size_t)ncells + 1) * sizeof(int)) != hipSuccess) {
     err.msg = "mirror buffers: init failed";
@@ -1856,7 +1895,7 @@ static void free_bufs(StepBufs& b) {
                   b.rtB, b.tagB, b.keys, b.rank, b.tmpIdx, b.cellStart, b.cellCount, b.partial,
                   b.sceneOf, b.kthD2, b.kthId, b.deckPos, b.deckVel, b.deckLvel, b.massD, b.ctl,
                   b.dbgRates, b.presA, b.presB, b.deckP, b.rt2B, b.dvB, b.mC, b.deckRl, b.deckPl,
-                  b.wallKind, b.wallRef, b.gflag, b.gstart, b.gpart, b.gpos, b.gid, b.gscene, b.gkey,
+                  b.wallKind, b.wallRef, b.gbits, b.gbits2, b.gwcnt, b.gwstart, b.gpart, b.gcand, b.gcandN, b.gpos, b.gid, b.gscene, b.gkey,
                   b.grank, b.gcellCount, b.gcellStart, b.gposS, b.gidS, b.gvel, b.glvel, b.grho, b.grl,
                   b.gP, b.gPl, b.gmass, b.predV, b.predVl, b.predRho, b.predRl, b.predP, b.predPl};
   for (void* p : ptrs)
@@ -1901,14 +1940,15 @@ static int launch_pairs(uph_handle* h, hipStream_t st, hipEvent_t pb) {
 }
 
 // The mirror rows of this step and their cell list (scenes with a virtualType that
-// has sections only): tests, scan, rows, bin, scan, scatter.
+// has sections only; the sort has set the test bits and listed the rows): counts,
+// scan, rows, bin, scan, scatter.
 static int launch_ghosts(uph_handle* h, hipStream_t st) {
   StepBufs& b = h->b;
-  const int gnb = (b.gtot + SCAN_TILE - 1) / SCAN_TILE;
-  hipLaunchKernelGGL(k_ghost_flags, dim3(grid_of(h->n)), dim3(BLK), 0, st, h->n, b, h->dsc);
-  hipLaunchKernelGGL(k_scan_reduce, dim3(gnb), dim3(SCAN_BLK), 0, st, b.gtot, b.gflag, b.gpart, (int*)nullptr);
-  hipLaunchKernelGGL(k_scan_down, dim3(gnb), dim3(SCAN_BLK), 0, st, b.gtot, b.gflag, b.gpart, b.gstart);
-  hipLaunchKernelGGL(k_ghost_emit, dim3(grid_of(h->n)), dim3(BLK), 0, st, h->n, b, h->dsc);
+  const int wnb = (b.wtot + SCAN_TILE - 1) / SCAN_TILE;
+  hipLaunchKernelGGL(k_ghost_count, dim3(grid_of(b.wtot)), dim3(BLK), 0, st, b);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(wnb), dim3(SCAN_BLK), 0, st, b.wtot, b.gwcnt, b.gpart, (int*)nullptr);
+  hipLaunchKernelGGL(k_scan_down, dim3(wnb), dim3(SCAN_BLK), 0, st, b.wtot, b.gwcnt, b.gpart, b.gwstart);
+  hipLaunchKernelGGL(k_ghost_emit, dim3(std::min(grid_of(h->gfluid), 256)), dim3(BLK), 0, st, b, h->dsc);
   hipLaunchKernelGGL(k_ghost_bin, dim3(grid_of(b.gtot)), dim3(BLK), 0, st, b, h->dsc);
   hipLaunchKernelGGL(k_scan_reduce, dim3(h->cs.nb), dim3(SCAN_BLK), 0, st, h->ncells, b.gcellCount, b.partial,
                      (int*)nullptr);
@@ -1950,7 +1990,7 @@ extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts*
   if (!(h->opts.cell_factor >= 1.0) || h->opts.cell_factor > 4.0) h->opts.cell_factor = 1.0;
   h->nscenes = n_scenes;
   h->hs.resize(n_scenes);
-  long n = 0, nc = 0, gtot = 0;
+  long n = 0, nc = 0, gtot = 0, wtot = 0, gfluid = 0;
   for (int s = 0; s < n_scenes; ++s) {
     const uph_scene& u = scenes[s];
     const int np = u.n_fluid + u.n_fixed + u.n_moving;
@@ -1991,9 +2031,12 @@ extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts*
     S.pb_gid = (int)n + (u.n_fluid + u.n_fixed + u.n_moving - 1);
     // mirror rows: one test per section and fluid row, ids after the scene's rows
     S.nsec = u.n_fluid > 0 ? mirror_sections(u.vtype, S.sec) : 0;
-    S.goff = (int)gtot;
+    S.nwords = (u.n_fluid + 31) / 32;
+    S.gw = (int)wtot;
     S.gbase = (int)n + np;
     gtot += (long)S.nsec * u.n_fluid;
+    wtot += (long)S.nsec * S.nwords;
+    if (S.nsec > 0) gfluid += u.n_fluid;
     if ((long)S.gbase + (long)S.nsec * u.n_fluid >= (long)UPH_ID_MASK) {
       g_create_err = "uph_create: too many particles and mirror rows";
       delete h;
@@ -2029,9 +2072,8 @@ extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts*
     return fail();
   }
   if (alloc_bufs(h->b, h->n, h->ncells, h->cs.nb, err, false)) return fail();
-  if (gtot > 0 &&
-      alloc_ghost_bufs(h->b, h->n, h->ncells, (int)gtot, (int)((gtot + SCAN_TILE - 1) / SCAN_TILE), err))
-    return fail();
+  h->gfluid = (int)gfluid;
+  if (gtot > 0 && alloc_ghost_bufs(h->b, h->n, h->ncells, (int)gtot, (int)wtot, err)) return fail();
   if (h->opts.keep_rates && dalloc(&h->b.dbgRates, (h->mode == UPH_MODE_REF ? 4 : 2) * (size_t)h->n, err))
     return fail();
   if (dalloc(&h->dsc, n_scenes, err)) return fail();
@@ -2233,8 +2275,8 @@ extern "C" int uph_get_ghosts(uph_t* h, int scene, int cap, double* pos, double*
   const DevScene& S = h->hs[scene];
   if (S.nsec == 0) return 0;
   int se[2];
-  HIPCHK(hipMemcpy(&se[0], h->b.gstart + S.goff, sizeof(int), hipMemcpyDeviceToHost), h->err);
-  HIPCHK(hipMemcpy(&se[1], h->b.gstart + S.goff + S.nsec * S.nf, sizeof(int), hipMemcpyDeviceToHost), h->err);
+  HIPCHK(hipMemcpy(&se[0], h->b.gwstart + S.gw, sizeof(int), hipMemcpyDeviceToHost), h->err);
+  HIPCHK(hipMemcpy(&se[1], h->b.gwstart + S.gw + S.nsec * S.nwords, sizeof(int), hipMemcpyDeviceToHost), h->err);
   const int cnt = se[1] - se[0];
   if (cnt < 0 || cnt > cap) {
     h->err.msg = "uph_get_ghosts: " + std::to_string(cnt) + " mirror rows, capacity " + std::to_string(cap);

@@ -41,11 +41,12 @@ struct DevScene {
   // lid / gate speed, the gate's stop time, the sloshing tank's right-wall offset
   double lid_u, t_stop, flat_bottom;
   // mirror ("virtual") particles: the section codes of the scene's virtualType in
-  // order (UPH_GS_*), the first entry of its section tests in the mirror flag array
-  // (nsec x nf entries, section-major), the id of its first mirror row (after its
-  // np rows); a sloshing moving set whose loop zeroes LeftBoundaryPosition; the
-  // boundary positions where the motion does not set them; LocalDepth, dx
-  int nsec, sec[5], goff, gbase, slosh_halt;
+  // order (UPH_GS_*); its section tests as bits, one word per 32 fluid rows
+  // (nwords) and section, section-major from word gw of the step's bit array; the id
+  // of its first mirror row (after its np rows); a sloshing moving set whose loop
+  // zeroes LeftBoundaryPosition; the boundary positions where the motion does not
+  // set them; LocalDepth, dx
+  int nsec, sec[5], gw, nwords, gbase, slosh_halt;
   double lbp0, rbp0, depth, dx;
 };
 
@@ -169,12 +170,14 @@ __host__ __device__ inline bool mirror_row(int sec, double lbp, double rbp, doub
 __host__ __device__ inline void boundary_lines(const DevScene& S, double t, double& lbp, double& rbp) {
   lbp = S.lbp0;
   rbp = S.rbp0;
-  const double L = 0.5 * S.m_amp * (1.0 - cos(S.m_omega * t));
-  if (S.motion == 1) {
-    lbp = L;
-  } else if (S.motion == 5) {
-    lbp = S.slosh_halt ? 0.0 : L;
-    rbp = S.flat_bottom + L;
+  if (S.motion == 1 || S.motion == 5) {
+    const double L = 0.5 * S.m_amp * (1.0 - cos(S.m_omega * t));
+    if (S.motion == 1) {
+      lbp = L;
+    } else {
+      lbp = S.slosh_halt ? 0.0 : L;
+      rbp = S.flat_bottom + L;
+    }
   } else if (S.motion == 0 || S.motion == 2) {
     lbp = 0.0;
   }
