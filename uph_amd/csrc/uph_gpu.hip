@@ -309,6 +309,9 @@ __device__ __forceinline__ bool mark_mirrors(const DevScene& S, const StepBufs& 
 // P + dP, localVelocity = v + dt/2 at + eps dv; at the first step they are
 // deckRl, deckPl (rho0 and 0 from uph_create, input.c:109-110) and the deck's
 // localVelocity.
+// MIRRORS: some scene has mirror rows (the section tests run here); without them the
+// kernel is built without that code and keeps its smaller register allocation.
+template <bool MIRRORS>
 __global__ __launch_bounds__(BLK) void k_reorder(int n, StepBufs b, const DevScene* __restrict__ sc, int search_only) {
   const int d = blockIdx.x * BLK + threadIdx.x;
   if (d >= n) return;
@@ -384,7 +387,7 @@ __global__ __launch_bounds__(BLK) void k_reorder(int n, StepBufs b, const DevSce
     b.mA[dst] = make_float4((float)vx, (float)vy, (float)lx, (float)ly);
     b.mB[dst] = make_float4((float)(rp - S.rho0), (float)Pp, rt.w, __uint_as_float(tg));
     b.mC[dst] = make_float2((float)(rl - S.rho0), (float)pl);
-    if (S.nsec > 0 && tag_type(tg) == 1 && mark_mirrors(S, b, itime, my, p, dst)) {
+    if (MIRRORS && S.nsec > 0 && tag_type(tg) == 1 && mark_mirrors(S, b, itime, my, p, dst)) {
       b.predV[dst] = make_double2(vx, vy);   // the fields this row's mirror rows copy
       b.predVl[dst] = make_double2(lx, ly);
       b.predRho[dst] = rp;
@@ -398,7 +401,7 @@ __global__ __launch_bounds__(BLK) void k_reorder(int n, StepBufs b, const DevSce
   const double pr = P / (rp * rp);
   b.mA[dst] = make_float4((float)vx, (float)vy, (float)lx, (float)ly);
   b.mB[dst] = make_float4((float)rp, (float)pr, rt.w, __uint_as_float(tg));
-  if (S.nsec > 0 && tag_type(tg) == 1 && mark_mirrors(S, b, itime, my, p, dst)) {
+  if (MIRRORS && S.nsec > 0 && tag_type(tg) == 1 && mark_mirrors(S, b, itime, my, p, dst)) {
     b.predV[dst] = make_double2(vx, vy);
     b.predVl[dst] = make_double2(lx, ly);
     b.predRho[dst] = rp;
@@ -1912,7 +1915,10 @@ static int launch_sort(hipStream_t st, int n, int ncells, int nb, StepBufs& b, c
                      step_ctl ? b.ctl : (int*)nullptr);
   hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(SCAN_BLK), 0, st, ncells, b.cellCount, b.partial, b.cellStart);
   hipLaunchKernelGGL(k_scatter, dim3(grid_of(n)), dim3(BLK), 0, st, n, b.keys, b.rank, b.cellStart, b.tmpIdx);
-  hipLaunchKernelGGL(k_reorder, dim3(grid_of(n)), dim3(BLK), 0, st, n, b, dsc, search_only);
+  if (b.gtot > 0 && !search_only)
+    hipLaunchKernelGGL(k_reorder<true>, dim3(grid_of(n)), dim3(BLK), 0, st, n, b, dsc, search_only);
+  else
+    hipLaunchKernelGGL(k_reorder<false>, dim3(grid_of(n)), dim3(BLK), 0, st, n, b, dsc, search_only);
   HIPCHK(hipGetLastError(), err);
   return 0;
 }
