@@ -199,3 +199,17 @@ def test_graph_replay_with_mirror_rows():
             assert np.array_equal(se[key], sg[key]), key
         ge, gg = eager.ghosts(), graph.ghosts()
         assert all(np.array_equal(ge[key], gg[key]) for key in ge)
+
+
+def test_reference_scheme_with_mirror_rows():
+    """The reference's run-phase scheme ranks the same mirror rows: a tank with
+    virtualType 7 at h = 2 dx in ref mode, rates of steps 1-2 and 40 steps."""
+    sc = _tank(20, 16, vtype=7, h_factor=2.0, seed=8)
+    sc.mode = scenes.MODE_REF
+    sc.kappa, sc.mu, sc.eps, sc.alpha_av = 0.0015, 1e-3, 1.0, 0.0
+    for steps in (1, 2):
+        err, gg, go, *_ = _rates_and_ghosts(sc, f"ref mode vtype 7 tank, step {steps}", steps)
+        for k, e in err.items():
+            assert e < 1e-5, (k, e)
+        assert len(gg["x"]) == len(go["x"]) > 0
+    g, sg, gg = _run_both(sc, 40, "ref mode vtype 7 tank, 40 steps")

@@ -227,17 +227,15 @@ __device__ __forceinline__ void for_candidates(const DevScene& S, double2 p, con
   }
 }
 
-// Kahan-compensated FP32 sum (c carries the rounding error of the last add).
+// Sum of FP32 pair terms accumulated in FP64: each term enters exactly, so the only
+// rounding is FP64's, and an add costs a conversion and one FP64 add (FP64 issues at
+// the rate of unpacked FP32 here) where compensated FP32 summation costs four FP32
+// operations, in the same two registers.
 struct CSum {
-  float s, c;
-  __device__ __forceinline__ CSum() : s(0.f), c(0.f) {}
-  __device__ __forceinline__ void add(float x) {
-    const float y = x - c;
-    const float t = s + y;
-    c = (t - s) - y;
-    s = t;
-  }
-  __device__ __forceinline__ float val() const { return s - c; }
+  double s;
+  __device__ __forceinline__ CSum() : s(0.0) {}
+  __device__ __forceinline__ void add(float x) { s += (double)x; }
+  __device__ __forceinline__ float val() const { return (float)s; }
 };
 
 // Load of base[idx] through a 32-bit byte offset from a uniform base (lets the
