@@ -1457,8 +1457,10 @@ __device__ __forceinline__ void compact_pairs(const DevScene& S, const StepBufs&
 // the set bits, so every lane spends its physics on in-range pairs only, with the
 // next pair's loads issued before the current pair's math.  A row range longer
 // than 64 slots, or more than K candidates in range, raises the step's overflow
-// flag and the exact direct kernel recomputes the step.
-template <int PB, int KT, int MODE>
+// flag and the exact direct kernel recomputes the step.  ONE: the handle holds a
+// single scene, so its constants are the same for every lane and stay in scalar
+// registers (101 VGPRs against 109: five wavefronts per SIMD instead of four).
+template <int PB, int KT, int MODE, bool ONE>
 __global__ __launch_bounds__(PB) void k_pair_compact(int n, StepBufs b, const DevScene* __restrict__ sc) {
   __shared__ KthLds lds[PB / 64];
   const int p = xcd_block() * PB + threadIdx.x;
@@ -1466,7 +1468,7 @@ __global__ __launch_bounds__(PB) void k_pair_compact(int n, StepBufs b, const De
   const int pp = live ? p : n - 1;
   const uint32_t tg = b.tagA[pp];
   const bool fluid = live && tag_type(tg) == 1;
-  const int sidx = b.sceneOf[pp];
+  const int sidx = ONE ? 0 : b.sceneOf[pp];
   const DevScene& S = sc[sidx];
   const int itime = b.ctl[0];
   const double2 xp = b.posA[pp];
@@ -1932,12 +1934,21 @@ static int launch_pairs(uph_handle* h, hipStream_t st, hipEvent_t pb) {
   } else {
     const int pbk = h->opts.pair_block;
     const int g = (((h->n + pbk - 1) / pbk + 7) / 8) * 8;
-    if (pbk == 128)
-      hipLaunchKernelGGL((k_pair_compact<128, KT, MODE>), dim3(g), dim3(128), 0, st, h->n, h->b, h->dsc);
-    else if (pbk == 512)
-      hipLaunchKernelGGL((k_pair_compact<512, KT, MODE>), dim3(g), dim3(512), 0, st, h->n, h->b, h->dsc);
-    else
-      hipLaunchKernelGGL((k_pair_compact<256, KT, MODE>), dim3(g), dim3(256), 0, st, h->n, h->b, h->dsc);
+    if (h->nscenes == 1) {
+      if (pbk == 128)
+        hipLaunchKernelGGL((k_pair_compact<128, KT, MODE, true>), dim3(g), dim3(128), 0, st, h->n, h->b, h->dsc);
+      else if (pbk == 512)
+        hipLaunchKernelGGL((k_pair_compact<512, KT, MODE, true>), dim3(g), dim3(512), 0, st, h->n, h->b, h->dsc);
+      else
+        hipLaunchKernelGGL((k_pair_compact<256, KT, MODE, true>), dim3(g), dim3(256), 0, st, h->n, h->b, h->dsc);
+    } else {
+      if (pbk == 128)
+        hipLaunchKernelGGL((k_pair_compact<128, KT, MODE, false>), dim3(g), dim3(128), 0, st, h->n, h->b, h->dsc);
+      else if (pbk == 512)
+        hipLaunchKernelGGL((k_pair_compact<512, KT, MODE, false>), dim3(g), dim3(512), 0, st, h->n, h->b, h->dsc);
+      else
+        hipLaunchKernelGGL((k_pair_compact<256, KT, MODE, false>), dim3(g), dim3(256), 0, st, h->n, h->b, h->dsc);
+    }
   }
   if (pb) HIPCHK(hipEventRecord(pb, st), h->err);
   const int gfix = std::min(grid8_of(h->n), FIX_GRID);
