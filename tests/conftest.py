@@ -30,16 +30,21 @@ def oracle_lib():
     return oracle
 
 
-def oracle_spread(sc, steps, so):
+def oracle_spread(sc, steps, so, amplitudes=(1e-12, 1e-10, 1e-8)):
     """Largest relative density and position (in dx) differences, after `steps`, of the
-    oracle from its own run `so` when the fluid positions start moved by up to
-    1e-12 dx: the conditioning of a trajectory whose capped neighbour sets change."""
+    oracle from its own run `so` when the fluid positions start moved by up to each of
+    `amplitudes` (times dx), one run each: the conditioning of a trajectory whose
+    capped neighbour sets change (each flip of a neighbour in or out of the capped
+    set moves the rates by a kernel weight, whatever the size of the perturbation)."""
     from oracle.oracle import Sim
-    sp = sc.copy()
     nf = sc.n_fluid
-    sp.pos[:nf] += np.random.default_rng(7).uniform(-1e-12, 1e-12, (nf, 2)) * sc.dx
-    o = Sim(sp, nthreads=0)
-    o.step(steps)
-    st = o.state()
-    return (float(np.max(np.abs(st["rho"][:nf] - so["rho"][:nf]) / so["rho"][:nf])),
-            float(np.abs(st["x"][:nf] - so["x"][:nf]).max()) / sc.dx)
+    dr, dx = 0.0, 0.0
+    for k, a in enumerate(amplitudes):
+        sp = sc.copy()
+        sp.pos[:nf] += np.random.default_rng(7 + k).uniform(-a, a, (nf, 2)) * sc.dx
+        o = Sim(sp, nthreads=0)
+        o.step(steps)
+        st = o.state()
+        dr = max(dr, float(np.max(np.abs(st["rho"][:nf] - so["rho"][:nf]) / so["rho"][:nf])))
+        dx = max(dx, float(np.abs(st["x"][:nf] - so["x"][:nf]).max()) / sc.dx)
+    return dr, dx

@@ -1457,18 +1457,21 @@ __device__ __forceinline__ void compact_pairs(const DevScene& S, const StepBufs&
 // the set bits, so every lane spends its physics on in-range pairs only, with the
 // next pair's loads issued before the current pair's math.  A row range longer
 // than 64 slots, or more than K candidates in range, raises the step's overflow
-// flag and the exact direct kernel recomputes the step.  ONE: the handle holds a
-// single scene, so its constants are the same for every lane and stay in scalar
-// registers (101 VGPRs against 109: five wavefronts per SIMD instead of four).
-template <int PB, int KT, int MODE, bool ONE>
-__global__ __launch_bounds__(PB) void k_pair_compact(int n, StepBufs b, const DevScene* __restrict__ sc) {
-  __shared__ KthLds lds[PB / 64];
-  const int p = xcd_block() * PB + threadIdx.x;
+// flag and the exact direct kernel recomputes the step.  UNI: the wavefront takes
+// its scene from its first lane, so the scene constants stay in scalar registers
+// (101 VGPRs against 109: five wavefronts per SIMD instead of four); a wavefront
+// whose slots belong to two scenes returns at once and k_pair_mixed runs it with
+// the scene per lane.  Every lane of the wavefront calls this.
+template <int KT, int MODE, bool UNI>
+__device__ __forceinline__ void compact_query(int n, int p, const StepBufs& b, const DevScene* __restrict__ sc,
+                                              KthLds& lds) {
   const bool live = p < n;
   const int pp = live ? p : n - 1;
   const uint32_t tg = b.tagA[pp];
   const bool fluid = live && tag_type(tg) == 1;
-  const int sidx = ONE ? 0 : b.sceneOf[pp];
+  const int slane = b.sceneOf[pp];
+  const int sidx = UNI ? __builtin_amdgcn_readfirstlane(slane) : slane;
+  if (UNI && __ballot(slane != sidx) != 0ull) return;   // a wavefront of two scenes
   const DevScene& S = sc[sidx];
   const int itime = b.ctl[0];
   const double2 xp = b.posA[pp];
@@ -1551,7 +1554,22 @@ __global__ __launch_bounds__(PB) void k_pair_compact(int n, StepBufs b, const De
     compact_pairs<KT, MODE>(S, b, p, tg, itime, xp, vn, mass, rb, m);
   };
   run();
-  wave_overflow<true>(b, sc, p, ovf, lds[threadIdx.x >> 6], sidx, xp, rb, rlf);
+  wave_overflow<true>(b, sc, p, ovf, lds, sidx, xp, rb, rlf);
+}
+
+template <int PB, int KT, int MODE>
+__global__ __launch_bounds__(PB) void k_pair_compact(int n, StepBufs b, const DevScene* __restrict__ sc) {
+  __shared__ KthLds lds[PB / 64];
+  compact_query<KT, MODE, true>(n, xcd_block() * PB + threadIdx.x, b, sc, lds[threadIdx.x >> 6]);
+}
+
+// The wavefronts whose slots belong to two scenes (listed by uph_create), with the
+// scene taken per lane: one wavefront per block.
+template <int KT, int MODE>
+__global__ __launch_bounds__(64) void k_pair_mixed(int n, StepBufs b, const DevScene* __restrict__ sc,
+                                                   const int* __restrict__ waves) {
+  __shared__ KthLds lds[1];
+  compact_query<KT, MODE, false>(n, waves[blockIdx.x] * 64 + (int)threadIdx.x, b, sc, lds[0]);
 }
 
 // ---------------------------------------------------------------- search
@@ -1789,6 +1807,10 @@ struct uph_handle {
   std::vector<int> wall_kind;
   std::vector<double> wall_ref;
   int gfluid = 0;   // fluid rows of the scenes with mirror rows (the most rows the sort lists)
+  // the wavefronts (64 consecutive slots) that hold two scenes' slots: the compacted pair
+  // kernel leaves them to k_pair_mixed
+  int nmixed = 0;
+  int* mixedWaves = nullptr;
 };
 
 static void fill_scene(DevScene& S, const uph_scene& u, double cell_factor) {
@@ -1934,21 +1956,15 @@ static int launch_pairs(uph_handle* h, hipStream_t st, hipEvent_t pb) {
   } else {
     const int pbk = h->opts.pair_block;
     const int g = (((h->n + pbk - 1) / pbk + 7) / 8) * 8;
-    if (h->nscenes == 1) {
-      if (pbk == 128)
-        hipLaunchKernelGGL((k_pair_compact<128, KT, MODE, true>), dim3(g), dim3(128), 0, st, h->n, h->b, h->dsc);
-      else if (pbk == 512)
-        hipLaunchKernelGGL((k_pair_compact<512, KT, MODE, true>), dim3(g), dim3(512), 0, st, h->n, h->b, h->dsc);
-      else
-        hipLaunchKernelGGL((k_pair_compact<256, KT, MODE, true>), dim3(g), dim3(256), 0, st, h->n, h->b, h->dsc);
-    } else {
-      if (pbk == 128)
-        hipLaunchKernelGGL((k_pair_compact<128, KT, MODE, false>), dim3(g), dim3(128), 0, st, h->n, h->b, h->dsc);
-      else if (pbk == 512)
-        hipLaunchKernelGGL((k_pair_compact<512, KT, MODE, false>), dim3(g), dim3(512), 0, st, h->n, h->b, h->dsc);
-      else
-        hipLaunchKernelGGL((k_pair_compact<256, KT, MODE, false>), dim3(g), dim3(256), 0, st, h->n, h->b, h->dsc);
-    }
+    if (pbk == 128)
+      hipLaunchKernelGGL((k_pair_compact<128, KT, MODE>), dim3(g), dim3(128), 0, st, h->n, h->b, h->dsc);
+    else if (pbk == 512)
+      hipLaunchKernelGGL((k_pair_compact<512, KT, MODE>), dim3(g), dim3(512), 0, st, h->n, h->b, h->dsc);
+    else
+      hipLaunchKernelGGL((k_pair_compact<256, KT, MODE>), dim3(g), dim3(256), 0, st, h->n, h->b, h->dsc);
+    if (h->nmixed > 0)
+      hipLaunchKernelGGL((k_pair_mixed<KT, MODE>), dim3(h->nmixed), dim3(64), 0, st, h->n, h->b, h->dsc,
+                         h->mixedWaves);
   }
   if (pb) HIPCHK(hipEventRecord(pb, st), h->err);
   const int gfix = std::min(grid8_of(h->n), FIX_GRID);
@@ -2080,6 +2096,7 @@ extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts*
     g_create_err = err.msg;
     free_bufs(h->b);
     if (h->dsc) (void)hipFree(h->dsc);
+    if (h->mixedWaves) (void)hipFree(h->mixedWaves);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return -1;
@@ -2127,6 +2144,23 @@ extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts*
       sceneOf[g] = s;
     }
     h->hs[s].slosh_halt = halted;   // uploaded with the scene table below
+  }
+  // wavefronts of the compacted pair kernel whose 64 slots (the last slot standing in
+  // for those past the end) belong to two scenes: scenes hold contiguous slot ranges,
+  // so the first and last slot decide
+  {
+    std::vector<int> mixed;
+    for (long w = 0; 64 * w < n; ++w) {
+      const long a = 64 * w, e = std::min(64 * w + 63, n - 1);
+      if (sceneOf[a] != sceneOf[e]) mixed.push_back((int)w);
+    }
+    h->nmixed = (int)mixed.size();
+    if (h->nmixed > 0 &&
+        (dalloc(&h->mixedWaves, mixed.size(), err) ||
+         hipMemcpy(h->mixedWaves, mixed.data(), mixed.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)) {
+      if (err.msg.empty()) err.msg = "uph_create: upload failed";
+      return fail();
+    }
   }
   if (hipMemcpy(h->b.deckPos, pos.data(), n * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess) {
     err.msg = "uph_create: upload failed";
@@ -2202,6 +2236,7 @@ extern "C" void uph_destroy(uph_t* h) {
   if (h->diag) (void)hipFree(h->diag);
   free_bufs(h->b);
   if (h->dsc) (void)hipFree(h->dsc);
+  if (h->mixedWaves) (void)hipFree(h->mixedWaves);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
 }
