@@ -155,8 +155,8 @@ def main():
                      "traffic": (traffic_from_profiles() if args.kernel == "cubic" and
                                  args.h_over_dx == 1.3 and args.scheme == "config" and not args.vtype
                                  else None),
-                     "kernel": f"k_pair_compact<256, {int(sc.kernel)}, {int(sc.mode)}> ({args.kernel}, "
-                               f"{args.scheme})",
+                     "kernel": f"k_pair_compact<256, {int(sc.kernel)}, {int(sc.mode)}, true> ({args.kernel}, "
+                               f"{args.scheme}, one scene)",
                      "kernel_ms": ms_pair},
     }
     if args.graph:

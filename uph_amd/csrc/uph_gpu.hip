@@ -1461,17 +1461,18 @@ __device__ __forceinline__ void compact_pairs(const DevScene& S, const StepBufs&
 // its scene from its first lane, so the scene constants stay in scalar registers
 // (101 VGPRs against 109: five wavefronts per SIMD instead of four); a wavefront
 // whose slots belong to two scenes returns at once and k_pair_mixed runs it with
-// the scene per lane.  Every lane of the wavefront calls this.
-template <int KT, int MODE, bool UNI>
+// the scene per lane.  ONE: the handle holds one scene (index 0, no check).  Every
+// lane of the wavefront calls this.
+template <int KT, int MODE, bool UNI, bool ONE = false>
 __device__ __forceinline__ void compact_query(int n, int p, const StepBufs& b, const DevScene* __restrict__ sc,
                                               KthLds& lds) {
   const bool live = p < n;
   const int pp = live ? p : n - 1;
   const uint32_t tg = b.tagA[pp];
   const bool fluid = live && tag_type(tg) == 1;
-  const int slane = b.sceneOf[pp];
-  const int sidx = UNI ? __builtin_amdgcn_readfirstlane(slane) : slane;
-  if (UNI && __ballot(slane != sidx) != 0ull) return;   // a wavefront of two scenes
+  const int slane = ONE ? 0 : b.sceneOf[pp];
+  const int sidx = ONE ? 0 : UNI ? __builtin_amdgcn_readfirstlane(slane) : slane;
+  if (!ONE && UNI && __ballot(slane != sidx) != 0ull) return;   // a wavefront of two scenes
   const DevScene& S = sc[sidx];
   const int itime = b.ctl[0];
   const double2 xp = b.posA[pp];
@@ -1557,10 +1558,10 @@ __device__ __forceinline__ void compact_query(int n, int p, const StepBufs& b, c
   wave_overflow<true>(b, sc, p, ovf, lds, sidx, xp, rb, rlf);
 }
 
-template <int PB, int KT, int MODE>
+template <int PB, int KT, int MODE, bool ONE>
 __global__ __launch_bounds__(PB) void k_pair_compact(int n, StepBufs b, const DevScene* __restrict__ sc) {
   __shared__ KthLds lds[PB / 64];
-  compact_query<KT, MODE, true>(n, xcd_block() * PB + threadIdx.x, b, sc, lds[threadIdx.x >> 6]);
+  compact_query<KT, MODE, true, ONE>(n, xcd_block() * PB + threadIdx.x, b, sc, lds[threadIdx.x >> 6]);
 }
 
 // The wavefronts whose slots belong to two scenes (listed by uph_create), with the
@@ -1956,12 +1957,21 @@ static int launch_pairs(uph_handle* h, hipStream_t st, hipEvent_t pb) {
   } else {
     const int pbk = h->opts.pair_block;
     const int g = (((h->n + pbk - 1) / pbk + 7) / 8) * 8;
-    if (pbk == 128)
-      hipLaunchKernelGGL((k_pair_compact<128, KT, MODE>), dim3(g), dim3(128), 0, st, h->n, h->b, h->dsc);
-    else if (pbk == 512)
-      hipLaunchKernelGGL((k_pair_compact<512, KT, MODE>), dim3(g), dim3(512), 0, st, h->n, h->b, h->dsc);
-    else
-      hipLaunchKernelGGL((k_pair_compact<256, KT, MODE>), dim3(g), dim3(256), 0, st, h->n, h->b, h->dsc);
+    if (h->nscenes == 1) {
+      if (pbk == 128)
+        hipLaunchKernelGGL((k_pair_compact<128, KT, MODE, true>), dim3(g), dim3(128), 0, st, h->n, h->b, h->dsc);
+      else if (pbk == 512)
+        hipLaunchKernelGGL((k_pair_compact<512, KT, MODE, true>), dim3(g), dim3(512), 0, st, h->n, h->b, h->dsc);
+      else
+        hipLaunchKernelGGL((k_pair_compact<256, KT, MODE, true>), dim3(g), dim3(256), 0, st, h->n, h->b, h->dsc);
+    } else {
+      if (pbk == 128)
+        hipLaunchKernelGGL((k_pair_compact<128, KT, MODE, false>), dim3(g), dim3(128), 0, st, h->n, h->b, h->dsc);
+      else if (pbk == 512)
+        hipLaunchKernelGGL((k_pair_compact<512, KT, MODE, false>), dim3(g), dim3(512), 0, st, h->n, h->b, h->dsc);
+      else
+        hipLaunchKernelGGL((k_pair_compact<256, KT, MODE, false>), dim3(g), dim3(256), 0, st, h->n, h->b, h->dsc);
+    }
     if (h->nmixed > 0)
       hipLaunchKernelGGL((k_pair_mixed<KT, MODE>), dim3(h->nmixed), dim3(64), 0, st, h->n, h->b, h->dsc,
                          h->mixedWaves);
