@@ -3129,8 +3129,9 @@ extern "C" const char* uph_search_last_error(void) { return g_search.err.msg.c_s
 // the fluid rows go to a device context (one scene), one step of the first-step
 // form runs with the caller's localVelocity, and the pair kernel's rates come
 // back into the System.  The boundary rows are reset to their state at the
-// first call (inputbp.c re-reads them from the decks every step) with the
-// piston kinematics of inputbp.c:194-207.
+// first call (inputbp.c re-reads them from the decks every step) and moved by the
+// kinematics of inputbp.c:192-283; the mirror rows of the System's virtualType
+// (derivatives.c:3193-4192) are written after them.
 
 #include "../../include/uph_system.h"
 
