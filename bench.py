@@ -143,7 +143,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 pair sums, fp64 integration",
+        "dtype": "fp32 pair terms summed in fp64, fp64 integration",
         "data": "synthetic (generated dam-break lattice)",
         "config": {"workload": "dam break 1.0x2.0 m column, dx=sqrt(2) mm, 4x3 m tank",
                    "fluid_particles": sc.n_fluid, "wall_particles": sc.n_boundary,

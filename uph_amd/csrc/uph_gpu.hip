@@ -573,9 +573,26 @@ __device__ void kth_key(const DevScene& S, int p, const double2* __restrict__ po
 #define GATHER_CHUNKS 6   // 64-slot chunks of candidates loaded per pass
 #endif
 
-struct KthLds {
+#define LANE_BUCKETS 16   // d2 buckets of the per-lane K-th key search
+#define LANE_CAP 12       // candidates the bucket holding rank K - 1 may have
+
+// A wavefront's LDS for its capped queries: the listed keys of the wavefront
+// search (k, 3 KB), or the per-lane search's bucket counts (two 16-bit counts a
+// word) and the slots of each lane's selected bucket, lane-major (l, 5 KB).  At
+// four wavefronts per SIMD (the VGPR limit) a CU holds 80 KB of them, half its LDS;
+// packing them into the list's 3 KB (8-bit counts, 16-bit row indices) measured
+// slower on capped steps (0.80 against 0.68 ms at 1 M) and no faster otherwise.
+struct KthList {
   double d2[KTH_CAP];
   uint32_t id[KTH_CAP];
+};
+struct LaneSel {
+  uint32_t cnt[LANE_BUCKETS / 2][64];
+  uint32_t slot[LANE_CAP][64];
+};
+union KthLds {
+  KthList k;
+  LaneSel l;
 };
 
 // LDS written by some lanes of a wavefront and then read by others: the
@@ -652,8 +669,8 @@ __device__ int wave_list(const DevScene& S, int q, double2 xq, const int* rb, co
       const unsigned long long m = __ballot(in);
       const int at = n + lanes_below(m);
       if (in && at < KTH_CAP) {
-        L.d2[at] = d2;
-        L.id[at] = tag_id(ts[u]);
+        L.k.d2[at] = d2;
+        L.k.id[at] = tag_id(ts[u]);
       }
       n += __popcll(m);
     }
@@ -709,8 +726,8 @@ __device__ int wave_list_ghosts(const DevScene& S, double2 xq, int n, const Step
     const unsigned long long m = __ballot(in);
     const int at = n + lanes_below(m);
     if (in && at < KTH_CAP) {
-      L.d2[at] = d2;
-      L.id[at] = id;
+      L.k.d2[at] = d2;
+      L.k.id[at] = id;
     }
     n += __popcll(m);
   }
@@ -752,8 +769,8 @@ __device__ void wave_kth_listed(const KthLds& L, int n, int K, uint32_t seed, do
   for (int s = 0; s < NS; ++s) {
     const int j = lane + 64 * s;
     act[s] = j < n;
-    kd[s] = act[s] ? (uint64_t)__double_as_longlong(L.d2[j]) : 0ull;
-    ki[s] = act[s] ? L.id[j] : 0u;
+    kd[s] = act[s] ? (uint64_t)__double_as_longlong(L.k.d2[j]) : 0ull;
+    ki[s] = act[s] ? L.k.id[j] : 0u;
   }
   int below = 0;   // keys ranked below every key still possible
   uint32_t rnd = seed * 2654435761u + 1u;
@@ -849,6 +866,80 @@ __device__ void wave_kth(const DevScene& S, int q, const double2* __restrict__ p
   wave_kth_of_list(S, q, pos, tag, cellStart, L, n, kd2, kid, exact);
 }
 
+// K-th key of query q (slot) at xq, candidate rows rb/rl, found by this lane alone:
+// a histogram of the in-range d2 over LANE_BUCKETS equal buckets of [0, cutoff)
+// finds the bucket that holds rank K - 1 (bucketing is monotone in d2, so every key
+// of a lower bucket is smaller); its slots are collected and the rank taken among
+// them by full (d2, id) keys.  Returns false, leaving the query to the wavefront
+// search, when that bucket holds more than LANE_CAP or the rows more than
+// CAND_MAX.  +inf when at most K are in range.
+__device__ bool lane_kth(const DevScene& S, int q, double2 xq, const int* rb, const int* rl,
+                         const double2* __restrict__ pos, const uint32_t* __restrict__ tag, LaneSel& L,
+                         double& kd2, uint32_t& kid) {
+  const int lane = threadIdx.x & 63;
+  kd2 = INFINITY;
+  kid = 0xffffffffu;
+  if (rl[0] + rl[1] + rl[2] > CAND_MAX) return false;   // 16-bit counts hold every bucket
+  const double cut = S.cutoff;
+  const double scale = (double)LANE_BUCKETS / cut;
+#pragma unroll
+  for (int w = 0; w < LANE_BUCKETS / 2; ++w) L.cnt[w][lane] = 0u;
+  int n = 0;
+#pragma unroll 1
+  for (int r = 0; r < 3; ++r)
+#pragma unroll 1
+    for (int t = rb[r]; t < rb[r] + rl[r]; ++t) {
+      const double d2 = dist2(xq, pos[t]);
+      if (t == q || !(d2 < cut)) continue;
+      const int bk = min((int)(d2 * scale), LANE_BUCKETS - 1);
+      L.cnt[bk >> 1][lane] += (bk & 1) ? 0x10000u : 1u;
+      ++n;
+    }
+  if (n <= S.K) return true;
+  int below = 0, kb = 0, m = 0;
+#pragma unroll 1
+  for (kb = 0; kb < LANE_BUCKETS; ++kb) {
+    const uint32_t w = L.cnt[kb >> 1][lane];
+    m = (int)((kb & 1) ? (w >> 16) : (w & 0xffffu));
+    if (below + m >= S.K) break;
+    below += m;
+  }
+  if (m > LANE_CAP) return false;
+  int j = 0;
+#pragma unroll 1
+  for (int r = 0; r < 3; ++r)
+#pragma unroll 1
+    for (int t = rb[r]; t < rb[r] + rl[r]; ++t) {
+      const double d2 = dist2(xq, pos[t]);
+      if (t == q || !(d2 < cut)) continue;
+      if (min((int)(d2 * scale), LANE_BUCKETS - 1) == kb) L.slot[j++][lane] = (uint32_t)t;
+    }
+  // rank K - 1 - below of the bucket: the minimum above the previous one, that many times
+  double td = -1.0;
+  uint32_t ti = 0u;
+#pragma unroll 1
+  for (int rr = 0; rr <= S.K - 1 - below; ++rr) {
+    double bd = INFINITY;
+    uint32_t bi = 0xffffffffu;
+#pragma unroll 1
+    for (int i = 0; i < m; ++i) {
+      const uint32_t t = L.slot[i][lane];
+      const double d2 = dist2(xq, pos[t]);
+      const uint32_t id = tag_id(tag[t]);
+      const bool above = d2 > td || (d2 == td && id > ti);
+      if (above && key_lt(d2, id, bd, bi)) {
+        bd = d2;
+        bi = id;
+      }
+    }
+    td = bd;
+    ti = bi;
+  }
+  kd2 = td;
+  kid = ti;
+  return true;
+}
+
 // The overflowing queries of this wavefront (more than K candidates in range,
 // or a row range the compacted kernel cannot hold; ovf set on their lanes): each
 // gets its K-th key, stamped with the step, and the step's flag makes the exact
@@ -863,6 +954,24 @@ __device__ void wave_overflow(const StepBufs& b, const DevScene* __restrict__ sc
   if (m == 0ull) return;
   const bool lead = (threadIdx.x & 63) == 0;
   if (lead) atomicOr(&b.ctl[1], 1);
+  // many overflowing lanes (the reference's smoothing length caps every query):
+  // each finds its own key at once; what it leaves goes to the wavefront search
+  if (ROWS && __popcll(m) >= 8) {
+    bool done = false;
+    if (ovf) {
+      const DevScene& S = sc[sidx];
+      double kd2;
+      uint32_t kid;
+      if (S.nsec == 0 && lane_kth(S, p, xp, rbp, rlp, b.posA, b.tagA, L.l, kd2, kid)) {
+        b.kthD2[p] = kd2;
+        b.kthId[p] = kid;
+        b.kthStep[p] = b.ctl[3];
+        done = true;
+      }
+    }
+    wave_lds_sync();   // the wavefront search below writes the same LDS
+    m = __ballot(ovf && !done);
+  }
 #pragma unroll 1
   while (m) {
     const int l = __ffsll((long long)m) - 1;
@@ -1627,7 +1736,7 @@ __global__ __launch_bounds__(BLK) void k_search_count(int nf_slots_unused, int n
     int m = 0;
     if (nin >= 0 && nin <= KTH_CAP) {
       // the listed keys this lane owns that the query keeps
-      for (int j = lane; j < nin; j += 64) m += search_keeps(S, idq, L.d2[j], L.id[j], kd2, kid) ? 1 : 0;
+      for (int j = lane; j < nin; j += 64) m += search_keeps(S, idq, L.k.d2[j], L.k.id[j], kd2, kid) ? 1 : 0;
       for (int o = 32; o > 0; o >>= 1) m += __shfl_xor(m, o, 64);
     } else if (lane == 0) {
       const double2 pq = b.posA[q];
@@ -1706,14 +1815,14 @@ __global__ __launch_bounds__(BLK) void k_search_emit(int n, StepBufs b, const De
       for (int j0 = 0; j0 < nin; j0 += 64) {
         const int j = j0 + lane;
         const bool own = j < nin;
-        const double dj = own ? L.d2[j] : 0.0;
-        const uint32_t jd = own ? L.id[j] : 0u;
+        const double dj = own ? L.k.d2[j] : 0.0;
+        const uint32_t jd = own ? L.k.id[j] : 0u;
         const bool keep = own && search_keeps(S, idq, dj, jd, kd2, kid);
         int rnk = 0;
 #pragma unroll 2
         for (int i = 0; i < nin; ++i) {
-          const double di = L.d2[i];
-          const uint32_t id = L.id[i];
+          const double di = L.k.d2[i];
+          const uint32_t id = L.k.id[i];
           rnk += (search_keeps(S, idq, di, id, kd2, kid) && key_lt(di, id, dj, jd)) ? 1 : 0;
         }
         if (keep) {
