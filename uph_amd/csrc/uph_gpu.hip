@@ -576,22 +576,23 @@ __device__ void kth_key(const DevScene& S, int p, const double2* __restrict__ po
 #define LANE_BUCKETS 16   // d2 buckets of the per-lane K-th key search
 #define LANE_CAP 12       // candidates the bucket holding rank K - 1 may have
 
-// A wavefront's LDS for its capped queries: the listed keys of the wavefront
-// search (k, 3 KB), or the per-lane search's bucket counts (two 16-bit counts a
-// word) and the slots of each lane's selected bucket, lane-major (l, 5 KB).  At
-// four wavefronts per SIMD (the VGPR limit) a CU holds 80 KB of them, half its LDS;
-// packing them into the list's 3 KB (8-bit counts, 16-bit row indices) measured
-// slower on capped steps (0.80 against 0.68 ms at 1 M) and no faster otherwise.
-struct KthList {
+// A wavefront's listed keys for the wavefront K-th key search (3 KB).
+struct KthLds {
   double d2[KTH_CAP];
   uint32_t id[KTH_CAP];
 };
+// The per-lane search's bucket counts (two 16-bit counts a word) and the slots of
+// each lane's selected bucket, lane-major (5 KB), sharing a wavefront's LDS with its
+// key list.  Only handles whose queries are expected to be capped build it (a
+// larger per-block LDS made the uncapped kernel 2 % slower); packing it into the
+// list's 3 KB (8-bit counts, 16-bit row indices) measured slower on capped steps
+// (0.80 against 0.70 ms at 1 M).
 struct LaneSel {
   uint32_t cnt[LANE_BUCKETS / 2][64];
   uint32_t slot[LANE_CAP][64];
 };
-union KthLds {
-  KthList k;
+union KthLaneLds {
+  KthLds k;
   LaneSel l;
 };
 
@@ -669,8 +670,8 @@ __device__ int wave_list(const DevScene& S, int q, double2 xq, const int* rb, co
       const unsigned long long m = __ballot(in);
       const int at = n + lanes_below(m);
       if (in && at < KTH_CAP) {
-        L.k.d2[at] = d2;
-        L.k.id[at] = tag_id(ts[u]);
+        L.d2[at] = d2;
+        L.id[at] = tag_id(ts[u]);
       }
       n += __popcll(m);
     }
@@ -726,8 +727,8 @@ __device__ int wave_list_ghosts(const DevScene& S, double2 xq, int n, const Step
     const unsigned long long m = __ballot(in);
     const int at = n + lanes_below(m);
     if (in && at < KTH_CAP) {
-      L.k.d2[at] = d2;
-      L.k.id[at] = id;
+      L.d2[at] = d2;
+      L.id[at] = id;
     }
     n += __popcll(m);
   }
@@ -769,8 +770,8 @@ __device__ void wave_kth_listed(const KthLds& L, int n, int K, uint32_t seed, do
   for (int s = 0; s < NS; ++s) {
     const int j = lane + 64 * s;
     act[s] = j < n;
-    kd[s] = act[s] ? (uint64_t)__double_as_longlong(L.k.d2[j]) : 0ull;
-    ki[s] = act[s] ? L.k.id[j] : 0u;
+    kd[s] = act[s] ? (uint64_t)__double_as_longlong(L.d2[j]) : 0ull;
+    ki[s] = act[s] ? L.id[j] : 0u;
   }
   int below = 0;   // keys ranked below every key still possible
   uint32_t rnd = seed * 2654435761u + 1u;
@@ -949,20 +950,20 @@ __device__ bool lane_kth(const DevScene& S, int q, double2 xq, const int* rb, co
 template <bool ROWS>
 __device__ void wave_overflow(const StepBufs& b, const DevScene* __restrict__ sc, int p, bool ovf, KthLds& L,
                               int sidx = 0, double2 xp = double2{}, const int* rbp = nullptr,
-                              const int* rlp = nullptr) {
+                              const int* rlp = nullptr, LaneSel* ls = nullptr) {
   unsigned long long m = __ballot(ovf);
   if (m == 0ull) return;
   const bool lead = (threadIdx.x & 63) == 0;
   if (lead) atomicOr(&b.ctl[1], 1);
   // many overflowing lanes (the reference's smoothing length caps every query):
   // each finds its own key at once; what it leaves goes to the wavefront search
-  if (ROWS && __popcll(m) >= 8) {
+  if (ROWS && ls != nullptr && __popcll(m) >= 8) {
     bool done = false;
     if (ovf) {
       const DevScene& S = sc[sidx];
       double kd2;
       uint32_t kid;
-      if (S.nsec == 0 && lane_kth(S, p, xp, rbp, rlp, b.posA, b.tagA, L.l, kd2, kid)) {
+      if (S.nsec == 0 && lane_kth(S, p, xp, rbp, rlp, b.posA, b.tagA, *ls, kd2, kid)) {
         b.kthD2[p] = kd2;
         b.kthId[p] = kid;
         b.kthStep[p] = b.ctl[3];
@@ -1574,7 +1575,7 @@ __device__ __forceinline__ void compact_pairs(const DevScene& S, const StepBufs&
 // lane of the wavefront calls this.
 template <int KT, int MODE, bool UNI, bool ONE = false>
 __device__ __forceinline__ void compact_query(int n, int p, const StepBufs& b, const DevScene* __restrict__ sc,
-                                              KthLds& lds) {
+                                              KthLds& lds, LaneSel* ls = nullptr) {
   const bool live = p < n;
   const int pp = live ? p : n - 1;
   const uint32_t tg = b.tagA[pp];
@@ -1664,13 +1665,21 @@ __device__ __forceinline__ void compact_query(int n, int p, const StepBufs& b, c
     compact_pairs<KT, MODE>(S, b, p, tg, itime, xp, vn, mass, rb, m);
   };
   run();
-  wave_overflow<true>(b, sc, p, ovf, lds, sidx, xp, rb, rlf);
+  wave_overflow<true>(b, sc, p, ovf, lds, sidx, xp, rb, rlf, ls);
 }
 
-template <int PB, int KT, int MODE, bool ONE>
+// LANE: the handle's queries are expected to be capped, so the kernel carries the
+// per-lane K-th key search and its LDS.
+template <int PB, int KT, int MODE, bool ONE, bool LANE>
 __global__ __launch_bounds__(PB) void k_pair_compact(int n, StepBufs b, const DevScene* __restrict__ sc) {
-  __shared__ KthLds lds[PB / 64];
-  compact_query<KT, MODE, true, ONE>(n, xcd_block() * PB + threadIdx.x, b, sc, lds[threadIdx.x >> 6]);
+  const int p = xcd_block() * PB + threadIdx.x;
+  if constexpr (LANE) {
+    __shared__ KthLaneLds lds[PB / 64];
+    compact_query<KT, MODE, true, ONE>(n, p, b, sc, lds[threadIdx.x >> 6].k, &lds[threadIdx.x >> 6].l);
+  } else {
+    __shared__ KthLds lds[PB / 64];
+    compact_query<KT, MODE, true, ONE>(n, p, b, sc, lds[threadIdx.x >> 6]);
+  }
 }
 
 // The wavefronts whose slots belong to two scenes (listed by uph_create), with the
@@ -1736,7 +1745,7 @@ __global__ __launch_bounds__(BLK) void k_search_count(int nf_slots_unused, int n
     int m = 0;
     if (nin >= 0 && nin <= KTH_CAP) {
       // the listed keys this lane owns that the query keeps
-      for (int j = lane; j < nin; j += 64) m += search_keeps(S, idq, L.k.d2[j], L.k.id[j], kd2, kid) ? 1 : 0;
+      for (int j = lane; j < nin; j += 64) m += search_keeps(S, idq, L.d2[j], L.id[j], kd2, kid) ? 1 : 0;
       for (int o = 32; o > 0; o >>= 1) m += __shfl_xor(m, o, 64);
     } else if (lane == 0) {
       const double2 pq = b.posA[q];
@@ -1815,14 +1824,14 @@ __global__ __launch_bounds__(BLK) void k_search_emit(int n, StepBufs b, const De
       for (int j0 = 0; j0 < nin; j0 += 64) {
         const int j = j0 + lane;
         const bool own = j < nin;
-        const double dj = own ? L.k.d2[j] : 0.0;
-        const uint32_t jd = own ? L.k.id[j] : 0u;
+        const double dj = own ? L.d2[j] : 0.0;
+        const uint32_t jd = own ? L.id[j] : 0u;
         const bool keep = own && search_keeps(S, idq, dj, jd, kd2, kid);
         int rnk = 0;
 #pragma unroll 2
         for (int i = 0; i < nin; ++i) {
-          const double di = L.k.d2[i];
-          const uint32_t id = L.k.id[i];
+          const double di = L.d2[i];
+          const uint32_t id = L.id[i];
           rnk += (search_keeps(S, idq, di, id, kd2, kid) && key_lt(di, id, dj, jd)) ? 1 : 0;
         }
         if (keep) {
@@ -1921,6 +1930,9 @@ struct uph_handle {
   // kernel leaves them to k_pair_mixed
   int nmixed = 0;
   int* mixedWaves = nullptr;
+  // some scene's queries are expected to be capped (4 pi (h/dx)^2 in range, a lattice's
+  // count, above 3/4 of K): the pair kernel carries the per-lane K-th key search
+  bool capped = false;
 };
 
 static void fill_scene(DevScene& S, const uph_scene& u, double cell_factor) {
@@ -2066,21 +2078,21 @@ static int launch_pairs(uph_handle* h, hipStream_t st, hipEvent_t pb) {
   } else {
     const int pbk = h->opts.pair_block;
     const int g = (((h->n + pbk - 1) / pbk + 7) / 8) * 8;
-    if (h->nscenes == 1) {
-      if (pbk == 128)
-        hipLaunchKernelGGL((k_pair_compact<128, KT, MODE, true>), dim3(g), dim3(128), 0, st, h->n, h->b, h->dsc);
-      else if (pbk == 512)
-        hipLaunchKernelGGL((k_pair_compact<512, KT, MODE, true>), dim3(g), dim3(512), 0, st, h->n, h->b, h->dsc);
-      else
-        hipLaunchKernelGGL((k_pair_compact<256, KT, MODE, true>), dim3(g), dim3(256), 0, st, h->n, h->b, h->dsc);
-    } else {
-      if (pbk == 128)
-        hipLaunchKernelGGL((k_pair_compact<128, KT, MODE, false>), dim3(g), dim3(128), 0, st, h->n, h->b, h->dsc);
-      else if (pbk == 512)
-        hipLaunchKernelGGL((k_pair_compact<512, KT, MODE, false>), dim3(g), dim3(512), 0, st, h->n, h->b, h->dsc);
-      else
-        hipLaunchKernelGGL((k_pair_compact<256, KT, MODE, false>), dim3(g), dim3(256), 0, st, h->n, h->b, h->dsc);
-    }
+    const bool one = h->nscenes == 1, lane = h->capped;
+#define UPH_COMPACT(PBV, ONEV, LANEV)                                                                        \
+  hipLaunchKernelGGL((k_pair_compact<PBV, KT, MODE, ONEV, LANEV>), dim3(g), dim3(PBV), 0, st, h->n, h->b, h->dsc)
+#define UPH_COMPACT_PB(ONEV, LANEV)              \
+  do {                                           \
+    if (pbk == 128) UPH_COMPACT(128, ONEV, LANEV); \
+    else if (pbk == 512) UPH_COMPACT(512, ONEV, LANEV); \
+    else UPH_COMPACT(256, ONEV, LANEV);          \
+  } while (0)
+    if (one && lane) UPH_COMPACT_PB(true, true);
+    else if (one) UPH_COMPACT_PB(true, false);
+    else if (lane) UPH_COMPACT_PB(false, true);
+    else UPH_COMPACT_PB(false, false);
+#undef UPH_COMPACT_PB
+#undef UPH_COMPACT
     if (h->nmixed > 0)
       hipLaunchKernelGGL((k_pair_mixed<KT, MODE>), dim3(h->nmixed), dim3(64), 0, st, h->n, h->b, h->dsc,
                          h->mixedWaves);
@@ -2182,6 +2194,7 @@ extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts*
     S.coff = (int)nc;
     S.pb_gid = (int)n + (u.n_fluid + u.n_fixed + u.n_moving - 1);
     // mirror rows: one test per section and fluid row, ids after the scene's rows
+    if (4.0 * M_PI * (u.h / u.dx) * (u.h / u.dx) > 0.75 * u.K) h->capped = true;
     S.nsec = u.n_fluid > 0 ? mirror_sections(u.vtype, S.sec) : 0;
     S.nwords = (u.n_fluid + 31) / 32;
     S.gw = (int)wtot;
