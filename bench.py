@@ -14,6 +14,7 @@ path, oracle/uph_oracle.c) on the same workload for a bounded number of steps.
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -155,8 +156,9 @@ def main():
                      "traffic": (traffic_from_profiles() if args.kernel == "cubic" and
                                  args.h_over_dx == 1.3 and args.scheme == "config" and not args.vtype
                                  else None),
-                     "kernel": f"k_pair_compact<256, {int(sc.kernel)}, {int(sc.mode)}, true> ({args.kernel}, "
-                               f"{args.scheme}, one scene)",
+                     "kernel": f"k_pair_compact<256, {int(sc.kernel)}, {int(sc.mode)}, true, "
+                               f"{'true' if 4 * math.pi * args.h_over_dx ** 2 > 0.75 * sc.K else 'false'}> "
+                               f"({args.kernel}, {args.scheme}, one scene)",
                      "kernel_ms": ms_pair},
     }
     if args.graph:
