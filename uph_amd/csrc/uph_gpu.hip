@@ -1050,7 +1050,7 @@ struct PairBase {
   // grad W = f (x_p - x_q) for the kernel KT at |x_p - x_q|^2 = r2, 1/|.| = rinv
   template <int KT>
   __device__ __forceinline__ float grad_factor(float r2, float r, float rinv) const {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
     if (KT == UPH_KERNEL_POLY7) {
       // PolyOrderSevenDeconvolutionKernel (derivatives.c:786-823); its correction
       // terms carry the integer quotients -131/329 and 150882/1840097, both 0, so
@@ -1081,7 +1081,7 @@ struct PairBase {
   // W at |x_p - x_q|^2 = r2, |.| = r for the kernel KT
   template <int KT>
   __device__ __forceinline__ float w_value(float r2, float r) const {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
     if (KT == UPH_KERNEL_POLY7) {
       // PolyOrderSevenConvolutionKernel (derivatives.c:767-784): a (1 - u/4)^8
       // (2048 - 3072 u + 2816 u^2 - 1984 u^3 + 1184 u^4 - 628 u^5 + 305 u^6)
@@ -1142,11 +1142,14 @@ struct PairCtx : PairBase {
   }
 
   // One admitted pair: x_p - x_q = (ex, ey), |.|^2 = d2 (FP64 geometry, FP32 sums).
-  // FP32 arithmetic may contract to FMA here; 1/r and 1/x use v_rsq_f32 / v_rcp_f32.
+  // FP32 a * b + c within one expression contracts to FMA here, and only there: a
+  // fusion across statements depends on the code this body is inlined into, and the
+  // kernels that inline it (compacted, direct, capped bit walk) must round alike.
+  // 1/r and 1/x use v_rsq_f32 / v_rcp_f32.
   template <int KT>
   __device__ __forceinline__ void add(const DevScene& S, double ex, double ey, uint32_t tq, float4 aq, float4 bq,
                                       float2) {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
     const bool p_is_i = tag_id(tq) > idp;   // pair (i=p, j=q) or (i=q, j=p)
     const float dx = (float)ex, dy = (float)ey;
     const float r2 = dx * dx + dy * dy;
@@ -1480,32 +1483,11 @@ __device__ __forceinline__ bool pair_direct(int p, const StepBufs& b, const DevS
   return false;
 }
 
-// EXACT (the capped path) runs as a small grid-stride launch that returns at
-// once unless the step's overflow flag is set.
-template <bool EXACT, int KT, int MODE>
-__global__ __launch_bounds__(BLK) void k_pair(int n, StepBufs b, const DevScene* __restrict__ sc) {
-  if (EXACT && (b.ctl[1] == 0 || b.ctl[4] != 0)) return;
-  if (EXACT && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&b.ctl[2], 1);
-  __shared__ KthLds lds[EXACT ? 1 : BLK / 64];
-  const int nlog = (n + BLK - 1) / BLK;
-  // the trip count is the same for the whole block, so every wavefront reaches
-  // wave_overflow with all of its lanes
-  for (int lb = xcd_block(); lb < nlog; lb += gridDim.x) {
-    const int p = lb * BLK + threadIdx.x;
-    bool ovf = false;
-    if (p < n) ovf = pair_direct<EXACT, KT, MODE>(p, b, sc);
-    if (!EXACT) wave_overflow<false>(b, sc, p, ovf, lds[threadIdx.x >> 6]);
-  }
-}
-
-// Pass 2 of k_pair_compact for one fluid particle: the pair sums over the set bits
-// of m[0..2] (row starts rb[0..2]) and the corrector.
+// The pair sums of fluid particle p over the set bits of m[0..2] (slot bases
+// rb[0..2]), in that order, into c.
 template <int KT, int MODE>
-__device__ __forceinline__ void compact_pairs(const DevScene& S, const StepBufs& b, int p, uint32_t tg, int itime,
-                                              double2 xp, double2 vn, float mass, const int* rb,
-                                              const unsigned long long* m) {
-  CtxOf<MODE> c;
-  c.init(S, tg, b.mA[p], b.mB[p], mirror_c<MODE>(b, (uint32_t)p));
+__device__ __forceinline__ void walk_pairs(const DevScene& S, const StepBufs& b, int p, double2 xp, CtxOf<MODE>& c,
+                                           const int* rb, const unsigned long long* m) {
   // the set bits of the three rows as one stream, advanced without branches;
   // -1 when exhausted
   int rr = 0;
@@ -1557,7 +1539,92 @@ __device__ __forceinline__ void compact_pairs(const DevScene& S, const StepBufs&
     load(n0, q0);
     pair(n1);
   }
+}
+
+// Pass 2 of k_pair_compact for one fluid particle: the pair sums over the set bits
+// of m[0..2] (row starts rb[0..2]) and the corrector.
+template <int KT, int MODE>
+__device__ __forceinline__ void compact_pairs(const DevScene& S, const StepBufs& b, int p, uint32_t tg, int itime,
+                                              double2 xp, double2 vn, float mass, const int* rb,
+                                              const unsigned long long* m) {
+  CtxOf<MODE> c;
+  c.init(S, tg, b.mA[p], b.mB[p], mirror_c<MODE>(b, (uint32_t)p));
+  walk_pairs<KT, MODE>(S, b, p, xp, c, rb, m);
   correct(S, b, p, itime, xp, vn, mass, c);
+}
+
+// The capped path's pairs of fluid slot p by the same bit walk: pass 1 marks, in two
+// 64-bit masks per candidate row (rows of up to 128 slots), the candidates in range
+// that Search.cc:27-40 keeps under the K cap (admit<true>: q ranks within p's K
+// nearest when q is the pair's second index, p within q's when it is the first);
+// pass 2 walks them row by row, in slot order as the direct loop sums them.
+// Returns false, leaving p to the direct loop, when a row is longer than 128 slots.
+template <int KT, int MODE>
+__device__ __forceinline__ bool exact_pairs(int p, const StepBufs& b, const DevScene* __restrict__ sc) {
+  const uint32_t tg = b.tagA[p];
+  const DevScene& S = sc[b.sceneOf[p]];
+  const double2 xp = b.posA[p];
+  int rb[3], rl[3];
+  candidate_rows(S, xp, b.cellStart, rb, rl);
+  if (rl[0] > 128 || rl[1] > 128 || rl[2] > 128) return false;
+  const int itime = b.ctl[0];
+  const double2 vn = b.velA[p];
+  const float mass = b.mB[p].z;
+  CtxOf<MODE> c;
+  c.init(S, tg, b.mA[p], b.mB[p], mirror_c<MODE>(b, (uint32_t)p));
+  double kpd2;
+  uint32_t kpid;
+  kth_of(b, p, kpd2, kpid);
+  const double cut = S.cutoff;
+  unsigned long long m[6];   // row r: bits of slots rb[r] + [0, 64) in m[2r], + [64, 128) in m[2r + 1]
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    unsigned long long lo = 0ull, hi = 0ull;
+#pragma unroll 1
+    for (int k = 0; k < rl[r]; ++k) {
+      const int q = rb[r] + k;
+      if (q == p) continue;
+      const double d2 = dist2(xp, b.posA[q]);
+      if (!(d2 < cut)) continue;
+      if (!c.template admit<true>(S, tag_id(b.tagA[q]), d2, kpd2, kpid, b, q)) continue;
+      if (k < 64)
+        lo |= 1ull << k;
+      else
+        hi |= 1ull << (k - 64);
+    }
+    m[2 * r] = lo;
+    m[2 * r + 1] = hi;
+  }
+  const int rb0[3] = {rb[0], rb[0] + 64, rb[1]};
+  const unsigned long long m0[3] = {m[0], m[1], m[2]};
+  walk_pairs<KT, MODE>(S, b, p, xp, c, rb0, m0);
+  const int rb1[3] = {rb[1] + 64, rb[2], rb[2] + 64};
+  const unsigned long long m1[3] = {m[3], m[4], m[5]};
+  walk_pairs<KT, MODE>(S, b, p, xp, c, rb1, m1);
+  correct(S, b, p, itime, xp, vn, mass, c);
+  return true;
+}
+
+// EXACT (the capped path) runs as a small grid-stride launch that returns at
+// once unless the step's overflow flag is set.
+template <bool EXACT, int KT, int MODE>
+__global__ __launch_bounds__(BLK) void k_pair(int n, StepBufs b, const DevScene* __restrict__ sc) {
+  if (EXACT && (b.ctl[1] == 0 || b.ctl[4] != 0)) return;
+  if (EXACT && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&b.ctl[2], 1);
+  __shared__ KthLds lds[EXACT ? 1 : BLK / 64];
+  const int nlog = (n + BLK - 1) / BLK;
+  // the trip count is the same for the whole block, so every wavefront reaches
+  // wave_overflow with all of its lanes
+  for (int lb = xcd_block(); lb < nlog; lb += gridDim.x) {
+    const int p = lb * BLK + threadIdx.x;
+    bool ovf = false;
+    if (p < n) {
+      // the capped step's fluid rows take the bit walk when their rows fit it
+      const bool walked = EXACT && tag_type(b.tagA[p]) == 1 && exact_pairs<KT, MODE>(p, b, sc);
+      if (!walked) ovf = pair_direct<EXACT, KT, MODE>(p, b, sc);
+    }
+    if (!EXACT) wave_overflow<false>(b, sc, p, ovf, lds[threadIdx.x >> 6]);
+  }
 }
 
 // Compacted form (default).  Pass 1 runs only the FP64 distance test over the
