@@ -210,6 +210,14 @@ __global__ __launch_bounds__(BLK) void k_scatter(int n, const int* __restrict__ 
   tmpIdx[cellStart[keys[i]] + rank[i]] = i;
 }
 
+// The K-th smallest (d2, id) key of a capped query and the step stamp it is valid
+// for, read by the pair admission in one 16-byte load.
+struct __align__(16) KthRec {
+  double d2;
+  uint32_t id;
+  int step;
+};
+
 struct StepBufs {
   // A: sorted input of the pair kernel
   double2 *posA, *velA;
@@ -224,11 +232,9 @@ struct StepBufs {
   // sort scratch
   int *keys, *rank, *tmpIdx, *cellStart, *cellCount, *partial;
   int* sceneOf;
-  // capped search: K-th key of overflowing queries, valid when kthStep equals the
+  // capped search: K-th key of overflowing queries, valid when its step equals the
   // step stamp ctl[3] (a counter that no restart resets, so a stamp never repeats)
-  double* kthD2;
-  uint32_t* kthId;
-  int* kthStep;
+  KthRec* kth;
   // per original index (global id)
   double2 *deckPos, *deckVel, *deckLvel;
   double* massD;   // FP64 mass per original index (diagnostics)
@@ -964,9 +970,7 @@ __device__ void wave_overflow(const StepBufs& b, const DevScene* __restrict__ sc
       double kd2;
       uint32_t kid;
       if (S.nsec == 0 && lane_kth(S, p, xp, rbp, rlp, b.posA, b.tagA, *ls, kd2, kid)) {
-        b.kthD2[p] = kd2;
-        b.kthId[p] = kid;
-        b.kthStep[p] = b.ctl[3];
+        b.kth[p] = KthRec{kd2, kid, b.ctl[3]};
         done = true;
       }
     }
@@ -1001,9 +1005,7 @@ __device__ void wave_overflow(const StepBufs& b, const DevScene* __restrict__ sc
       wave_kth_of_list(S, q, b.posA, b.tagA, b.cellStart, L, n, kd2, kid, false, S.nsec > 0 ? &b : nullptr);
     }
     if (lead) {
-      b.kthD2[q] = kd2;
-      b.kthId[q] = kid;
-      b.kthStep[q] = b.ctl[3];
+      b.kth[q] = KthRec{kd2, kid, b.ctl[3]};
       if (n < 0) atomicOr(&b.ctl[4], 1);
     }
   }
@@ -1011,9 +1013,10 @@ __device__ void wave_overflow(const StepBufs& b, const DevScene* __restrict__ sc
 
 // K-th key of slot q in this step, or +inf when q did not overflow.
 __device__ __forceinline__ void kth_of(const StepBufs& b, int q, double& kd2, uint32_t& kid) {
-  if (b.kthStep[q] == b.ctl[3]) {
-    kd2 = b.kthD2[q];
-    kid = b.kthId[q];
+  const KthRec k = b.kth[q];
+  if (k.step == b.ctl[3]) {
+    kd2 = k.d2;
+    kid = k.id;
   } else {
     kd2 = INFINITY;
     kid = 0xffffffffu;
@@ -1555,9 +1558,8 @@ __device__ __forceinline__ void compact_pairs(const DevScene& S, const StepBufs&
 
 // The capped path's pairs of fluid slot p by the same bit walk: pass 1 marks, in two
 // 64-bit masks per candidate row (rows of up to 128 slots), the candidates in range
-// that Search.cc:27-40 keeps under the K cap (admit<true>: q ranks within p's K
-// nearest when q is the pair's second index, p within q's when it is the first);
-// pass 2 walks them row by row, in slot order as the direct loop sums them.
+// that the K cap keeps; pass 2 walks them row by row, in slot order as the direct
+// loop sums them.
 // Returns false, leaving p to the direct loop, when a row is longer than 128 slots.
 template <int KT, int MODE>
 __device__ __forceinline__ bool exact_pairs(int p, const StepBufs& b, const DevScene* __restrict__ sc) {
@@ -1576,24 +1578,42 @@ __device__ __forceinline__ bool exact_pairs(int p, const StepBufs& b, const DevS
   uint32_t kpid;
   kth_of(b, p, kpd2, kpid);
   const double cut = S.cutoff;
-  unsigned long long m[6];   // row r: bits of slots rb[r] + [0, 64) in m[2r], + [64, 128) in m[2r + 1]
+  // pass 1a, the distance test alone: row r's candidates in range as bits of
+  // m[2r] (slots rb[r] + [0, 64)) and m[2r + 1] (slots rb[r] + [64, 128))
+  unsigned long long m[6];
 #pragma unroll
   for (int r = 0; r < 3; ++r) {
+    const double2* rp = b.posA + rb[r];
+    const int l0 = rl[r] < 64 ? rl[r] : 64;
     unsigned long long lo = 0ull, hi = 0ull;
 #pragma unroll 1
-    for (int k = 0; k < rl[r]; ++k) {
-      const int q = rb[r] + k;
-      if (q == p) continue;
-      const double d2 = dist2(xp, b.posA[q]);
-      if (!(d2 < cut)) continue;
-      if (!c.template admit<true>(S, tag_id(b.tagA[q]), d2, kpd2, kpid, b, q)) continue;
-      if (k < 64)
-        lo |= 1ull << k;
-      else
-        hi |= 1ull << (k - 64);
-    }
+    for (int k = 0; k < l0; ++k) lo |= (unsigned long long)(dist2(xp, rp[k]) < cut) << k;
+#pragma unroll 1
+    for (int k = 64; k < rl[r]; ++k) hi |= (unsigned long long)(dist2(xp, rp[k]) < cut) << (k - 64);
     m[2 * r] = lo;
     m[2 * r + 1] = hi;
+  }
+  const int self = p - rb[1];   // the particle itself, in its own row
+  if (self < 64)
+    m[2] &= ~(1ull << self);
+  else
+    m[3] &= ~(1ull << (self - 64));
+  // pass 1b: of those, the ones Search.cc:27-40 keeps under the K cap (admit<true>:
+  // q ranks within p's K nearest when q is the pair's second index, p within q's
+  // when it is the first)
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    const int base = rb[j >> 1] + 64 * (j & 1);
+    unsigned long long mm = m[j], keep = m[j];
+#pragma unroll 1
+    while (mm) {
+      const int k = __ffsll((long long)mm) - 1;
+      mm &= mm - 1ull;
+      const int q = base + k;
+      if (!c.template admit<true>(S, tag_id(b.tagA[q]), dist2(xp, b.posA[q]), kpd2, kpid, b, q))
+        keep &= ~(1ull << k);
+    }
+    m[j] = keep;
   }
   const int rb0[3] = {rb[0], rb[0] + 64, rb[1]};
   const unsigned long long m0[3] = {m[0], m[1], m[2]};
@@ -1791,8 +1811,7 @@ __global__ __launch_bounds__(BLK) void k_search_count(int nf_slots_unused, int n
     if (c > S.K) {
       capped = true;
     } else {
-      b.kthD2[p] = INFINITY;
-      b.kthId[p] = 0xffffffffu;
+      b.kth[p] = KthRec{INFINITY, 0xffffffffu, 0};
       qcount[idp] = m;
     }
   }
@@ -1823,8 +1842,7 @@ __global__ __launch_bounds__(BLK) void k_search_count(int nf_slots_unused, int n
       });
     }
     if (lane == 0) {
-      b.kthD2[q] = kd2;
-      b.kthId[q] = kid;
+      b.kth[q] = KthRec{kd2, kid, 0};
       qcount[idq] = m;
     }
     wave_lds_sync();
@@ -1849,8 +1867,8 @@ __global__ __launch_bounds__(BLK) void k_search_emit(int n, StepBufs b, const De
     if (qoff[idp + 1] - o > EMIT_LANE_MAX) {
       wide = true;
     } else {
-      const double kd2 = b.kthD2[p];
-      const uint32_t kid = b.kthId[p];
+      const double kd2 = b.kth[p].d2;
+      const uint32_t kid = b.kth[p].id;
       const double2 pp = b.posA[p];
       for_candidates(S, pp, b.cellStart, [&](int q) {
         if (q == p) return;
@@ -1882,8 +1900,8 @@ __global__ __launch_bounds__(BLK) void k_search_emit(int n, StepBufs b, const De
     w &= w - 1ull;
     const int q = bcast(p, l);
     const uint32_t idq = tag_id(b.tagA[q]);
-    const double kd2 = b.kthD2[q];
-    const uint32_t kid = b.kthId[q];
+    const double kd2 = b.kth[q].d2;
+    const uint32_t kid = b.kth[q].id;
     const int o = qoff[idq];
     const int nin = wave_gather(S, q, b.posA, b.tagA, b.cellStart, L);
     if (nin >= 0 && nin <= KTH_CAP) {
@@ -2062,12 +2080,12 @@ static int alloc_bufs(StepBufs& b, int n, int ncells, int nblocks_scan, Err& err
       dalloc(&b.tagB, n, err) || dalloc(&b.keys, n, err) || dalloc(&b.rank, n, err) ||
       dalloc(&b.tmpIdx, n, err) || dalloc(&b.cellStart, (size_t)ncells + 1, err) ||
       dalloc(&b.cellCount, ncells, err) || dalloc(&b.partial, nblocks_scan, err) ||
-      dalloc(&b.sceneOf, n, err) || dalloc(&b.kthD2, n, err) || dalloc(&b.kthId, n, err) ||
+      dalloc(&b.sceneOf, n, err) || dalloc(&b.kth, n, err) ||
       dalloc(&b.ctl, 8, err))
     return -1;
   if (search_only) return 0;
   if (dalloc(&b.velA, n, err) || dalloc(&b.velB, n, err) || dalloc(&b.rhoA, n, err) ||
-      dalloc(&b.rhoB, n, err) || dalloc(&b.rtB, n, err) || dalloc(&b.kthStep, n, err) ||
+      dalloc(&b.rhoB, n, err) || dalloc(&b.rtB, n, err) ||
       dalloc(&b.mA, n, err) || dalloc(&b.mB, n, err) || dalloc(&b.deckPos, n, err) ||
       dalloc(&b.deckVel, n, err) || dalloc(&b.deckLvel, n, err) || dalloc(&b.massD, n, err) ||
       dalloc(&b.presA, n, err) || dalloc(&b.presB, n, err) || dalloc(&b.deckP, n, err) ||
@@ -2107,9 +2125,9 @@ static int alloc_ghost_bufs(StepBufs& b, int n, int ncells, int gtot, int wtot, 
 }
 
 static void free_bufs(StepBufs& b) {
-  void* ptrs[] = {b.posA, b.velA, b.rhoA, b.kthStep, b.tagA, b.mA, b.mB, b.posB, b.velB, b.rhoB,
+  void* ptrs[] = {b.posA, b.velA, b.rhoA, b.tagA, b.mA, b.mB, b.posB, b.velB, b.rhoB,
                   b.rtB, b.tagB, b.keys, b.rank, b.tmpIdx, b.cellStart, b.cellCount, b.partial,
-                  b.sceneOf, b.kthD2, b.kthId, b.deckPos, b.deckVel, b.deckLvel, b.massD, b.ctl,
+                  b.sceneOf, b.kth, b.deckPos, b.deckVel, b.deckLvel, b.massD, b.ctl,
                   b.dbgRates, b.presA, b.presB, b.deckP, b.rt2B, b.dvB, b.mC, b.deckRl, b.deckPl,
                   b.wallKind, b.wallRef, b.gbits, b.gbits2, b.gwcnt, b.gwstart, b.gpart, b.gcand, b.gcandN, b.gpos, b.gid, b.gscene, b.gkey,
                   b.grank, b.gcellCount, b.gcellStart, b.gposS, b.gidS, b.gvel, b.glvel, b.grho, b.grl,
@@ -2409,7 +2427,7 @@ extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts*
   // cell counts start at zero; every scan leaves them zero again
   if (hipMemset(h->b.cellCount, 0, (size_t)h->ncells * sizeof(int)) != hipSuccess ||
       hipMemset(h->b.posA + h->n, 0, POS_PAD * sizeof(double2)) != hipSuccess ||
-      hipMemset(h->b.kthStep, 0, (size_t)h->n * sizeof(int)) != hipSuccess ||
+      hipMemset(h->b.kth, 0, (size_t)h->n * sizeof(KthRec)) != hipSuccess ||
       hipMemset(h->b.rt2B, 0, (size_t)h->n * sizeof(float4)) != hipSuccess ||
       hipMemset(h->b.dvB, 0, (size_t)h->n * sizeof(float2)) != hipSuccess ||
       hipMemset(h->b.partial, 0, (size_t)h->cs.nb * sizeof(int)) != hipSuccess ||
