@@ -33,7 +33,7 @@
 #include <type_traits>
 
 #define BLK 256
-#define SCAN_ITEMS 16
+#define SCAN_ITEMS 4
 #define SCAN_BLK 256
 #define SCAN_TILE (SCAN_ITEMS * SCAN_BLK)
 #define POS_PAD 128    // zeroed slots past the end of posA (read, never used, by k_pair_compact)
@@ -1628,7 +1628,10 @@ __device__ __forceinline__ bool exact_pairs(int p, const StepBufs& b, const DevS
 // EXACT (the capped path) runs as a small grid-stride launch that returns at
 // once unless the step's overflow flag is set.
 template <bool EXACT, int KT, int MODE>
-__global__ __launch_bounds__(BLK) void k_pair(int n, StepBufs b, const DevScene* __restrict__ sc) {
+// Occupancy targets: 4 wavefronts per SIMD for the configuration scheme (129 VGPRs
+// uncapped), 3 for the reference scheme (179 uncapped).
+__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(MODE == UPH_MODE_REF ? 3 : 4)))
+void k_pair(int n, StepBufs b, const DevScene* __restrict__ sc) {
   if (EXACT && (b.ctl[1] == 0 || b.ctl[4] != 0)) return;
   if (EXACT && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&b.ctl[2], 1);
   __shared__ KthLds lds[EXACT ? 1 : BLK / 64];
@@ -1758,7 +1761,10 @@ __device__ __forceinline__ void compact_query(int n, int p, const StepBufs& b, c
 // LANE: the handle's queries are expected to be capped, so the kernel carries the
 // per-lane K-th key search and its LDS.
 template <int PB, int KT, int MODE, bool ONE, bool LANE>
-__global__ __launch_bounds__(PB) void k_pair_compact(int n, StepBufs b, const DevScene* __restrict__ sc) {
+// The reference scheme's sums need about 133 VGPRs: capped at 128 (4 wavefronts per
+// SIMD instead of 3, a few spilled registers) it runs faster.
+__global__ __launch_bounds__(PB) __attribute__((amdgpu_waves_per_eu(MODE == UPH_MODE_REF ? 4 : 1)))
+void k_pair_compact(int n, StepBufs b, const DevScene* __restrict__ sc) {
   const int p = xcd_block() * PB + threadIdx.x;
   if constexpr (LANE) {
     __shared__ KthLaneLds lds[PB / 64];
