@@ -1628,9 +1628,10 @@ __device__ __forceinline__ bool exact_pairs(int p, const StepBufs& b, const DevS
 // EXACT (the capped path) runs as a small grid-stride launch that returns at
 // once unless the step's overflow flag is set.
 template <bool EXACT, int KT, int MODE>
-// Occupancy targets: 4 wavefronts per SIMD for the configuration scheme (129 VGPRs
-// uncapped), 3 for the reference scheme (179 uncapped).
-__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(MODE == UPH_MODE_REF ? 3 : 4)))
+// Four wavefronts per SIMD (128 VGPRs): the configuration scheme needs 129, the
+// reference scheme 179, which at this target spills 168 bytes per lane and still runs
+// the capped step faster than with three wavefronts (no spill) or five (392 bytes).
+__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4)))
 void k_pair(int n, StepBufs b, const DevScene* __restrict__ sc) {
   if (EXACT && (b.ctl[1] == 0 || b.ctl[4] != 0)) return;
   if (EXACT && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&b.ctl[2], 1);
