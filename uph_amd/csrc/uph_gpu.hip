@@ -1762,9 +1762,13 @@ __device__ __forceinline__ void compact_query(int n, int p, const StepBufs& b, c
 // LANE: the handle's queries are expected to be capped, so the kernel carries the
 // per-lane K-th key search and its LDS.
 template <int PB, int KT, int MODE, bool ONE, bool LANE>
-// The reference scheme's sums need about 133 VGPRs: capped at 128 (4 wavefronts per
-// SIMD instead of 3, a few spilled registers) it runs faster.
-__global__ __launch_bounds__(PB) __attribute__((amdgpu_waves_per_eu(MODE == UPH_MODE_REF ? 4 : 1)))
+// Occupancy targets, each measured faster than the compiler's own choice: the
+// reference scheme's sums need about 133 VGPRs, capped at 128 (4 wavefronts per SIMD
+// instead of 3, a few spilled registers); with the per-lane K-th search (LANE) the
+// configuration scheme runs 6 wavefronts per SIMD (80 VGPRs) instead of 5.  The
+// default instance (91 VGPRs, 5 wavefronts) measured slower at 6.
+__global__ __launch_bounds__(PB)
+__attribute__((amdgpu_waves_per_eu(MODE == UPH_MODE_REF ? 4 : LANE ? 6 : 1)))
 void k_pair_compact(int n, StepBufs b, const DevScene* __restrict__ sc) {
   const int p = xcd_block() * PB + threadIdx.x;
   if constexpr (LANE) {
