@@ -75,6 +75,28 @@ def test_rates_capped_neighbour_sets():
         assert e < 1e-5, (k, e)
 
 
+def test_rates_capped_rows_longer_than_128_slots():
+    """h = 4 dx: about 200 particles in range and candidate rows of up to ~190 slots, so the
+    capped pass takes its direct loop for rows past 128 slots and the bit walk for the
+    rest (near the walls); both against the oracle, and the compacted kernel against the
+    direct one bit for bit over 10 steps."""
+    from uph_amd.solver import GpuSolver
+    sc = _moving_scene(scenes.dam_break(width=0.4, height=0.4, dx=0.02, tank=(0.8, 0.8), h_factor=4.0,
+                                        jitter=0.1, seed=21), amp=0.1)
+    err, g = _compare_rates(sc, "h = 4 dx, rows past 128 slots (K=40 binds)")
+    assert g.sync() == 1
+    for k, e in err.items():
+        assert e < 1e-5, (k, e)
+    a = GpuSolver(sc, pair_variant=0)
+    b = GpuSolver(sc, pair_variant=1)
+    a.step(10)
+    b.step(10)
+    assert a.sync() == 10 and b.sync() == 10
+    sa, sb = a.state(), b.state()
+    for key in ("x", "v", "rho"):
+        assert np.array_equal(sa[key], sb[key]), key
+
+
 def _run_both(sc, steps, name, nthreads=0, **gopts):
     from oracle.oracle import Sim
     from uph_amd.solver import GpuSolver
