@@ -1578,42 +1578,56 @@ __device__ __forceinline__ bool exact_pairs(int p, const StepBufs& b, const DevS
   uint32_t kpid;
   kth_of(b, p, kpd2, kpid);
   const double cut = S.cutoff;
-  // pass 1a, the distance test alone: row r's candidates in range as bits of
-  // m[2r] (slots rb[r] + [0, 64)) and m[2r + 1] (slots rb[r] + [64, 128))
+  const uint32_t idp = tag_id(tg);
+  const bool pb_ok = (int)idp < S.pb_gid;
+  const int stamp = b.ctl[3];
+  // Row r's admitted candidates as bits of m[2r] (slots rb[r] + [0, 64)) and m[2r + 1]
+  // (slots rb[r] + [64, 128)), by the rules of Search.cc:27-40 (admit<true>).  Pass 1a
+  // tests distances and decides the candidates q with a larger original index, whose
+  // admission needs only p's K-th key (q ranks within p's K nearest, q below
+  // ParticleBoundary); it marks as pending those with a smaller one, whose admission
+  // needs q's K-th key (p ranks within q's K nearest).  Pass 1b reads those keys, two
+  // candidates in flight.
   unsigned long long m[6];
 #pragma unroll
   for (int r = 0; r < 3; ++r) {
     const double2* rp = b.posA + rb[r];
-    const int l0 = rl[r] < 64 ? rl[r] : 64;
-    unsigned long long lo = 0ull, hi = 0ull;
-#pragma unroll 1
-    for (int k = 0; k < l0; ++k) lo |= (unsigned long long)(dist2(xp, rp[k]) < cut) << k;
-#pragma unroll 1
-    for (int k = 64; k < rl[r]; ++k) hi |= (unsigned long long)(dist2(xp, rp[k]) < cut) << (k - 64);
-    m[2 * r] = lo;
-    m[2 * r + 1] = hi;
-  }
-  const int self = p - rb[1];   // the particle itself, in its own row
-  if (self < 64)
-    m[2] &= ~(1ull << self);
-  else
-    m[3] &= ~(1ull << (self - 64));
-  // pass 1b: of those, the ones Search.cc:27-40 keeps under the K cap (admit<true>:
-  // q ranks within p's K nearest when q is the pair's second index, p within q's
-  // when it is the first)
+    const uint32_t* tp = b.tagA + rb[r];
 #pragma unroll
-  for (int j = 0; j < 6; ++j) {
-    const int base = rb[j >> 1] + 64 * (j & 1);
-    unsigned long long mm = m[j], keep = m[j];
+    for (int w = 0; w < 2; ++w) {
+      const int k0 = 64 * w;
+      const int k1 = rl[r] < k0 + 64 ? rl[r] : k0 + 64;
+      unsigned long long am = 0ull, pm = 0ull;
 #pragma unroll 1
-    while (mm) {
-      const int k = __ffsll((long long)mm) - 1;
-      mm &= mm - 1ull;
-      const int q = base + k;
-      if (!c.template admit<true>(S, tag_id(b.tagA[q]), dist2(xp, b.posA[q]), kpd2, kpid, b, q))
-        keep &= ~(1ull << k);
+      for (int k = k0; k < k1; ++k) {
+        const double d2 = dist2(xp, rp[k]);
+        const uint32_t idq = tag_id(tp[k]);
+        const bool in = d2 < cut;
+        const bool adm = in && idq > idp && (int)idq < S.pb_gid && key_le(d2, idq, kpd2, kpid);
+        const bool pen = in && idq < idp && pb_ok;   // the particle itself is neither
+        am |= (unsigned long long)adm << (k - k0);
+        pm |= (unsigned long long)pen << (k - k0);
+      }
+      const int base = rb[r] + k0;
+      int ka = pm ? __ffsll((long long)pm) - 1 : -1;
+      pm &= pm - 1ull;
+      int qa = ka >= 0 ? base + ka : p;
+      KthRec ra = b.kth[qa];
+      double2 xa = b.posA[qa];
+#pragma unroll 1
+      while (ka >= 0) {
+        const int kb = pm ? __ffsll((long long)pm) - 1 : -1;
+        pm &= pm - 1ull;
+        const int qb = kb >= 0 ? base + kb : p;
+        const KthRec rk = b.kth[qb];
+        const double2 xb = b.posA[qb];
+        if (ra.step != stamp || key_le(dist2(xp, xa), idp, ra.d2, ra.id)) am |= 1ull << ka;
+        ka = kb;
+        ra = rk;
+        xa = xb;
+      }
+      m[2 * r + w] = am;
     }
-    m[j] = keep;
   }
   const int rb0[3] = {rb[0], rb[0] + 64, rb[1]};
   const unsigned long long m0[3] = {m[0], m[1], m[2]};
