@@ -899,7 +899,9 @@ __device__ bool lane_kth(const DevScene& S, int q, double2 xq, const int* rb, co
       const double d2 = dist2(xq, pos[t]);
       if (t == q || !(d2 < cut)) continue;
       const int bk = min((int)(d2 * scale), LANE_BUCKETS - 1);
-      L.cnt[bk >> 1][lane] += (bk & 1) ? 0x10000u : 1u;
+      // an LDS atomic whose result is unused (ds_add_u32): the next candidate does not
+      // wait for this count to be read back, as a read-modify-write would
+      atomicAdd(&L.cnt[bk >> 1][lane], (bk & 1) ? 0x10000u : 1u);
       ++n;
     }
   if (n <= S.K) return true;
