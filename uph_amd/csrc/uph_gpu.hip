@@ -930,15 +930,27 @@ __device__ bool lane_kth(const DevScene& S, int q, double2 xq, const int* rb, co
   for (int rr = 0; rr <= S.K - 1 - below; ++rr) {
     double bd = INFINITY;
     uint32_t bi = 0xffffffffu;
+    // four members per trip, their loads issued together (a position past the
+    // bucket's end repeats its last member and is not compared)
 #pragma unroll 1
-    for (int i = 0; i < m; ++i) {
-      const uint32_t t = L.slot[i][lane];
-      const double d2 = dist2(xq, pos[t]);
-      const uint32_t id = tag_id(tag[t]);
-      const bool above = d2 > td || (d2 == td && id > ti);
-      if (above && key_lt(d2, id, bd, bi)) {
-        bd = d2;
-        bi = id;
+    for (int i = 0; i < m; i += 4) {
+      double2 x[4];
+      uint32_t g[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t t = L.slot[min(i + u, m - 1)][lane];
+        x[u] = pos[t];
+        g[u] = tag[t];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const double d2 = dist2(xq, x[u]);
+        const uint32_t id = tag_id(g[u]);
+        const bool above = d2 > td || (d2 == td && id > ti);
+        if (i + u < m && above && key_lt(d2, id, bd, bi)) {
+          bd = d2;
+          bi = id;
+        }
       }
     }
     td = bd;
