@@ -886,23 +886,36 @@ __device__ bool lane_kth(const DevScene& S, int q, double2 xq, const int* rb, co
   const int lane = threadIdx.x & 63;
   kd2 = INFINITY;
   kid = 0xffffffffu;
-  if (rl[0] + rl[1] + rl[2] > CAND_MAX) return false;   // 16-bit counts hold every bucket
+  // rows of up to 128 slots: the in-range candidates are kept as bits, two 64-bit
+  // words per row, so the collecting pass visits only them (16-bit counts hold every
+  // bucket)
+  if (rl[0] > 128 || rl[1] > 128 || rl[2] > 128) return false;
   const double cut = S.cutoff;
   const double scale = (double)LANE_BUCKETS / cut;
 #pragma unroll
   for (int w = 0; w < LANE_BUCKETS / 2; ++w) L.cnt[w][lane] = 0u;
   int n = 0;
-#pragma unroll 1
+  unsigned long long mk[6];   // row r: slots rb[r] + [0, 64) in mk[2r], + [64, 128) in mk[2r + 1]
+#pragma unroll
   for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int w = 0; w < 2; ++w) {
+      const int k0 = 64 * w;
+      const int k1 = rl[r] < k0 + 64 ? rl[r] : k0 + 64;
+      unsigned long long mm = 0ull;
 #pragma unroll 1
-    for (int t = rb[r]; t < rb[r] + rl[r]; ++t) {
-      const double d2 = dist2(xq, pos[t]);
-      if (t == q || !(d2 < cut)) continue;
-      const int bk = min((int)(d2 * scale), LANE_BUCKETS - 1);
-      // an LDS atomic whose result is unused (ds_add_u32): the next candidate does not
-      // wait for this count to be read back, as a read-modify-write would
-      atomicAdd(&L.cnt[bk >> 1][lane], (bk & 1) ? 0x10000u : 1u);
-      ++n;
+      for (int k = k0; k < k1; ++k) {
+        const int t = rb[r] + k;
+        const double d2 = dist2(xq, pos[t]);
+        if (t == q || !(d2 < cut)) continue;
+        const int bk = min((int)(d2 * scale), LANE_BUCKETS - 1);
+        // an LDS atomic whose result is unused (ds_add_u32): the next candidate does
+        // not wait for this count to be read back, as a read-modify-write would
+        atomicAdd(&L.cnt[bk >> 1][lane], (bk & 1) ? 0x10000u : 1u);
+        ++n;
+        mm |= 1ull << (k - k0);
+      }
+      mk[2 * r + w] = mm;
     }
   if (n <= S.K) return true;
   int below = 0, kb = 0, m = 0;
@@ -915,14 +928,18 @@ __device__ bool lane_kth(const DevScene& S, int q, double2 xq, const int* rb, co
   }
   if (m > LANE_CAP) return false;
   int j = 0;
+#pragma unroll
+  for (int u = 0; u < 6; ++u) {
+    const int base = rb[u >> 1] + 64 * (u & 1);
+    unsigned long long mm = mk[u];
 #pragma unroll 1
-  for (int r = 0; r < 3; ++r)
-#pragma unroll 1
-    for (int t = rb[r]; t < rb[r] + rl[r]; ++t) {
+    while (mm) {
+      const int t = base + __ffsll((long long)mm) - 1;
+      mm &= mm - 1ull;
       const double d2 = dist2(xq, pos[t]);
-      if (t == q || !(d2 < cut)) continue;
       if (min((int)(d2 * scale), LANE_BUCKETS - 1) == kb) L.slot[j++][lane] = (uint32_t)t;
     }
+  }
   // rank K - 1 - below of the bucket: the minimum above the previous one, that many times
   double td = -1.0;
   uint32_t ti = 0u;
