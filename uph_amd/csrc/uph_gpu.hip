@@ -1629,15 +1629,29 @@ __device__ __forceinline__ bool exact_pairs(int p, const StepBufs& b, const DevS
       const int k0 = 64 * w;
       const int k1 = rl[r] < k0 + 64 ? rl[r] : k0 + 64;
       unsigned long long am = 0ull, pm = 0ull;
+      // four candidates per trip, their loads issued together (a slot past the row's
+      // end repeats its last one and sets no bit)
 #pragma unroll 1
-      for (int k = k0; k < k1; ++k) {
-        const double d2 = dist2(xp, rp[k]);
-        const uint32_t idq = tag_id(tp[k]);
-        const bool in = d2 < cut;
-        const bool adm = in && idq > idp && (int)idq < S.pb_gid && key_le(d2, idq, kpd2, kpid);
-        const bool pen = in && idq < idp && pb_ok;   // the particle itself is neither
-        am |= (unsigned long long)adm << (k - k0);
-        pm |= (unsigned long long)pen << (k - k0);
+      for (int k = k0; k < k1; k += 4) {
+        double2 x[4];
+        uint32_t g[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int kk = k + u < k1 ? k + u : k1 - 1;
+          x[u] = rp[kk];
+          g[u] = tp[kk];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const double d2 = dist2(xp, x[u]);
+          const uint32_t idq = tag_id(g[u]);
+          const bool in = k + u < k1 && d2 < cut;
+          const bool adm = in && idq > idp && (int)idq < S.pb_gid && key_le(d2, idq, kpd2, kpid);
+          const bool pen = in && idq < idp && pb_ok;   // the particle itself is neither
+          const int sh = (k + u - k0) & 63;   // past the row's end both bits are 0
+          am |= (unsigned long long)adm << sh;
+          pm |= (unsigned long long)pen << sh;
+        }
       }
       const int base = rb[r] + k0;
       int ka = pm ? __ffsll((long long)pm) - 1 : -1;
