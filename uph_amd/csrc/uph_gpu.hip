@@ -903,17 +903,24 @@ __device__ bool lane_kth(const DevScene& S, int q, double2 xq, const int* rb, co
       const int k0 = 64 * w;
       const int k1 = rl[r] < k0 + 64 ? rl[r] : k0 + 64;
       unsigned long long mm = 0ull;
+      // four candidates per trip, their loads issued together (a slot past the row's
+      // end repeats its last one and is not counted)
 #pragma unroll 1
-      for (int k = k0; k < k1; ++k) {
-        const int t = rb[r] + k;
-        const double d2 = dist2(xq, pos[t]);
-        if (t == q || !(d2 < cut)) continue;
-        const int bk = min((int)(d2 * scale), LANE_BUCKETS - 1);
-        // an LDS atomic whose result is unused (ds_add_u32): the next candidate does
-        // not wait for this count to be read back, as a read-modify-write would
-        atomicAdd(&L.cnt[bk >> 1][lane], (bk & 1) ? 0x10000u : 1u);
-        ++n;
-        mm |= 1ull << (k - k0);
+      for (int k = k0; k < k1; k += 4) {
+        double2 x[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) x[u] = pos[rb[r] + (k + u < k1 ? k + u : k1 - 1)];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const double d2 = dist2(xq, x[u]);
+          if (k + u >= k1 || rb[r] + k + u == q || !(d2 < cut)) continue;
+          const int bk = min((int)(d2 * scale), LANE_BUCKETS - 1);
+          // an LDS atomic whose result is unused (ds_add_u32): the next candidate
+          // does not wait for this count to be read back, as a read-modify-write would
+          atomicAdd(&L.cnt[bk >> 1][lane], (bk & 1) ? 0x10000u : 1u);
+          ++n;
+          mm |= 1ull << (k + u - k0);
+        }
       }
       mk[2 * r + w] = mm;
     }
