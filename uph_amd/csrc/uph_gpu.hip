@@ -233,8 +233,14 @@ struct StepBufs {
   int *keys, *rank, *tmpIdx, *cellStart, *cellCount, *partial;
   int* sceneOf;
   // capped search: K-th key of overflowing queries, valid when its step equals the
-  // step stamp ctl[3] (a counter that no restart resets, so a stamp never repeats)
+  // step stamp ctl[3] (a counter that no restart resets, so a stamp never repeats),
+  // or its negation when the per-lane search has also left the query's in-range
+  // candidates in kmask (six 64-bit words: row r's slots rb[r] + [0, 64) in word 2r,
+  // + [64, 128) in word 2r + 1; word u of slot q at kmask[u * kmask_n + q], so that a
+  // wavefront's stores and loads of one word are contiguous)
   KthRec* kth;
+  unsigned long long* kmask;
+  int kmask_n;
   // per original index (global id)
   double2 *deckPos, *deckVel, *deckLvel;
   double* massD;   // FP64 mass per original index (diagnostics)
@@ -882,7 +888,8 @@ __device__ void wave_kth(const DevScene& S, int q, const double2* __restrict__ p
 // CAND_MAX.  +inf when at most K are in range.
 __device__ bool lane_kth(const DevScene& S, int q, double2 xq, const int* rb, const int* rl,
                          const double2* __restrict__ pos, const uint32_t* __restrict__ tag, LaneSel& L,
-                         double& kd2, uint32_t& kid) {
+                         double& kd2, uint32_t& kid, unsigned long long* __restrict__ mask_out,
+                         int mask_stride) {
   const int lane = threadIdx.x & 63;
   kd2 = INFINITY;
   kid = 0xffffffffu;
@@ -924,6 +931,8 @@ __device__ bool lane_kth(const DevScene& S, int q, double2 xq, const int* rb, co
       }
       mk[2 * r + w] = mm;
     }
+#pragma unroll
+  for (int u = 0; u < 6; ++u) mask_out[(size_t)u * mask_stride] = mk[u];
   if (n <= S.K) return true;
   int below = 0, kb = 0, m = 0;
 #pragma unroll 1
@@ -1007,8 +1016,8 @@ __device__ void wave_overflow(const StepBufs& b, const DevScene* __restrict__ sc
       const DevScene& S = sc[sidx];
       double kd2;
       uint32_t kid;
-      if (S.nsec == 0 && lane_kth(S, p, xp, rbp, rlp, b.posA, b.tagA, *ls, kd2, kid)) {
-        b.kth[p] = KthRec{kd2, kid, b.ctl[3]};
+      if (S.nsec == 0 && lane_kth(S, p, xp, rbp, rlp, b.posA, b.tagA, *ls, kd2, kid, b.kmask + p, b.kmask_n)) {
+        b.kth[p] = KthRec{kd2, kid, -b.ctl[3]};   // negated: kmask holds its in-range candidates
         done = true;
       }
     }
@@ -1052,7 +1061,7 @@ __device__ void wave_overflow(const StepBufs& b, const DevScene* __restrict__ sc
 // K-th key of slot q in this step, or +inf when q did not overflow.
 __device__ __forceinline__ void kth_of(const StepBufs& b, int q, double& kd2, uint32_t& kid) {
   const KthRec k = b.kth[q];
-  if (k.step == b.ctl[3]) {
+  if (k.step == b.ctl[3] || k.step == -b.ctl[3]) {
     kd2 = k.d2;
     kid = k.id;
   } else {
@@ -1619,6 +1628,7 @@ __device__ __forceinline__ bool exact_pairs(int p, const StepBufs& b, const DevS
   const uint32_t idp = tag_id(tg);
   const bool pb_ok = (int)idp < S.pb_gid;
   const int stamp = b.ctl[3];
+  const bool have_mask = b.kth[p].step == -stamp;
   // Row r's admitted candidates as bits of m[2r] (slots rb[r] + [0, 64)) and m[2r + 1]
   // (slots rb[r] + [64, 128)), by the rules of Search.cc:27-40 (admit<true>).  Pass 1a
   // tests distances and decides the candidates q with a larger original index, whose
@@ -1636,6 +1646,35 @@ __device__ __forceinline__ bool exact_pairs(int p, const StepBufs& b, const DevS
       const int k0 = 64 * w;
       const int k1 = rl[r] < k0 + 64 ? rl[r] : k0 + 64;
       unsigned long long am = 0ull, pm = 0ull;
+      if (have_mask) {
+        // the per-lane K-th search left this word's in-range candidates: four per trip
+        unsigned long long mm = b.kmask[(size_t)(2 * r + w) * b.kmask_n + p];
+#pragma unroll 1
+        while (mm) {
+          int kk[4];
+          double2 x[4];
+          uint32_t g[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            kk[u] = mm ? __ffsll((long long)mm) - 1 : -1;
+            mm &= mm - 1ull;
+            const int qq = kk[u] >= 0 ? rb[r] + k0 + kk[u] : p;
+            x[u] = b.posA[qq];
+            g[u] = b.tagA[qq];
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const double d2 = dist2(xp, x[u]);
+            const uint32_t idq = tag_id(g[u]);
+            const bool in = kk[u] >= 0;
+            const bool adm = in && idq > idp && (int)idq < S.pb_gid && key_le(d2, idq, kpd2, kpid);
+            const bool pen = in && idq < idp && pb_ok;
+            const int sh = kk[u] & 63;
+            am |= (unsigned long long)adm << sh;
+            pm |= (unsigned long long)pen << sh;
+          }
+        }
+      } else {
       // four candidates per trip, their loads issued together (a slot past the row's
       // end repeats its last one and sets no bit)
 #pragma unroll 1
@@ -1660,6 +1699,7 @@ __device__ __forceinline__ bool exact_pairs(int p, const StepBufs& b, const DevS
           pm |= (unsigned long long)pen << sh;
         }
       }
+      }
       const int base = rb[r] + k0;
       int ka = pm ? __ffsll((long long)pm) - 1 : -1;
       pm &= pm - 1ull;
@@ -1673,7 +1713,8 @@ __device__ __forceinline__ bool exact_pairs(int p, const StepBufs& b, const DevS
         const int qb = kb >= 0 ? base + kb : p;
         const KthRec rk = b.kth[qb];
         const double2 xb = b.posA[qb];
-        if (ra.step != stamp || key_le(dist2(xp, xa), idp, ra.d2, ra.id)) am |= 1ull << ka;
+        if ((ra.step != stamp && ra.step != -stamp) || key_le(dist2(xp, xa), idp, ra.d2, ra.id))
+          am |= 1ull << ka;
         ka = kb;
         ra = rk;
         xa = xb;
@@ -2162,7 +2203,7 @@ static int alloc_bufs(StepBufs& b, int n, int ncells, int nblocks_scan, Err& err
     return -1;
   if (search_only) return 0;
   if (dalloc(&b.velA, n, err) || dalloc(&b.velB, n, err) || dalloc(&b.rhoA, n, err) ||
-      dalloc(&b.rhoB, n, err) || dalloc(&b.rtB, n, err) ||
+      dalloc(&b.rhoB, n, err) || dalloc(&b.rtB, n, err) || dalloc(&b.kmask, (size_t)n * 6, err) ||
       dalloc(&b.mA, n, err) || dalloc(&b.mB, n, err) || dalloc(&b.deckPos, n, err) ||
       dalloc(&b.deckVel, n, err) || dalloc(&b.deckLvel, n, err) || dalloc(&b.massD, n, err) ||
       dalloc(&b.presA, n, err) || dalloc(&b.presB, n, err) || dalloc(&b.deckP, n, err) ||
@@ -2170,6 +2211,7 @@ static int alloc_bufs(StepBufs& b, int n, int ncells, int nblocks_scan, Err& err
       dalloc(&b.deckRl, n, err) || dalloc(&b.deckPl, n, err) || dalloc(&b.wallKind, n, err) ||
       dalloc(&b.wallRef, n, err))
     return -1;
+  b.kmask_n = n;
   return 0;
 }
 
@@ -2204,7 +2246,7 @@ static int alloc_ghost_bufs(StepBufs& b, int n, int ncells, int gtot, int wtot, 
 static void free_bufs(StepBufs& b) {
   void* ptrs[] = {b.posA, b.velA, b.rhoA, b.tagA, b.mA, b.mB, b.posB, b.velB, b.rhoB,
                   b.rtB, b.tagB, b.keys, b.rank, b.tmpIdx, b.cellStart, b.cellCount, b.partial,
-                  b.sceneOf, b.kth, b.deckPos, b.deckVel, b.deckLvel, b.massD, b.ctl,
+                  b.sceneOf, b.kth, b.kmask, b.deckPos, b.deckVel, b.deckLvel, b.massD, b.ctl,
                   b.dbgRates, b.presA, b.presB, b.deckP, b.rt2B, b.dvB, b.mC, b.deckRl, b.deckPl,
                   b.wallKind, b.wallRef, b.gbits, b.gbits2, b.gwcnt, b.gwstart, b.gpart, b.gcand, b.gcandN, b.gpos, b.gid, b.gscene, b.gkey,
                   b.grank, b.gcellCount, b.gcellStart, b.gposS, b.gidS, b.gvel, b.glvel, b.grho, b.grl,
