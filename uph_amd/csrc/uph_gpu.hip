@@ -884,8 +884,8 @@ __device__ void wave_kth(const DevScene& S, int q, const double2* __restrict__ p
 // finds the bucket that holds rank K - 1 (bucketing is monotone in d2, so every key
 // of a lower bucket is smaller); its slots are collected and the rank taken among
 // them by full (d2, id) keys.  Returns false, leaving the query to the wavefront
-// search, when that bucket holds more than LANE_CAP or the rows more than
-// CAND_MAX.  +inf when at most K are in range.
+// search, when that bucket holds more than LANE_CAP or a row is longer than 128
+// slots.  +inf when at most K are in range.
 __device__ bool lane_kth(const DevScene& S, int q, double2 xq, const int* rb, const int* rl,
                          const double2* __restrict__ pos, const uint32_t* __restrict__ tag, LaneSel& L,
                          double& kd2, uint32_t& kid, unsigned long long* __restrict__ mask_out,
@@ -895,7 +895,7 @@ __device__ bool lane_kth(const DevScene& S, int q, double2 xq, const int* rb, co
   kid = 0xffffffffu;
   // rows of up to 128 slots: the in-range candidates are kept as bits, two 64-bit
   // words per row, so the collecting pass visits only them (16-bit counts hold every
-  // bucket)
+  // bucket: at most 384 are in range)
   if (rl[0] > 128 || rl[1] > 128 || rl[2] > 128) return false;
   const double cut = S.cutoff;
   const double scale = (double)LANE_BUCKETS / cut;
