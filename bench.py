@@ -6,8 +6,10 @@ cubic spline h = 1.3 dx, Tait EOS, artificial viscosity 0.1, dt = 0.25 h / c0.
 A step is one full leapfrog step of every particle on one GPU: cell binning and
 sort, fused neighbour search + pair sums, FP64 integration.  Inputs are resident
 in HBM before the timed region; the timed region is K steps bracketed by stream
-synchronisation, timed with HIP events on the library's stream, as is the fused
-pair kernel inside it (the dominant kernel, used for the roofline).
+synchronisation, timed with two HIP events on the library's stream around the
+loop.  K further steps with events between the phases give the split: sort
+kernels, the fused pair kernel (the dominant kernel, used for the roofline) and
+the capped pass.
 
 cpu_baseline: the FP64 CPU oracle (C/OpenMP restatement of the reference's
 path, oracle/uph_oracle.c) on the same workload for a bounded number of steps.
@@ -30,9 +32,42 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def host_cores():
+    """(physical cores, logical CPUs) of this host."""
+    phys = set()
+    try:
+        with open("/proc/cpuinfo") as f:
+            pid = cid = None
+            for line in f:
+                if line.startswith("physical id"):
+                    pid = line.split(":")[1].strip()
+                elif line.startswith("core id"):
+                    cid = line.split(":")[1].strip()
+                elif not line.strip():
+                    if cid is not None:
+                        phys.add((pid, cid))
+                    pid = cid = None
+    except OSError:
+        pass
+    return (len(phys) or None), os.cpu_count()
+
+
+def cpu_threads():
+    """Threads for the CPU baseline: the host-thread share this process is granted
+    (OMP_NUM_THREADS when set, else the CPUs of its affinity mask)."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
 def cpu_baseline(sc, budget_s):
     from oracle.oracle import Sim
-    nthreads = min(16, os.cpu_count() or 1)
+    nthreads = cpu_threads()
+    phys, logical = host_cores()
     o = Sim(sc, nthreads=nthreads)
     t0 = time.perf_counter()
     o.step(1)
@@ -43,9 +78,11 @@ def cpu_baseline(sc, budget_s):
     o.step(steps)
     el = time.perf_counter() - t0
     return dict(value=sc.n_fluid * steps / el, unit="particle-updates/s", cores=nthreads,
-                kind="port",
+                kind="port", host_physical_cores=phys, host_logical_cpus=logical,
                 sample=f"{steps} steps of the same 1M dam break after 1 untimed step, "
-                       f"FP64 oracle (C/OpenMP, {nthreads} threads), {el:.1f} s")
+                       f"FP64 oracle (C/OpenMP, {nthreads} threads: the host-thread share granted to "
+                       f"this process (OMP_NUM_THREADS); host has {phys} physical cores, "
+                       f"{logical} logical CPUs), {el:.1f} s")
 
 
 def traffic_from_profiles():
@@ -124,15 +161,23 @@ def main():
     g = GpuSolver(sc)
     g.step(args.warmup)
     g.sync()
+    # the timed region: K steps with no event between the kernels
     t0 = time.perf_counter()
-    ms_total, ms_pair = g.time_steps(args.steps)
+    ms_total = g.time_loop(args.steps)
     overflow = g.sync()
     wall = time.perf_counter() - t0
+    # then K more steps with events between the phases, for the per-kernel split
+    ph = g.time_phases(args.steps)
+    g.sync()
+    ms_pair = ph["pair"]
     ms_step = ms_total / args.steps
     value = sc.n_fluid * args.steps / (ms_total * 1e-3)
+    # kernel level: the compulsory bytes of one step over the compacted pair pass alone;
+    # step level: the same bytes at the measured step rate
     achieved = BYTES_PER_UPDATE * sc.n_fluid / (ms_pair * 1e-3) / 1e9
-    log(f"timed {args.steps} steps: {ms_total:.3f} ms (host wall {wall*1e3:.1f} ms), "
-        f"pair kernel {ms_pair*1e3:.1f} us")
+    step_achieved = BYTES_PER_UPDATE * value / 1e9
+    log(f"timed {args.steps} steps: {ms_total:.3f} ms (host wall {wall*1e3:.1f} ms); per step: "
+        f"sort {ph['sort']*1e3:.1f} us, pair {ms_pair*1e3:.1f} us, capped pass {ph['exact']*1e3:.1f} us")
     out = {
         "metric": "particle-updates/s, 1M-particle dam break (plain path)",
         "value": value,
@@ -152,6 +197,10 @@ def main():
                    "virtual_type": args.vtype, "capped_steps": overflow},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "frac_basis": "kernel level: 80 B x fluid particles per launch / mean duration of "
+                                   "the compacted pair pass (HIP events); step_frac: 80 B x "
+                                   "particle-updates/s of the whole step",
+                     "step_achieved": step_achieved, "step_frac": step_achieved / HBM_PEAK_GBS,
                      # the committed counter run covers the default workload only
                      "traffic": (traffic_from_profiles() if args.kernel == "cubic" and
                                  args.h_over_dx == 1.3 and args.scheme == "config" and not args.vtype
@@ -159,7 +208,7 @@ def main():
                      "kernel": f"k_pair_compact<256, {int(sc.kernel)}, {int(sc.mode)}, true, "
                                f"{'true' if 4 * math.pi * args.h_over_dx ** 2 > 0.75 * sc.K else 'false'}> "
                                f"({args.kernel}, {args.scheme}, one scene)",
-                     "kernel_ms": ms_pair},
+                     "kernel_ms": ms_pair, "sort_ms": ph["sort"], "capped_pass_ms": ph["exact"]},
     }
     if args.graph:
         gg = GpuSolver(sc, use_graph=True)

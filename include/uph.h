@@ -191,6 +191,15 @@ int uph_itime(const uph_t *h);
  * whole loop, *ms_pair is the mean duration of the fused pair kernel. */
 int uph_time_steps(uph_t *h, int n_steps, double *ms_total, double *ms_pair);
 
+/* As uph_time_steps, with the step split into its phases (means per step, any
+ * pointer may be NULL): *ms_sort the binning, scan and reorder kernels (and the
+ * mirror rows' kernels), *ms_pair the compacted pair pass, *ms_exact the capped
+ * (K-nearest) pass that follows it, which returns at once unless some query of
+ * the step had more than K neighbours in range.  With all three NULL no event is
+ * recorded between the kernels, and *ms_total times the bare loop. */
+int uph_time_phases(uph_t *h, int n_steps, double *ms_total, double *ms_sort, double *ms_pair,
+                    double *ms_exact);
+
 const char *uph_last_error(const uph_t *h);
 
 /* Diagnostics that sph.cc's output reports, over a scene's current fluid state:

@@ -1010,7 +1010,7 @@ __device__ void wave_overflow(const StepBufs& b, const DevScene* __restrict__ sc
   if (lead) atomicOr(&b.ctl[1], 1);
   // many overflowing lanes (the reference's smoothing length caps every query):
   // each finds its own key at once; what it leaves goes to the wavefront search
-  if (ROWS && ls != nullptr && __popcll(m) >= 8) {
+  if (ROWS && ls != nullptr && b.kmask != nullptr && __popcll(m) >= 8) {
     bool done = false;
     if (ovf) {
       const DevScene& S = sc[sidx];
@@ -1628,7 +1628,7 @@ __device__ __forceinline__ bool exact_pairs(int p, const StepBufs& b, const DevS
   const uint32_t idp = tag_id(tg);
   const bool pb_ok = (int)idp < S.pb_gid;
   const int stamp = b.ctl[3];
-  const bool have_mask = b.kth[p].step == -stamp;
+  const bool have_mask = b.kmask != nullptr && b.kth[p].step == -stamp;
   // Row r's admitted candidates as bits of m[2r] (slots rb[r] + [0, 64)) and m[2r + 1]
   // (slots rb[r] + [64, 128)), by the rules of Search.cc:27-40 (admit<true>).  Pass 1a
   // tests distances and decides the candidates q with a larger original index, whose
@@ -2192,7 +2192,10 @@ static void fill_scene(DevScene& S, const uph_scene& u, double cell_factor) {
   S.dx = u.dx;
 }
 
-static int alloc_bufs(StepBufs& b, int n, int ncells, int nblocks_scan, Err& err, bool search_only) {
+// kmask (48 bytes per slot) is written and read only by the per-lane K-th search of
+// handles whose queries are expected to be capped; other handles leave it null.
+static int alloc_bufs(StepBufs& b, int n, int ncells, int nblocks_scan, Err& err, bool search_only,
+                      bool capped = false) {
   memset(&b, 0, sizeof b);
   if (dalloc(&b.posA, (size_t)n + POS_PAD, err) || dalloc(&b.posB, n, err) || dalloc(&b.tagA, n, err) ||
       dalloc(&b.tagB, n, err) || dalloc(&b.keys, n, err) || dalloc(&b.rank, n, err) ||
@@ -2203,7 +2206,8 @@ static int alloc_bufs(StepBufs& b, int n, int ncells, int nblocks_scan, Err& err
     return -1;
   if (search_only) return 0;
   if (dalloc(&b.velA, n, err) || dalloc(&b.velB, n, err) || dalloc(&b.rhoA, n, err) ||
-      dalloc(&b.rhoB, n, err) || dalloc(&b.rtB, n, err) || dalloc(&b.kmask, (size_t)n * 6, err) ||
+      dalloc(&b.rhoB, n, err) || dalloc(&b.rtB, n, err) ||
+      (capped && dalloc(&b.kmask, (size_t)n * 6, err)) ||
       dalloc(&b.mA, n, err) || dalloc(&b.mB, n, err) || dalloc(&b.deckPos, n, err) ||
       dalloc(&b.deckVel, n, err) || dalloc(&b.deckLvel, n, err) || dalloc(&b.massD, n, err) ||
       dalloc(&b.presA, n, err) || dalloc(&b.presB, n, err) || dalloc(&b.deckP, n, err) ||
@@ -2211,7 +2215,7 @@ static int alloc_bufs(StepBufs& b, int n, int ncells, int nblocks_scan, Err& err
       dalloc(&b.deckRl, n, err) || dalloc(&b.deckPl, n, err) || dalloc(&b.wallKind, n, err) ||
       dalloc(&b.wallRef, n, err))
     return -1;
-  b.kmask_n = n;
+  b.kmask_n = capped ? n : 0;
   return 0;
 }
 
@@ -2276,7 +2280,7 @@ static int launch_sort(hipStream_t st, int n, int ncells, int nb, StepBufs& b, c
 // The pair kernels of one smoothing kernel KT and scheme MODE: the selected pass,
 // then the exact capped fixup (which returns at once unless the step overflowed).
 template <int KT, int MODE>
-static int launch_pairs(uph_handle* h, hipStream_t st, hipEvent_t pb) {
+static int launch_pairs(uph_handle* h, hipStream_t st, hipEvent_t pb, hipEvent_t pc) {
   if (h->opts.pair_variant == 1) {
     hipLaunchKernelGGL((k_pair<false, KT, MODE>), dim3(grid8_of(h->n)), dim3(BLK), 0, st, h->n, h->b, h->dsc);
   } else {
@@ -2304,6 +2308,7 @@ static int launch_pairs(uph_handle* h, hipStream_t st, hipEvent_t pb) {
   if (pb) HIPCHK(hipEventRecord(pb, st), h->err);
   const int gfix = std::min(grid8_of(h->n), FIX_GRID);
   hipLaunchKernelGGL((k_pair<true, KT, MODE>), dim3(gfix), dim3(BLK), 0, st, h->n, h->b, h->dsc);
+  if (pc) HIPCHK(hipEventRecord(pc, st), h->err);
   return 0;
 }
 
@@ -2327,18 +2332,20 @@ static int launch_ghosts(uph_handle* h, hipStream_t st) {
   return 0;
 }
 
-static int launch_step(uph_handle* h, hipEvent_t pa, hipEvent_t pb) {
+// pa, pb, pc (may be null): events recorded after the sort, after the compacted pair
+// pass and after the capped pass.
+static int launch_step(uph_handle* h, hipEvent_t pa, hipEvent_t pb, hipEvent_t pc = nullptr) {
   hipStream_t st = h->stream;
   if (launch_sort(st, h->n, h->ncells, h->cs.nb, h->b, h->dsc, 0, true, h->err)) return -1;
   if (h->b.gtot > 0 && launch_ghosts(h, st)) return -1;
   if (pa) HIPCHK(hipEventRecord(pa, st), h->err);
   int rc;
   if (h->mode == UPH_MODE_REF)
-    rc = h->kernel == UPH_KERNEL_POLY7 ? launch_pairs<UPH_KERNEL_POLY7, UPH_MODE_REF>(h, st, pb)
-                                       : launch_pairs<UPH_KERNEL_CUBIC, UPH_MODE_REF>(h, st, pb);
+    rc = h->kernel == UPH_KERNEL_POLY7 ? launch_pairs<UPH_KERNEL_POLY7, UPH_MODE_REF>(h, st, pb, pc)
+                                       : launch_pairs<UPH_KERNEL_CUBIC, UPH_MODE_REF>(h, st, pb, pc);
   else
-    rc = h->kernel == UPH_KERNEL_POLY7 ? launch_pairs<UPH_KERNEL_POLY7, UPH_MODE_CONFIG>(h, st, pb)
-                                       : launch_pairs<UPH_KERNEL_CUBIC, UPH_MODE_CONFIG>(h, st, pb);
+    rc = h->kernel == UPH_KERNEL_POLY7 ? launch_pairs<UPH_KERNEL_POLY7, UPH_MODE_CONFIG>(h, st, pb, pc)
+                                       : launch_pairs<UPH_KERNEL_CUBIC, UPH_MODE_CONFIG>(h, st, pb, pc);
   if (rc != 0) return -1;
   HIPCHK(hipGetLastError(), h->err);
   return 0;
@@ -2441,7 +2448,7 @@ extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts*
     err.msg = "hipStreamCreate failed";
     return fail();
   }
-  if (alloc_bufs(h->b, h->n, h->ncells, h->cs.nb, err, false)) return fail();
+  if (alloc_bufs(h->b, h->n, h->ncells, h->cs.nb, err, false, h->capped)) return fail();
   h->gfluid = (int)gfluid;
   if (gtot > 0 && alloc_ghost_bufs(h->b, h->n, h->ncells, (int)gtot, (int)wtot, err)) return fail();
   if (h->opts.keep_rates && dalloc(&h->b.dbgRates, (h->mode == UPH_MODE_REF ? 4 : 2) * (size_t)h->n, err))
@@ -2644,6 +2651,57 @@ extern "C" int uph_time_steps(uph_t* h, int n_steps, double* ms_total, double* m
   h->itime += n_steps;
   if (ms_total) *ms_total = tot;
   if (ms_pair) *ms_pair = pair / n_steps;
+  return 0;
+}
+
+extern "C" int uph_time_phases(uph_t* h, int n_steps, double* ms_total, double* ms_sort, double* ms_pair,
+                               double* ms_exact) {
+  if (!h || n_steps <= 0) return -1;
+  // per step: start, after the sort, after the compacted pass, after the capped pass;
+  // none when no phase is asked for (events between kernels cost a few us per step)
+  const bool split = ms_sort || ms_pair || ms_exact;
+  std::vector<hipEvent_t> ev(split ? 4 * n_steps : 0, nullptr);
+  int rc = 0;
+  for (auto& e : ev)
+    if (hipEventCreate(&e) != hipSuccess) {
+      h->err.msg = "uph_time_phases: hipEventCreate failed";
+      rc = -1;
+      break;
+    }
+  if (rc == 0 && hipStreamSynchronize(h->stream) != hipSuccess) rc = -1;
+  if (rc == 0 && hipEventRecord(h->ev0, h->stream) != hipSuccess) rc = -1;
+  for (int k = 0; rc == 0 && k < n_steps; ++k) {
+    if (!split) {
+      if (launch_step(h, nullptr, nullptr, nullptr)) rc = -1;
+    } else if (hipEventRecord(ev[4 * k], h->stream) != hipSuccess ||
+               launch_step(h, ev[4 * k + 1], ev[4 * k + 2], ev[4 * k + 3])) {
+      rc = -1;
+    }
+  }
+  if (rc == 0 && (hipEventRecord(h->ev1, h->stream) != hipSuccess || hipEventSynchronize(h->ev1) != hipSuccess))
+    rc = -1;
+  double sum[4] = {0.0, 0.0, 0.0, 0.0};
+  if (rc == 0) {
+    float tot = 0.f;
+    (void)hipEventElapsedTime(&tot, h->ev0, h->ev1);
+    sum[0] = tot;
+    for (int k = 0; split && k < n_steps; ++k)
+      for (int j = 0; j < 3; ++j) {
+        float t = 0.f;
+        (void)hipEventElapsedTime(&t, ev[4 * k + j], ev[4 * k + j + 1]);
+        sum[j + 1] += t;
+      }
+    h->itime += n_steps;
+  } else if (h->err.msg.empty()) {
+    h->err.msg = "uph_time_phases: a HIP call failed";
+  }
+  for (auto& e : ev)
+    if (e) (void)hipEventDestroy(e);
+  if (rc) return -1;
+  if (ms_total) *ms_total = sum[0];
+  if (ms_sort) *ms_sort = sum[1] / n_steps;
+  if (ms_pair) *ms_pair = sum[2] / n_steps;
+  if (ms_exact) *ms_exact = sum[3] / n_steps;
   return 0;
 }
 

@@ -73,6 +73,7 @@ SIGNATURES = {
     "uph_derivatives_scheme": (C.c_int, [C.c_int, C.c_int]),
     "uph_itime": (C.c_int, [C.c_void_p]),
     "uph_time_steps": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp]),
+    "uph_time_phases": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp]),
     "uph_last_error": (C.c_char_p, [C.c_void_p]),
     # surrogate kernels (pointers are device addresses passed as integers)
     "uph_dev_alloc": (C.c_int, [C.POINTER(C.c_void_p), C.c_size_t]),
@@ -343,3 +344,17 @@ class GpuSolver:
         tot = C.c_double(); pair = C.c_double()
         self._check(lib().uph_time_steps(self.h, int(n), C.byref(tot), C.byref(pair)))
         return tot.value, pair.value
+
+    def time_phases(self, n: int):
+        """Run n steps timed by HIP events: {"total": ms for the whole loop, then the mean
+        ms per step of "sort" (binning, scan, reorder), "pair" (the compacted pair pass)
+        and "exact" (the capped pass, empty unless a query had more than K in range)}."""
+        v = [C.c_double() for _ in range(4)]
+        self._check(lib().uph_time_phases(self.h, int(n), *[C.byref(x) for x in v]))
+        return dict(zip(("total", "sort", "pair", "exact"), (x.value for x in v)))
+
+    def time_loop(self, n: int) -> float:
+        """ms for n steps timed by two HIP events around the loop, none between kernels."""
+        tot = C.c_double()
+        self._check(lib().uph_time_phases(self.h, int(n), C.byref(tot), None, None, None))
+        return tot.value
