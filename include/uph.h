@@ -60,7 +60,26 @@ int uph_search_neighbors_cap(const double *pos_xy, int *i_pair, int *j_pair, dou
                              int n_fluid, int n_particles, int particle_boundary,
                              int n_neighbors, double dist, long capacity);
 
-/* Message of the last failure of SearchNeighbors / uph_search_neighbors_cap. */
+/* uph_search_neighbors_cap without any allocation: no hipMalloc, hipFree, pinned
+ * or host heap allocation in the call.  No counterpart in the reference; it is for
+ * callers that cannot allocate (serving loops, code whose device memory is fixed in
+ * advance).  Every device buffer of the search lives in one file-scope device array
+ * of 32.5 MB that each GPU the library runs on holds until the process ends; its
+ * device address is looked up once per device at the first call there.
+ * Limits: n_particles <= 65536 and at most 1,572,864 pairs; beyond them it returns
+ * -3 with a message and writes nothing.  -2: more pairs than `capacity` (nothing
+ * written); -1: invalid input or a device error.  Inputs and outputs are host
+ * arrays, as for uph_search_neighbors_cap; the work runs on the null stream and the
+ * call returns when the pairs are in the caller's buffers.  Every call on one GPU
+ * shares that GPU's array, which keeps the last call's contents (each call clears
+ * what it needs zeroed): calls from several host threads of this process are
+ * serialized by a lock, and two calls that overlapped on one GPU would race on it. */
+int uph_search_neighbors_noalloc(const double *pos_xy, int *i_pair, int *j_pair, double *rij2,
+                                 int n_fluid, int n_particles, int particle_boundary,
+                                 int n_neighbors, double dist, long capacity);
+
+/* Message of the last failure of SearchNeighbors / uph_search_neighbors_cap /
+ * uph_search_neighbors_noalloc. */
 const char *uph_search_last_error(void);
 
 /* ---------------------------------------------------------- stepping API */

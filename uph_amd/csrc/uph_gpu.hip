@@ -28,6 +28,7 @@
 #include <string>
 #include <vector>
 #include <mutex>
+#include <atomic>
 #include <algorithm>
 #include <cmath>
 #include <type_traits>
@@ -3012,6 +3013,8 @@ struct SearchCtx {
   double2* hpos = nullptr;  // pinned staging
 };
 SearchCtx g_search;
+// the last search call was uph_search_neighbors_noalloc: uph_search_last_error reports its message
+std::atomic<bool> g_search_err_noalloc{false};
 
 int search_reserve(SearchCtx& c, int n, int ncells, int nb) {
   if (!c.st) HIPCHK(hipStreamCreateWithFlags(&c.st, hipStreamNonBlocking), c.err);
@@ -3040,34 +3043,18 @@ int search_reserve(SearchCtx& c, int n, int ncells, int nb) {
   return 0;
 }
 
-int search_run(const double* const* rows, const double* flat, int* i_pair, int* j_pair, double* rij2, int nf, int n,
-               int pb, int K, double dist, long capacity) {
-  SearchCtx& c = g_search;
-  std::lock_guard<std::mutex> lk(c.mu);
-  c.err.msg.clear();
-  if (nf <= 0 || n <= 0) return 0;
-  if (nf > n || K <= 0 || !(dist > 0)) {
-    c.err.msg = "SearchNeighbors: invalid sizes";
-    return -1;
-  }
-  if (n >= (int)UPH_ID_MASK) {
-    c.err.msg = "SearchNeighbors: too many particles";
-    return -1;
-  }
-  // one grid over the bounding box of all particles
+// The search's scene: one grid over the bounding box of all particles, at most
+// 4 n + 1024 cells.  Returns false on a non-finite position.
+bool search_grid(const double* const* rows, const double* flat, int nf, int n, int pb, int K, double dist,
+                 DevScene& S) {
   double lx = INFINITY, ly = INFINITY, hx = -INFINITY, hy = -INFINITY;
-  std::vector<double2> tmp;
   for (int i = 0; i < n; ++i) {
     const double x = rows ? rows[i][0] : flat[2 * i];
     const double y = rows ? rows[i][1] : flat[2 * i + 1];
-    if (!std::isfinite(x) || !std::isfinite(y)) {
-      c.err.msg = "SearchNeighbors: non-finite position";
-      return -1;
-    }
+    if (!std::isfinite(x) || !std::isfinite(y)) return false;
     lx = std::min(lx, x); hx = std::max(hx, x);
     ly = std::min(ly, y); hy = std::max(hy, y);
   }
-  DevScene S;
   memset(&S, 0, sizeof S);
   S.cutoff = dist;
   const double cs = sqrt(dist) * (1.0 + 1e-9);
@@ -3092,6 +3079,29 @@ int search_run(const double* const* rows, const double* flat, int* i_pair, int* 
   S.nf = nf;
   S.pb_gid = pb;
   S.K = K;
+  return true;
+}
+
+int search_run(const double* const* rows, const double* flat, int* i_pair, int* j_pair, double* rij2, int nf, int n,
+               int pb, int K, double dist, long capacity) {
+  SearchCtx& c = g_search;
+  std::lock_guard<std::mutex> lk(c.mu);
+  g_search_err_noalloc = false;
+  c.err.msg.clear();
+  if (nf <= 0 || n <= 0) return 0;
+  if (nf > n || K <= 0 || !(dist > 0)) {
+    c.err.msg = "SearchNeighbors: invalid sizes";
+    return -1;
+  }
+  if (n >= (int)UPH_ID_MASK) {
+    c.err.msg = "SearchNeighbors: too many particles";
+    return -1;
+  }
+  DevScene S;
+  if (!search_grid(rows, flat, nf, n, pb, K, dist, S)) {
+    c.err.msg = "SearchNeighbors: non-finite position";
+    return -1;
+  }
   const int ncells = S.ncx * S.ncy;
   const int nb = (ncells + SCAN_TILE - 1) / SCAN_TILE;
   if (search_reserve(c, n, ncells, nb)) return -1;
@@ -3505,7 +3515,187 @@ extern "C" int uph_search_neighbors_cap(const double* pos_xy, int* i_pair, int* 
                     capacity);
 }
 
-extern "C" const char* uph_search_last_error(void) { return g_search.err.msg.c_str(); }
+// ------------------------------------- SearchNeighbors without allocation
+//
+// uph_search_neighbors_noalloc: the search of uph_search_neighbors_cap with every
+// device buffer (sort arrays, scene table, per-query counts, pair staging) carved out
+// of one file-scope device array of fixed size, so a call allocates nothing.  The
+// array's device address is looked up once per device with hipGetSymbolAddress and
+// only that address is handed to kernels and copies.
+
+namespace {
+
+#define NOALLOC_MAX_PARTICLES 65536
+#define NOALLOC_MAX_CELLS (4 * NOALLOC_MAX_PARTICLES + 1024)   // the search grid's bound
+#define NOALLOC_MAX_PAIRS 1572864
+#define NOALLOC_MAX_DEVICES 64
+
+constexpr size_t na_up(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Byte offsets of the buffers inside the scratch array.
+struct NaLayout {
+  size_t dsc, posA, posB, tagA, tagB, keys, rank, tmpIdx, sceneOf, kth, cellStart, cellCount, partial, ctl, qcount,
+      qoff, qpartial, ip, jp, r2, total;
+};
+
+constexpr NaLayout na_layout() {
+  const size_t P = NOALLOC_MAX_PARTICLES, C = NOALLOC_MAX_CELLS;
+  const size_t nb1 = (C + SCAN_TILE - 1) / SCAN_TILE, nb2 = (P + SCAN_TILE - 1) / SCAN_TILE + 1;
+  const size_t NB = nb1 > nb2 ? nb1 : nb2;
+  NaLayout L{};
+  size_t o = 0;
+  L.dsc = o; o += na_up(sizeof(DevScene));
+  L.posA = o; o += na_up((P + POS_PAD) * sizeof(double2));
+  L.posB = o; o += na_up(P * sizeof(double2));
+  L.tagA = o; o += na_up(P * sizeof(uint32_t));
+  L.tagB = o; o += na_up(P * sizeof(uint32_t));
+  L.keys = o; o += na_up(P * sizeof(int));
+  L.rank = o; o += na_up(P * sizeof(int));
+  L.tmpIdx = o; o += na_up(P * sizeof(int));
+  L.sceneOf = o; o += na_up(P * sizeof(int));
+  L.kth = o; o += na_up(P * sizeof(KthRec));
+  L.cellStart = o; o += na_up((C + 1) * sizeof(int));
+  L.cellCount = o; o += na_up(C * sizeof(int));
+  L.partial = o; o += na_up(NB * sizeof(int));
+  L.ctl = o; o += na_up(8 * sizeof(int));
+  L.qcount = o; o += na_up((P + 1) * sizeof(int));
+  L.qoff = o; o += na_up((P + 1) * sizeof(int));
+  L.qpartial = o; o += na_up(NB * sizeof(int));
+  L.ip = o; o += na_up((size_t)NOALLOC_MAX_PAIRS * sizeof(int));
+  L.jp = o; o += na_up((size_t)NOALLOC_MAX_PAIRS * sizeof(int));
+  L.r2 = o; o += na_up((size_t)NOALLOC_MAX_PAIRS * sizeof(double));
+  L.total = o;
+  return L;
+}
+
+constexpr NaLayout NA = na_layout();
+
+}  // namespace
+
+// 32.5 MB; every device the library runs on holds its own copy until the process ends.
+static __device__ __align__(256) unsigned char scratch[NA.total];
+
+namespace {
+
+std::mutex g_na_mu;
+unsigned char* g_na_ptr[NOALLOC_MAX_DEVICES] = {};   // scratch's device address, per device
+char g_na_err[256] = "";
+
+// The buffers' zero-start state of one call: input tags (fluid below nf), scene 0 for
+// every particle, empty cells, no pairs counted.
+__global__ void k_na_clear(int n, int nf, int ncells, uint32_t* __restrict__ tag, int* __restrict__ sceneOf,
+                           int* __restrict__ cellCount, int* __restrict__ qcount) {
+  const int stride = gridDim.x * blockDim.x;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ncells || i <= nf || i < n; i += stride) {
+    if (i < n) {
+      tag[i] = ((uint32_t)(i < nf ? 1 : 2) << UPH_ID_BITS) | (uint32_t)i;
+      sceneOf[i] = 0;
+    }
+    if (i < ncells) cellCount[i] = 0;
+    if (i <= nf) qcount[i] = 0;
+  }
+}
+
+int na_fail(int rc, const char* msg) {
+  snprintf(g_na_err, sizeof g_na_err, "uph_search_neighbors_noalloc: %s", msg);
+  return rc;
+}
+
+#define NACHK(x)                                              \
+  do {                                                        \
+    hipError_t e_ = (x);                                      \
+    if (e_ != hipSuccess) return na_fail(-1, hipGetErrorString(e_)); \
+  } while (0)
+
+}  // namespace
+
+extern "C" int uph_search_neighbors_noalloc(const double* pos_xy, int* i_pair, int* j_pair, double* rij2,
+                                            int n_fluid, int n_particles, int particle_boundary, int n_neighbors,
+                                            double dist, long capacity) {
+  std::lock_guard<std::mutex> lk(g_na_mu);
+  g_search_err_noalloc = true;
+  g_na_err[0] = 0;
+  const int nf = n_fluid, n = n_particles;
+  if (nf <= 0 || n <= 0) return 0;
+  if (nf > n || n_neighbors <= 0 || !(dist > 0)) return na_fail(-1, "invalid sizes");
+  if (n > NOALLOC_MAX_PARTICLES) {
+    char m[160];
+    snprintf(m, sizeof m, "%d particles exceed this entry point's fixed maximum of %d", n, NOALLOC_MAX_PARTICLES);
+    return na_fail(-3, m);
+  }
+  DevScene S;
+  if (!search_grid(nullptr, pos_xy, nf, n, particle_boundary, n_neighbors, dist, S))
+    return na_fail(-1, "non-finite position");
+  const int ncells = S.ncx * S.ncy;
+  if (ncells > NOALLOC_MAX_CELLS) return na_fail(-3, "search grid beyond the fixed maximum of cells");
+  const int nb = (ncells + SCAN_TILE - 1) / SCAN_TILE;
+  // set-up, once per device: the device address of this device's copy of scratch
+  int dev = 0;
+  NACHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return na_fail(-1, "device index beyond the supported range");
+  if (!g_na_ptr[dev]) {
+    void* ptr = nullptr;
+    NACHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_na_ptr[dev] = static_cast<unsigned char*>(ptr);
+  }
+  unsigned char* base = g_na_ptr[dev];
+  StepBufs b;
+  memset(&b, 0, sizeof b);
+  b.posA = reinterpret_cast<double2*>(base + NA.posA);
+  b.posB = reinterpret_cast<double2*>(base + NA.posB);
+  b.tagA = reinterpret_cast<uint32_t*>(base + NA.tagA);
+  b.tagB = reinterpret_cast<uint32_t*>(base + NA.tagB);
+  b.keys = reinterpret_cast<int*>(base + NA.keys);
+  b.rank = reinterpret_cast<int*>(base + NA.rank);
+  b.tmpIdx = reinterpret_cast<int*>(base + NA.tmpIdx);
+  b.sceneOf = reinterpret_cast<int*>(base + NA.sceneOf);
+  b.kth = reinterpret_cast<KthRec*>(base + NA.kth);
+  b.cellStart = reinterpret_cast<int*>(base + NA.cellStart);
+  b.cellCount = reinterpret_cast<int*>(base + NA.cellCount);
+  b.partial = reinterpret_cast<int*>(base + NA.partial);
+  b.ctl = reinterpret_cast<int*>(base + NA.ctl);
+  DevScene* dsc = reinterpret_cast<DevScene*>(base + NA.dsc);
+  int* qcount = reinterpret_cast<int*>(base + NA.qcount);
+  int* qoff = reinterpret_cast<int*>(base + NA.qoff);
+  int* qpartial = reinterpret_cast<int*>(base + NA.qpartial);
+  int* ip = reinterpret_cast<int*>(base + NA.ip);
+  int* jp = reinterpret_cast<int*>(base + NA.jp);
+  double* r2 = reinterpret_cast<double*>(base + NA.r2);
+  hipStream_t st = nullptr;   // the null stream: the copies below are synchronous
+  NACHK(hipMemcpy(dsc, &S, sizeof S, hipMemcpyHostToDevice));
+  NACHK(hipMemcpy(b.posB, pos_xy, (size_t)n * sizeof(double2), hipMemcpyHostToDevice));
+  const int span = std::max(std::max(ncells, nf + 1), n);
+  hipLaunchKernelGGL(k_na_clear, dim3(std::min(grid_of(span), 1024)), dim3(BLK), 0, st, n, nf, ncells, b.tagA,
+                     b.sceneOf, b.cellCount, qcount);
+  Err err;
+  if (launch_sort(st, n, ncells, nb, b, dsc, 1, false, err)) return na_fail(-1, "sort launch failed");
+  hipLaunchKernelGGL(k_search_count, dim3(grid8_of(n)), dim3(BLK), 0, st, nf, n, b, dsc, qcount);
+  const int qnb = (nf + SCAN_TILE - 1) / SCAN_TILE;
+  hipLaunchKernelGGL(k_scan_reduce, dim3(qnb), dim3(SCAN_BLK), 0, st, nf, qcount, qpartial, (int*)nullptr);
+  hipLaunchKernelGGL(k_scan_down, dim3(qnb), dim3(SCAN_BLK), 0, st, nf, qcount, qpartial, qoff);
+  NACHK(hipGetLastError());
+  int niac = 0;
+  NACHK(hipMemcpy(&niac, qoff + nf, sizeof(int), hipMemcpyDeviceToHost));
+  if (niac < 0) return na_fail(-1, "pair count overflow");
+  if ((long)niac > capacity) return na_fail(-2, "more pairs than the caller's capacity");
+  if (niac > NOALLOC_MAX_PAIRS) {
+    char m[160];
+    snprintf(m, sizeof m, "%d pairs exceed this entry point's fixed maximum of %d", niac, NOALLOC_MAX_PAIRS);
+    return na_fail(-3, m);
+  }
+  if (niac > 0) {
+    hipLaunchKernelGGL(k_search_emit, dim3(grid8_of(n)), dim3(BLK), 0, st, n, b, dsc, qoff, ip, jp, r2);
+    NACHK(hipGetLastError());
+    NACHK(hipMemcpy(i_pair, ip, (size_t)niac * sizeof(int), hipMemcpyDeviceToHost));
+    NACHK(hipMemcpy(j_pair, jp, (size_t)niac * sizeof(int), hipMemcpyDeviceToHost));
+    NACHK(hipMemcpy(rij2, r2, (size_t)niac * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  return niac;
+}
+
+extern "C" const char* uph_search_last_error(void) {
+  return g_search_err_noalloc ? g_na_err : g_search.err.msg.c_str();
+}
 
 // ------------------------------------------------------ derivatives() compat
 //

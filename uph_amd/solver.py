@@ -56,6 +56,8 @@ SIGNATURES = {
                                   C.c_int, C.c_double]),
     "uph_search_neighbors_cap": (C.c_int, [_dp, _ip, _ip, _dp, C.c_int, C.c_int, C.c_int, C.c_int,
                                            C.c_double, C.c_long]),
+    "uph_search_neighbors_noalloc": (C.c_int, [_dp, _ip, _ip, _dp, C.c_int, C.c_int, C.c_int, C.c_int,
+                                               C.c_double, C.c_long]),
     "uph_search_last_error": (C.c_char_p, []),
     "uph_create": (C.c_int, [C.POINTER(UphScene), C.c_int, C.POINTER(UphOpts),
                              C.POINTER(C.c_void_p)]),
