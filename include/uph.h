@@ -235,6 +235,19 @@ typedef struct uph_diag {
 } uph_diag;
 int uph_diagnostics(uph_t *h, int scene, double probe_x, double probe_y, uph_diag *out);
 
+/* The second pressure probe (NeighborsToPressurePointTwo + EnsembleAveragedPointPressureTwo,
+ * derivatives.c:2340-2393, written by output.c:77-92): the mean of the positive
+ * DynamicPressure of the fluid rows strictly within 2h of (x, y), and their count (0 and
+ * 0 when none).  The reference reads the location from pressureProbeXX/YY, which nothing
+ * assigns (System.c:130-131 is commented out); here the caller names it. */
+int uph_probe_two(uph_t *h, int scene, double x, double y, double *pressure, int *neighbors);
+
+/* approxVorticity of the fluid rows (ApproximateVortcity, derivatives.c:2840-2885) as the
+ * last step's derivatives() evaluation computed it (positions at the start of the step,
+ * predicted velocities and densities, the step's pair set), n_fluid values in the
+ * reference's order. */
+int uph_vorticity(uph_t *h, int scene, double *out);
+
 /* Device-side state exchange with the surrogate (no counterpart in the reference).
  * Arrays are device pointers in the reference's order (pos/vel: 2n doubles);
  * uph_state_to_device writes what uph_get_state returns.

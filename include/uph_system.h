@@ -173,7 +173,9 @@ typedef struct uph_system {
  * (where allocated) = 0; the diagnostics of derivatives.c:4317-4324
  * (totalKineticEnergy, potentialEnergy, pointNeighbors, pointPressure,
  * pressureNorm, consistencyDensity*, densityUncertainty) of the state it was
- * called with, probe at pressureProbeX/Y.  Boundary rows are reset to the state they had at the
+ * called with, probe at pressureProbeX/Y; pointPressureTwo and pointNeighborsTwo
+ * at pressureProbeXX/YY (derivatives.c:2340-2393); approxVorticity of the fluid rows
+ * (derivatives.c:2840-2885) where the System has that array.  Boundary rows are reset to the state they had at the
  * first call (the deck, as inputbp re-reads it every step) with the moving rows'
  * kinematics of inputbp.c:192-283 at time t: movingBoundary 1 (piston), 3 (lid
  * at lidVelocity), 2 (gate rising at lidVelocity from tempPositionGate until

@@ -12,6 +12,11 @@ same inputs:
   waves.npz            : FluidParticle.dat + Fixed/MovingBoundaryParticle.dat +
                          parameters.dat (piston wavemaker)
   hydrostatic_probe.npz: recorded time series PointPressure.dat / TotalKineticEnergy.dat
+  hydrostatic_start.npz: FluidParticle.dat (the lattice the recorded run started from) +
+                         FixedBoundaryParticle.dat + parameters.dat values; the recorded
+                         run's output() appended its state at step
+                         NumberOfInitializationSteps = 30000 as FluidParticleInit.dat
+                         (hydrostatic_init.npz)
 """
 import os
 import sys
@@ -40,6 +45,7 @@ def save(name, d, fluid_file="FluidParticle.dat"):
 
 def main():
     save("hydrostatic_init.npz", "hydrostaticPressure", "FluidParticleInit.dat")
+    save("hydrostatic_start.npz", "hydrostaticPressure")
     save("dambreak.npz", "dambreak")
     save("waves.npz", "waves")
     pp = np.loadtxt(os.path.join(REF, "hydrostaticPressure", "PointPressure.dat"))

@@ -2,6 +2,7 @@
 import numpy as np
 import pytest
 
+from conftest import load_golden_scene
 from oracle import diag_ref
 from uph_amd import scenes
 
@@ -58,3 +59,60 @@ def test_diagnostics_of_a_batch_scene():
     g.step(10)
     for k, sc in enumerate(scs):
         _close(g.diagnostics((0.2, 0.1), scene=k), _expect(sc, g.state(k), (0.2, 0.1)))
+
+
+def _moving(sc, seed=2, amp=0.3):
+    rng = np.random.default_rng(seed)
+    nf = sc.n_fluid
+    sc.vel[:nf] = rng.normal(0, amp, (nf, 2))
+    sc.lvel[:nf] = sc.vel[:nf] + rng.normal(0, 0.1 * amp, (nf, 2))
+    return sc
+
+
+def _vort_case(case):
+    if case == "cubic":
+        return _moving(scenes.dam_break(jitter=0.1, seed=4))
+    if case == "poly7":
+        sc = _moving(scenes.dam_break(jitter=0.1, seed=6))
+        sc.kernel = scenes.KERNEL_POLY7
+        return sc
+    if case == "capped":
+        return _moving(load_golden_scene("dambreak.npz"), amp=0.1)
+    sc = load_golden_scene("dambreak.npz", mode=scenes.MODE_REF, kernel=scenes.KERNEL_POLY7, alpha_av=0.0)
+    return _moving(sc, amp=0.1)
+
+
+@pytest.mark.parametrize("case", ["cubic", "poly7", "capped", "ref_poly7_capped"])
+def test_vorticity_matches_the_statement(case):
+    """approxVorticity of the first step's evaluation (the initial state: the first step
+    predicts nothing) against the FP64 statement over the oracle's pair list (K cap
+    included on the reference's dambreak deck).  The device sums FP64 pair terms of
+    the FP32 neighbour mirror; tolerance 1e-5 of the largest magnitude."""
+    from oracle import oracle as O
+    from uph_amd.solver import GpuSolver
+    sc = _vort_case(case)
+    g = GpuSolver(sc)
+    g.step(1)
+    got = g.vorticity()
+    ip, jp, _ = O.search(sc.pos, sc.n_fluid, sc.n, sc.particle_boundary, sc.K, sc.cutoff2)
+    exp = diag_ref.approx_vorticity(sc.pos, sc.vel, sc.rho, sc.mass, sc.h, ip, jp, sc.n_fluid,
+                                    1 if sc.kernel == scenes.KERNEL_POLY7 else 0)
+    err = float(np.abs(got - exp).max() / np.abs(exp).max())
+    _say(f"vorticity {case}: n={sc.n}, max |w| {np.abs(exp).max():.3e}, rel-to-max error {err:.2e}")
+    assert err < 1e-5
+
+
+def test_probe_two_matches_the_statement():
+    from uph_amd.solver import GpuSolver
+    sc = scenes.dam_break(jitter=0.05, seed=8)
+    g = GpuSolver(sc)
+    g.step(30)
+    st = g.state()
+    nf = sc.n_fluid
+    P = sc.B * ((st["rho"][:nf] / sc.rho0) ** sc.gamma - 1.0)
+    for loc in ((0.5, 0.3), (0.2, 1.0), (50.0, 50.0)):
+        p, n = g.probe_two(*loc)
+        pe, ne = diag_ref.probe_two(st["x"][:nf], P, sc.h, *loc)
+        _say(f"probe two at {loc}: {p:.6f} ({n}) vs {pe:.6f} ({ne})")
+        assert n == ne
+        assert np.isclose(p, pe, rtol=1e-11, atol=0)

@@ -6,10 +6,12 @@ Tolerances: rates to 1e-5 of their largest magnitude (fluctuations, whose sums c
 to a few per mille of their terms, to 1e-4), states to 1e-6 relative after the stated
 number of steps.
 """
+import os
+
 import numpy as np
 import pytest
 
-from conftest import load_golden_scene
+from conftest import GOLDEN, load_golden_scene
 from uph_amd import scenes
 
 pytestmark = pytest.mark.gpu
@@ -151,3 +153,50 @@ def test_mode_must_be_uniform_across_a_batch():
         GpuSolver([a, b])
     with pytest.raises(RuntimeError):
         GpuSolver(a, keep_rates=True).ref_rates()
+
+
+def test_ref_hydrostatic_deck_against_the_reference_recorded_run():
+    """The reference's own recorded run of tests/hydrostaticPressure: started from its
+    lattice deck (hydrostatic_start.npz), it appended the probe (0.98, 0.06) pressure and
+    the kinetic energy every 5000 steps (PointPressure.dat, TotalKineticEnergy.dat:
+    hydrostatic_probe.npz) and its fluid state at step 30000 (FluidParticleInit.dat:
+    hydrostatic_init.npz).  The same deck, scheme and kernel here (reference scheme,
+    PolyOrderSeven, hsml = eta dx, its springs) for 30000 steps, sampled at the same
+    steps.  The reference's run is not reproducible bit for bit (OpenMP reductions in
+    thread order), and its probe reads the state inside derivatives() rather than after
+    the step, so the comparison is loose."""
+    from uph_amd.solver import GpuSolver
+    sc = load_golden_scene("hydrostatic_start.npz", mode=scenes.MODE_REF, kernel=scenes.KERNEL_POLY7,
+                           alpha_av=0.0)
+    z = np.load(os.path.join(GOLDEN, "hydrostatic_probe.npz"), allow_pickle=False)
+    rec_p, rec_ke = z["point_pressure"], z["kinetic_energy"]
+    probe = (0.98, 0.06)
+    g = GpuSolver(sc, use_graph=True)
+    got_p, got_ke = [], []
+    for k in range(6):
+        g.step(5000)
+        d = g.diagnostics(probe)
+        got_p.append(d["probe_pressure"])
+        got_ke.append(d["kinetic_energy"])
+    assert g.sync() == 0
+    st = g.state()
+    ref = load_golden_scene("hydrostatic_init.npz")
+    nf = sc.n_fluid
+    drho = np.abs(st["rho"][:nf] - ref.rho[:nf]) / ref.rho[:nf]
+    dpos = np.linalg.norm(st["x"][:nf] - ref.pos[:nf], axis=1) / sc.dx
+    move = np.linalg.norm(ref.pos[:nf] - sc.pos[:nf], axis=1) / sc.dx
+    rp = np.asarray(got_p) / rec_p[:6, 1]
+    rk = np.asarray(got_ke) / rec_ke[:6, 1]
+    _say(f"recorded run, t = 0.05..0.3 s: probe P here {np.round(got_p, 1).tolist()}, recorded "
+         f"{np.round(rec_p[:6, 1], 1).tolist()} (ratio {np.round(rp, 4).tolist()}); KE here "
+         f"{np.round(got_ke, 5).tolist()}, recorded {np.round(rec_ke[:6, 1], 5).tolist()} "
+         f"(ratio {np.round(rk, 3).tolist()})")
+    _say(f"state at step 30000 vs the reference's FluidParticleInit.dat: density rel diff mean "
+         f"{drho.mean():.2e} max {drho.max():.2e}; position diff mean {dpos.mean():.2e} dx, max "
+         f"{dpos.max():.2e} dx (the reference's particles moved {move.mean():.3f} dx on average, max "
+         f"{move.max():.3f} dx)")
+    # the recorded values carry 10 decimals (positions 1e-10 m = 1e-8 dx)
+    assert np.all(np.abs(rp - 1.0) < 1e-3)
+    assert np.all(np.abs(rk - 1.0) < 2e-3)
+    assert drho.max() < 1e-6
+    assert dpos.max() < 1e-3

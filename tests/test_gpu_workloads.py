@@ -248,8 +248,17 @@ def test_c_host_driver_writes_the_reference_output(tmp_path):
             assert np.array_equal(rows[:, 0], np.arange(sc.n))
         ke = np.loadtxt(d / "TotalKineticEnergy.dat")
         pp = np.loadtxt(d / "PointPressure.dat")
-        assert ke.shape == (4, 3) and pp.shape == (4, 5)
+        p2 = np.loadtxt(d / "PointPressureTwo.dat")
+        assert ke.shape == (4, 3) and pp.shape == (4, 5) and p2.shape == (4, 3)
         assert np.allclose(ke[:, 0], np.array([25, 50, 75, 100]) * sc.dt, rtol=1e-9)
+        # the second probe (at the first probe's location by default): its mean of the
+        # positive pressures within 2h, in both files
+        assert np.all(p2[:, 1] > 0) and np.array_equal(pp[:, 4], p2[:, 2])
+        # damGateFlag (input.c:148-159): fluid rows by their deck x, 1 up to 0.38 m, else -1
+        rows = np.loadtxt(d / "out100.dat")
+        flag = np.where(sc.pos[:sc.n_fluid, 0] <= 0.38, 1, -1)
+        assert np.array_equal(rows[:sc.n_fluid, 9], flag) and np.all(rows[sc.n_fluid:, 9] == 0)
+        assert np.any(rows[:sc.n_fluid, 10] != 0) and np.all(rows[sc.n_fluid:, 10] == 0)
         final = np.loadtxt(d / "FluidParticleFinal.dat", skiprows=1)
         snap = np.loadtxt(d / "out100.dat")[:sc.n_fluid]
         assert np.allclose(snap[:, 1:3], final[:, 0:2], rtol=0, atol=2e-10)
@@ -258,6 +267,63 @@ def test_c_host_driver_writes_the_reference_output(tmp_path):
     nf = sc.n_fluid
     _say(f"out100.dat device vs compat: max |dx| {np.abs(a[:nf, 1:3] - b[:nf, 1:3]).max():.2e}")
     assert np.allclose(a[:nf, 1:6], b[:nf, 1:6], rtol=0, atol=1e-9)
+    # approxVorticity of the last evaluation: the device loop's pair kernel inputs and the
+    # state derivatives() was called with are the same state
+    wa, wb = a[:nf, 10], b[:nf, 10]
+    _say(f"approxVorticity device vs compat: max |diff| {np.abs(wa - wb).max():.2e} of max {np.abs(wb).max():.2e}")
+    assert np.abs(wa - wb).max() <= 1e-6 * np.abs(wb).max()
+
+
+def test_c_host_driver_restart_from_fluid_particle_init(tmp_path):
+    """FluidParticleInit.dat (output.c:113-128), written at step NumberOfInitializationSteps,
+    holds the fluid rows of that step in the deck format: its values are the snapshot's at
+    the printed decimals, and a run started from it (as FluidParticle.dat) by the C driver
+    equals the Python host started from the same file, bit for bit at the printed
+    decimals.  Against the uninterrupted run the restart differs by the rounding to 10
+    decimals and the first-step form of the leapfrog (sph.cc:154-167), which it takes
+    again (half a step of the rates where the run would take a whole one): reported,
+    and bounded loosely."""
+    import shutil
+    from uph_amd.scenes import load_deck_dir
+    from uph_amd.solver import GpuSolver
+    drv = os.path.join(ROOT, "uph_amd", "uph_run")
+    sc = _deck_dir(tmp_path, "hydrostatic_init.npz", print_step=25)
+    prm = (tmp_path / "parameters.dat").read_text().split()
+    prm[29] = "50"                          # NumberOfInitializationSteps (field 30)
+    (tmp_path / "parameters.dat").write_text("\n".join(prm) + "\n")
+    out = subprocess.run([drv, "100", "--alpha", "0.1"], cwd=tmp_path, capture_output=True, text=True,
+                         timeout=300)
+    assert out.returncode == 0, out.stderr
+    init = np.loadtxt(tmp_path / "FluidParticleInit.dat", skiprows=1)
+    with open(tmp_path / "FluidParticleInit.dat") as f:
+        assert int(f.readline()) == sc.n_fluid
+    snap = np.loadtxt(tmp_path / "out50.dat")[:sc.n_fluid]
+    assert init.shape == (sc.n_fluid, 12)
+    assert np.allclose(init[:, 0:5], snap[:, 1:6], rtol=0, atol=1.5e-10)     # x, y, vx, vy, rho
+    assert np.allclose(init[:, 5], snap[:, 8], rtol=0, atol=1.5e-10)         # DynamicPressure
+    assert np.array_equal(init[:, 9], np.ones(sc.n_fluid))                 # itype
+    rd = tmp_path / "restart"
+    rd.mkdir()
+    for f in ("parameters.dat", "FixedBoundaryParticle.dat", "MovingBoundaryParticle.dat", "SolidParticle.dat"):
+        shutil.copy(tmp_path / f, rd / f)
+    shutil.copy(tmp_path / "FluidParticleInit.dat", rd / "FluidParticle.dat")
+    out = subprocess.run([drv, "50", "--alpha", "0.1", "--no-output"], cwd=rd, capture_output=True, text=True,
+                         timeout=300)
+    assert out.returncode == 0, out.stderr
+    c_fin = np.loadtxt(rd / "FluidParticleFinal.dat", skiprows=1)
+    rs = load_deck_dir(str(rd), alpha_av=0.1)
+    g = GpuSolver(rs)
+    g.step(50)
+    st = g.state()
+    nf = sc.n_fluid
+    py_fin = np.column_stack([st["x"][:nf], st["v"][:nf], st["rho"][:nf]])
+    printed = np.array([float("%.10f" % v) for v in py_fin.ravel()]).reshape(py_fin.shape)
+    assert np.array_equal(printed, c_fin[:, 0:5])
+    full = np.loadtxt(tmp_path / "FluidParticleFinal.dat", skiprows=1)
+    drho = float(np.max(np.abs(c_fin[:, 4] - full[:, 4]) / full[:, 4]))
+    dx = float(np.abs(c_fin[:, 0:2] - full[:, 0:2]).max()) / sc.dx
+    _say(f"restart at step 50 vs the uninterrupted run at step 100: rho {drho:.2e}, x {dx:.2e} dx")
+    assert drho < 1e-3 and dx < 1e-2
 
 
 def test_c_host_driver_poly7_on_the_waves_deck(tmp_path):

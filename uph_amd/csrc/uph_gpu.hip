@@ -2124,6 +2124,8 @@ struct uph_handle {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double* diag = nullptr;   // per-block partial sums of uph_diagnostics
   int diag_cap = 0;
+  double* vort = nullptr;   // uph_vorticity's output, n_fluid of the largest scene asked for
+  int vort_cap = 0;
   // boundary rows' deck and kinematics (host copies, by global index), from which the
   // state getters place them at the last step's time
   std::vector<double2> deck_pos, deck_vel;
@@ -2578,6 +2580,7 @@ extern "C" void uph_destroy(uph_t* h) {
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
   if (h->diag) (void)hipFree(h->diag);
+  if (h->vort) (void)hipFree(h->vort);
   free_bufs(h->b);
   if (h->dsc) (void)hipFree(h->dsc);
   if (h->mixedWaves) (void)hipFree(h->mixedWaves);
@@ -2903,7 +2906,7 @@ extern "C" const char* uph_last_error(const uph_t* h) { return h ? h->err.msg.c_
 // (:2290-2338, pressure from the Tait equation of state) and
 // DensityConsistencyFunctions (:3090-3117).  Per-block partial sums in FP64 are
 // added on the host in block order, so the result does not depend on timing.
-#define DIAG_Q 7
+#define DIAG_Q 9
 #define DIAG_BLK 256
 
 __global__ __launch_bounds__(DIAG_BLK) void k_diag(int poff, int np, const uint32_t* __restrict__ tag,
@@ -2911,7 +2914,8 @@ __global__ __launch_bounds__(DIAG_BLK) void k_diag(int poff, int np, const uint3
                                                     const double* __restrict__ rho, const double* __restrict__ massD,
                                                     const double* __restrict__ pres,
                                                     const DevScene* __restrict__ sc, int scene, double px, double py,
-                                                    double inv_nf, double* __restrict__ part) {
+                                                    double px2, double py2, double inv_nf,
+                                                    double* __restrict__ part) {
   __shared__ double red[DIAG_Q][DIAG_BLK];
   const int k = blockIdx.x * DIAG_BLK + threadIdx.x;
   double q[DIAG_Q];
@@ -2941,6 +2945,14 @@ __global__ __launch_bounds__(DIAG_BLK) void k_diag(int poff, int np, const uint3
       }
       q[5] = rh * inv_nf;   // localDensity / Nf: localDensity equals density for fluid
       q[6] = rh * inv_nf;
+      // NeighborsToPressurePointTwo (derivatives.c:2340-2367): the positive
+      // pressures strictly within 2h of the second probe, averaged by
+      // EnsembleAveragedPointPressureTwo (derivatives.c:2382-2393)
+      const double ex = x.x - px2, ey = x.y - py2;
+      if (sqrt(ex * ex + ey * ey) < 2.0 * hh && P > 0.0) {
+        q[7] = P;
+        q[8] = 1.0;
+      }
     }
   }
 #pragma unroll
@@ -2955,8 +2967,9 @@ __global__ __launch_bounds__(DIAG_BLK) void k_diag(int poff, int np, const uint3
   if (threadIdx.x < DIAG_Q) part[(size_t)blockIdx.x * DIAG_Q + threadIdx.x] = red[threadIdx.x][0];
 }
 
-extern "C" int uph_diagnostics(uph_t* h, int scene, double probe_x, double probe_y, uph_diag* out) {
-  if (!h || !out) return -1;
+// Sums of k_diag over one scene's fluid rows (t: DIAG_Q values).
+static int diag_sums(uph_t* h, int scene, double px, double py, double px2, double py2, double* t) {
+  if (!h) return -1;
   if (scene < 0 || scene >= h->nscenes) {
     h->err.msg = "uph_diagnostics: scene out of range";
     return -1;
@@ -2971,16 +2984,32 @@ extern "C" int uph_diagnostics(uph_t* h, int scene, double probe_x, double probe
   }
   hipLaunchKernelGGL(k_diag, dim3(nb), dim3(DIAG_BLK), 0, h->stream, S.poff, S.np, h->b.tagA, h->b.posB,
                      h->b.velB, h->b.rhoB, h->b.massD, h->mode == UPH_MODE_REF ? h->b.presB : nullptr, h->dsc,
-                     scene, probe_x, probe_y,
+                     scene, px, py, px2, py2,
                      S.nf > 0 ? 1.0 / S.nf : 0.0, h->diag);
   HIPCHK(hipGetLastError(), h->err);
   std::vector<double> part((size_t)nb * DIAG_Q);
   HIPCHK(hipMemcpyAsync(part.data(), h->diag, part.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream),
          h->err);
   HIPCHK(hipStreamSynchronize(h->stream), h->err);
-  double t[DIAG_Q] = {0, 0, 0, 0, 0, 0, 0};
+  for (int j = 0; j < DIAG_Q; ++j) t[j] = 0.0;
   for (int b = 0; b < nb; ++b)
     for (int j = 0; j < DIAG_Q; ++j) t[j] += part[(size_t)b * DIAG_Q + j];
+  return 0;
+}
+
+extern "C" int uph_probe_two(uph_t* h, int scene, double x, double y, double* pressure, int* neighbors) {
+  double t[DIAG_Q];
+  if (diag_sums(h, scene, x, y, x, y, t)) return -1;
+  const int n = (int)t[8];
+  if (neighbors) *neighbors = n;
+  if (pressure) *pressure = n > 0 ? t[7] / n : 0.0;
+  return 0;
+}
+
+extern "C" int uph_diagnostics(uph_t* h, int scene, double probe_x, double probe_y, uph_diag* out) {
+  if (!out) return -1;
+  double t[DIAG_Q];
+  if (diag_sums(h, scene, probe_x, probe_y, probe_x, probe_y, t)) return -1;
   memset(out, 0, sizeof *out);
   out->kinetic_energy = t[0];
   out->potential_energy = t[1];
@@ -2994,6 +3023,86 @@ extern "C" int uph_diagnostics(uph_t* h, int scene, double probe_x, double probe
   out->consistency_three = t[6];
   out->density_uncertainty = out->consistency_one - t[5] * t[5] / t[6];
   return 0;
+}
+
+// ApproximateVortcity (derivatives.c:2840-2885) of the last step's derivatives()
+// evaluation, in gather form: for fluid p, the sum over its pair partners q of
+// (m_q / rho_q) (dv_y dx_x - dv_x dx_y) W / (r^2 + 0.01 h^2) with dx = x_p - x_q,
+// dv = v_p - v_q, on the state that evaluation saw (positions x_n, predicted
+// velocities and densities: the pair kernel's inputs, still in the A buffers).  The
+// pair term is symmetric in p and q (both differences change sign), so the gather
+// equals the reference's scatter to both particles of each pair.  The pair set is
+// the step's, K cap included.
+template <int KT>
+__global__ __launch_bounds__(BLK) void k_vort(int poff, int np, StepBufs b, const DevScene* __restrict__ sc,
+                                              int scene, double* __restrict__ out) {
+  const int k = blockIdx.x * BLK + threadIdx.x;
+  if (k >= np) return;
+  const int p = poff + k;
+  const uint32_t tg = b.tagA[p];
+  if (tag_type(tg) != 1) return;
+  const DevScene& S = sc[scene];
+  PairBase c;
+  c.init_base(S, tg);
+  double kpd2;
+  uint32_t kpid;
+  kth_of(b, p, kpd2, kpid);
+  const double2 xp = b.posA[p];
+  const float4 ap = b.mA[p];
+  const double rho_off = S.mode == UPH_MODE_REF ? S.rho0 : 0.0;   // ref mode's mirror holds rho - rho0
+  const double eta2 = 0.01 * S.h * S.h;
+  double acc = 0.0;
+  for_candidates(S, xp, b.cellStart, [&](int q) {
+    if (q == p) return;
+    const double2 xq = b.posA[q];
+    const double ex = xp.x - xq.x, ey = xp.y - xq.y;
+    const double d2 = __dadd_rn(__dmul_rn(ex, ex), __dmul_rn(ey, ey));
+    if (!(d2 < S.cutoff)) return;
+    const float4 bq = b.mB[q];
+    const uint32_t tq = __float_as_uint(bq.w);
+    if (!c.template admit<true>(S, tag_id(tq), d2, kpd2, kpid, b, q)) return;
+    const float4 aq = b.mA[q];
+    const float r2 = (float)d2;
+    const double w = (double)c.template w_value<KT>(r2, sqrtf(r2));
+    const double vol = (double)bq.z / ((double)bq.x + rho_off);
+    const double dvx = (double)ap.x - (double)aq.x, dvy = (double)ap.y - (double)aq.y;
+    acc += vol * (dvy * ex - dvx * ey) * w / (d2 + eta2);
+  });
+  out[tag_id(tg) - poff] = acc;
+}
+
+// approxVorticity of the last evaluation on the handle's stream, copied to host `out`.
+static int vort_run(uph_t* h, int scene, double* out) {
+  const DevScene& S = h->hs[scene];
+  if (S.nf > h->vort_cap) {
+    if (h->vort) HIPCHK(hipFree(h->vort), h->err);
+    h->vort = nullptr;
+    HIPCHK(hipMalloc((void**)&h->vort, (size_t)S.nf * sizeof(double)), h->err);
+    h->vort_cap = S.nf;
+  }
+  if (h->kernel == UPH_KERNEL_POLY7)
+    hipLaunchKernelGGL(k_vort<UPH_KERNEL_POLY7>, dim3(grid_of(S.np)), dim3(BLK), 0, h->stream, S.poff, S.np, h->b,
+                       h->dsc, scene, h->vort);
+  else
+    hipLaunchKernelGGL(k_vort<UPH_KERNEL_CUBIC>, dim3(grid_of(S.np)), dim3(BLK), 0, h->stream, S.poff, S.np, h->b,
+                       h->dsc, scene, h->vort);
+  HIPCHK(hipGetLastError(), h->err);
+  HIPCHK(hipMemcpyAsync(out, h->vort, (size_t)S.nf * sizeof(double), hipMemcpyDeviceToHost, h->stream), h->err);
+  HIPCHK(hipStreamSynchronize(h->stream), h->err);
+  return 0;
+}
+
+extern "C" int uph_vorticity(uph_t* h, int scene, double* out) {
+  if (!h || !out) return -1;
+  if (scene < 0 || scene >= h->nscenes) {
+    h->err.msg = "uph_vorticity: scene out of range";
+    return -1;
+  }
+  if (h->itime == 0) {
+    h->err.msg = "uph_vorticity: no step has run";
+    return -1;
+  }
+  return vort_run(h, scene, out);
 }
 
 // ------------------------------------------------------ SearchNeighbors
@@ -4028,10 +4137,17 @@ extern "C" int uph_derivatives(uph_system* s, double t) {
   s->consistencyDensityTwo = dg.consistency_two;
   s->consistencyDensityThree = dg.consistency_three;
   s->densityUncertainty = dg.density_uncertainty;
+  // the second probe at pressureProbeXX/YY (derivatives.c:4322-4323)
+  if (uph_probe_two(h, 0, s->pressureProbeXX, s->pressureProbeYY, &s->pointPressureTwo,
+                    &s->pointNeighborsTwo) != 0)
+    return compat_fail("derivatives: " + err.msg);
   if (launch_step(h, nullptr, nullptr)) {
     if (c.err.empty()) c.err = "derivatives: " + err.msg;
     return -1;
   }
+  // approxVorticity of this evaluation (derivatives.c:4251), where the System has it
+  if (s->approxVorticity && vort_run(h, 0, s->approxVorticity) != 0)
+    return compat_fail("derivatives: " + err.msg);
   if (get_rates_f64(h, 0, c.R.data(), c.Rt.data(), c.a.data(), c.at.data()) != 0)
     return compat_fail("derivatives: " + err.msg);
   if (ref && uph_get_ref_rates(h, 0, c.RP.data(), c.RPt.data(), c.fr.data(), c.fP.data(), c.fv.data()) != 0)

@@ -8,7 +8,7 @@
  * (12-column rows, input.c:57-72 / inputbp.c:88-150).
  *
  *   uph_run <nsteps> [--fluid FILE] [--h-factor F] [--alpha A] [--kernel cubic|poly7]
- *           [--scheme config|ref] [--graph] [--compat] [--out FILE] [--no-output]
+ *           [--scheme config|ref] [--graph] [--compat] [--out FILE] [--no-output] [--probe2 X Y]
  *
  * --compat runs the time loop on the host in the shape of sph.cc:88-168 over a
  * System object (include/uph_system.h) and calls the library's derivatives()
@@ -26,6 +26,8 @@
  * diffusion thermalDiffusivity0; with --kernel poly7 this is the reference's
  * configuration as its decks set it), device loop only.  After the run the
  * fluid state is written in the deck format (default FluidParticleFinal.dat).
+ * --probe2 places the second pressure probe (PointPressureTwo.dat and the fifth
+ * column of PointPressure.dat; default: the first probe's location).
  */
 #include <math.h>
 #include <stdio.h>
@@ -43,6 +45,7 @@
 typedef struct {
   int n;
   double *pos, *vel, *rho, *pres, *mass, *lvel, *gate;   /* gate: column 9, tempPositionGate */
+  double *tpos;                                         /* column 8, tempPosition */
   int *itype;
 } deck_t;
 
@@ -66,12 +69,12 @@ static int read_deck(const char *path, deck_t *d) {
   d->pres = calloc(n, sizeof(double));
   d->mass = calloc(n, sizeof(double));
   d->gate = calloc(n, sizeof(double));
+  d->tpos = calloc(n, sizeof(double));
   d->itype = calloc(n, sizeof(int));
   for (int i = 0; i < d->n; i++) {
-    double t0;
     int got = fscanf(f, "%lf %lf %lf %lf %lf %lf %lf %lf %lf %d %lf %lf", &d->pos[2 * i],
                      &d->pos[2 * i + 1], &d->vel[2 * i], &d->vel[2 * i + 1], &d->rho[i], &d->pres[i],
-                     &d->mass[i], &t0, &d->gate[i], &d->itype[i], &d->lvel[2 * i], &d->lvel[2 * i + 1]);
+                     &d->mass[i], &d->tpos[i], &d->gate[i], &d->itype[i], &d->lvel[2 * i], &d->lvel[2 * i + 1]);
     if (got != 12) {
       fprintf(stderr, "uph_run: %s row %d: expected 12 columns\n", path, i + 1);
       fclose(f);
@@ -84,6 +87,7 @@ static int read_deck(const char *path, deck_t *d) {
 
 static void free_deck(deck_t *d) {
   free(d->pos); free(d->vel); free(d->lvel); free(d->rho); free(d->pres); free(d->mass); free(d->gate);
+  free(d->tpos);
   free(d->itype);
 }
 
@@ -95,6 +99,11 @@ typedef struct snap {
   double h, t;
   double ke, pe, probe_p, probe_norm;
   int probe_n;
+  const int *gate_flag;             /* damGateFlag of every row (input.c:148-159) */
+  const double *vort;               /* approxVorticity of the fluid rows (nf), or NULL */
+  double probe2_p;                  /* second probe (derivatives.c:2340-2393) */
+  int probe2_n;
+  const double *tpos, *tgate;       /* deck columns 8 and 9 of the fluid rows (FluidParticleInit.dat) */
 } snap_t;
 static int write_output(const snap_t *o, int itime);
 static int write_init(const snap_t *o, const double *lvel);
@@ -118,6 +127,8 @@ static double *vec(int n) { return calloc(n > 0 ? (size_t)n : 1, sizeof(double))
 /* The host time loop of sph.cc:88-168 with the fluctuation fields zero (the
  * config-mode scheme) and derivatives() from the library.  On return s holds
  * the state after nsteps. */
+static const int *gate_flag_of;   /* damGateFlag of the run's rows */
+
 static int run_compat(uph_system *s, int nsteps, int print_step, int init_steps) {
   const int nf = s->NumberOfFluidParticles, np = s->MaxNumberOfParticles;
   const double dt = s->timeStep;
@@ -192,7 +203,9 @@ static int run_compat(uph_system *s, int nsteps, int print_step, int init_steps)
       const int rows = nf + s->NumberOfBoundaryParticles + s->NumberOfVirtualParticles;
       snap_t o = {rows, nf, s->Position[0], s->Velocity[0], s->density, s->mass, s->DynamicPressure,
                   s->localDensity, s->itype, s->hsml[0], itime * dt, s->totalKineticEnergy,
-                  s->potentialEnergy, s->pointPressure, s->pressureNorm, s->pointNeighbors};
+                  s->potentialEnergy, s->pointPressure, s->pressureNorm, s->pointNeighbors,
+                  gate_flag_of, s->approxVorticity, s->pointPressureTwo, s->pointNeighborsTwo,
+                  s->tempPosition, s->tempPositionGate};
       if (write_output(&o, itime) != 0) return -1;
       if (itime == init_steps && write_init(&o, s->localVelocity[0]) != 0) return -1;
     }
@@ -207,33 +220,40 @@ static double now_s(void) {
 }
 
 /* Output of sph.cc's printStep (output.c:19-130): the snapshot out<itime>.dat of
- * every particle, and one line appended to TotalKineticEnergy.dat and
- * PointPressure.dat.  Columns the config-mode scheme does not compute
- * (damGateFlag, approxVorticity, specificDissipationRate, the second probe) are
- * written as 0; the series of the reference's other diagnostics files are not
- * written. */
+ * every particle, and one line appended to TotalKineticEnergy.dat, PointPressure.dat
+ * and PointPressureTwo.dat.  specificDissipationRate (column 13) is written as 0:
+ * the reference accumulates it over every step (derivatives.c:2573-2628 adds into it
+ * and nothing resets it after input.c:128), from viscous pair sums that feed nothing
+ * else, so it would cost a pair pass per step; for the same reason the dissipation
+ * and control-volume series (PowerCalc.dat, ControlVolPower.dat,
+ * ControlVolKinetic.dat, output.c:93-112) are not written. */
 
 static int write_output(const snap_t *o, int itime) {
   char name[64];
   snprintf(name, sizeof name, "out%d.dat", itime);
   FILE *f = fopen(name, "w");
   FILE *fke = fopen("TotalKineticEnergy.dat", "a"), *fpp = fopen("PointPressure.dat", "a");
-  if (!f || !fke || !fpp) {
+  FILE *fp2 = fopen("PointPressureTwo.dat", "a");
+  if (!f || !fke || !fpp || !fp2) {
     fprintf(stderr, "uph_run: cannot write the output of step %d\n", itime);
     if (f) fclose(f);
     if (fke) fclose(fke);
     if (fpp) fclose(fpp);
+    if (fp2) fclose(fp2);
     return -1;
   }
   for (int i = 0; i < o->n; i++)
     fprintf(f, "%d \t%.10lf\t%.10lf\t%.10lf\t%.10lf\t%.10lf\t%.10lf\t%.10lf\t%.10lf\t%d\t%.10lf\t%d\t%.10lf\t%.10lf\n",
             i, o->pos[2 * i], o->pos[2 * i + 1], o->vel[2 * i], o->vel[2 * i + 1], o->rho[i], o->mass[i], o->h,
-            o->pres[i], 0, 0.0, o->itype[i], 0.0, o->rholoc[i]);
+            o->pres[i], o->gate_flag ? o->gate_flag[i] : 0, (o->vort && i < o->nf) ? o->vort[i] : 0.0,
+            o->itype[i], 0.0, o->rholoc[i]);
   fprintf(fke, "%.10lf\t%.10lf %.10lf\n", o->t, o->ke, o->pe);
-  fprintf(fpp, "%.10lf\t%.10lf\t%d\t%.10lf\t%.10lf\n", o->t, o->probe_p, o->probe_n, o->probe_norm, 0.0);
+  fprintf(fpp, "%.10lf\t%.10lf\t%d\t%.10lf\t%.10lf\n", o->t, o->probe_p, o->probe_n, o->probe_norm, o->probe2_p);
+  fprintf(fp2, "%.10lf\t%d\t%.10lf\n", o->t, o->probe2_n, o->probe2_p);
   fclose(f);
   fclose(fke);
   fclose(fpp);
+  fclose(fp2);
   return 0;
 }
 
@@ -246,15 +266,16 @@ static int write_init(const snap_t *o, const double *lvel) {
   for (int i = 0; i < o->nf; i++)
     fprintf(f, "%.10lf\t%.10lf\t%.10lf\t%.10lf\t%.10lf\t%.10lf\t%.10lf\t%.10lf\t%.10lf\t%d\t%.10lf\t%.10lf\n",
             o->pos[2 * i], o->pos[2 * i + 1], o->vel[2 * i], o->vel[2 * i + 1], o->rho[i], o->pres[i], o->mass[i],
-            0.0, 0.0, o->itype[i], lvel[2 * i], lvel[2 * i + 1]);
+            o->tpos ? o->tpos[i] : 0.0, o->tgate ? o->tgate[i] : 0.0, o->itype[i], lvel[2 * i], lvel[2 * i + 1]);
   fclose(f);
   return 0;
 }
 
 /* sph.cc:54-80 empties the series files at start */
 static void truncate_series(void) {
-  const char *names[] = {"TotalKineticEnergy.dat", "PointPressure.dat", "FluidParticleInit.dat"};
-  for (int k = 0; k < 3; k++) {
+  const char *names[] = {"TotalKineticEnergy.dat", "PointPressure.dat", "FluidParticleInit.dat",
+                         "PointPressureTwo.dat"};
+  for (int k = 0; k < 4; k++) {
     FILE *f = fopen(names[k], "w");
     if (f) fclose(f);
   }
@@ -269,6 +290,8 @@ int main(int argc, char **argv) {
   const char *fluid_file = "FluidParticle.dat", *out_file = "FluidParticleFinal.dat";
   double h_factor = 0.0, alpha = -1.0;
   int use_graph = 0, compat = 0, write = 1, kernel = UPH_KERNEL_CUBIC, mode = UPH_MODE_CONFIG;
+  int have_probe2 = 0;
+  double probe2[2] = {0.0, 0.0};
   for (int a = 2; a < argc; a++) {
     if (!strcmp(argv[a], "--fluid") && a + 1 < argc) fluid_file = argv[++a];
     else if (!strcmp(argv[a], "--h-factor") && a + 1 < argc) h_factor = atof(argv[++a]);
@@ -291,6 +314,11 @@ int main(int argc, char **argv) {
         fprintf(stderr, "uph_run: unknown scheme %s (config, ref)\n", k);
         return 1;
       }
+    }
+    else if (!strcmp(argv[a], "--probe2") && a + 2 < argc) {
+      probe2[0] = atof(argv[++a]);
+      probe2[1] = atof(argv[++a]);
+      have_probe2 = 1;
     }
     else if (!strcmp(argv[a], "--graph")) use_graph = 1;
     else if (!strcmp(argv[a], "--compat")) compat = 1;
@@ -341,6 +369,8 @@ int main(int argc, char **argv) {
   all.lvel = malloc(sizeof(double) * 2 * n); all.rho = malloc(sizeof(double) * n);
   all.mass = malloc(sizeof(double) * n); all.itype = malloc(sizeof(int) * n);
   all.pres = malloc(sizeof(double) * n); all.gate = malloc(sizeof(double) * n);
+  all.tpos = malloc(sizeof(double) * n);
+  int *gflag = calloc(n > 0 ? (size_t)n : 1, sizeof(int));
   deck_t *parts[3] = {&fl, &fx, &mv};
   int o = 0;
   double lo[2] = {1e300, 1e300}, hi[2] = {-1e300, -1e300};
@@ -356,6 +386,10 @@ int main(int argc, char **argv) {
       all.rho[o] = parts[p]->rho[i];
       all.pres[o] = parts[p]->pres[i];
       all.gate[o] = parts[p]->gate[i];
+      all.tpos[o] = parts[p]->tpos[i];
+      /* input.c:148-159: fluid rows of the dam column (x <= 0.38 at input) 1, the rest -1;
+       * boundary rows 0 (inputbp.c:311) */
+      if (p == 0) gflag[o] = parts[p]->pos[2 * i] <= 0.38 ? 1 : -1;
       all.mass[o] = parts[p]->mass[i];
       all.itype[o] = parts[p]->itype[i];
     }
@@ -381,6 +415,13 @@ int main(int argc, char **argv) {
   s.mode = mode;
   s.pres = all.pres;
   s.kappa = v[27]; s.mu = v[10]; s.eps = v[13];   /* thermalDiffusivity0, viscosity0, epsilon */
+  /* the second probe: pressureProbeXX/YY, which the reference never assigns
+   * (System.c:130-131 commented out): --probe2, else the first probe's location */
+  if (!have_probe2) {
+    probe2[0] = v[30];
+    probe2[1] = v[31];
+  }
+  gate_flag_of = gflag;
   const double margin = 10 * s.h + (moving == 1 ? stroke : 0.0);
   s.box[0] = lo[0] - margin; s.box[1] = lo[1] - margin;
   s.box[2] = hi[0] + margin; s.box[3] = hi[1] + 2 * (hi[1] - lo[1]);
@@ -437,6 +478,9 @@ int main(int argc, char **argv) {
     sy.virtualType = (int)v[15]; sy.lidVelocity = v[19]; sy.LocalDepth = v[2]; sy.FlatBottomLength = v[3];
     sy.tempPositionGate = all.gate;
     sy.pressureProbeX = v[30]; sy.pressureProbeY = v[31];
+    sy.pressureProbeXX = probe2[0]; sy.pressureProbeYY = probe2[1];
+    sy.approxVorticity = vec(nm);
+    sy.tempPosition = all.tpos;
     sy.viscosity0 = v[10]; sy.epsilon = v[13]; sy.thermalDiffusivity0 = v[27];
     if (print_step > 0) truncate_series();
     double tc = now_s();
@@ -464,8 +508,8 @@ int main(int argc, char **argv) {
     free(sy.mass); free(sy.density); free(sy.density_min); free(sy.localDensity); free(sy.Pressure);
     free(sy.DynamicPressure); free(sy.DynamicPressure_min); free(sy.localPressure); free(sy.dpdt);
     free(sy.dpdt_approx); free(sy.hsml); free(sy.ddensitydt); free(sy.ddensitydt_approx);
-    free(sy.densityFluctuation); free(sy.pressureFluctuation); free(sy.itype);
-    free(all.pos); free(all.vel); free(all.lvel); free(all.rho); free(all.mass); free(all.itype); free(all.pres); free(all.gate);
+    free(sy.densityFluctuation); free(sy.pressureFluctuation); free(sy.itype); free(sy.approxVorticity);
+    free(all.pos); free(all.vel); free(all.lvel); free(all.rho); free(all.mass); free(all.itype); free(all.pres); free(all.gate); free(all.tpos); free(gflag);
     free_deck(&fl); free_deck(&fx); free_deck(&mv); free_deck(&so);
     return 0;
   }
@@ -493,7 +537,9 @@ int main(int argc, char **argv) {
     truncate_series();
     double *pres = malloc(sizeof(double) * nx), *rhol = malloc(sizeof(double) * nx);
     double *massx = malloc(sizeof(double) * nx);
-    int *itypex = malloc(sizeof(int) * nx);
+    double *vort = malloc(sizeof(double) * (fl.n > 0 ? (size_t)fl.n : 1));
+    int *itypex = malloc(sizeof(int) * nx), *gflagx = calloc(nx, sizeof(int));
+    memcpy(gflagx, gflag, sizeof(int) * n);
     memcpy(massx, all.mass, sizeof(double) * n);
     memcpy(itypex, all.itype, sizeof(int) * n);
     for (int i = n; i < nx; i++) itypex[i] = 4;   /* mirror rows (derivatives.c:3220) */
@@ -505,7 +551,10 @@ int main(int argc, char **argv) {
     while (done < nsteps) {
       const int chunk = print_step < nsteps - done ? print_step : nsteps - done;
       uph_diag dg;
-      if (uph_get_state(h, 0, pos, vel, rho) != 0 || uph_diagnostics(h, 0, v[30], v[31], &dg) != 0) {
+      double p2 = 0.0;
+      int n2 = 0;
+      if (uph_get_state(h, 0, pos, vel, rho) != 0 || uph_diagnostics(h, 0, v[30], v[31], &dg) != 0 ||
+          uph_probe_two(h, 0, probe2[0], probe2[1], &p2, &n2) != 0 || uph_vorticity(h, 0, vort) != 0) {
         fprintf(stderr, "uph_run: %s\n", uph_last_error(h));
         return 1;
       }
@@ -537,13 +586,16 @@ int main(int argc, char **argv) {
       }
       if (done % print_step == 0) {
         snap_t o = {n + ng, fl.n, pos, vel, rho, massx, pres, rhol, itypex, s.h, done * dt, dg.kinetic_energy,
-                    dg.potential_energy, dg.probe_pressure, dg.probe_norm, dg.probe_neighbors};
+                    dg.potential_energy, dg.probe_pressure, dg.probe_norm, dg.probe_neighbors, gflagx, vort, p2,
+                    n2, all.tpos, all.gate};
         if (write_output(&o, done) != 0) return 1;
         if (done == init_steps && write_init(&o, vel) != 0) return 1;
       }
     }
     free(massx);
     free(itypex);
+    free(gflagx);
+    free(vort);
     if (uph_sync(h) < 0) {
       fprintf(stderr, "uph_run: %s\n", uph_last_error(h));
       return 1;
@@ -585,7 +637,7 @@ int main(int argc, char **argv) {
   fclose(fo);
   uph_destroy(h);
   free(pos); free(vel); free(rho); free(pfin);
-  free(all.pos); free(all.vel); free(all.lvel); free(all.rho); free(all.mass); free(all.itype); free(all.pres); free(all.gate);
+  free(all.pos); free(all.vel); free(all.lvel); free(all.rho); free(all.mass); free(all.itype); free(all.pres); free(all.gate); free(all.tpos); free(gflag);
   free_deck(&fl); free_deck(&fx); free_deck(&mv); free_deck(&so);
   return 0;
 }

@@ -138,6 +138,9 @@ SIGNATURES = {
     "uph_graph_last_error": (C.c_char_p, []),
     "uph_derivatives": (C.c_int, [C.c_void_p, C.c_double]),
     "uph_diagnostics": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_void_p]),
+    "uph_probe_two": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, _dp,
+                                C.POINTER(C.c_int)]),
+    "uph_vorticity": (C.c_int, [C.c_void_p, C.c_int, _dp]),
     "derivatives": (None, [C.c_void_p, C.c_double]),
     "uph_derivatives_last_error": (C.c_char_p, []),
 }
@@ -329,6 +332,20 @@ class GpuSolver:
         self._check(lib().uph_diagnostics(self.h, scene, float(probe[0]), float(probe[1]),
                                           C.byref(d)))
         return {k: getattr(d, k) for k, _ in UphDiag._fields_}
+
+    def probe_two(self, x: float, y: float, scene: int = 0):
+        """(mean positive pressure, count) of the fluid rows within 2h of (x, y)
+        (derivatives.c:2340-2393)."""
+        self._scene(scene)
+        p, n = C.c_double(), C.c_int()
+        self._check(lib().uph_probe_two(self.h, scene, float(x), float(y), C.byref(p), C.byref(n)))
+        return p.value, n.value
+
+    def vorticity(self, scene: int = 0) -> np.ndarray:
+        """approxVorticity of the fluid rows as the last step evaluated it (derivatives.c:2840-2885)."""
+        out = np.zeros(self._scene(scene).n_fluid)
+        self._check(lib().uph_vorticity(self.h, scene, _ptr(out)))
+        return out
 
     def state_to_device(self, pos: int, vel: int, rho: int, scene: int = 0):
         """Copies the scene's state, in reference order, into device buffers (pointers)."""
