@@ -200,3 +200,4 @@ def test_ref_hydrostatic_deck_against_the_reference_recorded_run():
     assert np.all(np.abs(rk - 1.0) < 2e-3)
     assert drho.max() < 1e-6
     assert dpos.max() < 1e-3
+
