@@ -344,6 +344,25 @@ int uph_lincomb_f64(long n, double a, const double *x, double b, const double *y
 int uph_scale_ratio_f64(long n, const double *x, const double *num, const double *den, double *out,
                         void *stream);
 
+/* The rollout network's edge kernels (no edge latent; width W <= 128).  Receivers are the
+ * rows i < n_recv of the receiver-ordered CSR (off, send); r_e = ((x_i - x_j)/R, |.|/R, 1)
+ * from the FP64 positions (pos: 2n doubles); A: FP32 rows 0..3 (row stride ldA);
+ * zb, zc: FP32 node rows.
+ * uph_sn_edge_agg: s_i = sum_e relu(r_e A + zb_i + zc_j) in columns 0..W-1 of s (row
+ *   stride lds >= W + 1), the edge count in column W, zeros up to lds.
+ * uph_sn_edge_bwd: with ds (the gradient of s's first W columns): dzb_i and dzc_j, the
+ *   sums of g_e = ds_i [pre_e > 0] over i's and j's edges (W columns, row stride lddz),
+ *   and dA (rows 0..3) += sum_e r_e^T g_e; part: 1024 W floats of scratch.  Every sum
+ *   runs in a fixed order (dzc through the sender-ordered permutation soff, sperm). */
+int uph_sn_edge_agg(int n, int n_recv, int W, const int *off, const int *send, const double *pos, double invR,
+                    const float *A, int ldA, const float *zb, int ldzb, const float *zc, int ldzc, float *s,
+                    int lds, void *stream);
+int uph_sn_edge_bwd(int n, int n_recv, int W, const int *off, const int *send, const int *soff, const int *sperm,
+                    const int *recv, const double *pos, double invR, const float *A, int ldA, const float *zb,
+                    int ldzb, const float *zc, int ldzc, const float *ds, int ldds, float *dzb, float *dzc,
+                    int lddz, float *dA, float *part, void *stream);
+const char *uph_sn_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -6,7 +6,8 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRCS = [os.path.join(HERE, "csrc", "uph_gpu.hip"), os.path.join(HERE, "csrc", "uph_gnn.hip")]
+SRCS = [os.path.join(HERE, "csrc", "uph_gpu.hip"), os.path.join(HERE, "csrc", "uph_gnn.hip"),
+        os.path.join(HERE, "csrc", "uph_surrogate.hip")]
 OUT = os.path.join(HERE, "libuph_gpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

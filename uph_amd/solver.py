@@ -129,6 +129,14 @@ SIGNATURES = {
                                     C.c_double, C.c_void_p]),
     "uph_sph_density": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
                                   C.c_void_p, C.c_void_p]),
+    "uph_sn_edge_agg": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                  C.c_double, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p,
+                                  C.c_int, C.c_void_p, C.c_int, C.c_void_p]),
+    "uph_sn_edge_bwd": (C.c_int, [C.c_int, C.c_int, C.c_int] + [C.c_void_p] * 6 +
+                        [C.c_double, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                         C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                         C.c_void_p, C.c_void_p]),
+    "uph_sn_last_error": (C.c_char_p, []),
     "uph_state_to_device": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     "uph_state_from_device": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                         C.c_int]),
