@@ -265,8 +265,9 @@ int uph_state_from_device(uph_t *h, int scene, const double *pos, const double *
 /* Radius graph of n device positions (2n doubles): receivers are the first n_recv
  * particles; an edge (i <- j) exists for every j != i with squared distance < dist.
  * off (n+1): CSR by receiver (0 edges for non-receivers); send/recv (E): edge ends in
- * candidate order; soff (n+1), sperm (E): for node j, the ids of its edges as a sender.
- * Returns -2 (and *n_edges) when E exceeds capacity. */
+ * candidate order; soff (n+1), sperm (E): for node j, the ids of its edges as a sender
+ * (both NULL: no sender view is built).  Returns -2 (and *n_edges) when E exceeds
+ * capacity. */
 int uph_graph_build(const double *pos, int n, int n_recv, double dist, int *off, int *send, int *recv,
                     int *soff, int *sperm, long capacity, long *n_edges);
 const char *uph_graph_last_error(void);
@@ -361,6 +362,19 @@ int uph_sn_edge_bwd(int n, int n_recv, int W, const int *off, const int *send, c
                     const int *recv, const double *pos, double invR, const float *A, int ldA, const float *zb,
                     int ldzb, const float *zc, int ldzc, const float *ds, int ldds, float *dzb, float *dzc,
                     int lddz, float *dA, float *part, void *stream);
+/* The rollout network's inference at width 128, fused: mode 0 the node encoder (xn: n x 16
+ * BF16; L1t, L2t the transposed encoder weights) writing h and step 0's projections; mode 1
+ * one message-passing step (the edge sums from A, zb, zc; the node MLP D, E, F, G; h
+ * updated in place) writing the next step's projections zb_out, zc_out (Bt, Ct); mode 2
+ * the last step with the decoder (P1t, P2t) writing out (n x 8 FP32, columns 0..1 used).
+ * Transposed weights are BF16 with row stride 136 (the bias at column 128; Ft: 128 x 128,
+ * P2t: 8 rows); zb, zb_out, h: n x 128 FP32; zc, zc_out: n x 128 BF16.  Buffers a mode
+ * does not read or write may be NULL. */
+int uph_sn_fused(int mode, int n, int n_recv, const int *off, const int *send, const double *pos, double invR,
+                 const float *A, const float *zb, const void *zc, const void *Dt, const void *Et, const void *Ft,
+                 const void *Gt, float *h, const void *Bt, const void *Ct, float *zb_out, void *zc_out,
+                 const void *P1t, const void *P2t, float *out, const void *xn, const void *L1t, const void *L2t,
+                 void *stream);
 const char *uph_sn_last_error(void);
 
 #ifdef __cplusplus

@@ -67,8 +67,8 @@ def _small_training(iters):
 
 def test_fallback_on_density_drift_is_the_plain_solver():
     """A network with its decoder scaled by 1e3 scrambles the particles: every block
-    exceeds the 1 % drift threshold, falls back, and the result is bit-identical to
-    an uninterrupted plain run."""
+    exceeds the 1 % drift threshold or flings particles out of the tank, falls back,
+    and the result is bit-identical to an uninterrupted plain run."""
     from uph_amd import surrogate as S
     from uph_amd.gnn import GraphNet
     from uph_amd.solver import GpuSolver
@@ -82,7 +82,7 @@ def test_fallback_on_density_drift_is_the_plain_solver():
     ro.run(40)
     _say(f"sabotaged: {ro.blocks} blocks, {ro.fallbacks} fallbacks, drifts {np.round(ro.drifts, 3)}")
     assert ro.blocks >= 3 and ro.fallbacks == ro.blocks
-    assert min(ro.drifts) > cfg.drift_tol
+    assert ro.escapes + sum(d > cfg.drift_tol for d in ro.drifts) == ro.blocks
     ref = GpuSolver(sc)
     ref.step(ro.macro * cfg.substeps)
     a, b = ro.state(), ref.state()

@@ -98,3 +98,61 @@ def test_forward_backward_vs_oracle(width, steps):
     w1 = net.params()
     for k in w0:
         assert np.abs(w1[k] - w0[k]).max() <= 1e-3 * 1.001, k
+
+
+def test_fused_inference_equals_the_layered_forward():
+    """uph_sn_fused (the rollout's path: 1 + S launches at width 128) against the layered
+    forward on the same graph and weights, and both against the FP64 oracle on the
+    BF16-rounded weights, at W 128 and S 8.  The fused path also rounds the gathered zc
+    to BF16 (the layered one keeps it in FP32), one more 2^-9 rounding per step, so both
+    bounds are the layered test's 3e-2 (measured: 1.3e-2 between the two)."""
+    from uph_amd.gnn import Graph
+    from uph_amd.surrogate_net import SurrogateNet
+    pos, nf = _cloud(nf=900, nw=200, seed=8)
+    n, R = len(pos), 2.6
+    dp = DevBuf.of(pos)
+    g = Graph(n, nf, 1 << 16).build(dp, R * R, sender=False)
+    rng = np.random.default_rng(2)
+    xn = DevBuf(n * 16 * 2)
+    check(lib().uph_gns_node_feats(n, nf, DevBuf.of(rng.normal(0, 0.3, (5, n, 2))).ptr, 2.0, xn.ptr, None))
+    params = gnn_ref.sn_init_params(128, 8, seed=5)
+    for k, w in params.items():
+        if w.shape[0] == 129:
+            w[-1] = rng.normal(0, 0.1, w.shape[1])
+    for k, w in params.items():
+        if k.endswith("_A"):
+            w[3] = rng.normal(0, 0.1, w.shape[1])
+    net = SurrogateNet(128, 8, params=params)
+    net.forward(g, dp, 1.0 / R, xn, train=False)
+    a = net.output(n)
+    net.forward_fused(g, dp, 1.0 / R, xn)
+    b = net.fused_output(n)
+    xn_h = from_bf16(xn.download((n, 16), np.uint16)).astype(np.float64)
+    send = g.send.download(g.E, np.int32).astype(np.int64)
+    recv = g.recv.download(g.E, np.int32).astype(np.int64)
+    pb = {k: (v.astype(np.float32).astype(np.float64) if k.endswith("_A")
+              else from_bf16(to_bf16(v.astype(np.float32))).astype(np.float64)) for k, v in params.items()}
+    ref = gnn_ref.sn_forward(pb, xn_h, gnn_ref.sn_edge_features(pos, send, recv, 1.0 / R), send, recv, 128, 8)
+    e, ea, eb = _rel(b[:nf], a[:nf]), _rel(a[:nf], ref[:nf]), _rel(b[:nf], ref[:nf])
+    _say(f"W128 S8, n={n} E={g.E}: fused vs layered rel L2 {e:.2e}; vs oracle: layered {ea:.2e}, fused {eb:.2e}")
+    assert np.all(np.isfinite(b)) and e < 3e-2 and ea < 3e-2 and eb < 3e-2
+
+
+def test_w128_s8_on_a_window_of_the_64k_dam_break():
+    """The rollout workload's network size (W 128, S 8) on a subgraph of the 64,000-particle
+    dam break after 100 plain steps: the fluid and wall particles of a 0.25 x 0.12 m window
+    at the bottom-left corner, graph radius 2h."""
+    from uph_amd import scenes
+    from uph_amd.solver import GpuSolver
+    sc = scenes.dam_break(width=0.8, height=2.0, dx=0.005)
+    g = GpuSolver(sc)
+    g.step(100)
+    x = g.state()["x"]
+    nf = sc.n_fluid
+    win = (x[:, 0] > -0.02) & (x[:, 0] < 0.23) & (x[:, 1] > -0.02) & (x[:, 1] < 0.10)
+    fl = np.nonzero(win & (np.arange(sc.n) < nf))[0]
+    wl = np.nonzero(win & (np.arange(sc.n) >= nf))[0]
+    pos = np.concatenate([x[fl], x[wl]])
+    _say(f"window: {len(fl)} fluid + {len(wl)} wall particles")
+    assert len(fl) > 500 and len(wl) > 50
+    _check(pos, len(fl), 2.0 * sc.h, 128, 8, seed=9, hist_scale=0.05)

@@ -3467,10 +3467,14 @@ __global__ void k_graph_sender_perm(int n, StepBufs b, const DevScene* __restric
   });
 }
 
-__global__ void k_bbox(int n, const double2* __restrict__ pos, double* __restrict__ out) {
+#define BBOX_BLOCKS 128
+
+// Block b's bounding box {min x, min y, max x, max y} of its grid-stride share of the
+// positions into part[4 b ..]; k_bbox_final folds the blocks' boxes into out[0..3].
+__global__ __launch_bounds__(256) void k_bbox(int n, const double2* __restrict__ pos, double* __restrict__ part) {
   __shared__ double red[4][256];
   double lx = INFINITY, ly = INFINITY, hx = -INFINITY, hy = -INFINITY;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
     const double2 p = pos[i];
     lx = fmin(lx, p.x); ly = fmin(ly, p.y); hx = fmax(hx, p.x); hy = fmax(hy, p.y);
   }
@@ -3485,8 +3489,26 @@ __global__ void k_bbox(int n, const double2* __restrict__ pos, double* __restric
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0)
-    for (int k = 0; k < 4; ++k) out[k] = red[k][0];
+  if (threadIdx.x < 4) part[4 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// One wavefront; nb <= 2 x 64.
+__global__ void k_bbox_final(int nb, const double* __restrict__ part, double* __restrict__ out) {
+  const int l = threadIdx.x;
+  double v[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    v[k] = k < 2 ? INFINITY : -INFINITY;
+    for (int b = l; b < nb; b += 64) v[k] = k < 2 ? fmin(v[k], part[4 * b + k]) : fmax(v[k], part[4 * b + k]);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const double w = __shfl_xor(v[k], o, 64);
+      v[k] = k < 2 ? fmin(v[k], w) : fmax(v[k], w);
+    }
+  }
+  if (l == 0)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) out[k] = v[k];
 }
 
 namespace {
@@ -3547,10 +3569,14 @@ extern "C" int uph_graph_build(const double* pos, int n, int n_recv, double dist
     return -1;
   }
   if (!c.st) HIPCHK(hipStreamCreateWithFlags(&c.st, hipStreamNonBlocking), c.err);
-  if (!c.bbox) HIPCHK(hipMalloc((void**)&c.bbox, 4 * sizeof(double)), c.err);
+  if (!c.bbox) HIPCHK(hipMalloc((void**)&c.bbox, 4 * (1 + BBOX_BLOCKS) * sizeof(double)), c.err);
   // the caller may have written pos on any stream
   HIPCHK(hipDeviceSynchronize(), c.err);
-  hipLaunchKernelGGL(k_bbox, dim3(1), dim3(256), 0, c.st, n, (const double2*)pos, c.bbox);
+  {
+    const int nbb = std::max(1, std::min(BBOX_BLOCKS, (n + 255) / 256));
+    hipLaunchKernelGGL(k_bbox, dim3(nbb), dim3(256), 0, c.st, n, (const double2*)pos, c.bbox + 4);
+    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, c.st, nbb, c.bbox + 4, c.bbox);
+  }
   double bb[4];
   HIPCHK(hipMemcpyAsync(bb, c.bbox, sizeof bb, hipMemcpyDeviceToHost, c.st), c.err);
   HIPCHK(hipStreamSynchronize(c.st), c.err);
@@ -3600,11 +3626,13 @@ extern "C" int uph_graph_build(const double* pos, int n, int n_recv, double dist
   }
   hipLaunchKernelGGL(k_graph_emit, dim3(grid8_of(n)), dim3(BLK), 0, st, n, c.b, c.dsc, n_recv, off, send, recv);
   // senders
-  hipLaunchKernelGGL(k_graph_count, dim3(grid8_of(n)), dim3(BLK), 0, st, n, c.b, c.dsc, n_recv, c.cnt, 1);
-  hipLaunchKernelGGL(k_scan_reduce, dim3(qnb), dim3(SCAN_BLK), 0, st, n, c.cnt, c.qpartial, (int*)nullptr);
-  hipLaunchKernelGGL(k_scan_down, dim3(qnb), dim3(SCAN_BLK), 0, st, n, c.cnt, c.qpartial, soff);
-  hipLaunchKernelGGL(k_graph_sender_perm, dim3(grid8_of(n)), dim3(BLK), 0, st, n, c.b, c.dsc, n_recv, off, send,
-                     soff, sperm);
+  if (soff && sperm) {
+    hipLaunchKernelGGL(k_graph_count, dim3(grid8_of(n)), dim3(BLK), 0, st, n, c.b, c.dsc, n_recv, c.cnt, 1);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(qnb), dim3(SCAN_BLK), 0, st, n, c.cnt, c.qpartial, (int*)nullptr);
+    hipLaunchKernelGGL(k_scan_down, dim3(qnb), dim3(SCAN_BLK), 0, st, n, c.cnt, c.qpartial, soff);
+    hipLaunchKernelGGL(k_graph_sender_perm, dim3(grid8_of(n)), dim3(BLK), 0, st, n, c.b, c.dsc, n_recv, off,
+                       send, soff, sperm);
+  }
   HIPCHK(hipGetLastError(), c.err);
   HIPCHK(hipStreamSynchronize(st), c.err);
   return 0;

@@ -39,19 +39,19 @@ class Graph:
         self.sperm = DevBuf(4 * capacity)
         self.E = 0
 
-    def build(self, pos: DevBuf, dist: float):
+    def build(self, pos: DevBuf, dist: float, sender: bool = True):
+        """sender=False skips the sender view (soff, sperm), which only backward passes read."""
         import ctypes as C
         ne = C.c_long()
+        so = lambda: (self.soff.ptr, self.sperm.ptr) if sender else (None, None)
         rc = lib().uph_graph_build(pos.ptr, self.n, self.n_recv, float(dist), self.off.ptr,
-                                   self.send.ptr, self.recv.ptr, self.soff.ptr, self.sperm.ptr,
-                                   self.cap, C.byref(ne))
+                                   self.send.ptr, self.recv.ptr, *so(), self.cap, C.byref(ne))
         if rc == -2:
             self.cap = int(ne.value * 1.25) + 64
             self.send, self.recv, self.sperm = (DevBuf(4 * self.cap), DevBuf(4 * self.cap),
                                                 DevBuf(4 * self.cap))
             rc = lib().uph_graph_build(pos.ptr, self.n, self.n_recv, float(dist), self.off.ptr,
-                                       self.send.ptr, self.recv.ptr, self.soff.ptr,
-                                       self.sperm.ptr, self.cap, C.byref(ne))
+                                       self.send.ptr, self.recv.ptr, *so(), self.cap, C.byref(ne))
         if rc != 0:
             raise RuntimeError("uph_graph_build: " + lib().uph_graph_last_error().decode())
         self.E = int(ne.value)

@@ -136,6 +136,8 @@ SIGNATURES = {
                         [C.c_double, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
                          C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
                          C.c_void_p, C.c_void_p]),
+    "uph_sn_fused": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double] +
+                     [C.c_void_p] * 19),
     "uph_sn_last_error": (C.c_char_p, []),
     "uph_state_to_device": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     "uph_state_from_device": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,

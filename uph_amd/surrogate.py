@@ -12,11 +12,15 @@ controller keeps the plain solver authoritative:
   density over the graph), which runs one macro step of plain steps: the
   correction;
 * if the mean summation density of the fluid drifts by more than `drift_tol`
-  over a block, or one particle's summation density changes by more than
-  `spike_tol` * rho0 (counting particles with at least 0.9 rho0 at either end,
-  so that free-surface particles, whose support is cut, do not count), or
-  anything is non-finite, the block is discarded and the plain solver, which
-  still holds the state of the block's start, runs it instead.
+  (1 %) over a block, or anything is non-finite, the block is discarded and the
+  plain solver, which still holds the state of the block's start, runs it
+  instead.  So is a block that leaves a fluid particle outside the scene's box
+  (a state the plain solver cannot continue).  The mean alone lets a block
+  through where a few particles are flung apart, and the next blocks build on
+  it, so by default a second test (`spike_tol`, None to switch it off) also
+  rejects a block where one particle's summation density changes by more than
+  spike_tol * rho0 (counting particles with at least 0.9 rho0 at either end, so
+  that free-surface particles, whose support is cut, do not count).
 
 Training data are plain-solver trajectories sampled every macro step; the
 target is the second difference of positions, (x[t+1] - 2 x[t] + x[t-1]) / Dt^2,
@@ -42,6 +46,7 @@ from . import scenes as scn
 from .devbuf import DevBuf, check, lib
 from .gnn import Graph, GraphNet
 from .solver import GpuSolver
+from .surrogate_net import SurrogateNet
 
 HIST = 5
 
@@ -55,7 +60,8 @@ class SurrogateConfig:
     noise: float = 0.1          # input-velocity noise, in units of ascale * macro dt
     every: int = 10             # network macro steps between plain corrections
     drift_tol: float = 0.01     # fallback threshold on the mean summation density
-    spike_tol: float = 0.05     # ... and on any one particle's (see Rollout.block)
+    spike_tol: Optional[float] = 0.05   # ... and on any one particle's (see Rollout.block; None: off)
+    arch: str = "rollout"       # "rollout": SurrogateNet (no edge latent); "gns": GraphNet
     lr: float = 1e-3
     lr_final: float = 1e-5
     iters: int = 2000
@@ -103,30 +109,46 @@ class Dataset:
 
 
 class _Feats:
-    """Per-scene device buffers for graph, features and inputs."""
+    """Per-scene device buffers for graph, features and inputs (edge features only for
+    the GraphNet form; the rollout network forms them from the positions)."""
 
-    def __init__(self, sc: scn.Scene, cap: int = 0):
+    def __init__(self, sc: scn.Scene, cap: int = 0, edge_feats: bool = False):
         n = sc.n
         self.n, self.nf = n, sc.n_fluid
         self.graph = Graph(n, sc.n_fluid, max(cap, 32 * n))
         self.pos = DevBuf(16 * n)
         self.hist = DevBuf(16 * HIST * n)
         self.xn = DevBuf(2 * 16 * n)
-        self.xe = DevBuf(2 * 8 * self.graph.cap)
+        self.edge_feats = edge_feats
+        self.xe = DevBuf(2 * 8 * self.graph.cap) if edge_feats else None
 
-    def build(self, dist: float, invR: float, inv_vscale: float):
-        g = self.graph.build(self.pos, dist)
-        if self.xe.nbytes < 16 * g.E:
-            self.xe = DevBuf(16 * g.cap)
+    def build(self, dist: float, invR: float, inv_vscale: float, sender: bool = True):
+        g = self.graph.build(self.pos, dist, sender=sender)
         check(lib().uph_gns_node_feats(self.n, self.nf, self.hist.ptr, inv_vscale, self.xn.ptr, None))
-        check(lib().uph_gns_edge_feats(g.E, g.send.ptr, g.recv.ptr, self.pos.ptr, invR, self.xe.ptr,
-                                       None))
+        if self.edge_feats:
+            if self.xe.nbytes < 16 * g.E:
+                self.xe = DevBuf(16 * g.cap)
+            check(lib().uph_gns_edge_feats(g.E, g.send.ptr, g.recv.ptr, self.pos.ptr, invR, self.xe.ptr,
+                                           None))
         return g
+
+
+def make_net(cfg: SurrogateConfig, seed: int = 0):
+    return (SurrogateNet(cfg.width, cfg.mp_steps, seed=seed) if cfg.arch == "rollout"
+            else GraphNet(cfg.width, cfg.mp_steps, seed=seed))
+
+
+def net_forward(net, f: "_Feats", g: Graph, invR: float, train: bool):
+    if isinstance(net, SurrogateNet):
+        if not train and net.W == 128:
+            return net.forward_fused(g, f.pos, invR, f.xn)
+        return net.forward(g, f.pos, invR, f.xn, train=train)
+    return net.forward(g, f.xn, f.xe, train=train)
 
 
 def train(net: GraphNet, data: Dataset, cfg: SurrogateConfig, log=None) -> List[float]:
     rng = np.random.default_rng(cfg.seed)
-    feats = [_Feats(sc) for sc in data.scenes]
+    feats = [_Feats(sc, edge_feats=not isinstance(net, SurrogateNet)) for sc in data.scenes]
     tg = [DevBuf(8 * sc.n) for sc in data.scenes]
     masks = [DevBuf.of((np.arange(sc.n) < sc.n_fluid).astype(np.int32)) for sc in data.scenes]
     losses = []
@@ -142,7 +164,7 @@ def train(net: GraphNet, data: Dataset, cfg: SurrogateConfig, log=None) -> List[
         tg[s].upload(tgt)
         R = cfg.radius_h * sc.h
         g = f.build(R * R, 1.0 / R, 1.0 / data.vscale)
-        net.forward(g, f.xn, f.xe, train=True)
+        net_forward(net, f, g, 1.0 / R, train=True)
         net.zero_grad()
         losses.append(net.loss(tg[s], masks[s], sc.n_fluid))
         net.backward()
@@ -164,7 +186,7 @@ class Rollout:
         self.dt = cfg.substeps * scene.dt
         self.P = GpuSolver(scene)
         n, nf = scene.n, scene.n_fluid
-        self.f = _Feats(scene)
+        self.f = _Feats(scene, edge_feats=net is not None and not isinstance(net, SurrogateNet))
         self.vel = DevBuf(16 * n)
         self.rho = DevBuf(8 * n)
         self.rho2 = DevBuf(8 * n)
@@ -172,7 +194,8 @@ class Rollout:
         self.hist2 = DevBuf(16 * HIST * n)
         self.mass = DevBuf.of(np.ascontiguousarray(scene.mass, np.float64))
         self.rs0, self.rs1 = DevBuf(8 * nf), DevBuf(8 * nf)
-        self.blocks = self.fallbacks = self.macro = 0
+        self.blocks = self.fallbacks = self.macro = self.escapes = 0
+        self.kept_drifts: List[float] = []
         self.drifts: List[float] = []
         self.spikes: List[float] = []
         self.t_net = self.t_plain = 0.0
@@ -236,20 +259,30 @@ class Rollout:
         ok = True
         t0 = time.perf_counter()
         for k in range(cfg.every):
-            g = self.f.build(self.dist, self.invR, 1.0 / self.vscale)
+            g = self.f.build(self.dist, self.invR, 1.0 / self.vscale, sender=False)
             if k == 0:
                 check(L.uph_sph_density(nf, g.off.ptr, g.send.ptr, self.f.pos.ptr, self.mass.ptr,
                                         sc.h, self.rs0.ptr, None))
-            out = self.net.forward(g, self.f.xn, self.f.xe, train=False)
+            out = net_forward(self.net, self.f, g, self.invR, train=False)
             check(L.uph_gns_integrate(nf, self.f.pos.ptr, self.vel.ptr, out.ptr, 8, self.dt,
                                       self.ascale, None))
             self._push_velocity(self.vel.ptr)
         check(L.uph_dev_sync())
         self.t_net += time.perf_counter() - t0
         try:
-            g = self.f.graph.build(self.f.pos, self.dist)
+            g = self.f.graph.build(self.f.pos, self.dist, sender=False)
         except RuntimeError:                  # non-finite positions
             ok = False
+        if ok:
+            # a state the plain solver can continue: every fluid position finite and inside
+            # the scene's box (a particle flung out of the tank keeps no neighbours, and a
+            # mean over the fluid hardly sees a few of them)
+            x = self.f.pos.download((n, 2), np.float64)[:nf]
+            x0, y0, x1, y1 = sc.box
+            ok = bool(np.all(np.isfinite(x)) and np.all((x[:, 0] > x0) & (x[:, 0] < x1) &
+                                                         (x[:, 1] > y0) & (x[:, 1] < y1)))
+            if not ok:
+                self.escapes += 1
         if ok:
             check(L.uph_sph_density(nf, g.off.ptr, g.send.ptr, self.f.pos.ptr, self.mass.ptr, sc.h,
                                     self.rs1.ptr, None))
@@ -260,7 +293,9 @@ class Rollout:
             self.drifts.append(drift)
             self.spikes.append(spike)
             ok = (math.isfinite(drift) and math.isfinite(spike) and drift <= cfg.drift_tol
-                  and spike <= cfg.spike_tol)
+                  and (cfg.spike_tol is None or spike <= cfg.spike_tol))
+            if ok:
+                self.kept_drifts.append(drift)
         if ok:
             # densities carried across the block; the plain solver takes the state and
             # runs the correcting macro step
@@ -303,7 +338,9 @@ def load(path: str):
     d = {k: z[k] for k in z.files}
     cfg = SurrogateConfig(**json.loads(str(d.pop("_cfg"))))
     vs, as_ = float(d.pop("_vscale")), float(d.pop("_ascale"))
-    return GraphNet.from_state(d), vs, as_, cfg
+    arch = str(d.pop("_arch", "gns"))
+    net = SurrogateNet.from_state(d) if arch == "rollout" else GraphNet.from_state(d)
+    return net, vs, as_, cfg
 
 
 def training_scenes(n_scenes: int, dx: float, eos_height: float, seed: int,
@@ -327,6 +364,39 @@ def errors(a: dict, b: dict, sc: scn.Scene) -> dict:
                 rho_err_max=float(drho.max()))
 
 
+def time_macro_step(dx: float = 0.005, width: float = 0.8, height: float = 2.0, reps: int = 20) -> dict:
+    """The rollout network's macro step on a dam break (default: the 64,000-particle
+    one) against the plain solver, with initial weights (the cost does not depend on
+    them): ms for the radius-graph build, the node features and the fused W128 / S8
+    forward, and for 20 plain steps; HIP work timed by device synchronisation."""
+    sc = scn.dam_break(width=width, height=height, dx=dx)
+    g = GpuSolver(sc)
+    g.step(50)
+    st = g.state()
+    n, nf = sc.n, sc.n_fluid
+    pos = DevBuf.of(np.ascontiguousarray(st["x"]))
+    R = 2.0 * sc.h
+    graph = Graph(n, nf, 32 * n)
+    hist = DevBuf.of(np.random.default_rng(0).normal(0.0, 0.1, (HIST, n, 2)))
+    xn = DevBuf(2 * 16 * n)
+    net = SurrogateNet(128, 8)
+
+    def t(f, k=reps):
+        f()
+        check(lib().uph_dev_sync())
+        t0 = time.perf_counter()
+        for _ in range(k):
+            f()
+        check(lib().uph_dev_sync())
+        return 1e3 * (time.perf_counter() - t0) / k
+    out = {"fluid": nf, "particles": n, "edges": graph.build(pos, R * R, sender=False).E}
+    out["graph_ms"] = t(lambda: graph.build(pos, R * R, sender=False))
+    out["node_feats_ms"] = t(lambda: check(lib().uph_gns_node_feats(n, nf, hist.ptr, 1.0, xn.ptr, None)))
+    out["fused_forward_ms"] = t(lambda: net.forward_fused(graph, pos, 1.0 / R, xn))
+    out["plain_20_steps_ms"] = t(lambda: (g.step(20), g.sync()), max(reps // 4, 1))
+    return out
+
+
 def main(argv=None):
     import argparse
     ap = argparse.ArgumentParser(description="Train the surrogate on dam breaks and measure a "
@@ -342,10 +412,22 @@ def main(argv=None):
     ap.add_argument("--substeps", type=int, default=20)
     ap.add_argument("--macro-steps", type=int, default=400)
     ap.add_argument("--model", default=None, help="save the trained model (.npz)")
+    ap.add_argument("--arch", choices=["rollout", "gns"], default="rollout",
+                    help="rollout: the network without edge latents (default); gns: edge latents")
+    ap.add_argument("--spike-tol", type=float, default=0.05,
+                    help="also fall back when one particle's summation density moves by more than "
+                         "this fraction of rho0 over a block (0: off)")
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--time-only", action="store_true",
+                    help="only time the network's macro step against 20 plain steps (time_macro_step)")
     a = ap.parse_args(argv)
+    if a.time_only:
+        res = time_macro_step(dx=a.dx, width=a.width, height=a.height)
+        print(json.dumps(res), flush=True)
+        return res
     say = lambda m: print(f"[surrogate] {m}", flush=True)
     cfg = SurrogateConfig(substeps=a.substeps, width=a.net_width, mp_steps=a.mp_steps,
-                          iters=a.iters)
+                          iters=a.iters, arch=a.arch, spike_tol=a.spike_tol or None, lr=a.lr)
     tr = training_scenes(a.train_scenes, a.dx, a.height, seed=20261203)
     t0 = time.perf_counter()
     trajs = [record(sc, a.frames, cfg.substeps) for sc in tr]
@@ -353,7 +435,7 @@ def main(argv=None):
     t_rec = time.perf_counter() - t0
     say(f"recorded {len(tr)} trajectories ({sum(s.n_fluid for s in tr)} fluid) x {a.frames} "
         f"frames in {t_rec:.1f} s; vscale {data.vscale:.3f} m/s, ascale {data.ascale:.2f} m/s^2")
-    net = GraphNet(cfg.width, cfg.mp_steps, seed=cfg.seed)
+    net = make_net(cfg, seed=cfg.seed)
     t0 = time.perf_counter()
     losses = train(net, data, cfg, log=say)
     t_train = time.perf_counter() - t0
@@ -379,7 +461,7 @@ def main(argv=None):
     t_plain = time.perf_counter() - t0
     err = errors(ro.state(), ref.state(), sc)
     # ablation: the same controller with the network's output forced to zero
-    zero = GraphNet(cfg.width, cfg.mp_steps, params={k: 0 * v for k, v in net.params().items()})
+    zero = type(net)(cfg.width, cfg.mp_steps, params={k: 0 * v for k, v in net.params().items()})
     rz = Rollout(sc, zero, data.vscale, data.ascale, cfg)
     rz.run(a.macro_steps)
     err0 = errors(rz.state(), ref.state(), sc)

@@ -228,3 +228,41 @@ class SurrogateNet(GraphNet):
         self._lin_dx(L["enc2"], b["g16a"].ptr, Wl, n, b["t1"].ptr, Wl)
         check(Lb.uph_relu_bwd_rows(n, W, b["t1"].ptr, Wl, b["un"][0].ptr, Wl, b["g16b"].ptr, Wl, None))
         self._lin_dw(L["enc1"], self._xn.ptr, NODE_IN, n, b["g16b"].ptr, Wl)
+
+    # ------------------------------------------------------------ fused inference
+
+    def forward_fused(self, g: Graph, pos: DevBuf, invR: float, xn: DevBuf) -> DevBuf:
+        """The same network as forward(), inference only, in 1 + S launches of
+        uph_sn_fused (width 128): the encoder, then one launch per message-passing step
+        (edge sums, node MLP, residual, next projections; the last one the decoder).
+        Returns the output buffer (n x 8 FP32, columns 0..1)."""
+        if self.W != 128:
+            raise ValueError("the fused path is built for width 128")
+        n = g.n
+        fb = getattr(self, "fb", None)
+        if fb is None or fb["n"] < n:
+            m = int(n * 1.1) + 64
+            fb = self.fb = {"n": m, "h": DevBuf(4 * 128 * m), "out": DevBuf(4 * 8 * m),
+                            "zb": [DevBuf(4 * 128 * m) for _ in range(2)],
+                            "zc": [DevBuf(2 * 128 * m) for _ in range(2)]}
+        L, Lb = self.lin, lib()
+        wt = lambda k: L[k].wt.ptr
+        _check_sn(Lb.uph_sn_fused(0, n, g.n_recv, None, None, None, 0.0, None, None, None, None, None, None,
+                                  None, fb["h"].ptr, wt("p0_B"), wt("p0_C"), fb["zb"][0].ptr, fb["zc"][0].ptr,
+                                  None, None, None, xn.ptr, wt("enc1"), wt("enc2"), None))
+        for l in range(self.S):
+            cur, nxt = l % 2, 1 - l % 2
+            last = l == self.S - 1
+            P = lambda k: wt(f"p{l}_{k}")
+            _check_sn(Lb.uph_sn_fused(
+                2 if last else 1, n, g.n_recv, g.off.ptr, g.send.ptr, pos.ptr, float(invR), L[f"p{l}_A"].w32.ptr,
+                fb["zb"][cur].ptr, fb["zc"][cur].ptr, P("D"), P("E"), P("F"), P("G"), fb["h"].ptr,
+                None if last else wt(f"p{l + 1}_B"), None if last else wt(f"p{l + 1}_C"),
+                None if last else fb["zb"][nxt].ptr, None if last else fb["zc"][nxt].ptr,
+                wt("dec1") if last else None, wt("dec2") if last else None, fb["out"].ptr if last else None,
+                None, None, None, None))
+        return fb["out"]
+
+    def fused_output(self, n: int) -> np.ndarray:
+        check(lib().uph_dev_sync())
+        return self.fb["out"].download((n, 8), np.float32)[:, :OUT]
