@@ -111,6 +111,38 @@ def copy_probe(mib):
     print(json.dumps({"copy_probe_bytes": n}), flush=True)
 
 
+def compare_launch(steps: int, warmup: int):
+    """ms per step of three ways of launching the same steps: one launch per kernel
+    (uph_step), the step captured as a hipGraph and replayed, and all steps in one
+    persistent cooperative kernel (uph_step_persistent); on the 1M dam break, the
+    512-scene batch of dam breaks (5.47 M fluid) and a 5,000-particle dam break.  Host
+    wall clock around the steps, bracketed by stream synchronisation."""
+    from uph_amd import scenes
+    from uph_amd.solver import GpuSolver
+    work = {"dam break 1M (999,698 fluid)": lambda: [scenes.dam_break(dx=2 ** 0.5 * 1e-3)],
+            "512 dam breaks (config 4)": scenes.batch_dam_breaks,
+            "dam break 5K": lambda: [scenes.dam_break()]}
+    out = {}
+    for name, make in work.items():
+        scs = make()
+        nf = sum(s.n_fluid for s in scs)
+        res = {}
+        for mode in ("eager", "graph", "persistent"):
+            g = GpuSolver(scs, use_graph=mode == "graph")
+            run = g.step_persistent if mode == "persistent" else g.step
+            run(warmup)
+            g.sync()
+            t0 = time.perf_counter()
+            run(steps)
+            g.sync()
+            ms = (time.perf_counter() - t0) * 1e3 / steps
+            res[mode] = {"ms_per_step": ms, "updates_per_s": nf * 1e3 / ms}
+            g.close()
+            log(f"{name}, {mode}: {ms:.4f} ms per step")
+        out[name] = res
+    print(json.dumps({"compare_launch": out, "steps": steps, "warmup": warmup}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -132,11 +164,17 @@ def main():
     ap.add_argument("--vtype", type=int, default=0,
                     help="virtualType of the tank (mirror particles of derivatives.c:3193-4192; 0: none; "
                          "8: the dam break's left, bottom, corner and right images)")
+    ap.add_argument("--compare-launch", action="store_true",
+                    help="only compare per-kernel launches, hipGraph replay and the persistent kernel "
+                         "(compare_launch) and exit")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run one D2D copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe(args.copy_probe)
+        return
+    if args.compare_launch:
+        compare_launch(args.steps, args.warmup)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "

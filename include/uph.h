@@ -168,6 +168,15 @@ void uph_destroy(uph_t *h);
 /* Advance every scene by n_steps leapfrog steps (asynchronous on the handle's stream). */
 int uph_step(uph_t *h, int n_steps);
 
+/* uph_step in one persistent launch: all n_steps whole steps (binning, scan, reorder,
+ * pair sums, capped pass) run in one cooperative kernel whose blocks meet at grid-wide
+ * barriers between the phases (7 per step), instead of 7 launches per step.  For
+ * handles without mirror rows (virtualType sections), whose queries are not expected to
+ * be capped (4 pi (h/dx)^2 <= 3/4 K) and which use the compacted pair kernel; -1
+ * otherwise.  Results equal uph_step's bit for bit.  No counterpart in the reference.
+ * A barrier that waits too long gives up rather than hang; uph_sync then fails. */
+int uph_step_persistent(uph_t *h, int n_steps);
+
 /* Wait for the handle's stream; returns the number of steps in which some fluid
  * particle had more than K neighbours in range (exact capped path taken). */
 int uph_sync(uph_t *h);

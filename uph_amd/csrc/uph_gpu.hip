@@ -51,11 +51,12 @@
 
 // ------------------------------------------------------------------ sort
 
-__global__ __launch_bounds__(BLK) void k_bin(int n, const double2* __restrict__ pos, const int* __restrict__ sceneOf,
-                                             const DevScene* __restrict__ sc, const uint32_t* __restrict__ tagA,
-                                             uint32_t* __restrict__ tagB, int* __restrict__ keys,
-                                             int* __restrict__ rank, int* __restrict__ count) {
-  const int i = blockIdx.x * BLK + threadIdx.x;
+// One block of the binning: slots blk * BLK + [0, BLK).
+__device__ __forceinline__ void bin_block(int blk, int n, const double2* __restrict__ pos,
+                                          const int* __restrict__ sceneOf, const DevScene* __restrict__ sc,
+                                          const uint32_t* __restrict__ tagA, uint32_t* __restrict__ tagB,
+                                          int* __restrict__ keys, int* __restrict__ rank, int* __restrict__ count) {
+  const int i = blk * BLK + threadIdx.x;
   const bool valid = i < n;
   int key = -1;
   if (valid) {
@@ -81,6 +82,13 @@ __global__ __launch_bounds__(BLK) void k_bin(int n, const double2* __restrict__ 
   if (valid && lane == rstart) base = atomicAdd(&count[key], rend - rstart);
   base = __shfl(base, rstart, 64);
   if (valid) rank[i] = base + (lane - rstart);
+}
+
+__global__ __launch_bounds__(BLK) void k_bin(int n, const double2* __restrict__ pos, const int* __restrict__ sceneOf,
+                                             const DevScene* __restrict__ sc, const uint32_t* __restrict__ tagA,
+                                             uint32_t* __restrict__ tagB, int* __restrict__ keys,
+                                             int* __restrict__ rank, int* __restrict__ count) {
+  bin_block(blockIdx.x, n, pos, sceneOf, sc, tagA, tagB, keys, rank, count);
 }
 
 // Block-wide exclusive scan of SCAN_TILE ints; returns the tile total.
@@ -123,15 +131,17 @@ __device__ int block_scan_tile(int* vals /* SCAN_ITEMS per thread, in registers 
 
 // Per-tile totals of `in`.  With `step_ctl` it also opens a step: itime += 1,
 // the overflow flag cleared, the K-th key stamp advanced.
-__global__ __launch_bounds__(SCAN_BLK) void k_scan_reduce(int m, const int* __restrict__ in, int* __restrict__ partial,
-                                                          int* __restrict__ step_ctl) {
-  __shared__ int lds[SCAN_BLK / 64 + 1];
-  if (step_ctl && blockIdx.x == 0 && threadIdx.x == 0) {
-    step_ctl[0] += 1;   // itime of the step being computed
-    step_ctl[1] = 0;    // overflow flag of this step
-    step_ctl[3] += 1;   // stamp of this step's K-th keys
-  }
-  const long base = (long)blockIdx.x * SCAN_TILE + threadIdx.x;
+// Opens a step: itime += 1, the overflow flag cleared, the K-th key stamp advanced.
+__device__ __forceinline__ void open_step(int* step_ctl) {
+  step_ctl[0] += 1;   // itime of the step being computed
+  step_ctl[1] = 0;    // overflow flag of this step
+  step_ctl[3] += 1;   // stamp of this step's K-th keys
+}
+
+// The total of tile `tile` into partial[tile] (lds: SCAN_BLK / 64 + 1 ints).
+__device__ __forceinline__ void scan_reduce_tile(int tile, int m, const int* __restrict__ in, int* __restrict__ partial,
+                                                 int* lds) {
+  const long base = (long)tile * SCAN_TILE + threadIdx.x;
   int s = 0;
 #pragma unroll
   for (int k = 0; k < SCAN_ITEMS; ++k) {
@@ -146,8 +156,16 @@ __global__ __launch_bounds__(SCAN_BLK) void k_scan_reduce(int m, const int* __re
   if (threadIdx.x == 0) {
     int t = 0;
     for (int w = 0; w < SCAN_BLK / 64; ++w) t += lds[w];
-    partial[blockIdx.x] = t;
+    partial[tile] = t;
   }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(SCAN_BLK) void k_scan_reduce(int m, const int* __restrict__ in, int* __restrict__ partial,
+                                                          int* __restrict__ step_ctl) {
+  __shared__ int lds[SCAN_BLK / 64 + 1];
+  if (step_ctl && blockIdx.x == 0 && threadIdx.x == 0) open_step(step_ctl);
+  scan_reduce_tile(blockIdx.x, m, in, partial, lds);
 }
 
 // Exclusive scan of `count` into `start` (m+1 entries), zeroing `count`.  The
@@ -155,11 +173,11 @@ __global__ __launch_bounds__(SCAN_BLK) void k_scan_reduce(int m, const int* __re
 // coalesced and each thread scans 16 consecutive entries without bank conflicts.
 __device__ __forceinline__ int tile_ix(int e) { return e + (e >> 4); }
 
-__global__ __launch_bounds__(SCAN_BLK) void k_scan_down(int m, int* __restrict__ count, const int* __restrict__ partial,
-                                                        int* __restrict__ start) {
-  __shared__ int tile[SCAN_TILE + SCAN_TILE / 16];
-  __shared__ int lds[SCAN_BLK / 64 + 1];
-  const long base = (long)blockIdx.x * SCAN_TILE;
+// Tile `tb` of the exclusive scan (tile: SCAN_TILE + SCAN_TILE / 16 ints of LDS, lds:
+// SCAN_BLK / 64 + 1 ints).
+__device__ __forceinline__ void scan_down_tile(int tb, int m, int* __restrict__ count, const int* __restrict__ partial,
+                                               int* __restrict__ start, int* tile, int* lds) {
+  const long base = (long)tb * SCAN_TILE;
   const int tid = threadIdx.x;
 #pragma unroll
   for (int k = 0; k < SCAN_ITEMS; ++k) {
@@ -179,7 +197,7 @@ __global__ __launch_bounds__(SCAN_BLK) void k_scan_down(int m, int* __restrict__
   block_scan_tile(v, lds);
   // offset of this tile: the sum of the preceding tiles' totals
   int pre = 0;
-  for (int t = tid; t < (int)blockIdx.x; t += SCAN_BLK) pre += partial[t];
+  for (int t = tid; t < tb; t += SCAN_BLK) pre += partial[t];
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) pre += __shfl_down(pre, o, 64);
   if ((tid & 63) == 0) lds[tid >> 6] = pre;
@@ -202,6 +220,14 @@ __global__ __launch_bounds__(SCAN_BLK) void k_scan_down(int m, int* __restrict__
     const long i = base + e;
     if (i < m) start[i] = tile[tile_ix(e)];
   }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(SCAN_BLK) void k_scan_down(int m, int* __restrict__ count, const int* __restrict__ partial,
+                                                        int* __restrict__ start) {
+  __shared__ int tile[SCAN_TILE + SCAN_TILE / 16];
+  __shared__ int lds[SCAN_BLK / 64 + 1];
+  scan_down_tile(blockIdx.x, m, count, partial, start, tile, lds);
 }
 
 __global__ __launch_bounds__(BLK) void k_scatter(int n, const int* __restrict__ keys, const int* __restrict__ rank,
@@ -325,8 +351,8 @@ __device__ __forceinline__ bool mark_mirrors(const DevScene& S, const StepBufs& 
 // MIRRORS: some scene has mirror rows (the section tests run here); without them the
 // kernel is built without that code and keeps its smaller register allocation.
 template <bool MIRRORS>
-__global__ __launch_bounds__(BLK) void k_reorder(int n, StepBufs b, const DevScene* __restrict__ sc, int search_only) {
-  const int d = blockIdx.x * BLK + threadIdx.x;
+__device__ __forceinline__ void reorder_slot(int d, int n, const StepBufs& b, const DevScene* __restrict__ sc,
+                                             int search_only) {
   if (d >= n) return;
   const int i = b.tmpIdx[d];
   const int c = b.keys[i];
@@ -422,6 +448,11 @@ __global__ __launch_bounds__(BLK) void k_reorder(int n, StepBufs b, const DevSce
     b.predP[dst] = P;
     b.predPl[dst] = 0.0;
   }
+}
+
+template <bool MIRRORS>
+__global__ __launch_bounds__(BLK) void k_reorder(int n, StepBufs b, const DevScene* __restrict__ sc, int search_only) {
+  reorder_slot<MIRRORS>(blockIdx.x * BLK + threadIdx.x, n, b, sc, search_only);
 }
 
 // ------------------------------------------------------ mirror particles
@@ -2069,6 +2100,181 @@ __global__ __launch_bounds__(BLK) void k_search_emit(int n, StepBufs b, const De
   }
 }
 
+// ------------------------------------------------------------ persistent step
+//
+// k_persist runs n_steps whole steps in one cooperative launch: the phases of the
+// launched step (bin, scan totals, scan, scatter, reorder, compacted pair pass with the
+// mixed-scene wavefronts, capped pass) as grid-stride loops over the same blocks the
+// separate kernels would take, separated by grid-wide barriers (7 per step).  For
+// handles whose queries are not expected to be capped and that have no mirror rows; a
+// capped step (a query with more than K in range, rare there) takes the direct capped
+// loop, which sums in the same slot order as the bit walk of k_pair<true>.
+
+// bar (BAR_WORDS words): the arrivals of each XCD's blocks (bar[BAR_GROUP + 16 x], one
+// cache line each), the arrivals of the XCD groups (bar[BAR_TOP]), the generation
+// (bar[BAR_GEN]) and the broken flag (bar[BAR_BROKEN]).  Two levels, so no counter takes
+// more than G / 8 atomics per barrier (one counter for every block serialised about
+// 80 ns per arrival: 85 us a barrier with 1024 blocks).  A block that polls the
+// generation PERSIST_SPIN times without a change gives up, sets the broken flag and goes
+// on; every later phase then skips its work, so the grid drains and the host reports
+// the failure.
+#define PERSIST_SPIN (1u << 24)
+#define BAR_GROUP 0
+#define BAR_TOP 128
+#define BAR_GEN 144
+#define BAR_BROKEN 160
+#define BAR_WORDS 176
+
+__device__ __forceinline__ bool grid_sync(unsigned* bar, unsigned nblocks) {
+  __shared__ int ok;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int good = 1;
+    const unsigned g = __hip_atomic_load(&bar[BAR_GEN], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __threadfence();
+    const unsigned x = blockIdx.x & 7;
+    bool bumped = false;
+    if (atomicAdd(&bar[BAR_GROUP + 16 * x], 1u) == (nblocks >> 3) - 1) {   // the last of its XCD group
+      atomicExch(&bar[BAR_GROUP + 16 * x], 0u);
+      __threadfence();                        // the reset lands before the generation can move
+      if (atomicAdd(&bar[BAR_TOP], 1u) == 7u) {                             // the last group
+        atomicExch(&bar[BAR_TOP], 0u);
+        __threadfence();
+        atomicAdd(&bar[BAR_GEN], 1u);
+        bumped = true;
+      }
+    }
+    if (!bumped) {
+      unsigned spins = 0;
+      while (__hip_atomic_load(&bar[BAR_GEN], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
+        if (++spins > PERSIST_SPIN) {
+          atomicOr(&bar[BAR_BROKEN], 1u);
+          good = 0;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    __threadfence();
+    if (__hip_atomic_load(&bar[BAR_BROKEN], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) good = 0;
+    ok = good;
+  }
+  __syncthreads();
+  return ok != 0;
+}
+
+// The virtual blocks [0, V) of a phase taken by this block: each XCD (blocks are dealt
+// to the 8 XCDs round robin; the grid is a multiple of 8) takes one contiguous run, as
+// xcd_block() arranges for the separate kernels.  The trip count is uniform per block.
+template <class F>
+__device__ __forceinline__ void for_vblocks(int V, F&& f) {
+  const int per = gridDim.x >> 3;
+  const int x = blockIdx.x & 7, li = blockIdx.x >> 3;
+  const int chunk = (V + 7) >> 3;
+  const int v0 = x * chunk, v1 = min(V, v0 + chunk);
+  for (int v = v0 + li; v < v1; v += per) f(v);
+}
+
+struct PersistArgs {
+  int n, ncells, nb, nmixed, n_steps;
+  StepBufs b;
+  const DevScene* sc;
+  const int* mixed;
+  unsigned* bar;
+};
+
+// The arguments sit in device memory and each phase reloads the fields it uses from an
+// opaque copy of the pointer, so no phase keeps the others' pointers live (loop-
+// invariant, the ~100 pointers would otherwise be hoisted and spill into VGPRs).
+__device__ __forceinline__ const PersistArgs& phase_args(const PersistArgs* pa) {
+  asm volatile("" : "+s"(pa));
+  return *pa;
+}
+
+// Four wavefronts per SIMD (128 VGPRs): inside the loop over a phase's blocks the pair
+// pass needs about 133 VGPRs (91 as a kernel of its own).
+template <int KT, int MODE, bool ONE>
+__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) void k_persist(
+    const PersistArgs* __restrict__ pa) {
+  __shared__ __align__(16) unsigned char sm[sizeof(KthLds) * (BLK / 64)];
+  static_assert(sizeof(KthLds) * (BLK / 64) >= sizeof(int) * (SCAN_TILE + SCAN_TILE / 16 + SCAN_BLK / 64 + 1),
+                "the scan tile fits the pair pass's LDS");
+  int* tile = reinterpret_cast<int*>(sm);
+  int* slds = tile + SCAN_TILE + SCAN_TILE / 16;
+  KthLds* klds = reinterpret_cast<KthLds*>(sm);
+  const unsigned G = gridDim.x;
+  const int n_steps = pa->n_steps;
+  unsigned* bar = pa->bar;
+  for (int s = 0; s < n_steps; ++s) {
+    {
+      const PersistArgs& a = phase_args(pa);
+      const StepBufs& b = a.b;
+      const int nbl = (a.n + BLK - 1) / BLK;
+      for_vblocks(nbl, [&](int v) {
+        bin_block(v, a.n, b.posB, b.sceneOf, a.sc, b.tagA, b.tagB, b.keys, b.rank, b.cellCount);
+      });
+    }
+    if (!grid_sync(bar, G)) break;
+    {
+      const PersistArgs& a = phase_args(pa);
+      if (blockIdx.x == 0 && threadIdx.x == 0) open_step(a.b.ctl);
+      for_vblocks(a.nb, [&](int t) { scan_reduce_tile(t, a.ncells, a.b.cellCount, a.b.partial, slds); });
+    }
+    if (!grid_sync(bar, G)) break;
+    {
+      const PersistArgs& a = phase_args(pa);
+      for_vblocks(a.nb, [&](int t) {
+        scan_down_tile(t, a.ncells, a.b.cellCount, a.b.partial, a.b.cellStart, tile, slds);
+      });
+    }
+    if (!grid_sync(bar, G)) break;
+    {
+      const PersistArgs& a = phase_args(pa);
+      const StepBufs& b = a.b;
+      const int nbl = (a.n + BLK - 1) / BLK;
+      for_vblocks(nbl, [&](int v) {
+        const int i = v * BLK + threadIdx.x;
+        if (i < a.n) b.tmpIdx[b.cellStart[b.keys[i]] + b.rank[i]] = i;
+      });
+    }
+    if (!grid_sync(bar, G)) break;
+    {
+      const PersistArgs& a = phase_args(pa);
+      const int nbl = (a.n + BLK - 1) / BLK;
+      for_vblocks(nbl, [&](int v) { reorder_slot<false>(v * BLK + threadIdx.x, a.n, a.b, a.sc, 0); });
+    }
+    if (!grid_sync(bar, G)) break;
+    {
+      const PersistArgs& a = phase_args(pa);
+      const int nbl = (a.n + BLK - 1) / BLK;
+      for_vblocks(nbl, [&](int v) {
+        compact_query<KT, MODE, true, ONE>(a.n, v * BLK + threadIdx.x, a.b, a.sc, klds[threadIdx.x >> 6]);
+      });
+      // the wavefronts holding two scenes' slots, one per wavefront with the scene per lane
+      const int wid = blockIdx.x * (BLK / 64) + (threadIdx.x >> 6), nw = G * (BLK / 64);
+      for (int w = wid; w < a.nmixed; w += nw)
+        compact_query<KT, MODE, false>(a.n, a.mixed[w] * 64 + (int)(threadIdx.x & 63), a.b, a.sc,
+                                       klds[threadIdx.x >> 6]);
+    }
+    if (!grid_sync(bar, G)) break;
+    {
+      const PersistArgs& a = phase_args(pa);
+      const StepBufs& b = a.b;
+      const int ovf = __hip_atomic_load(&b.ctl[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const int broken = __hip_atomic_load(&b.ctl[4], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (ovf != 0 && broken == 0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&b.ctl[2], 1);
+        const int nbl = (a.n + BLK - 1) / BLK;
+        for_vblocks(nbl, [&](int v) {
+          const int p = v * BLK + threadIdx.x;
+          if (p < a.n) (void)pair_direct<true, KT, MODE>(p, b, a.sc);
+        });
+      }
+    }
+    if (!grid_sync(bar, G)) break;
+  }
+}
+
 // ================================================================== host
 
 namespace {
@@ -2126,6 +2332,8 @@ struct uph_handle {
   int diag_cap = 0;
   double* vort = nullptr;   // uph_vorticity's output, n_fluid of the largest scene asked for
   int vort_cap = 0;
+  unsigned* bar = nullptr;  // grid barrier of uph_step_persistent: arrivals, generation, broken
+  struct PersistArgs* pargs = nullptr;   // its arguments, in device memory
   // boundary rows' deck and kinematics (host copies, by global index), from which the
   // state getters place them at the last step's time
   std::vector<double2> deck_pos, deck_vel;
@@ -2581,6 +2789,8 @@ extern "C" void uph_destroy(uph_t* h) {
   if (h->ev1) (void)hipEventDestroy(h->ev1);
   if (h->diag) (void)hipFree(h->diag);
   if (h->vort) (void)hipFree(h->vort);
+  if (h->bar) (void)hipFree(h->bar);
+  if (h->pargs) (void)hipFree(h->pargs);
   free_bufs(h->b);
   if (h->dsc) (void)hipFree(h->dsc);
   if (h->mixedWaves) (void)hipFree(h->mixedWaves);
@@ -2617,9 +2827,89 @@ extern "C" int uph_step(uph_t* h, int n_steps) {
   return 0;
 }
 
+template <int KT, int MODE, bool ONE>
+static int persist_launch(uph_t* h, int n_steps) {
+  const void* fn = (const void*)k_persist<KT, MODE, ONE>;
+  int dev = 0, ncu = 0, bpc = 0;
+  HIPCHK(hipGetDevice(&dev), h->err);
+  HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev), h->err);
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, fn, BLK, 0), h->err);
+  int grid = (bpc * ncu) & ~7;
+  if (grid < 8) {
+    h->err.msg = "uph_step_persistent: the kernel does not fit 8 co-resident blocks";
+    return -1;
+  }
+  // No more blocks than the widest phase has virtual blocks: a barrier's cost grows with
+  // the blocks that arrive at it, and a small scene needs only a few dozen.
+  const int widest = std::max((h->n + BLK - 1) / BLK, h->cs.nb);
+  grid = std::min(grid, std::max(8, (widest + 7) & ~7));
+  if (!h->pargs) HIPCHK(hipMalloc((void**)&h->pargs, sizeof(PersistArgs)), h->err);
+  PersistArgs a;
+  a.n = h->n;
+  a.ncells = h->ncells;
+  a.nb = h->cs.nb;
+  a.nmixed = h->nmixed;
+  a.n_steps = n_steps;
+  a.b = h->b;
+  a.sc = h->dsc;
+  a.mixed = h->mixedWaves;
+  a.bar = h->bar;
+  HIPCHK(hipMemcpyAsync(h->pargs, &a, sizeof a, hipMemcpyHostToDevice, h->stream), h->err);
+  const PersistArgs* dp = h->pargs;
+  void* args[] = {(void*)&dp};
+  HIPCHK(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(BLK), args, 0, h->stream), h->err);
+  // the host copy must outlive the asynchronous upload
+  HIPCHK(hipStreamSynchronize(h->stream), h->err);
+  return 0;
+}
+
+extern "C" int uph_step_persistent(uph_t* h, int n_steps) {
+  if (!h) return -1;
+  if (n_steps < 0) {
+    h->err.msg = "uph_step_persistent: negative step count";
+    return -1;
+  }
+  if (h->b.gtot > 0 || h->capped || h->opts.pair_variant != 0) {
+    h->err.msg = "uph_step_persistent: for handles without mirror rows, not expected to be capped, with the "
+                 "compacted pair kernel (use uph_step)";
+    return -1;
+  }
+  if (n_steps == 0) return 0;
+  if (!h->bar) {
+    HIPCHK(hipMalloc((void**)&h->bar, BAR_WORDS * sizeof(unsigned)), h->err);
+    HIPCHK(hipMemset(h->bar, 0, BAR_WORDS * sizeof(unsigned)), h->err);
+  }
+  int rc;
+  const bool one = h->nscenes == 1;
+  if (h->mode == UPH_MODE_REF)
+    rc = h->kernel == UPH_KERNEL_POLY7
+             ? (one ? persist_launch<UPH_KERNEL_POLY7, UPH_MODE_REF, true>(h, n_steps)
+                    : persist_launch<UPH_KERNEL_POLY7, UPH_MODE_REF, false>(h, n_steps))
+             : (one ? persist_launch<UPH_KERNEL_CUBIC, UPH_MODE_REF, true>(h, n_steps)
+                    : persist_launch<UPH_KERNEL_CUBIC, UPH_MODE_REF, false>(h, n_steps));
+  else
+    rc = h->kernel == UPH_KERNEL_POLY7
+             ? (one ? persist_launch<UPH_KERNEL_POLY7, UPH_MODE_CONFIG, true>(h, n_steps)
+                    : persist_launch<UPH_KERNEL_POLY7, UPH_MODE_CONFIG, false>(h, n_steps))
+             : (one ? persist_launch<UPH_KERNEL_CUBIC, UPH_MODE_CONFIG, true>(h, n_steps)
+                    : persist_launch<UPH_KERNEL_CUBIC, UPH_MODE_CONFIG, false>(h, n_steps));
+  if (rc) return -1;
+  h->itime += n_steps;
+  return 0;
+}
+
 extern "C" int uph_sync(uph_t* h) {
   if (!h) return -1;
   HIPCHK(hipStreamSynchronize(h->stream), h->err);
+  if (h->bar) {
+    unsigned broken = 0;
+    HIPCHK(hipMemcpy(&broken, h->bar + BAR_BROKEN, sizeof broken, hipMemcpyDeviceToHost), h->err);
+    if (broken) {
+      h->err.msg = "uph_sync: a grid barrier of uph_step_persistent gave up waiting; the steps' results are not "
+                   "valid";
+      return -1;
+    }
+  }
   int ctl[8];
   HIPCHK(hipMemcpy(ctl, h->b.ctl, sizeof ctl, hipMemcpyDeviceToHost), h->err);
   if (ctl[4]) {

@@ -63,6 +63,7 @@ SIGNATURES = {
                              C.POINTER(C.c_void_p)]),
     "uph_destroy": (None, [C.c_void_p]),
     "uph_step": (C.c_int, [C.c_void_p, C.c_int]),
+    "uph_step_persistent": (C.c_int, [C.c_void_p, C.c_int]),
     "uph_sync": (C.c_int, [C.c_void_p]),
     "uph_get_state": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _dp]),
     "uph_get_rates": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_float),
@@ -276,6 +277,10 @@ class GpuSolver:
 
     def step(self, n: int = 1):
         self._check(lib().uph_step(self.h, int(n)))
+
+    def step_persistent(self, n: int = 1):
+        """n steps in one persistent cooperative launch (uph_step_persistent)."""
+        self._check(lib().uph_step_persistent(self.h, int(n)))
 
     def sync(self) -> int:
         """Waits; returns how many steps took the capped (more than K neighbours) path."""
