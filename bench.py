@@ -120,7 +120,7 @@ def compare_launch(steps: int, warmup: int):
     from uph_amd import scenes
     from uph_amd.solver import GpuSolver
     work = {"dam break 1M (999,698 fluid)": lambda: [scenes.dam_break(dx=2 ** 0.5 * 1e-3)],
-            "512 dam breaks (config 4)": scenes.batch_dam_breaks,
+            "512 dam breaks (5.47 M fluid)": scenes.batch_dam_breaks,
             "dam break 5K": lambda: [scenes.dam_break()]}
     out = {}
     for name, make in work.items():

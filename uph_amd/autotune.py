@@ -18,6 +18,7 @@ import json
 import os
 from typing import Callable, Dict, List, Optional, Sequence
 
+from . import scenes as _scenes
 from .scenes import Scene
 
 DEFAULT_SPACE = {
@@ -102,3 +103,52 @@ def autotune(scenes: Sequence[Scene] | Scene, warmup: int = 3, steps: int = 10,
         with open(cache, "w") as f:
             json.dump(db, f, indent=1)
     return res
+
+
+WORKLOADS = {
+    "dam1m": lambda: _scenes.dam_break(dx=2 ** 0.5 * 1e-3),          # 999,698 fluid
+    "batch512": lambda: _scenes.batch_dam_breaks(),                   # 512 scenes, 5.47 M fluid
+    "dam5k": lambda: _scenes.dam_break(),
+}
+
+
+def main(argv=None):
+    """Tune one named workload, then time the pick against the library's defaults over
+    the same steps (one launch per kernel, device-synchronised wall clock)."""
+    import argparse
+    import sys
+    import time
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="batch512")
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--cache", default=None)
+    a = ap.parse_args(argv)
+    from .solver import GpuSolver
+    scs = WORKLOADS[a.workload]()
+    scs = [scs] if isinstance(scs, Scene) else scs
+    nf = sum(s.n_fluid for s in scs)
+
+    def say(m):
+        print(m, file=sys.stderr, flush=True)
+
+    say(f"{a.workload}: {len(scs)} scene(s), {nf} fluid")
+    res = autotune(scs, warmup=a.warmup, steps=a.steps, cache=a.cache, log=say)
+    out = {"workload": a.workload, "n_fluid": nf, "best": res["best"], "tuned_ms": res["ms_per_step"]}
+    for name, cfg in (("default", {}), ("best", res["best"])):
+        g = GpuSolver(scs, **cfg)
+        g.step(a.warmup)
+        g.sync()
+        t0 = time.perf_counter()
+        g.step(100)
+        g.sync()
+        ms = (time.perf_counter() - t0) * 10.0
+        g.close()
+        out[name + "_ms_per_step"] = ms
+        out[name + "_updates_per_s"] = nf * 1e3 / ms
+        say(f"{name} {cfg}: {ms:.4f} ms per step, {nf * 1e3 / ms:.3e} updates/s")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
