@@ -1,7 +1,7 @@
 """FP64 numpy statement of the diagnostics sph.cc's output reports.
 
 TEST INFRASTRUCTURE ONLY: the checker for uph_diagnostics (the device
-reductions in uph_amd/csrc/uph_gpu.hip).  Pinned by closed forms in
+reductions in uph_amd/csrc/uph_diag.hip).  Pinned by closed forms in
 tests/test_diag.py (uniform fields, the probe weights' normalisation, brute-force
 neighbour counts).
 

@@ -6,8 +6,9 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRCS = [os.path.join(HERE, "csrc", "uph_gpu.hip"), os.path.join(HERE, "csrc", "uph_gnn.hip"),
-        os.path.join(HERE, "csrc", "uph_surrogate.hip")]
+SRCS = [os.path.join(HERE, "csrc", f) for f in
+        ("uph_step.hip", "uph_search.hip", "uph_diag.hip", "uph_graph.hip", "uph_compat.hip", "uph_gnn.hip",
+         "uph_surrogate.hip")]
 OUT = os.path.join(HERE, "libuph_gpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -23,8 +24,26 @@ DRIVER = os.path.join(HERE, "uph_run")
 
 
 def build(verbose: bool = False) -> str:
+    """Compile the translation units side by side (objects under uph_amd/_build/), then link."""
+    from concurrent.futures import ThreadPoolExecutor
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    inc = ["-I", os.path.join(HERE, "..", "include")]
+    cflags = [f for f in FLAGS if f != "-shared"]
+
+    def compile_one(src: str) -> str:
+        obj = os.path.join(objdir, os.path.splitext(os.path.basename(src))[0] + ".o")
+        cmd = [HIPCC, *cflags, *inc, "-c", "-o", obj, src]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        return obj
+
+    jobs = max(1, min(len(SRCS), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with ThreadPoolExecutor(jobs) as ex:
+        objs = list(ex.map(compile_one, SRCS))
     tmp = OUT + ".tmp"
-    cmd = [HIPCC, *FLAGS, "-I", os.path.join(HERE, "..", "include"), "-o", tmp, *SRCS]
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
