@@ -1449,10 +1449,10 @@ __device__ __forceinline__ void correct(const DevScene& S, const StepBufs& b, in
 // EXACT applies the K-nearest cap (run only when some query overflowed).
 // Returns true for a query with more than K candidates in range when !EXACT
 // (nothing written; its K-th key is left to wave_overflow).
-template <bool EXACT, int KT, int MODE>
+template <bool EXACT, int KT, int MODE, bool ONE = false>
 __device__ __forceinline__ bool pair_direct(int p, const StepBufs& b, const DevScene* __restrict__ sc) {
   const uint32_t tg = b.tagA[p];
-  const DevScene& S = sc[b.sceneOf[p]];
+  const DevScene& S = sc[ONE ? 0 : b.sceneOf[p]];
   const int itime = b.ctl[0];
   const double2 xp = b.posA[p];
   const double2 vn = b.velA[p];
@@ -1569,10 +1569,10 @@ __device__ __forceinline__ void compact_pairs(const DevScene& S, const StepBufs&
 // that the K cap keeps; pass 2 walks them row by row, in slot order as the direct
 // loop sums them.
 // Returns false, leaving p to the direct loop, when a row is longer than 128 slots.
-template <int KT, int MODE>
+template <int KT, int MODE, bool ONE = false>
 __device__ __forceinline__ bool exact_pairs(int p, const StepBufs& b, const DevScene* __restrict__ sc) {
   const uint32_t tg = b.tagA[p];
-  const DevScene& S = sc[b.sceneOf[p]];
+  const DevScene& S = sc[ONE ? 0 : b.sceneOf[p]];
   const double2 xp = b.posA[p];
   int rb[3], rl[3];
   candidate_rows(S, xp, b.cellStart, rb, rl);
