@@ -112,11 +112,13 @@ def copy_probe(mib):
 
 
 def compare_launch(steps: int, warmup: int):
-    """ms per step of three ways of launching the same steps: one launch per kernel
+    """ms per step of the ways of launching the same steps: one launch per kernel
     (uph_step), the step captured as a hipGraph and replayed, and all steps in one
-    persistent cooperative kernel (uph_step_persistent); on the 1M dam break, the
-    512-scene batch of dam breaks (5.47 M fluid) and a 5,000-particle dam break.  Host
-    wall clock around the steps, bracketed by stream synchronisation."""
+    persistent cooperative kernel (uph_step_persistent), each re-sorting every step; and
+    the lazy re-sort (the default: the pair kernel and one cooperative launch per step);
+    on the 1M dam break, the 512-scene batch of dam breaks (5.47 M fluid) and a
+    5,000-particle dam break.  Host wall clock around the steps, bracketed by stream
+    synchronisation."""
     from uph_amd import scenes
     from uph_amd.solver import GpuSolver
     work = {"dam break 1M (999,698 fluid)": lambda: [scenes.dam_break(dx=2 ** 0.5 * 1e-3)],
@@ -127,8 +129,8 @@ def compare_launch(steps: int, warmup: int):
         scs = make()
         nf = sum(s.n_fluid for s in scs)
         res = {}
-        for mode in ("eager", "graph", "persistent"):
-            g = GpuSolver(scs, use_graph=mode == "graph")
+        for mode in ("eager", "graph", "persistent", "lazy"):
+            g = GpuSolver(scs, use_graph=mode == "graph", sort_every_step=mode != "lazy")
             run = g.step_persistent if mode == "persistent" else g.step
             run(warmup)
             g.sync()
@@ -168,6 +170,12 @@ def main():
     ap.add_argument("--vtype", type=int, default=0,
                     help="virtualType of the tank (mirror particles of derivatives.c:3193-4192; 0: none; "
                          "8: the dam break's left, bottom, corner and right images)")
+    ap.add_argument("--sort-every-step", action="store_true",
+                    help="re-sort the particles into their cells every step (default: the lazy re-sort, "
+                         "which re-sorts a scene only when a particle has left its cell by more than the "
+                         "cells' growth)")
+    ap.add_argument("--skin", type=float, default=0.0,
+                    help="lazy re-sort: cells grow by this fraction (0: the library's default, 0.05)")
     ap.add_argument("--compare-launch", action="store_true",
                     help="only compare per-kernel launches, hipGraph replay and the persistent kernel "
                          "(compare_launch) and exit")
@@ -200,14 +208,17 @@ def main():
         sc.vtype, sc.flat_bottom, sc.depth = args.vtype, 4.0, 2.0   # the tank's width, the column's height
     log(f"workload: {sc.n_fluid} fluid + {sc.n_boundary} wall particles, {args.kernel} kernel, "
         f"h = {args.h_over_dx} dx, {args.scheme} scheme, virtualType {args.vtype}")
-    g = GpuSolver(sc)
+    g = GpuSolver(sc, sort_every_step=args.sort_every_step, skin=args.skin)
     g.step(args.settle + args.warmup)
     g.sync()
+    lazy = g.resorts() >= 0
+    r0 = g.resorts()
     # the timed region: K steps with no event between the kernels
     t0 = time.perf_counter()
     ms_total = g.time_loop(args.steps)
     overflow = g.sync()
     wall = time.perf_counter() - t0
+    resorts_timed = g.resorts() - r0 if lazy else args.steps
     # then K more steps with events between the phases, for the per-kernel split
     ph = g.time_phases(args.steps)
     g.sync()
@@ -218,8 +229,9 @@ def main():
     # step level: the same bytes at the measured step rate
     achieved = BYTES_PER_UPDATE * sc.n_fluid / (ms_pair * 1e-3) / 1e9
     step_achieved = BYTES_PER_UPDATE * value / 1e9
-    log(f"timed {args.steps} steps: {ms_total:.3f} ms (host wall {wall*1e3:.1f} ms); per step: "
-        f"sort {ph['sort']*1e3:.1f} us, pair {ms_pair*1e3:.1f} us, capped pass {ph['exact']*1e3:.1f} us")
+    post = "k_post (capped pass and re-sort when needed)" if lazy else "capped pass"
+    log(f"timed {args.steps} steps: {ms_total:.3f} ms (host wall {wall*1e3:.1f} ms), {resorts_timed} re-sorts; "
+        f"per step: sort {ph['sort']*1e3:.1f} us, pair {ms_pair*1e3:.1f} us, {post} {ph['exact']*1e3:.1f} us")
     out = {
         "metric": "particle-updates/s, 1M-particle dam break (plain path)",
         "value": value,
@@ -237,7 +249,10 @@ def main():
                    "fluid_particles": sc.n_fluid, "wall_particles": sc.n_boundary,
                    "kernel": args.kernel, "scheme": args.scheme, "h_over_dx": args.h_over_dx, "dt": sc.dt,
                    "virtual_type": args.vtype, "capped_steps": overflow,
-                   "settle_steps": args.settle},
+                   "settle_steps": args.settle,
+                   "resort": (f"lazy, cells grown by {args.skin if args.skin > 0 else 0.05:g}" if lazy
+                              else "every step"),
+                   "resorts_in_timed_steps": resorts_timed},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "frac_basis": "kernel level: 80 B x fluid particles per launch / mean duration of "
@@ -251,7 +266,8 @@ def main():
                      "kernel": f"k_pair_compact<256, {int(sc.kernel)}, {int(sc.mode)}, true, "
                                f"{'true' if 4 * math.pi * args.h_over_dx ** 2 > 0.75 * sc.K else 'false'}> "
                                f"({args.kernel}, {args.scheme}, one scene)",
-                     "kernel_ms": ms_pair, "sort_ms": ph["sort"], "capped_pass_ms": ph["exact"]},
+                     "kernel_ms": ms_pair, "sort_ms": ph["sort"], "capped_pass_ms": ph["exact"],
+                     "capped_pass_kernel": post},
     }
     if args.graph:
         gg = GpuSolver(sc, use_graph=True)

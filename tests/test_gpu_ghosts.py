@@ -176,7 +176,7 @@ def test_batch_with_and_without_mirror_rows_equals_single_scenes():
     batch = GpuSolver([a, b, c])
     batch.step(25)
     for k, sc in enumerate((a, b, c)):
-        one = GpuSolver(sc)
+        one = GpuSolver(sc, sort_every_step=True)   # as the batch, which holds mirror rows
         one.step(25)
         sb, so = batch.state(k), one.state()
         for key in ("x", "v", "rho"):

@@ -1,5 +1,6 @@
 """uph_step_persistent (all phases of n steps in one cooperative launch, grid-wide barriers
-between the phases) against uph_step (one launch per kernel), bit for bit."""
+between the phases) against uph_step (one launch per kernel), bit for bit, on handles
+that re-sort every step (the persistent step's phases)."""
 import numpy as np
 import pytest
 
@@ -24,7 +25,7 @@ def test_persistent_batch_equals_stepped():
            scenes.dam_break(width=0.3, height=0.6, dx=0.015),
            scenes.dam_break(width=0.5, height=0.3, dx=0.025, jitter=0.05, seed=1)]
     _say("batch of 3: persistent vs stepped, 7 + 23 steps")
-    a, b = GpuSolver(scs), GpuSolver(scs)
+    a, b = GpuSolver(scs, sort_every_step=True), GpuSolver(scs, sort_every_step=True)
     for chunk in (7, 23):
         a.step_persistent(chunk)
         b.step(chunk)
@@ -45,7 +46,7 @@ def test_persistent_capped_steps_equal_stepped():
     patch = (x[:, 0] < 0.16) & (x[:, 1] < 0.16)
     sc.pos[:nf][patch] = x[patch] * 0.7
     _say(f"capped patch of {int(patch.sum())} particles: persistent vs stepped, 12 steps")
-    a, b = GpuSolver(sc), GpuSolver(sc)
+    a, b = GpuSolver(sc, sort_every_step=True), GpuSolver(sc, sort_every_step=True)
     a.step_persistent(12)
     b.step(12)
     ca, cb = a.sync(), b.sync()
@@ -63,4 +64,7 @@ def test_persistent_rejects_unsupported_handles():
     with pytest.raises(RuntimeError, match="persistent"):
         GpuSolver(mirrored).step_persistent(1)
     with pytest.raises(RuntimeError, match="persistent"):
-        GpuSolver(scenes.dam_break(width=0.3, height=0.3, dx=0.02), pair_variant=1).step_persistent(1)
+        GpuSolver(scenes.dam_break(width=0.3, height=0.3, dx=0.02), pair_variant=1,
+                  sort_every_step=True).step_persistent(1)
+    with pytest.raises(RuntimeError, match="persistent"):   # a lazy re-sort handle
+        GpuSolver(scenes.dam_break(width=0.3, height=0.3, dx=0.02)).step_persistent(1)

@@ -74,7 +74,7 @@ def test_batch_of_512_dam_breaks():
     assert gb.sync() == 0
     rng = np.random.default_rng(0)
     for k in rng.choice(len(scs), 4, replace=False):
-        gs = GpuSolver(scs[k])
+        gs = GpuSolver(scs[k], sort_every_step=True)   # the captured step sorts every step
         gs.step(20)
         a, b = gb.state(int(k)), gs.state(0)
         for key in ("x", "v", "rho"):
@@ -96,6 +96,38 @@ def test_batch_of_512_dam_breaks():
         st = gb.state(k)
         n = scs[k].n_fluid
         assert np.all(np.isfinite(st["x"])) and 950 < st["rho"][:n].min() <= st["rho"][:n].max() < 1100
+
+
+def test_batch_of_512_dam_breaks_lazy_resort():
+    """The same batch with the lazy re-sort (eager launches): 300 steps, a few scenes
+    re-sorted by then; sampled scenes equal their own single-scene lazy runs bit for bit
+    (each scene's re-sorts depend on its own particles only), the smallest matches the
+    oracle."""
+    from oracle.oracle import Sim
+    from uph_amd.solver import GpuSolver
+    scs = scenes.batch_dam_breaks()
+    _say("lazy batch of 512: 300 steps")
+    gb = GpuSolver(scs)
+    gb.step(300)
+    assert gb.sync() == 0
+    nres = gb.resorts()
+    _say(f"lazy batch: {nres} re-sorts in 300 steps")
+    assert nres >= 1
+    rng = np.random.default_rng(1)
+    for k in [int(np.argmin([s.n_fluid for s in scs]))] + [int(j) for j in rng.choice(len(scs), 3, replace=False)]:
+        gs = GpuSolver(scs[k])
+        gs.step(300)
+        a, b = gb.state(k), gs.state(0)
+        for key in ("x", "v", "rho"):
+            assert np.array_equal(a[key], b[key]), (k, key)
+    k = int(np.argmin([s.n_fluid for s in scs]))
+    o = Sim(scs[k])
+    o.step(300)
+    so, sg = o.state(), gb.state(k)
+    n = scs[k].n_fluid
+    drho = float(np.max(np.abs(sg["rho"][:n] - so["rho"][:n]) / so["rho"][:n]))
+    _say(f"lazy batch: smallest scene ({n} fluid) vs oracle after 300 steps: rho {drho:.2e}")
+    assert drho < 1e-6, drho
 
 
 def test_autotune_on_gpu():

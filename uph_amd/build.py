@@ -7,8 +7,8 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 SRCS = [os.path.join(HERE, "csrc", f) for f in
-        ("uph_step.hip", "uph_search.hip", "uph_diag.hip", "uph_graph.hip", "uph_compat.hip", "uph_gnn.hip",
-         "uph_surrogate.hip")]
+        ("uph_step.hip", "uph_pair.hip", "uph_search.hip", "uph_diag.hip", "uph_graph.hip", "uph_compat.hip",
+         "uph_gnn.hip", "uph_surrogate.hip")]
 OUT = os.path.join(HERE, "libuph_gpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

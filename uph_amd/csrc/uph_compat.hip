@@ -104,8 +104,10 @@ extern "C" int uph_state_to_device(uph_t* h, int scene, double* pos, double* vel
   const DevScene& S = h->hs[scene];
   // the caller's work on pos/vel/rho, on any stream, completes before they are overwritten
   HIPCHK(hipDeviceSynchronize(), h->err);
-  hipLaunchKernelGGL(k_state_out, dim3(grid_of(S.np)), dim3(BLK), 0, h->stream, S.poff, S.np, h->b.tagA, h->b.posB,
-                     h->b.velB, h->b.rhoB, (double2*)pos, (double2*)vel, rho);
+  StepBufs rb;
+  if (read_bufs(h, rb)) return -1;
+  hipLaunchKernelGGL(k_state_out, dim3(grid_of(S.np)), dim3(BLK), 0, h->stream, S.poff, S.np, rb.tagA, rb.posB,
+                     rb.velB, rb.rhoB, (double2*)pos, (double2*)vel, rho);
   HIPCHK(hipGetLastError(), h->err);
   if (S.motion != 0 && S.np > S.nf) {   // boundary rows at the last step's time, as uph_get_state
     hipLaunchKernelGGL(k_walls_out, dim3(grid_of(S.np - S.nf)), dim3(BLK), 0, h->stream, S.poff, S.np, S.nf,
@@ -127,8 +129,11 @@ extern "C" int uph_state_from_device(uph_t* h, int scene, const double* pos, con
   const DevScene& S = h->hs[scene];
   // the caller may have written pos/vel/rho on any stream
   HIPCHK(hipDeviceSynchronize(), h->err);
-  hipLaunchKernelGGL(k_state_in, dim3(grid_of(S.np)), dim3(BLK), 0, h->stream, S.poff, S.np, S.nf, h->b.tagA,
-                     h->b.posB, h->b.velB, h->b.rhoB, (const double2*)pos, (const double2*)vel, rho,
+  // into the B set of the state getters' view; a lazy handle sorts it from scratch next
+  StepBufs rb;
+  if (read_bufs(h, rb) || lazy_reset(h)) return -1;
+  hipLaunchKernelGGL(k_state_in, dim3(grid_of(S.np)), dim3(BLK), 0, h->stream, S.poff, S.np, S.nf, rb.tagA,
+                     rb.posB, rb.velB, rb.rhoB, (const double2*)pos, (const double2*)vel, rho,
                      (const double2*)nullptr, h->b.deckLvel, restart);
   HIPCHK(hipGetLastError(), h->err);
   if (restart) {
@@ -321,6 +326,7 @@ int compat_build(const uph_system* s) {
   o.keep_rates = 1;
   o.pair_block = 256;
   o.cell_factor = 1.0;
+  o.sort_every_step = 1;   // each call restarts the step from the caller's state
   if (uph_create(&u, 1, &o, &c.h) != 0) return compat_fail(std::string("derivatives: ") + uph_last_error(nullptr));
   // staging: pos, vel, lvel (nf double2 each), then rho, P, localDensity, localPressure (nf doubles each)
   if (hipMalloc((void**)&c.dstage, ((size_t)3 * nf + 2 * (size_t)nf + 1) * sizeof(double2)) != hipSuccess) {

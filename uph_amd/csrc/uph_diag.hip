@@ -85,8 +85,10 @@ static int diag_sums(uph_t* h, int scene, double px, double py, double px2, doub
     HIPCHK(hipMalloc((void**)&h->diag, (size_t)nb * DIAG_Q * sizeof(double)), h->err);
     h->diag_cap = nb;
   }
-  hipLaunchKernelGGL(k_diag, dim3(nb), dim3(DIAG_BLK), 0, h->stream, S.poff, S.np, h->b.tagA, h->b.posB,
-                     h->b.velB, h->b.rhoB, h->b.massD, h->mode == UPH_MODE_REF ? h->b.presB : nullptr, h->dsc,
+  StepBufs rb;
+  if (read_bufs(h, rb)) return -1;
+  hipLaunchKernelGGL(k_diag, dim3(nb), dim3(DIAG_BLK), 0, h->stream, S.poff, S.np, rb.tagA, rb.posB,
+                     rb.velB, rb.rhoB, h->b.massD, h->mode == UPH_MODE_REF ? rb.presB : nullptr, h->dsc,
                      scene, px, py, px2, py2,
                      S.nf > 0 ? 1.0 / S.nf : 0.0, h->diag);
   HIPCHK(hipGetLastError(), h->err);
@@ -183,11 +185,13 @@ int vort_run(uph_t* h, int scene, double* out) {
     HIPCHK(hipMalloc((void**)&h->vort, (size_t)S.nf * sizeof(double)), h->err);
     h->vort_cap = S.nf;
   }
+  StepBufs rb;
+  if (read_bufs(h, rb)) return -1;
   if (h->kernel == UPH_KERNEL_POLY7)
-    hipLaunchKernelGGL(k_vort<UPH_KERNEL_POLY7>, dim3(grid_of(S.np)), dim3(BLK), 0, h->stream, S.poff, S.np, h->b,
+    hipLaunchKernelGGL(k_vort<UPH_KERNEL_POLY7>, dim3(grid_of(S.np)), dim3(BLK), 0, h->stream, S.poff, S.np, rb,
                        h->dsc, scene, h->vort);
   else
-    hipLaunchKernelGGL(k_vort<UPH_KERNEL_CUBIC>, dim3(grid_of(S.np)), dim3(BLK), 0, h->stream, S.poff, S.np, h->b,
+    hipLaunchKernelGGL(k_vort<UPH_KERNEL_CUBIC>, dim3(grid_of(S.np)), dim3(BLK), 0, h->stream, S.poff, S.np, rb,
                        h->dsc, scene, h->vort);
   HIPCHK(hipGetLastError(), h->err);
   HIPCHK(hipMemcpyAsync(out, h->vort, (size_t)S.nf * sizeof(double), hipMemcpyDeviceToHost, h->stream), h->err);

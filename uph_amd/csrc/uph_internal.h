@@ -51,24 +51,54 @@
 // into the clamped edge cells, where an exact search would cost O(N) per query.
 #define CAND_MAX 65536
 
+// Control words (StepBufs::ctl).  A word that a kernel reads is never written by
+// another block of the same kernel: the lazy step's pair kernel copies the step
+// counter, the stamp and its view for k_post, which writes the next step's values,
+// and its flags are kept per itime parity (a step sets its own, k_post clears the
+// next step's).
+#define C_IT 0       // itime of the step being computed
+#define C_STAMP 1    // stamp of that step's K-th keys (never reset, so a stamp never repeats)
+#define C_NOVF 2     // steps whose capped pass ran (a query had more than K in range)
+#define C_OVF 3      // overflow flag of the step (sort-every-step handles)
+#define C_BROKEN 4   // broken state (a query with more than CAND_MAX candidates; sticky)
+#define C_ITC 5      // lazy: itime and stamp of the step, copied by the pair kernel for k_post
+#define C_STC 6
+#define C_V 7        // lazy: buffer view of the next pair kernel (1: the A and B sets exchanged)
+#define C_PEND 8     // lazy: a re-sort that the last step of a call held over to the next call
+#define C_VC 9       // lazy: the view the pair kernel ran with, for k_post
+#define C_LOVF 10    // lazy: overflow flag by itime parity (2 words)
+#define C_LSTALE 12  // lazy: some scene's cell list went stale, by itime parity (2 words)
+#define C_NRES 14    // lazy: re-sorts done (a count for reports)
+#define CTL_WORDS 16
+
 // ------------------------------------------------------------------ sort
 
-// One block of the binning: slots blk * BLK + [0, BLK).
+// One block of the binning: slots blk * BLK + [0, BLK).  With `keep` (a lazy
+// re-sort), a slot of a scene that is not stale (keep[scene] == 0) keeps the cell it
+// was binned into (cellOf), so that scene's slot order does not change.  tagB (when
+// given) receives a copy of tagA for the reorder.
 __device__ __forceinline__ void bin_block(int blk, int n, const double2* __restrict__ pos,
                                           const int* __restrict__ sceneOf, const DevScene* __restrict__ sc,
                                           const uint32_t* __restrict__ tagA, uint32_t* __restrict__ tagB,
-                                          int* __restrict__ keys, int* __restrict__ rank, int* __restrict__ count) {
+                                          int* __restrict__ keys, int* __restrict__ rank, int* __restrict__ count,
+                                          const int* __restrict__ keep = nullptr,
+                                          const int* __restrict__ cellOf = nullptr) {
   const int i = blk * BLK + threadIdx.x;
   const bool valid = i < n;
   int key = -1;
   if (valid) {
-    const DevScene& S = sc[sceneOf[i]];
-    const double2 p = pos[i];
-    int cx, cy;
-    cell_of(S, p.x, p.y, cx, cy);
-    key = S.coff + cy * S.ncx + cx;
+    const int s = sceneOf[i];
+    if (keep && keep[s] == 0) {
+      key = cellOf[i];
+    } else {
+      const DevScene& S = sc[s];
+      const double2 p = pos[i];
+      int cx, cy;
+      cell_of(S, p.x, p.y, cx, cy);
+      key = S.coff + cy * S.ncx + cx;
+    }
     keys[i] = key;
-    tagB[i] = tagA[i];
+    if (tagB) tagB[i] = tagA[i];
   }
   // Slots arrive sorted by the previous step's cells, so equal keys come in runs
   // of consecutive lanes: one atomic per run reserves the run's ranks.
@@ -124,13 +154,22 @@ __device__ int block_scan_tile(int* vals /* SCAN_ITEMS per thread, in registers 
   return total;
 }
 
-// Per-tile totals of `in`.  With `step_ctl` it also opens a step: itime += 1,
-// the overflow flag cleared, the K-th key stamp advanced.
 // Opens a step: itime += 1, the overflow flag cleared, the K-th key stamp advanced.
-__device__ __forceinline__ void open_step(int* step_ctl) {
-  step_ctl[0] += 1;   // itime of the step being computed
-  step_ctl[1] = 0;    // overflow flag of this step
-  step_ctl[3] += 1;   // stamp of this step's K-th keys
+// A lazy handle's first step after its state was set (sstale given) also clears that
+// step's parity flags and takes the buffer view its sort gathers into.
+__device__ __forceinline__ void open_step(int* ctl, int* sstale = nullptr, int nscenes = 0, int view = 0) {
+  const int it = ctl[C_IT] + 1;
+  ctl[C_IT] = it;            // itime of the step being computed
+  ctl[C_OVF] = 0;            // overflow flag of this step
+  ctl[C_STAMP] += 1;         // stamp of this step's K-th keys
+  if (sstale) {
+    ctl[C_NRES] += 1;
+    ctl[C_LOVF + (it & 1)] = 0;
+    ctl[C_LSTALE + (it & 1)] = 0;
+    ctl[C_V] = view;
+    ctl[C_PEND] = 0;
+    for (int s = 0; s < nscenes; ++s) sstale[(it & 1) * nscenes + s] = 0;
+  }
 }
 
 // The total of tile `tile` into partial[tile] (lds: SCAN_BLK / 64 + 1 ints).
@@ -230,6 +269,23 @@ struct StepBufs {
   double* rhoB;
   float4* rtB;
   uint32_t* tagB;
+  // Lazy re-sort handles (lazy = 1; 2 in the reference scheme, whose re-sort also
+  // gathers DynamicPressure and mC) keep two full state sets and exchange their roles
+  // between steps: the pair kernel's epilogue also writes the next step's predicted
+  // neighbour mirrors (mA2, mB2, mC2 on the B side), the tag and the rates, so that a
+  // step without a re-sort reads the B set as its A set.  rtA, rt2A, dvA are the A
+  // side's rates (written when the roles are exchanged).  cellOf: the cell (global
+  // key) each slot was binned into at the last sort.  sstale: per scene and itime
+  // parity, whether one of its particles left its binned cell by more than the margin.
+  // `it` points at {itime, stamp} of the step (the pair of words C_IT, C_STAMP, or their
+  // copies C_ITC, C_STC inside k_post).
+  float4 *mA2, *mB2, *rtA, *rt2A;
+  float2 *mC2, *dvA;
+  int* cellOf;
+  int* sstale;
+  int* it;
+  int lazy, nscenes;
+  int view;   // 1 in the view with the A and B sets exchanged (swapped)
   // sort scratch
   int *keys, *rank, *tmpIdx, *cellStart, *cellCount, *partial;
   int* sceneOf;
@@ -255,8 +311,7 @@ struct StepBufs {
   double *deckRl, *deckPl;   // localDensity, localPressure of the first step, per original index
   float4* rt2B;
   float2 *dvB, *mC;
-  // control: [0] itime, [1] overflow flag of the step, [2] overflowed steps, [3] step stamp,
-  // [4] broken state (a query with more than CAND_MAX candidates; sticky)
+  // control words, C_* (CTL_WORDS)
   int* ctl;
   float4* dbgRates;   // keep_rates: per slot {R, R~, a~x, a~y} (FP32), {ax, ay} (FP64);
                       // UPH_MODE_REF adds {dpdt, dpdt_approx, drho, dP}, {dvx, dvy, 0, 0}
@@ -282,6 +337,66 @@ struct StepBufs {
   double2 *predV, *predVl;
   double *predRho, *predRl, *predP, *predPl;
 };
+
+// The buffers with the roles of the A and B state sets exchanged (a lazy handle's
+// view 1).
+__host__ __device__ __forceinline__ StepBufs swapped(const StepBufs& b) {
+  StepBufs s = b;
+  s.posA = b.posB;
+  s.posB = b.posA;
+  s.velA = b.velB;
+  s.velB = b.velA;
+  s.rhoA = b.rhoB;
+  s.rhoB = b.rhoA;
+  s.tagA = b.tagB;
+  s.tagB = b.tagA;
+  s.mA = b.mA2;
+  s.mA2 = b.mA;
+  s.mB = b.mB2;
+  s.mB2 = b.mB;
+  s.mC = b.mC2;
+  s.mC2 = b.mC;
+  s.presA = b.presB;
+  s.presB = b.presA;
+  s.rtB = b.rtA;
+  s.rtA = b.rtB;
+  s.rt2B = b.rt2A;
+  s.rt2A = b.rt2B;
+  s.dvB = b.dvA;
+  s.dvA = b.dvB;
+  s.view = b.view ^ 1;
+  return s;
+}
+
+// The buffers of a lazy handle's step as the pair kernel and the sort see them: the
+// view k_post chose (C_V), exchanged once more when the last call held a re-sort over
+// (C_PEND: the gated sort of this call's first step then gathers the state back).
+__device__ __forceinline__ StepBufs lazy_view(const StepBufs& b) {
+  if (!b.lazy) return b;
+  const int v = (b.ctl[C_V] ^ b.ctl[C_PEND]) & 1;
+  return v ? swapped(b) : b;
+}
+
+// A lazy step's pair kernel: the buffers of the step's view (the A and B sets exchange
+// roles from step to step unless a re-sort gathered the state back), and block 0
+// leaves the step's itime, stamp and view for the kernels after it (which read only
+// these copies, so the last of them may write the next step's values while its other
+// blocks still run).
+__device__ __forceinline__ StepBufs pair_view(const StepBufs& b) {
+  int* ctl = b.ctl;
+  const int v = (ctl[C_V] ^ ctl[C_PEND]) & 1;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ctl[C_ITC] = ctl[C_IT];
+    ctl[C_STC] = ctl[C_STAMP];
+    ctl[C_VC] = v;
+  }
+  return v ? swapped(b) : b;
+}
+
+// The overflow flag of the step that b.it describes.
+__device__ __forceinline__ int* ovf_word(const StepBufs& b) {
+  return b.lazy ? &b.ctl[C_LOVF + (b.it[0] & 1)] : &b.ctl[C_OVF];
+}
 
 // The section tests of fluid row `id` (global) at position x in this step
 // (derivatives.c:3193-4192): each passing test sets the row's bit in its section's
@@ -313,15 +428,127 @@ __device__ __forceinline__ bool mark_mirrors(const DevScene& S, const StepBufs& 
   return any;
 }
 
+// The predictor of sph.cc:93-126 for one slot at the start of step itime, from
+// its state after the last step (position p, velocity v, density rho; UPH_MODE_REF:
+// DynamicPressure Pst) and the last evaluation's predicting rates rt = {R~, a~, mass}
+// (UPH_MODE_REF also r2 = {dpdt_approx, drho, dP} and the velocity fluctuation dv):
+// rho += dt/2 R~, v += dt/2 a~, vloc = v + dt/2 a~; Tait pressure of the predicted
+// density.  UPH_MODE_REF also predicts DynamicPressure (P += dt/2 dpdt_approx) and
+// adds the last step's fluctuations to the local fields: localDensity = rho + drho,
+// localPressure = P + dP, localVelocity = v + dt/2 a~ + eps dv; at the first step they
+// are deckRl, deckPl (rho0 and 0 from uph_create, input.c:109-110) and the deck's
+// localVelocity.  Boundary slots keep the deck's localVelocity (the kinematics' one
+// for moving walls), DynamicPressure from the deck, localDensity rho0 and
+// localPressure 0 (inputbp.c:117-150, 289-290).  The sort and a lazy handle's pair
+// epilogue both call this, so their mirrors agree bit for bit.
+struct Pred {
+  double vx, vy, lx, ly;   // predicted velocity, localVelocity
+  double rp;               // predicted density
+  double P, Pp, rl, pl;    // P: Tait pressure of rp (config) or DynamicPressure at the step's start (ref);
+                           // Pp: the predicted pressure; rl, pl: localDensity, localPressure
+};
+
+__device__ __forceinline__ Pred predict(const DevScene& S, const StepBufs& b, uint32_t tg, int itime, double2 p,
+                                        double2 v, double rho, float4 rt, double Pst, float4 r2, float2 dv) {
+  const uint32_t my = tag_id(tg);
+  const double dt = S.dt;
+  Pred o;
+  o.vx = v.x;
+  o.vy = v.y;
+  o.rp = rho;
+  if (tag_type(tg) == 1) {
+    if (itime != 1) {
+      o.rp = rho + 0.5 * dt * (double)rt.x;
+      o.vx = v.x + 0.5 * dt * (double)rt.y;
+      o.vy = v.y + 0.5 * dt * (double)rt.z;
+      o.lx = o.vx + 0.5 * dt * (double)rt.y;
+      o.ly = o.vy + 0.5 * dt * (double)rt.z;
+    } else {
+      const double2 l0 = b.deckLvel[my];
+      o.lx = l0.x;
+      o.ly = l0.y;
+    }
+  } else {
+    const double2 l0 = b.deckLvel[my];
+    o.lx = l0.x;
+    o.ly = l0.y;
+    if (S.motion != 0) {
+      double2 xx = p, vv = v;
+      double lvx;
+      if (wall_kin(S, b.wallKind[my], b.wallRef[my], S.t0 + itime * dt, b.deckPos[my], xx, vv, &lvx)) o.lx = lvx;
+    }
+  }
+  if (S.mode == 1) {
+    o.P = tag_type(tg) == 1 ? Pst : b.deckP[my];
+    o.Pp = o.P;
+    o.rl = S.rho0;
+    o.pl = 0.0;
+    if (tag_type(tg) == 1 && itime == 1) {
+      o.rl = b.deckRl[my];
+      o.pl = b.deckPl[my];
+    } else if (tag_type(tg) == 1) {
+      o.Pp = o.P + 0.5 * dt * (double)r2.x;
+      o.rl = o.rp + (double)r2.y;
+      o.pl = o.Pp + (double)r2.z;
+      o.lx += S.eps * (double)dv.x;
+      o.ly += S.eps * (double)dv.y;
+    }
+  } else {
+    o.P = tait(S, o.rp);
+    o.Pp = o.P;
+    o.rl = o.rp;   // config mode keeps localDensity equal to density
+    o.pl = 0.0;
+  }
+  return o;
+}
+
+// predict() of a fluid slot at a step after the first (the lazy pair epilogue's next
+// step): the same expressions, without the first step's deck fields.
+__device__ __forceinline__ Pred predict_fluid_next(const DevScene& S, double2 v, double rho, float4 rt, double Pst,
+                                                   float4 r2, float2 dv) {
+  const double dt = S.dt;
+  Pred o;
+  o.rp = rho + 0.5 * dt * (double)rt.x;
+  o.vx = v.x + 0.5 * dt * (double)rt.y;
+  o.vy = v.y + 0.5 * dt * (double)rt.z;
+  o.lx = o.vx + 0.5 * dt * (double)rt.y;
+  o.ly = o.vy + 0.5 * dt * (double)rt.z;
+  if (S.mode == 1) {
+    o.P = Pst;
+    o.Pp = o.P + 0.5 * dt * (double)r2.x;
+    o.rl = o.rp + (double)r2.y;
+    o.pl = o.Pp + (double)r2.z;
+    o.lx += S.eps * (double)dv.x;
+    o.ly += S.eps * (double)dv.y;
+  } else {
+    o.P = tait(S, o.rp);
+    o.Pp = o.P;
+    o.rl = o.rp;
+    o.pl = 0.0;
+  }
+  return o;
+}
+
+// The FP32 neighbour mirrors of a predicted slot: {v, vloc} and {rho, P/rho^2, m, tag}
+// (config) or {rho - rho0, P, m, tag} and {localDensity - rho0, localPressure} (ref).
+// In ref mode densities enter the mirror as deviations from rho0 (formed in FP64), so
+// the differences the fluctuations and diffusion terms take keep FP32's relative
+// precision instead of losing the leading digits of rho0.
+__device__ __forceinline__ void mirror_of(const DevScene& S, const Pred& o, float mass, uint32_t tg, float4& ma,
+                                          float4& mb, float2& mc) {
+  ma = make_float4((float)o.vx, (float)o.vy, (float)o.lx, (float)o.ly);
+  if (S.mode == 1) {
+    mb = make_float4((float)(o.rp - S.rho0), (float)o.Pp, mass, __uint_as_float(tg));
+    mc = make_float2((float)(o.rl - S.rho0), (float)o.pl);
+  } else {
+    mb = make_float4((float)o.rp, (float)(o.P / (o.rp * o.rp)), mass, __uint_as_float(tg));
+    mc = make_float2(0.f, 0.f);
+  }
+}
+
 // Final slot = cell start + number of same-cell particles with a smaller
 // original index (deterministic order); gather the state and build the
-// predictor mirror (sph.cc:93-126: rho += dt/2 Rt, v += dt/2 at,
-// vloc = v + dt/2 at; Tait pressure of the predicted density).  UPH_MODE_REF
-// also predicts DynamicPressure (P += dt/2 dpdt_approx) and adds the last step's
-// fluctuations to the local fields: localDensity = rho + drho, localPressure =
-// P + dP, localVelocity = v + dt/2 at + eps dv; at the first step they are
-// deckRl, deckPl (rho0 and 0 from uph_create, input.c:109-110) and the deck's
-// localVelocity.
+// predictor mirror of the step (predict, mirror_of); the slot's cell goes to cellOf.
 // MIRRORS: some scene has mirror rows (the section tests run here); without them the
 // kernel is built without that code and keeps its smaller register allocation.
 template <bool MIRRORS>
@@ -339,89 +566,72 @@ __device__ __forceinline__ void reorder_slot(int d, int n, const StepBufs& b, co
     for (int t = c0; t < c1; ++t) r += (tag_id(b.tagB[b.tmpIdx[t]]) < my) ? 1 : 0;
     dst = c0 + r;
   } else if (!search_only) {
-    atomicOr(&b.ctl[4], 1);
+    atomicOr(&b.ctl[C_BROKEN], 1);
   }
   const double2 p = b.posB[i];
   b.posA[dst] = p;
   b.tagA[dst] = tg;
   if (search_only) return;
+  if (b.cellOf) b.cellOf[dst] = c;
   const DevScene& S = sc[b.sceneOf[i]];
   const double2 v = b.velB[i];
   const double rho = b.rhoB[i];
   const float4 rt = b.rtB[i];
   b.velA[dst] = v;
   b.rhoA[dst] = rho;
-  const int itime = b.ctl[0];
-  const double dt = S.dt;
-  double vx = v.x, vy = v.y, lx, ly, rp = rho;
-  if (tag_type(tg) == 1) {
-    if (itime != 1) {
-      rp = rho + 0.5 * dt * (double)rt.x;
-      vx = v.x + 0.5 * dt * (double)rt.y;
-      vy = v.y + 0.5 * dt * (double)rt.z;
-      lx = vx + 0.5 * dt * (double)rt.y;
-      ly = vy + 0.5 * dt * (double)rt.z;
-    } else {
-      const double2 l0 = b.deckLvel[my];
-      lx = l0.x;
-      ly = l0.y;
-    }
-  } else {
-    const double2 l0 = b.deckLvel[my];
-    lx = l0.x;
-    ly = l0.y;
-    if (S.motion != 0) {
-      double2 xx = p, vv = v;
-      double lvx;
-      if (wall_kin(S, b.wallKind[my], b.wallRef[my], S.t0 + itime * dt, b.deckPos[my], xx, vv, &lvx)) lx = lvx;
-    }
+  const int itime = b.it[0];
+  const bool ref = S.mode == 1;
+  const Pred o = predict(S, b, tg, itime, p, v, rho, rt, ref ? b.presB[i] : 0.0,
+                         ref ? b.rt2B[i] : make_float4(0.f, 0.f, 0.f, 0.f),
+                         ref ? b.dvB[i] : make_float2(0.f, 0.f));
+  float4 ma, mb;
+  float2 mc;
+  mirror_of(S, o, rt.w, tg, ma, mb, mc);
+  b.mA[dst] = ma;
+  b.mB[dst] = mb;
+  if (ref) {
+    b.presA[dst] = o.P;
+    b.mC[dst] = mc;
   }
-  if (S.mode == 1) {
-    // walls: DynamicPressure from the deck, localDensity rho0, localPressure 0
-    // (inputbp.c:117-150, 289-290)
-    double P = tag_type(tg) == 1 ? b.presB[i] : b.deckP[my];
-    double Pp = P, rl = S.rho0, pl = 0.0;
-    if (tag_type(tg) == 1 && itime == 1) {
-      rl = b.deckRl[my];
-      pl = b.deckPl[my];
-    } else if (tag_type(tg) == 1) {
-      const float4 r2 = b.rt2B[i];
-      const float2 dv = b.dvB[i];
-      Pp = P + 0.5 * dt * (double)r2.x;
-      rl = rp + (double)r2.y;
-      pl = Pp + (double)r2.z;
-      lx += S.eps * (double)dv.x;
-      ly += S.eps * (double)dv.y;
-    }
-    // densities enter the mirror as deviations from rho0 (formed in FP64), so the
-    // differences the fluctuations and diffusion terms take keep FP32's relative
-    // precision instead of losing the leading digits of rho0
-    b.presA[dst] = P;
-    b.mA[dst] = make_float4((float)vx, (float)vy, (float)lx, (float)ly);
-    b.mB[dst] = make_float4((float)(rp - S.rho0), (float)Pp, rt.w, __uint_as_float(tg));
-    b.mC[dst] = make_float2((float)(rl - S.rho0), (float)pl);
-    if (MIRRORS && S.nsec > 0 && tag_type(tg) == 1 && mark_mirrors(S, b, itime, my, p, dst)) {
-      b.predV[dst] = make_double2(vx, vy);   // the fields this row's mirror rows copy
-      b.predVl[dst] = make_double2(lx, ly);
-      b.predRho[dst] = rp;
-      b.predRl[dst] = rl;
-      b.predP[dst] = Pp;
-      b.predPl[dst] = pl;
-    }
-    return;
-  }
-  const double P = tait(S, rp);
-  const double pr = P / (rp * rp);
-  b.mA[dst] = make_float4((float)vx, (float)vy, (float)lx, (float)ly);
-  b.mB[dst] = make_float4((float)rp, (float)pr, rt.w, __uint_as_float(tg));
   if (MIRRORS && S.nsec > 0 && tag_type(tg) == 1 && mark_mirrors(S, b, itime, my, p, dst)) {
-    b.predV[dst] = make_double2(vx, vy);
-    b.predVl[dst] = make_double2(lx, ly);
-    b.predRho[dst] = rp;
-    b.predRl[dst] = rp;   // config mode keeps localDensity equal to density
-    b.predP[dst] = P;
-    b.predPl[dst] = 0.0;
+    b.predV[dst] = make_double2(o.vx, o.vy);   // the fields this row's mirror rows copy
+    b.predVl[dst] = make_double2(o.lx, o.ly);
+    b.predRho[dst] = o.rp;
+    b.predRl[dst] = o.rl;
+    b.predP[dst] = o.Pp;
+    b.predPl[dst] = o.pl;
   }
+}
+
+// A lazy re-sort's reorder: the slot's final position as reorder_slot finds it, and
+// a plain gather of the B set into the A set (the pair epilogue has already predicted
+// the mirrors); the slot's cell goes to cellOf.
+__device__ __forceinline__ void permute_slot(int d, int n, const StepBufs& b, bool ref) {
+  if (d >= n) return;
+  const int i = b.tmpIdx[d];
+  const int c = b.keys[i];
+  const int c0 = b.cellStart[c], c1 = b.cellStart[c + 1];
+  const uint32_t tg = b.tagB[i];
+  const uint32_t my = tag_id(tg);
+  int dst = d;
+  if (c1 - c0 <= CAND_MAX) {
+    int r = 0;
+    for (int t = c0; t < c1; ++t) r += (tag_id(b.tagB[b.tmpIdx[t]]) < my) ? 1 : 0;
+    dst = c0 + r;
+  } else {
+    atomicOr(&b.ctl[C_BROKEN], 1);
+  }
+  b.posA[dst] = b.posB[i];
+  b.tagA[dst] = tg;
+  b.velA[dst] = b.velB[i];
+  b.rhoA[dst] = b.rhoB[i];
+  b.mA[dst] = b.mA2[i];
+  b.mB[dst] = b.mB2[i];
+  if (ref) {
+    b.presA[dst] = b.presB[i];
+    b.mC[dst] = b.mC2[i];
+  }
+  b.cellOf[dst] = c;
 }
 
 // ------------------------------------------------------ mirror particles
@@ -445,7 +655,7 @@ __device__ void emit_mirrors(const StepBufs& b, const DevScene* __restrict__ sc,
   const DevScene& S = sc[s];
   const int i = (int)tag_id(tg) - S.poff;
   double lbp, rbp;
-  boundary_lines(S, S.t0 + b.ctl[0] * S.dt, lbp, rbp);
+  boundary_lines(S, S.t0 + b.it[0] * S.dt, lbp, rbp);
   const double2 x = b.posA[d];
   const int first = b.gwstart[S.gw];
   const uint32_t below = (1u << (i & 31)) - 1u;
@@ -968,7 +1178,7 @@ __device__ void wave_overflow(const StepBufs& b, const DevScene* __restrict__ sc
   unsigned long long m = __ballot(ovf);
   if (m == 0ull) return;
   const bool lead = (threadIdx.x & 63) == 0;
-  if (lead) atomicOr(&b.ctl[1], 1);
+  if (lead) atomicOr(ovf_word(b), 1);
   // many overflowing lanes (the reference's smoothing length caps every query):
   // each finds its own key at once; what it leaves goes to the wavefront search
   if (ROWS && ls != nullptr && b.kmask != nullptr && __popcll(m) >= 8) {
@@ -978,7 +1188,7 @@ __device__ void wave_overflow(const StepBufs& b, const DevScene* __restrict__ sc
       double kd2;
       uint32_t kid;
       if (S.nsec == 0 && lane_kth(S, p, xp, rbp, rlp, b.posA, b.tagA, *ls, kd2, kid, b.kmask + p, b.kmask_n)) {
-        b.kth[p] = KthRec{kd2, kid, -b.ctl[3]};   // negated: kmask holds its in-range candidates
+        b.kth[p] = KthRec{kd2, kid, -b.it[1]};   // negated: kmask holds its in-range candidates
         done = true;
       }
     }
@@ -1013,8 +1223,8 @@ __device__ void wave_overflow(const StepBufs& b, const DevScene* __restrict__ sc
       wave_kth_of_list(S, q, b.posA, b.tagA, b.cellStart, L, n, kd2, kid, false, S.nsec > 0 ? &b : nullptr);
     }
     if (lead) {
-      b.kth[q] = KthRec{kd2, kid, b.ctl[3]};
-      if (n < 0) atomicOr(&b.ctl[4], 1);
+      b.kth[q] = KthRec{kd2, kid, b.it[1]};
+      if (n < 0) atomicOr(&b.ctl[C_BROKEN], 1);
     }
   }
 }
@@ -1022,7 +1232,7 @@ __device__ void wave_overflow(const StepBufs& b, const DevScene* __restrict__ sc
 // K-th key of slot q in this step, or +inf when q did not overflow.
 __device__ __forceinline__ void kth_of(const StepBufs& b, int q, double& kd2, uint32_t& kid) {
   const KthRec k = b.kth[q];
-  if (k.step == b.ctl[3] || k.step == -b.ctl[3]) {
+  if (k.step == b.it[1] || k.step == -b.it[1]) {
     kd2 = k.d2;
     kid = k.id;
   } else {
@@ -1360,7 +1570,57 @@ __device__ __forceinline__ float2 mirror_c(const StepBufs& b, uint32_t q) {
   return MODE == UPH_MODE_REF ? ld_off(b.mC, q) : make_float2(0.f, 0.f);
 }
 
+// A lazy handle's epilogue of slot p (after its state for the next step is written
+// to the B set): the tag; the next step's predicted neighbour mirrors, by the same
+// predict() the sort's reorder would run on this B-side state, so a step that is not
+// re-sorted reads exactly what a re-sorted one would; and the stale test of the new
+// position x1: the slot is stale when x1 lies outside the cell it was binned into
+// (cellOf) grown by the margin m = 1 - 2h/cell (in cells; the grid's outer cells are
+// open outward).  A query's 3x3 stencil is taken around its current cell, so while no
+// particle is stale every neighbour within 2h lies in the stencil's binned cells: a
+// neighbour binned two cells away would be more than (1 - m) cells = 2h away.  A stale
+// slot marks its scene and the step (one atomic per scene and wavefront).
+template <bool FLUID>
+__device__ __forceinline__ void lazy_out(const DevScene& S, const StepBufs& b, int p, uint32_t tg, int itime,
+                                         double2 x1, double2 v1, double rho1, float4 rt, double P1, float4 r2,
+                                         float2 dv) {
+  b.tagB[p] = tg;
+  const Pred o = FLUID ? predict_fluid_next(S, v1, rho1, rt, P1, r2, dv)
+                       : predict(S, b, tg, itime + 1, x1, v1, rho1, rt, P1, r2, dv);
+  float4 ma, mb;
+  float2 mc;
+  mirror_of(S, o, rt.w, tg, ma, mb, mc);
+  b.mA2[p] = ma;
+  b.mB2[p] = mb;
+  if (S.mode == 1) b.mC2[p] = mc;
+  int cx, cy;
+  cell_of(S, x1.x, x1.y, cx, cy);
+  const int bk = b.cellOf[p];
+  bool stale = false;
+  if (S.coff + cy * S.ncx + cx != bk) {
+    const int r = bk - S.coff;
+    const int bx = r % S.ncx, by = r / S.ncx;
+    const double u = (x1.x - S.xmin) * S.cell_inv, w = (x1.y - S.ymin) * S.cell_inv;
+    const double m = S.skin_m;
+    stale = (bx > 0 && u < bx - m) || (bx < S.ncx - 1 && u >= bx + 1 + m) || (by > 0 && w < by - m) ||
+            (by < S.ncy - 1 && w >= by + 1 + m);
+  }
+  const unsigned long long sm = __ballot(stale);
+  if (sm == 0ull) return;
+  const int par = itime & 1;
+  const int lead = __ffsll((long long)sm) - 1;
+  const int s0 = __shfl(S.sidx, lead, 64);
+  if ((int)(threadIdx.x & 63) == lead) {
+    atomicOr(&b.sstale[par * b.nscenes + s0], 1);
+    atomicOr(&b.ctl[C_LSTALE + par], 1);
+  } else if (stale && S.sidx != s0) {   // a wavefront of two scenes
+    atomicOr(&b.sstale[par * b.nscenes + S.sidx], 1);
+  }
+}
+
 // Boundary slot: wall state for the next step (deck + kinematics at t = (itime+1) dt).
+// LAZY: a lazy re-sort handle's pair kernel (lazy_out after the state).
+template <bool LAZY = false>
 __device__ __forceinline__ void wall_update(const DevScene& S, const StepBufs& b, int p, uint32_t tg, int itime,
                                             float mass) {
   const uint32_t id = tag_id(tg);
@@ -1369,14 +1629,21 @@ __device__ __forceinline__ void wall_update(const DevScene& S, const StepBufs& b
     double lvx;
     (void)wall_kin(S, b.wallKind[id], b.wallRef[id], S.t0 + (itime + 1) * S.dt, b.deckPos[id], x, v, &lvx);
   }
+  const double rho = b.rhoA[p];
+  const float4 rt = make_float4(0.f, 0.f, 0.f, mass);
   b.posB[p] = x;
   b.velB[p] = v;
-  b.rhoB[p] = b.rhoA[p];
-  b.rtB[p] = make_float4(0.f, 0.f, 0.f, mass);
-  if (S.mode == 1) b.presB[p] = b.deckP[id];
+  b.rhoB[p] = rho;
+  b.rtB[p] = rt;
+  const double P = S.mode == 1 ? b.deckP[id] : 0.0;
+  if (S.mode == 1) b.presB[p] = P;
+  if (LAZY)
+    lazy_out<false>(S, b, p, tg, itime, x, v, rho, rt, P, make_float4(0.f, 0.f, 0.f, 0.f), make_float2(0.f, 0.f));
 }
 
-// Gravity and the FP64 leapfrog corrector (sph.cc:137-168) for fluid slot p.
+// Gravity and the FP64 leapfrog corrector (sph.cc:137-168) for fluid slot p (LAZY: a
+// lazy re-sort handle's pair kernel, lazy_out after the state).
+template <bool LAZY = false>
 __device__ __forceinline__ void correct(const DevScene& S, const StepBufs& b, int p, int itime, double2 xp,
                                         double2 vn, float mass, const PairCtx& c) {
   const double dt = S.dt;
@@ -1395,20 +1662,26 @@ __device__ __forceinline__ void correct(const DevScene& S, const StepBufs& b, in
     v1x = vn.x + dt * ax;
     v1y = vn.y + dt * ay;
   }
-  b.posB[p] = make_double2(xp.x + dt * v1x, xp.y + dt * v1y);
+  const double2 x1 = make_double2(xp.x + dt * v1x, xp.y + dt * v1y);
+  const float4 rt = make_float4(Rt, (float)atx, (float)aty, mass);
+  b.posB[p] = x1;
   b.velB[p] = make_double2(v1x, v1y);
   b.rhoB[p] = rho1;
-  b.rtB[p] = make_float4(Rt, (float)atx, (float)aty, mass);
+  b.rtB[p] = rt;
   if (b.dbgRates) {
     // {R, R~, a~ (as the predictor reads them)} then the FP64 acceleration the
     // corrector used, in the same 32 bytes
     b.dbgRates[2 * p] = make_float4(R, Rt, (float)atx, (float)aty);
     *reinterpret_cast<double2*>(b.dbgRates + 2 * p + 1) = make_double2(ax, ay);
   }
+  if (LAZY)
+    lazy_out<true>(S, b, p, (1u << UPH_ID_BITS) | c.idp, itime, x1, make_double2(v1x, v1y), rho1, rt, 0.0,
+             make_float4(0.f, 0.f, 0.f, 0.f), make_float2(0.f, 0.f));
 }
 
 // UPH_MODE_REF: gravity, the sponge layer and the FP64 leapfrog corrector
 // (sph.cc:137-168) with DynamicPressure integrated like the density.
+template <bool LAZY = false>
 __device__ __forceinline__ void correct(const DevScene& S, const StepBufs& b, int p, int itime, double2 xp,
                                         double2 vn, float mass, const RefCtx& c) {
   const double dt = S.dt;
@@ -1430,13 +1703,19 @@ __device__ __forceinline__ void correct(const DevScene& S, const StepBufs& b, in
   const double rho1 = rho_n + hdt * (double)R;
   const double P1 = P_n + hdt * (double)RP;
   const double v1x = vn.x + hdt * ax, v1y = vn.y + hdt * ay;
-  b.posB[p] = make_double2(xp.x + dt * v1x, xp.y + dt * v1y);
+  const double2 x1 = make_double2(xp.x + dt * v1x, xp.y + dt * v1y);
+  const float4 rt = make_float4(Rt, (float)atx, (float)aty, mass);
+  const float4 r2 = make_float4(RPt, c.dr, c.dP, 0.f);
+  const float2 dv = make_float2(c.dvx, c.dvy);
+  b.posB[p] = x1;
   b.velB[p] = make_double2(v1x, v1y);
   b.rhoB[p] = rho1;
   b.presB[p] = P1;
-  b.rtB[p] = make_float4(Rt, (float)atx, (float)aty, mass);
-  b.rt2B[p] = make_float4(RPt, c.dr, c.dP, 0.f);
-  b.dvB[p] = make_float2(c.dvx, c.dvy);
+  b.rtB[p] = rt;
+  b.rt2B[p] = r2;
+  b.dvB[p] = dv;
+  if (LAZY)
+    lazy_out<true>(S, b, p, (1u << UPH_ID_BITS) | c.idp, itime, x1, make_double2(v1x, v1y), rho1, rt, P1, r2, dv);
   if (b.dbgRates) {
     b.dbgRates[4 * p] = make_float4(R, Rt, (float)atx, (float)aty);
     *reinterpret_cast<double2*>(b.dbgRates + 4 * p + 1) = make_double2(ax, ay);
@@ -1449,23 +1728,23 @@ __device__ __forceinline__ void correct(const DevScene& S, const StepBufs& b, in
 // EXACT applies the K-nearest cap (run only when some query overflowed).
 // Returns true for a query with more than K candidates in range when !EXACT
 // (nothing written; its K-th key is left to wave_overflow).
-template <bool EXACT, int KT, int MODE, bool ONE = false>
+template <bool EXACT, int KT, int MODE, bool ONE = false, bool LAZY = false>
 __device__ __forceinline__ bool pair_direct(int p, const StepBufs& b, const DevScene* __restrict__ sc) {
   const uint32_t tg = b.tagA[p];
   const DevScene& S = sc[ONE ? 0 : b.sceneOf[p]];
-  const int itime = b.ctl[0];
+  const int itime = b.it[0];
   const double2 xp = b.posA[p];
   const double2 vn = b.velA[p];
   const float mass = b.mB[p].z;
   if (tag_type(tg) != 1) {
-    wall_update(S, b, p, tg, itime, mass);
+    wall_update<LAZY>(S, b, p, tg, itime, mass);
     return false;
   }
   {
     int rb[3], rl[3];
     candidate_rows(S, xp, b.cellStart, rb, rl);
     if (rl[0] + rl[1] + rl[2] > CAND_MAX) {   // a broken state: flag it, skip the query
-      atomicOr(&b.ctl[4], 1);
+      atomicOr(&b.ctl[C_BROKEN], 1);
       return false;
     }
   }
@@ -1490,7 +1769,7 @@ __device__ __forceinline__ bool pair_direct(int p, const StepBufs& b, const DevS
   });
   if (!EXACT && S.nsec > 0 && cnt <= S.K) cnt += ghost_count(S, xp, b);
   if (!EXACT && cnt > S.K) return true;
-  correct(S, b, p, itime, xp, vn, mass, c);
+  correct<LAZY>(S, b, p, itime, xp, vn, mass, c);
   return false;
 }
 
@@ -1554,14 +1833,14 @@ __device__ __forceinline__ void walk_pairs(const DevScene& S, const StepBufs& b,
 
 // Pass 2 of k_pair_compact for one fluid particle: the pair sums over the set bits
 // of m[0..2] (row starts rb[0..2]) and the corrector.
-template <int KT, int MODE>
+template <int KT, int MODE, bool LAZY = false>
 __device__ __forceinline__ void compact_pairs(const DevScene& S, const StepBufs& b, int p, uint32_t tg, int itime,
                                               double2 xp, double2 vn, float mass, const int* rb,
                                               const unsigned long long* m) {
   CtxOf<MODE> c;
   c.init(S, tg, b.mA[p], b.mB[p], mirror_c<MODE>(b, (uint32_t)p));
   walk_pairs<KT, MODE>(S, b, p, xp, c, rb, m);
-  correct(S, b, p, itime, xp, vn, mass, c);
+  correct<LAZY>(S, b, p, itime, xp, vn, mass, c);
 }
 
 // The capped path's pairs of fluid slot p by the same bit walk: pass 1 marks, in two
@@ -1569,7 +1848,7 @@ __device__ __forceinline__ void compact_pairs(const DevScene& S, const StepBufs&
 // that the K cap keeps; pass 2 walks them row by row, in slot order as the direct
 // loop sums them.
 // Returns false, leaving p to the direct loop, when a row is longer than 128 slots.
-template <int KT, int MODE, bool ONE = false>
+template <int KT, int MODE, bool ONE = false, bool LAZY = false>
 __device__ __forceinline__ bool exact_pairs(int p, const StepBufs& b, const DevScene* __restrict__ sc) {
   const uint32_t tg = b.tagA[p];
   const DevScene& S = sc[ONE ? 0 : b.sceneOf[p]];
@@ -1577,7 +1856,7 @@ __device__ __forceinline__ bool exact_pairs(int p, const StepBufs& b, const DevS
   int rb[3], rl[3];
   candidate_rows(S, xp, b.cellStart, rb, rl);
   if (rl[0] > 128 || rl[1] > 128 || rl[2] > 128) return false;
-  const int itime = b.ctl[0];
+  const int itime = b.it[0];
   const double2 vn = b.velA[p];
   const float mass = b.mB[p].z;
   CtxOf<MODE> c;
@@ -1588,7 +1867,7 @@ __device__ __forceinline__ bool exact_pairs(int p, const StepBufs& b, const DevS
   const double cut = S.cutoff;
   const uint32_t idp = tag_id(tg);
   const bool pb_ok = (int)idp < S.pb_gid;
-  const int stamp = b.ctl[3];
+  const int stamp = b.it[1];
   const bool have_mask = b.kmask != nullptr && b.kth[p].step == -stamp;
   // Row r's admitted candidates as bits of m[2r] (slots rb[r] + [0, 64)) and m[2r + 1]
   // (slots rb[r] + [64, 128)), by the rules of Search.cc:27-40 (admit<true>).  Pass 1a
@@ -1689,7 +1968,7 @@ __device__ __forceinline__ bool exact_pairs(int p, const StepBufs& b, const DevS
   const int rb1[3] = {rb[1] + 64, rb[2], rb[2] + 64};
   const unsigned long long m1[3] = {m[3], m[4], m[5]};
   walk_pairs<KT, MODE>(S, b, p, xp, c, rb1, m1);
-  correct(S, b, p, itime, xp, vn, mass, c);
+  correct<LAZY>(S, b, p, itime, xp, vn, mass, c);
   return true;
 }
 
@@ -1706,7 +1985,7 @@ __device__ __forceinline__ bool exact_pairs(int p, const StepBufs& b, const DevS
 // whose slots belong to two scenes returns at once and k_pair_mixed runs it with
 // the scene per lane.  ONE: the handle holds one scene (index 0, no check).  Every
 // lane of the wavefront calls this.
-template <int KT, int MODE, bool UNI, bool ONE = false>
+template <int KT, int MODE, bool UNI, bool ONE = false, bool LAZY = false>
 __device__ __forceinline__ void compact_query(int n, int p, const StepBufs& b, const DevScene* __restrict__ sc,
                                               KthLds& lds, LaneSel* ls = nullptr) {
   const bool live = p < n;
@@ -1717,7 +1996,7 @@ __device__ __forceinline__ void compact_query(int n, int p, const StepBufs& b, c
   const int sidx = ONE ? 0 : UNI ? __builtin_amdgcn_readfirstlane(slane) : slane;
   if (!ONE && UNI && __ballot(slane != sidx) != 0ull) return;   // a wavefront of two scenes
   const DevScene& S = sc[sidx];
-  const int itime = b.ctl[0];
+  const int itime = b.it[0];
   const double2 xp = b.posA[pp];
   const double cut = S.cutoff;
   int cx, cy;
@@ -1783,19 +2062,28 @@ __device__ __forceinline__ void compact_query(int n, int p, const StepBufs& b, c
     const double2 vn = b.velA[p];
     const float mass = b.mB[p].z;
     if (!fluid) {
-      wall_update(S, b, p, tg, itime, mass);
+      wall_update<LAZY>(S, b, p, tg, itime, mass);
       return;
     }
-    // drop the particle itself (always in range, in its own row)
-    const int self = p - rb[1];
-    if (self >= 0 && self < 64) m[1] &= ~(1ull << self);
+    // drop the particle itself (always in range, in the row of the cell it was binned
+    // into: its current cell's, or with a lazy re-sort possibly the row above or below)
+    if (LAZY) {
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        const int self = p - rb[r];
+        if (self >= 0 && self < rl[r]) m[r] &= ~(1ull << self);
+      }
+    } else {
+      const int self = p - rb[1];
+      if (self >= 0 && self < 64) m[1] &= ~(1ull << self);
+    }
     int cnt = __popcll(m[0]) + __popcll(m[1]) + __popcll(m[2]);
     if (S.nsec > 0 && !bad && cnt <= S.K) cnt += ghost_count(S, xp, b);   // mirror rows rank too
     if (bad || cnt > S.K) {
       ovf = true;   // the exact kernel recomputes this step
       return;
     }
-    compact_pairs<KT, MODE>(S, b, p, tg, itime, xp, vn, mass, rb, m);
+    compact_pairs<KT, MODE, LAZY>(S, b, p, tg, itime, xp, vn, mass, rb, m);
   };
   run();
   wave_overflow<true>(b, sc, p, ovf, lds, sidx, xp, rb, rlf, ls);
@@ -1879,6 +2167,10 @@ struct uph_handle {
   // some scene's queries are expected to be capped (4 pi (h/dx)^2 in range, a lattice's
   // count, above 3/4 of K): the pair kernel carries the per-lane K-th key search
   bool capped = false;
+  // lazy re-sort (b.lazy): the next step sorts the state from scratch (after uph_create
+  // or a state set from outside), gathering the B set of view sort_view
+  bool need_sort = true;
+  int sort_view = 0;
 };
 
 // Host functions of uph_step.hip that the other translation units call.
@@ -1887,8 +2179,20 @@ int alloc_bufs(StepBufs& b, int n, int ncells, int nblocks_scan, Err& err, bool 
 void free_bufs(StepBufs& b);
 int launch_scan(hipStream_t st, int m, int nb, int* count, int* partial, int* start);
 int launch_sort(hipStream_t st, int n, int ncells, int nb, StepBufs& b, const DevScene* dsc, int search_only,
-                bool step_ctl, Err& err);
+                bool step_ctl, Err& err, int view = 0);
 int launch_step(uph_handle* h, hipEvent_t pa, hipEvent_t pb, hipEvent_t pc = nullptr);
+int launch_call_step(uph_handle* h, int k, int n_steps, hipEvent_t pa, hipEvent_t pb, hipEvent_t pc);
+// uph_pair.hip: the pair pass of a step (both views for a lazy handle), and a
+// sort-every-step handle's capped pass.
+int launch_pair_pass(uph_handle* h, hipStream_t st);
+int launch_capped(uph_handle* h, hipStream_t st);
+// The buffers as the state getters read them after the last call (waits for the
+// stream): the A set holds the last step's input and B its output, in one slot order.
+// A lazy handle's are those of the view its last pair kernel ran with.
+int read_bufs(uph_t* h, StepBufs& out);
+// The state of a lazy handle is about to be set from outside: the next step sorts it
+// from the B set of the current read view.
+int lazy_reset(uph_t* h);
 int get_rates_f64(uph_t* h, int scene, float* R, float* Rt, double* a, float* at);
 // uph_diag.hip: approxVorticity of scene `scene` of the last evaluation, copied to host out.
 int vort_run(uph_t* h, int scene, double* out);

@@ -48,6 +48,10 @@ struct DevScene {
   // set them; LocalDepth, dx
   int nsec, sec[5], gw, nwords, gbase, slosh_halt;
   double lbp0, rbp0, depth, dx;
+  // lazy re-sort: the scene's index in its batch, and the margin (in cells) by which a
+  // particle may leave the cell it was binned into before its scene is re-sorted
+  int sidx;
+  double skin_m;
 };
 
 // Sections of derivatives.c:3193-4192 (one loop over the fluid rows each).

@@ -1,0 +1,132 @@
+// uph_pair.hip -- the step's pair kernels (compacted, direct and capped passes, the
+// wavefronts of two scenes) and their launches; the device code is in uph_internal.h.
+#include "uph_internal.h"
+
+// EXACT (the capped path) runs as a small grid-stride launch that returns at
+// once unless the step's overflow flag is set.  ONE: a one-scene handle, whose scene
+// constants stay in scalar registers (the capped reference-scheme pass spills 124
+// bytes per lane instead of 160 and runs 4 % faster).
+template <bool EXACT, int KT, int MODE, bool ONE = false, bool LAZY = false>
+// Four wavefronts per SIMD (128 VGPRs): the configuration scheme needs 129, the
+// reference scheme 179, which at this target spills and still runs the capped step
+// faster than with three wavefronts (no spill) or five.
+__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4)))
+void k_pair(int n, StepBufs b0, const DevScene* __restrict__ sc) {
+  const StepBufs b = LAZY ? pair_view(b0) : b0;
+  if (EXACT && (b.ctl[C_OVF] == 0 || b.ctl[C_BROKEN] != 0)) return;
+  if (EXACT && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&b.ctl[C_NOVF], 1);
+  __shared__ KthLds lds[EXACT ? 1 : BLK / 64];
+  const int nlog = (n + BLK - 1) / BLK;
+  // the trip count is the same for the whole block, so every wavefront reaches
+  // wave_overflow with all of its lanes
+  for (int lb = xcd_block(); lb < nlog; lb += gridDim.x) {
+    const int p = lb * BLK + threadIdx.x;
+    bool ovf = false;
+    if (p < n) {
+      // the capped step's fluid rows take the bit walk when their rows fit it
+      const bool walked = EXACT && tag_type(b.tagA[p]) == 1 && exact_pairs<KT, MODE, ONE, LAZY>(p, b, sc);
+      if (!walked) ovf = pair_direct<EXACT, KT, MODE, ONE, LAZY>(p, b, sc);
+    }
+    if (!EXACT) wave_overflow<false>(b, sc, p, ovf, lds[threadIdx.x >> 6]);
+  }
+}
+
+// LANE: the handle's queries are expected to be capped, so the kernel carries the
+// per-lane K-th key search and its LDS.
+template <int PB, int KT, int MODE, bool ONE, bool LANE, bool LAZY>
+// Occupancy targets, each measured faster than the compiler's own choice: the
+// reference scheme's sums need about 133 VGPRs, capped at 128 (4 wavefronts per SIMD
+// instead of 3, a few spilled registers); with the per-lane K-th search (LANE) the
+// configuration scheme runs 6 wavefronts per SIMD (80 VGPRs) instead of 5.  The
+// default instance (91 VGPRs, 5 wavefronts) measured slower at 6.
+__global__ __launch_bounds__(PB)
+__attribute__((amdgpu_waves_per_eu(MODE == UPH_MODE_REF ? 4 : LANE ? 6 : 1)))
+void k_pair_compact(int n, StepBufs b0, const DevScene* __restrict__ sc) {
+  const StepBufs b = LAZY ? pair_view(b0) : b0;
+  const int p = xcd_block() * PB + threadIdx.x;
+  if constexpr (LANE) {
+    __shared__ KthLaneLds lds[PB / 64];
+    compact_query<KT, MODE, true, ONE, LAZY>(n, p, b, sc, lds[threadIdx.x >> 6].k, &lds[threadIdx.x >> 6].l);
+  } else {
+    __shared__ KthLds lds[PB / 64];
+    compact_query<KT, MODE, true, ONE, LAZY>(n, p, b, sc, lds[threadIdx.x >> 6]);
+  }
+}
+
+// The wavefronts whose slots belong to two scenes (listed by uph_create), with the
+// scene taken per lane: one wavefront per block.
+template <int KT, int MODE, bool LAZY>
+__global__ __launch_bounds__(64) void k_pair_mixed(int n, StepBufs b0, const DevScene* __restrict__ sc,
+                                                   const int* __restrict__ waves) {
+  const StepBufs b = LAZY ? pair_view(b0) : b0;
+  __shared__ KthLds lds[1];
+  compact_query<KT, MODE, false, false, LAZY>(n, waves[blockIdx.x] * 64 + (int)threadIdx.x, b, sc, lds[0]);
+}
+
+// The pair pass of one smoothing kernel KT and scheme MODE (compacted or direct, and
+// the wavefronts of two scenes) over the buffers b; LAZY: a lazy re-sort handle's (no
+// such handle is expected to be capped).
+template <int KT, int MODE, bool LAZY>
+static int launch_pair_pass_b(uph_handle* h, hipStream_t st, const StepBufs& b) {
+  if (h->opts.pair_variant == 1) {
+    hipLaunchKernelGGL((k_pair<false, KT, MODE, false, LAZY>), dim3(grid8_of(h->n)), dim3(BLK), 0, st, h->n, b,
+                       h->dsc);
+  } else {
+    const int pbk = h->opts.pair_block;
+    const int g = (((h->n + pbk - 1) / pbk + 7) / 8) * 8;
+    const bool one = h->nscenes == 1, lane = !LAZY && h->capped;
+#define UPH_COMPACT(PBV, ONEV, LANEV)                                                                        \
+  hipLaunchKernelGGL((k_pair_compact<PBV, KT, MODE, ONEV, LANEV, LAZY>), dim3(g), dim3(PBV), 0, st, h->n, b, h->dsc)
+#define UPH_COMPACT_PB(ONEV, LANEV)              \
+  do {                                           \
+    if (pbk == 128) UPH_COMPACT(128, ONEV, LANEV); \
+    else if (pbk == 512) UPH_COMPACT(512, ONEV, LANEV); \
+    else UPH_COMPACT(256, ONEV, LANEV);          \
+  } while (0)
+    if (one && lane) UPH_COMPACT_PB(true, !LAZY);
+    else if (one) UPH_COMPACT_PB(true, false);
+    else if (lane) UPH_COMPACT_PB(false, !LAZY);
+    else UPH_COMPACT_PB(false, false);
+#undef UPH_COMPACT_PB
+#undef UPH_COMPACT
+    if (h->nmixed > 0)
+      hipLaunchKernelGGL((k_pair_mixed<KT, MODE, LAZY>), dim3(h->nmixed), dim3(64), 0, st, h->n, b, h->dsc,
+                         h->mixedWaves);
+  }
+  HIPCHK(hipGetLastError(), h->err);
+  return 0;
+}
+
+template <int KT, int MODE>
+static int launch_pair_pass_kt(uph_handle* h, hipStream_t st) {
+  return h->b.lazy ? launch_pair_pass_b<KT, MODE, true>(h, st, h->b) : launch_pair_pass_b<KT, MODE, false>(h, st, h->b);
+}
+
+// The exact capped fixup of a sort-every-step handle (returns at once unless the step
+// overflowed).
+template <int KT, int MODE>
+static int launch_capped_kt(uph_handle* h, hipStream_t st) {
+  const int gfix = std::min(grid8_of(h->n), FIX_GRID);
+  if (h->nscenes == 1)
+    hipLaunchKernelGGL((k_pair<true, KT, MODE, true>), dim3(gfix), dim3(BLK), 0, st, h->n, h->b, h->dsc);
+  else
+    hipLaunchKernelGGL((k_pair<true, KT, MODE>), dim3(gfix), dim3(BLK), 0, st, h->n, h->b, h->dsc);
+  HIPCHK(hipGetLastError(), h->err);
+  return 0;
+}
+
+int launch_pair_pass(uph_handle* h, hipStream_t st) {
+  if (h->mode == UPH_MODE_REF)
+    return h->kernel == UPH_KERNEL_POLY7 ? launch_pair_pass_kt<UPH_KERNEL_POLY7, UPH_MODE_REF>(h, st)
+                                         : launch_pair_pass_kt<UPH_KERNEL_CUBIC, UPH_MODE_REF>(h, st);
+  return h->kernel == UPH_KERNEL_POLY7 ? launch_pair_pass_kt<UPH_KERNEL_POLY7, UPH_MODE_CONFIG>(h, st)
+                                       : launch_pair_pass_kt<UPH_KERNEL_CUBIC, UPH_MODE_CONFIG>(h, st);
+}
+
+int launch_capped(uph_handle* h, hipStream_t st) {
+  if (h->mode == UPH_MODE_REF)
+    return h->kernel == UPH_KERNEL_POLY7 ? launch_capped_kt<UPH_KERNEL_POLY7, UPH_MODE_REF>(h, st)
+                                         : launch_capped_kt<UPH_KERNEL_CUBIC, UPH_MODE_REF>(h, st);
+  return h->kernel == UPH_KERNEL_POLY7 ? launch_capped_kt<UPH_KERNEL_POLY7, UPH_MODE_CONFIG>(h, st)
+                                       : launch_capped_kt<UPH_KERNEL_CUBIC, UPH_MODE_CONFIG>(h, st);
+}

@@ -382,7 +382,7 @@ constexpr NaLayout na_layout() {
   L.cellStart = o; o += na_up((C + 1) * sizeof(int));
   L.cellCount = o; o += na_up(C * sizeof(int));
   L.partial = o; o += na_up(NB * sizeof(int));
-  L.ctl = o; o += na_up(8 * sizeof(int));
+  L.ctl = o; o += na_up(CTL_WORDS * sizeof(int));
   L.qcount = o; o += na_up((P + 1) * sizeof(int));
   L.qoff = o; o += na_up((P + 1) * sizeof(int));
   L.qpartial = o; o += na_up(NB * sizeof(int));
@@ -479,6 +479,7 @@ extern "C" int uph_search_neighbors_noalloc(const double* pos_xy, int* i_pair, i
   b.cellCount = reinterpret_cast<int*>(base + NA.cellCount);
   b.partial = reinterpret_cast<int*>(base + NA.partial);
   b.ctl = reinterpret_cast<int*>(base + NA.ctl);
+  b.it = b.ctl + C_IT;
   DevScene* dsc = reinterpret_cast<DevScene*>(base + NA.dsc);
   int* qcount = reinterpret_cast<int*>(base + NA.qcount);
   int* qoff = reinterpret_cast<int*>(base + NA.qoff);

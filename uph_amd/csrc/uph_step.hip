@@ -11,30 +11,54 @@ __global__ __launch_bounds__(BLK) void k_bin(int n, const double2* __restrict__ 
   bin_block(blockIdx.x, n, pos, sceneOf, sc, tagA, tagB, keys, rank, count);
 }
 
+// With `gate` (the first step of a lazy handle's call), the scan, scatter and bin
+// kernels return at once unless the last call held a re-sort over (*gate != 0).
 __global__ __launch_bounds__(SCAN_BLK) void k_scan_reduce(int m, const int* __restrict__ in, int* __restrict__ partial,
-                                                          int* __restrict__ step_ctl) {
+                                                          int* __restrict__ step_ctl, int* __restrict__ sstale,
+                                                          int nscenes, int view, const int* __restrict__ gate) {
   __shared__ int lds[SCAN_BLK / 64 + 1];
-  if (step_ctl && blockIdx.x == 0 && threadIdx.x == 0) open_step(step_ctl);
+  if (gate && *gate == 0) return;
+  if (step_ctl && blockIdx.x == 0 && threadIdx.x == 0) open_step(step_ctl, sstale, nscenes, view);
   scan_reduce_tile(blockIdx.x, m, in, partial, lds);
 }
 
 __global__ __launch_bounds__(SCAN_BLK) void k_scan_down(int m, int* __restrict__ count, const int* __restrict__ partial,
-                                                        int* __restrict__ start) {
+                                                        int* __restrict__ start, const int* __restrict__ gate) {
   __shared__ int tile[SCAN_TILE + SCAN_TILE / 16];
   __shared__ int lds[SCAN_BLK / 64 + 1];
+  if (gate && *gate == 0) return;
   scan_down_tile(blockIdx.x, m, count, partial, start, tile, lds);
 }
 
 __global__ __launch_bounds__(BLK) void k_scatter(int n, const int* __restrict__ keys, const int* __restrict__ rank,
-                                                 const int* __restrict__ cellStart, int* __restrict__ tmpIdx) {
+                                                 const int* __restrict__ cellStart, int* __restrict__ tmpIdx,
+                                                 const int* __restrict__ gate) {
   const int i = blockIdx.x * BLK + threadIdx.x;
-  if (i >= n) return;
+  if (i >= n || (gate && *gate == 0)) return;
   tmpIdx[cellStart[keys[i]] + rank[i]] = i;
 }
 
 template <bool MIRRORS>
 __global__ __launch_bounds__(BLK) void k_reorder(int n, StepBufs b, const DevScene* __restrict__ sc, int search_only) {
   reorder_slot<MIRRORS>(blockIdx.x * BLK + threadIdx.x, n, b, sc, search_only);
+}
+
+// The re-sort a lazy handle's last call held over (C_PEND), at the first step of the
+// next call: the stale scenes binned again, the others kept in their cells, the B set
+// of the step's view gathered into its A set.
+__global__ __launch_bounds__(BLK) void k_bin_held(int n, StepBufs b, const DevScene* __restrict__ sc) {
+  if (b.ctl[C_PEND] == 0) return;
+  const StepBufs v = lazy_view(b);
+  // the holding step's flags: every scene after a capped pass, else the stale ones
+  const int par = (b.ctl[C_IT] - 1) & 1;
+  const int* keep = b.ctl[C_LOVF + par] != 0 ? nullptr : b.sstale + par * b.nscenes;
+  bin_block(blockIdx.x, n, v.posB, b.sceneOf, sc, nullptr, nullptr, b.keys, b.rank, b.cellCount, keep, b.cellOf);
+}
+
+__global__ __launch_bounds__(BLK) void k_permute_held(int n, StepBufs b, int ref) {
+  if (b.ctl[C_PEND] == 0) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&b.ctl[C_NRES], 1);
+  permute_slot(blockIdx.x * BLK + threadIdx.x, n, lazy_view(b), ref != 0);
 }
 
 // Popcount of each word of section-test bits (the scan's input); the words move to
@@ -75,64 +99,6 @@ __global__ __launch_bounds__(BLK) void k_ghost_scatter(StepBufs b) {
   const int t = b.gcellStart[b.gkey[j]] + b.grank[j];
   b.gposS[t] = b.gpos[j];
   b.gidS[t] = b.gid[j];
-}
-
-// EXACT (the capped path) runs as a small grid-stride launch that returns at
-// once unless the step's overflow flag is set.  ONE: a one-scene handle, whose scene
-// constants stay in scalar registers (the capped reference-scheme pass spills 124
-// bytes per lane instead of 160 and runs 4 % faster).
-template <bool EXACT, int KT, int MODE, bool ONE = false>
-// Four wavefronts per SIMD (128 VGPRs): the configuration scheme needs 129, the
-// reference scheme 179, which at this target spills and still runs the capped step
-// faster than with three wavefronts (no spill) or five.
-__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4)))
-void k_pair(int n, StepBufs b, const DevScene* __restrict__ sc) {
-  if (EXACT && (b.ctl[1] == 0 || b.ctl[4] != 0)) return;
-  if (EXACT && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&b.ctl[2], 1);
-  __shared__ KthLds lds[EXACT ? 1 : BLK / 64];
-  const int nlog = (n + BLK - 1) / BLK;
-  // the trip count is the same for the whole block, so every wavefront reaches
-  // wave_overflow with all of its lanes
-  for (int lb = xcd_block(); lb < nlog; lb += gridDim.x) {
-    const int p = lb * BLK + threadIdx.x;
-    bool ovf = false;
-    if (p < n) {
-      // the capped step's fluid rows take the bit walk when their rows fit it
-      const bool walked = EXACT && tag_type(b.tagA[p]) == 1 && exact_pairs<KT, MODE, ONE>(p, b, sc);
-      if (!walked) ovf = pair_direct<EXACT, KT, MODE, ONE>(p, b, sc);
-    }
-    if (!EXACT) wave_overflow<false>(b, sc, p, ovf, lds[threadIdx.x >> 6]);
-  }
-}
-
-// LANE: the handle's queries are expected to be capped, so the kernel carries the
-// per-lane K-th key search and its LDS.
-template <int PB, int KT, int MODE, bool ONE, bool LANE>
-// Occupancy targets, each measured faster than the compiler's own choice: the
-// reference scheme's sums need about 133 VGPRs, capped at 128 (4 wavefronts per SIMD
-// instead of 3, a few spilled registers); with the per-lane K-th search (LANE) the
-// configuration scheme runs 6 wavefronts per SIMD (80 VGPRs) instead of 5.  The
-// default instance (91 VGPRs, 5 wavefronts) measured slower at 6.
-__global__ __launch_bounds__(PB)
-__attribute__((amdgpu_waves_per_eu(MODE == UPH_MODE_REF ? 4 : LANE ? 6 : 1)))
-void k_pair_compact(int n, StepBufs b, const DevScene* __restrict__ sc) {
-  const int p = xcd_block() * PB + threadIdx.x;
-  if constexpr (LANE) {
-    __shared__ KthLaneLds lds[PB / 64];
-    compact_query<KT, MODE, true, ONE>(n, p, b, sc, lds[threadIdx.x >> 6].k, &lds[threadIdx.x >> 6].l);
-  } else {
-    __shared__ KthLds lds[PB / 64];
-    compact_query<KT, MODE, true, ONE>(n, p, b, sc, lds[threadIdx.x >> 6]);
-  }
-}
-
-// The wavefronts whose slots belong to two scenes (listed by uph_create), with the
-// scene taken per lane: one wavefront per block.
-template <int KT, int MODE>
-__global__ __launch_bounds__(64) void k_pair_mixed(int n, StepBufs b, const DevScene* __restrict__ sc,
-                                                   const int* __restrict__ waves) {
-  __shared__ KthLds lds[1];
-  compact_query<KT, MODE, false>(n, waves[blockIdx.x] * 64 + (int)threadIdx.x, b, sc, lds[0]);
 }
 
 // ------------------------------------------------------------ persistent step
@@ -295,10 +261,10 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) void k
     {
       const PersistArgs& a = phase_args(pa);
       const StepBufs& b = a.b;
-      const int ovf = __hip_atomic_load(&b.ctl[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      const int broken = __hip_atomic_load(&b.ctl[4], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const int ovf = __hip_atomic_load(&b.ctl[C_OVF], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const int broken = __hip_atomic_load(&b.ctl[C_BROKEN], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       if (ovf != 0 && broken == 0) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&b.ctl[2], 1);
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&b.ctl[C_NOVF], 1);
         const int nbl = (a.n + BLK - 1) / BLK;
         for_vblocks(nbl, [&](int v) {
           const int p = v * BLK + threadIdx.x;
@@ -310,12 +276,114 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) void k
   }
 }
 
+// ----------------------------------------------------------- lazy re-sort
+//
+// A lazy handle's step is its pair kernel and five short kernels that decide on the
+// device whether there is work and return at once when there is none: k_post (the
+// capped pass over every slot when a query had more than K in range, as k_pair<true>;
+// then, for a re-sort, the binning: the stale scenes binned again and the others kept in
+// their cells, or every scene when the capped pass ran, since its stale flags are not
+// final until it ends), the two scan kernels, the scatter, and k_close (a plain gather of
+// the step's B set into its A set, the pair epilogue having predicted the mirrors; and the
+// next step's control words).  Without a re-sort the next step reads the B set as its A
+// set (the view flips).  The last step of a call (hold) leaves a needed re-sort to the
+// next call's first step (C_PEND), so after a call the A set still holds the last step's
+// input (the state getters, approxVorticity).  None of these kernels waits on another
+// block: the re-sort's phases are separated by kernel boundaries, whatever else runs on
+// the GPU.  They read the step's counters from the copies the pair kernel left; k_close,
+// the last, writes the next step's, and clears the next step's parity flags.
+
+// Whether the step re-sorts: a query overflowed or a scene went stale, and the step is
+// not the last of its call.
+__device__ __forceinline__ bool lazy_resort(const int* ctl, int hold) {
+  const int par = ctl[C_ITC] & 1;
+  return !hold && (ctl[C_LOVF + par] != 0 || ctl[C_LSTALE + par] != 0);
+}
+
+template <int KT, int MODE, bool ONE>
+__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) void k_post(int n, int hold, StepBufs b0,
+                                                                                      const DevScene* __restrict__ sc) {
+  int* ctl = b0.ctl;
+  const int par = ctl[C_ITC] & 1;
+  const bool ovf = ctl[C_LOVF + par] != 0;
+  const bool capped = ovf && ctl[C_BROKEN] == 0;
+  const bool resort = lazy_resort(ctl, hold);
+  if (!capped && !resort) return;
+  if (capped && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&ctl[C_NOVF], 1);
+  StepBufs b = (ctl[C_VC] & 1) ? swapped(b0) : b0;
+  b.it = ctl + C_ITC;   // {itime, stamp} of the step, as the pair kernel saw them
+  const int* keep = ovf ? nullptr : b.sstale + par * b.nscenes;
+  const int nbl = (n + BLK - 1) / BLK;
+  // the trip count is the same for the whole block (bin_block works by wavefronts)
+  for (int v = xcd_block(); v < nbl; v += gridDim.x) {
+    const int p = v * BLK + threadIdx.x;
+    if (capped && p < n) {
+      const bool walked = tag_type(b.tagA[p]) == 1 && exact_pairs<KT, MODE, ONE, true>(p, b, sc);
+      if (!walked) (void)pair_direct<true, KT, MODE, ONE, true>(p, b, sc);
+    }
+    // the slot's new position came from this thread (or the pair kernel)
+    if (resort) bin_block(v, n, b.posB, b.sceneOf, sc, nullptr, nullptr, b.keys, b.rank, b.cellCount, keep, b.cellOf);
+  }
+}
+
+__global__ __launch_bounds__(SCAN_BLK) void k_lscan_reduce(int m, const int* __restrict__ in,
+                                                           int* __restrict__ partial, const int* __restrict__ ctl,
+                                                           int hold) {
+  __shared__ int lds[SCAN_BLK / 64 + 1];
+  if (!lazy_resort(ctl, hold)) return;
+  scan_reduce_tile(blockIdx.x, m, in, partial, lds);
+}
+
+__global__ __launch_bounds__(SCAN_BLK) void k_lscan_down(int m, int* __restrict__ count,
+                                                         const int* __restrict__ partial, int* __restrict__ start,
+                                                         const int* __restrict__ ctl, int hold) {
+  __shared__ int tile[SCAN_TILE + SCAN_TILE / 16];
+  __shared__ int lds[SCAN_BLK / 64 + 1];
+  if (!lazy_resort(ctl, hold)) return;
+  scan_down_tile(blockIdx.x, m, count, partial, start, tile, lds);
+}
+
+__global__ __launch_bounds__(BLK) void k_lscatter(int n, const int* __restrict__ keys, const int* __restrict__ rank,
+                                                  const int* __restrict__ cellStart, int* __restrict__ tmpIdx,
+                                                  const int* __restrict__ ctl, int hold) {
+  if (!lazy_resort(ctl, hold)) return;
+  for (int i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK) tmpIdx[cellStart[keys[i]] + rank[i]] = i;
+}
+
+__global__ __launch_bounds__(BLK) void k_close(int n, int hold, StepBufs b0) {
+  int* ctl = b0.ctl;
+  const int itime = ctl[C_ITC];
+  const int par = itime & 1;
+  const int vc = ctl[C_VC] & 1;
+  const bool resort = lazy_resort(ctl, hold);
+  if (resort) {
+    const StepBufs b = vc ? swapped(b0) : b0;
+    const bool ref = b.lazy == 2;
+    for (int d = blockIdx.x * BLK + threadIdx.x; d < n; d += gridDim.x * BLK) permute_slot(d, n, b, ref);
+  }
+  if (blockIdx.x == 0) {
+    if (threadIdx.x == 0) {
+      const bool want = ctl[C_LOVF + par] != 0 || ctl[C_LSTALE + par] != 0;
+      ctl[C_IT] = itime + 1;
+      ctl[C_STAMP] = ctl[C_STC] + 1;
+      ctl[C_V] = resort ? vc : vc ^ 1;
+      ctl[C_PEND] = (want && hold) ? 1 : 0;
+      ctl[C_LOVF + (par ^ 1)] = 0;
+      ctl[C_LSTALE + (par ^ 1)] = 0;
+      if (resort) ctl[C_NRES] += 1;
+    }
+    for (int s = threadIdx.x; s < b0.nscenes; s += BLK) b0.sstale[(par ^ 1) * b0.nscenes + s] = 0;
+  }
+}
+
 void fill_scene(DevScene& S, const uph_scene& u, double cell_factor) {
   memset(&S, 0, sizeof S);
   S.cutoff = u.h * u.h * 4;
   // a cell is at least the search radius, so the 3x3 stencil holds every pair
   const double cs = sqrt(S.cutoff) * (1.0 + 1e-9) * (cell_factor > 1.0 ? cell_factor : 1.0);
   S.cell_inv = 1.0 / cs;
+  // the margin (in cells) of a lazy re-sort: 1 - 2h / cell, less a rounding guard
+  S.skin_m = std::max(0.0, 1.0 - sqrt(S.cutoff) * S.cell_inv - 1e-9);
   S.xmin = u.box[0];
   S.ymin = u.box[1];
   S.ncx = (int)floor((u.box[2] - u.box[0]) * S.cell_inv) + 1;
@@ -368,13 +436,20 @@ void fill_scene(DevScene& S, const uph_scene& u, double cell_factor) {
 // handles whose queries are expected to be capped; other handles leave it null.
 int alloc_bufs(StepBufs& b, int n, int ncells, int nblocks_scan, Err& err, bool search_only, bool capped) {
   memset(&b, 0, sizeof b);
-  if (dalloc(&b.posA, (size_t)n + POS_PAD, err) || dalloc(&b.posB, n, err) || dalloc(&b.tagA, n, err) ||
+  if (dalloc(&b.posA, (size_t)n + POS_PAD, err) || dalloc(&b.posB, (size_t)n + POS_PAD, err) || dalloc(&b.tagA, n, err) ||
       dalloc(&b.tagB, n, err) || dalloc(&b.keys, n, err) || dalloc(&b.rank, n, err) ||
       dalloc(&b.tmpIdx, n, err) || dalloc(&b.cellStart, (size_t)ncells + 1, err) ||
       dalloc(&b.cellCount, ncells, err) || dalloc(&b.partial, nblocks_scan, err) ||
       dalloc(&b.sceneOf, n, err) || dalloc(&b.kth, n, err) ||
-      dalloc(&b.ctl, 8, err))
+      dalloc(&b.ctl, CTL_WORDS, err))
     return -1;
+  b.it = b.ctl + C_IT;
+  if (hipMemset(b.ctl, 0, CTL_WORDS * sizeof(int)) != hipSuccess ||
+      hipMemset(b.posA + n, 0, POS_PAD * sizeof(double2)) != hipSuccess ||
+      hipMemset(b.posB + n, 0, POS_PAD * sizeof(double2)) != hipSuccess) {
+    err.msg = "step buffers: init failed";
+    return -1;
+  }
   if (search_only) return 0;
   if (dalloc(&b.velA, n, err) || dalloc(&b.velB, n, err) || dalloc(&b.rhoA, n, err) ||
       dalloc(&b.rhoB, n, err) || dalloc(&b.rtB, n, err) ||
@@ -387,6 +462,25 @@ int alloc_bufs(StepBufs& b, int n, int ncells, int nblocks_scan, Err& err, bool 
       dalloc(&b.wallRef, n, err))
     return -1;
   b.kmask_n = capped ? n : 0;
+  return 0;
+}
+
+// The second state set, the binned cells and the stale flags of a lazy re-sort
+// handle (see StepBufs), zeroed.
+static int alloc_lazy_bufs(StepBufs& b, int n, int nscenes, Err& err) {
+  if (dalloc(&b.mA2, n, err) || dalloc(&b.mB2, n, err) || dalloc(&b.rtA, n, err) || dalloc(&b.rt2A, n, err) ||
+      dalloc(&b.mC2, n, err) || dalloc(&b.dvA, n, err) || dalloc(&b.cellOf, n, err) ||
+      dalloc(&b.sstale, 2 * (size_t)nscenes, err))
+    return -1;
+  if (hipMemset(b.rtA, 0, (size_t)n * sizeof(float4)) != hipSuccess ||
+      hipMemset(b.rt2A, 0, (size_t)n * sizeof(float4)) != hipSuccess ||
+      hipMemset(b.dvA, 0, (size_t)n * sizeof(float2)) != hipSuccess ||
+      hipMemset(b.sstale, 0, 2 * (size_t)nscenes * sizeof(int)) != hipSuccess) {
+    err.msg = "lazy buffers: init failed";
+    return -1;
+  }
+  b.lazy = 1;
+  b.nscenes = nscenes;
   return 0;
 }
 
@@ -419,7 +513,8 @@ static int alloc_ghost_bufs(StepBufs& b, int n, int ncells, int gtot, int wtot, 
 }
 
 void free_bufs(StepBufs& b) {
-  void* ptrs[] = {b.posA, b.velA, b.rhoA, b.tagA, b.mA, b.mB, b.posB, b.velB, b.rhoB,
+  void* ptrs[] = {b.mA2, b.mB2, b.rtA, b.rt2A, b.mC2, b.dvA, b.cellOf, b.sstale,
+                  b.posA, b.velA, b.rhoA, b.tagA, b.mA, b.mB, b.posB, b.velB, b.rhoB,
                   b.rtB, b.tagB, b.keys, b.rank, b.tmpIdx, b.cellStart, b.cellCount, b.partial,
                   b.sceneOf, b.kth, b.kmask, b.deckPos, b.deckVel, b.deckLvel, b.massD, b.ctl,
                   b.dbgRates, b.presA, b.presB, b.deckP, b.rt2B, b.dvB, b.mC, b.deckRl, b.deckPl,
@@ -434,20 +529,27 @@ void free_bufs(StepBufs& b) {
 // Exclusive scan of count[0, m) into start[0, m] over nb tiles (partial: nb ints),
 // zeroing count.
 int launch_scan(hipStream_t st, int m, int nb, int* count, int* partial, int* start) {
-  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_BLK), 0, st, m, count, partial, (int*)nullptr);
-  hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(SCAN_BLK), 0, st, m, count, partial, start);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_BLK), 0, st, m, count, partial, (int*)nullptr, (int*)nullptr,
+                     0, 0, (const int*)nullptr);
+  hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(SCAN_BLK), 0, st, m, count, partial, start, (const int*)nullptr);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-// The sort kernels: bin, scan, scatter, reorder.
-int launch_sort(hipStream_t st, int n, int ncells, int nb, StepBufs& b, const DevScene* dsc, int search_only,
-                       bool step_ctl, Err& err) {
+// The sort kernels: bin, scan, scatter, reorder, gathering the B set of buffer view
+// `view` into its A set (step_ctl: the first kernel opens the step; for a lazy handle
+// also its parity flags, and the view becomes the pair kernel's).
+int launch_sort(hipStream_t st, int n, int ncells, int nb, StepBufs& b0, const DevScene* dsc, int search_only,
+                bool step_ctl, Err& err, int view) {
+  const StepBufs b = view ? swapped(b0) : b0;
   hipLaunchKernelGGL(k_bin, dim3(grid_of(n)), dim3(BLK), 0, st, n, b.posB, b.sceneOf, dsc, b.tagA, b.tagB, b.keys,
                      b.rank, b.cellCount);
   hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_BLK), 0, st, ncells, b.cellCount, b.partial,
-                     step_ctl ? b.ctl : (int*)nullptr);
-  hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(SCAN_BLK), 0, st, ncells, b.cellCount, b.partial, b.cellStart);
-  hipLaunchKernelGGL(k_scatter, dim3(grid_of(n)), dim3(BLK), 0, st, n, b.keys, b.rank, b.cellStart, b.tmpIdx);
+                     step_ctl ? b.ctl : (int*)nullptr, (step_ctl && b.lazy) ? b.sstale : (int*)nullptr, b.nscenes,
+                     view, (const int*)nullptr);
+  hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(SCAN_BLK), 0, st, ncells, b.cellCount, b.partial, b.cellStart,
+                     (const int*)nullptr);
+  hipLaunchKernelGGL(k_scatter, dim3(grid_of(n)), dim3(BLK), 0, st, n, b.keys, b.rank, b.cellStart, b.tmpIdx,
+                     (const int*)nullptr);
   if (b.gtot > 0 && !search_only)
     hipLaunchKernelGGL(k_reorder<true>, dim3(grid_of(n)), dim3(BLK), 0, st, n, b, dsc, search_only);
   else
@@ -456,42 +558,43 @@ int launch_sort(hipStream_t st, int n, int ncells, int nb, StepBufs& b, const De
   return 0;
 }
 
-// The pair kernels of one smoothing kernel KT and scheme MODE: the selected pass,
-// then the exact capped fixup (which returns at once unless the step overflowed).
-template <int KT, int MODE>
-static int launch_pairs(uph_handle* h, hipStream_t st, hipEvent_t pb, hipEvent_t pc) {
-  if (h->opts.pair_variant == 1) {
-    hipLaunchKernelGGL((k_pair<false, KT, MODE>), dim3(grid8_of(h->n)), dim3(BLK), 0, st, h->n, h->b, h->dsc);
-  } else {
-    const int pbk = h->opts.pair_block;
-    const int g = (((h->n + pbk - 1) / pbk + 7) / 8) * 8;
-    const bool one = h->nscenes == 1, lane = h->capped;
-#define UPH_COMPACT(PBV, ONEV, LANEV)                                                                        \
-  hipLaunchKernelGGL((k_pair_compact<PBV, KT, MODE, ONEV, LANEV>), dim3(g), dim3(PBV), 0, st, h->n, h->b, h->dsc)
-#define UPH_COMPACT_PB(ONEV, LANEV)              \
-  do {                                           \
-    if (pbk == 128) UPH_COMPACT(128, ONEV, LANEV); \
-    else if (pbk == 512) UPH_COMPACT(512, ONEV, LANEV); \
-    else UPH_COMPACT(256, ONEV, LANEV);          \
-  } while (0)
-    if (one && lane) UPH_COMPACT_PB(true, true);
-    else if (one) UPH_COMPACT_PB(true, false);
-    else if (lane) UPH_COMPACT_PB(false, true);
-    else UPH_COMPACT_PB(false, false);
-#undef UPH_COMPACT_PB
-#undef UPH_COMPACT
-    if (h->nmixed > 0)
-      hipLaunchKernelGGL((k_pair_mixed<KT, MODE>), dim3(h->nmixed), dim3(64), 0, st, h->n, h->b, h->dsc,
-                         h->mixedWaves);
-  }
-  if (pb) HIPCHK(hipEventRecord(pb, st), h->err);
-  const int gfix = std::min(grid8_of(h->n), FIX_GRID);
-  if (h->nscenes == 1)
-    hipLaunchKernelGGL((k_pair<true, KT, MODE, true>), dim3(gfix), dim3(BLK), 0, st, h->n, h->b, h->dsc);
-  else
-    hipLaunchKernelGGL((k_pair<true, KT, MODE>), dim3(gfix), dim3(BLK), 0, st, h->n, h->b, h->dsc);
-  if (pc) HIPCHK(hipEventRecord(pc, st), h->err);
+// The re-sort a lazy handle's last call held over, at the first step of a call: each
+// kernel returns at once unless C_PEND is set.
+static int launch_held_sort(uph_handle* h, hipStream_t st) {
+  StepBufs& b = h->b;
+  const int* gate = b.ctl + C_PEND;
+  const int n = h->n, nb = h->cs.nb;
+  hipLaunchKernelGGL(k_bin_held, dim3(grid_of(n)), dim3(BLK), 0, st, n, b, h->dsc);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_BLK), 0, st, h->ncells, b.cellCount, b.partial,
+                     (int*)nullptr, (int*)nullptr, 0, 0, gate);
+  hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(SCAN_BLK), 0, st, h->ncells, b.cellCount, b.partial, b.cellStart,
+                     gate);
+  hipLaunchKernelGGL(k_scatter, dim3(grid_of(n)), dim3(BLK), 0, st, n, b.keys, b.rank, b.cellStart, b.tmpIdx, gate);
+  hipLaunchKernelGGL(k_permute_held, dim3(grid_of(n)), dim3(BLK), 0, st, n, b, (int)(h->mode == UPH_MODE_REF));
+  HIPCHK(hipGetLastError(), h->err);
   return 0;
+}
+
+// The kernels after a lazy step's pair pass (hold: the last step of a call).
+template <int KT, int MODE, bool ONE>
+static int launch_post_kt(uph_handle* h, hipStream_t st, int hold) {
+  StepBufs& b = h->b;
+  const int n = h->n, nb = h->cs.nb, g = std::min(grid8_of(n), FIX_GRID);
+  hipLaunchKernelGGL((k_post<KT, MODE, ONE>), dim3(g), dim3(BLK), 0, st, n, hold, b, h->dsc);
+  hipLaunchKernelGGL(k_lscan_reduce, dim3(nb), dim3(SCAN_BLK), 0, st, h->ncells, b.cellCount, b.partial,
+                     (const int*)b.ctl, hold);
+  hipLaunchKernelGGL(k_lscan_down, dim3(nb), dim3(SCAN_BLK), 0, st, h->ncells, b.cellCount, b.partial, b.cellStart,
+                     (const int*)b.ctl, hold);
+  hipLaunchKernelGGL(k_lscatter, dim3(g), dim3(BLK), 0, st, n, b.keys, b.rank, b.cellStart, b.tmpIdx,
+                     (const int*)b.ctl, hold);
+  hipLaunchKernelGGL(k_close, dim3(g), dim3(BLK), 0, st, n, hold, b);
+  HIPCHK(hipGetLastError(), h->err);
+  return 0;
+}
+
+template <int KT, int MODE>
+static int launch_post(uph_handle* h, hipStream_t st, int hold) {
+  return h->nscenes == 1 ? launch_post_kt<KT, MODE, true>(h, st, hold) : launch_post_kt<KT, MODE, false>(h, st, hold);
 }
 
 // The mirror rows of this step and their cell list (scenes with a virtualType that
@@ -501,35 +604,81 @@ static int launch_ghosts(uph_handle* h, hipStream_t st) {
   StepBufs& b = h->b;
   const int wnb = (b.wtot + SCAN_TILE - 1) / SCAN_TILE;
   hipLaunchKernelGGL(k_ghost_count, dim3(grid_of(b.wtot)), dim3(BLK), 0, st, b);
-  hipLaunchKernelGGL(k_scan_reduce, dim3(wnb), dim3(SCAN_BLK), 0, st, b.wtot, b.gwcnt, b.gpart, (int*)nullptr);
-  hipLaunchKernelGGL(k_scan_down, dim3(wnb), dim3(SCAN_BLK), 0, st, b.wtot, b.gwcnt, b.gpart, b.gwstart);
+  if (launch_scan(st, b.wtot, wnb, b.gwcnt, b.gpart, b.gwstart)) return -1;
   hipLaunchKernelGGL(k_ghost_emit, dim3(std::min(grid_of(h->gfluid), 256)), dim3(BLK), 0, st, b, h->dsc);
   hipLaunchKernelGGL(k_ghost_bin, dim3(grid_of(b.gtot)), dim3(BLK), 0, st, b, h->dsc);
-  hipLaunchKernelGGL(k_scan_reduce, dim3(h->cs.nb), dim3(SCAN_BLK), 0, st, h->ncells, b.gcellCount, b.partial,
-                     (int*)nullptr);
-  hipLaunchKernelGGL(k_scan_down, dim3(h->cs.nb), dim3(SCAN_BLK), 0, st, h->ncells, b.gcellCount, b.partial,
-                     b.gcellStart);
+  if (launch_scan(st, h->ncells, h->cs.nb, b.gcellCount, b.partial, b.gcellStart)) return -1;
   hipLaunchKernelGGL(k_ghost_scatter, dim3(grid_of(b.gtot)), dim3(BLK), 0, st, b);
   HIPCHK(hipGetLastError(), h->err);
   return 0;
 }
 
-// pa, pb, pc (may be null): events recorded after the sort, after the compacted pair
-// pass and after the capped pass.
-int launch_step(uph_handle* h, hipEvent_t pa, hipEvent_t pb, hipEvent_t pc) {
+template <int KT, int MODE>
+static int launch_step_kt(uph_handle* h, int k, int n_steps, hipEvent_t pa, hipEvent_t pb, hipEvent_t pc) {
   hipStream_t st = h->stream;
-  if (launch_sort(st, h->n, h->ncells, h->cs.nb, h->b, h->dsc, 0, true, h->err)) return -1;
-  if (h->b.gtot > 0 && launch_ghosts(h, st)) return -1;
+  if (!h->b.lazy) {
+    if (launch_sort(st, h->n, h->ncells, h->cs.nb, h->b, h->dsc, 0, true, h->err)) return -1;
+    if (h->b.gtot > 0 && launch_ghosts(h, st)) return -1;
+  } else if (h->need_sort) {
+    if (launch_sort(st, h->n, h->ncells, h->cs.nb, h->b, h->dsc, 0, true, h->err, h->sort_view)) return -1;
+    h->need_sort = false;
+  } else if (k == 0) {
+    // the held re-sort's kernels return at once when none is pending; with the stream
+    // idle the flag is read here instead of launching them
+    int pend = 1;
+    if (hipStreamQuery(st) == hipSuccess &&
+        hipMemcpy(&pend, h->b.ctl + C_PEND, sizeof pend, hipMemcpyDeviceToHost) != hipSuccess)
+      pend = 1;
+    if (pend && launch_held_sort(h, st)) return -1;
+  }
   if (pa) HIPCHK(hipEventRecord(pa, st), h->err);
+  if (launch_pair_pass(h, st)) return -1;
+  if (pb) HIPCHK(hipEventRecord(pb, st), h->err);
+  if (h->b.lazy ? launch_post<KT, MODE>(h, st, (int)(k == n_steps - 1)) : launch_capped(h, st)) return -1;
+  if (pc) HIPCHK(hipEventRecord(pc, st), h->err);
+  return 0;
+}
+
+// Step k of a call of n_steps steps.  pa, pb, pc (may be null): events recorded after
+// the sort, after the pair pass and after the capped pass (k_post for a lazy handle).
+int launch_call_step(uph_handle* h, int k, int n_steps, hipEvent_t pa, hipEvent_t pb, hipEvent_t pc) {
   int rc;
   if (h->mode == UPH_MODE_REF)
-    rc = h->kernel == UPH_KERNEL_POLY7 ? launch_pairs<UPH_KERNEL_POLY7, UPH_MODE_REF>(h, st, pb, pc)
-                                       : launch_pairs<UPH_KERNEL_CUBIC, UPH_MODE_REF>(h, st, pb, pc);
+    rc = h->kernel == UPH_KERNEL_POLY7 ? launch_step_kt<UPH_KERNEL_POLY7, UPH_MODE_REF>(h, k, n_steps, pa, pb, pc)
+                                       : launch_step_kt<UPH_KERNEL_CUBIC, UPH_MODE_REF>(h, k, n_steps, pa, pb, pc);
   else
-    rc = h->kernel == UPH_KERNEL_POLY7 ? launch_pairs<UPH_KERNEL_POLY7, UPH_MODE_CONFIG>(h, st, pb, pc)
-                                       : launch_pairs<UPH_KERNEL_CUBIC, UPH_MODE_CONFIG>(h, st, pb, pc);
+    rc = h->kernel == UPH_KERNEL_POLY7 ? launch_step_kt<UPH_KERNEL_POLY7, UPH_MODE_CONFIG>(h, k, n_steps, pa, pb, pc)
+                                       : launch_step_kt<UPH_KERNEL_CUBIC, UPH_MODE_CONFIG>(h, k, n_steps, pa, pb, pc);
   if (rc != 0) return -1;
   HIPCHK(hipGetLastError(), h->err);
+  return 0;
+}
+
+// One step of a sort-every-step handle (the hipGraph capture, the derivatives() seam).
+int launch_step(uph_handle* h, hipEvent_t pa, hipEvent_t pb, hipEvent_t pc) {
+  return launch_call_step(h, 0, 1, pa, pb, pc);
+}
+
+int read_bufs(uph_t* h, StepBufs& out) {
+  out = h->b;
+  if (!h->b.lazy) return 0;
+  HIPCHK(hipStreamSynchronize(h->stream), h->err);
+  int v = 0;
+  HIPCHK(hipMemcpy(&v, h->b.ctl + C_VC, sizeof v, hipMemcpyDeviceToHost), h->err);
+  if (v & 1) out = swapped(h->b);
+  out.it = h->b.ctl + C_ITC;   // the last step's itime and K-th key stamp
+  return 0;
+}
+
+int lazy_reset(uph_t* h) {
+  if (!h->b.lazy) return 0;
+  HIPCHK(hipStreamSynchronize(h->stream), h->err);
+  int v = 0;
+  HIPCHK(hipMemcpy(&v, h->b.ctl + C_VC, sizeof v, hipMemcpyDeviceToHost), h->err);
+  h->sort_view = v & 1;
+  h->need_sort = true;
+  // k_post left the next step's itime; the sort that opens that step counts it again
+  HIPCHK(hipMemcpy(h->b.ctl + C_IT, &h->itime, sizeof(int), hipMemcpyHostToDevice), h->err);
   return 0;
 }
 
@@ -547,6 +696,20 @@ extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts*
   if (!(h->opts.cell_factor >= 1.0) || h->opts.cell_factor > 4.0) h->opts.cell_factor = 1.0;
   h->nscenes = n_scenes;
   h->hs.resize(n_scenes);
+  // lazy re-sort where it applies: no scene with mirror rows (the sort evaluates their
+  // section tests every step), none expected to be capped, no hipGraph (the step's
+  // launches depend on the call); its cells grow by the skin
+  bool any_mirrors = false, any_capped = false;
+  for (int s = 0; s < n_scenes; ++s) {
+    const uph_scene& u = scenes[s];
+    int sec[5];
+    if (u.n_fluid > 0 && mirror_sections(u.vtype, sec) > 0) any_mirrors = true;
+    if (!(4.0 * M_PI * (u.h / u.dx) * (u.h / u.dx) <= 0.75 * u.K)) any_capped = true;
+  }
+  const bool lazy = !h->opts.sort_every_step && !h->opts.use_graph && !any_mirrors && !any_capped;
+  if (!(h->opts.skin > 0.0)) h->opts.skin = 0.05;
+  if (h->opts.skin > 1.0) h->opts.skin = 1.0;
+  const double cell_factor = lazy ? h->opts.cell_factor * (1.0 + h->opts.skin) : h->opts.cell_factor;
   long n = 0, nc = 0, gtot = 0, wtot = 0, gfluid = 0;
   for (int s = 0; s < n_scenes; ++s) {
     const uph_scene& u = scenes[s];
@@ -581,8 +744,9 @@ extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts*
         delete h;
         return -1;
       }
-    fill_scene(h->hs[s], u, h->opts.cell_factor);
+    fill_scene(h->hs[s], u, cell_factor);
     DevScene& S = h->hs[s];
+    S.sidx = s;
     S.poff = (int)n;
     S.coff = (int)nc;
     S.pb_gid = (int)n + (u.n_fluid + u.n_fixed + u.n_moving - 1);
@@ -631,6 +795,10 @@ extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts*
     return fail();
   }
   if (alloc_bufs(h->b, h->n, h->ncells, h->cs.nb, err, false, h->capped)) return fail();
+  if (lazy) {
+    if (alloc_lazy_bufs(h->b, h->n, n_scenes, err)) return fail();
+    if (h->mode == UPH_MODE_REF) h->b.lazy = 2;   // the re-sort's gather carries DynamicPressure and mC
+  }
   h->gfluid = (int)gfluid;
   if (gtot > 0 && alloc_ghost_bufs(h->b, h->n, h->ncells, (int)gtot, (int)wtot, err)) return fail();
   if (h->opts.keep_rates && dalloc(&h->b.dbgRates, (h->mode == UPH_MODE_REF ? 4 : 2) * (size_t)h->n, err))
@@ -711,7 +879,7 @@ extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts*
     for (int i = 0; i < h->hs[s].np; ++i) vdeck[h->hs[s].poff + i] = make_double2(u.vel[2 * i], u.vel[2 * i + 1]);
   }
   h->deck_vel = vdeck;
-  int ctl[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int ctl[CTL_WORDS] = {0};
   if (hipMemcpy(h->b.deckVel, vdeck.data(), n * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(h->b.deckLvel, lv.data(), n * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(h->b.massD, massv.data(), n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
@@ -792,7 +960,7 @@ extern "C" int uph_step(uph_t* h, int n_steps) {
     for (int k = 0; k < n_steps; ++k) HIPCHK(hipGraphLaunch(h->gexec, h->stream), h->err);
   } else {
     for (int k = 0; k < n_steps; ++k)
-      if (launch_step(h, nullptr, nullptr)) return -1;
+      if (launch_call_step(h, k, n_steps, nullptr, nullptr, nullptr)) return -1;
   }
   h->itime += n_steps;
   return 0;
@@ -840,9 +1008,9 @@ extern "C" int uph_step_persistent(uph_t* h, int n_steps) {
     h->err.msg = "uph_step_persistent: negative step count";
     return -1;
   }
-  if (h->b.gtot > 0 || h->capped || h->opts.pair_variant != 0) {
+  if (h->b.gtot > 0 || h->capped || h->opts.pair_variant != 0 || h->b.lazy) {
     h->err.msg = "uph_step_persistent: for handles without mirror rows, not expected to be capped, with the "
-                 "compacted pair kernel (use uph_step)";
+                 "compacted pair kernel, created to re-sort every step (use uph_step)";
     return -1;
   }
   if (n_steps == 0) return 0;
@@ -881,18 +1049,27 @@ extern "C" int uph_sync(uph_t* h) {
       return -1;
     }
   }
-  int ctl[8];
+  int ctl[CTL_WORDS];
   HIPCHK(hipMemcpy(ctl, h->b.ctl, sizeof ctl, hipMemcpyDeviceToHost), h->err);
-  if (ctl[4]) {
+  if (ctl[C_BROKEN]) {
     h->err.msg = "uph_sync: a particle had more than " + std::to_string(CAND_MAX) +
                  " candidates in its 3x3 cells: the state has blown up (particles escaped or collapsed "
                  "into one cell); the step's results are not valid";
     return -1;
   }
-  return ctl[2];
+  return ctl[C_NOVF];
 }
 
 extern "C" int uph_itime(const uph_t* h) { return h ? h->itime : -1; }
+
+extern "C" int uph_resorts(uph_t* h) {
+  if (!h) return -1;
+  if (!h->b.lazy) return -1;
+  HIPCHK(hipStreamSynchronize(h->stream), h->err);
+  int v = 0;
+  HIPCHK(hipMemcpy(&v, h->b.ctl + C_NRES, sizeof v, hipMemcpyDeviceToHost), h->err);
+  return v;
+}
 
 extern "C" int uph_time_steps(uph_t* h, int n_steps, double* ms_total, double* ms_pair) {
   if (!h || n_steps <= 0) return -1;
@@ -901,7 +1078,7 @@ extern "C" int uph_time_steps(uph_t* h, int n_steps, double* ms_total, double* m
   HIPCHK(hipStreamSynchronize(h->stream), h->err);
   HIPCHK(hipEventRecord(h->ev0, h->stream), h->err);
   for (int k = 0; k < n_steps; ++k)
-    if (launch_step(h, ev[2 * k], ev[2 * k + 1])) return -1;
+    if (launch_call_step(h, k, n_steps, ev[2 * k], ev[2 * k + 1], nullptr)) return -1;
   HIPCHK(hipEventRecord(h->ev1, h->stream), h->err);
   HIPCHK(hipEventSynchronize(h->ev1), h->err);
   float tot = 0.f;
@@ -937,9 +1114,9 @@ extern "C" int uph_time_phases(uph_t* h, int n_steps, double* ms_total, double* 
   if (rc == 0 && hipEventRecord(h->ev0, h->stream) != hipSuccess) rc = -1;
   for (int k = 0; rc == 0 && k < n_steps; ++k) {
     if (!split) {
-      if (launch_step(h, nullptr, nullptr, nullptr)) rc = -1;
+      if (launch_call_step(h, k, n_steps, nullptr, nullptr, nullptr)) rc = -1;
     } else if (hipEventRecord(ev[4 * k], h->stream) != hipSuccess ||
-               launch_step(h, ev[4 * k + 1], ev[4 * k + 2], ev[4 * k + 3])) {
+               launch_call_step(h, k, n_steps, ev[4 * k + 1], ev[4 * k + 2], ev[4 * k + 3])) {
       rc = -1;
     }
   }
@@ -1021,10 +1198,12 @@ extern "C" int uph_get_state(uph_t* h, int scene, double* pos, double* vel, doub
   std::vector<double2> p(S.np), v(S.np);
   std::vector<double> r(S.np);
   std::vector<uint32_t> t(S.np);
-  HIPCHK(hipMemcpy(p.data(), h->b.posB + S.poff, S.np * sizeof(double2), hipMemcpyDeviceToHost), h->err);
-  HIPCHK(hipMemcpy(v.data(), h->b.velB + S.poff, S.np * sizeof(double2), hipMemcpyDeviceToHost), h->err);
-  HIPCHK(hipMemcpy(r.data(), h->b.rhoB + S.poff, S.np * sizeof(double), hipMemcpyDeviceToHost), h->err);
-  HIPCHK(hipMemcpy(t.data(), h->b.tagA + S.poff, S.np * sizeof(uint32_t), hipMemcpyDeviceToHost), h->err);
+  StepBufs rb;
+  if (read_bufs(h, rb)) return -1;
+  HIPCHK(hipMemcpy(p.data(), rb.posB + S.poff, S.np * sizeof(double2), hipMemcpyDeviceToHost), h->err);
+  HIPCHK(hipMemcpy(v.data(), rb.velB + S.poff, S.np * sizeof(double2), hipMemcpyDeviceToHost), h->err);
+  HIPCHK(hipMemcpy(r.data(), rb.rhoB + S.poff, S.np * sizeof(double), hipMemcpyDeviceToHost), h->err);
+  HIPCHK(hipMemcpy(t.data(), rb.tagA + S.poff, S.np * sizeof(uint32_t), hipMemcpyDeviceToHost), h->err);
   for (int k = 0; k < S.np; ++k) {
     const int i = (int)((t[k] & UPH_ID_MASK) - (uint32_t)S.poff);
     if (i < 0 || i >= S.np) {
@@ -1069,10 +1248,12 @@ int get_rates_f64(uph_t* h, int scene, float* R, float* Rt, double* a, float* at
   const int st = h->mode == UPH_MODE_REF ? 4 : 2;   // float4s per slot
   std::vector<float4> d(st * (size_t)S.np);
   std::vector<uint32_t> t(S.np);
+  StepBufs rb;
+  if (read_bufs(h, rb)) return -1;
   HIPCHK(hipMemcpy(d.data(), h->b.dbgRates + st * (size_t)S.poff, st * S.np * sizeof(float4),
                    hipMemcpyDeviceToHost),
          h->err);
-  HIPCHK(hipMemcpy(t.data(), h->b.tagA + S.poff, S.np * sizeof(uint32_t), hipMemcpyDeviceToHost), h->err);
+  HIPCHK(hipMemcpy(t.data(), rb.tagA + S.poff, S.np * sizeof(uint32_t), hipMemcpyDeviceToHost), h->err);
   for (int k = 0; k < S.np; ++k) {
     const int i = (int)((t[k] & UPH_ID_MASK) - (uint32_t)S.poff);
     if (i < 0 || i >= S.np) continue;
@@ -1104,9 +1285,11 @@ extern "C" int uph_get_ref_rates(uph_t* h, int scene, float* RP, float* RPt, flo
   const DevScene& S = h->hs[scene];
   std::vector<float4> d(4 * (size_t)S.np);
   std::vector<uint32_t> t(S.np);
+  StepBufs rb;
+  if (read_bufs(h, rb)) return -1;
   HIPCHK(hipMemcpy(d.data(), h->b.dbgRates + 4 * (size_t)S.poff, 4 * S.np * sizeof(float4), hipMemcpyDeviceToHost),
          h->err);
-  HIPCHK(hipMemcpy(t.data(), h->b.tagA + S.poff, S.np * sizeof(uint32_t), hipMemcpyDeviceToHost), h->err);
+  HIPCHK(hipMemcpy(t.data(), rb.tagA + S.poff, S.np * sizeof(uint32_t), hipMemcpyDeviceToHost), h->err);
   for (int k = 0; k < S.np; ++k) {
     const int i = (int)((t[k] & UPH_ID_MASK) - (uint32_t)S.poff);
     if (i < 0 || i >= S.nf) continue;
@@ -1130,10 +1313,12 @@ extern "C" int uph_get_pressure(uph_t* h, int scene, double* P) {
   const DevScene& S = h->hs[scene];
   std::vector<double> v(S.np);
   std::vector<uint32_t> t(S.np);
-  HIPCHK(hipMemcpy(v.data(), (h->mode == UPH_MODE_REF ? h->b.presB : h->b.rhoB) + S.poff, S.np * sizeof(double),
+  StepBufs rb;
+  if (read_bufs(h, rb)) return -1;
+  HIPCHK(hipMemcpy(v.data(), (h->mode == UPH_MODE_REF ? rb.presB : rb.rhoB) + S.poff, S.np * sizeof(double),
                    hipMemcpyDeviceToHost),
          h->err);
-  HIPCHK(hipMemcpy(t.data(), h->b.tagA + S.poff, S.np * sizeof(uint32_t), hipMemcpyDeviceToHost), h->err);
+  HIPCHK(hipMemcpy(t.data(), rb.tagA + S.poff, S.np * sizeof(uint32_t), hipMemcpyDeviceToHost), h->err);
   for (int k = 0; k < S.np; ++k) {
     const int i = (int)((t[k] & UPH_ID_MASK) - (uint32_t)S.poff);
     if (i < 0 || i >= S.np) {

@@ -513,7 +513,7 @@ int main(int argc, char **argv) {
     free_deck(&fl); free_deck(&fx); free_deck(&mv); free_deck(&so);
     return 0;
   }
-  uph_opts opts = {use_graph, 0, 0, 256, 1.0};
+  uph_opts opts = {use_graph, 0, 0, 256, 1.0, 0, 0.0};
   uph_t *h = NULL;
   if (uph_create(&s, 1, &opts, &h) != 0) {
     fprintf(stderr, "uph_run: %s\n", uph_last_error(NULL));

@@ -47,7 +47,8 @@ class UphDiag(C.Structure):
 
 class UphOpts(C.Structure):
     _fields_ = [("use_graph", C.c_int), ("keep_rates", C.c_int), ("pair_variant", C.c_int),
-                ("pair_block", C.c_int), ("cell_factor", C.c_double)]
+                ("pair_block", C.c_int), ("cell_factor", C.c_double), ("sort_every_step", C.c_int),
+                ("skin", C.c_double)]
 
 
 # every entry point of include/uph.h: name -> (restype, argtypes)
@@ -75,6 +76,7 @@ SIGNATURES = {
     "uph_ghost_capacity": (C.c_int, [C.c_void_p, C.c_int]),
     "uph_derivatives_scheme": (C.c_int, [C.c_int, C.c_int]),
     "uph_itime": (C.c_int, [C.c_void_p]),
+    "uph_resorts": (C.c_int, [C.c_void_p]),
     "uph_time_steps": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp]),
     "uph_time_phases": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp]),
     "uph_last_error": (C.c_char_p, [C.c_void_p]),
@@ -214,7 +216,7 @@ class GpuSolver:
 
     def __init__(self, scenes: Sequence[Scene] | Scene, use_graph: bool = False,
                  keep_rates: bool = False, pair_variant: int = 0, pair_block: int = 256,
-                 cell_factor: float = 1.0):
+                 cell_factor: float = 1.0, sort_every_step: bool = False, skin: float = 0.0):
         if isinstance(scenes, Scene):
             scenes = [scenes]
         self.scenes = list(scenes)
@@ -253,9 +255,10 @@ class GpuSolver:
             u.vtype = int(getattr(sc, "vtype", 0))
             u.lbp, u.rbp = 0.0, 0.0
         opts = UphOpts(int(bool(use_graph)), int(bool(keep_rates)), int(pair_variant),
-                       int(pair_block), float(cell_factor))
+                       int(pair_block), float(cell_factor), int(bool(sort_every_step)), float(skin))
         self.opts = dict(use_graph=bool(use_graph), pair_variant=int(pair_variant),
-                         pair_block=int(pair_block), cell_factor=float(cell_factor))
+                         pair_block=int(pair_block), cell_factor=float(cell_factor),
+                         sort_every_step=bool(sort_every_step), skin=float(skin))
         h = C.c_void_p()
         rc = lib().uph_create(arr, len(self.scenes), C.byref(opts), C.byref(h))
         if rc != 0:
@@ -289,6 +292,10 @@ class GpuSolver:
     @property
     def itime(self) -> int:
         return lib().uph_itime(self.h)
+
+    def resorts(self) -> int:
+        """Re-sorts of a lazy handle so far (waits); -1 for a handle that sorts every step."""
+        return lib().uph_resorts(self.h)
 
     def _scene(self, scene: int):
         if not 0 <= scene < len(self.scenes):
