@@ -260,12 +260,13 @@ def main():
                                    "particle-updates/s of the whole step",
                      "step_achieved": step_achieved, "step_frac": step_achieved / HBM_PEAK_GBS,
                      # the committed counter run covers the default workload only
-                     "traffic": (traffic_from_profiles() if args.kernel == "cubic" and
+                     "traffic": (traffic_from_profiles() if args.kernel == "cubic" and lazy and
                                  args.h_over_dx == 1.3 and args.scheme == "config" and not args.vtype
-                                 else None),
+                                 and args.skin <= 0 else None),
                      "kernel": f"k_pair_compact<256, {int(sc.kernel)}, {int(sc.mode)}, true, "
-                               f"{'true' if 4 * math.pi * args.h_over_dx ** 2 > 0.75 * sc.K else 'false'}> "
-                               f"({args.kernel}, {args.scheme}, one scene)",
+                               f"{'true' if 4 * math.pi * args.h_over_dx ** 2 > 0.75 * sc.K else 'false'}, "
+                               f"{'true' if lazy else 'false'}> ({args.kernel}, {args.scheme}, one scene"
+                               f"{', lazy re-sort' if lazy else ''})",
                      "kernel_ms": ms_pair, "sort_ms": ph["sort"], "capped_pass_ms": ph["exact"],
                      "capped_pass_kernel": post},
     }
