@@ -88,10 +88,11 @@ __device__ __forceinline__ void bin_block(int blk, int n, const double2* __restr
   int key = -1;
   if (valid) {
     const int s = sceneOf[i];
+    const DevScene& S = sc[s];
     if (keep && keep[s] == 0) {
-      key = cellOf[i];
+      const int pk = cellOf[i];
+      key = S.coff + (pk >> 16) * S.ncx + (pk & 0xffff);
     } else {
-      const DevScene& S = sc[s];
       const double2 p = pos[i];
       int cx, cy;
       cell_of(S, p.x, p.y, cx, cy);
@@ -278,7 +279,8 @@ struct StepBufs {
   // key) each slot was binned into at the last sort.  sstale: per scene and itime
   // parity, whether one of its particles left its binned cell by more than the margin.
   // `it` points at {itime, stamp} of the step (the pair of words C_IT, C_STAMP, or their
-  // copies C_ITC, C_STC inside k_post).
+  // copies C_ITC, C_STC inside k_post).  cellOf holds the binned cell's coordinates
+  // packed as cx | cy << 16 (cell_pack).
   float4 *mA2, *mB2, *rtA, *rt2A;
   float2 *mC2, *dvA;
   int* cellOf;
@@ -337,6 +339,13 @@ struct StepBufs {
   double2 *predV, *predVl;
   double *predRho, *predRl, *predP, *predPl;
 };
+
+// A cell's coordinates in one int (cellOf), and from its global key in scene S.
+__host__ __device__ __forceinline__ int cell_pack(int cx, int cy) { return cx | (cy << 16); }
+__device__ __forceinline__ int cell_pack_key(const DevScene& S, int key) {
+  const int r = key - S.coff;
+  return cell_pack(r % S.ncx, r / S.ncx);
+}
 
 // The buffers with the roles of the A and B state sets exchanged (a lazy handle's
 // view 1).
@@ -572,8 +581,8 @@ __device__ __forceinline__ void reorder_slot(int d, int n, const StepBufs& b, co
   b.posA[dst] = p;
   b.tagA[dst] = tg;
   if (search_only) return;
-  if (b.cellOf) b.cellOf[dst] = c;
   const DevScene& S = sc[b.sceneOf[i]];
+  if (b.cellOf) b.cellOf[dst] = cell_pack_key(S, c);
   const double2 v = b.velB[i];
   const double rho = b.rhoB[i];
   const float4 rt = b.rtB[i];
@@ -606,7 +615,8 @@ __device__ __forceinline__ void reorder_slot(int d, int n, const StepBufs& b, co
 // A lazy re-sort's reorder: the slot's final position as reorder_slot finds it, and
 // a plain gather of the B set into the A set (the pair epilogue has already predicted
 // the mirrors); the slot's cell goes to cellOf.
-__device__ __forceinline__ void permute_slot(int d, int n, const StepBufs& b, bool ref) {
+__device__ __forceinline__ void permute_slot(int d, int n, const StepBufs& b, const DevScene* __restrict__ sc,
+                                             bool ref) {
   if (d >= n) return;
   const int i = b.tmpIdx[d];
   const int c = b.keys[i];
@@ -631,7 +641,7 @@ __device__ __forceinline__ void permute_slot(int d, int n, const StepBufs& b, bo
     b.presA[dst] = b.presB[i];
     b.mC[dst] = b.mC2[i];
   }
-  b.cellOf[dst] = c;
+  b.cellOf[dst] = cell_pack_key(sc[b.sceneOf[i]], c);
 }
 
 // ------------------------------------------------------ mirror particles
@@ -1593,18 +1603,12 @@ __device__ __forceinline__ void lazy_out(const DevScene& S, const StepBufs& b, i
   b.mA2[p] = ma;
   b.mB2[p] = mb;
   if (S.mode == 1) b.mC2[p] = mc;
-  int cx, cy;
-  cell_of(S, x1.x, x1.y, cx, cy);
-  const int bk = b.cellOf[p];
-  bool stale = false;
-  if (S.coff + cy * S.ncx + cx != bk) {
-    const int r = bk - S.coff;
-    const int bx = r % S.ncx, by = r / S.ncx;
-    const double u = (x1.x - S.xmin) * S.cell_inv, w = (x1.y - S.ymin) * S.cell_inv;
-    const double m = S.skin_m;
-    stale = (bx > 0 && u < bx - m) || (bx < S.ncx - 1 && u >= bx + 1 + m) || (by > 0 && w < by - m) ||
-            (by < S.ncy - 1 && w >= by + 1 + m);
-  }
+  const int pk = b.cellOf[p];
+  const int bx = pk & 0xffff, by = pk >> 16;
+  const double u = (x1.x - S.xmin) * S.cell_inv, w = (x1.y - S.ymin) * S.cell_inv;
+  const double m = S.skin_m;
+  const bool stale = (bx > 0 && u < bx - m) || (bx < S.ncx - 1 && u >= bx + 1 + m) || (by > 0 && w < by - m) ||
+                     (by < S.ncy - 1 && w >= by + 1 + m);
   const unsigned long long sm = __ballot(stale);
   if (sm == 0ull) return;
   const int par = itime & 1;

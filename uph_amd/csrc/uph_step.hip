@@ -55,10 +55,10 @@ __global__ __launch_bounds__(BLK) void k_bin_held(int n, StepBufs b, const DevSc
   bin_block(blockIdx.x, n, v.posB, b.sceneOf, sc, nullptr, nullptr, b.keys, b.rank, b.cellCount, keep, b.cellOf);
 }
 
-__global__ __launch_bounds__(BLK) void k_permute_held(int n, StepBufs b, int ref) {
+__global__ __launch_bounds__(BLK) void k_permute_held(int n, StepBufs b, const DevScene* __restrict__ sc, int ref) {
   if (b.ctl[C_PEND] == 0) return;
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&b.ctl[C_NRES], 1);
-  permute_slot(blockIdx.x * BLK + threadIdx.x, n, lazy_view(b), ref != 0);
+  permute_slot(blockIdx.x * BLK + threadIdx.x, n, lazy_view(b), sc, ref != 0);
 }
 
 // Popcount of each word of section-test bits (the scan's input); the words move to
@@ -350,7 +350,7 @@ __global__ __launch_bounds__(BLK) void k_lscatter(int n, const int* __restrict__
   for (int i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK) tmpIdx[cellStart[keys[i]] + rank[i]] = i;
 }
 
-__global__ __launch_bounds__(BLK) void k_close(int n, int hold, StepBufs b0) {
+__global__ __launch_bounds__(BLK) void k_close(int n, int hold, StepBufs b0, const DevScene* __restrict__ sc) {
   int* ctl = b0.ctl;
   const int itime = ctl[C_ITC];
   const int par = itime & 1;
@@ -359,7 +359,7 @@ __global__ __launch_bounds__(BLK) void k_close(int n, int hold, StepBufs b0) {
   if (resort) {
     const StepBufs b = vc ? swapped(b0) : b0;
     const bool ref = b.lazy == 2;
-    for (int d = blockIdx.x * BLK + threadIdx.x; d < n; d += gridDim.x * BLK) permute_slot(d, n, b, ref);
+    for (int d = blockIdx.x * BLK + threadIdx.x; d < n; d += gridDim.x * BLK) permute_slot(d, n, b, sc, ref);
   }
   if (blockIdx.x == 0) {
     if (threadIdx.x == 0) {
@@ -570,7 +570,8 @@ static int launch_held_sort(uph_handle* h, hipStream_t st) {
   hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(SCAN_BLK), 0, st, h->ncells, b.cellCount, b.partial, b.cellStart,
                      gate);
   hipLaunchKernelGGL(k_scatter, dim3(grid_of(n)), dim3(BLK), 0, st, n, b.keys, b.rank, b.cellStart, b.tmpIdx, gate);
-  hipLaunchKernelGGL(k_permute_held, dim3(grid_of(n)), dim3(BLK), 0, st, n, b, (int)(h->mode == UPH_MODE_REF));
+  hipLaunchKernelGGL(k_permute_held, dim3(grid_of(n)), dim3(BLK), 0, st, n, b, (const DevScene*)h->dsc,
+                     (int)(h->mode == UPH_MODE_REF));
   HIPCHK(hipGetLastError(), h->err);
   return 0;
 }
@@ -587,7 +588,7 @@ static int launch_post_kt(uph_handle* h, hipStream_t st, int hold) {
                      (const int*)b.ctl, hold);
   hipLaunchKernelGGL(k_lscatter, dim3(g), dim3(BLK), 0, st, n, b.keys, b.rank, b.cellStart, b.tmpIdx,
                      (const int*)b.ctl, hold);
-  hipLaunchKernelGGL(k_close, dim3(g), dim3(BLK), 0, st, n, hold, b);
+  hipLaunchKernelGGL(k_close, dim3(g), dim3(BLK), 0, st, n, hold, b, (const DevScene*)h->dsc);
   HIPCHK(hipGetLastError(), h->err);
   return 0;
 }
@@ -706,7 +707,7 @@ extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts*
     if (u.n_fluid > 0 && mirror_sections(u.vtype, sec) > 0) any_mirrors = true;
     if (!(4.0 * M_PI * (u.h / u.dx) * (u.h / u.dx) <= 0.75 * u.K)) any_capped = true;
   }
-  const bool lazy = !h->opts.sort_every_step && !h->opts.use_graph && !any_mirrors && !any_capped;
+  bool lazy = !h->opts.sort_every_step && !h->opts.use_graph && !any_mirrors && !any_capped;
   if (!(h->opts.skin > 0.0)) h->opts.skin = 0.05;
   if (h->opts.skin > 1.0) h->opts.skin = 1.0;
   const double cell_factor = lazy ? h->opts.cell_factor * (1.0 + h->opts.skin) : h->opts.cell_factor;
@@ -747,6 +748,7 @@ extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts*
     fill_scene(h->hs[s], u, cell_factor);
     DevScene& S = h->hs[s];
     S.sidx = s;
+    if (S.ncx > 65535 || S.ncy > 65535) lazy = false;   // cellOf packs a cell's coordinates in 16 bits each
     S.poff = (int)n;
     S.coff = (int)nc;
     S.pb_gid = (int)n + (u.n_fluid + u.n_fixed + u.n_moving - 1);
