@@ -175,7 +175,8 @@ def main():
                          "which re-sorts a scene only when a particle has left its cell by more than the "
                          "cells' growth)")
     ap.add_argument("--skin", type=float, default=0.0,
-                    help="lazy re-sort: cells grow by this fraction (0: the library's default, 0.05)")
+                    help="lazy re-sort: cells grow by this fraction (0: the library's default, 0.02 for one "
+                         "scene)")
     ap.add_argument("--compare-launch", action="store_true",
                     help="only compare per-kernel launches, hipGraph replay and the persistent kernel "
                          "(compare_launch) and exit")
@@ -208,7 +209,7 @@ def main():
         sc.vtype, sc.flat_bottom, sc.depth = args.vtype, 4.0, 2.0   # the tank's width, the column's height
     log(f"workload: {sc.n_fluid} fluid + {sc.n_boundary} wall particles, {args.kernel} kernel, "
         f"h = {args.h_over_dx} dx, {args.scheme} scheme, virtualType {args.vtype}")
-    g = GpuSolver(sc, sort_every_step=args.sort_every_step, skin=args.skin)
+    g = GpuSolver(sc, sort_every_step=True if args.sort_every_step else None, skin=args.skin)
     g.step(args.settle + args.warmup)
     g.sync()
     lazy = g.resorts() >= 0
@@ -250,7 +251,7 @@ def main():
                    "kernel": args.kernel, "scheme": args.scheme, "h_over_dx": args.h_over_dx, "dt": sc.dt,
                    "virtual_type": args.vtype, "capped_steps": overflow,
                    "settle_steps": args.settle,
-                   "resort": (f"lazy, cells grown by {args.skin if args.skin > 0 else 0.05:g}" if lazy
+                   "resort": (f"lazy, cells grown by {g.skin():g}" if lazy
                               else "every step"),
                    "resorts_in_timed_steps": resorts_timed},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,

@@ -158,12 +158,15 @@ typedef struct uph_opts {
   int pair_variant;     /* 0: compacted two-pass pair kernel (default), 1: direct loop */
   int pair_block;       /* work-group size of the compacted pair kernel: 128, 256 (default), 512 */
   double cell_factor;   /* cell edge = cell_factor x search radius, 1 <= f <= 4 (default 1) */
-  int sort_every_step;  /* 1: re-sort the particles into their cells every step; 0 (default):
-                           lazy re-sort, where the implementation allows it (no mirror rows, no
+  int sort_every_step;  /* 1: re-sort the particles into their cells every step; -1: lazy
+                           re-sort, where the implementation allows it (no mirror rows, no
                            scene expected to be capped, no hipGraph): cells grow by the skin
-                           and a scene is re-sorted only when one of its particles has left
-                           its cell by more than the growth; pair sets are unchanged */
-  double skin;          /* lazy re-sort: cell edge x (1 + skin); <= 0: 0.05 (at most 1) */
+                           and the particles are re-sorted only when one of them has left its
+                           cell by more than the growth; pair sets are unchanged.  0 (default):
+                           lazy for handles of at most 16 scenes, every step for larger
+                           batches (a batch re-sorts when any of its scenes needs it) */
+  double skin;          /* lazy re-sort: cell edge x (1 + skin); <= 0: 0.02 for one scene,
+                           0.05 for a batch (at most 1) */
 } uph_opts;
 
 typedef struct uph_handle uph_t;
@@ -221,7 +224,11 @@ int uph_get_ref_rates(uph_t *h, int scene, float *RP, float *RPt, float *drho, f
 /* Completed steps. */
 int uph_itime(const uph_t *h);
 
-/* Lazy re-sort (uph_opts.sort_every_step = 0): the re-sorts run so far, counting the
+/* The cells' growth of a lazy re-sort handle (uph_opts.skin after its defaults), 0 for a
+ * handle that re-sorts every step.  No counterpart in the reference. */
+double uph_skin(const uph_t *h);
+
+/* Lazy re-sort (uph_opts.sort_every_step <= 0): the re-sorts run so far, counting the
  * first step's sort and one after each state set from outside (waits for the
  * handle's stream); -1 for a handle that re-sorts every step.  No counterpart in the
  * reference, whose kd-tree is rebuilt from scratch every step (Search.cc:5-44). */

@@ -99,15 +99,16 @@ def test_batch_of_512_dam_breaks():
 
 
 def test_batch_of_512_dam_breaks_lazy_resort():
-    """The same batch with the lazy re-sort (eager launches): 300 steps, a few scenes
-    re-sorted by then; sampled scenes equal their own single-scene lazy runs bit for bit
-    (each scene's re-sorts depend on its own particles only), the smallest matches the
-    oracle."""
+    """The same batch with the lazy re-sort asked for (the default sorts a batch of this
+    size every step), eager launches: 300 steps with re-sorts; sampled scenes equal
+    their own single-scene lazy runs with the same skin bit for bit (each scene's
+    re-sorts depend on its own particles only), the smallest matches the oracle."""
     from oracle.oracle import Sim
     from uph_amd.solver import GpuSolver
     scs = scenes.batch_dam_breaks()
     _say("lazy batch of 512: 300 steps")
-    gb = GpuSolver(scs)
+    gb = GpuSolver(scs, sort_every_step=False, skin=0.05)
+    assert gb.skin() == 0.05
     gb.step(300)
     assert gb.sync() == 0
     nres = gb.resorts()
@@ -115,7 +116,7 @@ def test_batch_of_512_dam_breaks_lazy_resort():
     assert nres >= 1
     rng = np.random.default_rng(1)
     for k in [int(np.argmin([s.n_fluid for s in scs]))] + [int(j) for j in rng.choice(len(scs), 3, replace=False)]:
-        gs = GpuSolver(scs[k])
+        gs = GpuSolver(scs[k], skin=0.05)
         gs.step(300)
         a, b = gb.state(k), gs.state(0)
         for key in ("x", "v", "rho"):

@@ -109,6 +109,7 @@ WORKLOADS = {
     "dam1m": lambda: _scenes.dam_break(dx=2 ** 0.5 * 1e-3),          # 999,698 fluid
     "batch512": lambda: _scenes.batch_dam_breaks(),                   # 512 scenes, 5.47 M fluid
     "dam5k": lambda: _scenes.dam_break(),
+    "wave240k": lambda: _scenes.wave_tank(),                          # 237,900 fluid, piston
 }
 
 
@@ -123,6 +124,15 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cache", default=None)
+    ap.add_argument("--no-tune", action="store_true",
+                    help="only time the library's defaults (for a kernel trace of the workload)")
+    ap.add_argument("--sort-every-step", action="store_true",
+                    help="time handles that re-sort every step (default: the library's choice, lazy "
+                         "re-sort for at most 16 scenes)")
+    ap.add_argument("--lazy", action="store_true",
+                    help="time handles that re-sort lazily whatever the batch size")
+    ap.add_argument("--skin", type=float, default=0.0,
+                    help="lazy re-sort: cells grow by this fraction (0: the library's default)")
     a = ap.parse_args(argv)
     from .solver import GpuSolver
     scs = WORKLOADS[a.workload]()
@@ -133,20 +143,29 @@ def main(argv=None):
         print(m, file=sys.stderr, flush=True)
 
     say(f"{a.workload}: {len(scs)} scene(s), {nf} fluid")
-    res = autotune(scs, warmup=a.warmup, steps=a.steps, cache=a.cache, log=say)
-    out = {"workload": a.workload, "n_fluid": nf, "best": res["best"], "tuned_ms": res["ms_per_step"]}
-    for name, cfg in (("default", {}), ("best", res["best"])):
-        g = GpuSolver(scs, **cfg)
+    out = {"workload": a.workload, "n_fluid": nf}
+    legs = [("default", {})]
+    if not a.no_tune:
+        res = autotune(scs, warmup=a.warmup, steps=a.steps, cache=a.cache, log=say)
+        out.update(best=res["best"], tuned_ms=res["ms_per_step"])
+        legs.append(("best", res["best"]))
+    for name, cfg in legs:
+        ses = True if a.sort_every_step else (False if a.lazy else None)
+        g = GpuSolver(scs, sort_every_step=ses, skin=a.skin, **cfg)
         g.step(a.warmup)
         g.sync()
+        r0 = g.resorts()
         t0 = time.perf_counter()
         g.step(100)
         g.sync()
         ms = (time.perf_counter() - t0) * 10.0
+        nres = g.resorts() - r0 if r0 >= 0 else -1
         g.close()
         out[name + "_ms_per_step"] = ms
         out[name + "_updates_per_s"] = nf * 1e3 / ms
-        say(f"{name} {cfg}: {ms:.4f} ms per step, {nf * 1e3 / ms:.3e} updates/s")
+        out[name + "_resorts_in_100_steps"] = nres
+        say(f"{name} {cfg}: {ms:.4f} ms per step, {nf * 1e3 / ms:.3e} updates/s, "
+            f"{'sorted every step' if nres < 0 else f'{nres} re-sorts'}")
     print(json.dumps(out), flush=True)
 
 

@@ -44,6 +44,8 @@
 // fills the chip (4 blocks of 256 per CU on 256 CUs); when it has none it returns
 // at once (measured no slower than a 512-block launch).
 #define FIX_GRID 1024
+// the most scenes a handle re-sorts lazily by default (uph_opts.sort_every_step = 0)
+#define LAZY_MAX_SCENES 16
 // Candidates (slots of a query's 3x3 cells, or of one cell) beyond which a step
 // gives up on the query and flags the state as broken: no physical particle distribution comes
 // near it (a smoothing length of 6 lattice spacings with 4-fold coarsened cells

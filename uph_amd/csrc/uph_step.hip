@@ -699,7 +699,11 @@ extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts*
   h->hs.resize(n_scenes);
   // lazy re-sort where it applies: no scene with mirror rows (the sort evaluates their
   // section tests every step), none expected to be capped, no hipGraph (the step's
-  // launches depend on the call); its cells grow by the skin
+  // launches depend on the call); its cells grow by the skin.  By default (0) also only
+  // for batches of at most LAZY_MAX_SCENES scenes: a batch re-sorts at every step where
+  // one of its scenes has gone stale, so a large batch of moving scenes re-sorts nearly
+  // every step, and a lazy re-sort step costs more than an every-step sort; -1 asks for
+  // it whatever the batch size.
   bool any_mirrors = false, any_capped = false;
   for (int s = 0; s < n_scenes; ++s) {
     const uph_scene& u = scenes[s];
@@ -707,8 +711,11 @@ extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts*
     if (u.n_fluid > 0 && mirror_sections(u.vtype, sec) > 0) any_mirrors = true;
     if (!(4.0 * M_PI * (u.h / u.dx) * (u.h / u.dx) <= 0.75 * u.K)) any_capped = true;
   }
-  bool lazy = !h->opts.sort_every_step && !h->opts.use_graph && !any_mirrors && !any_capped;
-  if (!(h->opts.skin > 0.0)) h->opts.skin = 0.05;
+  bool lazy = h->opts.sort_every_step <= 0 && !h->opts.use_graph && !any_mirrors && !any_capped &&
+              (h->opts.sort_every_step < 0 || n_scenes <= LAZY_MAX_SCENES);
+  // default growth: 2 % of the cell for one scene, 5 % for a batch (a batch re-sorts when
+  // any of its scenes needs it, so its cells carry more slack)
+  if (!(h->opts.skin > 0.0)) h->opts.skin = n_scenes == 1 ? 0.02 : 0.05;
   if (h->opts.skin > 1.0) h->opts.skin = 1.0;
   const double cell_factor = lazy ? h->opts.cell_factor * (1.0 + h->opts.skin) : h->opts.cell_factor;
   long n = 0, nc = 0, gtot = 0, wtot = 0, gfluid = 0;
@@ -1063,6 +1070,11 @@ extern "C" int uph_sync(uph_t* h) {
 }
 
 extern "C" int uph_itime(const uph_t* h) { return h ? h->itime : -1; }
+
+extern "C" double uph_skin(const uph_t* h) {
+  if (!h || !h->b.lazy) return 0.0;
+  return h->opts.skin;
+}
 
 extern "C" int uph_resorts(uph_t* h) {
   if (!h) return -1;

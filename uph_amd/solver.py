@@ -10,7 +10,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
-from typing import Sequence
+from typing import Optional, Sequence
 
 import numpy as np
 
@@ -77,6 +77,7 @@ SIGNATURES = {
     "uph_derivatives_scheme": (C.c_int, [C.c_int, C.c_int]),
     "uph_itime": (C.c_int, [C.c_void_p]),
     "uph_resorts": (C.c_int, [C.c_void_p]),
+    "uph_skin": (C.c_double, [C.c_void_p]),
     "uph_time_steps": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp]),
     "uph_time_phases": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp]),
     "uph_last_error": (C.c_char_p, [C.c_void_p]),
@@ -212,11 +213,16 @@ def search_neighbors_rows(pos: np.ndarray, i_pair: np.ndarray, j_pair: np.ndarra
 
 
 class GpuSolver:
-    """Device-resident leapfrog over a batch of independent scenes."""
+    """Device-resident leapfrog over a batch of independent scenes.
+
+    sort_every_step: None (default) re-sorts lazily for handles of at most 16 scenes
+    and every step for larger batches; True re-sorts every step; False re-sorts lazily
+    whatever the batch size.  skin: the lazy re-sort's cell growth (0: 0.02 for one
+    scene, 0.05 for a batch)."""
 
     def __init__(self, scenes: Sequence[Scene] | Scene, use_graph: bool = False,
                  keep_rates: bool = False, pair_variant: int = 0, pair_block: int = 256,
-                 cell_factor: float = 1.0, sort_every_step: bool = False, skin: float = 0.0):
+                 cell_factor: float = 1.0, sort_every_step: Optional[bool] = None, skin: float = 0.0):
         if isinstance(scenes, Scene):
             scenes = [scenes]
         self.scenes = list(scenes)
@@ -255,10 +261,11 @@ class GpuSolver:
             u.vtype = int(getattr(sc, "vtype", 0))
             u.lbp, u.rbp = 0.0, 0.0
         opts = UphOpts(int(bool(use_graph)), int(bool(keep_rates)), int(pair_variant),
-                       int(pair_block), float(cell_factor), int(bool(sort_every_step)), float(skin))
+                       int(pair_block), float(cell_factor),
+                       0 if sort_every_step is None else (1 if sort_every_step else -1), float(skin))
         self.opts = dict(use_graph=bool(use_graph), pair_variant=int(pair_variant),
                          pair_block=int(pair_block), cell_factor=float(cell_factor),
-                         sort_every_step=bool(sort_every_step), skin=float(skin))
+                         sort_every_step=sort_every_step, skin=float(skin))
         h = C.c_void_p()
         rc = lib().uph_create(arr, len(self.scenes), C.byref(opts), C.byref(h))
         if rc != 0:
@@ -292,6 +299,10 @@ class GpuSolver:
     @property
     def itime(self) -> int:
         return lib().uph_itime(self.h)
+
+    def skin(self) -> float:
+        """The cells' growth of a lazy re-sort handle; 0.0 for one that re-sorts every step."""
+        return lib().uph_skin(self.h)
 
     def resorts(self) -> int:
         """Re-sorts of a lazy handle so far (waits); -1 for a handle that sorts every step."""
