@@ -1592,20 +1592,22 @@ __device__ __forceinline__ float2 mirror_c(const StepBufs& b, uint32_t q) {
 // particle is stale every neighbour within 2h lies in the stencil's binned cells: a
 // neighbour binned two cells away would be more than (1 - m) cells = 2h away.  A stale
 // slot marks its scene and the step (one atomic per scene and wavefront).
+// Its loads come before its stores, and callers store the slot's state after it: on
+// this GPU a wait for a load also waits for every store issued before it.
 template <bool FLUID>
 __device__ __forceinline__ void lazy_out(const DevScene& S, const StepBufs& b, int p, uint32_t tg, int itime,
                                          double2 x1, double2 v1, double rho1, float4 rt, double P1, float4 r2,
                                          float2 dv) {
-  b.tagB[p] = tg;
+  const int pk = b.cellOf[p];
   const Pred o = FLUID ? predict_fluid_next(S, v1, rho1, rt, P1, r2, dv)
                        : predict(S, b, tg, itime + 1, x1, v1, rho1, rt, P1, r2, dv);
   float4 ma, mb;
   float2 mc;
   mirror_of(S, o, rt.w, tg, ma, mb, mc);
+  b.tagB[p] = tg;
   b.mA2[p] = ma;
   b.mB2[p] = mb;
   if (S.mode == 1) b.mC2[p] = mc;
-  const int pk = b.cellOf[p];
   const int bx = pk & 0xffff, by = pk >> 16;
   const double u = (x1.x - S.xmin) * S.cell_inv, w = (x1.y - S.ymin) * S.cell_inv;
   const double m = S.skin_m;
@@ -1625,7 +1627,7 @@ __device__ __forceinline__ void lazy_out(const DevScene& S, const StepBufs& b, i
 }
 
 // Boundary slot: wall state for the next step (deck + kinematics at t = (itime+1) dt).
-// LAZY: a lazy re-sort handle's pair kernel (lazy_out after the state).
+// LAZY: a lazy re-sort handle's pair kernel (lazy_out, then the state).
 template <bool LAZY = false>
 __device__ __forceinline__ void wall_update(const DevScene& S, const StepBufs& b, int p, uint32_t tg, int itime,
                                             float mass) {
@@ -1637,18 +1639,18 @@ __device__ __forceinline__ void wall_update(const DevScene& S, const StepBufs& b
   }
   const double rho = b.rhoA[p];
   const float4 rt = make_float4(0.f, 0.f, 0.f, mass);
+  const double P = S.mode == 1 ? b.deckP[id] : 0.0;
+  if (LAZY)
+    lazy_out<false>(S, b, p, tg, itime, x, v, rho, rt, P, make_float4(0.f, 0.f, 0.f, 0.f), make_float2(0.f, 0.f));
   b.posB[p] = x;
   b.velB[p] = v;
   b.rhoB[p] = rho;
   b.rtB[p] = rt;
-  const double P = S.mode == 1 ? b.deckP[id] : 0.0;
   if (S.mode == 1) b.presB[p] = P;
-  if (LAZY)
-    lazy_out<false>(S, b, p, tg, itime, x, v, rho, rt, P, make_float4(0.f, 0.f, 0.f, 0.f), make_float2(0.f, 0.f));
 }
 
 // Gravity and the FP64 leapfrog corrector (sph.cc:137-168) for fluid slot p (LAZY: a
-// lazy re-sort handle's pair kernel, lazy_out after the state).
+// lazy re-sort handle's pair kernel, lazy_out, then the state).
 template <bool LAZY = false>
 __device__ __forceinline__ void correct(const DevScene& S, const StepBufs& b, int p, int itime, double2 xp,
                                         double2 vn, float mass, const PairCtx& c) {
@@ -1670,6 +1672,9 @@ __device__ __forceinline__ void correct(const DevScene& S, const StepBufs& b, in
   }
   const double2 x1 = make_double2(xp.x + dt * v1x, xp.y + dt * v1y);
   const float4 rt = make_float4(Rt, (float)atx, (float)aty, mass);
+  if (LAZY)
+    lazy_out<true>(S, b, p, (1u << UPH_ID_BITS) | c.idp, itime, x1, make_double2(v1x, v1y), rho1, rt, 0.0,
+             make_float4(0.f, 0.f, 0.f, 0.f), make_float2(0.f, 0.f));
   b.posB[p] = x1;
   b.velB[p] = make_double2(v1x, v1y);
   b.rhoB[p] = rho1;
@@ -1680,9 +1685,6 @@ __device__ __forceinline__ void correct(const DevScene& S, const StepBufs& b, in
     b.dbgRates[2 * p] = make_float4(R, Rt, (float)atx, (float)aty);
     *reinterpret_cast<double2*>(b.dbgRates + 2 * p + 1) = make_double2(ax, ay);
   }
-  if (LAZY)
-    lazy_out<true>(S, b, p, (1u << UPH_ID_BITS) | c.idp, itime, x1, make_double2(v1x, v1y), rho1, rt, 0.0,
-             make_float4(0.f, 0.f, 0.f, 0.f), make_float2(0.f, 0.f));
 }
 
 // UPH_MODE_REF: gravity, the sponge layer and the FP64 leapfrog corrector
@@ -1713,6 +1715,8 @@ __device__ __forceinline__ void correct(const DevScene& S, const StepBufs& b, in
   const float4 rt = make_float4(Rt, (float)atx, (float)aty, mass);
   const float4 r2 = make_float4(RPt, c.dr, c.dP, 0.f);
   const float2 dv = make_float2(c.dvx, c.dvy);
+  if (LAZY)
+    lazy_out<true>(S, b, p, (1u << UPH_ID_BITS) | c.idp, itime, x1, make_double2(v1x, v1y), rho1, rt, P1, r2, dv);
   b.posB[p] = x1;
   b.velB[p] = make_double2(v1x, v1y);
   b.rhoB[p] = rho1;
@@ -1720,8 +1724,6 @@ __device__ __forceinline__ void correct(const DevScene& S, const StepBufs& b, in
   b.rtB[p] = rt;
   b.rt2B[p] = r2;
   b.dvB[p] = dv;
-  if (LAZY)
-    lazy_out<true>(S, b, p, (1u << UPH_ID_BITS) | c.idp, itime, x1, make_double2(v1x, v1y), rho1, rt, P1, r2, dv);
   if (b.dbgRates) {
     b.dbgRates[4 * p] = make_float4(R, Rt, (float)atx, (float)aty);
     *reinterpret_cast<double2*>(b.dbgRates + 4 * p + 1) = make_double2(ax, ay);
