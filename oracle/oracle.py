@@ -28,7 +28,7 @@ class Params(C.Structure):
                 ("m_omega", C.c_double), ("kernel", C.c_int), ("mode", C.c_int),
                 ("kappa", C.c_double), ("mu", C.c_double), ("eps", C.c_double),
                 ("lid_u", C.c_double), ("depth", C.c_double), ("flat_bottom", C.c_double),
-                ("vtype", C.c_int)]
+                ("vtype", C.c_int), ("stress_type", C.c_int)]
 
 
 def build() -> None:
@@ -67,6 +67,8 @@ def lib():
         L.orc_get_state.argtypes = [C.c_void_p] + [C.c_void_p] * 5
         L.orc_get_rates.argtypes = [C.c_void_p] + [C.c_void_p] * 5
         L.orc_get_ref.argtypes = [C.c_void_p] + [C.c_void_p] * 7
+        L.orc_get_stress.argtypes = [C.c_void_p, C.c_void_p]
+        L.orc_get_stress.restype = None
         L.orc_set_gate_y.argtypes = [C.c_void_p, _dp]
         L.orc_set_local.argtypes = [C.c_void_p] + [C.c_void_p] * 3
         L.orc_get_ghosts.argtypes = [C.c_void_p] + [C.c_void_p] * 8
@@ -123,7 +125,7 @@ def params_of(sc) -> Params:
                   float(getattr(sc, "kappa", 0.0)), float(getattr(sc, "mu", 0.0)),
                   float(getattr(sc, "eps", 0.0)), float(getattr(sc, "lid_u", 0.0)),
                   float(getattr(sc, "depth", 0.0)), float(getattr(sc, "flat_bottom", 0.0)),
-                  int(getattr(sc, "vtype", 0)))
+                  int(getattr(sc, "vtype", 0)), int(getattr(sc, "stress_type", 1)))
 
 
 class Sim:
@@ -187,6 +189,13 @@ class Sim:
         out["dv"] = np.zeros((n, 2))
         lib().orc_get_ref(self.h, *[out[k].ctypes.data for k in
                                     ("P", "Ploc", "RP", "RPt", "drho", "dP", "dv")])
+        return out
+
+    def stress(self):
+        """ViscousStress xx, xy, yx, yy of the last derivatives() call, (n, 4)
+        (ClassicShearStress, ref mode)."""
+        out = np.zeros((self.n, 4))
+        lib().orc_get_stress(self.h, out.ctypes.data)
         return out
 
     def ghosts(self):

@@ -54,6 +54,8 @@ typedef struct {
   double kappa, mu, eps;                /* ref mode: thermalDiffusivity0, viscosity0, epsilon */
   double lid_u, depth, flat_bottom;     /* lidVelocity, LocalDepth, FlatBottomLength (motions 3, 4, 5) */
   int vtype;                            /* virtualType: mirror particles of derivatives.c:3193-4192 */
+  int stress_type;                      /* ref mode: stressType, 1 single-step viscous forces, 2 two-step
+                                           (ClassicShearStress + ViscousForcesTwoStep) */
 } orc_params;
 
 typedef struct {
@@ -65,6 +67,7 @@ typedef struct {
   /* ref mode: DynamicPressure_min, localPressure, dpdt, dpdt_approx, the fluctuation
    * fields, the pair kernel values w, spongeFactor */
   double *Pmin, *Ploc, *RP, *RPt, *drho, *dP, *dv, *w, *sponge;
+  double *stress;                       /* ViscousStress{xx,xy,yx,yy} per particle (stressType 2) */
   double *gate_y;                       /* tempPositionGate (deck column 9) */
   /* mirror particles of the last derivatives() call: rows n .. n+ng-1 of x, and their
    * Velocity, localVelocity, density, localDensity, DynamicPressure, localPressure, mass */
@@ -359,7 +362,7 @@ orc_sim *orc_create(int nf, int nfixed, int nmoving, const orc_params *p, const 
   A(v, 2); A(vmin, 2); A(vloc, 2); A(rho, 1); A(rhomin, 1); A(rholoc, 1); A(P, 1);
   A(mass, 1); A(hsml, 1); A(R, 1); A(Rt, 1); A(a, 2); A(at, 2);
   A(Pmin, 1); A(Ploc, 1); A(RP, 1); A(RPt, 1); A(drho, 1); A(dP, 1); A(dv, 2); A(sponge, 1);
-  A(gate_y, 1);
+  A(gate_y, 1); A(stress, 4);
   A(deck_x, 2); A(deck_v, 2); A(deck_rho, 1); A(deck_P, 1); A(deck_lvel, 2);
 #undef A
   s->itype = calloc(n1, sizeof(int));
@@ -409,7 +412,7 @@ void orc_destroy(orc_sim *s) {
   free(s->a); free(s->at); free(s->deck_x); free(s->deck_v); free(s->deck_rho); free(s->deck_P);
   free(s->deck_lvel); free(s->itype); free(s->ip); free(s->jp); free(s->rij2); free(s->dwdx);
   free(s->tmp); free(s->Pmin); free(s->Ploc); free(s->RP); free(s->RPt); free(s->drho);
-  free(s->dP); free(s->dv); free(s->w); free(s->sponge); free(s->gate_y);
+  free(s->dP); free(s->dv); free(s->w); free(s->sponge); free(s->gate_y); free(s->stress);
   free(s->gv); free(s->glv); free(s->grho); free(s->grl); free(s->gP); free(s->gPl); free(s->gmass);
   free(s);
 }
@@ -691,6 +694,61 @@ static void ref_viscous(orc_sim *s, int predicting) {
   SCATTER_END(predicting ? s->at : s->a)
 }
 
+/* ClassicShearStress (derivatives.c:1686-1721), serial in pair order as there:
+ * stress[4 i + {0,1,2,3}] = ViscousStress{xx,xy,yx,yy}[i], zeroed each call
+ * (:3150-3153).  dvij = v_i - vloc_j, dvji = v_j - vloc_i; the reduced volume of j is
+ * (m_j / rho_j) rholoc_j / rho_i and of i (m_i / rho_i) rholoc_i / rho_j;
+ * viscosity[] = viscosity0 (input.c:108, inputbp.c:288). */
+static void ref_shear_stress(orc_sim *s) {
+  const double mu = s->p.mu;
+  double *S = s->stress;
+  memset(S, 0, sizeof(double) * 4 * s->n);
+  for (int k = 0; k < s->niac; k++) {
+    const int i = s->ip[k], j = s->jp[k];
+    const double *dw = &s->dwdx[2 * k];
+    double dvij[2], dvji[2];
+    for (int d = 0; d < 2; d++) {
+      dvij[d] = s->v[2 * i + d] - s->vloc[2 * j + d];
+      dvji[d] = s->v[2 * j + d] - s->vloc[2 * i + d];
+    }
+    const double vi = s->mass[i] / s->rho[i], vj = s->mass[j] / s->rho[j];
+    const double rvi = vi * s->rholoc[i] / s->rho[j];
+    const double rvj = vj * s->rholoc[j] / s->rho[i];
+    S[4 * i + 0] -= mu * rvj * (dvij[0] * dw[0] + dvij[0] * dw[0]);
+    S[4 * j + 0] += mu * rvi * (dvji[0] * dw[0] + dvji[0] * dw[0]);
+    S[4 * i + 1] -= mu * rvj * (dvij[0] * dw[1] + dvij[1] * dw[0]);
+    S[4 * j + 1] += mu * rvi * (dvji[0] * dw[1] + dvji[1] * dw[0]);
+    S[4 * i + 2] -= mu * rvj * (dvij[1] * dw[0] + dvij[0] * dw[1]);
+    S[4 * j + 2] += mu * rvi * (dvji[1] * dw[0] + dvji[0] * dw[1]);
+    S[4 * i + 3] -= mu * rvj * (dvij[1] * dw[1] + dvij[1] * dw[1]);
+    S[4 * j + 3] += mu * rvi * (dvji[1] * dw[1] + dvji[1] * dw[1]);
+  }
+}
+
+/* ViscousForcesTwoStep (derivatives.c:1724-1768) -> a: the divergence of the stresses
+ * of both particles of a pair, weighted rholoc_j/(rho_i^2 rho_j) and
+ * rholoc_i/(rho_j^2 rho_i); the second particle takes its share only when it is a
+ * fluid particle. */
+static void ref_viscous_two_step(orc_sim *s) {
+  const double *S = s->stress;
+  SCATTER_BEGIN(2)
+    const int i = s->ip[k], j = s->jp[k];
+    const double *dw = &s->dwdx[2 * k];
+    const double rhoi = s->rholoc[j] / (s->rho[i] * s->rho[i] * s->rho[j]);
+    const double rhoj = s->rholoc[i] / (s->rho[j] * s->rho[j] * s->rho[i]);
+    double ax = (S[4 * i + 0] * rhoi + S[4 * j + 0] * rhoj) * dw[0];
+    ax += (S[4 * i + 1] * rhoi + S[4 * j + 1] * rhoj) * dw[1];
+    double ay = (S[4 * i + 2] * rhoi + S[4 * j + 2] * rhoj) * dw[0];
+    ay += (S[4 * i + 3] * rhoi + S[4 * j + 3] * rhoj) * dw[1];
+    tb[2 * i] += s->mass[j] * ax;
+    tb[2 * i + 1] += s->mass[j] * ay;
+    if (j < s->nf) {
+      tb[2 * j] -= s->mass[i] * ax;
+      tb[2 * j + 1] -= s->mass[i] * ay;
+    }
+  SCATTER_END(s->a)
+}
+
 /* Ref-mode sums that do not touch dvdt: the pressure rates and the fluctuation
  * fields (derivatives.c:4204-4237). */
 static void ref_pressure_and_fluctuations(orc_sim *s) {
@@ -902,6 +960,9 @@ int orc_derivatives(orc_sim *s, double t) {
   }
 
   if (p->mode == 1) ref_pressure_and_fluctuations(s);
+  /* ClassicShearStress runs in every ref-mode call (derivatives.c:4225); only stressType
+   * 2 reads it */
+  if (p->mode == 1) ref_shear_stress(s);
 
   /* PredictingDensityEquation (derivatives.c:1217-1252) */
   SCATTER_BEGIN(1)
@@ -967,9 +1028,13 @@ int orc_derivatives(orc_sim *s, double t) {
     }
   SCATTER_END(s->a)
 
-  /* ref mode, stressType 1 (derivatives.c:4243-4247) */
+  /* ref mode: stressType 2 takes the two-step viscous forces, any other the single-step
+   * ones; the predicting form is single-step either way (derivatives.c:4231-4244) */
   if (p->mode == 1) {
-    ref_viscous(s, 0);
+    if (p->stress_type == 2)
+      ref_viscous_two_step(s);
+    else
+      ref_viscous(s, 0);
     ref_viscous(s, 1);
   }
 
@@ -1115,6 +1180,11 @@ void orc_get_rates(const orc_sim *s, double *R, double *Rt, double *a, double *a
 
 /* Ref-mode fields: DynamicPressure, localPressure, dpdt, dpdt_approx, density and
  * pressure fluctuations (n each), velocity fluctuation (2n). */
+/* ViscousStress{xx,xy,yx,yy} of the last derivatives() call, 4 per particle. */
+void orc_get_stress(const orc_sim *s, double *stress) {
+  memcpy(stress, s->stress, sizeof(double) * 4 * s->n);
+}
+
 void orc_get_ref(const orc_sim *s, double *P, double *Ploc, double *RP, double *RPt, double *drho,
                  double *dP, double *dv) {
   if (P) memcpy(P, s->P, sizeof(double) * s->n);

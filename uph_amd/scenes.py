@@ -83,6 +83,7 @@ class Scene:
     lid_u: float = 0.0       # lidVelocity (lid and gate)
     depth: float = 0.0       # LocalDepth (the gate's stop time 1.5 (depth - 15 dx) / lid_u)
     vtype: int = 0           # virtualType: mirror particles ranked by the search (derivatives.c:3193-4192)
+    stress_type: int = 1     # stressType (ref mode): 2 takes ClassicShearStress + ViscousForcesTwoStep
     flat_bottom: float = 0.0  # FlatBottomLength (the sloshing tank's right wall)
     gate_y: Optional[np.ndarray] = None   # (N,) deck column 9, tempPositionGate
     name: str = "scene"
@@ -446,7 +447,8 @@ def scene_parameters(sc: Scene, **over) -> dict:
                nearNeighbor=4.0 * math.pi * (sc.h / sc.dx) ** 2, selectBoundaryForce=3, lennardSelect=0,
                springConstant0=sc.k_spring, dampingConstant0=sc.c_damp, flagInitSimul=1,
                gravityFlag=sc.gravity, thermalDiffusivity0=sc.kappa, movingBoundary=mb,
-               NumberOfInitializationSteps=0, pressureProbeX=0.0, pressureProbeY=0.0, stressType=1,
+               NumberOfInitializationSteps=0, pressureProbeX=0.0, pressureProbeY=0.0,
+               stressType=sc.stress_type,
                disspationControlVolume=0)
     prm.update(over)
     return prm
@@ -527,7 +529,8 @@ def scene_from_rows(fluid: np.ndarray, fixed: np.ndarray, moving: np.ndarray,
               lid_u=prm["lidVelocity"], depth=prm["LocalDepth"], flat_bottom=prm["FlatBottomLength"],
               m_amp=prm["stroke"], m_omega=2.0 * math.pi / prm["wavePeriod"],
               kappa=prm["thermalDiffusivity0"], mu=prm["viscosity0"], eps=prm["epsilon"],
-              vtype=int(prm["virtualType"]), box=(lo[0], lo[1], hi[0], hi[1]), name="deck")
+              vtype=int(prm["virtualType"]), stress_type=int(prm["stressType"]),
+              box=(lo[0], lo[1], hi[0], hi[1]), name="deck")
     kw.update(over)
     return Scene(pos=pos, vel=rows[:, 2:4].copy(), rho=rows[:, 4].copy(),
                  pressure=rows[:, 5].copy(), mass=rows[:, 6].copy(), gate_y=rows[:, 8].copy(),
