@@ -164,6 +164,9 @@ def main():
                          "run-phase scheme (pressure transport, viscous forces, fluctuations) with its "
                          "decks' viscosity0 1e-6 and epsilon 1 and thermalDiffusivity0 0.0015 (the decks' "
                          "0.015 is explicitly unstable at this resolution)")
+    ap.add_argument("--stress-type", type=int, choices=[1, 2], default=1,
+                    help="ref scheme: stressType 1 (single-step viscous forces, the decks' choice) or 2 "
+                         "(ClassicShearStress + ViscousForcesTwoStep; sorts every step)")
     ap.add_argument("--h-over-dx", type=float, default=1.3,
                     help="smoothing length in lattice spacings (1.3 default; the reference's decks "
                          "use 2.691, where the 40-neighbour cap binds for every particle)")
@@ -205,6 +208,7 @@ def main():
         # the pressure diffusion is explicit: with the decks' 0.015 m^2/s its SPH Laplacian
         # needs dt < h^2 / (10 gamma kappa) = 3.2 us at h = 1.84 mm, below this dt of 7.3 us
         sc.kappa, sc.mu, sc.eps, sc.alpha_av = 0.0015, 1e-6, 1.0, 0.0
+        sc.stress_type = args.stress_type
     if args.vtype:
         sc.vtype, sc.flat_bottom, sc.depth = args.vtype, 4.0, 2.0   # the tank's width, the column's height
     log(f"workload: {sc.n_fluid} fluid + {sc.n_boundary} wall particles, {args.kernel} kernel, "
@@ -249,6 +253,7 @@ def main():
         "config": {"workload": "dam break 1.0x2.0 m column, dx=sqrt(2) mm, 4x3 m tank",
                    "fluid_particles": sc.n_fluid, "wall_particles": sc.n_boundary,
                    "kernel": args.kernel, "scheme": args.scheme, "h_over_dx": args.h_over_dx, "dt": sc.dt,
+                   "stress_type": args.stress_type if args.scheme == "ref" else None,
                    "virtual_type": args.vtype, "capped_steps": overflow,
                    "settle_steps": args.settle,
                    "resort": (f"lazy, cells grown by {g.skin():g}" if lazy

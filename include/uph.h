@@ -150,6 +150,12 @@ typedef struct uph_scene {
   double lbp, rbp;      /* LeftBoundaryPosition, RightBoundaryPosition where the motion does
                            not set them (UPH_MOTION_LID, UPH_MOTION_GATE, and rbp without
                            UPH_MOTION_SLOSH); the mirror sections read them */
+  int stress_type;      /* stressType (UPH_MODE_REF, derivatives.c:4231-4244): 2 = the viscous
+                           stress of ClassicShearStress (:1686-1721) and its divergence,
+                           ViscousForcesTwoStep (:1724-1768), in dvdt; other values the
+                           single-step viscous forces (:1770-1839).  The same for every scene
+                           of a batch; 2 needs a handle without mirror rows whose queries are
+                           never capped (the stress sums take every neighbour in range) */
 } uph_scene;
 
 typedef struct uph_opts {

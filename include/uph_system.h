@@ -203,7 +203,8 @@ const char *uph_derivatives_last_error(void);
  * reference's scheme with its PolyOrderSeven kernel), "ref-cubic" or "config".
  *
  * UPH_MODE_REF evaluates derivatives.c:4196-4315 as the reference's decks run it
- * (flagInitSimul 1, stressType 1, spring walls) at the System's state: it reads
+ * (flagInitSimul 1, spring walls; stressType 1, or 2 as the System says) at the
+ * System's state: it reads
  * the fluid rows' DynamicPressure, localDensity, localPressure and localVelocity
  * as the caller's host loop (sph.cc:90-168) left them, plus thermalDiffusivity0,
  * viscosity0 and epsilon, and writes ddensitydt, ddensitydt_approx, dvdt,

@@ -247,7 +247,7 @@ std::vector<double> compat_key(const uph_system* s) {
           s->LocalDepth, s->FlatBottomLength,
           // the boundary positions the mirror sections read where no motion sets them
           (s->movingBoundary == 2 || s->movingBoundary == 3) ? s->LeftBoundaryPosition : 0.0,
-          compat_motion(s) != UPH_MOTION_SLOSH ? s->RightBoundaryPosition : 0.0};
+          compat_motion(s) != UPH_MOTION_SLOSH ? s->RightBoundaryPosition : 0.0, (double)s->stressType};
 }
 
 int compat_build(const uph_system* s) {
@@ -309,6 +309,7 @@ int compat_build(const uph_system* s) {
   u.mu = s->viscosity0;
   u.eps = s->epsilon;
   u.vtype = s->virtualType;
+  u.stress_type = s->stressType;
   u.lbp = s->LeftBoundaryPosition;
   u.rbp = s->RightBoundaryPosition;
   std::vector<double> pres(np, 0.0);

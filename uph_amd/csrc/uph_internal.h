@@ -71,6 +71,7 @@
 #define C_LOVF 10    // lazy: overflow flag by itime parity (2 words)
 #define C_LSTALE 12  // lazy: some scene's cell list went stale, by itime parity (2 words)
 #define C_NRES 14    // lazy: re-sorts done (a count for reports)
+#define C_SCAP 15    // stressType 2: a query had more than K neighbours in range (sticky)
 #define CTL_WORDS 16
 
 // ------------------------------------------------------------------ sort
@@ -285,6 +286,9 @@ struct StepBufs {
   // packed as cx | cy << 16 (cell_pack).
   float4 *mA2, *mB2, *rtA, *rt2A;
   float2 *mC2, *dvA;
+  // stressType 2 handles: each slot's viscous stress {xx, xy (= yx), yy, 0} of the
+  // step (k_stress), read by the pair kernel for the slot and its neighbours
+  float4* sst;
   int* cellOf;
   int* sstale;
   int* it;
@@ -1367,7 +1371,7 @@ struct PairCtx : PairBase {
   CSum sR, sRt, sAx, sAy;   // sAx/sAy: the predicting acceleration (dvdt_approx)
   float dAx = 0.f, dAy = 0.f;   // dvdt - dvdt_approx: wall pairs with localDensity != density
 
-  __device__ __forceinline__ void init(const DevScene& S, uint32_t tg, float4 ap, float4 bp, float2) {
+  __device__ __forceinline__ void init(const DevScene& S, uint32_t tg, float4 ap, float4 bp, float2, float4) {
     init_base(S, tg);
     vpx = ap.x; vpy = ap.y; lpx = ap.z; lpy = ap.w;
     rhop = bp.x; prp = bp.y; mp = bp.z;
@@ -1381,7 +1385,7 @@ struct PairCtx : PairBase {
   // 1/r and 1/x use v_rsq_f32 / v_rcp_f32.
   template <int KT>
   __device__ __forceinline__ void add(const DevScene& S, double ex, double ey, uint32_t tq, float4 aq, float4 bq,
-                                      float2) {
+                                      float2, float4) {
 #pragma clang fp contract(on)
     const bool p_is_i = tag_id(tq) > idp;   // pair (i=p, j=q) or (i=q, j=p)
     const float dx = (float)ex, dy = (float)ey;
@@ -1440,8 +1444,11 @@ struct PairCtx : PairBase {
 
 // Ref-mode sums of one fluid particle p: the reference's run-phase rates in gather
 // form (every neighbour q of p; terms whose reference form depends on which of p, q
-// is the pair's first index branch on the original ids, as a5 does).
-struct RefCtx : PairBase {
+// is the pair's first index branch on the original ids, as a5 does).  TWO: stressType
+// 2, dvdt's viscous term from the stresses of p and q (k_stress) instead of the
+// single-step form.
+template <bool TWO>
+struct RefCtxT : PairBase {
   float vpx, vpy, lpx, lpy;   // predicted velocity, localVelocity
   float rhop, Pp, mp, rlp;    // predicted density, DynamicPressure, mass, localDensity
   float drp, drlp, r0;        // density and localDensity minus rho0; rho0
@@ -1450,9 +1457,12 @@ struct RefCtx : PairBase {
   CSum sL, sC2, sLt;          // dpdt diffusion part; sum m_q (v_p - v_q).grad W; dpdt_approx diffusion part
   CSum sAx, sAy, sTx, sTy;    // dvdt, dvdt_approx
   float dr = 0.f, dP = 0.f, dvx = 0.f, dvy = 0.f;   // fluctuations
+  float4 sp;                  // TWO: p's viscous stress {xx, xy, yy, 0}
 
-  __device__ __forceinline__ void init(const DevScene& S, uint32_t tg, float4 ap, float4 bp, float2 cp) {
+  __device__ __forceinline__ void init(const DevScene& S, uint32_t tg, float4 ap, float4 bp, float2 cp,
+                                       float4 spv) {
     init_base(S, tg);
+    if constexpr (TWO) sp = spv;
     vpx = ap.x; vpy = ap.y; lpx = ap.z; lpy = ap.w;
     r0 = S.frho0;
     drp = bp.x; Pp = bp.y; mp = bp.z;
@@ -1470,7 +1480,7 @@ struct RefCtx : PairBase {
 
   template <int KT>
   __device__ __forceinline__ void add(const DevScene& S, double ex, double ey, uint32_t tq, float4 aq, float4 bq,
-                                      float2 cq) {
+                                      float2 cq, float4 sq) {
     // no contraction: the compacted and the direct kernel inline this body into
     // different code, and only unfused arithmetic rounds the same in both
 #pragma clang fp contract(off)
@@ -1545,11 +1555,21 @@ struct RefCtx : PairBase {
       const float sgx = s00 * gx + s01 * gy, sgy = s01 * gx - s00 * gy;
       // rholoc_q/(rho_p^2 rho_q) - rholoc_p/(rho_q^2 rho_p) = (rholoc_q rho_q -
       // rholoc_p rho_p) / (rho_p rho_q)^2, the numerator from the deviations
-      const float num = r0 * ((drlq + drq) - (drlp + drp)) + (drlq * drq - drlp * drp);
-      const float dA = num * irp2 * irq * irq;
-      const float sg = p_is_i ? dA : -dA;
-      ax += mq * sg * sgx;
-      ay += mq * sg * sgy;
+      if constexpr (TWO) {
+        // ViscousForcesTwoStep (:1724-1768): m_q (S_p c_p + S_q c_q) . grad_p W with
+        // c_p = rholoc_q / (rho_p^2 rho_q), c_q = rholoc_p / (rho_q^2 rho_p), the same
+        // for either pair order
+        const float cp = rlq * irp2 * irq, cq = rlp * irq * irq * irp;
+        const float txx = sp.x * cp + sq.x * cq, txy = sp.y * cp + sq.y * cq, tyy = sp.z * cp + sq.z * cq;
+        ax += mq * (txx * gx + txy * gy);
+        ay += mq * (txy * gx + tyy * gy);
+      } else {
+        const float num = r0 * ((drlq + drq) - (drlp + drp)) + (drlq * drq - drlp * drp);
+        const float dA = num * irp2 * irq * irq;
+        const float sg = p_is_i ? dA : -dA;
+        ax += mq * sg * sgx;
+        ay += mq * sg * sgy;
+      }
       const float st = mq * (irp2 + irq * irq);
       tx += st * sgx;
       ty += st * sgy;
@@ -1572,14 +1592,34 @@ struct RefCtx : PairBase {
   }
 };
 
-// The pair context of a scheme.
+// The pair context of a scheme; UPH_MODE_REF2 is the reference scheme with
+// stressType 2 (a template value only: such a handle's scenes say UPH_MODE_REF).
+#define UPH_MODE_REF2 2
 template <int MODE>
-using CtxOf = typename std::conditional<MODE == UPH_MODE_REF, RefCtx, PairCtx>::type;
+struct CtxSel {
+  using type = PairCtx;
+};
+template <>
+struct CtxSel<UPH_MODE_REF> {
+  using type = RefCtxT<false>;
+};
+template <>
+struct CtxSel<UPH_MODE_REF2> {
+  using type = RefCtxT<true>;
+};
+template <int MODE>
+using CtxOf = typename CtxSel<MODE>::type;
 
 // The {localDensity, localPressure} mirror of slot q, read by the ref-mode scheme only.
 template <int MODE>
 __device__ __forceinline__ float2 mirror_c(const StepBufs& b, uint32_t q) {
-  return MODE == UPH_MODE_REF ? ld_off(b.mC, q) : make_float2(0.f, 0.f);
+  return MODE != UPH_MODE_CONFIG ? ld_off(b.mC, q) : make_float2(0.f, 0.f);
+}
+
+// The viscous stress of slot q, read with stressType 2 only.
+template <int MODE>
+__device__ __forceinline__ float4 mirror_s(const StepBufs& b, uint32_t q) {
+  return MODE == UPH_MODE_REF2 ? ld_off(b.sst, q) : make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
 // A lazy handle's epilogue of slot p (after its state for the next step is written
@@ -1689,9 +1729,9 @@ __device__ __forceinline__ void correct(const DevScene& S, const StepBufs& b, in
 
 // UPH_MODE_REF: gravity, the sponge layer and the FP64 leapfrog corrector
 // (sph.cc:137-168) with DynamicPressure integrated like the density.
-template <bool LAZY = false>
+template <bool LAZY = false, bool TWO>
 __device__ __forceinline__ void correct(const DevScene& S, const StepBufs& b, int p, int itime, double2 xp,
-                                        double2 vn, float mass, const RefCtx& c) {
+                                        double2 vn, float mass, const RefCtxT<TWO>& c) {
   const double dt = S.dt;
   const float R = c.sR.val(), Rt = c.sRt.val();
   const float RP = c.Kp * c.irp * R + c.sL.val();      // compression + diffusion
@@ -1757,7 +1797,7 @@ __device__ __forceinline__ bool pair_direct(int p, const StepBufs& b, const DevS
     }
   }
   CtxOf<MODE> c;
-  c.init(S, tg, b.mA[p], b.mB[p], mirror_c<MODE>(b, (uint32_t)p));
+  c.init(S, tg, b.mA[p], b.mB[p], mirror_c<MODE>(b, (uint32_t)p), mirror_s<MODE>(b, (uint32_t)p));
   const double cut = S.cutoff;
   double kpd2 = INFINITY;
   uint32_t kpid = 0xffffffffu;
@@ -1773,7 +1813,7 @@ __device__ __forceinline__ bool pair_direct(int p, const StepBufs& b, const DevS
     const float4 bq = b.mB[q];
     const uint32_t tq = __float_as_uint(bq.w);
     if (!c.template admit<EXACT>(S, tag_id(tq), d2, kpd2, kpid, b, q)) return;
-    c.template add<KT>(S, ex, ey, tq, b.mA[q], bq, mirror_c<MODE>(b, (uint32_t)q));
+    c.template add<KT>(S, ex, ey, tq, b.mA[q], bq, mirror_c<MODE>(b, (uint32_t)q), mirror_s<MODE>(b, (uint32_t)q));
   });
   if (!EXACT && S.nsec > 0 && cnt <= S.K) cnt += ghost_count(S, xp, b);
   if (!EXACT && cnt > S.K) return true;
@@ -1807,22 +1847,30 @@ __device__ __forceinline__ void walk_pairs(const DevScene& S, const StepBufs& b,
   };
   // two register sets in ping-pong: the next pair's loads are in flight while the
   // current pair's physics runs (an exhausted stream loads slot p, unused)
-  struct Nb {
+  struct NbBase {
     double2 x;
     float4 a, b;
     float2 c;
   };
+  struct NbS : NbBase {
+    float4 s;
+  };
+  using Nb = typename std::conditional<MODE == UPH_MODE_REF2, NbS, NbBase>::type;
   auto load = [&](Nb& o, int q) {
     const uint32_t s = (uint32_t)(q >= 0 ? q : p);
     o.x = ld_off(b.posA, s);
     o.a = ld_off(b.mA, s);
     o.b = ld_off(b.mB, s);
     o.c = mirror_c<MODE>(b, s);
+    if constexpr (MODE == UPH_MODE_REF2) o.s = mirror_s<MODE>(b, s);
   };
   auto pair = [&](const Nb& o) {
     const uint32_t tq = __float_as_uint(o.b.w);
-    if (c.template admit<false>(S, tag_id(tq), 0.0, 0.0, 0u, b, 0))
-      c.template add<KT>(S, xp.x - o.x.x, xp.y - o.x.y, tq, o.a, o.b, o.c);
+    if (c.template admit<false>(S, tag_id(tq), 0.0, 0.0, 0u, b, 0)) {
+      float4 sq = make_float4(0.f, 0.f, 0.f, 0.f);
+      if constexpr (MODE == UPH_MODE_REF2) sq = o.s;
+      c.template add<KT>(S, xp.x - o.x.x, xp.y - o.x.y, tq, o.a, o.b, o.c, sq);
+    }
   };
   Nb n0, n1;
   int q0 = next();
@@ -1846,7 +1894,7 @@ __device__ __forceinline__ void compact_pairs(const DevScene& S, const StepBufs&
                                               double2 xp, double2 vn, float mass, const int* rb,
                                               const unsigned long long* m) {
   CtxOf<MODE> c;
-  c.init(S, tg, b.mA[p], b.mB[p], mirror_c<MODE>(b, (uint32_t)p));
+  c.init(S, tg, b.mA[p], b.mB[p], mirror_c<MODE>(b, (uint32_t)p), mirror_s<MODE>(b, (uint32_t)p));
   walk_pairs<KT, MODE>(S, b, p, xp, c, rb, m);
   correct<LAZY>(S, b, p, itime, xp, vn, mass, c);
 }
@@ -1868,7 +1916,7 @@ __device__ __forceinline__ bool exact_pairs(int p, const StepBufs& b, const DevS
   const double2 vn = b.velA[p];
   const float mass = b.mB[p].z;
   CtxOf<MODE> c;
-  c.init(S, tg, b.mA[p], b.mB[p], mirror_c<MODE>(b, (uint32_t)p));
+  c.init(S, tg, b.mA[p], b.mB[p], mirror_c<MODE>(b, (uint32_t)p), mirror_s<MODE>(b, (uint32_t)p));
   double kpd2;
   uint32_t kpid;
   kth_of(b, p, kpd2, kpid);
@@ -2175,6 +2223,8 @@ struct uph_handle {
   // some scene's queries are expected to be capped (4 pi (h/dx)^2 in range, a lattice's
   // count, above 3/4 of K): the pair kernel carries the per-lane K-th key search
   bool capped = false;
+  // the scenes take stressType 2 (UPH_MODE_REF2 kernels, k_stress before the pair pass)
+  bool stress2 = false;
   // lazy re-sort (b.lazy): the next step sorts the state from scratch (after uph_create
   // or a state set from outside), gathering the B set of view sort_view
   bool need_sort = true;
@@ -2193,6 +2243,7 @@ int launch_call_step(uph_handle* h, int k, int n_steps, hipEvent_t pa, hipEvent_
 // uph_pair.hip: the pair pass of a step (both views for a lazy handle), and a
 // sort-every-step handle's capped pass.
 int launch_pair_pass(uph_handle* h, hipStream_t st);
+int launch_stress(uph_handle* h, hipStream_t st);
 int launch_capped(uph_handle* h, hipStream_t st);
 // The buffers as the state getters read them after the last call (waits for the
 // stream): the A set holds the last step's input and B its output, in one slot order.

@@ -52,6 +52,8 @@ struct DevScene {
   // particle may leave the cell it was binned into before its scene is re-sorted
   int sidx;
   double skin_m;
+  // stressType 2 (reference scheme): ClassicShearStress and ViscousForcesTwoStep
+  int stress2;
 };
 
 // Sections of derivatives.c:3193-4192 (one loop over the fluid rows each).

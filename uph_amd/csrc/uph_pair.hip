@@ -40,7 +40,7 @@ template <int PB, int KT, int MODE, bool ONE, bool LANE, bool LAZY>
 // configuration scheme runs 6 wavefronts per SIMD (80 VGPRs) instead of 5.  The
 // default instance (91 VGPRs, 5 wavefronts) measured slower at 6.
 __global__ __launch_bounds__(PB)
-__attribute__((amdgpu_waves_per_eu(MODE == UPH_MODE_REF ? 4 : LANE ? 6 : 1)))
+__attribute__((amdgpu_waves_per_eu(MODE != UPH_MODE_CONFIG ? 4 : LANE ? 6 : 1)))
 void k_pair_compact(int n, StepBufs b0, const DevScene* __restrict__ sc) {
   const StepBufs b = LAZY ? pair_view(b0) : b0;
   const int p = xcd_block() * PB + threadIdx.x;
@@ -63,6 +63,84 @@ __global__ __launch_bounds__(64) void k_pair_mixed(int n, StepBufs b0, const Dev
   compact_query<KT, MODE, false, false, LAZY>(n, waves[blockIdx.x] * 64 + (int)threadIdx.x, b, sc, lds[0]);
 }
 
+// ClassicShearStress (derivatives.c:1686-1721) in gather form for stressType 2, before
+// the pair pass: S_p = -mu sum_q m_q rholoc_q / (rho_q rho_p) (D g^T + g D^T) with
+// D = v_p - vloc_q and g = grad W(x_p - x_q), the same for either order of the pair,
+// over p's pairs: a fluid slot's admitted neighbours (Search.cc:33), a wall slot's
+// fluid neighbours (a wall is only ever a pair's second particle), every neighbour in
+// range while no query is capped.  A fluid query with more than K in range sets C_SCAP
+// (uph_sync reports it).  FP32 terms from the step's mirrors, FP64 sums in candidate
+// order.  The result is {xx, xy (= yx), yy, 0}.
+template <int KT, bool ONE>
+__global__ __launch_bounds__(BLK) void k_stress(int n, StepBufs b, const DevScene* __restrict__ sc) {
+  const int p = blockIdx.x * BLK + threadIdx.x;
+  if (p >= n) return;
+  const DevScene& S = sc[ONE ? 0 : b.sceneOf[p]];
+  const uint32_t tg = b.tagA[p];
+  PairBase c;
+  c.init_base(S, tg);
+  const bool fluid = tag_type(tg) == 1;
+  float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (fluid || c.pb_ok) {   // a wall at or past ParticleBoundary is in no pair
+#pragma clang fp contract(off)
+    const double2 xp = b.posA[p];
+    const float4 ap = b.mA[p];
+    const float r0 = S.frho0;
+    const float irp = 1.0f / (r0 + b.mB[p].x);
+    const double cut = S.cutoff;
+    CSum sxx, sxy, syy;
+    int cnt = 0;
+    for_candidates(S, xp, b.cellStart, [&](int q) {
+      if (q == p) return;
+      const double2 xq = b.posA[q];
+      const double ex = xp.x - xq.x, ey = xp.y - xq.y;
+      const double d2 = __dadd_rn(__dmul_rn(ex, ex), __dmul_rn(ey, ey));
+      if (!(d2 < cut)) return;
+      ++cnt;
+      const float4 bq = b.mB[q];
+      const uint32_t tq = __float_as_uint(bq.w);
+      const bool pair = fluid ? c.template admit<false>(S, tag_id(tq), d2, 0.0, 0u, b, q) : tag_type(tq) == 1;
+      if (!pair) return;
+      const float4 aq = b.mA[q];
+      const float2 cq = b.mC[q];
+      const float dx = (float)ex, dy = (float)ey;
+      const float r2 = dx * dx + dy * dy;
+      const float rinv = __builtin_amdgcn_rsqf(r2);
+      const float f = c.template grad_factor<KT>(r2, r2 * rinv, rinv);
+      const float gx = f * dx, gy = f * dy;
+      const float rhoq = r0 + bq.x, rlq = r0 + cq.x;
+      const float vr = bq.z * rlq * __builtin_amdgcn_rcpf(rhoq) * irp;
+      const float Dx = ap.x - aq.z, Dy = ap.y - aq.w;
+      sxx.add(vr * (2.0f * Dx * gx));
+      sxy.add(vr * (Dx * gy + Dy * gx));
+      syy.add(vr * (2.0f * Dy * gy));
+    });
+    if (fluid && cnt > S.K) atomicOr(&b.ctl[C_SCAP], 1);
+    const float mu = S.fmu;
+    out = make_float4(-mu * sxx.val(), -mu * sxy.val(), -mu * syy.val(), 0.f);
+  }
+  b.sst[p] = out;
+}
+
+int launch_stress(uph_handle* h, hipStream_t st) {
+  const bool one = h->nscenes == 1;
+  if (h->kernel == UPH_KERNEL_POLY7) {
+    if (one)
+      hipLaunchKernelGGL((k_stress<UPH_KERNEL_POLY7, true>), dim3(grid_of(h->n)), dim3(BLK), 0, st, h->n, h->b, h->dsc);
+    else
+      hipLaunchKernelGGL((k_stress<UPH_KERNEL_POLY7, false>), dim3(grid_of(h->n)), dim3(BLK), 0, st, h->n, h->b,
+                         h->dsc);
+  } else {
+    if (one)
+      hipLaunchKernelGGL((k_stress<UPH_KERNEL_CUBIC, true>), dim3(grid_of(h->n)), dim3(BLK), 0, st, h->n, h->b, h->dsc);
+    else
+      hipLaunchKernelGGL((k_stress<UPH_KERNEL_CUBIC, false>), dim3(grid_of(h->n)), dim3(BLK), 0, st, h->n, h->b,
+                         h->dsc);
+  }
+  HIPCHK(hipGetLastError(), h->err);
+  return 0;
+}
+
 // The pair pass of one smoothing kernel KT and scheme MODE (compacted or direct, and
 // the wavefronts of two scenes) over the buffers b; LAZY: a lazy re-sort handle's (no
 // such handle is expected to be capped).
@@ -74,7 +152,8 @@ static int launch_pair_pass_b(uph_handle* h, hipStream_t st, const StepBufs& b) 
   } else {
     const int pbk = h->opts.pair_block;
     const int g = (((h->n + pbk - 1) / pbk + 7) / 8) * 8;
-    const bool one = h->nscenes == 1, lane = !LAZY && h->capped;
+    // a stressType 2 handle is never expected to be capped: no per-lane K-th instance
+    const bool one = h->nscenes == 1, lane = !LAZY && MODE != UPH_MODE_REF2 && h->capped;
 #define UPH_COMPACT(PBV, ONEV, LANEV)                                                                        \
   hipLaunchKernelGGL((k_pair_compact<PBV, KT, MODE, ONEV, LANEV, LAZY>), dim3(g), dim3(PBV), 0, st, h->n, b, h->dsc)
 #define UPH_COMPACT_PB(ONEV, LANEV)              \
@@ -83,9 +162,9 @@ static int launch_pair_pass_b(uph_handle* h, hipStream_t st, const StepBufs& b) 
     else if (pbk == 512) UPH_COMPACT(512, ONEV, LANEV); \
     else UPH_COMPACT(256, ONEV, LANEV);          \
   } while (0)
-    if (one && lane) UPH_COMPACT_PB(true, !LAZY);
+    if (one && lane) UPH_COMPACT_PB(true, !LAZY && MODE != UPH_MODE_REF2);
     else if (one) UPH_COMPACT_PB(true, false);
-    else if (lane) UPH_COMPACT_PB(false, !LAZY);
+    else if (lane) UPH_COMPACT_PB(false, !LAZY && MODE != UPH_MODE_REF2);
     else UPH_COMPACT_PB(false, false);
 #undef UPH_COMPACT_PB
 #undef UPH_COMPACT
@@ -99,7 +178,10 @@ static int launch_pair_pass_b(uph_handle* h, hipStream_t st, const StepBufs& b) 
 
 template <int KT, int MODE>
 static int launch_pair_pass_kt(uph_handle* h, hipStream_t st) {
-  return h->b.lazy ? launch_pair_pass_b<KT, MODE, true>(h, st, h->b) : launch_pair_pass_b<KT, MODE, false>(h, st, h->b);
+  if constexpr (MODE == UPH_MODE_REF2) return launch_pair_pass_b<KT, MODE, false>(h, st, h->b);   // sorts every step
+  else
+    return h->b.lazy ? launch_pair_pass_b<KT, MODE, true>(h, st, h->b)
+                     : launch_pair_pass_b<KT, MODE, false>(h, st, h->b);
 }
 
 // The exact capped fixup of a sort-every-step handle (returns at once unless the step
@@ -116,6 +198,9 @@ static int launch_capped_kt(uph_handle* h, hipStream_t st) {
 }
 
 int launch_pair_pass(uph_handle* h, hipStream_t st) {
+  if (h->stress2)
+    return h->kernel == UPH_KERNEL_POLY7 ? launch_pair_pass_kt<UPH_KERNEL_POLY7, UPH_MODE_REF2>(h, st)
+                                         : launch_pair_pass_kt<UPH_KERNEL_CUBIC, UPH_MODE_REF2>(h, st);
   if (h->mode == UPH_MODE_REF)
     return h->kernel == UPH_KERNEL_POLY7 ? launch_pair_pass_kt<UPH_KERNEL_POLY7, UPH_MODE_REF>(h, st)
                                          : launch_pair_pass_kt<UPH_KERNEL_CUBIC, UPH_MODE_REF>(h, st);
@@ -124,6 +209,9 @@ int launch_pair_pass(uph_handle* h, hipStream_t st) {
 }
 
 int launch_capped(uph_handle* h, hipStream_t st) {
+  if (h->stress2)
+    return h->kernel == UPH_KERNEL_POLY7 ? launch_capped_kt<UPH_KERNEL_POLY7, UPH_MODE_REF2>(h, st)
+                                         : launch_capped_kt<UPH_KERNEL_CUBIC, UPH_MODE_REF2>(h, st);
   if (h->mode == UPH_MODE_REF)
     return h->kernel == UPH_KERNEL_POLY7 ? launch_capped_kt<UPH_KERNEL_POLY7, UPH_MODE_REF>(h, st)
                                          : launch_capped_kt<UPH_KERNEL_CUBIC, UPH_MODE_REF>(h, st);

@@ -418,6 +418,7 @@ void fill_scene(DevScene& S, const uph_scene& u, double cell_factor) {
   S.kernel = u.kernel;
   S.mode = u.mode;
   S.eps = u.eps;
+  S.stress2 = u.mode == UPH_MODE_REF && u.stress_type == 2;
   S.lid_u = u.lid_u;
   S.t_stop = u.lid_u != 0.0 ? 1.5 * (u.depth - 15.0 * u.dx) / u.lid_u : 0.0;   // inputbp.c:227
   S.flat_bottom = u.flat_bottom;
@@ -520,7 +521,7 @@ void free_bufs(StepBufs& b) {
                   b.dbgRates, b.presA, b.presB, b.deckP, b.rt2B, b.dvB, b.mC, b.deckRl, b.deckPl,
                   b.wallKind, b.wallRef, b.gbits, b.gbits2, b.gwcnt, b.gwstart, b.gpart, b.gcand, b.gcandN, b.gpos, b.gid, b.gscene, b.gkey,
                   b.grank, b.gcellCount, b.gcellStart, b.gposS, b.gidS, b.gvel, b.glvel, b.grho, b.grl,
-                  b.gP, b.gPl, b.gmass, b.predV, b.predVl, b.predRho, b.predRl, b.predP, b.predPl};
+                  b.gP, b.gPl, b.gmass, b.predV, b.predVl, b.predRho, b.predRl, b.predP, b.predPl, b.sst};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   memset(&b, 0, sizeof b);
@@ -633,9 +634,16 @@ static int launch_step_kt(uph_handle* h, int k, int n_steps, hipEvent_t pa, hipE
     if (pend && launch_held_sort(h, st)) return -1;
   }
   if (pa) HIPCHK(hipEventRecord(pa, st), h->err);
+  if constexpr (MODE == UPH_MODE_REF2) {
+    if (launch_stress(h, st)) return -1;
+  }
   if (launch_pair_pass(h, st)) return -1;
   if (pb) HIPCHK(hipEventRecord(pb, st), h->err);
-  if (h->b.lazy ? launch_post<KT, MODE>(h, st, (int)(k == n_steps - 1)) : launch_capped(h, st)) return -1;
+  if constexpr (MODE == UPH_MODE_REF2) {
+    if (launch_capped(h, st)) return -1;   // such a handle sorts every step
+  } else {
+    if (h->b.lazy ? launch_post<KT, MODE>(h, st, (int)(k == n_steps - 1)) : launch_capped(h, st)) return -1;
+  }
   if (pc) HIPCHK(hipEventRecord(pc, st), h->err);
   return 0;
 }
@@ -644,7 +652,10 @@ static int launch_step_kt(uph_handle* h, int k, int n_steps, hipEvent_t pa, hipE
 // the sort, after the pair pass and after the capped pass (k_post for a lazy handle).
 int launch_call_step(uph_handle* h, int k, int n_steps, hipEvent_t pa, hipEvent_t pb, hipEvent_t pc) {
   int rc;
-  if (h->mode == UPH_MODE_REF)
+  if (h->stress2)
+    rc = h->kernel == UPH_KERNEL_POLY7 ? launch_step_kt<UPH_KERNEL_POLY7, UPH_MODE_REF2>(h, k, n_steps, pa, pb, pc)
+                                       : launch_step_kt<UPH_KERNEL_CUBIC, UPH_MODE_REF2>(h, k, n_steps, pa, pb, pc);
+  else if (h->mode == UPH_MODE_REF)
     rc = h->kernel == UPH_KERNEL_POLY7 ? launch_step_kt<UPH_KERNEL_POLY7, UPH_MODE_REF>(h, k, n_steps, pa, pb, pc)
                                        : launch_step_kt<UPH_KERNEL_CUBIC, UPH_MODE_REF>(h, k, n_steps, pa, pb, pc);
   else
@@ -711,7 +722,24 @@ extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts*
     if (u.n_fluid > 0 && mirror_sections(u.vtype, sec) > 0) any_mirrors = true;
     if (!(4.0 * M_PI * (u.h / u.dx) * (u.h / u.dx) <= 0.75 * u.K)) any_capped = true;
   }
-  bool lazy = h->opts.sort_every_step <= 0 && !h->opts.use_graph && !any_mirrors && !any_capped &&
+  // stressType 2: the same for every scene; its stress sums take every neighbour in
+  // range, so no mirror rows (which rank among the K nearest) and no capped queries; it
+  // sorts every step
+  const bool stress2 = scenes[0].mode == UPH_MODE_REF && scenes[0].stress_type == 2;
+  for (int s = 0; s < n_scenes; ++s)
+    if ((scenes[s].mode == UPH_MODE_REF && scenes[s].stress_type == 2) != stress2) {
+      g_create_err = "uph_create: scene " + std::to_string(s) + " has not the stressType of scene 0";
+      delete h;
+      return -1;
+    }
+  if (stress2 && (any_mirrors || any_capped)) {
+    g_create_err = "uph_create: stressType 2 needs scenes without mirror particles whose queries are not "
+                   "expected to be capped (4 pi (h/dx)^2 at most 3/4 of K)";
+    delete h;
+    return -1;
+  }
+  h->stress2 = stress2;
+  bool lazy = h->opts.sort_every_step <= 0 && !h->opts.use_graph && !any_mirrors && !any_capped && !stress2 &&
               (h->opts.sort_every_step < 0 || n_scenes <= LAZY_MAX_SCENES);
   // default growth: 2 % of the cell for one scene, 5 % for a batch (a batch re-sorts when
   // any of its scenes needs it, so its cells carry more slack)
@@ -804,6 +832,7 @@ extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts*
     return fail();
   }
   if (alloc_bufs(h->b, h->n, h->ncells, h->cs.nb, err, false, h->capped)) return fail();
+  if (h->stress2 && dalloc(&h->b.sst, h->n, err)) return fail();
   if (lazy) {
     if (alloc_lazy_bufs(h->b, h->n, n_scenes, err)) return fail();
     if (h->mode == UPH_MODE_REF) h->b.lazy = 2;   // the re-sort's gather carries DynamicPressure and mC
@@ -1017,9 +1046,9 @@ extern "C" int uph_step_persistent(uph_t* h, int n_steps) {
     h->err.msg = "uph_step_persistent: negative step count";
     return -1;
   }
-  if (h->b.gtot > 0 || h->capped || h->opts.pair_variant != 0 || h->b.lazy) {
+  if (h->b.gtot > 0 || h->capped || h->opts.pair_variant != 0 || h->b.lazy || h->stress2) {
     h->err.msg = "uph_step_persistent: for handles without mirror rows, not expected to be capped, with the "
-                 "compacted pair kernel, created to re-sort every step (use uph_step)";
+                 "compacted pair kernel, created to re-sort every step, without stressType 2 (use uph_step)";
     return -1;
   }
   if (n_steps == 0) return 0;
@@ -1060,6 +1089,11 @@ extern "C" int uph_sync(uph_t* h) {
   }
   int ctl[CTL_WORDS];
   HIPCHK(hipMemcpy(ctl, h->b.ctl, sizeof ctl, hipMemcpyDeviceToHost), h->err);
+  if (ctl[C_SCAP]) {
+    h->err.msg = "uph_sync: with stressType 2 a fluid particle had more than K neighbours in range; the viscous "
+                 "stresses take every neighbour in range, so the steps' results are not valid";
+    return -1;
+  }
   if (ctl[C_BROKEN]) {
     h->err.msg = "uph_sync: a particle had more than " + std::to_string(CAND_MAX) +
                  " candidates in its 3x3 cells: the state has blown up (particles escaped or collapsed "

@@ -415,6 +415,7 @@ int main(int argc, char **argv) {
   s.mode = mode;
   s.pres = all.pres;
   s.kappa = v[27]; s.mu = v[10]; s.eps = v[13];   /* thermalDiffusivity0, viscosity0, epsilon */
+  s.stress_type = (int)v[32];                       /* stressType */
   /* the second probe: pressureProbeXX/YY, which the reference never assigns
    * (System.c:130-131 commented out): --probe2, else the first probe's location */
   if (!have_probe2) {
@@ -482,6 +483,7 @@ int main(int argc, char **argv) {
     sy.approxVorticity = vec(nm);
     sy.tempPosition = all.tpos;
     sy.viscosity0 = v[10]; sy.epsilon = v[13]; sy.thermalDiffusivity0 = v[27];
+    sy.stressType = (int)v[32];
     if (print_step > 0) truncate_series();
     double tc = now_s();
     if (run_compat(&sy, nsteps, print_step, init_steps) != 0) return 1;

@@ -34,7 +34,7 @@ class UphScene(C.Structure):
                 ("kernel", C.c_int), ("pres", _dp), ("mode", C.c_int), ("kappa", C.c_double),
                 ("mu", C.c_double), ("eps", C.c_double), ("lid_u", C.c_double), ("depth", C.c_double),
                 ("flat_bottom", C.c_double), ("gate_y", _dp), ("vtype", C.c_int),
-                ("lbp", C.c_double), ("rbp", C.c_double)]
+                ("lbp", C.c_double), ("rbp", C.c_double), ("stress_type", C.c_int)]
 
 
 class UphDiag(C.Structure):
@@ -260,6 +260,7 @@ class GpuSolver:
                 u.gate_y = _ptr(gy)
             u.vtype = int(getattr(sc, "vtype", 0))
             u.lbp, u.rbp = 0.0, 0.0
+            u.stress_type = int(getattr(sc, "stress_type", 1))
         opts = UphOpts(int(bool(use_graph)), int(bool(keep_rates)), int(pair_variant),
                        int(pair_block), float(cell_factor),
                        0 if sort_every_step is None else (1 if sort_every_step else -1), float(skin))
