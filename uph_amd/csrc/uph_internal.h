@@ -2124,11 +2124,12 @@ __device__ __forceinline__ void compact_query(int n, int p, const StepBufs& b, c
     // drop the particle itself (always in range, in the row of the cell it was binned
     // into: its current cell's, or with a lazy re-sort possibly the row above or below)
     if (LAZY) {
-#pragma unroll
-      for (int r = 0; r < 3; ++r) {
-        const int self = p - rb[r];
-        if (self >= 0 && self < rl[r]) m[r] &= ~(1ull << self);
-      }
+      // written out per row: a loop over the row arrays here keeps them in memory,
+      // and the compiler then moves them to LDS and reads the walk's row bases there
+      const int s0 = p - rb[0], s1 = p - rb[1], s2 = p - rb[2];
+      if (s0 >= 0 && s0 < rl[0]) m[0] &= ~(1ull << s0);
+      if (s1 >= 0 && s1 < rl[1]) m[1] &= ~(1ull << s1);
+      if (s2 >= 0 && s2 < rl[2]) m[2] &= ~(1ull << s2);
     } else {
       const int self = p - rb[1];
       if (self >= 0 && self < 64) m[1] &= ~(1ull << self);
