@@ -1823,8 +1823,8 @@ __device__ __forceinline__ bool pair_direct(int p, const StepBufs& b, const DevS
 
 // The pair sums of fluid particle p over the set bits of m[0..2] (slot bases
 // rb[0..2]), in that order, into c.
-template <int KT, int MODE>
-__device__ __forceinline__ void walk_pairs(const DevScene& S, const StepBufs& b, int p, double2 xp, CtxOf<MODE>& c,
+template <int KT, int MODE, class Ctx = CtxOf<MODE>>
+__device__ __forceinline__ void walk_pairs(const DevScene& S, const StepBufs& b, int p, double2 xp, Ctx& c,
                                            const int* rb, const unsigned long long* m) {
   // the set bits of the three rows as one stream, advanced without branches;
   // -1 when exhausted

@@ -68,27 +68,112 @@ __global__ __launch_bounds__(64) void k_pair_mixed(int n, StepBufs b0, const Dev
 // D = v_p - vloc_q and g = grad W(x_p - x_q), the same for either order of the pair,
 // over p's pairs: a fluid slot's admitted neighbours (Search.cc:33), a wall slot's
 // fluid neighbours (a wall is only ever a pair's second particle), every neighbour in
-// range while no query is capped.  A fluid query with more than K in range sets C_SCAP
-// (uph_sync reports it).  FP32 terms from the step's mirrors, FP64 sums in candidate
-// order.  The result is {xx, xy (= yx), yy, 0}.
+// range while no query is capped.  FP32 terms from the step's mirrors, FP64 sums in
+// slot order.  The sums of one slot, for walk_pairs (or the direct loop).
+struct StressCtx : PairBase {
+  bool fluid;
+  int fend;   // first id past the scene's fluid particles
+  float vpx, vpy, r0, irp;
+  CSum sxx, sxy, syy;
+
+  __device__ __forceinline__ void init(const DevScene& S, uint32_t tg, float4 ap, float4 bp, float2, float4) {
+    init_base(S, tg);
+    fluid = tag_type(tg) == 1;
+    fend = S.poff + S.nf;
+    vpx = ap.x;
+    vpy = ap.y;
+    r0 = S.frho0;
+    irp = 1.0f / (r0 + bp.x);
+  }
+
+  template <bool EXACT>
+  __device__ __forceinline__ bool admit(const DevScene&, uint32_t idq, double, double, uint32_t, const StepBufs&,
+                                        int) const {
+    return fluid ? (idq > idp ? (int)idq < pb : pb_ok) : (int)idq < fend;
+  }
+
+  template <int KT>
+  __device__ __forceinline__ void add(const DevScene&, double ex, double ey, uint32_t, float4 aq, float4 bq,
+                                      float2 cq, float4) {
+#pragma clang fp contract(off)
+    const float dx = (float)ex, dy = (float)ey;
+    const float r2 = dx * dx + dy * dy;
+    const float rinv = __builtin_amdgcn_rsqf(r2);
+    const float f = grad_factor<KT>(r2, r2 * rinv, rinv);
+    const float gx = f * dx, gy = f * dy;
+    const float rhoq = r0 + bq.x, rlq = r0 + cq.x;
+    const float vr = bq.z * rlq * __builtin_amdgcn_rcpf(rhoq) * irp;
+    const float Dx = vpx - aq.z, Dy = vpy - aq.w;
+    sxx.add(vr * (2.0f * Dx * gx));
+    sxy.add(vr * (Dx * gy + Dy * gx));
+    syy.add(vr * (2.0f * Dy * gy));
+  }
+};
+
+// One thread per slot; the compacted kernel's pass 1 (bit masks of the in-range
+// candidates of the three rows, a trip count uniform over the wavefront) and its bit
+// walk; a slot whose row is longer than 64 slots takes the direct loop.  A fluid query
+// with more than K in range sets C_SCAP (uph_sync reports it).  The result is
+// {xx, xy (= yx), yy, 0}.
 template <int KT, bool ONE>
 __global__ __launch_bounds__(BLK) void k_stress(int n, StepBufs b, const DevScene* __restrict__ sc) {
   const int p = blockIdx.x * BLK + threadIdx.x;
-  if (p >= n) return;
-  const DevScene& S = sc[ONE ? 0 : b.sceneOf[p]];
-  const uint32_t tg = b.tagA[p];
-  PairBase c;
-  c.init_base(S, tg);
-  const bool fluid = tag_type(tg) == 1;
-  float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
-  if (fluid || c.pb_ok) {   // a wall at or past ParticleBoundary is in no pair
-#pragma clang fp contract(off)
-    const double2 xp = b.posA[p];
-    const float4 ap = b.mA[p];
-    const float r0 = S.frho0;
-    const float irp = 1.0f / (r0 + b.mB[p].x);
-    const double cut = S.cutoff;
-    CSum sxx, sxy, syy;
+  const bool live = p < n;
+  const int pp = live ? p : n - 1;
+  const DevScene& S = sc[ONE ? 0 : b.sceneOf[pp]];
+  const uint32_t tg = b.tagA[pp];
+  StressCtx c;
+  c.init(S, tg, b.mA[pp], b.mB[pp], make_float2(0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f));
+  const bool act = live && S.stress2 && (c.fluid || c.pb_ok);   // a wall at or past ParticleBoundary is in no pair
+  const double2 xp = b.posA[pp];
+  const double cut = S.cutoff;
+  int cx, cy;
+  cell_of(S, xp.x, xp.y, cx, cy);
+  const int x0 = cx > 0 ? cx - 1 : 0;
+  const int x1 = cx < S.ncx - 1 ? cx + 1 : S.ncx - 1;
+  int rb[3], rl[3];
+  unsigned long long m[3];
+  bool bad = false;
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const int row = cy - 1 + r;
+    rb[r] = 0;
+    rl[r] = 0;
+    m[r] = 0ull;
+    if (act && row >= 0 && row < S.ncy) {
+      const int base = S.coff + row * S.ncx;
+      rb[r] = b.cellStart[base + x0];
+      rl[r] = b.cellStart[base + x1 + 1] - rb[r];
+    }
+    if (rl[r] > 64) {
+      bad = true;
+      rl[r] = 0;
+    }
+    const int lmax = __builtin_amdgcn_readfirstlane(wave_max(rl[r]));
+    const double2* rp = b.posA + rb[r];
+    unsigned long long bits = 0ull;
+#pragma unroll 1
+    for (int k = 0; k < lmax; k += 4) {
+      double2 xs[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) xs[u] = rp[k + u];   // posA has POS_PAD zeroed slots past n
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const double d2 = dist2(xp, xs[u]);
+        bits |= (unsigned long long)(d2 < cut && k + u < rl[r]) << ((k + u) & 63);   // 0 past the row's end
+      }
+    }
+    m[r] = bits;
+  }
+  // the slot itself, in whichever row holds it (rows written out, as in compact_query)
+  const int s0 = p - rb[0], s1 = p - rb[1], s2 = p - rb[2];
+  if (s0 >= 0 && s0 < rl[0]) m[0] &= ~(1ull << s0);
+  if (s1 >= 0 && s1 < rl[1]) m[1] &= ~(1ull << s1);
+  if (s2 >= 0 && s2 < rl[2]) m[2] &= ~(1ull << s2);
+  if (act && !bad) {
+    if (c.fluid && __popcll(m[0]) + __popcll(m[1]) + __popcll(m[2]) > S.K) atomicOr(&b.ctl[C_SCAP], 1);
+    walk_pairs<KT, UPH_MODE_REF, StressCtx>(S, b, p, xp, c, rb, m);
+  } else if (act) {
     int cnt = 0;
     for_candidates(S, xp, b.cellStart, [&](int q) {
       if (q == p) return;
@@ -99,27 +184,16 @@ __global__ __launch_bounds__(BLK) void k_stress(int n, StepBufs b, const DevScen
       ++cnt;
       const float4 bq = b.mB[q];
       const uint32_t tq = __float_as_uint(bq.w);
-      const bool pair = fluid ? c.template admit<false>(S, tag_id(tq), d2, 0.0, 0u, b, q) : tag_type(tq) == 1;
-      if (!pair) return;
-      const float4 aq = b.mA[q];
-      const float2 cq = b.mC[q];
-      const float dx = (float)ex, dy = (float)ey;
-      const float r2 = dx * dx + dy * dy;
-      const float rinv = __builtin_amdgcn_rsqf(r2);
-      const float f = c.template grad_factor<KT>(r2, r2 * rinv, rinv);
-      const float gx = f * dx, gy = f * dy;
-      const float rhoq = r0 + bq.x, rlq = r0 + cq.x;
-      const float vr = bq.z * rlq * __builtin_amdgcn_rcpf(rhoq) * irp;
-      const float Dx = ap.x - aq.z, Dy = ap.y - aq.w;
-      sxx.add(vr * (2.0f * Dx * gx));
-      sxy.add(vr * (Dx * gy + Dy * gx));
-      syy.add(vr * (2.0f * Dy * gy));
+      if (!c.template admit<false>(S, tag_id(tq), d2, 0.0, 0u, b, q)) return;
+      c.template add<KT>(S, ex, ey, tq, b.mA[q], bq, b.mC[q], make_float4(0.f, 0.f, 0.f, 0.f));
     });
-    if (fluid && cnt > S.K) atomicOr(&b.ctl[C_SCAP], 1);
-    const float mu = S.fmu;
-    out = make_float4(-mu * sxx.val(), -mu * sxy.val(), -mu * syy.val(), 0.f);
+    if (c.fluid && cnt > S.K) atomicOr(&b.ctl[C_SCAP], 1);
   }
-  b.sst[p] = out;
+  if (live) {
+    const float mu = S.fmu;
+    b.sst[p] = act ? make_float4(-mu * c.sxx.val(), -mu * c.sxy.val(), -mu * c.syy.val(), 0.f)
+                   : make_float4(0.f, 0.f, 0.f, 0.f);
+  }
 }
 
 int launch_stress(uph_handle* h, hipStream_t st) {
