@@ -209,6 +209,7 @@ def main():
         # needs dt < h^2 / (10 gamma kappa) = 3.2 us at h = 1.84 mm, below this dt of 7.3 us
         sc.kappa, sc.mu, sc.eps, sc.alpha_av = 0.0015, 1e-6, 1.0, 0.0
         sc.stress_type = args.stress_type
+    stress2 = args.scheme == "ref" and args.stress_type == 2
     if args.vtype:
         sc.vtype, sc.flat_bottom, sc.depth = args.vtype, 4.0, 2.0   # the tank's width, the column's height
     log(f"workload: {sc.n_fluid} fluid + {sc.n_boundary} wall particles, {args.kernel} kernel, "
@@ -269,7 +270,10 @@ def main():
                      "traffic": (traffic_from_profiles() if args.kernel == "cubic" and lazy and
                                  args.h_over_dx == 1.3 and args.scheme == "config" and not args.vtype
                                  and args.skin <= 0 else None),
-                     "kernel": f"k_pair_compact<256, {int(sc.kernel)}, {int(sc.mode)}, true, "
+                     # stressType 2 runs the stress pass inside the pair phase, and its pair
+                     # kernel is the mode-2 instance
+                     "kernel": (f"k_stress<{int(sc.kernel)}, true> + " if stress2 else "") +
+                               f"k_pair_compact<256, {int(sc.kernel)}, {2 if stress2 else int(sc.mode)}, true, "
                                f"{'true' if 4 * math.pi * args.h_over_dx ** 2 > 0.75 * sc.K else 'false'}, "
                                f"{'true' if lazy else 'false'}> ({args.kernel}, {args.scheme}, one scene"
                                f"{', lazy re-sort' if lazy else ''})",
