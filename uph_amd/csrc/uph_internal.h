@@ -299,10 +299,12 @@ struct StepBufs {
   int* sceneOf;
   // capped search: K-th key of overflowing queries, valid when its step equals the
   // step stamp ctl[3] (a counter that no restart resets, so a stamp never repeats),
-  // or its negation when the per-lane search has also left the query's in-range
-  // candidates in kmask (six 64-bit words: row r's slots rb[r] + [0, 64) in word 2r,
-  // + [64, 128) in word 2r + 1; word u of slot q at kmask[u * kmask_n + q], so that a
-  // wavefront's stores and loads of one word are contiguous)
+  // or its negation when the per-lane search has also left the query's pair
+  // partners in kmask (twelve 64-bit words: row r's slots rb[r] + [0, 64) in word 2r,
+  // + [64, 128) in word 2r + 1, for the admitted candidates above the query; words
+  // 6-11 the same for its candidates below, pending their own K-th keys; word u of
+  // slot q at kmask[u * kmask_n + q], so that a wavefront's stores and loads of one
+  // word are contiguous)
   KthRec* kth;
   unsigned long long* kmask;
   int kmask_n;
@@ -1073,6 +1075,12 @@ __device__ void wave_kth(const DevScene& S, int q, const double2* __restrict__ p
 // them by full (d2, id) keys.  Returns false, leaving the query to the wavefront
 // search, when that bucket holds more than LANE_CAP or a row is longer than 128
 // slots.  +inf when at most K are in range.
+// With the K-th key known, the query's pair partners by the rules of
+// Search.cc:27-40 go to mask_out in two sets of six words (row r's slots
+// rb[r] + [0, 64) in word 2r, + [64, 128) in word 2r + 1): words 0-5 the candidates
+// above q (by original index, below ParticleBoundary) that rank within q's K
+// nearest, admitted whatever their own lists hold; words 6-11 the candidates below
+// q, whose admission needs their own K-th key (the capped pass reads it).
 __device__ bool lane_kth(const DevScene& S, int q, double2 xq, const int* rb, const int* rl,
                          const double2* __restrict__ pos, const uint32_t* __restrict__ tag, LaneSel& L,
                          double& kd2, uint32_t& kid, unsigned long long* __restrict__ mask_out,
@@ -1118,66 +1126,112 @@ __device__ bool lane_kth(const DevScene& S, int q, double2 xq, const int* rb, co
       }
       mk[2 * r + w] = mm;
     }
-#pragma unroll
-  for (int u = 0; u < 6; ++u) mask_out[(size_t)u * mask_stride] = mk[u];
-  if (n <= S.K) return true;
-  int below = 0, kb = 0, m = 0;
+  // the bucket holding rank K - 1; none (LANE_BUCKETS) when every key in range ranks
+  // within K
+  int below = 0, kb = LANE_BUCKETS, m = 0;
+  if (n > S.K) {
 #pragma unroll 1
-  for (kb = 0; kb < LANE_BUCKETS; ++kb) {
-    const uint32_t w = L.cnt[kb >> 1][lane];
-    m = (int)((kb & 1) ? (w >> 16) : (w & 0xffffu));
-    if (below + m >= S.K) break;
-    below += m;
+    for (kb = 0; kb < LANE_BUCKETS; ++kb) {
+      const uint32_t w = L.cnt[kb >> 1][lane];
+      m = (int)((kb & 1) ? (w >> 16) : (w & 0xffffu));
+      if (below + m >= S.K) break;
+      below += m;
+    }
+    if (m > LANE_CAP) return false;
   }
-  if (m > LANE_CAP) return false;
+  const uint32_t idq = tag_id(tag[q]);
+  const bool pb_ok = (int)idq < S.pb_gid;
+  // one pass over the in-range candidates, four per trip with their loads issued
+  // together: a lower bucket ranks within K, a higher one does not; the selected
+  // bucket's slots are collected for the rank below
+  unsigned long long am[6];
   int j = 0;
 #pragma unroll
   for (int u = 0; u < 6; ++u) {
     const int base = rb[u >> 1] + 64 * (u & 1);
-    unsigned long long mm = mk[u];
+    unsigned long long mm = mk[u], aw = 0ull, pw = 0ull;
 #pragma unroll 1
     while (mm) {
-      const int t = base + __ffsll((long long)mm) - 1;
-      mm &= mm - 1ull;
-      const double d2 = dist2(xq, pos[t]);
-      if (min((int)(d2 * scale), LANE_BUCKETS - 1) == kb) L.slot[j++][lane] = (uint32_t)t;
-    }
-  }
-  // rank K - 1 - below of the bucket: the minimum above the previous one, that many times
-  double td = -1.0;
-  uint32_t ti = 0u;
-#pragma unroll 1
-  for (int rr = 0; rr <= S.K - 1 - below; ++rr) {
-    double bd = INFINITY;
-    uint32_t bi = 0xffffffffu;
-    // four members per trip, their loads issued together (a position past the
-    // bucket's end repeats its last member and is not compared)
-#pragma unroll 1
-    for (int i = 0; i < m; i += 4) {
+      int kk[4];
       double2 x[4];
       uint32_t g[4];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const uint32_t t = L.slot[min(i + u, m - 1)][lane];
-        x[u] = pos[t];
-        g[u] = tag[t];
+      for (int v = 0; v < 4; ++v) {
+        kk[v] = mm ? __ffsll((long long)mm) - 1 : -1;
+        mm &= mm - 1ull;
+        const int t = kk[v] >= 0 ? base + kk[v] : q;
+        x[v] = pos[t];
+        g[v] = tag[t];
       }
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const double d2 = dist2(xq, x[u]);
-        const uint32_t id = tag_id(g[u]);
-        const bool above = d2 > td || (d2 == td && id > ti);
-        if (i + u < m && above && key_lt(d2, id, bd, bi)) {
-          bd = d2;
-          bi = id;
-        }
+      for (int v = 0; v < 4; ++v) {
+        const bool in = kk[v] >= 0;
+        const double d2 = dist2(xq, x[v]);
+        const int bk = min((int)(d2 * scale), LANE_BUCKETS - 1);
+        const uint32_t id = tag_id(g[v]);
+        if (in && bk == kb) L.slot[j++][lane] = (uint32_t)(base + kk[v]);
+        const int sh = kk[v] & 63;
+        aw |= (unsigned long long)(in && bk < kb && id > idq && (int)id < S.pb_gid) << sh;
+        pw |= (unsigned long long)(in && id < idq && pb_ok) << sh;
       }
     }
-    td = bd;
-    ti = bi;
+    am[u] = aw;
+    mask_out[(size_t)(6 + u) * mask_stride] = pw;
   }
-  kd2 = td;
-  kid = ti;
+  if (n > S.K) {
+    // rank K - 1 - below of the bucket: the minimum above the previous one, that many times
+    double td = -1.0;
+    uint32_t ti = 0u;
+#pragma unroll 1
+    for (int rr = 0; rr <= S.K - 1 - below; ++rr) {
+      double bd = INFINITY;
+      uint32_t bi = 0xffffffffu;
+      // four members per trip, their loads issued together (a position past the
+      // bucket's end repeats its last member and is not compared)
+#pragma unroll 1
+      for (int i = 0; i < m; i += 4) {
+        double2 x[4];
+        uint32_t g[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const uint32_t t = L.slot[min(i + u, m - 1)][lane];
+          x[u] = pos[t];
+          g[u] = tag[t];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const double d2 = dist2(xq, x[u]);
+          const uint32_t id = tag_id(g[u]);
+          const bool above = d2 > td || (d2 == td && id > ti);
+          if (i + u < m && above && key_lt(d2, id, bd, bi)) {
+            bd = d2;
+            bi = id;
+          }
+        }
+      }
+      td = bd;
+      ti = bi;
+    }
+    kd2 = td;
+    kid = ti;
+    // the selected bucket's members within the K nearest, above q and below
+    // ParticleBoundary, join the admitted words (each slot lies in one word of its row)
+#pragma unroll 1
+    for (int i = 0; i < m; ++i) {
+      const int t = (int)L.slot[i][lane];
+      const double d2 = dist2(xq, pos[t]);
+      const uint32_t id = tag_id(tag[t]);
+      const bool ok = id > idq && (int)id < S.pb_gid && key_le(d2, id, td, ti);
+#pragma unroll
+      for (int u = 0; u < 6; ++u) {
+        const int s = t - (rb[u >> 1] + 64 * (u & 1));
+        const int len = rl[u >> 1] - 64 * (u & 1);
+        if (ok && s >= 0 && s < 64 && s < len) am[u] |= 1ull << s;
+      }
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < 6; ++u) mask_out[(size_t)u * mask_stride] = am[u];
   return true;
 }
 
@@ -1204,7 +1258,7 @@ __device__ void wave_overflow(const StepBufs& b, const DevScene* __restrict__ sc
       double kd2;
       uint32_t kid;
       if (S.nsec == 0 && lane_kth(S, p, xp, rbp, rlp, b.posA, b.tagA, *ls, kd2, kid, b.kmask + p, b.kmask_n)) {
-        b.kth[p] = KthRec{kd2, kid, -b.it[1]};   // negated: kmask holds its in-range candidates
+        b.kth[p] = KthRec{kd2, kid, -b.it[1]};   // negated: kmask holds its pair partners
         done = true;
       }
     }
@@ -1930,8 +1984,9 @@ __device__ __forceinline__ bool exact_pairs(int p, const StepBufs& b, const DevS
   // tests distances and decides the candidates q with a larger original index, whose
   // admission needs only p's K-th key (q ranks within p's K nearest, q below
   // ParticleBoundary); it marks as pending those with a smaller one, whose admission
-  // needs q's K-th key (p ranks within q's K nearest).  Pass 1b reads those keys, two
-  // candidates in flight.
+  // needs q's K-th key (p ranks within q's K nearest).  A query whose K-th key the
+  // per-lane search found takes both sets from it instead.  Pass 1b reads the pending
+  // candidates' keys, two in flight.
   unsigned long long m[6];
 #pragma unroll
   for (int r = 0; r < 3; ++r) {
@@ -1943,58 +1998,35 @@ __device__ __forceinline__ bool exact_pairs(int p, const StepBufs& b, const DevS
       const int k1 = rl[r] < k0 + 64 ? rl[r] : k0 + 64;
       unsigned long long am = 0ull, pm = 0ull;
       if (have_mask) {
-        // the per-lane K-th search left this word's in-range candidates: four per trip
-        unsigned long long mm = b.kmask[(size_t)(2 * r + w) * b.kmask_n + p];
+        // the per-lane K-th search left this word's admitted candidates above p and
+        // its candidates below p, which wait for their own K-th keys
+        am = b.kmask[(size_t)(2 * r + w) * b.kmask_n + p];
+        pm = b.kmask[(size_t)(6 + 2 * r + w) * b.kmask_n + p];
+      } else {
+        // four candidates per trip, their loads issued together (a slot past the row's
+        // end repeats its last one and sets no bit)
 #pragma unroll 1
-        while (mm) {
-          int kk[4];
+        for (int k = k0; k < k1; k += 4) {
           double2 x[4];
           uint32_t g[4];
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
-            kk[u] = mm ? __ffsll((long long)mm) - 1 : -1;
-            mm &= mm - 1ull;
-            const int qq = kk[u] >= 0 ? rb[r] + k0 + kk[u] : p;
-            x[u] = b.posA[qq];
-            g[u] = b.tagA[qq];
+            const int kk = k + u < k1 ? k + u : k1 - 1;
+            x[u] = rp[kk];
+            g[u] = tp[kk];
           }
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
             const double d2 = dist2(xp, x[u]);
             const uint32_t idq = tag_id(g[u]);
-            const bool in = kk[u] >= 0;
+            const bool in = k + u < k1 && d2 < cut;
             const bool adm = in && idq > idp && (int)idq < S.pb_gid && key_le(d2, idq, kpd2, kpid);
-            const bool pen = in && idq < idp && pb_ok;
-            const int sh = kk[u] & 63;
+            const bool pen = in && idq < idp && pb_ok;   // the particle itself is neither
+            const int sh = (k + u - k0) & 63;   // past the row's end both bits are 0
             am |= (unsigned long long)adm << sh;
             pm |= (unsigned long long)pen << sh;
           }
         }
-      } else {
-      // four candidates per trip, their loads issued together (a slot past the row's
-      // end repeats its last one and sets no bit)
-#pragma unroll 1
-      for (int k = k0; k < k1; k += 4) {
-        double2 x[4];
-        uint32_t g[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int kk = k + u < k1 ? k + u : k1 - 1;
-          x[u] = rp[kk];
-          g[u] = tp[kk];
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const double d2 = dist2(xp, x[u]);
-          const uint32_t idq = tag_id(g[u]);
-          const bool in = k + u < k1 && d2 < cut;
-          const bool adm = in && idq > idp && (int)idq < S.pb_gid && key_le(d2, idq, kpd2, kpid);
-          const bool pen = in && idq < idp && pb_ok;   // the particle itself is neither
-          const int sh = (k + u - k0) & 63;   // past the row's end both bits are 0
-          am |= (unsigned long long)adm << sh;
-          pm |= (unsigned long long)pen << sh;
-        }
-      }
       }
       const int base = rb[r] + k0;
       int ka = pm ? __ffsll((long long)pm) - 1 : -1;

@@ -433,7 +433,7 @@ void fill_scene(DevScene& S, const uph_scene& u, double cell_factor) {
   S.dx = u.dx;
 }
 
-// kmask (48 bytes per slot) is written and read only by the per-lane K-th search of
+// kmask (96 bytes per slot) is written and read only by the per-lane K-th search of
 // handles whose queries are expected to be capped; other handles leave it null.
 int alloc_bufs(StepBufs& b, int n, int ncells, int nblocks_scan, Err& err, bool search_only, bool capped) {
   memset(&b, 0, sizeof b);
@@ -454,7 +454,7 @@ int alloc_bufs(StepBufs& b, int n, int ncells, int nblocks_scan, Err& err, bool 
   if (search_only) return 0;
   if (dalloc(&b.velA, n, err) || dalloc(&b.velB, n, err) || dalloc(&b.rhoA, n, err) ||
       dalloc(&b.rhoB, n, err) || dalloc(&b.rtB, n, err) ||
-      (capped && dalloc(&b.kmask, (size_t)n * 6, err)) ||
+      (capped && dalloc(&b.kmask, (size_t)n * 12, err)) ||
       dalloc(&b.mA, n, err) || dalloc(&b.mB, n, err) || dalloc(&b.deckPos, n, err) ||
       dalloc(&b.deckVel, n, err) || dalloc(&b.deckLvel, n, err) || dalloc(&b.massD, n, err) ||
       dalloc(&b.presA, n, err) || dalloc(&b.presB, n, err) || dalloc(&b.deckP, n, err) ||
