@@ -1,7 +1,9 @@
 """The capped (K-nearest) path's hand-off from the per-lane K-th search to the capped
-pass, bit for bit: the per-lane search leaves each capped query's in-range candidates
-as bits (kmask) and the capped pass visits only those.  Three instances are checked
-against runs that take other code paths over the same inputs:
+pass, bit for bit: the per-lane search leaves each capped query's pair partners as
+bits (kmask: the admitted candidates above it by original index, and the candidates
+below it, which wait for their own K-th keys), and the capped pass walks those
+without testing distances again.  Three instances are checked against runs that take
+other code paths over the same inputs:
 
 * a batch holding a capped scene without mirror sections (the reference's dambreak
   deck, virtualType 77, h = 2.691 dx: about 88 particles in range, K = 40) and an

@@ -2256,6 +2256,10 @@ struct uph_handle {
   // some scene's queries are expected to be capped (4 pi (h/dx)^2 in range, a lattice's
   // count, above 3/4 of K): the pair kernel carries the per-lane K-th key search
   bool capped = false;
+  // every scene's queries are expected to have more than K in range (4 pi (h/dx)^2
+  // above K) and no scene has mirror rows: the capped pass recomputes every slot of
+  // every step, so the step's first kernel only finds the K-th keys (k_kth)
+  bool kth_all = false;
   // the scenes take stressType 2 (UPH_MODE_REF2 kernels, k_stress before the pair pass)
   bool stress2 = false;
   // lazy re-sort (b.lazy): the next step sorts the state from scratch (after uph_create

@@ -63,6 +63,32 @@ __global__ __launch_bounds__(64) void k_pair_mixed(int n, StepBufs b0, const Dev
   compact_query<KT, MODE, false, false, LAZY>(n, waves[blockIdx.x] * 64 + (int)threadIdx.x, b, sc, lds[0]);
 }
 
+// The step's first kernel for handles whose every query is expected to have more than
+// K in range and that have no mirror rows (uph_handle::kth_all): the capped pass
+// recomputes every slot of such a step, so this kernel only finds each fluid query's
+// K-th key and pair partners (lane_kth; the wavefront search for the queries it leaves)
+// and raises the step's overflow flag.  Each lane takes its own scene, so wavefronts
+// of two scenes need no other pass.  Its registers would allow six wavefronts per
+// SIMD; it runs four (KTH_BLOCK_LDS), since with more the candidate positions of the
+// resident wavefronts outgrow the caches: on the capped 1 M reference-scheme step it
+// took 684 us at six, 594 us at four and 605 us at three wavefronts per SIMD in one
+// run, and 1,061 us at eight against 667 us at six in another.
+constexpr int KTH_BLOCK_LDS = 40 * 1024;   // a quarter of the CU's 160 KB: four blocks of four wavefronts
+template <bool ONE>
+__global__ __launch_bounds__(BLK) void k_kth(int n, StepBufs b, const DevScene* __restrict__ sc) {
+  __shared__ KthLaneLds lds[BLK / 64];
+  const int p = xcd_block() * BLK + threadIdx.x;
+  const bool live = p < n;
+  const int pp = live ? p : n - 1;
+  const bool fluid = live && tag_type(b.tagA[pp]) == 1;
+  const int sidx = ONE ? 0 : b.sceneOf[pp];
+  const double2 xp = b.posA[pp];
+  int rb[3], rl[3];
+  candidate_rows(sc[sidx], xp, b.cellStart, rb, rl);
+  // every lane of the wavefront reaches this, fluid or not
+  wave_overflow<true>(b, sc, p, fluid, lds[threadIdx.x >> 6].k, sidx, xp, rb, rl, &lds[threadIdx.x >> 6].l);
+}
+
 // ClassicShearStress (derivatives.c:1686-1721) in gather form for stressType 2, before
 // the pair pass: S_p = -mu sum_q m_q rholoc_q / (rho_q rho_p) (D g^T + g D^T) with
 // D = v_p - vloc_q and g = grad W(x_p - x_q), the same for either order of the pair,
@@ -223,6 +249,15 @@ static int launch_pair_pass_b(uph_handle* h, hipStream_t st, const StepBufs& b) 
   if (h->opts.pair_variant == 1) {
     hipLaunchKernelGGL((k_pair<false, KT, MODE, false, LAZY>), dim3(grid8_of(h->n)), dim3(BLK), 0, st, h->n, b,
                        h->dsc);
+  } else if (!LAZY && MODE != UPH_MODE_REF2 && h->kth_all) {
+    // the pair sums of such a step all come from the capped pass; the dynamic LDS only
+    // brings the block to KTH_BLOCK_LDS
+    const int pad = KTH_BLOCK_LDS - (int)sizeof(KthLaneLds) * (BLK / 64);
+    static_assert(KTH_BLOCK_LDS >= (int)sizeof(KthLaneLds) * (BLK / 64), "k_kth LDS");
+    if (h->nscenes == 1)
+      hipLaunchKernelGGL((k_kth<true>), dim3(grid8_of(h->n)), dim3(BLK), pad, st, h->n, b, h->dsc);
+    else
+      hipLaunchKernelGGL((k_kth<false>), dim3(grid8_of(h->n)), dim3(BLK), pad, st, h->n, b, h->dsc);
   } else {
     const int pbk = h->opts.pair_block;
     const int g = (((h->n + pbk - 1) / pbk + 7) / 8) * 8;

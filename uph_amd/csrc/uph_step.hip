@@ -747,6 +747,7 @@ extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts*
   if (h->opts.skin > 1.0) h->opts.skin = 1.0;
   const double cell_factor = lazy ? h->opts.cell_factor * (1.0 + h->opts.skin) : h->opts.cell_factor;
   long n = 0, nc = 0, gtot = 0, wtot = 0, gfluid = 0;
+  bool kth_all = !stress2 && h->opts.pair_variant == 0;
   for (int s = 0; s < n_scenes; ++s) {
     const uph_scene& u = scenes[s];
     const int np = u.n_fluid + u.n_fixed + u.n_moving;
@@ -790,6 +791,7 @@ extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts*
     // mirror rows: one test per section and fluid row, ids after the scene's rows
     if (4.0 * M_PI * (u.h / u.dx) * (u.h / u.dx) > 0.75 * u.K) h->capped = true;
     S.nsec = u.n_fluid > 0 ? mirror_sections(u.vtype, S.sec) : 0;
+    if (!(4.0 * M_PI * (u.h / u.dx) * (u.h / u.dx) > u.K) || S.nsec > 0) kth_all = false;
     S.nwords = (u.n_fluid + 31) / 32;
     S.gw = (int)wtot;
     S.gbase = (int)n + np;
@@ -812,6 +814,7 @@ extern "C" int uph_create(const uph_scene* scenes, int n_scenes, const uph_opts*
     return -1;
   }
   h->n = (int)n;
+  h->kth_all = kth_all && h->capped && !lazy;
   h->kernel = scenes[0].kernel;
   h->mode = scenes[0].mode;
   h->ncells = (int)nc;
