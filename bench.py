@@ -210,6 +210,9 @@ def main():
         sc.kappa, sc.mu, sc.eps, sc.alpha_av = 0.0015, 1e-6, 1.0, 0.0
         sc.stress_type = args.stress_type
     stress2 = args.scheme == "ref" and args.stress_type == 2
+    # (for the kernel's name in the output) every query expected to be capped and no
+    # mirror rows: the handle's first pair kernel is the K-th key search alone
+    kth_all = 4 * math.pi * args.h_over_dx ** 2 > sc.K and not args.vtype and not stress2
     if args.vtype:
         sc.vtype, sc.flat_bottom, sc.depth = args.vtype, 4.0, 2.0   # the tank's width, the column's height
     log(f"workload: {sc.n_fluid} fluid + {sc.n_boundary} wall particles, {args.kernel} kernel, "
@@ -271,12 +274,17 @@ def main():
                                  args.h_over_dx == 1.3 and args.scheme == "config" and not args.vtype
                                  and args.skin <= 0 else None),
                      # stressType 2 runs the stress pass inside the pair phase, and its pair
-                     # kernel is the mode-2 instance
-                     "kernel": (f"k_stress<{int(sc.kernel)}, true> + " if stress2 else "") +
-                               f"k_pair_compact<256, {int(sc.kernel)}, {2 if stress2 else int(sc.mode)}, true, "
-                               f"{'true' if 4 * math.pi * args.h_over_dx ** 2 > 0.75 * sc.K else 'false'}, "
-                               f"{'true' if lazy else 'false'}> ({args.kernel}, {args.scheme}, one scene"
-                               f"{', lazy re-sort' if lazy else ''})",
+                     # kernel is the mode-2 instance; a scene whose every query is expected
+                     # to be capped (more than K in range on the lattice, no mirror rows)
+                     # runs the K-th key search alone first, and all its pair sums in the
+                     # capped pass
+                     "kernel": (f"k_kth<true> ({args.kernel}, {args.scheme}, one scene; K-th keys only, "
+                                f"the pair sums in the capped pass)" if kth_all else
+                                (f"k_stress<{int(sc.kernel)}, true> + " if stress2 else "") +
+                                f"k_pair_compact<256, {int(sc.kernel)}, {2 if stress2 else int(sc.mode)}, true, "
+                                f"{'true' if 4 * math.pi * args.h_over_dx ** 2 > 0.75 * sc.K else 'false'}, "
+                                f"{'true' if lazy else 'false'}> ({args.kernel}, {args.scheme}, one scene"
+                                f"{', lazy re-sort' if lazy else ''})"),
                      "kernel_ms": ms_pair, "sort_ms": ph["sort"], "capped_pass_ms": ph["exact"],
                      "capped_pass_kernel": post},
     }
