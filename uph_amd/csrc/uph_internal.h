@@ -1069,6 +1069,19 @@ __device__ void wave_kth(const DevScene& S, int q, const double2* __restrict__ p
 }
 
 // K-th key of query q (slot) at xq, candidate rows rb/rl, found by this lane alone:
+// Where the per-lane search reads a candidate's position, given its row r and slot s:
+// posA itself, or a copy of the block's candidate rows in LDS (k_kth), row r's slots
+// [lo_r, hi_r) at so[r] + s.
+struct GlobalPos {
+  const double2* __restrict__ p;
+  __device__ __forceinline__ double2 operator()(int, int s) const { return p[s]; }
+};
+struct StagedPos {
+  const double2* sp;
+  int so[3];
+  __device__ __forceinline__ double2 operator()(int r, int s) const { return sp[so[r] + s]; }
+};
+
 // a histogram of the in-range d2 over LANE_BUCKETS equal buckets of [0, cutoff)
 // finds the bucket that holds rank K - 1 (bucketing is monotone in d2, so every key
 // of a lower bucket is smaller); its slots are collected and the rank taken among
@@ -1081,10 +1094,13 @@ __device__ void wave_kth(const DevScene& S, int q, const double2* __restrict__ p
 // above q (by original index, below ParticleBoundary) that rank within q's K
 // nearest, admitted whatever their own lists hold; words 6-11 the candidates below
 // q, whose admission needs their own K-th key (the capped pass reads it).
+// The candidates of the histogram and collecting passes are read through `at`; the
+// selected bucket's members from pos.
+template <class PosAt>
 __device__ bool lane_kth(const DevScene& S, int q, double2 xq, const int* rb, const int* rl,
                          const double2* __restrict__ pos, const uint32_t* __restrict__ tag, LaneSel& L,
                          double& kd2, uint32_t& kid, unsigned long long* __restrict__ mask_out,
-                         int mask_stride) {
+                         int mask_stride, const PosAt& at) {
   const int lane = threadIdx.x & 63;
   kd2 = INFINITY;
   kid = 0xffffffffu;
@@ -1111,7 +1127,7 @@ __device__ bool lane_kth(const DevScene& S, int q, double2 xq, const int* rb, co
       for (int k = k0; k < k1; k += 4) {
         double2 x[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) x[u] = pos[rb[r] + (k + u < k1 ? k + u : k1 - 1)];
+        for (int u = 0; u < 4; ++u) x[u] = at(r, rb[r] + (k + u < k1 ? k + u : k1 - 1));
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           const double d2 = dist2(xq, x[u]);
@@ -1159,9 +1175,9 @@ __device__ bool lane_kth(const DevScene& S, int q, double2 xq, const int* rb, co
       for (int v = 0; v < 4; ++v) {
         kk[v] = mm ? __ffsll((long long)mm) - 1 : -1;
         mm &= mm - 1ull;
-        const int t = kk[v] >= 0 ? base + kk[v] : q;
-        x[v] = pos[t];
-        g[v] = tag[t];
+        // an exhausted word reads its row's first slot (the query's own, for tag)
+        x[v] = at(u >> 1, kk[v] >= 0 ? base + kk[v] : rb[u >> 1]);
+        g[v] = tag[kk[v] >= 0 ? base + kk[v] : q];
       }
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
@@ -1241,10 +1257,11 @@ __device__ bool lane_kth(const DevScene& S, int q, double2 xq, const int* rb, co
 // kernel run.  Every lane of the wavefront calls this.  With ROWS, each lane
 // passes its own scene index, position and candidate rows, and the wavefront takes
 // them from the query's lane instead of reading them again.
-template <bool ROWS>
+template <bool ROWS, class PosAt = GlobalPos>
 __device__ void wave_overflow(const StepBufs& b, const DevScene* __restrict__ sc, int p, bool ovf, KthLds& L,
                               int sidx = 0, double2 xp = double2{}, const int* rbp = nullptr,
-                              const int* rlp = nullptr, LaneSel* ls = nullptr) {
+                              const int* rlp = nullptr, LaneSel* ls = nullptr,
+                              const PosAt& at = GlobalPos{nullptr}) {
   unsigned long long m = __ballot(ovf);
   if (m == 0ull) return;
   const bool lead = (threadIdx.x & 63) == 0;
@@ -1257,7 +1274,8 @@ __device__ void wave_overflow(const StepBufs& b, const DevScene* __restrict__ sc
       const DevScene& S = sc[sidx];
       double kd2;
       uint32_t kid;
-      if (S.nsec == 0 && lane_kth(S, p, xp, rbp, rlp, b.posA, b.tagA, *ls, kd2, kid, b.kmask + p, b.kmask_n)) {
+      if (S.nsec == 0 && lane_kth(S, p, xp, rbp, rlp, b.posA, b.tagA, *ls, kd2, kid, b.kmask + p, b.kmask_n,
+                                        at)) {
         b.kth[p] = KthRec{kd2, kid, -b.it[1]};   // negated: kmask holds its pair partners
         done = true;
       }
@@ -2175,7 +2193,7 @@ __device__ __forceinline__ void compact_query(int n, int p, const StepBufs& b, c
     compact_pairs<KT, MODE, LAZY>(S, b, p, tg, itime, xp, vn, mass, rb, m);
   };
   run();
-  wave_overflow<true>(b, sc, p, ovf, lds, sidx, xp, rb, rlf, ls);
+  wave_overflow<true>(b, sc, p, ovf, lds, sidx, xp, rb, rlf, ls, GlobalPos{b.posA});
 }
 
 // ---------------------------------------------------------------- search

@@ -1,5 +1,7 @@
 // uph_pair.hip -- the step's pair kernels (compacted, direct and capped passes, the
 // wavefronts of two scenes) and their launches; the device code is in uph_internal.h.
+#include <climits>
+
 #include "uph_internal.h"
 
 // EXACT (the capped path) runs as a small grid-stride launch that returns at
@@ -68,25 +70,99 @@ __global__ __launch_bounds__(64) void k_pair_mixed(int n, StepBufs b0, const Dev
 // recomputes every slot of such a step, so this kernel only finds each fluid query's
 // K-th key and pair partners (lane_kth; the wavefront search for the queries it leaves)
 // and raises the step's overflow flag.  Each lane takes its own scene, so wavefronts
-// of two scenes need no other pass.  Its registers would allow six wavefronts per
-// SIMD; it runs four (KTH_BLOCK_LDS), since with more the candidate positions of the
-// resident wavefronts outgrow the caches: on the capped 1 M reference-scheme step it
-// took 684 us at six, 594 us at four and 605 us at three wavefronts per SIMD in one
-// run, and 1,061 us at eight against 667 us at six in another.
+// of two scenes need no other pass.
+//
+// The search is bound by its candidate loads (each a 16-byte position per lane; about
+// 720 per wavefront, served from L1 at about 21 cache accesses each), not by their
+// misses (1 % of them reach L2) or its arithmetic.  So a one-scene block whose fluid
+// slots share one cell row copies the union of their three candidate rows into LDS
+// with coalesced loads (KTH_STAGE positions at most, else it reads posA as before),
+// and the histogram and collecting passes read them there.
+//
+// Its registers would allow six wavefronts per SIMD; it runs four (KTH_BLOCK_LDS),
+// since with more the candidate positions of the resident wavefronts outgrow the
+// caches: on the capped 1 M reference-scheme step it took 684 us at six, 594 us at four
+// and 605 us at three wavefronts per SIMD in one run, and 1,061 us at eight against
+// 667 us at six in another.
 constexpr int KTH_BLOCK_LDS = 40 * 1024;   // a quarter of the CU's 160 KB: four blocks of four wavefronts
+constexpr int KTH_STAGE = 1272;            // staged positions: 20,352 bytes with the search's 20,480
+static_assert(sizeof(KthLaneLds) * (BLK / 64) + sizeof(double2) * KTH_STAGE + 8 * sizeof(int) <= KTH_BLOCK_LDS,
+              "k_kth LDS");
 template <bool ONE>
 __global__ __launch_bounds__(BLK) void k_kth(int n, StepBufs b, const DevScene* __restrict__ sc) {
   __shared__ KthLaneLds lds[BLK / 64];
+  __shared__ double2 spos[KTH_STAGE];
+  // the block's fluid cell rows (min, max) and, per candidate row r, the union of the
+  // fluid slots' ranges [lo_r, hi_r)
+  __shared__ int sred[8];
   const int p = xcd_block() * BLK + threadIdx.x;
   const bool live = p < n;
   const int pp = live ? p : n - 1;
   const bool fluid = live && tag_type(b.tagA[pp]) == 1;
   const int sidx = ONE ? 0 : b.sceneOf[pp];
+  const DevScene& S = sc[sidx];
   const double2 xp = b.posA[pp];
   int rb[3], rl[3];
-  candidate_rows(sc[sidx], xp, b.cellStart, rb, rl);
+  candidate_rows(S, xp, b.cellStart, rb, rl);
+  bool staged = false;
+  int so[3] = {0, 0, 0};
+  if constexpr (ONE) {
+    int cx, cy;
+    cell_of(S, xp.x, xp.y, cx, cy);
+    if (threadIdx.x == 0) {
+      sred[0] = INT_MAX;
+      sred[1] = INT_MIN;
+      sred[2] = sred[3] = sred[4] = INT_MAX;
+      sred[5] = sred[6] = sred[7] = INT_MIN;
+    }
+    __syncthreads();
+    // wavefront minima and maxima first (a non-fluid lane contributes nothing), then
+    // one LDS atomic per wavefront and value
+    const int v[8] = {fluid ? cy : INT_MAX, fluid ? cy : INT_MIN, fluid ? rb[0] : INT_MAX,
+                      fluid ? rb[1] : INT_MAX, fluid ? rb[2] : INT_MAX, fluid ? rb[0] + rl[0] : INT_MIN,
+                      fluid ? rb[1] + rl[1] : INT_MIN, fluid ? rb[2] + rl[2] : INT_MIN};
+    const int w0 = -wave_max(-v[0]), w1 = wave_max(v[1]);
+    const int w2 = -wave_max(-v[2]), w3 = -wave_max(-v[3]), w4 = -wave_max(-v[4]);
+    const int w5 = wave_max(v[5]), w6 = wave_max(v[6]), w7 = wave_max(v[7]);
+    if ((threadIdx.x & 63) == 0) {
+      atomicMin(&sred[0], w0);
+      atomicMax(&sred[1], w1);
+      atomicMin(&sred[2], w2);
+      atomicMin(&sred[3], w3);
+      atomicMin(&sred[4], w4);
+      atomicMax(&sred[5], w5);
+      atomicMax(&sred[6], w6);
+      atomicMax(&sred[7], w7);
+    }
+    __syncthreads();
+    // one cell row for every fluid slot of the block (and some fluid slot): row r's
+    // ranges are contiguous cells of one cell row, so their union is one range
+    int lo0 = 0, lo1 = 0, lo2 = 0, len0 = 0, len1 = 0, len2 = 0;
+    if (sred[0] == sred[1]) {
+      lo0 = sred[2];
+      lo1 = sred[3];
+      lo2 = sred[4];
+      len0 = sred[5] - lo0;
+      len1 = sred[6] - lo1;
+      len2 = sred[7] - lo2;
+      staged = len0 + len1 + len2 <= KTH_STAGE;
+    }
+    if (staged) {
+      for (int i = threadIdx.x; i < len0 + len1 + len2; i += BLK)
+        spos[i] = b.posA[i < len0 ? lo0 + i : i < len0 + len1 ? lo1 + (i - len0) : lo2 + (i - len0 - len1)];
+      so[0] = -lo0;
+      so[1] = len0 - lo1;
+      so[2] = len0 + len1 - lo2;
+    }
+    __syncthreads();
+  }
   // every lane of the wavefront reaches this, fluid or not
-  wave_overflow<true>(b, sc, p, fluid, lds[threadIdx.x >> 6].k, sidx, xp, rb, rl, &lds[threadIdx.x >> 6].l);
+  if (staged)
+    wave_overflow<true>(b, sc, p, fluid, lds[threadIdx.x >> 6].k, sidx, xp, rb, rl, &lds[threadIdx.x >> 6].l,
+                        StagedPos{spos, {so[0], so[1], so[2]}});
+  else
+    wave_overflow<true>(b, sc, p, fluid, lds[threadIdx.x >> 6].k, sidx, xp, rb, rl, &lds[threadIdx.x >> 6].l,
+                        GlobalPos{b.posA});
 }
 
 // ClassicShearStress (derivatives.c:1686-1721) in gather form for stressType 2, before
@@ -251,13 +327,15 @@ static int launch_pair_pass_b(uph_handle* h, hipStream_t st, const StepBufs& b) 
                        h->dsc);
   } else if (!LAZY && MODE != UPH_MODE_REF2 && h->kth_all) {
     // the pair sums of such a step all come from the capped pass; the dynamic LDS only
-    // brings the block to KTH_BLOCK_LDS
-    const int pad = KTH_BLOCK_LDS - (int)sizeof(KthLaneLds) * (BLK / 64);
-    static_assert(KTH_BLOCK_LDS >= (int)sizeof(KthLaneLds) * (BLK / 64), "k_kth LDS");
+    // brings the block to KTH_BLOCK_LDS (the one-scene instance also holds the staged rows)
+    const int lane_lds = (int)(sizeof(KthLaneLds) * (BLK / 64));
+    const int stage_lds = (int)(sizeof(double2) * KTH_STAGE + 8 * sizeof(int));
     if (h->nscenes == 1)
-      hipLaunchKernelGGL((k_kth<true>), dim3(grid8_of(h->n)), dim3(BLK), pad, st, h->n, b, h->dsc);
+      hipLaunchKernelGGL((k_kth<true>), dim3(grid8_of(h->n)), dim3(BLK), KTH_BLOCK_LDS - lane_lds - stage_lds, st,
+                         h->n, b, h->dsc);
     else
-      hipLaunchKernelGGL((k_kth<false>), dim3(grid8_of(h->n)), dim3(BLK), pad, st, h->n, b, h->dsc);
+      hipLaunchKernelGGL((k_kth<false>), dim3(grid8_of(h->n)), dim3(BLK), KTH_BLOCK_LDS - lane_lds, st, h->n, b,
+                         h->dsc);
   } else {
     const int pbk = h->opts.pair_block;
     const int g = (((h->n + pbk - 1) / pbk + 7) / 8) * 8;
