@@ -1072,14 +1072,21 @@ __device__ void wave_kth(const DevScene& S, int q, const double2* __restrict__ p
 // Where the per-lane search reads a candidate's position, given its row r and slot s:
 // posA itself, or a copy of the block's candidate rows in LDS (k_kth), row r's slots
 // [lo_r, hi_r) at so[r] + s.
+// slot(s): the same for a slot of any of the three rows (rows lie in slot order; hi0 and
+// hi1 end rows 0 and 1, an absent row being empty).
 struct GlobalPos {
   const double2* __restrict__ p;
   __device__ __forceinline__ double2 operator()(int, int s) const { return p[s]; }
+  __device__ __forceinline__ double2 slot(int s) const { return p[s]; }
 };
 struct StagedPos {
   const double2* sp;
   int so[3];
+  int hi0, hi1;
   __device__ __forceinline__ double2 operator()(int r, int s) const { return sp[so[r] + s]; }
+  __device__ __forceinline__ double2 slot(int s) const {
+    return sp[s + (s < hi0 ? so[0] : s < hi1 ? so[1] : so[2])];
+  }
 };
 
 // a histogram of the in-range d2 over LANE_BUCKETS equal buckets of [0, cutoff)
@@ -1094,8 +1101,7 @@ struct StagedPos {
 // above q (by original index, below ParticleBoundary) that rank within q's K
 // nearest, admitted whatever their own lists hold; words 6-11 the candidates below
 // q, whose admission needs their own K-th key (the capped pass reads it).
-// The candidates of the histogram and collecting passes are read through `at`; the
-// selected bucket's members from pos.
+// Candidate positions are read through `at`.
 template <class PosAt>
 __device__ bool lane_kth(const DevScene& S, int q, double2 xq, const int* rb, const int* rl,
                          const double2* __restrict__ pos, const uint32_t* __restrict__ tag, LaneSel& L,
@@ -1211,7 +1217,7 @@ __device__ bool lane_kth(const DevScene& S, int q, double2 xq, const int* rb, co
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           const uint32_t t = L.slot[min(i + u, m - 1)][lane];
-          x[u] = pos[t];
+          x[u] = at.slot((int)t);
           g[u] = tag[t];
         }
 #pragma unroll
@@ -1235,7 +1241,7 @@ __device__ bool lane_kth(const DevScene& S, int q, double2 xq, const int* rb, co
 #pragma unroll 1
     for (int i = 0; i < m; ++i) {
       const int t = (int)L.slot[i][lane];
-      const double d2 = dist2(xq, pos[t]);
+      const double d2 = dist2(xq, at.slot(t));
       const uint32_t id = tag_id(tag[t]);
       const bool ok = id > idq && (int)id < S.pb_gid && key_le(d2, id, td, ti);
 #pragma unroll

@@ -77,7 +77,7 @@ __global__ __launch_bounds__(64) void k_pair_mixed(int n, StepBufs b0, const Dev
 // misses (1 % of them reach L2) or its arithmetic.  So a one-scene block whose fluid
 // slots share one cell row copies the union of their three candidate rows into LDS
 // with coalesced loads (KTH_STAGE positions at most, else it reads posA as before),
-// and the histogram and collecting passes read them there.
+// and the per-lane search reads its candidates there.
 //
 // Its registers would allow six wavefronts per SIMD; it runs four (KTH_BLOCK_LDS),
 // since with more the candidate positions of the resident wavefronts outgrow the
@@ -105,7 +105,7 @@ __global__ __launch_bounds__(BLK) void k_kth(int n, StepBufs b, const DevScene* 
   int rb[3], rl[3];
   candidate_rows(S, xp, b.cellStart, rb, rl);
   bool staged = false;
-  int so[3] = {0, 0, 0};
+  int so[3] = {0, 0, 0}, hi[2] = {0, 0};
   if constexpr (ONE) {
     int cx, cy;
     cell_of(S, xp.x, xp.y, cx, cy);
@@ -153,13 +153,15 @@ __global__ __launch_bounds__(BLK) void k_kth(int n, StepBufs b, const DevScene* 
       so[0] = -lo0;
       so[1] = len0 - lo1;
       so[2] = len0 + len1 - lo2;
+      hi[0] = lo0 + len0;
+      hi[1] = lo1 + len1;
     }
     __syncthreads();
   }
   // every lane of the wavefront reaches this, fluid or not
   if (staged)
     wave_overflow<true>(b, sc, p, fluid, lds[threadIdx.x >> 6].k, sidx, xp, rb, rl, &lds[threadIdx.x >> 6].l,
-                        StagedPos{spos, {so[0], so[1], so[2]}});
+                        StagedPos{spos, {so[0], so[1], so[2]}, hi[0], hi[1]});
   else
     wave_overflow<true>(b, sc, p, fluid, lds[threadIdx.x >> 6].k, sidx, xp, rb, rl, &lds[threadIdx.x >> 6].l,
                         GlobalPos{b.posA});
