@@ -2,15 +2,17 @@
 pass, bit for bit: the per-lane search leaves each capped query's pair partners as
 bits (kmask: the admitted candidates above it by original index, and the candidates
 below it, which wait for their own K-th keys), and the capped pass walks those
-without testing distances again.  Three instances are checked against runs that take
+without testing distances again.  Four instances are checked against runs that take
 other code paths over the same inputs:
 
 * a batch holding a capped scene without mirror sections (the reference's dambreak
   deck, virtualType 77, h = 2.691 dx: about 88 particles in range, K = 40) and an
   uncapped one, against each scene stepped alone (the multi-scene instance of the
-  per-lane kernel against the one-scene instance);
-* the reference scheme with PolyOrderSeven on that deck, compacted kernel (per-lane
-  search + capped bit walk with the mask) against the direct kernel (no mask);
+  per-lane kernel against the one-scene search);
+* a batch of two scenes whose queries are all capped (the K-th key search alone first,
+  with the scene per lane) against each scene alone (its candidate rows staged in LDS);
+* the reference scheme with PolyOrderSeven on that deck, the K-th key search and the
+  capped bit walk with the masks against the direct kernel (no mask);
 * hipGraph replay of capped steps against eager launches, with eager work and
   synchronisations between the replays.
 """
@@ -52,6 +54,27 @@ def test_batch_with_capped_scene_equals_single_scenes():
         gs = GpuSolver(sc, sort_every_step=True)   # as the batch, which holds a capped scene
         gs.step(15)
         gs.sync()
+        _same(gb.state(k), gs.state(0), f"scene {k}")
+
+
+def test_batch_of_capped_scenes_equals_single_scenes():
+    """Every scene expects more than K in range and none has mirror sections: the batch
+    takes the K-th key search alone first with the scene per lane (no staged rows), each
+    scene alone the one-scene instance (candidate rows staged in LDS), and the capped
+    pass computes every slot in both."""
+    from uph_amd.solver import GpuSolver
+    deck = _moving(load_golden_scene("dambreak.npz"))
+    gen = _moving(scenes.dam_break(width=0.3, height=0.4, dx=0.01, tank=(0.8, 0.8), jitter=0.05, seed=6,
+                                   h_factor=2.691),
+                  seed=4)
+    _say(f"batch: capped deck ({deck.n} rows) + capped dam break ({gen.n} rows), 12 steps")
+    gb = GpuSolver([deck, gen])
+    gb.step(12)
+    assert gb.sync() == 12
+    for k, sc in enumerate((deck, gen)):
+        gs = GpuSolver(sc)
+        gs.step(12)
+        assert gs.sync() == 12
         _same(gb.state(k), gs.state(0), f"scene {k}")
 
 
