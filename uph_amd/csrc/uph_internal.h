@@ -23,6 +23,7 @@
 #include "uph_kernels.h"
 #include "../../include/uph.h"
 
+#include <limits.h>
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -2097,9 +2098,14 @@ __device__ __forceinline__ bool exact_pairs(int p, const StepBufs& b, const DevS
 // whose slots belong to two scenes returns at once and k_pair_mixed runs it with
 // the scene per lane.  ONE: the handle holds one scene (index 0, no check).  Every
 // lane of the wavefront calls this.
-template <int KT, int MODE, bool UNI, bool ONE = false, bool LAZY = false>
+// STG (a one-scene block of the compacted kernel, every thread calling this): when the
+// block's fluid slots share one cell row, pass 1 reads the candidates from a copy of
+// the union of their three rows in LDS (spos, at most stg_max positions less a margin
+// for the reads past a row's end; sred: 8 ints of LDS for the block's reduction).
+template <int KT, int MODE, bool UNI, bool ONE = false, bool LAZY = false, bool STG = false>
 __device__ __forceinline__ void compact_query(int n, int p, const StepBufs& b, const DevScene* __restrict__ sc,
-                                              KthLds& lds, LaneSel* ls = nullptr) {
+                                              KthLds& lds, LaneSel* ls = nullptr, double2* spos = nullptr,
+                                              int* sred = nullptr, int stg_max = 0) {
   const bool live = p < n;
   const int pp = live ? p : n - 1;
   const uint32_t tg = b.tagA[pp];
@@ -2134,39 +2140,89 @@ __device__ __forceinline__ void compact_query(int n, int p, const StepBufs& b, c
       bad = true;
       rl[r] = 0;
     }
-    // the trip count is the wave's maximum row length (uniform, in an SGPR); slots
-    // past this lane's row end are tested too and their bits cleared below
-    const int lmax = __builtin_amdgcn_readfirstlane(wave_max(rl[r]));
-    const double2* rp = b.posA + rb[r];
-    uint32_t mw[2] = {0u, 0u};
-#pragma unroll
-    for (int hw = 0; hw < 2; ++hw) {
-      const int k0 = 32 * hw;
-      const int kend = hw == 0 ? (lmax < 32 ? lmax : 32) : lmax;
-      // candidate k0 + j enters as the low bit and is shifted up by each later one:
-      // one add-with-carry per candidate; bit-reversed into place after the loop
-      uint32_t bits = 0u;
-#pragma unroll 1
-      for (int k = k0; k < kend; k += 4) {
-        double2 xs[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) xs[u] = rp[u];   // posA has POS_PAD zeroed slots past n
-        rp += 4;
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const double ex = xp.x - xs[u].x, ey = xp.y - xs[u].y;
-          const double d2 = __dadd_rn(__dmul_rn(ex, ex), __dmul_rn(ey, ey));
-          bits = bits + bits + (d2 < cut ? 1u : 0u);
-        }
-      }
-      const int nb = (kend - k0 + 3) & ~3;   // candidates tested in this half (uniform)
-      if (nb > 0) mw[hw] = __builtin_bitreverse32(bits) >> (32 - nb);
-    }
-    const int l = rl[r];
-    mw[0] &= l >= 32 ? 0xffffffffu : ((1u << l) - 1u);
-    mw[1] &= l >= 64 ? 0xffffffffu : (l > 32 ? ((1u << (l - 32)) - 1u) : 0u);
-    m[r] = ((unsigned long long)mw[1] << 32) | mw[0];
   }
+  bool staged = false;
+  int so[3] = {0, 0, 0};
+  if constexpr (STG) {
+    if (threadIdx.x == 0) {
+      sred[0] = sred[2] = sred[3] = sred[4] = INT_MAX;
+      sred[1] = sred[5] = sred[6] = sred[7] = INT_MIN;
+    }
+    __syncthreads();
+    const int w0 = -wave_max(fluid ? -cy : -INT_MAX), w1 = wave_max(fluid ? cy : INT_MIN);
+    const int w2 = -wave_max(fluid ? -rb[0] : -INT_MAX), w3 = -wave_max(fluid ? -rb[1] : -INT_MAX);
+    const int w4 = -wave_max(fluid ? -rb[2] : -INT_MAX);
+    const int w5 = wave_max(fluid ? rb[0] + rl[0] : INT_MIN), w6 = wave_max(fluid ? rb[1] + rl[1] : INT_MIN);
+    const int w7 = wave_max(fluid ? rb[2] + rl[2] : INT_MIN);
+    if ((threadIdx.x & 63) == 0) {
+      atomicMin(&sred[0], w0);
+      atomicMax(&sred[1], w1);
+      atomicMin(&sred[2], w2);
+      atomicMin(&sred[3], w3);
+      atomicMin(&sred[4], w4);
+      atomicMax(&sred[5], w5);
+      atomicMax(&sred[6], w6);
+      atomicMax(&sred[7], w7);
+    }
+    __syncthreads();
+    if (sred[0] == sred[1]) {
+      // one cell row for every fluid slot: row r's ranges are contiguous cells of one
+      // cell row, so their union is one range; a lane reads up to 66 slots past its
+      // row's end (the wavefront's longest row, four per trip), kept inside spos
+      const int lo0 = sred[2], lo1 = sred[3], lo2 = sred[4];
+      const int len0 = sred[5] - lo0, len1 = sred[6] - lo1, len2 = sred[7] - lo2;
+      if (len0 + len1 + len2 + 68 <= stg_max) {
+        staged = true;
+        for (int i = threadIdx.x; i < len0 + len1 + len2; i += blockDim.x)
+          spos[i] = b.posA[i < len0 ? lo0 + i : i < len0 + len1 ? lo1 + (i - len0) : lo2 + (i - len0 - len1)];
+        so[0] = -lo0;
+        so[1] = len0 - lo1;
+        so[2] = len0 + len1 - lo2;
+      }
+    }
+    __syncthreads();
+  }
+  auto pass1 = [&](auto at) {
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      // the trip count is the wave's maximum row length (uniform, in an SGPR); slots
+      // past this lane's row end are tested too and their bits cleared below
+      const int lmax = __builtin_amdgcn_readfirstlane(wave_max(rl[r]));
+      int kr = rb[r];
+      uint32_t mw[2] = {0u, 0u};
+#pragma unroll
+      for (int hw = 0; hw < 2; ++hw) {
+        const int k0 = 32 * hw;
+        const int kend = hw == 0 ? (lmax < 32 ? lmax : 32) : lmax;
+        // candidate k0 + j enters as the low bit and is shifted up by each later one:
+        // one add-with-carry per candidate; bit-reversed into place after the loop
+        uint32_t bits = 0u;
+#pragma unroll 1
+        for (int k = k0; k < kend; k += 4) {
+          double2 xs[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) xs[u] = at(r, kr + u);   // posA has POS_PAD zeroed slots past n
+          kr += 4;
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const double ex = xp.x - xs[u].x, ey = xp.y - xs[u].y;
+            const double d2 = __dadd_rn(__dmul_rn(ex, ex), __dmul_rn(ey, ey));
+            bits = bits + bits + (d2 < cut ? 1u : 0u);
+          }
+        }
+        const int nb = (kend - k0 + 3) & ~3;   // candidates tested in this half (uniform)
+        if (nb > 0) mw[hw] = __builtin_bitreverse32(bits) >> (32 - nb);
+      }
+      const int l = rl[r];
+      mw[0] &= l >= 32 ? 0xffffffffu : ((1u << l) - 1u);
+      mw[1] &= l >= 64 ? 0xffffffffu : (l > 32 ? ((1u << (l - 32)) - 1u) : 0u);
+      m[r] = ((unsigned long long)mw[1] << 32) | mw[0];
+    }
+  };
+  if (staged)
+    pass1([&](int r, int s) { return spos[so[r] + s]; });
+  else
+    pass1([&](int, int s) { return b.posA[s]; });
   // every lane reaches wave_overflow: the work of a lane is a call of this
   bool ovf = false;
   auto run = [&]() {
