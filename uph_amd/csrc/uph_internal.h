@@ -2098,10 +2098,11 @@ __device__ __forceinline__ bool exact_pairs(int p, const StepBufs& b, const DevS
 // whose slots belong to two scenes returns at once and k_pair_mixed runs it with
 // the scene per lane.  ONE: the handle holds one scene (index 0, no check).  Every
 // lane of the wavefront calls this.
-// STG (a one-scene block of the compacted kernel, every thread calling this): when the
-// block's fluid slots share one cell row, pass 1 reads the candidates from a copy of
-// the union of their three rows in LDS (spos, at most stg_max positions less a margin
-// for the reads past a row's end; sred: 8 ints of LDS for the block's reduction).
+// STG (every thread of the compacted kernel's block calling this): when the block's
+// fluid slots share one scene and one cell row, pass 1 reads the candidates from a
+// copy of the union of their three rows in LDS (spos, at most stg_max positions less a
+// margin for the reads past a row's end; sred: 10 ints of LDS for the block's
+// reduction).  A wavefront of two scenes returns only after the block's barriers.
 template <int KT, int MODE, bool UNI, bool ONE = false, bool LAZY = false, bool STG = false>
 __device__ __forceinline__ void compact_query(int n, int p, const StepBufs& b, const DevScene* __restrict__ sc,
                                               KthLds& lds, LaneSel* ls = nullptr, double2* spos = nullptr,
@@ -2112,13 +2113,14 @@ __device__ __forceinline__ void compact_query(int n, int p, const StepBufs& b, c
   const bool fluid = live && tag_type(tg) == 1;
   const int slane = ONE ? 0 : b.sceneOf[pp];
   const int sidx = ONE ? 0 : UNI ? __builtin_amdgcn_readfirstlane(slane) : slane;
-  if (!ONE && UNI && __ballot(slane != sidx) != 0ull) return;   // a wavefront of two scenes
+  const bool mixed = !ONE && UNI && __ballot(slane != sidx) != 0ull;   // a wavefront of two scenes
+  if (!STG && mixed) return;
   const DevScene& S = sc[sidx];
   const int itime = b.it[0];
   const double2 xp = b.posA[pp];
   const double cut = S.cutoff;
   int cx, cy;
-  cell_of(S, xp.x, xp.y, cx, cy);
+  cell_of(S, xp.x, xp.y, cx, cy);   // in a wavefront of two scenes, a lane's rows below are not used
   const int x0 = cx > 0 ? cx - 1 : 0;
   const int x1 = cx < S.ncx - 1 ? cx + 1 : S.ncx - 1;
   int rb[3], rl[3], rlf[3];   // rlf: the row lengths before the 64-slot limit
@@ -2145,11 +2147,12 @@ __device__ __forceinline__ void compact_query(int n, int p, const StepBufs& b, c
   int so[3] = {0, 0, 0};
   if constexpr (STG) {
     if (threadIdx.x == 0) {
-      sred[0] = sred[2] = sred[3] = sred[4] = INT_MAX;
-      sred[1] = sred[5] = sred[6] = sred[7] = INT_MIN;
+      sred[0] = sred[2] = sred[3] = sred[4] = sred[8] = INT_MAX;
+      sred[1] = sred[5] = sred[6] = sred[7] = sred[9] = INT_MIN;
     }
     __syncthreads();
     const int w0 = -wave_max(fluid ? -cy : -INT_MAX), w1 = wave_max(fluid ? cy : INT_MIN);
+    const int w8 = -wave_max(fluid ? -slane : -INT_MAX), w9 = wave_max(fluid ? slane : INT_MIN);
     const int w2 = -wave_max(fluid ? -rb[0] : -INT_MAX), w3 = -wave_max(fluid ? -rb[1] : -INT_MAX);
     const int w4 = -wave_max(fluid ? -rb[2] : -INT_MAX);
     const int w5 = wave_max(fluid ? rb[0] + rl[0] : INT_MIN), w6 = wave_max(fluid ? rb[1] + rl[1] : INT_MIN);
@@ -2163,9 +2166,11 @@ __device__ __forceinline__ void compact_query(int n, int p, const StepBufs& b, c
       atomicMax(&sred[5], w5);
       atomicMax(&sred[6], w6);
       atomicMax(&sred[7], w7);
+      atomicMin(&sred[8], w8);
+      atomicMax(&sred[9], w9);
     }
     __syncthreads();
-    if (sred[0] == sred[1]) {
+    if (sred[0] == sred[1] && sred[8] == sred[9]) {
       // one cell row for every fluid slot: row r's ranges are contiguous cells of one
       // cell row, so their union is one range; a lane reads up to 66 slots past its
       // row's end (the wavefront's longest row, four per trip), kept inside spos
@@ -2181,6 +2186,7 @@ __device__ __forceinline__ void compact_query(int n, int p, const StepBufs& b, c
       }
     }
     __syncthreads();
+    if (mixed) return;
   }
   auto pass1 = [&](auto at) {
 #pragma unroll

@@ -49,17 +49,14 @@ void k_pair_compact(int n, StepBufs b0, const DevScene* __restrict__ sc) {
   if constexpr (LANE) {
     __shared__ KthLaneLds lds[PB / 64];
     compact_query<KT, MODE, true, ONE, LAZY>(n, p, b, sc, lds[threadIdx.x >> 6].k, &lds[threadIdx.x >> 6].l);
-  } else if constexpr (ONE) {
+  } else {
     // pass 1 reads its candidates from the block's staged rows where they fit (20 KB,
     // within the LDS of the kernel's five wavefronts per SIMD)
     constexpr int STG_N = PB == 128 ? 624 : 1248;
     __shared__ KthLds lds[PB / 64];
     __shared__ double2 spos[STG_N];
-    __shared__ int sred[8];
+    __shared__ int sred[10];
     compact_query<KT, MODE, true, ONE, LAZY, true>(n, p, b, sc, lds[threadIdx.x >> 6], nullptr, spos, sred, STG_N);
-  } else {
-    __shared__ KthLds lds[PB / 64];
-    compact_query<KT, MODE, true, ONE, LAZY>(n, p, b, sc, lds[threadIdx.x >> 6]);
   }
 }
 
