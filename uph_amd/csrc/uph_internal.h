@@ -2183,6 +2183,13 @@ __device__ __forceinline__ void compact_query(int n, int p, const StepBufs& b, c
         so[0] = -lo0;
         so[1] = len0 - lo1;
         so[2] = len0 + len1 - lo2;
+        if (!fluid) {
+          // its rows are empty, but pass 1 tests the wavefront's trip count of slots
+          // from their starts (bits cleared after): start them inside the copy
+          rb[0] = lo0;
+          rb[1] = lo1;
+          rb[2] = lo2;
+        }
       }
     }
     __syncthreads();
